@@ -1,0 +1,288 @@
+/*
+ * meshtryoshka_hip.h - C ABI of the MI355X (gfx950) extract -> rasterize -> composite
+ * kernels.
+ *
+ * Conventions (every entry point):
+ *   - Pointers are device pointers to contiguous row-major arrays owned by the
+ *     caller. Native code never allocates or frees device memory: operations that
+ *     need scratch space take a caller-owned workspace sized by the matching
+ *     *_workspace_bytes query.
+ *   - Element counts are int64_t. Array shapes are stated per argument.
+ *   - All work is enqueued on `stream` (a hipStream_t; NULL = the default stream).
+ *     Nothing synchronises the host.
+ *   - The return value is MT_OK (0) or an MT_ERR_* code; mt_get_last_error_string()
+ *     describes the last failure on the calling host thread.
+ *
+ * Each entry point cites the reference interface it replaces (path:line inside the
+ * reference repository dcharatan/meshtryoshka). The reference binds Slang kernels by
+ * keyword (slangtorch `launchRaw`) and two CUB pybind functions; here the same
+ * operations are plain C functions.
+ */
+#ifndef MESHTRYOSHKA_HIP_H_
+#define MESHTRYOSHKA_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* mt_stream_t;
+
+enum {
+  MT_OK = 0,
+  MT_ERR_INVALID_ARGUMENT = 1,
+  MT_ERR_HIP = 2,
+  MT_ERR_WORKSPACE_TOO_SMALL = 3,
+};
+
+/* Bit layout of a 64-bit sort key (triangle_rasterization/keys.py:4-22): offsets and
+ * lengths of the camera, shell and tile fields; the depth field is always 32 bits. */
+typedef struct mt_key_layout {
+  int32_t camera_offset, camera_bits;
+  int32_t shell_offset, shell_bits;
+  int32_t tile_offset, tile_bits;
+  int32_t depth_offset;
+} mt_key_layout;
+
+/* Human-readable description of the last error on this host thread. */
+const char* mt_get_last_error_string(void);
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int32_t mt_version(void);
+
+/* ---------------------------------------------------------------------------------
+ * Prefix sums
+ * ------------------------------------------------------------------------------- */
+
+/* Workspace for the single-pass (decoupled look-back) scans below, for n elements. */
+size_t mt_scan_workspace_bytes(int64_t n);
+
+/* In-place exclusive prefix sum of int32 data[n] (int32 wrap-around).
+ * Replaces compute_exclusive_cumsum (triangle_extraction/compute_exclusive_cumsum.cu:8-39,
+ * bound at compute_exclusive_cumsum.py:19-21). */
+int mt_exclusive_scan_i32(int32_t* data, int64_t n, void* workspace, size_t workspace_bytes,
+                          mt_stream_t stream);
+
+/* out[i] = sum_{j <= i} in[j] as int64. Replaces the int64 torch.cumsum of per-(camera,
+ * face) tile counts in generate_keys (triangle_rasterization/generate_keys.py:48). */
+int mt_inclusive_scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, void* workspace,
+                                 size_t workspace_bytes, mt_stream_t stream);
+
+/* ---------------------------------------------------------------------------------
+ * Voxel structure (triangle_extraction/__init__.py:31-62)
+ * ------------------------------------------------------------------------------- */
+
+/* occupancy int32[I, J, KP] (bit b of word (i, j, kp) = cell (i, j, 32 kp + b)).
+ * Writes vertex_occupancy int32[I+1, J+1, KP+1] and vertex_counts int32[(I+1)(J+1)(KP+1)]
+ * (popcounts). Replaces compute_vertex_occupancy.slang:9-68 (.py:34-61). */
+int mt_compute_vertex_occupancy(const int32_t* occupancy, int32_t I, int32_t J, int32_t KP,
+                                int32_t* vertex_occupancy, int32_t* vertex_counts,
+                                mt_stream_t stream);
+
+/* Writes voxel_counts int32[(I+1)(J+1)(KP+1)] (popcounts of occupancy, 0 on the padded
+ * high faces). Replaces count_occupancy.slang:3-20 (.py:26-44). */
+int mt_count_occupancy(const int32_t* occupancy, int32_t I, int32_t J, int32_t KP,
+                       int32_t* voxel_counts, mt_stream_t stream);
+
+/* Builds the Voxels structure from the exclusive offsets of the two count arrays.
+ * vertices f32[num_vertices, 3]; neighbors int32[7, num_voxels];
+ * lower_corners int32[4, num_vertices]; upper_corners int32[4, num_voxels];
+ * indices int32[num_voxels]. Replaces create_voxels.slang:3-276 (.py:47-96). */
+int mt_create_voxels(const int32_t* occupancy, const int32_t* voxel_offsets,
+                     const int32_t* vertex_occupancy, const int32_t* vertex_offsets, int32_t I,
+                     int32_t J, int32_t KP, int64_t num_voxels, int64_t num_vertices,
+                     float* vertices, int32_t* neighbors, int32_t* lower_corners,
+                     int32_t* upper_corners, int32_t* indices, mt_stream_t stream);
+
+/* ---------------------------------------------------------------------------------
+ * Marching cubes over level sets (triangle_extraction/__init__.py:75-157)
+ * ------------------------------------------------------------------------------- */
+
+/* counts int32[S * num_columns] (level-set major): crossings of the column's owned
+ * +x/+y/+z edges. Replaces countTriangleVertices (count_triangle_vertices.slang:3-37,
+ * .py:23-50). lower_corners int32[4, num_columns]; sdf f32[num grid vertices]. */
+int mt_count_triangle_vertices(const float* sdf, const int32_t* lower_corners,
+                               int64_t num_columns, const float* level_sets, int32_t S,
+                               int32_t* counts, mt_stream_t stream);
+
+/* Forward of CreateTriangleVertices (create_triangle_vertices.slang:145-199,
+ * .py:31-102). vertex_offsets int32[S * num_columns + 1] (exclusive). grid_sh
+ * f32[num_sh, num_grid_vertices, 3]. Outputs f32[V_t, 3], f32[V_t], f32[num_sh, V_t, 3],
+ * uint8[V_t] with V_t = num_triangle_vertices. */
+int mt_create_triangle_vertices_forward(
+    const float* grid_vertices, const float* grid_sdf, const float* grid_sh,
+    int64_t num_grid_vertices, int32_t num_sh, const int32_t* lower_corners,
+    int64_t num_columns, const float* level_sets, int32_t S, const int32_t* vertex_offsets,
+    int64_t num_triangle_vertices, float* out_vertices, float* out_sdf, float* out_sh,
+    uint8_t* out_types, mt_stream_t stream);
+
+/* Adjoint of the forward (create_triangle_vertices.slang:201-291, .py:104-170).
+ * Accumulates into zero-initialised grid gradients. Any grad_* input may be NULL
+ * (no gradient flows from that output); any grid_*_grad output may be NULL (not
+ * requested). */
+int mt_create_triangle_vertices_backward(
+    const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh,
+    const int32_t* lower_corners, int64_t num_columns, const float* level_sets, int32_t S,
+    const int32_t* vertex_offsets, int64_t num_triangle_vertices, const float* grad_vertices,
+    const float* grad_sdf, const float* grad_sh, float* grid_vertices_grad,
+    float* grid_sdf_grad, float* grid_sh_grad, mt_stream_t stream);
+
+/* face_counts int32[S * num_voxels] (3 x triangles) and cell_codes uint8[S, num_voxels].
+ * Replaces count_triangle_faces.slang:13-50 (.py:28-63). upper_corners
+ * int32[4, num_voxels]. */
+int mt_count_triangle_faces(const float* sdf, const int32_t* lower_corners, int64_t num_columns,
+                            const int32_t* upper_corners, int64_t num_voxels,
+                            const float* level_sets, int32_t S, int32_t* face_counts,
+                            uint8_t* cell_codes, mt_stream_t stream);
+
+/* faces int32[F, 3] and voxel_indices int32[F] from the exclusive face offsets
+ * int32[S * num_voxels + 1]. Replaces create_triangle_faces.slang:3-78 (.py:28-65). */
+int mt_create_triangle_faces(const int32_t* neighbors, const int32_t* indices,
+                             int64_t num_voxels, int64_t num_columns,
+                             const int32_t* vertex_offsets, const uint8_t* vertex_types,
+                             const uint8_t* cell_codes, int32_t S, const int32_t* face_offsets,
+                             int32_t* faces, int32_t* voxel_indices, mt_stream_t stream);
+
+/* ---------------------------------------------------------------------------------
+ * Rasterization (triangle_rasterization/__init__.py:22-329)
+ * ------------------------------------------------------------------------------- */
+
+/* vertices f32[V, 3]; extrinsics f32[B, 4, 4]; intrinsics f32[B, 3, 3];
+ * out_xy f32[B, V, 2]; out_depth f32[B, V]. Replaces project_vertices.slang:61-92
+ * (.py:31-56). */
+int mt_project_vertices(const float* vertices, int64_t V, const float* extrinsics,
+                        const float* intrinsics, int32_t B, float* out_xy, float* out_depth,
+                        mt_stream_t stream);
+
+/* Back-face / near-plane culling and tile bounds. faces int32[F, 3]; outputs
+ * tile_min int32[B, F, 2] (row, col) inclusive, tile_max int32[B, F, 2] exclusive,
+ * counts int32[B, F]. Replaces bound_triangles.slang:144-192 (.py:35-71). */
+int mt_bound_triangles(const float* xy, const float* depths, int64_t V, const int32_t* faces,
+                       int64_t F, int32_t B, int32_t tile_h, int32_t tile_w, int32_t row_min,
+                       int32_t col_min, int32_t row_max, int32_t col_max, float near_plane,
+                       int32_t* tile_min, int32_t* tile_max, int32_t* counts,
+                       mt_stream_t stream);
+
+/* One key per (camera, face, covered tile), written at the inclusive offsets
+ * int64[B * F] of the tile counts. keys int64[K], values int32[K] (face ids).
+ * Replaces generate_keys.slang:4-68 (.py:34-70). */
+int mt_generate_keys(const float* depths, int64_t V, const int32_t* faces, int64_t F,
+                     const int32_t* shell_face_boundaries, int32_t S, const int32_t* tile_min,
+                     const int32_t* tile_max, const int64_t* offsets, int32_t B,
+                     int32_t grid_cols, mt_key_layout layout, int64_t* keys, int32_t* values,
+                     mt_stream_t stream);
+
+/* Workspace for mt_sort_keys on n pairs. */
+size_t mt_sort_workspace_bytes(int64_t n);
+
+/* Stable ascending sort of (signed int64 key, int32 value) pairs over all 64 key bits
+ * (hand-written onesweep LSD radix sort; passes whose digit is constant are skipped on
+ * the device). Inputs are not modified. Replaces sort_keys (sort_keys.cu:215-270,
+ * sort_keys.py:18-29). */
+int mt_sort_keys(const int64_t* keys_in, const int32_t* values_in, int64_t n, int64_t* keys_out,
+                 int32_t* values_out, void* workspace, size_t workspace_bytes,
+                 mt_stream_t stream);
+
+/* [start, end) of every (camera[, shell], tile) run of the sorted keys into the
+ * zero-initialised int64[B, S', T, 2] (S' = S if separate_shells else 1).
+ * Replaces delineate_tiles.slang:12-58 (.py:24-55). */
+int mt_delineate_tiles(const int64_t* sorted_keys, int64_t n, mt_key_layout layout,
+                       int32_t separate_shells, int32_t B, int32_t S_out, int32_t T,
+                       int64_t* tile_ranges, mt_stream_t stream);
+
+/* out f32[B, 3] = inverse(extrinsics[b])[:3, 3] (computed in double precision).
+ * Replaces torch.inverse(extrinsics)[:, :3, 3] (evaluate_spherical_harmonics.py:38). */
+int mt_camera_positions(const float* extrinsics, int32_t B, float* out, mt_stream_t stream);
+
+/* colors f32[B, V, 3] from sh f32[num_sh, V, 3] (first (deg+1)^2 coefficients used).
+ * Replaces evaluate_spherical_harmonics_forward (evaluate_spherical_harmonics.slang:
+ * 187-222, .py:25-60). */
+int mt_evaluate_sh_forward(const float* vertices, int64_t V, const float* sh, int32_t num_sh,
+                           int32_t sh_degree, const float* camera_positions, int32_t B,
+                           float* colors, mt_stream_t stream);
+
+/* Accumulates into zero-initialised grad_sh f32[num_sh, V, 3].
+ * Replaces evaluate_spherical_harmonics_backward (slang:224-270, .py:62-97). */
+int mt_evaluate_sh_backward(const float* vertices, int64_t V, const float* sh, int32_t num_sh,
+                            int32_t sh_degree, const float* camera_positions, int32_t B,
+                            const float* grad_colors, float* grad_sh, mt_stream_t stream);
+
+/* Arguments shared by the fused intersect-and-composite kernels. */
+typedef struct mt_composite_args {
+  const float* xy;            /* f32[B, V, 2] projected vertices */
+  const float* colors;        /* f32[B, V, 3] */
+  const float* sdf;           /* f32[V] */
+  int64_t num_vertices;
+  const int32_t* faces;       /* int32[F, 3] */
+  const float* sharpness;     /* f32[1] (device) */
+  const int64_t* sorted_keys; /* int64[K] */
+  const int32_t* sorted_values; /* int32[K] */
+  const int64_t* tile_ranges; /* int64[B, T, 2] */
+  int32_t num_cameras, num_shells, grid_rows, grid_cols, tile_h, tile_w, height, width;
+  mt_key_layout layout;
+  int32_t chunk; /* triangles staged in LDS per round: 0 = tile pixel count */
+} mt_composite_args;
+
+/* out f32[B, 4, H, W] (rgb, transmittance). Replaces intersect_and_composite_forward
+ * (intersect_and_composite.slang:224-339, .py:25-80). */
+int mt_intersect_and_composite_forward(const mt_composite_args* args, float* out,
+                                       mt_stream_t stream);
+
+/* Recomputes the traversal and accumulates into zero-initialised colors_grad
+ * f32[B, V, 3], sdf_grad f32[V], sharpness_grad f32[1]. reference_quirk = 1 scatters each
+ * delayed sample gradient with the next triangle's barycentrics exactly as the
+ * reference does; 0 uses the sample's own triangle (the true adjoint).
+ * Replaces intersect_and_composite_backward (slang:341-564, .py:82-147). */
+int mt_intersect_and_composite_backward(const mt_composite_args* args, const float* out,
+                                        const float* grad_out, float* colors_grad,
+                                        float* sdf_grad, float* sharpness_grad,
+                                        int32_t reference_quirk, mt_stream_t stream);
+
+/* Explicit mode: first hit per (camera, shell, pixel) in depth order. tile_ranges
+ * int64[B, S, T, 2]; out_uv f32[B*S, H, W, 2]; out_index int32[B*S, H, W] (-1 = miss).
+ * Replaces intersect.slang:157-252 (.py:33-70). */
+int mt_intersect(const int32_t* sorted_values, const int64_t* tile_ranges, const float* xy,
+                 int64_t V, const int32_t* faces, int32_t B, int32_t S, int32_t grid_rows,
+                 int32_t grid_cols, int32_t tile_h, int32_t tile_w, int32_t height,
+                 int32_t width, float* out_uv, int32_t* out_index, mt_stream_t stream);
+
+/* (rgb, sdf) = u a + v b + (1 - u - v) c per pixel; misses write 0.
+ * uv f32[B*S, H, W, 2]; index int32[B*S, H, W]; colors f32[B, V, 3].
+ * Replaces interpolate.slang:156-210 (.py:21-75). */
+int mt_interpolate_forward(const float* uv, const int32_t* index, const int32_t* faces,
+                           const float* sdf, const float* colors, int64_t V, int32_t B,
+                           int32_t S, int32_t H, int32_t W, float* out_sdf, float* out_colors,
+                           mt_stream_t stream);
+
+/* Adjoint of the above into zero-initialised sdf_grad f32[V], colors_grad f32[B, V, 3]
+ * (interpolate.slang auto-derivative, .py:76-121). */
+int mt_interpolate_backward(const float* uv, const int32_t* index, const int32_t* faces,
+                            int64_t V, int32_t B, int32_t S, int32_t H, int32_t W,
+                            const float* grad_sdf, const float* grad_colors, float* sdf_grad,
+                            float* colors_grad, mt_stream_t stream);
+
+/* Per-pixel front-to-back composite over S masked layers. sdf f32[P, S],
+ * colors f32[P, S, 3], mask uint8[P, S]; out_colors f32[P, 3], out_trans f32[P].
+ * Replaces composite_forward (composite.slang:179-221, .py:22-61). */
+int mt_composite_forward(const float* sdf, const float* colors, const uint8_t* mask, int64_t P,
+                         int32_t S, const float* sharpness, float* out_colors, float* out_trans,
+                         mt_stream_t stream);
+
+/* Adjoint into zero-initialised sdf_grad f32[P, S], colors_grad f32[P, S, 3] and
+ * sharpness_grad f32[1]. reference_quirk = 1 sends each delayed SDF gradient to layer
+ * s - 1 and the last one to layer S - 1 as the reference does; 0 sends them to the
+ * previous and the last unmasked layer. Replaces composite_backward (composite.slang:
+ * 225-335, .py:63-101). */
+int mt_composite_backward(const float* sdf, const float* colors, const uint8_t* mask, int64_t P,
+                          int32_t S, const float* sharpness, const float* out_colors,
+                          const float* out_trans, const float* grad_colors,
+                          const float* grad_trans, float* sdf_grad, float* colors_grad,
+                          float* sharpness_grad, int32_t reference_quirk, mt_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MESHTRYOSHKA_HIP_H_ */

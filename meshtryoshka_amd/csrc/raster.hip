@@ -1,0 +1,435 @@
+// Projection, triangle bounding, key generation, tile delineation and spherical
+// harmonics. Floating-point expressions keep the reference's operation order; the
+// library is compiled without FP contraction so that keys (which embed depth bits) and
+// tile bounds are reproducible bit for bit.
+#include "common.h"
+
+namespace mt {
+namespace {
+
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------------------------------
+// Projection (project_vertices.slang:51-92)
+// ------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) project_kernel(const float* verts, int64_t V,
+                                                         const float* E, const float* K, int B,
+                                                         float* out_xy, float* out_d) {
+  const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  const float x = verts[3 * v + 0], y = verts[3 * v + 1], z = verts[3 * v + 2];
+  for (int b = 0; b < B; ++b) {
+    const float* e = E + 16 * b;
+    const float* k = K + 9 * b;
+    float cs[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) cs[r] = ((e[4 * r + 0] * x + e[4 * r + 1] * y) + e[4 * r + 2] * z) + e[4 * r + 3];
+    const float depth = cs[2];
+    const float px = (k[0] * cs[0] + k[1] * cs[1]) + k[2] * cs[2];
+    const float py = (k[3] * cs[0] + k[4] * cs[1]) + k[5] * cs[2];
+    const int64_t o = int64_t(b) * V + v;
+    reinterpret_cast<float2*>(out_xy)[o] = make_float2(px / depth, py / depth);
+    out_d[o] = depth;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Bounds (bound_triangles.slang:62-192)
+// ------------------------------------------------------------------------------------
+
+// float -> int32 rounding toward zero, saturating, NaN -> 0.
+__device__ __forceinline__ int32_t f2i_rz(float x) {
+  if (!(x == x)) return 0;
+  if (x >= 2147483647.0f) return 2147483647;
+  if (x <= -2147483648.0f) return -2147483647 - 1;
+  return static_cast<int32_t>(x);
+}
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+__global__ void __launch_bounds__(kBlock) bound_kernel(const float* xy, const float* depths,
+                                                       int64_t V, const int32_t* faces, int64_t F,
+                                                       int B, int th, int tw, int rmin, int cmin,
+                                                       int rmax, int cmax, float near_plane,
+                                                       int32_t* tmin, int32_t* tmax,
+                                                       int32_t* counts) {
+  const int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const int64_t ia = faces[3 * f + 0], ib = faces[3 * f + 1], ic = faces[3 * f + 2];
+  const float2* pxy = reinterpret_cast<const float2*>(xy);
+  for (int b = 0; b < B; ++b) {
+    const int64_t o = int64_t(b) * V;
+    const float2 a = pxy[o + ia], bb = pxy[o + ib], c = pxy[o + ic];
+    const float da = depths[o + ia], db = depths[o + ib], dc = depths[o + ic];
+    int y0 = 0, y1 = 0, x0 = 0, x1 = 0;
+    const float check = (bb.x - a.x) * (c.y - a.y) - (bb.y - a.y) * (c.x - a.x);
+    const float min_depth = fminf(fminf(da, db), dc);
+    if (!(check < 0.0f) && !(min_depth < near_plane)) {
+      const float ylo = fminf(fminf(a.y, bb.y), c.y), yhi = fmaxf(fmaxf(a.y, bb.y), c.y);
+      const float xlo = fminf(fminf(a.x, bb.x), c.x), xhi = fmaxf(fmaxf(a.x, bb.x), c.x);
+      y0 = clampi(f2i_rz(ylo) / th, rmin, rmax);
+      y1 = clampi(f2i_rz(ceilf(yhi / float(th))), rmin, rmax);
+      x0 = clampi(f2i_rz(xlo) / tw, cmin, cmax);
+      x1 = clampi(f2i_rz(ceilf(xhi / float(tw))), cmin, cmax);
+    }
+    const int64_t q = int64_t(b) * F + f;
+    reinterpret_cast<int2*>(tmin)[q] = make_int2(y0, x0);
+    reinterpret_cast<int2*>(tmax)[q] = make_int2(y1, x1);
+    counts[q] = (y1 - y0) * (x1 - x0);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Keys (generate_keys.slang:4-68). Each block owns 256 consecutive faces of one camera;
+// their keys form one contiguous output range, which the block writes cooperatively
+// (coalesced) after a block scan of the per-face tile counts.
+// ------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) keys_kernel(const float* depths, int64_t V,
+                                                      const int32_t* faces, int64_t F,
+                                                      const int32_t* boundaries, int S,
+                                                      const int32_t* tmin, const int32_t* tmax,
+                                                      const int64_t* offsets, int grid_cols,
+                                                      KeyFields kf, int64_t* keys,
+                                                      int32_t* values) {
+  __shared__ int32_t s_prefix[kBlock + 1];
+  __shared__ int32_t s_r0[kBlock], s_c0[kBlock], s_w[kBlock];
+  __shared__ uint64_t s_key_hi[kBlock];  // camera | shell | depth bits (tile added later)
+  __shared__ int32_t s_wave[kBlock / kWave];
+  const int b = blockIdx.y;
+  const int64_t f = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  int32_t count = 0;
+  int64_t block_start = 0;
+  if (f < F) {
+    const int64_t q = int64_t(b) * F + f;
+    const int2 lo = reinterpret_cast<const int2*>(tmin)[q];
+    const int2 hi = reinterpret_cast<const int2*>(tmax)[q];
+    count = (hi.x - lo.x) * (hi.y - lo.y);
+    s_r0[threadIdx.x] = lo.x;
+    s_c0[threadIdx.x] = lo.y;
+    s_w[threadIdx.x] = hi.y - lo.y;
+    if (count > 0) {
+      uint32_t shell = 0;
+      while (shell < uint32_t(S) && !(f < boundaries[shell])) ++shell;
+      const int64_t o = int64_t(b) * V;
+      const float d0 = depths[o + faces[3 * f + 0]];
+      const float d1 = depths[o + faces[3 * f + 1]];
+      const float d2 = depths[o + faces[3 * f + 2]];
+      const float mean_depth = ((d0 + d1) + d2) / 3.f;
+      s_key_hi[threadIdx.x] = pack_key(kf, uint32_t(b), shell, 0u, mean_depth);
+    }
+  }
+  // Block-exclusive scan of counts.
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / kWave;
+  int32_t inc = wave_inclusive_sum(count);
+  if (lane == kWave - 1) s_wave[wave] = inc;
+  __syncthreads();
+  int32_t before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    if (w < wave) before += s_wave[w];
+    total += s_wave[w];
+  }
+  s_prefix[threadIdx.x] = before + inc - count;
+  if (threadIdx.x == 0) s_prefix[kBlock] = total;
+  // Output position of the block's first key (the inclusive offset of the face before).
+  __shared__ int64_t s_block_start;
+  if (threadIdx.x == 0) {
+    const int64_t q0 = int64_t(b) * F + int64_t(blockIdx.x) * kBlock;
+    block_start = (q0 == 0) ? 0 : offsets[q0 - 1];
+    s_block_start = block_start;
+  }
+  __syncthreads();
+  const int64_t base = s_block_start;
+  const uint64_t tile_mask = field_mask(kf.tile_bits);
+  for (int32_t j = threadIdx.x; j < total; j += kBlock) {
+    // Owner: the last face whose prefix is <= j (binary search over 256 entries).
+    int lo = 0, hi = kBlock - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_prefix[mid] <= j) lo = mid; else hi = mid - 1;
+    }
+    const int32_t local = j - s_prefix[lo];
+    const int32_t w = s_w[lo];
+    const uint32_t row = uint32_t(s_r0[lo] + local / w);
+    const uint32_t col = uint32_t(s_c0[lo] + local % w);
+    const uint32_t tile = row * uint32_t(grid_cols) + col;
+    const uint64_t key = s_key_hi[lo] | ((uint64_t(tile) & tile_mask) << kf.tile_offset);
+    keys[base + j] = static_cast<int64_t>(key);
+    values[base + j] = static_cast<int32_t>(int64_t(blockIdx.x) * kBlock + lo);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Delineation (delineate_tiles.slang:12-58)
+// ------------------------------------------------------------------------------------
+
+__device__ __forceinline__ void run_index(uint64_t key, const KeyFields& kf, bool separate,
+                                          uint32_t& cam, uint32_t& shell, uint32_t& tile) {
+  cam = key_field(key, kf.camera_offset, kf.camera_bits);
+  shell = separate ? key_field(key, kf.shell_offset, kf.shell_bits) : 0u;
+  tile = key_field(key, kf.tile_offset, kf.tile_bits);
+}
+
+__global__ void __launch_bounds__(kBlock) delineate_kernel(const int64_t* keys, int64_t n,
+                                                           KeyFields kf, int separate, int B,
+                                                           int S_out, int T, int64_t* ranges) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c, s, t;
+  run_index(static_cast<uint64_t>(keys[i]), kf, separate != 0, c, s, t);
+  auto slot = [&](uint32_t cc, uint32_t ss, uint32_t tt) -> int64_t {
+    return ((int64_t(cc) * S_out + ss) * T + tt) * 2;
+  };
+  if (i > 0) {
+    uint32_t pc, ps, pt;
+    run_index(static_cast<uint64_t>(keys[i - 1]), kf, separate != 0, pc, ps, pt);
+    if (pc != c || ps != s || pt != t) {
+      ranges[slot(pc, ps, pt) + 1] = i;
+      ranges[slot(c, s, t) + 0] = i;
+    }
+  }
+  if (i == n - 1) ranges[slot(c, s, t) + 1] = n;
+}
+
+// ------------------------------------------------------------------------------------
+// Camera positions: inverse(E)[:3, 3] in double precision (Gauss-Jordan, partial pivot).
+// ------------------------------------------------------------------------------------
+
+__global__ void camera_positions_kernel(const float* E, int B, float* out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  double m[4][8];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 8; ++c) m[r][c] = (c < 4) ? double(E[16 * b + 4 * r + c]) : (c - 4 == r ? 1.0 : 0.0);
+  for (int c = 0; c < 4; ++c) {
+    int p = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (fabs(m[r][c]) > fabs(m[p][c])) p = r;
+    if (p != c)
+      for (int q = 0; q < 8; ++q) {
+        const double tmp = m[c][q];
+        m[c][q] = m[p][q];
+        m[p][q] = tmp;
+      }
+    const double inv = 1.0 / m[c][c];
+    for (int q = 0; q < 8; ++q) m[c][q] *= inv;
+    for (int r = 0; r < 4; ++r) {
+      if (r == c) continue;
+      const double fct = m[r][c];
+      for (int q = 0; q < 8; ++q) m[r][q] -= fct * m[c][q];
+    }
+  }
+  for (int r = 0; r < 3; ++r) out[3 * b + r] = static_cast<float>(m[r][4 + 3]);
+}
+
+// ------------------------------------------------------------------------------------
+// Spherical harmonics (evaluate_spherical_harmonics.slang:124-270)
+// ------------------------------------------------------------------------------------
+
+constexpr float kC0 = 0.28209479177387814f;
+constexpr float kC1 = 0.4886025119029199f;
+constexpr float kC2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f,
+                          -1.0925484305920792f, 0.5462742152960396f};
+constexpr float kC3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
+                          0.3731763325901154f,  -0.4570457994644658f, 1.445305721320277f,
+                          -0.5900435899266435f};
+
+// Basis values with the reference's signs; only the first (deg+1)^2 are written.
+__device__ __forceinline__ void sh_basis(float x, float y, float z, int deg, float* out) {
+  out[0] = kC0;
+  if (deg > 0) {
+    out[1] = -kC1 * y;
+    out[2] = kC1 * z;
+    out[3] = -kC1 * x;
+  }
+  if (deg > 1) {
+    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+    out[4] = kC2[0] * xy;
+    out[5] = kC2[1] * yz;
+    out[6] = kC2[2] * (2.0f * zz - xx - yy);
+    out[7] = kC2[3] * xz;
+    out[8] = kC2[4] * (xx - yy);
+    if (deg > 2) {
+      out[9] = kC3[0] * y * (3.0f * xx - yy);
+      out[10] = kC3[1] * xy * z;
+      out[11] = kC3[2] * y * (4.0f * zz - xx - yy);
+      out[12] = kC3[3] * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+      out[13] = kC3[4] * x * (4.0f * zz - xx - yy);
+      out[14] = kC3[5] * z * (xx - yy);
+      out[15] = kC3[6] * x * (xx - 3.0f * yy);
+    }
+  }
+}
+
+__device__ __forceinline__ float3 view_dir(const float* verts, int64_t v, const float* cam) {
+  float dx = verts[3 * v + 0] - cam[0];
+  float dy = verts[3 * v + 1] - cam[1];
+  float dz = verts[3 * v + 2] - cam[2];
+  const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+  return make_float3(dx * inv, dy * inv, dz * inv);
+}
+
+template <int kDeg>
+__global__ void __launch_bounds__(kBlock) sh_forward_kernel(const float* verts, int64_t V,
+                                                            const float* sh, const float* cams,
+                                                            int B, float* colors) {
+  constexpr int kN = (kDeg + 1) * (kDeg + 1);
+  const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  float coef[kN][3];
+#pragma unroll
+  for (int k = 0; k < kN; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) coef[k][c] = sh[(int64_t(k) * V + v) * 3 + c];
+  for (int b = 0; b < B; ++b) {
+    const float3 d = view_dir(verts, v, cams + 3 * b);
+    float basis[16];
+    sh_basis(d.x, d.y, d.z, kDeg, basis);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float acc = basis[0] * coef[0][c];
+#pragma unroll
+      for (int k = 1; k < kN; ++k) acc += basis[k] * coef[k][c];
+      colors[(int64_t(b) * V + v) * 3 + c] = fmaxf(acc + 0.5f, 0.0f);
+    }
+  }
+}
+
+template <int kDeg>
+__global__ void __launch_bounds__(kBlock) sh_backward_kernel(const float* verts, int64_t V,
+                                                             const float* sh, const float* cams,
+                                                             int B, const float* gcol,
+                                                             float* gsh) {
+  constexpr int kN = (kDeg + 1) * (kDeg + 1);
+  const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  float coef[kN][3];
+  float grad[kN][3];
+#pragma unroll
+  for (int k = 0; k < kN; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      coef[k][c] = sh[(int64_t(k) * V + v) * 3 + c];
+      grad[k][c] = 0.f;
+    }
+  for (int b = 0; b < B; ++b) {
+    const float3 d = view_dir(verts, v, cams + 3 * b);
+    float basis[16];
+    sh_basis(d.x, d.y, d.z, kDeg, basis);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float acc = basis[0] * coef[0][c];
+#pragma unroll
+      for (int k = 1; k < kN; ++k) acc += basis[k] * coef[k][c];
+      // d max(x, 0) / dx passes only for x > 0.
+      const float g = (acc + 0.5f > 0.0f) ? gcol[(int64_t(b) * V + v) * 3 + c] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < kN; ++k) grad[k][c] += basis[k] * g;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kN; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gsh[(int64_t(k) * V + v) * 3 + c] += grad[k][c];
+}
+
+}  // namespace
+}  // namespace mt
+
+extern "C" int mt_project_vertices(const float* vertices, int64_t V, const float* extrinsics,
+                                   const float* intrinsics, int32_t B, float* out_xy,
+                                   float* out_depth, mt_stream_t stream) {
+  MT_REQUIRE(V >= 0 && B >= 0, "mt_project_vertices: negative size");
+  if (V == 0 || B == 0) return MT_OK;
+  mt::project_kernel<<<mt::blocks_for(V, mt::kBlock), mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+      vertices, V, extrinsics, intrinsics, B, out_xy, out_depth);
+  return mt::check_launch("mt_project_vertices");
+}
+
+extern "C" int mt_bound_triangles(const float* xy, const float* depths, int64_t V,
+                                  const int32_t* faces, int64_t F, int32_t B, int32_t tile_h,
+                                  int32_t tile_w, int32_t row_min, int32_t col_min,
+                                  int32_t row_max, int32_t col_max, float near_plane,
+                                  int32_t* tile_min, int32_t* tile_max, int32_t* counts,
+                                  mt_stream_t stream) {
+  MT_REQUIRE(F >= 0 && B >= 0 && tile_h > 0 && tile_w > 0, "mt_bound_triangles: bad sizes");
+  if (F == 0 || B == 0) return MT_OK;
+  mt::bound_kernel<<<mt::blocks_for(F, mt::kBlock), mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+      xy, depths, V, faces, F, B, tile_h, tile_w, row_min, col_min, row_max, col_max, near_plane,
+      tile_min, tile_max, counts);
+  return mt::check_launch("mt_bound_triangles");
+}
+
+extern "C" int mt_generate_keys(const float* depths, int64_t V, const int32_t* faces, int64_t F,
+                                const int32_t* shell_face_boundaries, int32_t S,
+                                const int32_t* tile_min, const int32_t* tile_max,
+                                const int64_t* offsets, int32_t B, int32_t grid_cols,
+                                mt_key_layout layout, int64_t* keys, int32_t* values,
+                                mt_stream_t stream) {
+  MT_REQUIRE(F >= 0 && B >= 0 && B <= 65535, "mt_generate_keys: bad sizes");
+  if (F == 0 || B == 0) return MT_OK;
+  dim3 grid(mt::blocks_for(F, mt::kBlock), static_cast<unsigned>(B));
+  mt::keys_kernel<<<grid, mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+      depths, V, faces, F, shell_face_boundaries, S, tile_min, tile_max, offsets, grid_cols,
+      mt::to_fields(layout), keys, values);
+  return mt::check_launch("mt_generate_keys");
+}
+
+extern "C" int mt_delineate_tiles(const int64_t* sorted_keys, int64_t n, mt_key_layout layout,
+                                  int32_t separate_shells, int32_t B, int32_t S_out, int32_t T,
+                                  int64_t* tile_ranges, mt_stream_t stream) {
+  MT_REQUIRE(n >= 0, "mt_delineate_tiles: negative length");
+  if (n == 0) return MT_OK;
+  mt::delineate_kernel<<<mt::blocks_for(n, mt::kBlock), mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+      sorted_keys, n, mt::to_fields(layout), separate_shells, B, S_out, T, tile_ranges);
+  return mt::check_launch("mt_delineate_tiles");
+}
+
+extern "C" int mt_camera_positions(const float* extrinsics, int32_t B, float* out,
+                                   mt_stream_t stream) {
+  MT_REQUIRE(B >= 0, "mt_camera_positions: negative size");
+  if (B == 0) return MT_OK;
+  mt::camera_positions_kernel<<<mt::blocks_for(B, 64), 64, 0, static_cast<hipStream_t>(stream)>>>(
+      extrinsics, B, out);
+  return mt::check_launch("mt_camera_positions");
+}
+
+#define MT_SH_DISPATCH(KERNEL, ...)                                                          \
+  switch (sh_degree) {                                                                       \
+    case 0: mt::KERNEL<0><<<grid, mt::kBlock, 0, s>>>(__VA_ARGS__); break;                   \
+    case 1: mt::KERNEL<1><<<grid, mt::kBlock, 0, s>>>(__VA_ARGS__); break;                   \
+    case 2: mt::KERNEL<2><<<grid, mt::kBlock, 0, s>>>(__VA_ARGS__); break;                   \
+    default: mt::KERNEL<3><<<grid, mt::kBlock, 0, s>>>(__VA_ARGS__); break;                  \
+  }
+
+extern "C" int mt_evaluate_sh_forward(const float* vertices, int64_t V, const float* sh,
+                                      int32_t num_sh, int32_t sh_degree,
+                                      const float* camera_positions, int32_t B, float* colors,
+                                      mt_stream_t stream) {
+  MT_REQUIRE(sh_degree >= 0 && sh_degree <= 3, "mt_evaluate_sh_forward: degree must be in [0, 3]");
+  MT_REQUIRE(num_sh >= (sh_degree + 1) * (sh_degree + 1),
+             "mt_evaluate_sh_forward: %d coefficients < (deg+1)^2", num_sh);
+  if (V == 0 || B == 0) return MT_OK;
+  const dim3 grid(mt::blocks_for(V, mt::kBlock));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  MT_SH_DISPATCH(sh_forward_kernel, vertices, V, sh, camera_positions, B, colors);
+  return mt::check_launch("mt_evaluate_sh_forward");
+}
+
+extern "C" int mt_evaluate_sh_backward(const float* vertices, int64_t V, const float* sh,
+                                       int32_t num_sh, int32_t sh_degree,
+                                       const float* camera_positions, int32_t B,
+                                       const float* grad_colors, float* grad_sh,
+                                       mt_stream_t stream) {
+  MT_REQUIRE(sh_degree >= 0 && sh_degree <= 3, "mt_evaluate_sh_backward: degree must be in [0, 3]");
+  MT_REQUIRE(num_sh >= (sh_degree + 1) * (sh_degree + 1),
+             "mt_evaluate_sh_backward: %d coefficients < (deg+1)^2", num_sh);
+  if (V == 0 || B == 0) return MT_OK;
+  const dim3 grid(mt::blocks_for(V, mt::kBlock));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  MT_SH_DISPATCH(sh_backward_kernel, vertices, V, sh, camera_positions, B, grad_colors, grad_sh);
+  return mt::check_launch("mt_evaluate_sh_backward");
+}

@@ -1,0 +1,352 @@
+// Stable LSD radix sort of (int64 key, int32 value) pairs in signed key order:
+// onesweep, 8-bit digits, wave64 multi-split ranking, decoupled look-back across tiles.
+//
+//   init       zero the digit histograms, tickets and look-back words
+//   histogram  one read of all keys -> counts of every digit position (8 x 256)
+//   prepare    exclusive digit offsets per pass; a pass whose digit is the same for every
+//              key is marked inactive; buffer routing so that the last active pass lands
+//              in the output (input -> [tmp <-> out] ... -> out)
+//   pass x 8   inactive passes return at once; active ones read 12 B and write 12 B per
+//              pair. A tile of 4096 pairs is ranked in registers + LDS, published to the
+//              look-back array, reordered in LDS by digit and written as contiguous runs.
+//   copy       only if no pass was active (all keys equal): out = in
+//
+// All decisions are taken on the device, so the sequence is graph-capturable and never
+// synchronises the host.
+#include "common.h"
+
+namespace mt {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / kWave;
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;
+constexpr int kRadix = 256;
+constexpr int kPasses = 8;
+
+// Look-back word: [63:62] state, [61:59] pass tag, [58:0] count.
+constexpr uint64_t kStateAggregate = 1ull << 62;
+constexpr uint64_t kStatePrefix = 2ull << 62;
+constexpr uint64_t kStateMask = 3ull << 62;
+constexpr uint64_t kValueMask = (1ull << 59) - 1;
+
+struct SortWorkspace {
+  unsigned long long* hist;  // [8][256]
+  int64_t* digit_base;       // [8][256]
+  int32_t* info;             // [0..7] active, [8..15] src, [16..23] dst, [24] num_active
+  uint32_t* tickets;         // [8]
+  uint64_t* lookback;        // [tiles][256]
+  int64_t* tmp_keys;
+  int32_t* tmp_values;
+};
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+size_t workspace_bytes(int64_t n) {
+  const int64_t tiles = (n + kTile - 1) / kTile;
+  size_t b = 0;
+  b += align_up(kPasses * kRadix * 8, 256);
+  b += align_up(kPasses * kRadix * 8, 256);
+  b += 256;
+  b += 256;
+  b += align_up(size_t(tiles) * kRadix * 8, 256);
+  b += align_up(size_t(n) * 8, 256);
+  b += align_up(size_t(n) * 4, 256);
+  return b;
+}
+
+SortWorkspace carve(void* ws, int64_t n) {
+  const int64_t tiles = (n + kTile - 1) / kTile;
+  char* p = static_cast<char*>(ws);
+  SortWorkspace w;
+  w.hist = reinterpret_cast<unsigned long long*>(p);
+  p += align_up(kPasses * kRadix * 8, 256);
+  w.digit_base = reinterpret_cast<int64_t*>(p);
+  p += align_up(kPasses * kRadix * 8, 256);
+  w.info = reinterpret_cast<int32_t*>(p);
+  p += 256;
+  w.tickets = reinterpret_cast<uint32_t*>(p);
+  p += 256;
+  w.lookback = reinterpret_cast<uint64_t*>(p);
+  p += align_up(size_t(tiles) * kRadix * 8, 256);
+  w.tmp_keys = reinterpret_cast<int64_t*>(p);
+  p += align_up(size_t(n) * 8, 256);
+  w.tmp_values = reinterpret_cast<int32_t*>(p);
+  return w;
+}
+
+__device__ __forceinline__ uint32_t digit_of(uint64_t key, int pass) {
+  // Flipping the sign bit turns signed order into unsigned order.
+  return static_cast<uint32_t>(((key ^ (1ull << 63)) >> (8 * pass)) & 0xFFull);
+}
+
+__global__ void init_kernel(SortWorkspace w, int64_t lookback_words) {
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = tid; i < kPasses * kRadix; i += stride) w.hist[i] = 0ull;
+  if (tid < kPasses) w.tickets[tid] = 0u;
+  for (int64_t i = tid; i < lookback_words; i += stride) w.lookback[i] = 0ull;
+}
+
+__global__ void __launch_bounds__(kBlock) histogram_kernel(const int64_t* keys, int64_t n,
+                                                           SortWorkspace w) {
+  __shared__ uint32_t s_hist[kPasses][kRadix];
+  for (int i = threadIdx.x; i < kPasses * kRadix; i += kBlock) (&s_hist[0][0])[i] = 0u;
+  __syncthreads();
+  // Each thread walks its keys keeping, per digit position, the current digit and its run
+  // length, and flushes a run when the digit changes: uniform high digits cost one
+  // atomic per run instead of one per key.
+  uint32_t cur[kPasses];
+  uint32_t run[kPasses];
+#pragma unroll
+  for (int p = 0; p < kPasses; ++p) {
+    cur[p] = 0u;
+    run[p] = 0u;
+  }
+  const int64_t stride = int64_t(gridDim.x) * kBlock;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+    const uint64_t k = static_cast<uint64_t>(keys[i]);
+#pragma unroll
+    for (int p = 0; p < kPasses; ++p) {
+      const uint32_t d = digit_of(k, p);
+      if (run[p] != 0u && d != cur[p]) {
+        atomicAdd(&s_hist[p][cur[p]], run[p]);
+        run[p] = 0u;
+      }
+      cur[p] = d;
+      ++run[p];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < kPasses; ++p)
+    if (run[p]) atomicAdd(&s_hist[p][cur[p]], run[p]);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kPasses * kRadix; i += kBlock) {
+    const uint32_t c = (&s_hist[0][0])[i];
+    if (c) atomicAdd(&w.hist[i], static_cast<unsigned long long>(c));
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) prepare_kernel(SortWorkspace w, int64_t n) {
+  __shared__ int32_t s_active[kPasses];
+  __shared__ int64_t s_wave[kWaves];
+  const int d = threadIdx.x;
+  for (int p = 0; p < kPasses; ++p) {
+    const int64_t c = static_cast<int64_t>(w.hist[p * kRadix + d]);
+    // Block exclusive scan of the 256 bins.
+    const int lane = __lane_id();
+    const int wave = d / kWave;
+    int64_t inc = wave_inclusive_sum(c);
+    if (lane == kWave - 1) s_wave[wave] = inc;
+    __syncthreads();
+    int64_t before = 0;
+    for (int q = 0; q < wave; ++q) before += s_wave[q];
+    w.digit_base[p * kRadix + d] = before + inc - c;
+    const int full = __syncthreads_or(c == n);
+    if (d == 0) s_active[p] = full ? 0 : 1;
+    __syncthreads();
+  }
+  if (d == 0) {
+    int num_active = 0;
+    for (int p = 0; p < kPasses; ++p) num_active += s_active[p];
+    // Buffers: 0 = input, 1 = output, 2 = temporary. The last active pass writes 1.
+    int remaining = num_active;
+    int src = 0;
+    for (int p = 0; p < kPasses; ++p) {
+      w.info[p] = s_active[p];
+      if (!s_active[p]) {
+        w.info[8 + p] = -1;
+        w.info[16 + p] = -1;
+        continue;
+      }
+      const int dst = (remaining % 2 == 1) ? 1 : 2;
+      w.info[8 + p] = src;
+      w.info[16 + p] = dst;
+      src = dst;
+      --remaining;
+    }
+    w.info[24] = num_active;
+  }
+}
+
+__device__ __forceinline__ uint64_t lb_load(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys,
+                                                          const int32_t* in_values,
+                                                          int64_t* out_keys, int32_t* out_values,
+                                                          int64_t n, int pass, SortWorkspace w) {
+  if (w.info[pass] == 0) return;
+  const int src_sel = w.info[8 + pass];
+  const int dst_sel = w.info[16 + pass];
+  const int64_t* src_k = src_sel == 0 ? in_keys : (src_sel == 1 ? out_keys : w.tmp_keys);
+  const int32_t* src_v = src_sel == 0 ? in_values : (src_sel == 1 ? out_values : w.tmp_values);
+  int64_t* dst_k = dst_sel == 1 ? out_keys : w.tmp_keys;
+  int32_t* dst_v = dst_sel == 1 ? out_values : w.tmp_values;
+
+  __shared__ uint32_t s_wave_count[kWaves][kRadix];
+  __shared__ uint32_t s_local_start[kRadix];
+  __shared__ int64_t s_global_base[kRadix];
+  __shared__ int64_t s_keys[kTile];
+  __shared__ int32_t s_vals[kTile];
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_scan_wave[kWaves];
+
+  for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&s_wave_count[0][0])[i] = 0u;
+  if (threadIdx.x == 0) s_tile = atomicAdd(&w.tickets[pass], 1u);
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int64_t base = tile * kTile;
+  const int wave = threadIdx.x / kWave;
+  const int lane = __lane_id();
+  const uint64_t lt = lanemask_lt();
+
+  // Items: wave w owns [base + w * 1024, base + (w + 1) * 1024), item i of lane l is at
+  // offset i * 64 + l, so (wave, item, lane) order is the input order.
+  int64_t keys[kItems];
+  int32_t vals[kItems];
+  uint32_t ranks[kItems];
+  const int64_t wbase = base + int64_t(wave) * (kItems * kWave);
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int64_t idx = wbase + i * kWave + lane;
+    const bool valid = idx < n;
+    keys[i] = valid ? src_k[idx] : 0;
+    vals[i] = valid ? src_v[idx] : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int64_t idx = wbase + i * kWave + lane;
+    const bool valid = idx < n;
+    const uint32_t d = digit_of(static_cast<uint64_t>(keys[i]), pass);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const bool set = (d >> bit) & 1u;
+      const uint64_t m = __ballot(set);
+      peers &= set ? m : ~m;
+    }
+    const uint32_t before = static_cast<uint32_t>(__popcll(peers & lt));
+    const uint32_t old = valid ? s_wave_count[wave][d] : 0u;
+    ranks[i] = old + before;
+    if (valid && before == 0) s_wave_count[wave][d] = old + static_cast<uint32_t>(__popcll(peers));
+  }
+  __syncthreads();
+
+  // Per digit: offsets of each wave inside the tile's run of that digit, and the count.
+  const int d = threadIdx.x;
+  uint32_t tile_count = 0;
+#pragma unroll
+  for (int q = 0; q < kWaves; ++q) {
+    const uint32_t c = s_wave_count[q][d];
+    s_wave_count[q][d] = tile_count;
+    tile_count += c;
+  }
+  const uint64_t tag = uint64_t(pass) << 59;
+  uint64_t* lb = w.lookback;
+  if (tile == 0) {
+    lb_store(&lb[d], kStatePrefix | tag | tile_count);
+  } else {
+    lb_store(&lb[tile * kRadix + d], kStateAggregate | tag | tile_count);
+  }
+  // Exclusive scan of the tile counts over digits: start of each digit's run in LDS.
+  uint32_t inc = wave_inclusive_sum(tile_count);
+  if (lane == kWave - 1) s_scan_wave[wave] = inc;
+  // Look-back over preceding tiles for this digit.
+  int64_t prefix = 0;
+  if (tile > 0) {
+    int64_t t = tile - 1;
+    while (true) {
+      uint64_t word;
+      do {
+        word = lb_load(&lb[t * kRadix + d]);
+      } while ((word & kStateMask) == 0 || (word & ~kStateMask & ~kValueMask) != tag);
+      prefix += static_cast<int64_t>(word & kValueMask);
+      if ((word & kStateMask) == kStatePrefix) break;
+      --t;
+    }
+    lb_store(&lb[tile * kRadix + d], kStatePrefix | tag | uint64_t(prefix + tile_count));
+  }
+  __syncthreads();
+  uint32_t wave_before = 0;
+  for (int q = 0; q < wave; ++q) wave_before += s_scan_wave[q];
+  const uint32_t local_start = wave_before + inc - tile_count;
+  s_local_start[d] = local_start;
+  s_global_base[d] = w.digit_base[pass * kRadix + d] + prefix;
+  __syncthreads();
+
+  // Reorder the tile by digit in LDS.
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int64_t idx = wbase + i * kWave + lane;
+    if (idx < n) {
+      const uint32_t dd = digit_of(static_cast<uint64_t>(keys[i]), pass);
+      const uint32_t pos = s_local_start[dd] + s_wave_count[wave][dd] + ranks[i];
+      s_keys[pos] = keys[i];
+      s_vals[pos] = vals[i];
+    }
+  }
+  __syncthreads();
+  const int64_t valid_count = (n - base) < kTile ? (n - base) : kTile;
+  for (int j = threadIdx.x; j < valid_count; j += kBlock) {
+    const int64_t k = s_keys[j];
+    const uint32_t dd = digit_of(static_cast<uint64_t>(k), pass);
+    const int64_t g = s_global_base[dd] + (j - static_cast<int64_t>(s_local_start[dd]));
+    dst_k[g] = k;
+    dst_v[g] = s_vals[j];
+  }
+}
+
+__global__ void copy_if_idle_kernel(const int64_t* in_keys, const int32_t* in_values,
+                                    int64_t* out_keys, int32_t* out_values, int64_t n,
+                                    const int32_t* info) {
+  if (info[24] != 0) return;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    out_keys[i] = in_keys[i];
+    out_values[i] = in_values[i];
+  }
+}
+
+}  // namespace
+}  // namespace mt
+
+extern "C" size_t mt_sort_workspace_bytes(int64_t n) { return mt::workspace_bytes(n < 1 ? 1 : n); }
+
+extern "C" int mt_sort_keys(const int64_t* keys_in, const int32_t* values_in, int64_t n,
+                            int64_t* keys_out, int32_t* values_out, void* workspace,
+                            size_t workspace_bytes, mt_stream_t stream) {
+  using namespace mt;
+  MT_REQUIRE(n >= 0, "mt_sort_keys: negative length");
+  if (n == 0) return MT_OK;
+  MT_REQUIRE(keys_in && values_in && keys_out && values_out, "mt_sort_keys: null pointer");
+  MT_REQUIRE(keys_in != keys_out && values_in != values_out, "mt_sort_keys: in-place sort is not supported");
+  const size_t need = mt::workspace_bytes(n);
+  if (workspace == nullptr || workspace_bytes < need) {
+    set_error("mt_sort_keys: workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
+    return MT_ERR_WORKSPACE_TOO_SMALL;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  SortWorkspace w = carve(workspace, n);
+  const int64_t tiles = (n + kTile - 1) / kTile;
+  const int64_t lb_words = tiles * kRadix;
+  init_kernel<<<blocks_for(lb_words < 4096 ? 4096 : lb_words, 256 * 8) , 256, 0, s>>>(w, lb_words);
+  const unsigned hist_blocks = static_cast<unsigned>(
+      (n + int64_t(kBlock) * 64 - 1) / (int64_t(kBlock) * 64) < 2048
+          ? (n + int64_t(kBlock) * 64 - 1) / (int64_t(kBlock) * 64)
+          : 2048);
+  histogram_kernel<<<hist_blocks, kBlock, 0, s>>>(keys_in, n, w);
+  prepare_kernel<<<1, kBlock, 0, s>>>(w, n);
+  for (int p = 0; p < kPasses; ++p) {
+    onesweep_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, s>>>(keys_in, values_in, keys_out,
+                                                                   values_out, n, p, w);
+  }
+  copy_if_idle_kernel<<<blocks_for(n < 65536 * 4 ? n : 65536 * 4, 256 * 4), 256, 0, s>>>(
+      keys_in, values_in, keys_out, values_out, n, w.info);
+  return check_launch("mt_sort_keys");
+}

@@ -1,0 +1,318 @@
+// Multi-level-set marching cubes over the sparse column structure.
+//
+// Column c owns the edges from its origin vertex lower[0][c] along +x/+y/+z to
+// lower[1..3][c] (negative = no edge). For level set s an edge crosses iff exactly one
+// end is below the iso value; the triangle vertex is placed at
+// t = clamp((iso - d0) / (d1 - d0), 0, 1) and written at
+// vertex_offsets[s * N + c] + (number of crossing axes before this one).
+#include "common.h"
+
+#define MT_MC_TABLE_QUALIFIER static __device__ const
+#include "mc_tables.h"
+
+namespace mt {
+namespace {
+
+constexpr int kBlock = 256;
+
+struct ColumnSdf {
+  float d0;
+  float d[3];
+  int32_t v0;
+  int32_t v[3];
+};
+
+__device__ __forceinline__ ColumnSdf load_column(const float* sdf, const int32_t* lower,
+                                                 int64_t n, int64_t c) {
+  ColumnSdf col;
+  col.v0 = lower[c];
+  col.d0 = sdf[col.v0];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    col.v[a] = lower[(a + 1) * n + c];
+    col.d[a] = (col.v[a] >= 0) ? sdf[col.v[a]] : col.d0;
+  }
+  return col;
+}
+
+__device__ __forceinline__ bool crosses(float d0, float d1, float iso) {
+  return (d0 < iso) != (d1 < iso);
+}
+
+__device__ __forceinline__ float edge_t(float d0, float d1, float iso) {
+  if (d0 == d1) return 0.5f;
+  const float t = (iso - d0) / (d1 - d0);
+  return fminf(fmaxf(t, 0.0f), 1.0f);
+}
+
+__global__ void __launch_bounds__(kBlock) count_vertices_kernel(const float* sdf, const int32_t* lower,
+                                                                int64_t n, const float* level_sets,
+                                                                int S, int32_t* counts) {
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const ColumnSdf col = load_column(sdf, lower, n, c);
+  for (int s = 0; s < S; ++s) {
+    const float iso = level_sets[s];
+    counts[s * n + c] = int(crosses(col.d0, col.d[0], iso)) + int(crosses(col.d0, col.d[1], iso)) +
+                        int(crosses(col.d0, col.d[2], iso));
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) create_vertices_kernel(
+    const float* gv, const float* gsdf, const float* gsh, int64_t ng, int nsh, const int32_t* lower,
+    int64_t n, const float* level_sets, int S, const int32_t* offsets, int64_t nt, float* ov,
+    float* od, float* osh, uint8_t* otype) {
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const ColumnSdf col = load_column(gsdf, lower, n, c);
+  const float x0 = gv[3 * int64_t(col.v0) + 0];
+  const float y0 = gv[3 * int64_t(col.v0) + 1];
+  const float z0 = gv[3 * int64_t(col.v0) + 2];
+  for (int s = 0; s < S; ++s) {
+    const float iso = level_sets[s];
+    bool cr[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) cr[a] = crosses(col.d0, col.d[a], iso);
+    if (!(cr[0] || cr[1] || cr[2])) continue;
+    int64_t idx = offsets[s * n + c];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (!cr[a]) continue;
+      const int64_t g1 = col.v[a];
+      const float d1 = col.d[a];
+      const float t = edge_t(col.d0, d1, iso);
+      const float x1 = gv[3 * g1 + 0], y1 = gv[3 * g1 + 1], z1 = gv[3 * g1 + 2];
+      ov[3 * idx + 0] = x0 + t * (x1 - x0);
+      ov[3 * idx + 1] = y0 + t * (y1 - y0);
+      ov[3 * idx + 2] = z0 + t * (z1 - z0);
+      od[idx] = col.d0 + t * (d1 - col.d0);
+      otype[idx] = static_cast<uint8_t>(a);
+      for (int k = 0; k < nsh; ++k) {
+        const float* s0 = gsh + (int64_t(k) * ng + col.v0) * 3;
+        const float* s1 = gsh + (int64_t(k) * ng + g1) * 3;
+        float* o = osh + (int64_t(k) * nt + idx) * 3;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) o[ch] = s0[ch] + t * (s1[ch] - s0[ch]);
+      }
+      ++idx;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) create_vertices_backward_kernel(
+    const float* gsdf, int64_t ng, int nsh, const int32_t* lower, int64_t n,
+    const float* level_sets, int S, const int32_t* offsets, int64_t nt, const float* g_v,
+    const float* g_d, const float* g_sh, float* gg_v, float* gg_d, float* gg_sh) {
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const ColumnSdf col = load_column(gsdf, lower, n, c);
+  const bool want_v = g_v != nullptr && gg_v != nullptr;
+  const bool want_d = g_d != nullptr && gg_d != nullptr;
+  const bool want_sh = g_sh != nullptr && gg_sh != nullptr;
+  // The origin's gradient is accumulated over all of the column's crossings, then
+  // added once; each axis end receives the sum over level sets.
+  float acc0_v[3] = {0.f, 0.f, 0.f};
+  float acc0_d = 0.f;
+  bool any = false;
+  for (int a = 0; a < 3; ++a) {
+    if (col.v[a] < 0) continue;
+    const int64_t g1 = col.v[a];
+    float acc1_v[3] = {0.f, 0.f, 0.f};
+    float acc1_d = 0.f;
+    bool axis_any = false;
+    for (int s = 0; s < S; ++s) {
+      const float iso = level_sets[s];
+      if (!crosses(col.d0, col.d[a], iso)) continue;
+      int64_t idx = offsets[s * n + c];
+      for (int b = 0; b < a; ++b) idx += crosses(col.d0, col.d[b], iso) ? 1 : 0;
+      const float t = edge_t(col.d0, col.d[a], iso);
+      const float w0 = 1.0f - t;
+      axis_any = true;
+      if (want_v) {
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+          const float g = g_v[3 * idx + ch];
+          acc0_v[ch] += w0 * g;
+          acc1_v[ch] += t * g;
+        }
+      }
+      if (want_d) {
+        const float g = g_d[idx];
+        acc0_d += w0 * g;
+        acc1_d += t * g;
+      }
+      if (want_sh) {
+        for (int k = 0; k < nsh; ++k) {
+          const float* gs = g_sh + (int64_t(k) * nt + idx) * 3;
+          float* o0 = gg_sh + (int64_t(k) * ng + col.v0) * 3;
+          float* o1 = gg_sh + (int64_t(k) * ng + g1) * 3;
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) {
+            const float g = gs[ch];
+            atomicAdd(o0 + ch, w0 * g);
+            atomicAdd(o1 + ch, t * g);
+          }
+        }
+      }
+    }
+    if (!axis_any) continue;
+    any = true;
+    if (want_v) {
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) atomicAdd(gg_v + 3 * g1 + ch, acc1_v[ch]);
+    }
+    if (want_d) atomicAdd(gg_d + g1, acc1_d);
+  }
+  if (!any) return;
+  if (want_v) {
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) atomicAdd(gg_v + 3 * int64_t(col.v0) + ch, acc0_v[ch]);
+  }
+  if (want_d) atomicAdd(gg_d + col.v0, acc0_d);
+}
+
+__global__ void __launch_bounds__(kBlock) count_faces_kernel(const float* sdf, const int32_t* lower,
+                                                             int64_t ncol, const int32_t* upper,
+                                                             int64_t n, const float* level_sets,
+                                                             int S, int32_t* counts,
+                                                             uint8_t* codes) {
+  const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  // Corner order (000, 100, 110, 010, 001, 101, 111, 011) in (x, y, z).
+  float cs[8];
+  cs[0] = sdf[lower[0 * ncol + v]];
+  cs[1] = sdf[lower[1 * ncol + v]];
+  cs[2] = sdf[upper[3 * n + v]];
+  cs[3] = sdf[lower[2 * ncol + v]];
+  cs[4] = sdf[lower[3 * ncol + v]];
+  cs[5] = sdf[upper[2 * n + v]];
+  cs[6] = sdf[upper[0 * n + v]];
+  cs[7] = sdf[upper[1 * n + v]];
+  for (int s = 0; s < S; ++s) {
+    const float iso = level_sets[s];
+    int code = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) code |= (cs[q] >= iso) ? (1 << q) : 0;
+    codes[s * n + v] = static_cast<uint8_t>(code);
+    counts[s * n + v] = MT_MC_CASE_START[code + 1] - MT_MC_CASE_START[code];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) create_faces_kernel(
+    const int32_t* neighbors, const int32_t* indices, int64_t n, int64_t ncol,
+    const int32_t* voffsets, const uint8_t* vtypes, const uint8_t* codes, int S,
+    const int32_t* foffsets, int32_t* faces, int32_t* voxel_indices) {
+  const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int32_t linear = indices[v];
+  for (int s = 0; s < S; ++s) {
+    const int code = codes[s * n + v];
+    const int first = MT_MC_CASE_START[code];
+    const int num = MT_MC_CASE_START[code + 1] - first;
+    if (num == 0) continue;
+    const int64_t base = foffsets[s * n + v];
+    for (int q = 0; q < num; ++q) {
+      const int e = MT_MC_CASE_EDGES[first + q];
+      const int slot = MT_MC_EDGE_OWNER[e][0];
+      const int axis = MT_MC_EDGE_OWNER[e][1];
+      const int64_t owner = (slot < 0) ? v : neighbors[slot * n + v];
+      const int64_t vo = s * ncol + owner;
+      const int32_t start = voffsets[vo];
+      const int32_t cnt = voffsets[vo + 1] - start;
+      int32_t target = -1;
+      for (int k = 0; k < cnt; ++k) {
+        if (vtypes[start + k] == axis) {
+          target = start + k;
+          break;
+        }
+      }
+      const int64_t pos = base + q;
+      const int64_t tri = pos / 3;
+      const int corner = 2 - static_cast<int>(pos % 3);
+      faces[3 * tri + corner] = target;
+      if (corner == 0) voxel_indices[tri] = linear;
+    }
+  }
+}
+
+}  // namespace
+}  // namespace mt
+
+extern "C" int mt_count_triangle_vertices(const float* sdf, const int32_t* lower_corners,
+                                          int64_t num_columns, const float* level_sets, int32_t S,
+                                          int32_t* counts, mt_stream_t stream) {
+  MT_REQUIRE(num_columns >= 0 && S >= 0, "mt_count_triangle_vertices: negative size");
+  MT_REQUIRE(int64_t(S) * num_columns < (int64_t(1) << 31),
+             "mt_count_triangle_vertices: S * columns exceeds int32 offsets");
+  if (num_columns == 0 || S == 0) return MT_OK;
+  mt::count_vertices_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0,
+                              static_cast<hipStream_t>(stream)>>>(sdf, lower_corners, num_columns,
+                                                                  level_sets, S, counts);
+  return mt::check_launch("mt_count_triangle_vertices");
+}
+
+extern "C" int mt_create_triangle_vertices_forward(
+    const float* grid_vertices, const float* grid_sdf, const float* grid_sh,
+    int64_t num_grid_vertices, int32_t num_sh, const int32_t* lower_corners, int64_t num_columns,
+    const float* level_sets, int32_t S, const int32_t* vertex_offsets,
+    int64_t num_triangle_vertices, float* out_vertices, float* out_sdf, float* out_sh,
+    uint8_t* out_types, mt_stream_t stream) {
+  MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0,
+             "mt_create_triangle_vertices_forward: negative size");
+  if (num_columns == 0 || S == 0 || num_triangle_vertices == 0) return MT_OK;
+  mt::create_vertices_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0,
+                               static_cast<hipStream_t>(stream)>>>(
+      grid_vertices, grid_sdf, grid_sh, num_grid_vertices, num_sh, lower_corners, num_columns,
+      level_sets, S, vertex_offsets, num_triangle_vertices, out_vertices, out_sdf, out_sh,
+      out_types);
+  return mt::check_launch("mt_create_triangle_vertices_forward");
+}
+
+extern "C" int mt_create_triangle_vertices_backward(
+    const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh, const int32_t* lower_corners,
+    int64_t num_columns, const float* level_sets, int32_t S, const int32_t* vertex_offsets,
+    int64_t num_triangle_vertices, const float* grad_vertices, const float* grad_sdf,
+    const float* grad_sh, float* grid_vertices_grad, float* grid_sdf_grad, float* grid_sh_grad,
+    mt_stream_t stream) {
+  MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0,
+             "mt_create_triangle_vertices_backward: negative size");
+  if (num_columns == 0 || S == 0 || num_triangle_vertices == 0) return MT_OK;
+  mt::create_vertices_backward_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0,
+                                        static_cast<hipStream_t>(stream)>>>(
+      grid_sdf, num_grid_vertices, num_sh, lower_corners, num_columns, level_sets, S,
+      vertex_offsets, num_triangle_vertices, grad_vertices, grad_sdf, grad_sh, grid_vertices_grad,
+      grid_sdf_grad, grid_sh_grad);
+  return mt::check_launch("mt_create_triangle_vertices_backward");
+}
+
+extern "C" int mt_count_triangle_faces(const float* sdf, const int32_t* lower_corners,
+                                       int64_t num_columns, const int32_t* upper_corners,
+                                       int64_t num_voxels, const float* level_sets, int32_t S,
+                                       int32_t* face_counts, uint8_t* cell_codes,
+                                       mt_stream_t stream) {
+  MT_REQUIRE(num_voxels >= 0 && num_voxels <= num_columns && S >= 0,
+             "mt_count_triangle_faces: bad sizes");
+  MT_REQUIRE(int64_t(S) * num_voxels < (int64_t(1) << 31),
+             "mt_count_triangle_faces: S * voxels exceeds int32 offsets");
+  if (num_voxels == 0 || S == 0) return MT_OK;
+  mt::count_faces_kernel<<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
+                           static_cast<hipStream_t>(stream)>>>(
+      sdf, lower_corners, num_columns, upper_corners, num_voxels, level_sets, S, face_counts,
+      cell_codes);
+  return mt::check_launch("mt_count_triangle_faces");
+}
+
+extern "C" int mt_create_triangle_faces(const int32_t* neighbors, const int32_t* indices,
+                                        int64_t num_voxels, int64_t num_columns,
+                                        const int32_t* vertex_offsets, const uint8_t* vertex_types,
+                                        const uint8_t* cell_codes, int32_t S,
+                                        const int32_t* face_offsets, int32_t* faces,
+                                        int32_t* voxel_indices, mt_stream_t stream) {
+  MT_REQUIRE(num_voxels >= 0 && S >= 0, "mt_create_triangle_faces: bad sizes");
+  if (num_voxels == 0 || S == 0) return MT_OK;
+  mt::create_faces_kernel<<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
+                            static_cast<hipStream_t>(stream)>>>(
+      neighbors, indices, num_voxels, num_columns, vertex_offsets, vertex_types, cell_codes, S,
+      face_offsets, faces, voxel_indices);
+  return mt::check_launch("mt_create_triangle_faces");
+}

@@ -1,0 +1,125 @@
+"""Drop-in replacement for the reference ``triangle_extraction`` package
+(triangle_extraction/__init__.py:1-157), running on hand-written gfx950 HIP kernels.
+
+Implemented: the voxel-structure builder (``extract_voxels`` and its three kernels), the
+multi-level-set marching cubes (``tessellate_voxels`` and its kernels), the exclusive
+scan and occupancy packing. The reference's coarse-to-fine subdivision and regularizer
+ops are exported under their names but raise ``NotImplementedError``: they are not part
+of the extract -> rasterize -> composite path.
+"""
+
+import logging
+from typing import NamedTuple
+
+import torch
+from torch import Tensor
+
+from .common import pack_occupancy as pack_occupancy
+from .common import record_function
+from .common import unpack_occupancy as unpack_occupancy
+from .compute_exclusive_cumsum import compute_exclusive_cumsum
+from .compute_vertex_occupancy import compute_vertex_occupancy
+from .count_occupancy import count_occupancy
+from .count_triangle_faces import count_triangle_faces
+from .count_triangle_vertices import count_triangle_vertices
+from .create_triangle_faces import create_triangle_faces
+from .create_triangle_vertices import CreateTriangleVertices, create_triangle_vertices
+from .create_voxels import Voxels, create_voxels
+
+
+def _not_in_this_build(name: str):
+    def fn(*args, **kwargs):
+        raise NotImplementedError(
+            f"triangle_extraction.{name} is outside the extract -> rasterize -> composite path "
+            "implemented by meshtryoshka_amd"
+        )
+
+    fn.__name__ = name
+    return fn
+
+
+compute_sdf_regularizers = _not_in_this_build("compute_sdf_regularizers")
+dilate_occupancy = _not_in_this_build("dilate_occupancy")
+get_occupancy_mesh = _not_in_this_build("get_occupancy_mesh")
+index_vertices = _not_in_this_build("index_vertices")
+upscale_occupancy = _not_in_this_build("upscale_occupancy")
+upscale_parameters = _not_in_this_build("upscale_parameters")
+write_occupancy = _not_in_this_build("write_occupancy")
+
+
+class UpscaledParameters(NamedTuple):
+    signed_distances: Tensor
+    spherical_harmonics: Tensor
+
+
+@record_function("extract_voxels")
+def extract_voxels(occupancy: Tensor) -> Voxels:
+    """Packed occupancy int32[i, j, k/32] -> Voxels (triangle_extraction/__init__.py:31-62)."""
+    i, j, k_packed = occupancy.shape
+    logging.debug(f"Extracting voxels for grid with shape ({i}, {j}, {k_packed * 32}).")
+    vertex_occupancy, vertex_offsets = compute_vertex_occupancy(occupancy)
+    compute_exclusive_cumsum(vertex_offsets)
+    num_vertices = vertex_offsets[-1].item()
+    vertex_offsets = vertex_offsets[:-1].view(vertex_occupancy.shape)
+    voxel_offsets = count_occupancy(occupancy)
+    compute_exclusive_cumsum(voxel_offsets)
+    num_voxels = voxel_offsets[-1].item()
+    voxel_offsets = voxel_offsets[:-1].view(vertex_occupancy.shape)
+    logging.debug(f"Sampled {num_vertices} vertices and {num_voxels} voxels.")
+    return create_voxels(occupancy, voxel_offsets, num_voxels, vertex_occupancy, vertex_offsets,
+                         num_vertices)
+
+
+class Triangles(NamedTuple):
+    vertices: Tensor  # f32[vertex, 3]
+    signed_distances: Tensor  # f32[vertex]
+    spherical_harmonics: Tensor  # f32[sh, vertex, 3]
+    faces: Tensor  # int32[face, 3]
+    voxel_indices: Tensor  # int32[face]
+    vertex_boundaries: Tensor  # int32[level_set]
+    face_boundaries: Tensor  # int32[level_set]
+
+
+@record_function("tessellate_voxels")
+def tessellate_voxels(signed_distances: Tensor, vertices: Tensor, spherical_harmonics: Tensor,
+                      neighbors: Tensor, lower_corners: Tensor, upper_corners: Tensor,
+                      indices: Tensor, level_sets: Tensor) -> Triangles:
+    """Nested level-set meshes of the sparse grid (triangle_extraction/__init__.py:75-157).
+
+    Level sets are expected in descending order (shell 0 outermost); each shell's
+    vertices and faces occupy one contiguous block, delimited by the cumulative
+    ``vertex_boundaries`` / ``face_boundaries``."""
+    (num_level_sets,) = level_sets.shape
+    logging.debug(f"Tessellating {neighbors.shape[1]} voxels with {num_level_sets} level sets.")
+    with torch.no_grad():
+        vertex_offsets = count_triangle_vertices(signed_distances, lower_corners, level_sets)
+        compute_exclusive_cumsum(vertex_offsets)
+    (
+        triangle_vertices,
+        triangle_signed_distances,
+        triangle_spherical_harmonics,
+        triangle_vertex_types,
+    ) = create_triangle_vertices(vertices, signed_distances, spherical_harmonics, lower_corners,
+                                 level_sets, vertex_offsets)
+    with torch.no_grad():
+        face_offsets, voxel_cell_codes = count_triangle_faces(signed_distances, lower_corners,
+                                                              upper_corners, level_sets)
+        compute_exclusive_cumsum(face_offsets)
+        faces, voxel_indices = create_triangle_faces(signed_distances, neighbors, indices,
+                                                     vertex_offsets, triangle_vertex_types,
+                                                     voxel_cell_codes, level_sets, face_offsets)
+    vertex_boundaries = vertex_offsets[1:].view(num_level_sets, -1)[:, -1].clone()
+    face_boundaries = face_offsets[1:].view(num_level_sets, -1)[:, -1].clone() // 3
+    return Triangles(triangle_vertices, triangle_signed_distances, triangle_spherical_harmonics,
+                     faces, voxel_indices, vertex_boundaries, face_boundaries)
+
+
+__all__ = [
+    "Triangles", "Voxels", "CreateTriangleVertices", "UpscaledParameters",
+    "compute_exclusive_cumsum", "compute_sdf_regularizers", "compute_vertex_occupancy",
+    "count_occupancy", "count_triangle_faces", "count_triangle_vertices",
+    "create_triangle_faces", "create_triangle_vertices", "create_voxels", "dilate_occupancy",
+    "extract_voxels", "get_occupancy_mesh", "index_vertices", "pack_occupancy",
+    "tessellate_voxels", "unpack_occupancy", "upscale_occupancy", "upscale_parameters",
+    "write_occupancy",
+]

@@ -1,0 +1,47 @@
+"""Helpers shared by the extraction ops (triangle_extraction/common.py:13-51)."""
+
+from functools import wraps
+from typing import TypeVar
+
+import torch
+from torch import Tensor
+from torch.profiler import record_function as record_function_torch
+
+T = TypeVar("T")
+
+
+def ceildiv(numerator: T, denominator: T) -> T:
+    return (numerator + denominator - 1) // denominator
+
+
+def pack_occupancy(occupancy: Tensor) -> Tensor:
+    """bool[i, j, k] -> int32[i, j, k // 32]; bit b of word (i, j, kp) is cell 32 kp + b
+    (triangle_extraction/common.py:17-26)."""
+    assert occupancy.shape[-1] % 32 == 0
+    i, j, k = occupancy.shape
+    bits = occupancy.reshape(i, j, k // 32, 32).to(torch.int64)
+    powers = 1 << torch.arange(32, device=occupancy.device, dtype=torch.int64)
+    return (bits * powers).sum(dim=-1).to(torch.uint32).view(torch.int32)
+
+
+def unpack_occupancy(packed_occupancy: Tensor) -> Tensor:
+    """Inverse of :func:`pack_occupancy` (triangle_extraction/common.py:29-35)."""
+    occupancy = packed_occupancy.view(torch.int32)
+    powers = 1 << torch.arange(32, device=occupancy.device, dtype=torch.int32)
+    bits = (occupancy[..., None] & powers) != 0
+    i, j, kp = packed_occupancy.shape
+    return bits.reshape(i, j, kp * 32)
+
+
+def record_function(tag: str):
+    """torch.profiler.record_function that keeps the wrapped signature."""
+
+    def decorator(f):
+        @wraps(f)
+        def wrapper(*args, **kwargs):
+            with record_function_torch(tag):
+                return f(*args, **kwargs)
+
+        return wrapper
+
+    return decorator

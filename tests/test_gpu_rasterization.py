@@ -1,0 +1,249 @@
+"""GPU parity of the rasterization kernels against the CPU oracle.
+
+Bit-exact: projected positions and depths, tile bounds, keys, sorted keys and values,
+tile ranges, explicit hit indices and barycentrics. Images: within 1e-4 max-abs of the
+oracle (FP32 and FP64-continuous). Gradients: within 1e-3 relative of the oracle adjoint.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from meshtryoshka_amd import config
+from meshtryoshka_amd import triangle_rasterization as tr
+from meshtryoshka_amd.triangle_rasterization.common import TileGrid
+from oracle import rasterization as orr
+from tests.helpers import OracleScene, small_sphere_spec, to_dev
+
+pytestmark = pytest.mark.gpu
+
+IMG_TOL = 1e-4
+GRAD_TOL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def scene():
+    return OracleScene(small_sphere_spec(resolution=32, image=64, sh_degree=2))
+
+
+@pytest.fixture(scope="module")
+def scene_explicit():
+    return OracleScene(small_sphere_spec(resolution=32, image=64, sh_degree=1), explicit=True)
+
+
+def _grid(sc):
+    return TileGrid(sc.tile_shape, (sc.H, sc.W), sc.B, sc.S)
+
+
+def _key_defines(explicit=False):
+    if explicit:
+        return tr.create_key_defines(("camera", 6), ("shell", 6), ("tile", 20), ("depth", 32))
+    return tr.create_key_defines(("camera", 6), ("tile", 20), ("depth", 32), ("shell", 6))
+
+
+def _front_end(sc, explicit=False):
+    t = sc.tri
+    verts = to_dev(t.vertices)
+    faces = to_dev(t.faces)
+    xy, depth = tr.project_vertices(verts, to_dev(sc.E), to_dev(sc.K))
+    b = tr.bound_triangles(xy, depth, faces, _grid(sc), sc.spec.near_plane)
+    kd = _key_defines(explicit)
+    pk = tr.generate_keys(depth, faces, to_dev(t.face_boundaries), b.tile_minima, b.tile_maxima,
+                          b.num_tiles_overlapped, _grid(sc), kd)
+    spk = tr.sort_keys(pk, _grid(sc))
+    ranges = tr.delineate_tiles(spk, _grid(sc), kd, explicit)
+    return xy, depth, b, pk, spk, ranges
+
+
+def test_front_end_bit_exact(scene):
+    xy, depth, b, pk, spk, ranges = _front_end(scene)
+    assert np.array_equal(xy.cpu().numpy().view(np.uint32), scene.xy.view(np.uint32))
+    assert np.array_equal(depth.cpu().numpy().view(np.uint32), scene.depth.view(np.uint32))
+    np.testing.assert_array_equal(b.tile_minima.cpu().numpy(), scene.tmin)
+    np.testing.assert_array_equal(b.tile_maxima.cpu().numpy(), scene.tmax)
+    np.testing.assert_array_equal(b.num_tiles_overlapped.cpu().numpy(), scene.counts)
+    np.testing.assert_array_equal(pk.keys.cpu().numpy(), scene.keys)
+    np.testing.assert_array_equal(pk.triangle_indices.cpu().numpy(), scene.values)
+    np.testing.assert_array_equal(spk.keys.cpu().numpy(), scene.sorted_keys)
+    np.testing.assert_array_equal(spk.triangle_indices.cpu().numpy(), scene.sorted_values)
+    np.testing.assert_array_equal(ranges.cpu().numpy(), scene.ranges)
+
+
+@pytest.mark.parametrize("n", [1, 2, 100, 4096, 4097, 65_537, 1_000_003])
+def test_sort_matches_stable_argsort(n):
+    rng = np.random.default_rng(n)
+    # Few distinct keys (many ties), both signs (camera >= 32 sets bit 63), varying digits.
+    base = rng.integers(-(2**62), 2**62, size=max(1, n // 7), dtype=np.int64)
+    keys = base[rng.integers(0, base.size, size=n)]
+    keys[rng.random(n) < 0.1] |= np.int64(-(2**63))
+    values = np.arange(n, dtype=np.int32)
+    out = tr.sort_keys(tr.PairedKeys(to_dev(keys), to_dev(values)), None)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(out.keys.cpu().numpy(), keys[order])
+    np.testing.assert_array_equal(out.triangle_indices.cpu().numpy(), values[order])
+
+
+def test_sort_all_equal_and_structured_keys():
+    n = 50_000
+    keys = np.full(n, 123456789, dtype=np.int64)
+    values = np.arange(n, dtype=np.int32)[::-1].copy()
+    out = tr.sort_keys(tr.PairedKeys(to_dev(keys), to_dev(values)), None)
+    np.testing.assert_array_equal(out.triangle_indices.cpu().numpy(), values)
+    # Realistic fused keys: only a few digit positions vary.
+    rng = np.random.default_rng(3)
+    lay = orr.fused_layout()
+    depth = rng.uniform(0.5, 4.0, n).astype(np.float32)
+    keys = orr.pack_keys(lay, rng.integers(0, 4, n), rng.integers(0, 5, n), rng.integers(0, 4056, n), depth)
+    out = tr.sort_keys(tr.PairedKeys(to_dev(keys), to_dev(values)), None)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(out.keys.cpu().numpy(), keys[order])
+    np.testing.assert_array_equal(out.triangle_indices.cpu().numpy(), values[order])
+
+
+def test_spherical_harmonics(scene):
+    t = scene.tri
+    verts = to_dev(t.vertices)
+    sh = to_dev(t.spherical_harmonics).requires_grad_(True)
+    col = tr.evaluate_spherical_harmonics(verts, sh, to_dev(scene.E), scene.spec.sh_degree)
+    want, _ = orr.evaluate_spherical_harmonics(t.vertices, t.spherical_harmonics, scene.E, scene.spec.sh_degree)
+    assert np.abs(col.detach().cpu().numpy() - want).max() < 1e-5
+    g = np.random.default_rng(1).standard_normal(col.shape).astype(np.float32)
+    col.backward(to_dev(g))
+    wg = orr.evaluate_spherical_harmonics_backward(t.vertices, t.spherical_harmonics, scene.E,
+                                                   scene.spec.sh_degree, g)
+    got = sh.grad.cpu().numpy()
+    assert np.abs(got - wg).max() / (np.abs(wg).max() + 1e-12) < 1e-5
+
+
+def _fused(sc, chunk=0):
+    t = sc.tri
+    xy, depth, b, pk, spk, ranges = _front_end(sc)
+    colors = to_dev(sc.colors).requires_grad_(True)
+    sdf = to_dev(t.signed_distances).requires_grad_(True)
+    sharp = torch.tensor(sc.spec.sharpness, device="cuda").requires_grad_(True)
+    prev = config.composite_chunk
+    config.set_composite_chunk(chunk)
+    try:
+        out = tr.intersect_and_composite(xy, colors, sdf, to_dev(t.faces), sharp, spk.keys,
+                                         spk.triangle_indices, ranges.view(sc.B, sc.T, 2), sc.S,
+                                         _grid(sc), _key_defines())
+    finally:
+        config.set_composite_chunk(prev)
+    return out, colors, sdf, sharp
+
+
+@pytest.mark.parametrize("chunk", [0, 32, 100])
+def test_fused_forward_matches_oracle(scene, chunk):
+    out, *_ = _fused(scene, chunk)
+    got = out.detach().cpu().numpy()
+    assert np.abs(got - scene.image).max() < IMG_TOL
+    # FP64-continuous reference on the same hit lists.
+    t = scene.tri
+    img64 = orr.fused_forward_image(scene.hits, scene.colors.astype(np.float64),
+                                    t.signed_distances.astype(np.float64), t.faces,
+                                    scene.spec.sharpness, scene.B, (scene.H, scene.W), dtype=np.float64)
+    assert np.abs(got - img64).max() < IMG_TOL
+
+
+@pytest.mark.parametrize("quirk", [False, True])
+def test_fused_backward_matches_oracle(scene, quirk):
+    out, colors, sdf, sharp = _fused(scene)
+    g = np.random.default_rng(2).standard_normal(out.shape).astype(np.float32)
+    prev = config.reference_gradient_quirks
+    config.set_reference_gradient_quirks(quirk)
+    try:
+        out.backward(to_dev(g))
+    finally:
+        config.set_reference_gradient_quirks(prev)
+    t = scene.tri
+    wc, ws, wk = orr.fused_backward(scene.hits, scene.colors.astype(np.float64),
+                                    t.signed_distances.astype(np.float64), t.faces,
+                                    scene.spec.sharpness, out.detach().cpu().numpy(), g,
+                                    reference_quirk=quirk)
+    for got, want in ((colors.grad, wc), (sdf.grad, ws)):
+        gg = got.cpu().numpy().astype(np.float64)
+        assert np.abs(gg - want).max() / (np.abs(want).max() + 1e-12) < GRAD_TOL
+    assert abs(float(sharp.grad) - wk) / (abs(wk) + 1e-12) < GRAD_TOL
+
+
+def test_explicit_mode_matches_oracle(scene_explicit):
+    sc = scene_explicit
+    t = sc.tri
+    xy, depth, b, pk, spk, ranges = _front_end(sc, explicit=True)
+    np.testing.assert_array_equal(spk.keys.cpu().numpy(), sc.sorted_keys)
+    np.testing.assert_array_equal(ranges.cpu().numpy(), sc.ranges)
+    uv, index = tr.intersect(xy, to_dev(t.faces), spk.triangle_indices, ranges, _grid(sc), _key_defines(True))
+    want_uv, want_index = orr.explicit_intersect(sc.xy, t.faces, sc.sorted_values, sc.ranges,
+                                                 sc.tile_shape, (sc.H, sc.W))
+    np.testing.assert_array_equal(index.cpu().numpy(), want_index)
+    assert np.array_equal(uv.cpu().numpy().view(np.uint32), want_uv.view(np.uint32))
+    colors = to_dev(sc.colors).requires_grad_(True)
+    sdf = to_dev(t.signed_distances).requires_grad_(True)
+    s_img, c_img = tr.interpolate(uv, index, to_dev(t.faces), sdf, colors)
+    ws_img, wc_img = orr.interpolate(want_uv, want_index, t.faces, t.signed_distances, sc.colors)
+    assert np.abs(s_img.detach().cpu().numpy() - ws_img).max() < 1e-6
+    assert np.abs(c_img.detach().cpu().numpy() - wc_img).max() < 1e-6
+    B, S, H, W = want_index.shape
+    sdf_pl = s_img.permute(0, 2, 3, 1).reshape(-1, S)
+    col_pl = c_img.permute(0, 2, 3, 1, 4).reshape(-1, S, 3)
+    mask = (index != -1).permute(0, 2, 3, 1).reshape(-1, S).contiguous()
+    sharp = torch.tensor(sc.spec.sharpness, device="cuda").requires_grad_(True)
+    from meshtryoshka_amd.triangle_rasterization.composite import Composite
+    oc, ot = Composite.apply(sdf_pl.contiguous(), col_pl.contiguous(), mask, sharp)
+    wcol, wtr = orr.composite(ws_img.transpose(0, 2, 3, 1).reshape(-1, S),
+                              wc_img.transpose(0, 2, 3, 1, 4).reshape(-1, S, 3),
+                              (want_index != -1).transpose(0, 2, 3, 1).reshape(-1, S), sc.spec.sharpness)
+    assert np.abs(oc.detach().cpu().numpy() - wcol).max() < IMG_TOL
+    assert np.abs(ot.detach().cpu().numpy() - wtr).max() < IMG_TOL
+
+
+@pytest.mark.parametrize("quirk", [False, True])
+def test_composite_backward_matches_oracle(quirk):
+    rng = np.random.default_rng(5)
+    P, S = 5000, 5
+    sdf = (rng.standard_normal((P, S)) * 0.05).astype(np.float32)
+    col = rng.random((P, S, 3)).astype(np.float32)
+    mask = rng.random((P, S)) < 0.7
+    k = 40.0
+    ts, tc = to_dev(sdf).requires_grad_(True), to_dev(col).requires_grad_(True)
+    sharp = torch.tensor(k, device="cuda").requires_grad_(True)
+    from meshtryoshka_amd.triangle_rasterization.composite import Composite
+    oc, ot = Composite.apply(ts, tc, to_dev(mask), sharp)
+    gc = rng.standard_normal((P, 3)).astype(np.float32)
+    gt = rng.standard_normal(P).astype(np.float32)
+    prev = config.reference_gradient_quirks
+    config.set_reference_gradient_quirks(quirk)
+    try:
+        torch.autograd.backward([oc, ot], [to_dev(gc), to_dev(gt)])
+    finally:
+        config.set_reference_gradient_quirks(prev)
+    ws, wc, wk = orr.composite_backward(sdf, col, mask, k, oc.detach().cpu().numpy(),
+                                        ot.detach().cpu().numpy(), gc, gt, reference_quirk=quirk)
+    for got, want in ((ts.grad, ws), (tc.grad, wc)):
+        gg = got.cpu().numpy().astype(np.float64)
+        assert np.abs(gg - want).max() / (np.abs(want).max() + 1e-12) < GRAD_TOL
+    assert abs(float(sharp.grad) - wk) / (abs(wk) + 1e-12) < GRAD_TOL
+
+
+def test_render_fused_end_to_end(scene):
+    t = scene.tri
+    img = tr.render(to_dev(scene.E), to_dev(scene.K), (scene.H, scene.W), to_dev(t.vertices),
+                    to_dev(t.faces), to_dev(t.face_boundaries), to_dev(t.signed_distances),
+                    to_dev(t.spherical_harmonics), scene.spec.sh_degree,
+                    torch.tensor(scene.spec.sharpness, device="cuda"),
+                    torch.ones(3, device="cuda"), mode="fused")
+    want = np.clip(scene.image[:, :3] + scene.image[:, 3:4], 0, 1)
+    assert np.abs(img.cpu().numpy() - want).max() < IMG_TOL
+
+
+def test_tile_shape_invariance(scene):
+    t = scene.tri
+    imgs = []
+    for ts in [(16, 16), (8, 8), (8, 32), (32, 16)]:
+        imgs.append(tr.render(to_dev(scene.E), to_dev(scene.K), (scene.H, scene.W), to_dev(t.vertices),
+                              to_dev(t.faces), to_dev(t.face_boundaries), to_dev(t.signed_distances),
+                              to_dev(t.spherical_harmonics), scene.spec.sh_degree,
+                              torch.tensor(scene.spec.sharpness, device="cuda"),
+                              torch.ones(3, device="cuda"), tile_shape=ts).cpu().numpy())
+    for im in imgs[1:]:
+        assert np.array_equal(im, imgs[0])
