@@ -109,14 +109,16 @@ int mt_count_triangle_vertices(const float* sdf, const int32_t* lower_corners,
 
 /* Forward of CreateTriangleVertices (create_triangle_vertices.slang:145-199,
  * .py:31-102). vertex_offsets int32[S * num_columns + 1] (exclusive). grid_sh
- * f32[num_sh, num_grid_vertices, 3]. Outputs f32[V_t, 3], f32[V_t], f32[num_sh, V_t, 3],
- * uint8[V_t] with V_t = num_triangle_vertices. */
+ * f32[num_sh, num_grid_vertices, 3]. Outputs f32[V, 3], f32[V], f32[num_sh, V, 3],
+ * uint8[V] with V = num_triangle_vertices, the capacity of the output arrays (the
+ * plain path passes the exact count). Vertices at or beyond the capacity are not
+ * written; if any exist and `overflow` is not NULL, *overflow is set to 1. */
 int mt_create_triangle_vertices_forward(
     const float* grid_vertices, const float* grid_sdf, const float* grid_sh,
     int64_t num_grid_vertices, int32_t num_sh, const int32_t* lower_corners,
     int64_t num_columns, const float* level_sets, int32_t S, const int32_t* vertex_offsets,
     int64_t num_triangle_vertices, float* out_vertices, float* out_sdf, float* out_sh,
-    uint8_t* out_types, mt_stream_t stream);
+    uint8_t* out_types, int32_t* overflow, mt_stream_t stream);
 
 /* Adjoint of the forward (create_triangle_vertices.slang:201-291, .py:104-170).
  * Accumulates into zero-initialised grid gradients. Any grad_* input may be NULL
@@ -138,12 +140,15 @@ int mt_count_triangle_faces(const float* sdf, const int32_t* lower_corners, int6
                             uint8_t* cell_codes, mt_stream_t stream);
 
 /* faces int32[F, 3] and voxel_indices int32[F] from the exclusive face offsets
- * int32[S * num_voxels + 1]. Replaces create_triangle_faces.slang:3-78 (.py:28-65). */
+ * int32[S * num_voxels + 1]; F = face_capacity (faces at or beyond it are dropped and
+ * flagged in *overflow when not NULL). Replaces create_triangle_faces.slang:3-78
+ * (.py:28-65). */
 int mt_create_triangle_faces(const int32_t* neighbors, const int32_t* indices,
                              int64_t num_voxels, int64_t num_columns,
                              const int32_t* vertex_offsets, const uint8_t* vertex_types,
                              const uint8_t* cell_codes, int32_t S, const int32_t* face_offsets,
-                             int32_t* faces, int32_t* voxel_indices, mt_stream_t stream);
+                             int32_t* faces, int32_t* voxel_indices, int64_t face_capacity,
+                             int32_t* overflow, mt_stream_t stream);
 
 /* ---------------------------------------------------------------------------------
  * Rasterization (triangle_rasterization/__init__.py:22-329)
@@ -158,39 +163,44 @@ int mt_project_vertices(const float* vertices, int64_t V, const float* extrinsic
 
 /* Back-face / near-plane culling and tile bounds. faces int32[F, 3]; outputs
  * tile_min int32[B, F, 2] (row, col) inclusive, tile_max int32[B, F, 2] exclusive,
- * counts int32[B, F]. Replaces bound_triangles.slang:144-192 (.py:35-71). */
+ * counts int32[B, F]. If num_faces_x3 is not NULL, only faces below (*num_faces_x3) / 3
+ * are live (F is then the capacity) and the rest get empty bounds.
+ * Replaces bound_triangles.slang:144-192 (.py:35-71). */
 int mt_bound_triangles(const float* xy, const float* depths, int64_t V, const int32_t* faces,
                        int64_t F, int32_t B, int32_t tile_h, int32_t tile_w, int32_t row_min,
                        int32_t col_min, int32_t row_max, int32_t col_max, float near_plane,
                        int32_t* tile_min, int32_t* tile_max, int32_t* counts,
-                       mt_stream_t stream);
+                       const int32_t* num_faces_x3, mt_stream_t stream);
 
 /* One key per (camera, face, covered tile), written at the inclusive offsets
- * int64[B * F] of the tile counts. keys int64[K], values int32[K] (face ids).
- * Replaces generate_keys.slang:4-68 (.py:34-70). */
+ * int64[B * F] of the tile counts. keys int64[K], values int32[K] (face ids) with
+ * K = key_capacity; keys at or beyond it are dropped and flagged in *overflow when not
+ * NULL. Replaces generate_keys.slang:4-68 (.py:34-70). */
 int mt_generate_keys(const float* depths, int64_t V, const int32_t* faces, int64_t F,
                      const int32_t* shell_face_boundaries, int32_t S, const int32_t* tile_min,
                      const int32_t* tile_max, const int64_t* offsets, int32_t B,
                      int32_t grid_cols, mt_key_layout layout, int64_t* keys, int32_t* values,
-                     mt_stream_t stream);
+                     int64_t key_capacity, int32_t* overflow, mt_stream_t stream);
 
 /* Workspace for mt_sort_keys on n pairs. */
 size_t mt_sort_workspace_bytes(int64_t n);
 
 /* Stable ascending sort of (signed int64 key, int32 value) pairs over all 64 key bits
  * (hand-written onesweep LSD radix sort; passes whose digit is constant are skipped on
- * the device). Inputs are not modified. Replaces sort_keys (sort_keys.cu:215-270,
- * sort_keys.py:18-29). */
+ * the device). Inputs are not modified. If n_device is not NULL the device value
+ * min(*n_device, n) is the number of pairs and n only sizes the launch (capacity).
+ * Replaces sort_keys (sort_keys.cu:215-270, sort_keys.py:18-29). */
 int mt_sort_keys(const int64_t* keys_in, const int32_t* values_in, int64_t n, int64_t* keys_out,
                  int32_t* values_out, void* workspace, size_t workspace_bytes,
-                 mt_stream_t stream);
+                 const int64_t* n_device, mt_stream_t stream);
 
 /* [start, end) of every (camera[, shell], tile) run of the sorted keys into the
- * zero-initialised int64[B, S', T, 2] (S' = S if separate_shells else 1).
- * Replaces delineate_tiles.slang:12-58 (.py:24-55). */
+ * zero-initialised int64[B, S', T, 2] (S' = S if separate_shells else 1). If n_device
+ * is not NULL, min(*n_device, n) keys are live. Replaces delineate_tiles.slang:12-58
+ * (.py:24-55). */
 int mt_delineate_tiles(const int64_t* sorted_keys, int64_t n, mt_key_layout layout,
                        int32_t separate_shells, int32_t B, int32_t S_out, int32_t T,
-                       int64_t* tile_ranges, mt_stream_t stream);
+                       int64_t* tile_ranges, const int64_t* n_device, mt_stream_t stream);
 
 /* out f32[B, 3] = inverse(extrinsics[b])[:3, 3] (computed in double precision).
  * Replaces torch.inverse(extrinsics)[:, :3, 3] (evaluate_spherical_harmonics.py:38). */
@@ -223,6 +233,9 @@ typedef struct mt_composite_args {
   int32_t num_cameras, num_shells, grid_rows, grid_cols, tile_h, tile_w, height, width;
   mt_key_layout layout;
   int32_t chunk; /* triangles staged in LDS per round: 0 = tile pixel count */
+  /* Optional (may be NULL): uint64[2] incremented by the number of (pixel, triangle)
+   * barycentric tests executed and of accepted hits. */
+  unsigned long long* stats;
 } mt_composite_args;
 
 /* out f32[B, 4, H, W] (rgb, transmittance). Replaces intersect_and_composite_forward

@@ -57,6 +57,7 @@ class CompositeArgs(ctypes.Structure):
         ("width", c_i32),
         ("layout", KeyLayout),
         ("chunk", c_i32),
+        ("stats", c_p),
     ]
 
 
@@ -76,7 +77,8 @@ SIGNATURES = {
     "mt_count_triangle_vertices": (c_i32, [c_p, c_p, c_i64, c_p, c_i32, c_p, c_p]),
     "mt_create_triangle_vertices_forward": (
         c_i32,
-        [c_p, c_p, c_p, c_i64, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p],
+        [c_p, c_p, c_p, c_i64, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p,
+         c_p],
     ),
     "mt_create_triangle_vertices_backward": (
         c_i32,
@@ -85,21 +87,22 @@ SIGNATURES = {
     "mt_count_triangle_faces": (c_i32, [c_p, c_p, c_i64, c_p, c_i64, c_p, c_i32, c_p, c_p, c_p]),
     "mt_create_triangle_faces": (
         c_i32,
-        [c_p, c_p, c_i64, c_i64, c_p, c_p, c_p, c_i32, c_p, c_p, c_p, c_p],
+        [c_p, c_p, c_i64, c_i64, c_p, c_p, c_p, c_i32, c_p, c_p, c_p, c_i64, c_p, c_p],
     ),
     "mt_project_vertices": (c_i32, [c_p, c_i64, c_p, c_p, c_i32, c_p, c_p, c_p]),
     "mt_bound_triangles": (
         c_i32,
         [c_p, c_p, c_i64, c_p, c_i64, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_f32,
-         c_p, c_p, c_p, c_p],
+         c_p, c_p, c_p, c_p, c_p],
     ),
     "mt_generate_keys": (
         c_i32,
-        [c_p, c_i64, c_p, c_i64, c_p, c_i32, c_p, c_p, c_p, c_i32, c_i32, KeyLayout, c_p, c_p, c_p],
+        [c_p, c_i64, c_p, c_i64, c_p, c_i32, c_p, c_p, c_p, c_i32, c_i32, KeyLayout, c_p, c_p,
+         c_i64, c_p, c_p],
     ),
     "mt_sort_workspace_bytes": (c_sz, [c_i64]),
-    "mt_sort_keys": (c_i32, [c_p, c_p, c_i64, c_p, c_p, c_p, c_sz, c_p]),
-    "mt_delineate_tiles": (c_i32, [c_p, c_i64, KeyLayout, c_i32, c_i32, c_i32, c_i32, c_p, c_p]),
+    "mt_sort_keys": (c_i32, [c_p, c_p, c_i64, c_p, c_p, c_p, c_sz, c_p, c_p]),
+    "mt_delineate_tiles": (c_i32, [c_p, c_i64, KeyLayout, c_i32, c_i32, c_i32, c_i32, c_p, c_p, c_p]),
     "mt_camera_positions": (c_i32, [c_p, c_i32, c_p, c_p]),
     "mt_evaluate_sh_forward": (c_i32, [c_p, c_i64, c_p, c_i32, c_i32, c_p, c_i32, c_p, c_p]),
     "mt_evaluate_sh_backward": (c_i32, [c_p, c_i64, c_p, c_i32, c_i32, c_p, c_i32, c_p, c_p, c_p]),

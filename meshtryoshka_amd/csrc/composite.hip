@@ -159,6 +159,27 @@ __device__ __forceinline__ void stage_chunk(const mt_composite_args& a, int cam,
   }
 }
 
+// Block-reduced statistics counters (one atomic pair per block).
+__device__ void add_stats(unsigned long long* stats, uint32_t tests, uint32_t hits) {
+  __shared__ unsigned long long s_t[1024 / kWave], s_h[1024 / kWave];
+  unsigned long long t = wave_sum<unsigned long long>(tests);
+  unsigned long long h = wave_sum<unsigned long long>(hits);
+  if (__lane_id() == 0) {
+    s_t[threadIdx.x / kWave] = t;
+    s_h[threadIdx.x / kWave] = h;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long st = 0, sh = 0;
+    for (int w = 0; w < (int(blockDim.x) + kWave - 1) / kWave; ++w) {
+      st += s_t[w];
+      sh += s_h[w];
+    }
+    atomicAdd(stats + 0, st);
+    atomicAdd(stats + 1, sh);
+  }
+}
+
 __global__ void __launch_bounds__(1024) fused_forward_kernel(mt_composite_args a, float* out) {
   __shared__ Staged s_geo[kMaxChunk];
   __shared__ StagedColor s_col[kMaxChunk];
@@ -176,6 +197,7 @@ __global__ void __launch_bounds__(1024) fused_forward_kernel(mt_composite_args a
   int last_shell = -1;
   bool active = tc.in_image;
   const int chunk = a.chunk;
+  uint32_t n_tests = 0, n_hits = 0;
   for (int64_t first = start; first < end; first += chunk) {
     const int count = static_cast<int>((end - first) < chunk ? (end - first) : chunk);
     __syncthreads();
@@ -186,8 +208,10 @@ __global__ void __launch_bounds__(1024) fused_forward_kernel(mt_composite_args a
         const Staged st = s_geo[i];
         const int shell = shell_of(st);
         if (shell <= last_shell) continue;
+        ++n_tests;
         const Bary bc = bary_test(st, px, py);
         if (!bc.hit) continue;
+        ++n_hits;
         const StagedColor c = s_col[i];
         const float sdf = (bc.u * c.a.w + bc.v * c.b.w) + bc.w * c.c.w;
         const float cr = (bc.u * c.a.x + bc.v * c.b.x) + bc.w * c.c.x;
@@ -218,6 +242,7 @@ __global__ void __launch_bounds__(1024) fused_forward_kernel(mt_composite_args a
     o[2 * plane] = bl;
     o[3 * plane] = trans;
   }
+  if (a.stats) add_stats(a.stats, n_tests, n_hits);
 }
 
 // Scatter one sample's gradient (colour and SDF) to its triangle's corners.
@@ -274,6 +299,7 @@ __global__ void __launch_bounds__(1024) fused_backward_kernel(mt_composite_args 
   bool active = tc.in_image;
   float dk = 0.f;
   // Sample waiting for its outer-role SDF gradient (set by the next accepted hit).
+  uint32_t n_tests = 0, n_hits = 0;
   bool pending = false;
   int32_t p_face = 0;
   float p_u = 0.f, p_v = 0.f, p_w = 0.f, p_dr = 0.f, p_dg = 0.f, p_db = 0.f, p_dsdf = 0.f;
@@ -289,8 +315,10 @@ __global__ void __launch_bounds__(1024) fused_backward_kernel(mt_composite_args 
         const Staged st = s_geo[i];
         const int shell = shell_of(st);
         if (shell <= last_shell) continue;
+        ++n_tests;
         const Bary bc = bary_test(st, px, py);
         if (!bc.hit) continue;
+        ++n_hits;
         const StagedColor c = s_col[i];
         const float sdf = (bc.u * c.a.w + bc.v * c.b.w) + bc.w * c.c.w;
         const float cr = (bc.u * c.a.x + bc.v * c.b.x) + bc.w * c.c.x;
@@ -359,6 +387,7 @@ __global__ void __launch_bounds__(1024) fused_backward_kernel(mt_composite_args 
     for (int w = 0; w < (int(blockDim.x) + kWave - 1) / kWave; ++w) t += s_red[w];
     if (t != 0.0f) atomicAdd(sharp_grad, t);
   }
+  if (a.stats) add_stats(a.stats, n_tests, n_hits);
 }
 
 // ------------------------------------------------------------------------------------

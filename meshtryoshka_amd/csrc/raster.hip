@@ -53,9 +53,18 @@ __global__ void __launch_bounds__(kBlock) bound_kernel(const float* xy, const fl
                                                        int B, int th, int tw, int rmin, int cmin,
                                                        int rmax, int cmax, float near_plane,
                                                        int32_t* tmin, int32_t* tmax,
-                                                       int32_t* counts) {
+                                                       int32_t* counts, const int32_t* live_x3) {
   const int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (f >= F) return;
+  if (live_x3 != nullptr && f >= int64_t(*live_x3) / 3) {
+    for (int b = 0; b < B; ++b) {
+      const int64_t q = int64_t(b) * F + f;
+      reinterpret_cast<int2*>(tmin)[q] = make_int2(0, 0);
+      reinterpret_cast<int2*>(tmax)[q] = make_int2(0, 0);
+      counts[q] = 0;
+    }
+    return;
+  }
   const int64_t ia = faces[3 * f + 0], ib = faces[3 * f + 1], ic = faces[3 * f + 2];
   const float2* pxy = reinterpret_cast<const float2*>(xy);
   for (int b = 0; b < B; ++b) {
@@ -92,7 +101,8 @@ __global__ void __launch_bounds__(kBlock) keys_kernel(const float* depths, int64
                                                       const int32_t* tmin, const int32_t* tmax,
                                                       const int64_t* offsets, int grid_cols,
                                                       KeyFields kf, int64_t* keys,
-                                                      int32_t* values) {
+                                                      int32_t* values, int64_t kcap,
+                                                      int32_t* overflow) {
   __shared__ int32_t s_prefix[kBlock + 1];
   __shared__ int32_t s_r0[kBlock], s_c0[kBlock], s_w[kBlock];
   __shared__ uint64_t s_key_hi[kBlock];  // camera | shell | depth bits (tile added later)
@@ -157,6 +167,10 @@ __global__ void __launch_bounds__(kBlock) keys_kernel(const float* depths, int64
     const uint32_t col = uint32_t(s_c0[lo] + local % w);
     const uint32_t tile = row * uint32_t(grid_cols) + col;
     const uint64_t key = s_key_hi[lo] | ((uint64_t(tile) & tile_mask) << kf.tile_offset);
+    if (base + j >= kcap) {
+      if (overflow) atomicOr(overflow, 1);
+      continue;
+    }
     keys[base + j] = static_cast<int64_t>(key);
     values[base + j] = static_cast<int32_t>(int64_t(blockIdx.x) * kBlock + lo);
   }
@@ -175,7 +189,9 @@ __device__ __forceinline__ void run_index(uint64_t key, const KeyFields& kf, boo
 
 __global__ void __launch_bounds__(kBlock) delineate_kernel(const int64_t* keys, int64_t n,
                                                            KeyFields kf, int separate, int B,
-                                                           int S_out, int T, int64_t* ranges) {
+                                                           int S_out, int T, int64_t* ranges,
+                                                           const int64_t* n_device) {
+  if (n_device != nullptr && *n_device < n) n = *n_device;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t c, s, t;
@@ -354,12 +370,12 @@ extern "C" int mt_bound_triangles(const float* xy, const float* depths, int64_t 
                                   int32_t tile_w, int32_t row_min, int32_t col_min,
                                   int32_t row_max, int32_t col_max, float near_plane,
                                   int32_t* tile_min, int32_t* tile_max, int32_t* counts,
-                                  mt_stream_t stream) {
+                                  const int32_t* num_faces_x3, mt_stream_t stream) {
   MT_REQUIRE(F >= 0 && B >= 0 && tile_h > 0 && tile_w > 0, "mt_bound_triangles: bad sizes");
   if (F == 0 || B == 0) return MT_OK;
   mt::bound_kernel<<<mt::blocks_for(F, mt::kBlock), mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
       xy, depths, V, faces, F, B, tile_h, tile_w, row_min, col_min, row_max, col_max, near_plane,
-      tile_min, tile_max, counts);
+      tile_min, tile_max, counts, num_faces_x3);
   return mt::check_launch("mt_bound_triangles");
 }
 
@@ -368,23 +384,24 @@ extern "C" int mt_generate_keys(const float* depths, int64_t V, const int32_t* f
                                 const int32_t* tile_min, const int32_t* tile_max,
                                 const int64_t* offsets, int32_t B, int32_t grid_cols,
                                 mt_key_layout layout, int64_t* keys, int32_t* values,
-                                mt_stream_t stream) {
-  MT_REQUIRE(F >= 0 && B >= 0 && B <= 65535, "mt_generate_keys: bad sizes");
+                                int64_t key_capacity, int32_t* overflow, mt_stream_t stream) {
+  MT_REQUIRE(F >= 0 && B >= 0 && B <= 65535 && key_capacity >= 0, "mt_generate_keys: bad sizes");
   if (F == 0 || B == 0) return MT_OK;
   dim3 grid(mt::blocks_for(F, mt::kBlock), static_cast<unsigned>(B));
   mt::keys_kernel<<<grid, mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
       depths, V, faces, F, shell_face_boundaries, S, tile_min, tile_max, offsets, grid_cols,
-      mt::to_fields(layout), keys, values);
+      mt::to_fields(layout), keys, values, key_capacity, overflow);
   return mt::check_launch("mt_generate_keys");
 }
 
 extern "C" int mt_delineate_tiles(const int64_t* sorted_keys, int64_t n, mt_key_layout layout,
                                   int32_t separate_shells, int32_t B, int32_t S_out, int32_t T,
-                                  int64_t* tile_ranges, mt_stream_t stream) {
+                                  int64_t* tile_ranges, const int64_t* n_device,
+                                  mt_stream_t stream) {
   MT_REQUIRE(n >= 0, "mt_delineate_tiles: negative length");
   if (n == 0) return MT_OK;
   mt::delineate_kernel<<<mt::blocks_for(n, mt::kBlock), mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
-      sorted_keys, n, mt::to_fields(layout), separate_shells, B, S_out, T, tile_ranges);
+      sorted_keys, n, mt::to_fields(layout), separate_shells, B, S_out, T, tile_ranges, n_device);
   return mt::check_launch("mt_delineate_tiles");
 }
 

@@ -89,8 +89,15 @@ __global__ void init_kernel(SortWorkspace w, int64_t lookback_words) {
   for (int64_t i = tid; i < lookback_words; i += stride) w.lookback[i] = 0ull;
 }
 
+__device__ __forceinline__ int64_t live_count(int64_t n, const int64_t* n_device) {
+  if (n_device == nullptr) return n;
+  const int64_t d = *n_device;
+  return d < n ? (d < 0 ? 0 : d) : n;
+}
+
 __global__ void __launch_bounds__(kBlock) histogram_kernel(const int64_t* keys, int64_t n,
-                                                           SortWorkspace w) {
+                                                           SortWorkspace w, const int64_t* n_device) {
+  n = live_count(n, n_device);
   __shared__ uint32_t s_hist[kPasses][kRadix];
   for (int i = threadIdx.x; i < kPasses * kRadix; i += kBlock) (&s_hist[0][0])[i] = 0u;
   __syncthreads();
@@ -128,7 +135,9 @@ __global__ void __launch_bounds__(kBlock) histogram_kernel(const int64_t* keys, 
   }
 }
 
-__global__ void __launch_bounds__(kBlock) prepare_kernel(SortWorkspace w, int64_t n) {
+__global__ void __launch_bounds__(kBlock) prepare_kernel(SortWorkspace w, int64_t n,
+                                                         const int64_t* n_device) {
+  n = live_count(n, n_device);
   __shared__ int32_t s_active[kPasses];
   __shared__ int64_t s_wave[kWaves];
   const int d = threadIdx.x;
@@ -180,8 +189,10 @@ __device__ __forceinline__ void lb_store(uint64_t* p, uint64_t v) {
 __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys,
                                                           const int32_t* in_values,
                                                           int64_t* out_keys, int32_t* out_values,
-                                                          int64_t n, int pass, SortWorkspace w) {
+                                                          int64_t n, int pass, SortWorkspace w,
+                                                          const int64_t* n_device) {
   if (w.info[pass] == 0) return;
+  n = live_count(n, n_device);
   const int src_sel = w.info[8 + pass];
   const int dst_sel = w.info[16 + pass];
   const int64_t* src_k = src_sel == 0 ? in_keys : (src_sel == 1 ? out_keys : w.tmp_keys);
@@ -202,6 +213,7 @@ __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys
   __syncthreads();
   const int64_t tile = s_tile;
   const int64_t base = tile * kTile;
+  if (base >= n) return;  // beyond the live keys (capacity-sized launch); uniform per block
   const int wave = threadIdx.x / kWave;
   const int lane = __lane_id();
   const uint64_t lt = lanemask_lt();
@@ -304,8 +316,9 @@ __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys
 
 __global__ void copy_if_idle_kernel(const int64_t* in_keys, const int32_t* in_values,
                                     int64_t* out_keys, int32_t* out_values, int64_t n,
-                                    const int32_t* info) {
+                                    const int32_t* info, const int64_t* n_device) {
   if (info[24] != 0) return;
+  n = live_count(n, n_device);
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     out_keys[i] = in_keys[i];
@@ -320,7 +333,8 @@ extern "C" size_t mt_sort_workspace_bytes(int64_t n) { return mt::workspace_byte
 
 extern "C" int mt_sort_keys(const int64_t* keys_in, const int32_t* values_in, int64_t n,
                             int64_t* keys_out, int32_t* values_out, void* workspace,
-                            size_t workspace_bytes, mt_stream_t stream) {
+                            size_t workspace_bytes, const int64_t* n_device,
+                            mt_stream_t stream) {
   using namespace mt;
   MT_REQUIRE(n >= 0, "mt_sort_keys: negative length");
   if (n == 0) return MT_OK;
@@ -340,13 +354,13 @@ extern "C" int mt_sort_keys(const int64_t* keys_in, const int32_t* values_in, in
       (n + int64_t(kBlock) * 64 - 1) / (int64_t(kBlock) * 64) < 2048
           ? (n + int64_t(kBlock) * 64 - 1) / (int64_t(kBlock) * 64)
           : 2048);
-  histogram_kernel<<<hist_blocks, kBlock, 0, s>>>(keys_in, n, w);
-  prepare_kernel<<<1, kBlock, 0, s>>>(w, n);
+  histogram_kernel<<<hist_blocks, kBlock, 0, s>>>(keys_in, n, w, n_device);
+  prepare_kernel<<<1, kBlock, 0, s>>>(w, n, n_device);
   for (int p = 0; p < kPasses; ++p) {
     onesweep_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, s>>>(keys_in, values_in, keys_out,
-                                                                   values_out, n, p, w);
+                                                                   values_out, n, p, w, n_device);
   }
   copy_if_idle_kernel<<<blocks_for(n < 65536 * 4 ? n : 65536 * 4, 256 * 4), 256, 0, s>>>(
-      keys_in, values_in, keys_out, values_out, n, w.info);
+      keys_in, values_in, keys_out, values_out, n, w.info, n_device);
   return check_launch("mt_sort_keys");
 }

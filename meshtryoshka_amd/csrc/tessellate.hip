@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(kBlock) count_vertices_kernel(const float* sdf
 __global__ void __launch_bounds__(kBlock) create_vertices_kernel(
     const float* gv, const float* gsdf, const float* gsh, int64_t ng, int nsh, const int32_t* lower,
     int64_t n, const float* level_sets, int S, const int32_t* offsets, int64_t nt, float* ov,
-    float* od, float* osh, uint8_t* otype) {
+    float* od, float* osh, uint8_t* otype, int32_t* overflow) {
   const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const ColumnSdf col = load_column(gsdf, lower, n, c);
@@ -78,6 +78,11 @@ __global__ void __launch_bounds__(kBlock) create_vertices_kernel(
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if (!cr[a]) continue;
+      if (idx >= nt) {  // capacity exceeded: drop the vertex, report it
+        if (overflow) atomicOr(overflow, 1);
+        ++idx;
+        continue;
+      }
       const int64_t g1 = col.v[a];
       const float d1 = col.d[a];
       const float t = edge_t(col.d0, d1, iso);
@@ -125,6 +130,7 @@ __global__ void __launch_bounds__(kBlock) create_vertices_backward_kernel(
       if (!crosses(col.d0, col.d[a], iso)) continue;
       int64_t idx = offsets[s * n + c];
       for (int b = 0; b < a; ++b) idx += crosses(col.d0, col.d[b], iso) ? 1 : 0;
+      if (idx >= nt) continue;
       const float t = edge_t(col.d0, col.d[a], iso);
       const float w0 = 1.0f - t;
       axis_any = true;
@@ -201,7 +207,8 @@ __global__ void __launch_bounds__(kBlock) count_faces_kernel(const float* sdf, c
 __global__ void __launch_bounds__(kBlock) create_faces_kernel(
     const int32_t* neighbors, const int32_t* indices, int64_t n, int64_t ncol,
     const int32_t* voffsets, const uint8_t* vtypes, const uint8_t* codes, int S,
-    const int32_t* foffsets, int32_t* faces, int32_t* voxel_indices) {
+    const int32_t* foffsets, int32_t* faces, int32_t* voxel_indices, int64_t fcap,
+    int32_t* overflow) {
   const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const int32_t linear = indices[v];
@@ -229,6 +236,10 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
       const int64_t pos = base + q;
       const int64_t tri = pos / 3;
       const int corner = 2 - static_cast<int>(pos % 3);
+      if (tri >= fcap) {
+        if (overflow) atomicOr(overflow, 1);
+        continue;
+      }
       faces[3 * tri + corner] = target;
       if (corner == 0) voxel_indices[tri] = linear;
     }
@@ -256,7 +267,7 @@ extern "C" int mt_create_triangle_vertices_forward(
     int64_t num_grid_vertices, int32_t num_sh, const int32_t* lower_corners, int64_t num_columns,
     const float* level_sets, int32_t S, const int32_t* vertex_offsets,
     int64_t num_triangle_vertices, float* out_vertices, float* out_sdf, float* out_sh,
-    uint8_t* out_types, mt_stream_t stream) {
+    uint8_t* out_types, int32_t* overflow, mt_stream_t stream) {
   MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0,
              "mt_create_triangle_vertices_forward: negative size");
   if (num_columns == 0 || S == 0 || num_triangle_vertices == 0) return MT_OK;
@@ -264,7 +275,7 @@ extern "C" int mt_create_triangle_vertices_forward(
                                static_cast<hipStream_t>(stream)>>>(
       grid_vertices, grid_sdf, grid_sh, num_grid_vertices, num_sh, lower_corners, num_columns,
       level_sets, S, vertex_offsets, num_triangle_vertices, out_vertices, out_sdf, out_sh,
-      out_types);
+      out_types, overflow);
   return mt::check_launch("mt_create_triangle_vertices_forward");
 }
 
@@ -307,12 +318,13 @@ extern "C" int mt_create_triangle_faces(const int32_t* neighbors, const int32_t*
                                         const int32_t* vertex_offsets, const uint8_t* vertex_types,
                                         const uint8_t* cell_codes, int32_t S,
                                         const int32_t* face_offsets, int32_t* faces,
-                                        int32_t* voxel_indices, mt_stream_t stream) {
-  MT_REQUIRE(num_voxels >= 0 && S >= 0, "mt_create_triangle_faces: bad sizes");
+                                        int32_t* voxel_indices, int64_t face_capacity,
+                                        int32_t* overflow, mt_stream_t stream) {
+  MT_REQUIRE(num_voxels >= 0 && S >= 0 && face_capacity >= 0, "mt_create_triangle_faces: bad sizes");
   if (num_voxels == 0 || S == 0) return MT_OK;
   mt::create_faces_kernel<<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
                             static_cast<hipStream_t>(stream)>>>(
       neighbors, indices, num_voxels, num_columns, vertex_offsets, vertex_types, cell_codes, S,
-      face_offsets, faces, voxel_indices);
+      face_offsets, faces, voxel_indices, face_capacity, overflow);
   return mt::check_launch("mt_create_triangle_faces");
 }

@@ -21,7 +21,9 @@ def pack_occupancy(occupancy: Tensor) -> Tensor:
     i, j, k = occupancy.shape
     bits = occupancy.reshape(i, j, k // 32, 32).to(torch.int64)
     powers = 1 << torch.arange(32, device=occupancy.device, dtype=torch.int64)
-    return (bits * powers).sum(dim=-1).to(torch.uint32).view(torch.int32)
+    words = (bits * powers).sum(dim=-1)
+    # Reinterpret the low 32 bits as int32 (two's complement).
+    return torch.where(words >= 2**31, words - 2**32, words).to(torch.int32)
 
 
 def unpack_occupancy(packed_occupancy: Tensor) -> Tensor:

@@ -49,7 +49,7 @@ class CreateTriangleVertices(torch.autograd.Function):
             grid_signed_distances.data_ptr(), grid_spherical_harmonics.data_ptr(), num_grid,
             num_sh, voxel_lower_corners.data_ptr(), num_columns, level_sets.data_ptr(),
             num_level_sets, vertex_offsets.data_ptr(), num_vertices, tv.data_ptr(), td.data_ptr(),
-            tsh.data_ptr(), ttype.data_ptr(), _native.stream(grid_vertices),
+            tsh.data_ptr(), ttype.data_ptr(), None, _native.stream(grid_vertices),
         )
         ctx.save_for_backward(grid_vertices, grid_signed_distances, grid_spherical_harmonics,
                               voxel_lower_corners, vertex_offsets, level_sets)

@@ -34,7 +34,7 @@ def bound_triangles(vertices: Tensor, depths: Tensor, faces: Tensor, grid: TileG
     _native.call(
         "mt_bound_triangles", vertices.contiguous().data_ptr(), depths.contiguous().data_ptr(), v,
         faces.contiguous().data_ptr(), t, c, grid.tile_shape[0], grid.tile_shape[1], 0, 0, rows,
-        cols, float(near_plane), tmin.data_ptr(), tmax.data_ptr(), counts.data_ptr(),
+        cols, float(near_plane), tmin.data_ptr(), tmax.data_ptr(), counts.data_ptr(), None,
         _native.stream(vertices),
     )
     return TriangleBounds(tmin, tmax, counts)

@@ -49,6 +49,6 @@ def generate_keys(depths: Tensor, faces: Tensor, shell_face_boundaries: Tensor, 
             num_faces, shell_face_boundaries.contiguous().to(torch.int32).data_ptr(), s,
             tile_minima.contiguous().data_ptr(), tile_maxima.contiguous().data_ptr(),
             offsets.data_ptr(), b, grid.grid_shape[1], to_layout(key_defines), out_keys.data_ptr(),
-            out_values.data_ptr(), stream,
+            out_values.data_ptr(), num_keys, None, stream,
         )
     return PairedKeys(out_keys, out_values)

@@ -23,5 +23,5 @@ def sort_keys(paired_keys: PairedKeys, grid: TileGrid) -> PairedKeys:
     if n > 0:
         ws = _native.workspace(_native.query("mt_sort_workspace_bytes", n), keys.device)
         _native.call("mt_sort_keys", keys.data_ptr(), values.data_ptr(), n, out_k.data_ptr(),
-                     out_v.data_ptr(), ws.data_ptr(), ws.numel(), _native.stream(keys))
+                     out_v.data_ptr(), ws.data_ptr(), ws.numel(), None, _native.stream(keys))
     return PairedKeys(out_k, out_v)

@@ -83,9 +83,9 @@ def dense_sphere(seed: int = 0, resolution: int = 64, image_size: int = 128) -> 
 
     def field_fn(unit_vertices: torch.Tensor, gen: torch.Generator):
         p = 2 * unit_vertices - 1
-        noise = torch.randn(p.shape[0], generator=gen, dtype=torch.float32)
+        noise = torch.randn(p.shape[0], generator=gen, dtype=torch.float32).to(p.device)
         sdf = p.norm(dim=-1) - 0.5 + 0.02 * noise
-        sh = torch.rand((1, p.shape[0], 3), generator=gen, dtype=torch.float32)
+        sh = torch.rand((1, p.shape[0], 3), generator=gen, dtype=torch.float32).to(p.device)
         return p.contiguous(), sdf.float().contiguous(), sh.contiguous()
 
     e = torch.tensor(
@@ -118,16 +118,18 @@ def _clean_sdf(p: torch.Tensor, centres: torch.Tensor, radii: torch.Tensor, chun
     r = radii.to(p.device)
     for s in range(0, p.shape[0], chunk):
         q = p[s : s + chunk]
-        d = torch.cdist(q, c) - r[None]
+        d = (q[:, None, :] - c[None]).norm(dim=-1) - r[None]
         out[s : s + chunk] = torch.minimum(d.min(dim=1).values, q[:, 1] + 0.6)
     return out
 
 
-def sphere_occupancy(resolution: int, centres, radii, band_cells: float, device="cpu"):
+def sphere_occupancy(resolution: int, centres, radii, band_cells: float, device="cpu",
+                     packed: bool = True):
     """Occupancy of cells whose centre lies within band_cells cells of the zero set.
 
     Evaluated coarse-to-fine: the SDF is 1-Lipschitz, so a coarse block whose centre
-    value exceeds the band plus the block's half-diagonal holds no occupied cell."""
+    value exceeds the band plus the block's half-diagonal holds no occupied cell.
+    Returns the packed int32[n, n, n / 32] grid (or bool[n, n, n] if packed=False)."""
     n = resolution
     h = 2.0 / n
     band = band_cells * h
@@ -140,24 +142,32 @@ def sphere_occupancy(resolution: int, centres, radii, band_cells: float, device=
     pc = torch.stack([gx, gy, gz], dim=-1).reshape(-1, 3)
     sc = _clean_sdf(pc, centres, radii)
     keep = (sc.abs() <= band + math.sqrt(3) * hb / 2 + 1e-6).nonzero().squeeze(1)
-    occ = torch.zeros((n, n, n), dtype=torch.bool, device=device)
-    if keep.numel() == 0:
+    words = torch.zeros((n, n, n // 32), dtype=torch.int64, device=device)
+    occ = None if packed else torch.zeros((n, n, n), dtype=torch.bool, device=device)
+    if keep.numel() > 0:
+        bi = keep // (nb * nb)
+        bj = (keep // nb) % nb
+        bk = keep % nb
+        local = torch.arange(block, device=device)
+        li, lj, lk = torch.meshgrid(local, local, local, indexing="ij")
+        li, lj, lk = li.reshape(-1), lj.reshape(-1), lk.reshape(-1)
+        step = 4096
+        for s in range(0, keep.numel(), step):
+            ci = (bi[s : s + step, None] * block + li[None]).reshape(-1)
+            cj = (bj[s : s + step, None] * block + lj[None]).reshape(-1)
+            ck = (bk[s : s + step, None] * block + lk[None]).reshape(-1)
+            p = torch.stack([-1 + (ck + 0.5) * h, -1 + (cj + 0.5) * h, -1 + (ci + 0.5) * h], dim=-1)
+            d = _clean_sdf(p.float(), centres, radii)
+            m = d.abs() <= band
+            ci, cj, ck = ci[m], cj[m], ck[m]
+            if occ is not None:
+                occ[ci, cj, ck] = True
+            # Each cell is visited once, so summing distinct bits is a bitwise or.
+            words.index_put_((ci, cj, ck // 32), torch.ones_like(ck) << (ck % 32), accumulate=True)
+    if occ is not None:
         return occ
-    bi = keep // (nb * nb)
-    bj = (keep // nb) % nb
-    bk = keep % nb
-    local = torch.arange(block, device=device)
-    li, lj, lk = torch.meshgrid(local, local, local, indexing="ij")
-    li, lj, lk = li.reshape(-1), lj.reshape(-1), lk.reshape(-1)
-    for s in range(0, keep.numel(), 4096):
-        ci = (bi[s : s + 4096, None] * block + li[None]).reshape(-1)
-        cj = (bj[s : s + 4096, None] * block + lj[None]).reshape(-1)
-        ck = (bk[s : s + 4096, None] * block + lk[None]).reshape(-1)
-        p = torch.stack([-1 + (ck + 0.5) * h, -1 + (cj + 0.5) * h, -1 + (ci + 0.5) * h], dim=-1)
-        d = _clean_sdf(p.float(), centres, radii)
-        m = d.abs() <= band
-        occ[ci[m], cj[m], ck[m]] = True
-    return occ
+    # Reinterpret the low 32 bits as int32 (two's complement) without a uint32 dtype.
+    return torch.where(words >= 2**31, words - 2**32, words).to(torch.int32)
 
 
 def hemisphere_cameras(gen: torch.Generator, num_cameras: int, image_shape, radius=2.0, fov_y=60.0,
@@ -189,7 +199,7 @@ def sphere_field(
     gen = torch.Generator("cpu").manual_seed(seed)
     centres = (torch.rand((num_spheres, 3), generator=gen) * 1.6 - 0.8).float()
     radii = (torch.rand((num_spheres,), generator=gen) * 0.25 + 0.05).float()
-    occ = sphere_occupancy(resolution, centres, radii, band_cells, device=device)
+    packed = sphere_occupancy(resolution, centres, radii, band_cells, device=device)
     nsh = (sh_degree + 1) ** 2
 
     def field_fn(unit_vertices: torch.Tensor, gen2: torch.Generator):
@@ -205,7 +215,7 @@ def sphere_field(
     half = (num_shells - 1) / 2
     level_sets = torch.tensor([(half - s) / 512.0 for s in range(num_shells)], dtype=torch.float32)
     spec = SceneSpec(
-        occupancy=pack_occupancy(occ),
+        occupancy=packed,
         level_sets=level_sets,
         sh_degree=sh_degree,
         sharpness=sharpness,
@@ -220,12 +230,15 @@ def sphere_field(
 
 
 def finish_fields(spec: SceneSpec, unit_vertices: torch.Tensor):
-    """Draw the per-corner fields in the scene's seeded order, after the occupancy."""
+    """Draw the per-corner fields in the scene's seeded order, after the occupancy.
+
+    The clean SDF is evaluated on ``unit_vertices``' device; random draws always come from
+    the scene's CPU generator, so results do not depend on the device."""
     gen = spec.extra.get("generator")
     if gen is None:
         gen = torch.Generator("cpu").manual_seed(spec.seed)
         spec.extra["generator"] = gen
-    world, sdf, sh = spec.field_fn(unit_vertices.cpu(), gen)
+    world, sdf, sh = spec.field_fn(unit_vertices, gen)
     if spec.cameras is None:
         nc = spec.extra["num_cameras"]
         cams = hemisphere_cameras(gen, nc, spec.extra["image_shape"])
