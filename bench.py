@@ -42,6 +42,7 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-tiles", type=int, default=24)
     p.add_argument("--breakdown", action="store_true", help="print the per-kernel table to stderr")
+    p.add_argument("--no-autotune", action="store_true", help="use 16x16 tiles, chunk 256")
     return p.parse_args()
 
 
@@ -81,152 +82,85 @@ def build_scene(args, device):
 
 
 # ------------------------------------------------------------------------------------
-# Plain step (the reference's calling pattern, per-op calls with host size syncs),
-# instrumented with HIP events around each kernel stage.
+# Plain per-camera path: the reference's public API, one render call per camera
+# (host size syncs inside every call, no graph), loss averaged over cameras.
 # ------------------------------------------------------------------------------------
 
 
-class StageTimer:
-    """Pairs of torch.cuda events on the current stream (the stream every op launches on)."""
+def plain_per_camera_step(sc):
+    import torch
 
+    from meshtryoshka_amd import triangle_extraction as te
+    from meshtryoshka_amd import triangle_rasterization as tr
+
+    v = sc.voxels
+    sc.sdf.grad = None
+    sc.sh.grad = None
+    tri = te.tessellate_voxels(sc.sdf, sc.world, sc.sh, v.neighbors, v.lower_corners,
+                               v.upper_corners, v.indices, sc.level_sets)
+    b = sc.cams.extrinsics.shape[0]
+    loss = 0.0
+    for c in range(b):
+        img = tr.render(sc.cams.extrinsics[c : c + 1], sc.cams.intrinsics[c : c + 1],
+                        sc.cams.image_shape, tri.vertices.detach(), tri.faces, tri.face_boundaries,
+                        tri.signed_distances, tri.spherical_harmonics, sc.spec.sh_degree,
+                        sc.sharpness, sc.background, tile_shape=sc.spec.tile_shape,
+                        near_plane=sc.spec.near_plane)
+        loss = loss + ((img - sc.target[c : c + 1]) ** 2).mean() / b
+    loss.backward()
+    return loss
+
+
+# ------------------------------------------------------------------------------------
+# Batched path: autotuned, graph-captured step; an eager instrumented run of the same
+# kernels gives the per-kernel durations (HIP events on the launch stream).
+# ------------------------------------------------------------------------------------
+
+
+class LaunchTimer:
     def __init__(self):
         import torch
 
         self.torch = torch
-        self.pending = []  # (name, start, end)
+        self.marks = []  # (name, event)
         self.totals = {}
         self.counts = {}
 
-    def event(self):
+    def start(self):
         ev = self.torch.cuda.Event(enable_timing=True)
         ev.record()
-        return ev
+        self.marks.append(("start", ev))
 
-    def add(self, name, start, end):
-        self.pending.append((name, start, end))
+    def hook(self, name):
+        ev = self.torch.cuda.Event(enable_timing=True)
+        ev.record()
+        self.marks.append((name, ev))
 
     def collect(self):
-        for name, s, e in self.pending:
-            self.totals[name] = self.totals.get(name, 0.0) + s.elapsed_time(e)
-            self.counts[name] = self.counts.get(name, 0) + 1
-        self.pending = []
+        prev = None
+        for name, ev in self.marks:
+            if name != "start" and prev is not None:
+                ms = prev.elapsed_time(ev)
+                self.totals[name] = self.totals.get(name, 0.0) + ms
+                self.counts[name] = self.counts.get(name, 0) + 1
+            prev = ev
+        self.marks = []
 
     def mean_ms(self, name):
         return self.totals[name] / self.counts[name]
 
 
-def plain_step(sc, timer, stats=None):
-    import torch
+def make_batched(sc, tile_shape, chunk, margin=1.15):
+    from meshtryoshka_amd import batched
 
-    from meshtryoshka_amd import triangle_extraction as te
-    from meshtryoshka_amd import triangle_rasterization as tr
-    from meshtryoshka_amd.triangle_rasterization.common import TileGrid
-
-    v = sc.voxels
-    sc.sdf.grad = None
-    sc.sh.grad = None
-    ev = timer.event
-    # --- tessellate_voxels (triangle_extraction/__init__.py:75-157)
-    e0 = ev()
-    with torch.no_grad():
-        vertex_offsets = te.count_triangle_vertices(sc.sdf, v.lower_corners, sc.level_sets)
-        e1 = ev()
-        te.compute_exclusive_cumsum(vertex_offsets)
-        e2 = ev()
-    tv, td, tsh, ttype = te.create_triangle_vertices(sc.world, sc.sdf, sc.sh, v.lower_corners,
-                                                     sc.level_sets, vertex_offsets)
-    e3 = ev()
-    with torch.no_grad():
-        face_offsets, codes = te.count_triangle_faces(sc.sdf, v.lower_corners, v.upper_corners,
-                                                      sc.level_sets)
-        e4 = ev()
-        te.compute_exclusive_cumsum(face_offsets)
-        e5 = ev()
-        faces, _ = te.create_triangle_faces(sc.sdf, v.neighbors, v.indices, vertex_offsets, ttype,
-                                            codes, sc.level_sets, face_offsets)
-        e6 = ev()
-    s_count = sc.level_sets.shape[0]
-    face_boundaries = face_offsets[1:].view(s_count, -1)[:, -1].clone() // 3
-    timer.add("count_triangle_vertices", e0, e1)
-    timer.add("exclusive_scan(vertices)", e1, e2)
-    timer.add("create_triangle_vertices", e2, e3)
-    timer.add("count_triangle_faces", e3, e4)
-    timer.add("exclusive_scan(faces)", e4, e5)
-    timer.add("create_triangle_faces", e5, e6)
-    # --- render_fused (triangle_rasterization/__init__.py:167-287)
-    b = sc.cams.extrinsics.shape[0]
-    h, w = sc.cams.image_shape
-    grid = TileGrid(sc.spec.tile_shape, (h, w), b, s_count)
-    kd = tr.create_key_defines(("camera", 6), ("tile", 20), ("depth", 32), ("shell", 6))
-    f0 = ev()
-    xy, depth = tr.project_vertices(tv.detach(), sc.cams.extrinsics, sc.cams.intrinsics)
-    f1 = ev()
-    bounds = tr.bound_triangles(xy, depth, faces, grid, sc.spec.near_plane)
-    f2 = ev()
-    pk = tr.generate_keys(depth, faces, face_boundaries, bounds.tile_minima, bounds.tile_maxima,
-                          bounds.num_tiles_overlapped, grid, kd)
-    f3 = ev()
-    spk = tr.sort_keys(pk, grid)
-    f4 = ev()
-    ranges = tr.delineate_tiles(spk, grid, kd, False)
-    f5 = ev()
-    colors = tr.evaluate_spherical_harmonics(tv.detach(), tsh, sc.cams.extrinsics, sc.spec.sh_degree)
-    f6 = ev()
-    t = grid.num_tiles_per_image
-    img = _composite_with_stats(xy, colors, td, faces, sc.sharpness, spk, ranges.view(b, t, 2),
-                                s_count, grid, kd, stats)
-    f7 = ev()
-    timer.add("project_vertices", f0, f1)
-    timer.add("bound_triangles", f1, f2)
-    timer.add("generate_keys(+scan)", f2, f3)
-    timer.add("sort_keys", f3, f4)
-    timer.add("delineate_tiles", f4, f5)
-    timer.add("evaluate_sh_forward", f5, f6)
-    timer.add("intersect_and_composite_forward", f6, f7)
-    rgb = img[:, :3] + img[:, 3:4] * sc.background[:, None, None]
-    rgb = rgb.clip(min=0, max=1)
-    loss = ((rgb - sc.target) ** 2).mean()
-    # Backward stage boundaries from gradient hooks (each fires on the current stream).
-    marks = {}
-    img.register_hook(lambda g: marks.__setitem__("c0", ev()))
-    colors.register_hook(lambda g: marks.__setitem__("c1", ev()))
-    tsh.register_hook(lambda g: marks.__setitem__("s1", ev()))
-    sc.sdf.register_hook(lambda g: marks.__setitem__("v1", ev()))
-    loss.backward()
-    timer.add("intersect_and_composite_backward", marks["c0"], marks["c1"])
-    timer.add("evaluate_sh_backward", marks["c1"], marks["s1"])
-    if "v1" in marks:
-        timer.add("create_triangle_vertices_backward", marks["s1"], marks["v1"])
-    sizes = {
-        "N_lc": int(v.lower_corners.shape[1]), "N_vox": int(v.upper_corners.shape[1]),
-        "V_t": int(tv.shape[0]), "F": int(faces.shape[0]), "K": int(spk.keys.shape[0]),
-        "B": b, "S": s_count, "H": h, "W": w, "T": t, "NSH": int(sc.sh.shape[0]),
-    }
-    sc.last_sorted_keys = spk.keys
-    return loss, sizes
-
-
-def _composite_with_stats(xy, colors, sdf, faces, sharpness, spk, ranges, s_count, grid, kd, stats):
-    """intersect_and_composite through the public autograd Function; when `stats` is given,
-    the kernels also count their (pixel, triangle) tests and hits into it."""
-    from meshtryoshka_amd.triangle_rasterization import intersect_and_composite as iac
-
-    if stats is None:
-        return iac.intersect_and_composite(xy, colors, sdf, faces, sharpness, spk.keys,
-                                           spk.triangle_indices, ranges, s_count, grid, kd)
-    orig = iac._args
-
-    def with_stats(*a, **k):
-        args = orig(*a, **k)
-        args.stats = stats.data_ptr()
-        return args
-
-    iac._args = with_stats
-    try:
-        return iac.intersect_and_composite(xy, colors, sdf, faces, sharpness, spk.keys,
-                                           spk.triangle_indices, ranges, s_count, grid, kd)
-    finally:
-        iac._args = orig
+    cap = batched.measure_capacities(sc.voxels, sc.level_sets, sc.cams.extrinsics, sc.cams.intrinsics,
+                                     sc.cams.image_shape, sc.world, sc.sdf.detach(), sc.sh.detach(),
+                                     tile_shape=tile_shape, near_plane=sc.spec.near_plane).scaled(margin)
+    return batched.BatchedTrainingStep(sc.voxels, sc.level_sets, sc.cams.extrinsics,
+                                       sc.cams.intrinsics, sc.cams.image_shape, sc.sh.shape[0],
+                                       sc.spec.sh_degree, sc.spec.sharpness, sc.spec.background,
+                                       cap, tile_shape=tile_shape, chunk=chunk,
+                                       near_plane=sc.spec.near_plane)
 
 
 # ------------------------------------------------------------------------------------
@@ -240,22 +174,26 @@ def algorithmic_bytes(z):
     n_lc, n_vox, vt, f, k, b, s = z["N_lc"], z["N_vox"], z["V_t"], z["F"], z["K"], z["B"], z["S"]
     hw = z["H"] * z["W"]
     return {
-        "count_triangle_vertices": n_lc * 16 + n_lc * 4 + 4 * s * n_lc,
-        "exclusive_scan(vertices)": 8 * (s * n_lc + 1),
-        "create_triangle_vertices": n_lc * 16 + n_lc * (12 + 4 + nb) + 4 * s * n_lc + vt * (12 + 4 + nb + 1),
-        "count_triangle_faces": n_vox * 32 + n_lc * 4 + s * n_vox * (4 + 1),
-        "exclusive_scan(faces)": 8 * (s * n_vox + 1),
-        "create_triangle_faces": n_vox * (28 + 4) + s * n_vox * (1 + 8) + 4 * s * n_lc + vt + f * 16,
-        "project_vertices": vt * 12 + b * vt * 12,
-        "bound_triangles": f * 12 + b * vt * 12 + b * f * 20,
-        "generate_keys(+scan)": b * f * (4 + 8 + 8) + f * 12 + b * vt * 4 + k * 12,
-        "sort_keys": None,  # filled from the number of active passes
-        "delineate_tiles": 8 * k + 16 * b * z["T"],
-        "evaluate_sh_forward": vt * (12 + nb) + b * vt * 12,
-        "intersect_and_composite_forward": 96 * k + 16 * b * hw + 16 * b * z["T"],
-        "intersect_and_composite_backward": 96 * k + 32 * b * hw + 16 * b * z["T"] + 2 * (12 * b * vt + 4 * vt),
-        "evaluate_sh_backward": vt * (12 + nb) + b * vt * 12 + vt * nb,
-        "create_triangle_vertices_backward": n_lc * 16 + vt * (12 + 4 + nb) + 2 * n_lc * (4 + nb),
+        "mt_count_triangle_vertices": n_lc * 16 + n_lc * 4 + 4 * s * n_lc,
+        "mt_exclusive_scan_i32": 8 * (s * n_lc + 1) + 8 * (s * n_vox + 1),  # both scans
+        "mt_create_triangle_vertices_forward": n_lc * 16 + n_lc * (12 + 4 + nb) + 4 * s * n_lc
+        + vt * (12 + 4 + nb + 1),
+        "mt_count_triangle_faces": n_vox * 32 + n_lc * 4 + s * n_vox * (4 + 1),
+        "mt_create_triangle_faces": n_vox * (28 + 4) + s * n_vox * (1 + 8) + 4 * s * n_lc + vt + f * 16,
+        "mt_project_vertices": vt * 12 + b * vt * 12,
+        "mt_bound_triangles": f * 12 + b * vt * 12 + b * f * 20,
+        "mt_inclusive_scan_i32_to_i64": 12 * b * f,
+        "mt_generate_keys": b * f * (4 + 8 + 8) + f * 12 + b * vt * 4 + k * 12,
+        "mt_sort_keys": 8 * k + 24 * k * z.get("sort_passes", 8),
+        "mt_delineate_tiles": 8 * k + 16 * b * z["T"],
+        "mt_evaluate_sh_forward": vt * (12 + nb) + b * vt * 12,
+        "mt_intersect_and_composite_forward": 96 * k + 16 * b * hw + 16 * b * z["T"],
+        "mt_image_l2_loss": 16 * b * hw + 12 * b * hw + 16 * b * hw,
+        "mt_intersect_and_composite_backward": 96 * k + 32 * b * hw + 16 * b * z["T"]
+        + 2 * (12 * b * vt + 4 * vt),
+        "mt_evaluate_sh_backward": vt * (12 + nb) + b * vt * 12 + vt * nb,
+        "mt_create_triangle_vertices_backward": n_lc * 16 + vt * (12 + 4 + nb) + 2 * n_lc * (4 + nb),
+        "zero_fill": 4 * (3 * b * vt + vt + vt * z["NSH"] * 3 + n_lc * (1 + 3 * z["NSH"])),
     }
 
 
@@ -351,68 +289,109 @@ def main():
         sys.exit(2)
     import torch
 
+    from meshtryoshka_amd import autotune
+
     device = torch.device("cuda:0")
     torch.cuda.set_device(device)
-    print("[bench] building scene", file=sys.stderr, flush=True)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    log("[bench] building scene")
     t0 = time.time()
     sc = build_scene(args, device)
     torch.cuda.synchronize()
-    print(f"[bench] scene ready in {time.time() - t0:.1f} s: {sc.voxels.num_voxels} voxels, "
-          f"{sc.voxels.lower_corners.shape[1]} columns", file=sys.stderr, flush=True)
+    log(f"[bench] scene ready in {time.time() - t0:.1f} s: {sc.voxels.num_voxels} voxels, "
+        f"{sc.voxels.lower_corners.shape[1]} columns")
+    inputs = (sc.world, sc.sdf.detach(), sc.sh.detach(), sc.target)
+    b = sc.cams.extrinsics.shape[0]
+    h, w = sc.cams.image_shape
 
-    timer = StageTimer()
-    for i in range(args.warmup):
-        loss, sizes = plain_step(sc, timer)
-        torch.cuda.synchronize()
-        print(f"[bench] warmup {i} loss {loss.item():.6f}", file=sys.stderr, flush=True)
-    timer = StageTimer()
-    stats = torch.zeros(2, dtype=torch.int64, device=device)
+    # 1. Plain per-camera API path.
+    for _ in range(max(1, args.warmup // 2)):
+        plain_per_camera_step(sc)
+    torch.cuda.synchronize()
+    plain_steps = max(2, args.steps // 3)
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        plain_loss = plain_per_camera_step(sc)
+    torch.cuda.synchronize()
+    plain_s = (time.perf_counter() - t0) / plain_steps
+    log(f"[bench] plain per-camera path: {1000 * plain_s:.2f} ms/step (loss {float(plain_loss):.6f})")
+
+    # 2. Autotune the batched step (tile shape x chunk).
+    if args.no_autotune:
+        best = autotune.Choice((16, 16), 256, float("nan"))
+        tuned = []
+    else:
+        best, tuned = autotune.autotune(sc.voxels, sc.level_sets, sc.cams.extrinsics, sc.cams.intrinsics,
+                                        (h, w), sc.sh.shape[0], sc.spec.sh_degree, sc.spec.sharpness,
+                                        sc.spec.background, inputs, near_plane=sc.spec.near_plane, log=log)
+    log(f"[bench] autotune choice: tile {best.tile_shape} chunk {best.chunk}")
+    step = make_batched(sc, best.tile_shape, best.chunk)
+
+    # 3. Graph-captured batched step: the timed region.
+    step.capture(*inputs)
+    for _ in range(args.warmup):
+        step.replay()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        loss, sizes = plain_step(sc, timer, stats)
+        step.replay()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    timer.collect()
+    if step.overflowed():
+        raise RuntimeError("batched step overflowed its capacities")
     ms_per_step = 1000 * elapsed / args.steps
-    print(f"[bench] {args.steps} steps in {elapsed:.3f} s ({ms_per_step:.2f} ms/step); sizes {sizes}",
-          file=sys.stderr, flush=True)
+    live = step.live_sizes()
+    log(f"[bench] batched graph step: {ms_per_step:.2f} ms/step, loss {float(step.loss):.6f}, {live}")
 
-    # Per-kernel table.
+    # 4. Per-kernel durations: the same kernels, eager, bracketed by HIP events.
+    timer = LaunchTimer()
+    stats = torch.zeros(2, dtype=torch.int64, device=device)
+    inst_steps = 2
+    for _ in range(inst_steps):
+        step_args = step._composite_args
+        def with_stats(_orig=step_args):
+            a = _orig()
+            a.stats = stats.data_ptr()
+            return a
+        step._composite_args = with_stats
+        timer.start()
+        step.run(*inputs, hook=timer.hook)
+        step._composite_args = step_args
+    torch.cuda.synchronize()
+    timer.collect()
+    sizes = {
+        "N_lc": int(sc.voxels.lower_corners.shape[1]), "N_vox": int(sc.voxels.upper_corners.shape[1]),
+        "V_t": live["V_t"], "F": live["F"], "K": live["K"], "B": b, "S": int(sc.level_sets.shape[0]),
+        "H": h, "W": w, "T": step.T, "NSH": int(sc.sh.shape[0]),
+        "tile_shape": list(best.tile_shape), "chunk": best.chunk,
+    }
+    sizes["sort_passes"] = sort_passes(step.sorted_keys[: live["K"]])
     alg = algorithmic_bytes(sizes)
-    passes = sort_passes(sc.last_sorted_keys)
-    sizes["sort_passes"] = passes
     table = {}
     for name in timer.totals:
         ms = timer.mean_ms(name)
         entry = {"ms": round(ms, 4)}
-        b_alg = alg.get(name)
-        if name == "sort_keys":
-            # Histogram read + (read + write) of 12-byte pairs per active pass.
-            b_alg = 8 * sizes["K"] + 24 * sizes["K"] * passes
-        if b_alg:
-            entry["GB/s"] = round(b_alg / (ms * 1e-3) / 1e9, 1)
+        if alg.get(name):
+            entry["GB/s"] = round(alg[name] / (ms * 1e-3) / 1e9, 1)
             entry["hbm_frac"] = round(entry["GB/s"] / HBM_PEAK_GBS, 4)
         table[name] = entry
     s = stats.tolist()
-    n_steps = args.steps
-    tests_per_launch = s[0] / (2 * n_steps)  # forward and backward both count
-    hits_per_launch = s[1] / (2 * n_steps)
-    dom = max(table, key=lambda k: table[k]["ms"])
-    flops_fwd = 58 * tests_per_launch + 25 * hits_per_launch
+    tests = s[0] / (2 * inst_steps)
+    hits = s[1] / (2 * inst_steps)
+    sizes["composite_tests_per_launch"] = tests
+    sizes["composite_hits_per_launch"] = hits
     if args.breakdown:
         for name, e in sorted(table.items(), key=lambda kv: -kv[1]["ms"]):
-            print(f"  {name:36s} {e}", file=sys.stderr)
-        print(f"  composite tests/launch {tests_per_launch:.3e} hits/launch {hits_per_launch:.3e}",
-              file=sys.stderr)
-    if dom.startswith("intersect_and_composite"):
+            log(f"  {name:40s} {e}")
+    dom = max((k for k in table if k.startswith("mt_")), key=lambda k: table[k]["ms"])
+    if dom.startswith("mt_intersect_and_composite"):
         ms = timer.mean_ms(dom)
-        fl = flops_fwd if dom.endswith("forward") else 58 * tests_per_launch + 60 * hits_per_launch
+        fl = 58 * tests + (25 if dom.endswith("forward") else 60) * hits
         ach = fl / (ms * 1e-3) / 1e12
         roof = {"bound": "valu", "kernel": dom, "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4), "traffic": None,
                 "ms": round(ms, 4), "flops_per_launch": fl,
-                "note": "FP32 vector work: 58 flops per (pixel, triangle) test + 25 (fwd) / 60 (bwd) per hit"}
+                "note": "FP32 vector ops: 58 per (pixel, triangle) test + 25 (fwd) / 60 (bwd) per hit"}
     else:
         e = table[dom]
         roof = {"bound": "hbm", "kernel": dom, "achieved": e.get("GB/s"), "peak": HBM_PEAK_GBS,
@@ -420,11 +399,11 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        print("[bench] CPU baseline (oracle on a bounded sample)", file=sys.stderr, flush=True)
+        log("[bench] CPU baseline (oracle on a bounded sample)")
         cpu = cpu_baseline(sc, args)
 
     out = {
-        "metric": "train steps/s fwd+bwd (sphere-field step, plain per-op path)",
+        "metric": "train steps/s fwd+bwd (batched graph-captured step; plain per-camera path beside it)",
         "value": round(args.steps / elapsed, 4),
         "unit": "steps/s",
         "n_gpus": 1,
@@ -439,10 +418,13 @@ def main():
         "config": {
             "workload": (f"sphere-field training step: {args.resolution}^3 virtual grid, "
                          f"{args.spheres} spheres + ground plane, band 3 cells, SH deg 3, 5 shells, "
-                         f"{args.cameras} cameras {args.width}x{args.height}, tiles 16x16"),
-            "path": "plain",
+                         f"{args.cameras} cameras {args.width}x{args.height}"),
             "sizes": sizes,
         },
+        "plain_per_camera": {"steps_per_s": round(1.0 / plain_s, 4), "ms_per_step": round(1000 * plain_s, 3)},
+        "speedup_vs_plain": round(plain_s / (elapsed / args.steps), 3),
+        "autotune": {"tile_shape": list(best.tile_shape), "chunk": best.chunk,
+                     "candidates": [[list(c.tile_shape), c.chunk, round(c.ms, 3)] for c in tuned]},
         "roofline": roof,
         "kernels": table,
         "cpu_baseline": cpu,

@@ -120,16 +120,22 @@ int mt_create_triangle_vertices_forward(
     int64_t num_triangle_vertices, float* out_vertices, float* out_sdf, float* out_sh,
     uint8_t* out_types, int32_t* overflow, mt_stream_t stream);
 
-/* Adjoint of the forward (create_triangle_vertices.slang:201-291, .py:104-170).
- * Accumulates into zero-initialised grid gradients. Any grad_* input may be NULL
- * (no gradient flows from that output); any grid_*_grad output may be NULL (not
- * requested). */
+/* Workspace for the atomic-free path of mt_create_triangle_vertices_backward. */
+size_t mt_create_triangle_vertices_backward_workspace_bytes(int64_t num_grid_vertices);
+
+/* Adjoint of the forward (create_triangle_vertices.slang:201-291, .py:104-170) into
+ * zero-initialised grid gradients. Any grad_* input may be NULL (no gradient flows from
+ * that output); any grid_*_grad output may be NULL (not requested). With a workspace of
+ * at least the queried size and num_sh in {1, 4, 9, 16}, every grid vertex gathers its
+ * gradient from the columns that reference it (deterministic, no atomics); otherwise, or
+ * if two columns share an origin or an axis end, the columns scatter with atomics. */
 int mt_create_triangle_vertices_backward(
     const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh,
     const int32_t* lower_corners, int64_t num_columns, const float* level_sets, int32_t S,
     const int32_t* vertex_offsets, int64_t num_triangle_vertices, const float* grad_vertices,
     const float* grad_sdf, const float* grad_sh, float* grid_vertices_grad,
-    float* grid_sdf_grad, float* grid_sh_grad, mt_stream_t stream);
+    float* grid_sdf_grad, float* grid_sh_grad, void* workspace, size_t workspace_bytes,
+    mt_stream_t stream);
 
 /* face_counts int32[S * num_voxels] (3 x triangles) and cell_codes uint8[S, num_voxels].
  * Replaces count_triangle_faces.slang:13-50 (.py:28-63). upper_corners
@@ -141,14 +147,15 @@ int mt_count_triangle_faces(const float* sdf, const int32_t* lower_corners, int6
 
 /* faces int32[F, 3] and voxel_indices int32[F] from the exclusive face offsets
  * int32[S * num_voxels + 1]; F = face_capacity (faces at or beyond it are dropped and
- * flagged in *overflow when not NULL). Replaces create_triangle_faces.slang:3-78
- * (.py:28-65). */
+ * flagged in *overflow when not NULL). vertex_types holds vertex_capacity entries; a
+ * vertex range reaching past it yields corner -1 and raises the flag.
+ * Replaces create_triangle_faces.slang:3-78 (.py:28-65). */
 int mt_create_triangle_faces(const int32_t* neighbors, const int32_t* indices,
                              int64_t num_voxels, int64_t num_columns,
                              const int32_t* vertex_offsets, const uint8_t* vertex_types,
                              const uint8_t* cell_codes, int32_t S, const int32_t* face_offsets,
                              int32_t* faces, int32_t* voxel_indices, int64_t face_capacity,
-                             int32_t* overflow, mt_stream_t stream);
+                             int64_t vertex_capacity, int32_t* overflow, mt_stream_t stream);
 
 /* ---------------------------------------------------------------------------------
  * Rasterization (triangle_rasterization/__init__.py:22-329)
@@ -164,7 +171,8 @@ int mt_project_vertices(const float* vertices, int64_t V, const float* extrinsic
 /* Back-face / near-plane culling and tile bounds. faces int32[F, 3]; outputs
  * tile_min int32[B, F, 2] (row, col) inclusive, tile_max int32[B, F, 2] exclusive,
  * counts int32[B, F]. If num_faces_x3 is not NULL, only faces below (*num_faces_x3) / 3
- * are live (F is then the capacity) and the rest get empty bounds.
+ * are live (F is then the capacity) and the rest get empty bounds. A face with a corner
+ * outside [0, V) is culled (empty bounds).
  * Replaces bound_triangles.slang:144-192 (.py:35-71). */
 int mt_bound_triangles(const float* xy, const float* depths, int64_t V, const int32_t* faces,
                        int64_t F, int32_t B, int32_t tile_h, int32_t tile_w, int32_t row_min,
@@ -232,7 +240,7 @@ typedef struct mt_composite_args {
   const int64_t* tile_ranges; /* int64[B, T, 2] */
   int32_t num_cameras, num_shells, grid_rows, grid_cols, tile_h, tile_w, height, width;
   mt_key_layout layout;
-  int32_t chunk; /* triangles staged in LDS per round: 0 = tile pixel count */
+  int32_t chunk; /* triangles staged in LDS per round (<= 512): 0 = tile pixel count */
   /* Optional (may be NULL): uint64[2] incremented by the number of (pixel, triangle)
    * barycentric tests executed and of accepted hits. */
   unsigned long long* stats;
@@ -252,6 +260,14 @@ int mt_intersect_and_composite_backward(const mt_composite_args* args, const flo
                                         const float* grad_out, float* colors_grad,
                                         float* sdf_grad, float* sharpness_grad,
                                         int32_t reference_quirk, mt_stream_t stream);
+
+/* Training loss of the batched step: loss += mean((clip(rgb + T * background, 0, 1) -
+ * target)^2) over B x 3 x H x W, and grad_out f32[B, 4, H, W] = its gradient with
+ * respect to out f32[B, 4, H, W] (rgb, transmittance). The reference computes this loss
+ * with torch ops in the model (model_meshtryoshka.py:249-255, L2 part). background f32[3],
+ * target f32[B, 3, H, W]; *loss must be zeroed by the caller. */
+int mt_image_l2_loss(const float* out, const float* background, const float* target, int32_t B,
+                     int32_t H, int32_t W, float* loss, float* grad_out, mt_stream_t stream);
 
 /* Explicit mode: first hit per (camera, shell, pixel) in depth order. tile_ranges
  * int64[B, S, T, 2]; out_uv f32[B*S, H, W, 2]; out_index int32[B*S, H, W] (-1 = miss).

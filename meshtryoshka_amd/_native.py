@@ -80,14 +80,16 @@ SIGNATURES = {
         [c_p, c_p, c_p, c_i64, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p,
          c_p],
     ),
+    "mt_create_triangle_vertices_backward_workspace_bytes": (c_sz, [c_i64]),
     "mt_create_triangle_vertices_backward": (
         c_i32,
-        [c_p, c_i64, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p],
+        [c_p, c_i64, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
+         c_sz, c_p],
     ),
     "mt_count_triangle_faces": (c_i32, [c_p, c_p, c_i64, c_p, c_i64, c_p, c_i32, c_p, c_p, c_p]),
     "mt_create_triangle_faces": (
         c_i32,
-        [c_p, c_p, c_i64, c_i64, c_p, c_p, c_p, c_i32, c_p, c_p, c_p, c_i64, c_p, c_p],
+        [c_p, c_p, c_i64, c_i64, c_p, c_p, c_p, c_i32, c_p, c_p, c_p, c_i64, c_i64, c_p, c_p],
     ),
     "mt_project_vertices": (c_i32, [c_p, c_i64, c_p, c_p, c_i32, c_p, c_p, c_p]),
     "mt_bound_triangles": (
@@ -111,6 +113,7 @@ SIGNATURES = {
         c_i32,
         [ctypes.POINTER(CompositeArgs), c_p, c_p, c_p, c_p, c_p, c_i32, c_p],
     ),
+    "mt_image_l2_loss": (c_i32, [c_p, c_p, c_p, c_i32, c_i32, c_i32, c_p, c_p, c_p]),
     "mt_intersect": (
         c_i32,
         [c_p, c_p, c_p, c_i64, c_p, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_p,
@@ -149,7 +152,9 @@ def load_library(path: Path | str | None = None) -> ctypes.CDLL:
             )
         lib = ctypes.CDLL(str(p), mode=os.RTLD_NOW | os.RTLD_LOCAL)
         for name, (res, args) in SIGNATURES.items():
-            fn = getattr(lib, name)
+            fn = getattr(lib, name, None)
+            if fn is None:  # a stale build; calling it raises in call()
+                continue
             fn.restype = res
             fn.argtypes = args
         _lib = lib
@@ -158,7 +163,10 @@ def load_library(path: Path | str | None = None) -> ctypes.CDLL:
 
 def call(name: str, *args) -> None:
     lib = load_library()
-    rc = getattr(lib, name)(*args)
+    fn = getattr(lib, name, None)
+    if fn is None:
+        raise RuntimeError(f"{name} is missing from {LIB_PATH}: rebuild the library")
+    rc = fn(*args)
     if rc != 0:
         msg = lib.mt_get_last_error_string().decode(errors="replace")
         raise RuntimeError(f"{name} failed (code {rc}): {msg}")

@@ -22,6 +22,6 @@ def set_reference_gradient_quirks(enabled: bool) -> None:
 
 def set_composite_chunk(chunk: int) -> None:
     global composite_chunk
-    if not 0 <= int(chunk) <= 256:
-        raise ValueError("composite chunk must be in [0, 256]")
+    if not 0 <= int(chunk) <= 512:
+        raise ValueError("composite chunk must be in [0, 512]")
     composite_chunk = int(chunk)

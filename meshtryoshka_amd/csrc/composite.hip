@@ -114,8 +114,28 @@ __device__ __forceinline__ float sdf_to_cdf(float sdf, float k) { return 1.0f / 
 
 __device__ __forceinline__ int shell_of(const Staged& s) { return __float_as_int(s.g3.w); }
 
-// LDS budget: chunk <= kMaxChunk triangles of 112 + 48 + 4 bytes.
-constexpr int kMaxChunk = 256;
+// Fused kernels stage up to kMaxChunk triangles per round in dynamic LDS
+// (64 + 48 + 4 bytes each: 58 KB at 512); the explicit intersect kernel stages up to
+// kMaxIntersectChunk (geometry + face only).
+constexpr int kMaxChunk = 512;
+constexpr int kMaxIntersectChunk = 256;
+constexpr int kStagedBytes = sizeof(Staged) + sizeof(StagedColor) + sizeof(int32_t);
+
+struct StageBuffers {
+  Staged* geo;
+  StagedColor* col;
+  int32_t* face;
+};
+
+__device__ __forceinline__ StageBuffers carve_stage(char* smem, int chunk) {
+  StageBuffers b;
+  b.geo = reinterpret_cast<Staged*>(smem);
+  b.col = reinterpret_cast<StagedColor*>(smem + sizeof(Staged) * chunk);
+  b.face = reinterpret_cast<int32_t*>(smem + (sizeof(Staged) + sizeof(StagedColor)) * chunk);
+  return b;
+}
+
+extern __shared__ __align__(16) char g_dynamic_lds[];
 
 struct TileCoord {
   int cam, tile, x, y;
@@ -181,9 +201,10 @@ __device__ void add_stats(unsigned long long* stats, uint32_t tests, uint32_t hi
 }
 
 __global__ void __launch_bounds__(1024) fused_forward_kernel(mt_composite_args a, float* out) {
-  __shared__ Staged s_geo[kMaxChunk];
-  __shared__ StagedColor s_col[kMaxChunk];
-  __shared__ int32_t s_face[kMaxChunk];
+  const StageBuffers sb = carve_stage(g_dynamic_lds, a.chunk);
+  Staged* s_geo = sb.geo;
+  StagedColor* s_col = sb.col;
+  int32_t* s_face = sb.face;
   const TileCoord tc = tile_coord(a);
   const int T = a.grid_rows * a.grid_cols;
   const int64_t start = a.tile_ranges[2 * (int64_t(tc.cam) * T + tc.tile) + 0];
@@ -267,9 +288,10 @@ __global__ void __launch_bounds__(1024) fused_backward_kernel(mt_composite_args 
                                                               const float* gout, float* colors_grad,
                                                               float* sdf_grad, float* sharp_grad,
                                                               int quirk) {
-  __shared__ Staged s_geo[kMaxChunk];
-  __shared__ StagedColor s_col[kMaxChunk];
-  __shared__ int32_t s_face[kMaxChunk];
+  const StageBuffers sb = carve_stage(g_dynamic_lds, a.chunk);
+  Staged* s_geo = sb.geo;
+  StagedColor* s_col = sb.col;
+  int32_t* s_face = sb.face;
   __shared__ float s_red[1024 / kWave];
   const TileCoord tc = tile_coord(a);
   const int T = a.grid_rows * a.grid_cols;
@@ -400,8 +422,8 @@ __global__ void __launch_bounds__(1024) intersect_kernel(const int32_t* values, 
                                                          int grid_rows, int grid_cols, int th,
                                                          int tw, int H, int W, int chunk,
                                                          float* out_uv, int32_t* out_index) {
-  __shared__ Staged s_geo[kMaxChunk];
-  __shared__ int32_t s_face[kMaxChunk];
+  __shared__ Staged s_geo[kMaxIntersectChunk];
+  __shared__ int32_t s_face[kMaxIntersectChunk];
   const int T = grid_rows * grid_cols;
   const uint32_t bid = xcd_swizzle(blockIdx.x, gridDim.x);
   const int tile = bid % T;
@@ -592,6 +614,41 @@ __global__ void __launch_bounds__(256) composite_backward_kernel(
   }
 }
 
+// Mean squared error of the background-composited, [0, 1]-clipped image against a target
+// and its gradient with respect to the (rgb, transmittance) accumulators. The clip passes
+// gradient where 0 <= x <= 1, like torch.clamp.
+__global__ void __launch_bounds__(256) image_l2_loss_kernel(const float* out, const float* bg,
+                                                            const float* target, int64_t B,
+                                                            int64_t HW, float inv_n, float* loss,
+                                                            float* grad) {
+  __shared__ float s_red[256 / kWave];
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  float acc = 0.f;
+  if (p < B * HW) {
+    const int64_t b = p / HW, q = p % HW;
+    const float* o = out + b * 4 * HW + q;
+    const float* t = target + b * 3 * HW + q;
+    float* g = grad + b * 4 * HW + q;
+    const float trans = o[3 * HW];
+    float gt = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float pre = o[c * HW] + trans * bg[c];
+      const float img = fminf(fmaxf(pre, 0.0f), 1.0f);
+      const float diff = img - t[c * HW];
+      acc += diff * diff;
+      const float gc = (pre >= 0.0f && pre <= 1.0f) ? 2.0f * diff * inv_n : 0.0f;
+      g[c * HW] = gc;
+      gt += gc * bg[c];
+    }
+    g[3 * HW] = gt;
+  }
+  const float s = wave_sum_f(acc);
+  if (__lane_id() == 0) s_red[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(loss, (s_red[0] + s_red[1] + s_red[2] + s_red[3]) * inv_n);
+}
+
 int check_composite_args(const mt_composite_args* a, const char* name) {
   MT_REQUIRE(a != nullptr, "%s: null args", name);
   MT_REQUIRE(a->tile_h > 0 && a->tile_w > 0 && a->tile_h * a->tile_w <= 1024,
@@ -619,7 +676,8 @@ extern "C" int mt_intersect_and_composite_forward(const mt_composite_args* args,
   a.chunk = resolve_chunk(args);
   const int64_t blocks = int64_t(a.num_cameras) * a.grid_rows * a.grid_cols;
   if (blocks == 0) return MT_OK;
-  mt::fused_forward_kernel<<<static_cast<unsigned>(blocks), a.tile_h * a.tile_w, 0,
+  mt::fused_forward_kernel<<<static_cast<unsigned>(blocks), a.tile_h * a.tile_w,
+                             static_cast<size_t>(a.chunk) * mt::kStagedBytes,
                              static_cast<hipStream_t>(stream)>>>(a, out);
   return mt::check_launch("mt_intersect_and_composite_forward");
 }
@@ -633,10 +691,22 @@ extern "C" int mt_intersect_and_composite_backward(const mt_composite_args* args
   a.chunk = resolve_chunk(args);
   const int64_t blocks = int64_t(a.num_cameras) * a.grid_rows * a.grid_cols;
   if (blocks == 0) return MT_OK;
-  mt::fused_backward_kernel<<<static_cast<unsigned>(blocks), a.tile_h * a.tile_w, 0,
+  mt::fused_backward_kernel<<<static_cast<unsigned>(blocks), a.tile_h * a.tile_w,
+                              static_cast<size_t>(a.chunk) * mt::kStagedBytes,
                               static_cast<hipStream_t>(stream)>>>(
       a, out, grad_out, colors_grad, sdf_grad, sharpness_grad, reference_quirk);
   return mt::check_launch("mt_intersect_and_composite_backward");
+}
+
+extern "C" int mt_image_l2_loss(const float* out, const float* background, const float* target,
+                                int32_t B, int32_t H, int32_t W, float* loss, float* grad_out,
+                                mt_stream_t stream) {
+  MT_REQUIRE(B >= 0 && H >= 0 && W >= 0, "mt_image_l2_loss: negative size");
+  const int64_t n = int64_t(B) * H * W;
+  if (n == 0) return MT_OK;
+  mt::image_l2_loss_kernel<<<mt::blocks_for(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(
+      out, background, target, B, int64_t(H) * W, 1.0f / float(3 * n), loss, grad_out);
+  return mt::check_launch("mt_image_l2_loss");
 }
 
 extern "C" int mt_intersect(const int32_t* sorted_values, const int64_t* tile_ranges, const float* xy,
@@ -649,7 +719,7 @@ extern "C" int mt_intersect(const int32_t* sorted_values, const int64_t* tile_ra
   MT_REQUIRE(blocks < (int64_t(1) << 31), "mt_intersect: too many tiles");
   if (blocks == 0) return MT_OK;
   int chunk = tile_h * tile_w;
-  if (chunk > mt::kMaxChunk) chunk = mt::kMaxChunk;
+  if (chunk > mt::kMaxIntersectChunk) chunk = mt::kMaxIntersectChunk;
   mt::intersect_kernel<<<static_cast<unsigned>(blocks), tile_h * tile_w, 0,
                          static_cast<hipStream_t>(stream)>>>(sorted_values, tile_ranges, xy, V,
                                                               faces, B, S, grid_rows, grid_cols,

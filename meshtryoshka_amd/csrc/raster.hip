@@ -56,7 +56,9 @@ __global__ void __launch_bounds__(kBlock) bound_kernel(const float* xy, const fl
                                                        int32_t* counts, const int32_t* live_x3) {
   const int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (f >= F) return;
-  if (live_x3 != nullptr && f >= int64_t(*live_x3) / 3) {
+  const int64_t ia = faces[3 * f + 0], ib = faces[3 * f + 1], ic = faces[3 * f + 2];
+  const bool bad_corner = ia < 0 || ib < 0 || ic < 0 || ia >= V || ib >= V || ic >= V;
+  if ((live_x3 != nullptr && f >= int64_t(*live_x3) / 3) || bad_corner) {
     for (int b = 0; b < B; ++b) {
       const int64_t q = int64_t(b) * F + f;
       reinterpret_cast<int2*>(tmin)[q] = make_int2(0, 0);
@@ -65,7 +67,6 @@ __global__ void __launch_bounds__(kBlock) bound_kernel(const float* xy, const fl
     }
     return;
   }
-  const int64_t ia = faces[3 * f + 0], ib = faces[3 * f + 1], ic = faces[3 * f + 2];
   const float2* pxy = reinterpret_cast<const float2*>(xy);
   for (int b = 0; b < B; ++b) {
     const int64_t o = int64_t(b) * V;

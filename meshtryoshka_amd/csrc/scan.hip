@@ -108,12 +108,17 @@ __device__ Out look_back(const ScanState<Out>& st, int64_t tile) {
   return running;
 }
 
+// LDS index with one pad slot per 16 elements: thread t's 16 consecutive items start at
+// 17 t, so a wave's simultaneous accesses fall in distinct banks.
+__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+constexpr int kPadded = kTile + kTile / 16;
+
 template <typename In, typename Out, bool kInclusive>
 __global__ void __launch_bounds__(kBlock) scan_kernel(const In* in, Out* out, int64_t n,
                                                       ScanState<Out> st) {
   __shared__ union {
-    In in[kTile];
-    Out out[kTile];
+    In in[kPadded];
+    Out out[kPadded];
   } s_buf;
   __shared__ Out s_wave[kBlock / kWave];
   __shared__ int64_t s_tile;
@@ -126,13 +131,13 @@ __global__ void __launch_bounds__(kBlock) scan_kernel(const In* in, Out* out, in
   const int64_t count = (n - base) < kTile ? (n - base) : kTile;
 
   // Coalesced striped load into LDS.
-  for (int i = threadIdx.x; i < kTile; i += kBlock) s_buf.in[i] = (i < count) ? in[base + i] : In(0);
+  for (int i = threadIdx.x; i < kTile; i += kBlock) s_buf.in[pad(i)] = (i < count) ? in[base + i] : In(0);
   __syncthreads();
   Out items[kItems];
   Out local = 0;
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    items[k] = static_cast<Out>(s_buf.in[threadIdx.x * kItems + k]);
+    items[k] = static_cast<Out>(s_buf.in[pad(threadIdx.x * kItems + k)]);
     local += items[k];
   }
   Out block_total;
@@ -164,14 +169,14 @@ __global__ void __launch_bounds__(kBlock) scan_kernel(const In* in, Out* out, in
   for (int k = 0; k < kItems; ++k) {
     if (kInclusive) {
       run += items[k];
-      s_buf.out[threadIdx.x * kItems + k] = run;
+      s_buf.out[pad(threadIdx.x * kItems + k)] = run;
     } else {
-      s_buf.out[threadIdx.x * kItems + k] = run;
+      s_buf.out[pad(threadIdx.x * kItems + k)] = run;
       run += items[k];
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < count; i += kBlock) out[base + i] = s_buf.out[i];
+  for (int i = threadIdx.x; i < count; i += kBlock) out[base + i] = s_buf.out[pad(i)];
 }
 
 template <typename In, typename Out, bool kInclusive>

@@ -30,6 +30,9 @@ constexpr uint64_t kStateAggregate = 1ull << 62;
 constexpr uint64_t kStatePrefix = 2ull << 62;
 constexpr uint64_t kStateMask = 3ull << 62;
 constexpr uint64_t kValueMask = (1ull << 59) - 1;
+constexpr uint64_t kTagMask = ~kStateMask & ~kValueMask;
+// Predecessor look-back words loaded at once per digit (independent loads in flight).
+constexpr int kLookback = 16;
 
 struct SortWorkspace {
   unsigned long long* hist;  // [8][256]
@@ -201,6 +204,7 @@ __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys
   int32_t* dst_v = dst_sel == 1 ? out_values : w.tmp_values;
 
   __shared__ uint32_t s_wave_count[kWaves][kRadix];
+  __shared__ uint32_t s_hist[kRadix];
   __shared__ uint32_t s_local_start[kRadix];
   __shared__ int64_t s_global_base[kRadix];
   __shared__ int64_t s_keys[kTile];
@@ -209,6 +213,7 @@ __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys
   __shared__ uint32_t s_scan_wave[kWaves];
 
   for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&s_wave_count[0][0])[i] = 0u;
+  s_hist[threadIdx.x] = 0u;
   if (threadIdx.x == 0) s_tile = atomicAdd(&w.tickets[pass], 1u);
   __syncthreads();
   const int64_t tile = s_tile;
@@ -222,6 +227,7 @@ __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys
   // offset i * 64 + l, so (wave, item, lane) order is the input order.
   int64_t keys[kItems];
   int32_t vals[kItems];
+  uint32_t digits[kItems];
   uint32_t ranks[kItems];
   const int64_t wbase = base + int64_t(wave) * (kItems * kWave);
 #pragma unroll
@@ -230,57 +236,81 @@ __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys
     const bool valid = idx < n;
     keys[i] = valid ? src_k[idx] : 0;
     vals[i] = valid ? src_v[idx] : 0;
+    digits[i] = valid ? digit_of(static_cast<uint64_t>(keys[i]), pass) : kRadix;
   }
+  // 1. Tile histogram first, published before the (longer) ranking so that successors'
+  //    look-back finds this tile's aggregate early.
+#pragma unroll
+  for (int i = 0; i < kItems; ++i)
+    if (digits[i] < kRadix) atomicAdd(&s_hist[digits[i]], 1u);
+  __syncthreads();
+  const int d = threadIdx.x;
+  const uint32_t tile_count = s_hist[d];
+  const uint64_t tag = uint64_t(pass) << 59;
+  uint64_t* lb = w.lookback;
+  lb_store(&lb[tile * kRadix + d], (tile == 0 ? kStatePrefix : kStateAggregate) | tag | tile_count);
+
+  // 2. Stable rank within each wave: peers with the same digit via 8 ballots.
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    const int64_t idx = wbase + i * kWave + lane;
-    const bool valid = idx < n;
-    const uint32_t d = digit_of(static_cast<uint64_t>(keys[i]), pass);
+    const bool valid = digits[i] < kRadix;
+    const uint32_t dd = valid ? digits[i] : 0u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int bit = 0; bit < 8; ++bit) {
-      const bool set = (d >> bit) & 1u;
+      const bool set = (dd >> bit) & 1u;
       const uint64_t m = __ballot(set);
       peers &= set ? m : ~m;
     }
     const uint32_t before = static_cast<uint32_t>(__popcll(peers & lt));
-    const uint32_t old = valid ? s_wave_count[wave][d] : 0u;
+    const uint32_t old = valid ? s_wave_count[wave][dd] : 0u;
     ranks[i] = old + before;
-    if (valid && before == 0) s_wave_count[wave][d] = old + static_cast<uint32_t>(__popcll(peers));
+    if (valid && before == 0) s_wave_count[wave][dd] = old + static_cast<uint32_t>(__popcll(peers));
   }
   __syncthreads();
 
-  // Per digit: offsets of each wave inside the tile's run of that digit, and the count.
-  const int d = threadIdx.x;
-  uint32_t tile_count = 0;
+  // Per digit: offsets of each wave inside the tile's run of that digit.
+  uint32_t running = 0;
 #pragma unroll
   for (int q = 0; q < kWaves; ++q) {
     const uint32_t c = s_wave_count[q][d];
-    s_wave_count[q][d] = tile_count;
-    tile_count += c;
-  }
-  const uint64_t tag = uint64_t(pass) << 59;
-  uint64_t* lb = w.lookback;
-  if (tile == 0) {
-    lb_store(&lb[d], kStatePrefix | tag | tile_count);
-  } else {
-    lb_store(&lb[tile * kRadix + d], kStateAggregate | tag | tile_count);
+    s_wave_count[q][d] = running;
+    running += c;
   }
   // Exclusive scan of the tile counts over digits: start of each digit's run in LDS.
   uint32_t inc = wave_inclusive_sum(tile_count);
   if (lane == kWave - 1) s_scan_wave[wave] = inc;
-  // Look-back over preceding tiles for this digit.
+
+  // 3. Look-back over preceding tiles for this digit, kLookback words per round.
   int64_t prefix = 0;
   if (tile > 0) {
     int64_t t = tile - 1;
-    while (true) {
-      uint64_t word;
-      do {
-        word = lb_load(&lb[t * kRadix + d]);
-      } while ((word & kStateMask) == 0 || (word & ~kStateMask & ~kValueMask) != tag);
-      prefix += static_cast<int64_t>(word & kValueMask);
-      if ((word & kStateMask) == kStatePrefix) break;
-      --t;
+    bool done = false;
+    while (!done) {
+      uint64_t words[kLookback];
+#pragma unroll
+      for (int j = 0; j < kLookback; ++j)
+        words[j] = (t - j >= 0) ? lb_load(&lb[(t - j) * kRadix + d]) : (kStatePrefix | tag);
+      bool stop = false;
+      int consumed = 0;
+#pragma unroll
+      for (int j = 0; j < kLookback; ++j) {
+        const uint64_t wd = words[j];
+        const bool valid = (wd & kStateMask) != 0 && (wd & kTagMask) == tag;
+        if (!stop) {
+          if (!valid) {
+            stop = true;  // not published yet: poll again from this predecessor
+          } else {
+            prefix += static_cast<int64_t>(wd & kValueMask);
+            ++consumed;
+            if ((wd & kStateMask) == kStatePrefix) {
+              stop = true;
+              done = true;
+            }
+          }
+        }
+      }
+      t -= consumed;
     }
     lb_store(&lb[tile * kRadix + d], kStatePrefix | tag | uint64_t(prefix + tile_count));
   }
@@ -295,9 +325,8 @@ __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys
   // Reorder the tile by digit in LDS.
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    const int64_t idx = wbase + i * kWave + lane;
-    if (idx < n) {
-      const uint32_t dd = digit_of(static_cast<uint64_t>(keys[i]), pass);
+    if (digits[i] < kRadix) {
+      const uint32_t dd = digits[i];
       const uint32_t pos = s_local_start[dd] + s_wave_count[wave][dd] + ranks[i];
       s_keys[pos] = keys[i];
       s_vals[pos] = vals[i];

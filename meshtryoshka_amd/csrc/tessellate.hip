@@ -107,7 +107,9 @@ __global__ void __launch_bounds__(kBlock) create_vertices_kernel(
 __global__ void __launch_bounds__(kBlock) create_vertices_backward_kernel(
     const float* gsdf, int64_t ng, int nsh, const int32_t* lower, int64_t n,
     const float* level_sets, int S, const int32_t* offsets, int64_t nt, const float* g_v,
-    const float* g_d, const float* g_sh, float* gg_v, float* gg_d, float* gg_sh) {
+    const float* g_d, const float* g_sh, float* gg_v, float* gg_d, float* gg_sh,
+    const int32_t* skip_unless_flag) {
+  if (skip_unless_flag != nullptr && *skip_unless_flag == 0) return;
   const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const ColumnSdf col = load_column(gsdf, lower, n, c);
@@ -177,6 +179,102 @@ __global__ void __launch_bounds__(kBlock) create_vertices_backward_kernel(
   if (want_d) atomicAdd(gg_d + col.v0, acc0_d);
 }
 
+// ------------------------------------------------------------------------------------
+// Atomic-free adjoint. Each grid vertex g gathers from the (at most four) columns that
+// reference it: the column whose origin is g, and the columns whose +x / +y / +z end is g.
+// The inverse maps are built per call by a scatter over columns; if two columns claim the
+// same slot (a structure whose origins are not a permutation of the grid vertices) a flag
+// is raised and the atomic-scatter kernel above runs instead (decided on the device).
+// ------------------------------------------------------------------------------------
+
+__global__ void fill_maps_kernel(int32_t* maps, int64_t n_words) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n_words) maps[i] = -1;
+  if (i == 0) maps[n_words] = 0;  // collision flag
+}
+
+__global__ void __launch_bounds__(kBlock) build_maps_kernel(const int32_t* lower, int64_t n,
+                                                            int64_t ng, int32_t* maps) {
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  int32_t* flag = maps + 4 * ng;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int32_t g = lower[a * n + c];
+    if (g < 0 || g >= ng) {
+      if (a == 0) atomicOr(flag, 1);  // an origin outside the grid: use the scatter path
+      continue;
+    }
+    if (atomicCAS(maps + a * ng + g, -1, static_cast<int32_t>(c)) != -1) atomicOr(flag, 1);
+  }
+}
+
+template <int kNsh>
+__global__ void __launch_bounds__(kBlock) gather_vertices_backward_kernel(
+    const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
+    int S, const int32_t* offsets, int64_t nt, const float* g_v, const float* g_d,
+    const float* g_sh, float* gg_v, float* gg_d, float* gg_sh, const int32_t* maps) {
+  if (maps[4 * ng] != 0) return;  // not injective: the scatter kernel handles it
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  float acc_v[3] = {0.f, 0.f, 0.f};
+  float acc_d = 0.f;
+  float acc_sh[kNsh][3];
+#pragma unroll
+  for (int k = 0; k < kNsh; ++k) acc_sh[k][0] = acc_sh[k][1] = acc_sh[k][2] = 0.f;
+  // Role 0: g is the origin (weight 1 - t) of every crossing of its column; roles 1..3:
+  // g is the +x / +y / +z end (weight t) of that axis' crossings of another column.
+  for (int role = 0; role < 4; ++role) {
+    const int32_t c = maps[role * ng + g];
+    if (c < 0) continue;
+    const ColumnSdf col = load_column(gsdf, lower, n, c);
+    for (int s = 0; s < S; ++s) {
+      const float iso = level_sets[s];
+      bool cr[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) cr[a] = crosses(col.d0, col.d[a], iso);
+      int64_t idx = offsets[s * n + c];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (!cr[a]) continue;
+        const int64_t here = idx++;
+        if (role != 0 && role != a + 1) continue;
+        if (here >= nt) continue;
+        const float t = edge_t(col.d0, col.d[a], iso);
+        const float wgt = (role == 0) ? 1.0f - t : t;
+        if (g_v) {
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) acc_v[ch] += wgt * g_v[3 * here + ch];
+        }
+        if (g_d) acc_d += wgt * g_d[here];
+        if (g_sh) {
+#pragma unroll
+          for (int k = 0; k < kNsh; ++k) {
+            const float* gs = g_sh + (int64_t(k) * nt + here) * 3;
+            acc_sh[k][0] += wgt * gs[0];
+            acc_sh[k][1] += wgt * gs[1];
+            acc_sh[k][2] += wgt * gs[2];
+          }
+        }
+      }
+    }
+  }
+  if (gg_v && g_v) {
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) gg_v[3 * g + ch] = acc_v[ch];
+  }
+  if (gg_d && g_d) gg_d[g] = acc_d;
+  if (gg_sh && g_sh) {
+#pragma unroll
+    for (int k = 0; k < kNsh; ++k) {
+      float* o = gg_sh + (int64_t(k) * ng + g) * 3;
+      o[0] = acc_sh[k][0];
+      o[1] = acc_sh[k][1];
+      o[2] = acc_sh[k][2];
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) count_faces_kernel(const float* sdf, const int32_t* lower,
                                                              int64_t ncol, const int32_t* upper,
                                                              int64_t n, const float* level_sets,
@@ -208,7 +306,7 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
     const int32_t* neighbors, const int32_t* indices, int64_t n, int64_t ncol,
     const int32_t* voffsets, const uint8_t* vtypes, const uint8_t* codes, int S,
     const int32_t* foffsets, int32_t* faces, int32_t* voxel_indices, int64_t fcap,
-    int32_t* overflow) {
+    int64_t vcap, int32_t* overflow) {
   const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const int32_t linear = indices[v];
@@ -225,8 +323,12 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
       const int64_t owner = (slot < 0) ? v : neighbors[slot * n + v];
       const int64_t vo = s * ncol + owner;
       const int32_t start = voffsets[vo];
-      const int32_t cnt = voffsets[vo + 1] - start;
+      int32_t cnt = voffsets[vo + 1] - start;
       int32_t target = -1;
+      if (int64_t(start) + cnt > vcap) {  // vertices past the capacity were never written
+        if (overflow) atomicOr(overflow, 1);
+        cnt = 0;
+      }
       for (int k = 0; k < cnt; ++k) {
         if (vtypes[start + k] == axis) {
           target = start + k;
@@ -279,20 +381,55 @@ extern "C" int mt_create_triangle_vertices_forward(
   return mt::check_launch("mt_create_triangle_vertices_forward");
 }
 
+extern "C" size_t mt_create_triangle_vertices_backward_workspace_bytes(int64_t num_grid_vertices) {
+  return static_cast<size_t>(4 * num_grid_vertices + 1) * sizeof(int32_t);
+}
+
 extern "C" int mt_create_triangle_vertices_backward(
     const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh, const int32_t* lower_corners,
     int64_t num_columns, const float* level_sets, int32_t S, const int32_t* vertex_offsets,
     int64_t num_triangle_vertices, const float* grad_vertices, const float* grad_sdf,
     const float* grad_sh, float* grid_vertices_grad, float* grid_sdf_grad, float* grid_sh_grad,
-    mt_stream_t stream) {
+    void* workspace, size_t workspace_bytes, mt_stream_t stream) {
   MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0,
              "mt_create_triangle_vertices_backward: negative size");
   if (num_columns == 0 || S == 0 || num_triangle_vertices == 0) return MT_OK;
-  mt::create_vertices_backward_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0,
-                                        static_cast<hipStream_t>(stream)>>>(
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const float* gv = grid_vertices_grad ? grad_vertices : nullptr;
+  const float* gd = grid_sdf_grad ? grad_sdf : nullptr;
+  const float* gs = grid_sh_grad ? grad_sh : nullptr;
+  const size_t need = mt_create_triangle_vertices_backward_workspace_bytes(num_grid_vertices);
+  const bool gather = workspace != nullptr && workspace_bytes >= need &&
+                      (num_sh == 1 || num_sh == 4 || num_sh == 9 || num_sh == 16);
+  if (!gather) {
+    mt::create_vertices_backward_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
+        grid_sdf, num_grid_vertices, num_sh, lower_corners, num_columns, level_sets, S,
+        vertex_offsets, num_triangle_vertices, gv, gd, gs, grid_vertices_grad, grid_sdf_grad,
+        grid_sh_grad, nullptr);
+    return mt::check_launch("mt_create_triangle_vertices_backward");
+  }
+  int32_t* maps = static_cast<int32_t*>(workspace);
+  const int64_t words = 4 * num_grid_vertices;
+  mt::fill_maps_kernel<<<mt::blocks_for(words, 256), 256, 0, st>>>(maps, words);
+  mt::build_maps_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
+      lower_corners, num_columns, num_grid_vertices, maps);
+  const dim3 ggrid(mt::blocks_for(num_grid_vertices, mt::kBlock));
+#define MT_GATHER(N)                                                                          \
+  mt::gather_vertices_backward_kernel<N><<<ggrid, mt::kBlock, 0, st>>>(                       \
+      grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S, vertex_offsets, \
+      num_triangle_vertices, gv, gd, gs, grid_vertices_grad, grid_sdf_grad, grid_sh_grad, maps)
+  switch (num_sh) {
+    case 1: MT_GATHER(1); break;
+    case 4: MT_GATHER(4); break;
+    case 9: MT_GATHER(9); break;
+    default: MT_GATHER(16); break;
+  }
+#undef MT_GATHER
+  // Runs only if the maps found a collision (device-side decision).
+  mt::create_vertices_backward_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
       grid_sdf, num_grid_vertices, num_sh, lower_corners, num_columns, level_sets, S,
-      vertex_offsets, num_triangle_vertices, grad_vertices, grad_sdf, grad_sh, grid_vertices_grad,
-      grid_sdf_grad, grid_sh_grad);
+      vertex_offsets, num_triangle_vertices, gv, gd, gs, grid_vertices_grad, grid_sdf_grad,
+      grid_sh_grad, maps + words);
   return mt::check_launch("mt_create_triangle_vertices_backward");
 }
 
@@ -319,12 +456,13 @@ extern "C" int mt_create_triangle_faces(const int32_t* neighbors, const int32_t*
                                         const uint8_t* cell_codes, int32_t S,
                                         const int32_t* face_offsets, int32_t* faces,
                                         int32_t* voxel_indices, int64_t face_capacity,
-                                        int32_t* overflow, mt_stream_t stream) {
+                                        int64_t vertex_capacity, int32_t* overflow,
+                                        mt_stream_t stream) {
   MT_REQUIRE(num_voxels >= 0 && S >= 0 && face_capacity >= 0, "mt_create_triangle_faces: bad sizes");
   if (num_voxels == 0 || S == 0) return MT_OK;
   mt::create_faces_kernel<<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
                             static_cast<hipStream_t>(stream)>>>(
       neighbors, indices, num_voxels, num_columns, vertex_offsets, vertex_types, cell_codes, S,
-      face_offsets, faces, voxel_indices, face_capacity, overflow);
+      face_offsets, faces, voxel_indices, face_capacity, vertex_capacity, overflow);
   return mt::check_launch("mt_create_triangle_faces");
 }

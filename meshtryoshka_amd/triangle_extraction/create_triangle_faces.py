@@ -33,6 +33,7 @@ def create_triangle_faces(signed_distances: Tensor, neighbors: Tensor, indices: 
         num_voxels, num_columns, vertex_offsets.contiguous().data_ptr(),
         triangle_vertex_types.contiguous().data_ptr(), voxel_cell_codes.contiguous().data_ptr(),
         num_level_sets, face_offsets.contiguous().data_ptr(), faces.data_ptr(),
-        voxel_indices.data_ptr(), num_faces, None, _native.stream(neighbors),
+        voxel_indices.data_ptr(), num_faces, triangle_vertex_types.shape[0], None,
+        _native.stream(neighbors),
     )
     return CreateTriangleFacesResult(faces, voxel_indices)

@@ -74,13 +74,17 @@ class CreateTriangleVertices(torch.autograd.Function):
         gd = grad_signed_distances.contiguous() if (need_d and grad_signed_distances is not None) else None
         gs = grad_spherical_harmonics.contiguous() if (need_sh and grad_spherical_harmonics is not None) else None
         if any(g is not None for g in (gv, gd, gs)):
+            ws = _native.workspace(
+                _native.query("mt_create_triangle_vertices_backward_workspace_bytes", num_grid),
+                grid_sdf.device)
             _native.call(
                 "mt_create_triangle_vertices_backward", grid_sdf.data_ptr(), num_grid, num_sh,
                 lower.data_ptr(), num_columns, level_sets.data_ptr(), num_level_sets,
                 vertex_offsets.data_ptr(), num_vertices, _native.ptr(gv), _native.ptr(gd),
                 _native.ptr(gs), _native.ptr(gg_v if gv is not None else None),
                 _native.ptr(gg_d if gd is not None else None),
-                _native.ptr(gg_sh if gs is not None else None), _native.stream(grid_sdf),
+                _native.ptr(gg_sh if gs is not None else None), ws.data_ptr(), ws.numel(),
+                _native.stream(grid_sdf),
             )
         return gg_v, gg_d, gg_sh, None, None, None
 

@@ -134,3 +134,27 @@ def test_create_vertices_backward_matches_oracle(scene):
         g = got.cpu().numpy().astype(np.float64)
         scale = np.abs(want).max() + 1e-12
         assert np.abs(g - want).max() / scale < 1e-5
+
+
+def test_vertices_backward_with_shared_origins_uses_scatter_path(scene):
+    """Columns whose origins repeat (not a permutation of the grid vertices) make the
+    gather path's inverse maps collide; the device then runs the atomic scatter instead."""
+    v = scene.vox
+    lower = np.concatenate([v.lower_corners, v.lower_corners[:, -50:]], axis=1)
+    ls = scene.level_sets
+    counts = ox.count_triangle_vertices(scene.sdf, lower, ls)
+    offsets = ox.exclusive_cumsum(counts)
+    _, _, _, _, record = ox.create_triangle_vertices(scene.world, scene.sdf, scene.sh, lower, ls, offsets)
+    grid_d = to_dev(scene.sdf).requires_grad_(True)
+    grid_sh = to_dev(scene.sh).requires_grad_(True)
+    t_off = to_dev(offsets)
+    out = te.create_triangle_vertices(to_dev(scene.world), grid_d, grid_sh, to_dev(lower), to_dev(ls), t_off)
+    rng = np.random.default_rng(11)
+    gd = rng.standard_normal(out.signed_distances.shape).astype(np.float32)
+    gs = rng.standard_normal(out.spherical_harmonics.shape).astype(np.float32)
+    torch.autograd.backward([out.signed_distances, out.spherical_harmonics], [to_dev(gd), to_dev(gs)])
+    _, wd, ws = ox.create_triangle_vertices_backward(record, scene.sdf.shape[0], scene.sh.shape[0],
+                                                     None, gd, gs)
+    for got, want in ((grid_d.grad, wd), (grid_sh.grad, ws)):
+        g = got.cpu().numpy().astype(np.float64)
+        assert np.abs(g - want).max() / (np.abs(want).max() + 1e-12) < 1e-5

@@ -1,0 +1,78 @@
+"""Continuous-batching render server: a request's image is the same alone and in a mixed
+batch, and equals the plain render API's image."""
+
+import pytest
+import torch
+
+from meshtryoshka_amd import serving
+from meshtryoshka_amd import triangle_extraction as te
+from meshtryoshka_amd import triangle_rasterization as tr
+from meshtryoshka_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def scenes():
+    dev = torch.device("cuda")
+    out = {}
+    for sid, n in enumerate((4, 12)):
+        spec = workloads.sphere_field(seed=sid, num_spheres=n, resolution=128, num_cameras=1,
+                                      image_shape=(64, 64), device=dev)
+        vox = te.extract_voxels(spec.occupancy.to(dev))
+        world, sdf, sh = workloads.finish_fields(spec, vox.vertices)
+        sc = serving.tessellate_scene(vox, world.to(dev), sdf.to(dev), sh.to(dev),
+                                      spec.level_sets.to(dev), spec.sh_degree, spec.sharpness,
+                                      spec.background)
+        cams = spec.cameras.to(dev)
+        sc.keys_per_pixel = serving.calibrate_key_density(sc, cams.extrinsics, cams.intrinsics, (64, 64))
+        out[sid] = sc
+    return out
+
+
+def _poses(n, seed, shape):
+    gen = torch.Generator("cpu").manual_seed(seed)
+    return workloads.hemisphere_cameras(gen, n, shape)
+
+
+def _request(rid, sid, pose):
+    e, k, shape = pose
+    return serving.Request(arrival=0.0, rid=rid, scene_id=sid, extrinsics=e, intrinsics=k,
+                           image_shape=shape)
+
+
+def test_same_image_alone_and_in_mix(scenes):
+    target_pose = _poses(1, 7, (64, 96))[0]
+    server = serving.RenderServer(scenes, hbm_budget_bytes=8 << 30, max_batch=4, keep_images=True)
+    alone = _request(0, 1, target_pose)
+    server.submit(alone)
+    server.drain()
+    torch.cuda.synchronize()
+    mix = [_request(1, 1, p) for p in _poses(3, 8, (64, 96))]
+    mix += [_request(10, 0, p) for p in _poses(3, 9, (48, 48))]
+    mixed = _request(99, 1, target_pose)
+    mix.insert(2, mixed)
+    server2 = serving.RenderServer(scenes, hbm_budget_bytes=8 << 30, max_batch=4, keep_images=True)
+    for r in mix:
+        server2.submit(r)
+    server2.drain()
+    torch.cuda.synchronize()
+    assert torch.equal(alone.image, mixed.image)
+    # And both equal the plain API render of that one camera.
+    sc = scenes[1]
+    e, k, shape = target_pose
+    img = tr.render(e[None].cuda(), k[None].cuda(), shape, sc.vertices, sc.faces, sc.face_boundaries,
+                    sc.signed_distances, sc.spherical_harmonics, sc.sh_degree, sc.sharpness[0],
+                    sc.background)
+    assert torch.equal(img[0], alone.image)
+
+
+def test_budget_limits_inflight_work(scenes):
+    server = serving.RenderServer(scenes, hbm_budget_bytes=1, max_batch=2, keep_images=False)
+    for i, p in enumerate(_poses(5, 11, (64, 64))):
+        server.submit(_request(i, 0, p))
+    server.schedule()
+    # With no memory to spare, only the head request is admitted.
+    assert len(server.inflight) == 1 and len(server.queue) == 4
+    server.drain()
+    assert len(server.completed) == 5
