@@ -43,6 +43,8 @@ def parse_args():
     p.add_argument("--cpu-sample-tiles", type=int, default=24)
     p.add_argument("--breakdown", action="store_true", help="print the per-kernel table to stderr")
     p.add_argument("--no-autotune", action="store_true", help="use 16x16 tiles, chunk 256")
+    p.add_argument("--probe", action="store_true",
+                   help="build the scene, print its sizes and memory, and exit")
     return p.parse_args()
 
 
@@ -303,6 +305,17 @@ def main():
     inputs = (sc.world, sc.sdf.detach(), sc.sh.detach(), sc.target)
     b = sc.cams.extrinsics.shape[0]
     h, w = sc.cams.image_shape
+    if args.probe:
+        from meshtryoshka_amd import batched
+
+        for ts in ((16, 16), (8, 8)):
+            cap = batched.measure_capacities(sc.voxels, sc.level_sets, sc.cams.extrinsics,
+                                             sc.cams.intrinsics, (h, w), sc.world, sc.sdf.detach(),
+                                             sc.sh.detach(), tile_shape=ts)
+            log(f"[probe] tiles {ts}: {cap}")
+        log(f"[probe] voxels {sc.voxels.num_voxels} columns {sc.voxels.lower_corners.shape[1]} "
+            f"peak memory {torch.cuda.max_memory_allocated() / 2**30:.1f} GiB")
+        return
 
     # 1. Plain per-camera API path.
     for _ in range(max(1, args.warmup // 2)):
