@@ -42,7 +42,13 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-tiles", type=int, default=24)
     p.add_argument("--breakdown", action="store_true", help="print the per-kernel table to stderr")
-    p.add_argument("--no-autotune", action="store_true", help="use 16x16 tiles, chunk 256")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="skip the autotuner and use --tile / --chunk")
+    p.add_argument("--tile", type=int, nargs=2, default=(16, 16), metavar=("H", "W"))
+    p.add_argument("--chunk", type=int, default=256)
+    p.add_argument("--only-batched", action="store_true",
+                   help="time only the batched step with --tile / --chunk (no plain path, no "
+                        "autotuner, no CPU baseline): the run profiled per kernel")
     p.add_argument("--probe", action="store_true",
                    help="build the scene, print its sizes and memory, and exit")
     return p.parse_args()
@@ -318,20 +324,22 @@ def main():
         return
 
     # 1. Plain per-camera API path.
-    for _ in range(max(1, args.warmup // 2)):
-        plain_per_camera_step(sc)
-    torch.cuda.synchronize()
-    plain_steps = max(2, args.steps // 3)
-    t0 = time.perf_counter()
-    for _ in range(plain_steps):
-        plain_loss = plain_per_camera_step(sc)
-    torch.cuda.synchronize()
-    plain_s = (time.perf_counter() - t0) / plain_steps
-    log(f"[bench] plain per-camera path: {1000 * plain_s:.2f} ms/step (loss {float(plain_loss):.6f})")
+    plain_s = None
+    if not args.only_batched:
+        for _ in range(max(1, args.warmup // 2)):
+            plain_per_camera_step(sc)
+        torch.cuda.synchronize()
+        plain_steps = max(2, args.steps // 3)
+        t0 = time.perf_counter()
+        for _ in range(plain_steps):
+            plain_loss = plain_per_camera_step(sc).detach()
+        torch.cuda.synchronize()
+        plain_s = (time.perf_counter() - t0) / plain_steps
+        log(f"[bench] plain per-camera path: {1000 * plain_s:.2f} ms/step (loss {float(plain_loss):.6f})")
 
     # 2. Autotune the batched step (tile shape x chunk).
-    if args.no_autotune:
-        best = autotune.Choice((16, 16), 256, float("nan"))
+    if args.no_autotune or args.only_batched:
+        best = autotune.Choice(tuple(args.tile), args.chunk, float("nan"))
         tuned = []
     else:
         best, tuned = autotune.autotune(sc.voxels, sc.level_sets, sc.cams.extrinsics, sc.cams.intrinsics,
@@ -411,7 +419,7 @@ def main():
                 "unit": "GB/s", "frac": e.get("hbm_frac"), "traffic": None, "ms": e["ms"]}
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not (args.no_cpu_baseline or args.only_batched):
         log("[bench] CPU baseline (oracle on a bounded sample)")
         cpu = cpu_baseline(sc, args)
 
@@ -434,8 +442,9 @@ def main():
                          f"{args.cameras} cameras {args.width}x{args.height}"),
             "sizes": sizes,
         },
-        "plain_per_camera": {"steps_per_s": round(1.0 / plain_s, 4), "ms_per_step": round(1000 * plain_s, 3)},
-        "speedup_vs_plain": round(plain_s / (elapsed / args.steps), 3),
+        "plain_per_camera": None if plain_s is None else {
+            "steps_per_s": round(1.0 / plain_s, 4), "ms_per_step": round(1000 * plain_s, 3)},
+        "speedup_vs_plain": None if plain_s is None else round(plain_s / (elapsed / args.steps), 3),
         "autotune": {"tile_shape": list(best.tile_shape), "chunk": best.chunk,
                      "candidates": [[list(c.tile_shape), c.chunk, round(c.ms, 3)] for c in tuned]},
         "roofline": roof,

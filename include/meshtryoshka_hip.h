@@ -238,6 +238,7 @@ typedef struct mt_composite_args {
   const int64_t* sorted_keys; /* int64[K] */
   const int32_t* sorted_values; /* int32[K] */
   const int64_t* tile_ranges; /* int64[B, T, 2] */
+  /* A tile holds at most 512 pixels (tile_h * tile_w <= 512). */
   int32_t num_cameras, num_shells, grid_rows, grid_cols, tile_h, tile_w, height, width;
   mt_key_layout layout;
   int32_t chunk; /* triangles staged in LDS per round (<= 512): 0 = tile pixel count */

@@ -1,66 +1,68 @@
 // Tile compositing: fused intersect-and-composite (forward and adjoint) and the explicit
 // mode's intersect / interpolate / composite.
 //
-// One workgroup per (camera, tile), one lane per pixel (wave64, so a 16x16 tile is four
-// waves). The tile's depth-sorted keys are staged through LDS in chunks; for each staged
-// triangle the per-triangle terms of the barycentric solve are computed once at staging,
-// so a pixel spends ~25 VALU ops per (pixel, triangle) test. The three IEEE divisions
-// of the test are only evaluated when a cheap reciprocal estimate cannot already prove
-// a miss (see bary_test): the hit decisions stay bit-identical to the reference formula.
+// One workgroup per (camera, tile) with one thread per pixel (rounded up to whole
+// waves). The tile's depth-sorted keys are processed in chunks staged in LDS:
+//
+//   stage    one thread per key: the triangle's barycentric terms (TriTerms) and the
+//            rectangle of tile pixels it can possibly hit (candidate_rect, proven
+//            conservative), then a block scan of the rectangle areas;
+//   test     (pixel, triangle) candidate pairs spread evenly over all threads; each
+//            accepted pair sets bit i of its pixel's hit mask (atomicOr in LDS);
+//   composite one thread per pixel walks its mask bits in key order and applies the
+//            reference's per-pixel compositing rule to the accepted hits, recomputing
+//            their barycentrics (the same IEEE operations, so the same values).
+//
+// A pixel therefore tests only the triangles whose rectangle covers it instead of every
+// key of the tile, and the per-pixel traversal order, hits and arithmetic are exactly
+// those of the reference's sequential loop.
 #include "common.h"
+#include "geometry.h"
 
 namespace mt {
 namespace {
 
-constexpr float kEpsBary = 1e-8f;   // intersect_and_composite.slang:99
-constexpr float kSumLo = 0.999f;    // :132
-constexpr float kSumHi = 1.001f;
-constexpr float kAlphaEps = 1e-5f;  // :161
+constexpr float kAlphaEps = 1e-5f;  // intersect_and_composite.slang:161
 constexpr float kSuffixEps = 1e-6f; // :3
 constexpr float kDummySdf = 1e10f;  // :318
 
-// Staged triangle (7 x float4 = 112 bytes).
+// Staged triangle (4 x float4 = 64 bytes).
 struct Staged {
   float4 g0;  // p1.x, p1.y, p2.x, p2.y
   float4 g1;  // e12.x, e12.y, e13.x, e13.y
   float4 g2;  // d_13_13, d_12_13, d_12_12, d_12_23
   float4 g3;  // d_13_23, D = denom + eps, 1 / D (estimate), shell (bits)
 };
-struct StagedColor {
-  float4 a, b, c;  // rgb + sdf of the three corners
-};
 
-__device__ __forceinline__ Staged stage_geometry(float2 p1, float2 p2, float2 p3, int shell) {
-  const float e12x = p2.x - p1.x, e12y = p2.y - p1.y;
-  const float e23x = p3.x - p2.x, e23y = p3.y - p2.y;
-  const float e13x = p3.x - p1.x, e13y = p3.y - p1.y;
-  const float d_12_12 = e12x * e12x + e12y * e12y;
-  const float d_12_23 = e12x * e23x + e12y * e23y;
-  const float d_12_13 = e12x * e13x + e12y * e13y;
-  const float d_13_13 = e13x * e13x + e13y * e13y;
-  const float d_13_23 = e13x * e23x + e13y * e23y;
-  const float d_23_23 = e23x * e23x + e23y * e23y;
-  const float a = fabsf(d_12_13), b = fabsf(d_12_23), c = fabsf(d_13_23);
-  float denom;
-  if (a <= b && a <= c) {
-    denom = d_12_12 * d_13_13 - d_12_13 * d_12_13;
-  } else if (b <= a && b <= c) {
-    denom = d_12_12 * d_23_23 - d_12_23 * d_12_23;
-  } else if (c <= a && c <= b) {
-    denom = d_13_13 * d_23_23 - d_13_23 * d_13_23;
-  } else {
-    denom = __builtin_nanf("");  // only reachable with NaN inputs; every test then misses
-  }
-  const float D = denom + kEpsBary;
+__device__ __forceinline__ Staged to_staged(const TriTerms& t, int shell) {
   // The estimate is only trusted where its error bound holds (normal, moderate |D|).
-  const float ad = fabsf(D);
-  const float r = (ad >= 0x1p-60f && ad <= 0x1p60f) ? __builtin_amdgcn_rcpf(D) : 0.0f;
+  const float ad = fabsf(t.D);
+  const float r = (ad >= 0x1p-60f && ad <= 0x1p60f) ? __builtin_amdgcn_rcpf(t.D) : 0.0f;
   Staged s;
-  s.g0 = make_float4(p1.x, p1.y, p2.x, p2.y);
-  s.g1 = make_float4(e12x, e12y, e13x, e13y);
-  s.g2 = make_float4(d_13_13, d_12_13, d_12_12, d_12_23);
-  s.g3 = make_float4(d_13_23, D, r, __int_as_float(shell));
+  s.g0 = make_float4(t.p1x, t.p1y, t.p2x, t.p2y);
+  s.g1 = make_float4(t.e12x, t.e12y, t.e13x, t.e13y);
+  s.g2 = make_float4(t.d1313, t.d1213, t.d1212, t.d1223);
+  s.g3 = make_float4(t.d1323, t.D, r, __int_as_float(shell));
   return s;
+}
+
+__device__ __forceinline__ TriTerms to_terms(const Staged& s) {
+  TriTerms t;
+  t.p1x = s.g0.x;
+  t.p1y = s.g0.y;
+  t.p2x = s.g0.z;
+  t.p2y = s.g0.w;
+  t.e12x = s.g1.x;
+  t.e12y = s.g1.y;
+  t.e13x = s.g1.z;
+  t.e13y = s.g1.w;
+  t.d1313 = s.g2.x;
+  t.d1213 = s.g2.y;
+  t.d1212 = s.g2.z;
+  t.d1223 = s.g2.w;
+  t.d1323 = s.g3.x;
+  t.D = s.g3.y;
+  return t;
 }
 
 struct Bary {
@@ -76,18 +78,10 @@ struct Bary {
 // r == 0 (D outside [2^-60, 2^60] or NaN) disables the shortcut.
 __device__ __forceinline__ Bary bary_test(const Staged& s, float px, float py) {
   Bary out;
-  const float a1x = px - s.g0.x, a1y = py - s.g0.y;
-  const float a2x = px - s.g0.z, a2y = py - s.g0.w;
-  const float d_1x_12 = a1x * s.g1.x + a1y * s.g1.y;
-  const float d_1x_13 = a1x * s.g1.z + a1y * s.g1.w;
-  const float d_2x_12 = a2x * s.g1.x + a2y * s.g1.y;
-  const float d_2x_13 = a2x * s.g1.z + a2y * s.g1.w;
-  const float vn = s.g2.x * d_1x_12 - s.g2.y * d_1x_13;
-  const float wn = s.g2.z * d_1x_13 - s.g2.y * d_1x_12;
-  const float un = s.g2.w * d_2x_13 - s.g3.x * d_2x_12;
+  const BaryNum n = bary_numerators(to_terms(s), px, py);
   const float r = s.g3.z;
   if (r != 0.0f) {
-    const float ua = un * r, va = vn * r, wa = wn * r;
+    const float ua = n.un * r, va = n.vn * r, wa = n.wn * r;
     constexpr float kNeg = -0x1p-100f;
     constexpr float kBig = 1.0f + 0x1p-19f;
     if (ua < kNeg || va < kNeg || wa < kNeg || ua > kBig || va > kBig || wa > kBig) {
@@ -97,16 +91,10 @@ __device__ __forceinline__ Bary bary_test(const Staged& s, float px, float py) {
     }
   }
   const float D = s.g3.y;
-  const float v = vn / D;
-  const float w = wn / D;
-  const float u = un / D;
-  const float sum = (u + v) + w;
-  const bool miss = isnan(u) || isnan(v) || isnan(w) || u < 0.0f || u > 1.0f || v < 0.0f ||
-                    v > 1.0f || w < 0.0f || w > 1.0f || sum < kSumLo || sum > kSumHi;
-  out.u = u;
-  out.v = v;
-  out.w = w;
-  out.hit = !miss;
+  out.v = n.vn / D;
+  out.w = n.wn / D;
+  out.u = n.un / D;
+  out.hit = bary_accept(out.u, out.v, out.w);
   return out;
 }
 
@@ -114,74 +102,180 @@ __device__ __forceinline__ float sdf_to_cdf(float sdf, float k) { return 1.0f / 
 
 __device__ __forceinline__ int shell_of(const Staged& s) { return __float_as_int(s.g3.w); }
 
-// Fused kernels stage up to kMaxChunk triangles per round in dynamic LDS
-// (64 + 48 + 4 bytes each: 58 KB at 512); the explicit intersect kernel stages up to
-// kMaxIntersectChunk (geometry + face only).
-constexpr int kMaxChunk = 512;
-constexpr int kMaxIntersectChunk = 256;
-constexpr int kStagedBytes = sizeof(Staged) + sizeof(StagedColor) + sizeof(int32_t);
+// ------------------------------------------------------------------------------------
+// Chunk staging, candidate pairs and per-pixel hit masks (LDS)
+// ------------------------------------------------------------------------------------
 
-struct StageBuffers {
-  Staged* geo;
-  StagedColor* col;
-  int32_t* face;
+constexpr int kMaxChunk = 512;
+constexpr int kMaxLds = 64 * 1024;
+constexpr int kMaxTilePixels = 512;  // one thread per pixel, at most 8 waves per workgroup
+
+struct ChunkLds {
+  Staged* geo;       // [chunk]
+  uint2* rect;       // [chunk] (x | w << 16, y | h << 16), tile-local pixels
+  int32_t* face;     // [chunk]
+  int32_t* prefix;   // [chunk + 1] exclusive prefix of rectangle areas
+  uint32_t* mask;    // [words][npix]: bit i of word i / 32 = triangle i hits the pixel
+  uint8_t* live;     // [npix] pixel still compositing
+  int words, npix;
 };
 
-__device__ __forceinline__ StageBuffers carve_stage(char* smem, int chunk) {
-  StageBuffers b;
-  b.geo = reinterpret_cast<Staged*>(smem);
-  b.col = reinterpret_cast<StagedColor*>(smem + sizeof(Staged) * chunk);
-  b.face = reinterpret_cast<int32_t*>(smem + (sizeof(Staged) + sizeof(StagedColor)) * chunk);
-  return b;
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+__host__ __device__ inline size_t chunk_lds_bytes(int chunk, int npix) {
+  const int words = (chunk + 31) / 32;
+  return align16(sizeof(Staged) * chunk) + align16(sizeof(uint2) * chunk) +
+         align16(4 * size_t(chunk)) + align16(4 * size_t(chunk + 1)) +
+         align16(4 * size_t(words) * npix) + align16(npix);
 }
 
 extern __shared__ __align__(16) char g_dynamic_lds[];
 
+__device__ __forceinline__ ChunkLds carve_chunk(int chunk, int npix) {
+  ChunkLds L;
+  char* p = g_dynamic_lds;
+  L.words = (chunk + 31) / 32;
+  L.npix = npix;
+  L.geo = reinterpret_cast<Staged*>(p);
+  p += align16(sizeof(Staged) * chunk);
+  L.rect = reinterpret_cast<uint2*>(p);
+  p += align16(sizeof(uint2) * chunk);
+  L.face = reinterpret_cast<int32_t*>(p);
+  p += align16(4 * size_t(chunk));
+  L.prefix = reinterpret_cast<int32_t*>(p);
+  p += align16(4 * size_t(chunk + 1));
+  L.mask = reinterpret_cast<uint32_t*>(p);
+  p += align16(4 * size_t(L.words) * npix);
+  L.live = reinterpret_cast<uint8_t*>(p);
+  return L;
+}
+
+// Exclusive block prefix sum (blockDim.x a multiple of 64, at most 512); *total gets the
+// block sum. Ends with a barrier, so s_wave may be reused at once.
+__device__ __forceinline__ int block_exclusive_sum(int x, int* s_wave, int* total) {
+  const int wave = threadIdx.x / kWave;
+  const int lane = __lane_id();
+  const int inc = wave_inclusive_sum(x);
+  if (lane == kWave - 1) s_wave[wave] = inc;
+  __syncthreads();
+  int before = 0, sum = 0;
+  const int nw = blockDim.x / kWave;
+  for (int w = 0; w < nw; ++w) {
+    const int v = s_wave[w];
+    if (w < wave) before += v;
+    sum += v;
+  }
+  *total = sum;
+  __syncthreads();
+  return before + inc - x;
+}
+
 struct TileCoord {
-  int cam, tile, x, y;
+  int cam, tile, x0, y0;  // tile origin in pixels
+  int pix;                // this thread's pixel (tile-local), -1 for padding threads
+  int x, y;
   bool in_image;
 };
 
-__device__ __forceinline__ TileCoord tile_coord(const mt_composite_args& a) {
-  const int T = a.grid_rows * a.grid_cols;
-  const uint32_t bid = xcd_swizzle(blockIdx.x, gridDim.x);
+__device__ __forceinline__ TileCoord tile_coord(int bid_linear, int grid_cols, int T, int th,
+                                                int tw, int height, int width) {
   TileCoord c;
-  c.cam = bid / T;
-  c.tile = bid % T;
-  const int ty = c.tile / a.grid_cols, tx = c.tile % a.grid_cols;
-  const int ly = threadIdx.x / a.tile_w, lx = threadIdx.x % a.tile_w;
-  c.x = tx * a.tile_w + lx;
-  c.y = ty * a.tile_h + ly;
-  c.in_image = c.x < a.width && c.y < a.height;
+  c.cam = bid_linear / T;
+  c.tile = bid_linear % T;
+  const int ty = c.tile / grid_cols, tx = c.tile % grid_cols;
+  c.x0 = tx * tw;
+  c.y0 = ty * th;
+  const int npix = th * tw;
+  c.pix = int(threadIdx.x) < npix ? int(threadIdx.x) : -1;
+  const int ly = c.pix >= 0 ? c.pix / tw : 0, lx = c.pix >= 0 ? c.pix % tw : 0;
+  c.x = c.x0 + lx;
+  c.y = c.y0 + ly;
+  c.in_image = c.pix >= 0 && c.x < width && c.y < height;
   return c;
 }
 
-__device__ __forceinline__ void stage_chunk(const mt_composite_args& a, int cam, int64_t first,
-                                            int count, Staged* sg, StagedColor* sc, int32_t* sf,
-                                            bool want_color) {
-  const float2* xy = reinterpret_cast<const float2*>(a.xy) + int64_t(cam) * a.num_vertices;
-  const float* col = a.colors + int64_t(cam) * a.num_vertices * 3;
+// Stage keys [first, first + count) of one tile: terms, candidate rectangle, face; clear
+// the hit masks. Returns nothing; callers barrier afterwards.
+__device__ __forceinline__ void stage_chunk(const ChunkLds& L, const float2* xy,
+                                            const int32_t* faces, const int32_t* values,
+                                            const int64_t* keys, int shell_offset,
+                                            int shell_bits, int64_t first, int count, int x0,
+                                            int y0, int tw, int th) {
   for (int j = threadIdx.x; j < count; j += blockDim.x) {
-    const int64_t key_index = first + j;
-    const int32_t face = a.sorted_values[key_index];
-    const uint64_t key = static_cast<uint64_t>(a.sorted_keys[key_index]);
-    const int shell = static_cast<int>(key_field(key, a.layout.shell_offset, a.layout.shell_bits));
-    const int64_t va = a.faces[3 * int64_t(face) + 0];
-    const int64_t vb = a.faces[3 * int64_t(face) + 1];
-    const int64_t vc = a.faces[3 * int64_t(face) + 2];
-    sg[j] = stage_geometry(xy[va], xy[vb], xy[vc], shell);
-    sf[j] = face;
-    if (want_color) {
-      sc[j].a = make_float4(col[3 * va + 0], col[3 * va + 1], col[3 * va + 2], a.sdf[va]);
-      sc[j].b = make_float4(col[3 * vb + 0], col[3 * vb + 1], col[3 * vb + 2], a.sdf[vb]);
-      sc[j].c = make_float4(col[3 * vc + 0], col[3 * vc + 1], col[3 * vc + 2], a.sdf[vc]);
+    const int32_t face = values[first + j];
+    const int shell =
+        keys ? static_cast<int>(key_field(static_cast<uint64_t>(keys[first + j]), shell_offset,
+                                          shell_bits))
+             : 0;
+    const int64_t va = faces[3 * int64_t(face) + 0];
+    const int64_t vb = faces[3 * int64_t(face) + 1];
+    const int64_t vc = faces[3 * int64_t(face) + 2];
+    const float2 p1 = xy[va], p2 = xy[vb], p3 = xy[vc];
+    const TriTerms t = tri_terms(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
+    const Rect r = candidate_rect(t, x0, y0, tw, th);
+    L.geo[j] = to_staged(t, shell);
+    L.rect[j] = make_uint2(uint32_t(r.x) | (uint32_t(r.w) << 16), uint32_t(r.y) | (uint32_t(r.h) << 16));
+    L.face[j] = face;
+  }
+  const int mask_words = L.words * L.npix;
+  for (int i = threadIdx.x; i < mask_words; i += blockDim.x) L.mask[i] = 0u;
+}
+
+// Exclusive prefix of the staged rectangle areas into L.prefix[0..count]; returns the
+// number of candidate pairs. Each thread scans a contiguous run of entries.
+__device__ __forceinline__ int scan_areas(const ChunkLds& L, int count, int* s_wave) {
+  const int per = (count + int(blockDim.x) - 1) / int(blockDim.x);
+  const int begin = int(threadIdx.x) * per;
+  int local = 0;
+  for (int k = 0; k < per; ++k) {
+    const int j = begin + k;
+    if (j < count) {
+      const uint2 rc = L.rect[j];
+      local += int(rc.x >> 16) * int(rc.y >> 16);
     }
   }
+  int total;
+  int run = block_exclusive_sum(local, s_wave, &total);
+  for (int k = 0; k < per; ++k) {
+    const int j = begin + k;
+    if (j < count) {
+      L.prefix[j] = run;
+      const uint2 rc = L.rect[j];
+      run += int(rc.x >> 16) * int(rc.y >> 16);
+    }
+  }
+  if (threadIdx.x == 0) L.prefix[count] = total;
+  return total;
+}
+
+// Test every candidate pair of the chunk; set mask bits of accepted hits. Pairs of pixels
+// that no longer composite are skipped. Returns the number of tests run by this thread.
+__device__ __forceinline__ uint32_t test_pairs(const ChunkLds& L, int count, int total, int x0,
+                                               int y0, int tw) {
+  uint32_t tests = 0;
+  for (int q = threadIdx.x; q < total; q += blockDim.x) {
+    int lo = 0, hi = count - 1;  // last entry whose prefix is <= q (it has a nonzero area)
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (L.prefix[mid] <= q) lo = mid; else hi = mid - 1;
+    }
+    const uint2 rc = L.rect[lo];
+    const int w = int(rc.x >> 16);
+    const int local = q - L.prefix[lo];
+    const int lx = int(rc.x & 0xFFFFu) + local % w;
+    const int ly = int(rc.y & 0xFFFFu) + local / w;
+    const int pix = ly * tw + lx;
+    if (!L.live[pix]) continue;
+    ++tests;
+    const Bary bc = bary_test(L.geo[lo], float(x0 + lx) + 0.5f, float(y0 + ly) + 0.5f);
+    if (bc.hit) atomicOr(&L.mask[(lo >> 5) * L.npix + pix], 1u << (lo & 31));
+  }
+  return tests;
 }
 
 // Block-reduced statistics counters (one atomic pair per block).
 __device__ void add_stats(unsigned long long* stats, uint32_t tests, uint32_t hits) {
-  __shared__ unsigned long long s_t[1024 / kWave], s_h[1024 / kWave];
+  __shared__ unsigned long long s_t[kMaxTilePixels / kWave], s_h[kMaxTilePixels / kWave];
   unsigned long long t = wave_sum<unsigned long long>(tests);
   unsigned long long h = wave_sum<unsigned long long>(hits);
   if (__lane_id() == 0) {
@@ -191,7 +285,7 @@ __device__ void add_stats(unsigned long long* stats, uint32_t tests, uint32_t hi
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long st = 0, sh = 0;
-    for (int w = 0; w < (int(blockDim.x) + kWave - 1) / kWave; ++w) {
+    for (int w = 0; w < int(blockDim.x) / kWave; ++w) {
       st += s_t[w];
       sh += s_h[w];
     }
@@ -200,58 +294,98 @@ __device__ void add_stats(unsigned long long* stats, uint32_t tests, uint32_t hi
   }
 }
 
-__global__ void __launch_bounds__(1024) fused_forward_kernel(mt_composite_args a, float* out) {
-  const StageBuffers sb = carve_stage(g_dynamic_lds, a.chunk);
-  Staged* s_geo = sb.geo;
-  StagedColor* s_col = sb.col;
-  int32_t* s_face = sb.face;
-  const TileCoord tc = tile_coord(a);
+struct Corner {
+  float r, g, b, sdf;
+};
+
+// Barycentric interpolation of colour and SDF of one hit (intersect_and_composite.slang:
+// 150-158); the three corners are read from global memory (hits are rare per key).
+struct Sample {
+  float r, g, b, sdf;
+  int64_t v[3];
+};
+
+__device__ __forceinline__ Sample interpolate_hit(const mt_composite_args& a, const float* col,
+                                                  int32_t face, const Bary& bc) {
+  Sample s;
+  s.v[0] = a.faces[3 * int64_t(face) + 0];
+  s.v[1] = a.faces[3 * int64_t(face) + 1];
+  s.v[2] = a.faces[3 * int64_t(face) + 2];
+  const float* ca = col + 3 * s.v[0];
+  const float* cb = col + 3 * s.v[1];
+  const float* cc = col + 3 * s.v[2];
+  s.sdf = (bc.u * a.sdf[s.v[0]] + bc.v * a.sdf[s.v[1]]) + bc.w * a.sdf[s.v[2]];
+  s.r = (bc.u * ca[0] + bc.v * cb[0]) + bc.w * cc[0];
+  s.g = (bc.u * ca[1] + bc.v * cb[1]) + bc.w * cc[1];
+  s.b = (bc.u * ca[2] + bc.v * cb[2]) + bc.w * cc[2];
+  return s;
+}
+
+// ------------------------------------------------------------------------------------
+// Fused forward (intersect_and_composite.slang:224-339)
+// ------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(512) fused_forward_kernel(mt_composite_args a, float* out) {
+  const int npix = a.tile_h * a.tile_w;
+  const ChunkLds L = carve_chunk(a.chunk, npix);
+  __shared__ int s_wave[kMaxTilePixels / kWave];
   const int T = a.grid_rows * a.grid_cols;
+  const TileCoord tc = tile_coord(int(xcd_swizzle(blockIdx.x, gridDim.x)), a.grid_cols, T,
+                                  a.tile_h, a.tile_w, a.height, a.width);
   const int64_t start = a.tile_ranges[2 * (int64_t(tc.cam) * T + tc.tile) + 0];
   const int64_t end = a.tile_ranges[2 * (int64_t(tc.cam) * T + tc.tile) + 1];
   const float k = a.sharpness[0];
   const float px = float(tc.x) + 0.5f, py = float(tc.y) + 0.5f;
-  const float dummy_cdf = sdf_to_cdf(kDummySdf, k);
+  const float2* xy = reinterpret_cast<const float2*>(a.xy) + int64_t(tc.cam) * a.num_vertices;
+  const float* col = a.colors + int64_t(tc.cam) * a.num_vertices * 3;
 
   float r = 0.f, g = 0.f, bl = 0.f, trans = 1.f;
-  float outer_cdf = dummy_cdf;
+  float outer_cdf = sdf_to_cdf(kDummySdf, k);
   int last_shell = -1;
   bool active = tc.in_image;
-  const int chunk = a.chunk;
+  if (tc.pix >= 0) L.live[tc.pix] = active ? 1 : 0;
   uint32_t n_tests = 0, n_hits = 0;
+  const int chunk = a.chunk;
   for (int64_t first = start; first < end; first += chunk) {
     const int count = static_cast<int>((end - first) < chunk ? (end - first) : chunk);
     __syncthreads();
-    stage_chunk(a, tc.cam, first, count, s_geo, s_col, s_face, true);
+    stage_chunk(L, xy, a.faces, a.sorted_values, a.sorted_keys, a.layout.shell_offset,
+                a.layout.shell_bits, first, count, tc.x0, tc.y0, a.tile_w, a.tile_h);
+    __syncthreads();
+    const int total = scan_areas(L, count, s_wave);
+    __syncthreads();
+    n_tests += test_pairs(L, count, total, tc.x0, tc.y0, a.tile_w);
     __syncthreads();
     if (active) {
-      for (int i = 0; i < count; ++i) {
-        const Staged st = s_geo[i];
-        const int shell = shell_of(st);
-        if (shell <= last_shell) continue;
-        ++n_tests;
-        const Bary bc = bary_test(st, px, py);
-        if (!bc.hit) continue;
-        ++n_hits;
-        const StagedColor c = s_col[i];
-        const float sdf = (bc.u * c.a.w + bc.v * c.b.w) + bc.w * c.c.w;
-        const float cr = (bc.u * c.a.x + bc.v * c.b.x) + bc.w * c.c.x;
-        const float cg = (bc.u * c.a.y + bc.v * c.b.y) + bc.w * c.c.y;
-        const float cb = (bc.u * c.a.z + bc.v * c.b.z) + bc.w * c.c.z;
-        const float inner_cdf = sdf_to_cdf(sdf, k);
-        float alpha = ((outer_cdf - inner_cdf) + kAlphaEps) / (outer_cdf + kAlphaEps);
-        alpha = fminf(fmaxf(alpha, 0.0f), 1.0f);
-        r = r + (cr * trans) * alpha;
-        g = g + (cg * trans) * alpha;
-        bl = bl + (cb * trans) * alpha;
-        trans = trans * (1.0f - alpha);
-        outer_cdf = inner_cdf;
-        last_shell = shell;
-        if (shell == a.num_shells - 1) {
-          active = false;
-          break;
+      const int words = (count + 31) / 32;
+      for (int wd = 0; wd < words && active; ++wd) {
+        uint32_t bits = L.mask[wd * npix + tc.pix];
+        while (bits) {
+          const int i = wd * 32 + __builtin_ctz(bits);
+          bits &= bits - 1u;
+          const Staged st = L.geo[i];
+          const int shell = shell_of(st);
+          if (shell <= last_shell) continue;
+          const Bary bc = bary_test(st, px, py);
+          if (!bc.hit) continue;
+          ++n_hits;
+          const Sample sm = interpolate_hit(a, col, L.face[i], bc);
+          const float inner_cdf = sdf_to_cdf(sm.sdf, k);
+          float alpha = ((outer_cdf - inner_cdf) + kAlphaEps) / (outer_cdf + kAlphaEps);
+          alpha = fminf(fmaxf(alpha, 0.0f), 1.0f);
+          r = r + (sm.r * trans) * alpha;
+          g = g + (sm.g * trans) * alpha;
+          bl = bl + (sm.b * trans) * alpha;
+          trans = trans * (1.0f - alpha);
+          outer_cdf = inner_cdf;
+          last_shell = shell;
+          if (shell == a.num_shells - 1) {
+            active = false;
+            break;
+          }
         }
       }
+      L.live[tc.pix] = active ? 1 : 0;
     }
     if (!__syncthreads_or(active)) break;
   }
@@ -267,12 +401,10 @@ __global__ void __launch_bounds__(1024) fused_forward_kernel(mt_composite_args a
 }
 
 // Scatter one sample's gradient (colour and SDF) to its triangle's corners.
-__device__ __forceinline__ void scatter_sample(const mt_composite_args& a, int cam, int32_t face,
-                                               float bu, float bv, float bw, float dr, float dg,
-                                               float db, float dsdf, float* colors_grad,
-                                               float* sdf_grad) {
-  const int64_t v[3] = {a.faces[3 * int64_t(face) + 0], a.faces[3 * int64_t(face) + 1],
-                        a.faces[3 * int64_t(face) + 2]};
+__device__ __forceinline__ void scatter_sample(const mt_composite_args& a, int cam,
+                                               const int64_t* v, float bu, float bv, float bw,
+                                               float dr, float dg, float db, float dsdf,
+                                               float* colors_grad, float* sdf_grad) {
   const float wgt[3] = {bu, bv, bw};
   float* cgrad = colors_grad + int64_t(cam) * a.num_vertices * 3;
 #pragma unroll
@@ -284,22 +416,28 @@ __device__ __forceinline__ void scatter_sample(const mt_composite_args& a, int c
   }
 }
 
-__global__ void __launch_bounds__(1024) fused_backward_kernel(mt_composite_args a, const float* out,
+// ------------------------------------------------------------------------------------
+// Fused backward (intersect_and_composite.slang:341-564): the same traversal, recomputed,
+// with the suffix trick for the accumulated colour.
+// ------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a, const float* out,
                                                               const float* gout, float* colors_grad,
                                                               float* sdf_grad, float* sharp_grad,
                                                               int quirk) {
-  const StageBuffers sb = carve_stage(g_dynamic_lds, a.chunk);
-  Staged* s_geo = sb.geo;
-  StagedColor* s_col = sb.col;
-  int32_t* s_face = sb.face;
-  __shared__ float s_red[1024 / kWave];
-  const TileCoord tc = tile_coord(a);
+  const int npix = a.tile_h * a.tile_w;
+  const ChunkLds L = carve_chunk(a.chunk, npix);
+  __shared__ int s_wave[kMaxTilePixels / kWave];
+  __shared__ float s_red[kMaxTilePixels / kWave];
   const int T = a.grid_rows * a.grid_cols;
+  const TileCoord tc = tile_coord(int(xcd_swizzle(blockIdx.x, gridDim.x)), a.grid_cols, T,
+                                  a.tile_h, a.tile_w, a.height, a.width);
   const int64_t start = a.tile_ranges[2 * (int64_t(tc.cam) * T + tc.tile) + 0];
   const int64_t end = a.tile_ranges[2 * (int64_t(tc.cam) * T + tc.tile) + 1];
   const float k = a.sharpness[0];
   const float px = float(tc.x) + 0.5f, py = float(tc.y) + 0.5f;
-  const float dummy_cdf = sdf_to_cdf(kDummySdf, k);
+  const float2* xy = reinterpret_cast<const float2*>(a.xy) + int64_t(tc.cam) * a.num_vertices;
+  const float* col = a.colors + int64_t(tc.cam) * a.num_vertices * 3;
 
   float fr = 0.f, fg = 0.f, fb = 0.f, ft = 0.f;    // final accumulator
   float gr = 0.f, gg = 0.f, gb = 0.f, gt = 0.f;    // its gradient
@@ -316,88 +454,100 @@ __global__ void __launch_bounds__(1024) fused_backward_kernel(mt_composite_args 
     gt = gout[o + 3 * plane];
   }
   float ar = 0.f, ag = 0.f, ab = 0.f, trans = 1.f;
-  float outer_cdf = dummy_cdf, outer_sdf = kDummySdf;
+  float outer_cdf = sdf_to_cdf(kDummySdf, k), outer_sdf = kDummySdf;
   int last_shell = -1;
   bool active = tc.in_image;
+  if (tc.pix >= 0) L.live[tc.pix] = active ? 1 : 0;
   float dk = 0.f;
-  // Sample waiting for its outer-role SDF gradient (set by the next accepted hit).
   uint32_t n_tests = 0, n_hits = 0;
+  // Sample waiting for its outer-role SDF gradient (set by the next accepted hit).
   bool pending = false;
-  int32_t p_face = 0;
-  float p_u = 0.f, p_v = 0.f, p_w = 0.f, p_dr = 0.f, p_dg = 0.f, p_db = 0.f, p_dsdf = 0.f;
+  int64_t p_v[3] = {0, 0, 0};
+  float p_u = 0.f, p_v_ = 0.f, p_w = 0.f, p_dr = 0.f, p_dg = 0.f, p_db = 0.f, p_dsdf = 0.f;
 
   const int chunk = a.chunk;
   for (int64_t first = start; first < end; first += chunk) {
     const int count = static_cast<int>((end - first) < chunk ? (end - first) : chunk);
     __syncthreads();
-    stage_chunk(a, tc.cam, first, count, s_geo, s_col, s_face, true);
+    stage_chunk(L, xy, a.faces, a.sorted_values, a.sorted_keys, a.layout.shell_offset,
+                a.layout.shell_bits, first, count, tc.x0, tc.y0, a.tile_w, a.tile_h);
+    __syncthreads();
+    const int total = scan_areas(L, count, s_wave);
+    __syncthreads();
+    n_tests += test_pairs(L, count, total, tc.x0, tc.y0, a.tile_w);
     __syncthreads();
     if (active) {
-      for (int i = 0; i < count; ++i) {
-        const Staged st = s_geo[i];
-        const int shell = shell_of(st);
-        if (shell <= last_shell) continue;
-        ++n_tests;
-        const Bary bc = bary_test(st, px, py);
-        if (!bc.hit) continue;
-        ++n_hits;
-        const StagedColor c = s_col[i];
-        const float sdf = (bc.u * c.a.w + bc.v * c.b.w) + bc.w * c.c.w;
-        const float cr = (bc.u * c.a.x + bc.v * c.b.x) + bc.w * c.c.x;
-        const float cg = (bc.u * c.a.y + bc.v * c.b.y) + bc.w * c.c.y;
-        const float cb = (bc.u * c.a.z + bc.v * c.b.z) + bc.w * c.c.z;
-        const float ci = sdf_to_cdf(sdf, k);
-        const float co = outer_cdf;
-        const float den = co + kAlphaEps;
-        const float x = ((co - ci) + kAlphaEps) / den;
-        const float alpha = fminf(fmaxf(x, 0.0f), 1.0f);
-        // Suffix trick (slang:460-473).
-        const float weight = alpha * trans;
-        ar += weight * cr;
-        ag += weight * cg;
-        ab += weight * cb;
-        const float inv = 1.0f / (1.0f - alpha + kSuffixEps);
-        const float d_alpha = gr * (trans * cr - (fr - ar) * inv) + gg * (trans * cg - (fg - ag) * inv) +
-                              gb * (trans * cb - (fb - ab) * inv) - gt * ft * inv;
-        float d_in = 0.f, d_out = 0.f;
-        if (x > 0.0f && x < 1.0f) {
-          const float dci = ci * (1.0f - ci);
-          const float dco = co * (1.0f - co);
-          const float dx_di = -1.0f / den;
-          const float dx_do = ci / (den * den);
-          d_in = d_alpha * dx_di * k * dci;
-          d_out = d_alpha * dx_do * k * dco;
-          dk += d_alpha * (dx_di * sdf * dci + dx_do * outer_sdf * dco);
-        }
-        if (pending) {
-          p_dsdf += d_out;
-          const bool q = quirk != 0;
-          scatter_sample(a, tc.cam, p_face, q ? bc.u : p_u, q ? bc.v : p_v, q ? bc.w : p_w, p_dr,
-                         p_dg, p_db, p_dsdf, colors_grad, sdf_grad);
-        }
-        pending = true;
-        p_face = s_face[i];
-        p_u = bc.u;
-        p_v = bc.v;
-        p_w = bc.w;
-        p_dr = weight * gr;
-        p_dg = weight * gg;
-        p_db = weight * gb;
-        p_dsdf = d_in;
-        trans *= 1.0f - alpha;
-        outer_cdf = ci;
-        outer_sdf = sdf;
-        last_shell = shell;
-        if (shell == a.num_shells - 1) {
-          active = false;
-          break;
+      const int words = (count + 31) / 32;
+      for (int wd = 0; wd < words && active; ++wd) {
+        uint32_t bits = L.mask[wd * npix + tc.pix];
+        while (bits) {
+          const int i = wd * 32 + __builtin_ctz(bits);
+          bits &= bits - 1u;
+          const Staged st = L.geo[i];
+          const int shell = shell_of(st);
+          if (shell <= last_shell) continue;
+          const Bary bc = bary_test(st, px, py);
+          if (!bc.hit) continue;
+          ++n_hits;
+          const Sample sm = interpolate_hit(a, col, L.face[i], bc);
+          const float sdf = sm.sdf;
+          const float ci = sdf_to_cdf(sdf, k);
+          const float co = outer_cdf;
+          const float den = co + kAlphaEps;
+          const float x = ((co - ci) + kAlphaEps) / den;
+          const float alpha = fminf(fmaxf(x, 0.0f), 1.0f);
+          // Suffix trick (slang:460-473).
+          const float weight = alpha * trans;
+          ar += weight * sm.r;
+          ag += weight * sm.g;
+          ab += weight * sm.b;
+          const float inv = 1.0f / (1.0f - alpha + kSuffixEps);
+          const float d_alpha = gr * (trans * sm.r - (fr - ar) * inv) +
+                                gg * (trans * sm.g - (fg - ag) * inv) +
+                                gb * (trans * sm.b - (fb - ab) * inv) - gt * ft * inv;
+          float d_in = 0.f, d_out = 0.f;
+          if (x > 0.0f && x < 1.0f) {
+            const float dci = ci * (1.0f - ci);
+            const float dco = co * (1.0f - co);
+            const float dx_di = -1.0f / den;
+            const float dx_do = ci / (den * den);
+            d_in = d_alpha * dx_di * k * dci;
+            d_out = d_alpha * dx_do * k * dco;
+            dk += d_alpha * (dx_di * sdf * dci + dx_do * outer_sdf * dco);
+          }
+          if (pending) {
+            p_dsdf += d_out;
+            const bool q = quirk != 0;
+            scatter_sample(a, tc.cam, p_v, q ? bc.u : p_u, q ? bc.v : p_v_, q ? bc.w : p_w, p_dr,
+                           p_dg, p_db, p_dsdf, colors_grad, sdf_grad);
+          }
+          pending = true;
+          p_v[0] = sm.v[0];
+          p_v[1] = sm.v[1];
+          p_v[2] = sm.v[2];
+          p_u = bc.u;
+          p_v_ = bc.v;
+          p_w = bc.w;
+          p_dr = weight * gr;
+          p_dg = weight * gg;
+          p_db = weight * gb;
+          p_dsdf = d_in;
+          trans *= 1.0f - alpha;
+          outer_cdf = ci;
+          outer_sdf = sdf;
+          last_shell = shell;
+          if (shell == a.num_shells - 1) {
+            active = false;
+            break;
+          }
         }
       }
+      L.live[tc.pix] = active ? 1 : 0;
     }
     if (!__syncthreads_or(active)) break;
   }
   if (pending) {
-    scatter_sample(a, tc.cam, p_face, p_u, p_v, p_w, p_dr, p_dg, p_db, p_dsdf, colors_grad,
+    scatter_sample(a, tc.cam, p_v, p_u, p_v_, p_w, p_dr, p_dg, p_db, p_dsdf, colors_grad,
                    sdf_grad);
   }
   // Sharpness gradient: block reduction, one atomic per block.
@@ -406,66 +556,72 @@ __global__ void __launch_bounds__(1024) fused_backward_kernel(mt_composite_args 
   __syncthreads();
   if (threadIdx.x == 0) {
     float t = 0.f;
-    for (int w = 0; w < (int(blockDim.x) + kWave - 1) / kWave; ++w) t += s_red[w];
+    for (int w = 0; w < int(blockDim.x) / kWave; ++w) t += s_red[w];
     if (t != 0.0f) atomicAdd(sharp_grad, t);
   }
   if (a.stats) add_stats(a.stats, n_tests, n_hits);
 }
 
 // ------------------------------------------------------------------------------------
-// Explicit mode
+// Explicit mode: first hit per (camera, shell, pixel) (intersect.slang:157-252)
 // ------------------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(1024) intersect_kernel(const int32_t* values, const int64_t* ranges,
+constexpr int kIntersectChunk = 256;
+
+__global__ void __launch_bounds__(512) intersect_kernel(const int32_t* values, const int64_t* ranges,
                                                          const float* xy, int64_t V,
                                                          const int32_t* faces, int B, int S,
                                                          int grid_rows, int grid_cols, int th,
                                                          int tw, int H, int W, int chunk,
                                                          float* out_uv, int32_t* out_index) {
-  __shared__ Staged s_geo[kMaxIntersectChunk];
-  __shared__ int32_t s_face[kMaxIntersectChunk];
+  const int npix = th * tw;
+  const ChunkLds L = carve_chunk(chunk, npix);
+  __shared__ int s_wave[kMaxTilePixels / kWave];
   const int T = grid_rows * grid_cols;
-  const uint32_t bid = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int tile = bid % T;
-  const int bs = bid / T;  // camera * S + shell
+  // Block = (camera * S + shell, tile).
+  const TileCoord tc = tile_coord(int(xcd_swizzle(blockIdx.x, gridDim.x)), grid_cols, T, th, tw,
+                                  H, W);
+  const int bs = tc.cam;
   const int cam = bs / S;
-  const int ty = tile / grid_cols, tx = tile % grid_cols;
-  const int x = tx * tw + threadIdx.x % tw;
-  const int y = ty * th + threadIdx.x / tw;
-  const bool in_image = x < W && y < H;
-  const int64_t start = ranges[2 * (int64_t(bs) * T + tile) + 0];
-  const int64_t end = ranges[2 * (int64_t(bs) * T + tile) + 1];
-  const float px = float(x) + 0.5f, py = float(y) + 0.5f;
+  const int64_t start = ranges[2 * (int64_t(bs) * T + tc.tile) + 0];
+  const int64_t end = ranges[2 * (int64_t(bs) * T + tc.tile) + 1];
+  const float px = float(tc.x) + 0.5f, py = float(tc.y) + 0.5f;
   const float2* pxy = reinterpret_cast<const float2*>(xy) + int64_t(cam) * V;
-  bool active = in_image;
+  bool active = tc.in_image;
+  if (tc.pix >= 0) L.live[tc.pix] = active ? 1 : 0;
   float hu = 0.f, hv = 0.f;
   int32_t hidx = -1;
   for (int64_t first = start; first < end; first += chunk) {
     const int count = static_cast<int>((end - first) < chunk ? (end - first) : chunk);
     __syncthreads();
-    for (int j = threadIdx.x; j < count; j += blockDim.x) {
-      const int32_t face = values[first + j];
-      const int64_t va = faces[3 * int64_t(face)], vb = faces[3 * int64_t(face) + 1],
-                    vc = faces[3 * int64_t(face) + 2];
-      s_geo[j] = stage_geometry(pxy[va], pxy[vb], pxy[vc], 0);
-      s_face[j] = face;
-    }
+    stage_chunk(L, pxy, faces, values, nullptr, 0, 0, first, count, tc.x0, tc.y0, tw, th);
+    __syncthreads();
+    const int total = scan_areas(L, count, s_wave);
+    __syncthreads();
+    test_pairs(L, count, total, tc.x0, tc.y0, tw);
     __syncthreads();
     if (active) {
-      for (int i = 0; i < count; ++i) {
-        const Bary bc = bary_test(s_geo[i], px, py);
-        if (!bc.hit) continue;
-        hu = bc.u;
-        hv = bc.v;
-        hidx = s_face[i];
-        active = false;
-        break;
+      const int words = (count + 31) / 32;
+      for (int wd = 0; wd < words && active; ++wd) {
+        uint32_t bits = L.mask[wd * npix + tc.pix];
+        while (bits) {
+          const int i = wd * 32 + __builtin_ctz(bits);
+          bits &= bits - 1u;
+          const Bary bc = bary_test(L.geo[i], px, py);
+          if (!bc.hit) continue;
+          hu = bc.u;
+          hv = bc.v;
+          hidx = L.face[i];
+          active = false;
+          break;
+        }
       }
+      L.live[tc.pix] = active ? 1 : 0;
     }
     if (!__syncthreads_or(active)) break;
   }
-  if (in_image) {
-    const int64_t p = (int64_t(bs) * H + y) * W + x;
+  if (tc.in_image) {
+    const int64_t p = (int64_t(bs) * H + tc.y) * W + tc.x;
     reinterpret_cast<float2*>(out_uv)[p] = make_float2(hu, hv);
     out_index[p] = hidx;
   }
@@ -651,8 +807,9 @@ __global__ void __launch_bounds__(256) image_l2_loss_kernel(const float* out, co
 
 int check_composite_args(const mt_composite_args* a, const char* name) {
   MT_REQUIRE(a != nullptr, "%s: null args", name);
-  MT_REQUIRE(a->tile_h > 0 && a->tile_w > 0 && a->tile_h * a->tile_w <= 1024,
-             "%s: tile of %dx%d pixels must hold 1..1024 pixels", name, a->tile_h, a->tile_w);
+  MT_REQUIRE(a->tile_h > 0 && a->tile_w > 0 && a->tile_h * a->tile_w <= kMaxTilePixels,
+             "%s: tile of %dx%d pixels must hold 1..%d pixels", name, a->tile_h, a->tile_w,
+             kMaxTilePixels);
   MT_REQUIRE(a->chunk >= 0 && a->chunk <= kMaxChunk, "%s: chunk must be in [0, %d]", name, kMaxChunk);
   MT_REQUIRE(a->num_cameras >= 0 && a->grid_rows >= 0 && a->grid_cols >= 0, "%s: bad grid", name);
   MT_REQUIRE(int64_t(a->num_cameras) * a->grid_rows * a->grid_cols < (int64_t(1) << 31),
@@ -663,22 +820,28 @@ int check_composite_args(const mt_composite_args* a, const char* name) {
 }  // namespace
 }  // namespace mt
 
-static int resolve_chunk(const mt_composite_args* a) {
-  const int pix = a->tile_h * a->tile_w;
-  int c = a->chunk > 0 ? a->chunk : pix;
-  return c > mt::kMaxChunk ? mt::kMaxChunk : c;
+// Chunk actually staged: the requested one (0 = one key per pixel), at most kMaxChunk and
+// small enough that the stage buffers and hit masks fit the LDS budget.
+static int resolve_chunk(int requested, int npix) {
+  int c = requested > 0 ? requested : npix;
+  if (c > mt::kMaxChunk) c = mt::kMaxChunk;
+  while (c > 32 && mt::chunk_lds_bytes(c, npix) > size_t(mt::kMaxLds)) c -= 32;
+  return c;
 }
+
+static unsigned block_threads(int npix) { return static_cast<unsigned>((npix + 63) / 64 * 64); }
 
 extern "C" int mt_intersect_and_composite_forward(const mt_composite_args* args, float* out,
                                                   mt_stream_t stream) {
   if (int rc = mt::check_composite_args(args, "mt_intersect_and_composite_forward")) return rc;
   mt_composite_args a = *args;
-  a.chunk = resolve_chunk(args);
+  const int npix = a.tile_h * a.tile_w;
+  a.chunk = resolve_chunk(args->chunk, npix);
   const int64_t blocks = int64_t(a.num_cameras) * a.grid_rows * a.grid_cols;
   if (blocks == 0) return MT_OK;
-  mt::fused_forward_kernel<<<static_cast<unsigned>(blocks), a.tile_h * a.tile_w,
-                             static_cast<size_t>(a.chunk) * mt::kStagedBytes,
-                             static_cast<hipStream_t>(stream)>>>(a, out);
+  mt::fused_forward_kernel<<<static_cast<unsigned>(blocks), block_threads(npix),
+                             mt::chunk_lds_bytes(a.chunk, npix), static_cast<hipStream_t>(stream)>>>(
+      a, out);
   return mt::check_launch("mt_intersect_and_composite_forward");
 }
 
@@ -688,12 +851,12 @@ extern "C" int mt_intersect_and_composite_backward(const mt_composite_args* args
                                                    int32_t reference_quirk, mt_stream_t stream) {
   if (int rc = mt::check_composite_args(args, "mt_intersect_and_composite_backward")) return rc;
   mt_composite_args a = *args;
-  a.chunk = resolve_chunk(args);
+  const int npix = a.tile_h * a.tile_w;
+  a.chunk = resolve_chunk(args->chunk, npix);
   const int64_t blocks = int64_t(a.num_cameras) * a.grid_rows * a.grid_cols;
   if (blocks == 0) return MT_OK;
-  mt::fused_backward_kernel<<<static_cast<unsigned>(blocks), a.tile_h * a.tile_w,
-                              static_cast<size_t>(a.chunk) * mt::kStagedBytes,
-                              static_cast<hipStream_t>(stream)>>>(
+  mt::fused_backward_kernel<<<static_cast<unsigned>(blocks), block_threads(npix),
+                              mt::chunk_lds_bytes(a.chunk, npix), static_cast<hipStream_t>(stream)>>>(
       a, out, grad_out, colors_grad, sdf_grad, sharpness_grad, reference_quirk);
   return mt::check_launch("mt_intersect_and_composite_backward");
 }
@@ -714,14 +877,16 @@ extern "C" int mt_intersect(const int32_t* sorted_values, const int64_t* tile_ra
                             int32_t grid_rows, int32_t grid_cols, int32_t tile_h, int32_t tile_w,
                             int32_t height, int32_t width, float* out_uv, int32_t* out_index,
                             mt_stream_t stream) {
-  MT_REQUIRE(tile_h > 0 && tile_w > 0 && tile_h * tile_w <= 1024, "mt_intersect: bad tile shape");
+  MT_REQUIRE(tile_h > 0 && tile_w > 0 && tile_h * tile_w <= mt::kMaxTilePixels,
+             "mt_intersect: tile of %dx%d pixels must hold 1..%d pixels", tile_h, tile_w,
+             mt::kMaxTilePixels);
   const int64_t blocks = int64_t(B) * S * grid_rows * grid_cols;
   MT_REQUIRE(blocks < (int64_t(1) << 31), "mt_intersect: too many tiles");
   if (blocks == 0) return MT_OK;
-  int chunk = tile_h * tile_w;
-  if (chunk > mt::kMaxIntersectChunk) chunk = mt::kMaxIntersectChunk;
-  mt::intersect_kernel<<<static_cast<unsigned>(blocks), tile_h * tile_w, 0,
-                         static_cast<hipStream_t>(stream)>>>(sorted_values, tile_ranges, xy, V,
+  const int npix = tile_h * tile_w;
+  const int chunk = resolve_chunk(mt::kIntersectChunk, npix);
+  mt::intersect_kernel<<<static_cast<unsigned>(blocks), block_threads(npix),
+                         mt::chunk_lds_bytes(chunk, npix), static_cast<hipStream_t>(stream)>>>(sorted_values, tile_ranges, xy, V,
                                                               faces, B, S, grid_rows, grid_cols,
                                                               tile_h, tile_w, height, width, chunk,
                                                               out_uv, out_index);

@@ -3,7 +3,7 @@
 // Each block takes a tile of kTile = 256 x 16 elements in ticket order, scans it in
 // registers (16 consecutive items per thread, wave64 shuffles across threads), publishes
 // its aggregate, then one wave walks back over the predecessors' published aggregates /
-// inclusive prefixes 64 tiles at a time. Every element is read once and written once:
+// inclusive prefixes up to 64 tiles at a time. Every element is read once and written once:
 // 8 bytes per element for the int32 scan, 12 for int32 -> int64.
 #include "common.h"
 
@@ -14,50 +14,53 @@ constexpr int kBlock = 256;
 constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;
 
-enum : uint32_t { kNone = 0, kAggregate = 1, kPrefix = 2 };
+// Look-back word per tile: [63:62] state, [61:0] value as 62-bit two's complement. The
+// word carries its own value, so publishing and polling need only relaxed agent-scope
+// atomics: no release/acquire fences (which on gfx950 write back / invalidate caches) sit
+// on the serial chain between neighbouring tiles.
+constexpr uint64_t kNone = 0;
+constexpr uint64_t kAggregate = 1ull << 62;
+constexpr uint64_t kPrefix = 2ull << 62;
+constexpr uint64_t kStateMask = 3ull << 62;
+constexpr uint64_t kValueMask = (1ull << 62) - 1;
+// Predecessor words loaded at once per look-back round (independent loads in flight).
+constexpr int kWindow = 64;
 
-template <typename Out>
 struct ScanState {
   uint32_t* ticket;
-  uint32_t* flags;
-  Out* aggregate;
-  Out* prefix;
+  uint64_t* words;
 };
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-template <typename Out>
 size_t scan_state_bytes(int64_t n) {
   const int64_t tiles = (n + kTile - 1) / kTile;
-  return 256 + align_up(tiles * sizeof(uint32_t), 256) + 2 * align_up(tiles * sizeof(Out), 256);
+  return 256 + align_up(tiles * sizeof(uint64_t), 256);
 }
 
-template <typename Out>
-ScanState<Out> scan_state(void* ws, int64_t n) {
-  const int64_t tiles = (n + kTile - 1) / kTile;
+ScanState scan_state(void* ws) {
   char* p = static_cast<char*>(ws);
-  ScanState<Out> s;
+  ScanState s;
   s.ticket = reinterpret_cast<uint32_t*>(p);
-  p += 256;
-  s.flags = reinterpret_cast<uint32_t*>(p);
-  p += align_up(tiles * sizeof(uint32_t), 256);
-  s.aggregate = reinterpret_cast<Out*>(p);
-  p += align_up(tiles * sizeof(Out), 256);
-  s.prefix = reinterpret_cast<Out*>(p);
+  s.words = reinterpret_cast<uint64_t*>(p + 256);
   return s;
 }
 
-__global__ void reset_scan_state(uint32_t* ticket, uint32_t* flags, int64_t tiles) {
+__global__ void reset_scan_state(uint32_t* ticket, uint64_t* words, int64_t tiles) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i == 0) *ticket = 0;
-  if (i < tiles) flags[i] = kNone;
+  if (i < tiles) words[i] = kNone;
 }
 
-__device__ __forceinline__ uint32_t load_flag(const uint32_t* f) {
-  return __hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ uint64_t load_word(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void store_flag(uint32_t* f, uint32_t v) {
-  __hip_atomic_store(f, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void store_word(uint64_t* p, uint64_t state, int64_t value) {
+  __hip_atomic_store(p, state | (static_cast<uint64_t>(value) & kValueMask), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int64_t word_value(uint64_t w) {
+  return static_cast<int64_t>(w << 2) >> 2;  // sign-extend the 62-bit field
 }
 
 // Block-wide exclusive scan of one value per thread; returns the exclusive prefix and
@@ -80,30 +83,28 @@ __device__ __forceinline__ T block_exclusive(T x, T* s_wave, T* total) {
   return before + inc - x;
 }
 
-// Look-back by the first wave: returns the sum of all tiles before `tile`.
-template <typename Out>
-__device__ Out look_back(const ScanState<Out>& st, int64_t tile) {
+// Look-back by the first wave: returns the sum of all tiles before `tile`. Lane l polls
+// tile (base - l); the wave consumes the published words up to the nearest inclusive
+// prefix, or up to the first unpublished one, and polls again from there.
+__device__ int64_t look_back(const ScanState& st, int64_t tile) {
   const int lane = __lane_id();
-  Out running = 0;
+  int64_t running = 0;
   int64_t base = tile - 1;
   while (base >= 0) {
     const int64_t t = base - lane;
-    uint32_t f = kPrefix;  // lanes before tile 0 count as an empty inclusive prefix
-    Out v = 0;
-    if (t >= 0) {
-      do {
-        f = load_flag(&st.flags[t]);
-      } while (f == kNone);
-      v = (f == kPrefix) ? st.prefix[t] : st.aggregate[t];
-    }
-    const uint64_t has_prefix = __ballot(f == kPrefix);
-    // Lanes up to and including the nearest inclusive prefix contribute.
-    const int stop = has_prefix ? __ffsll(static_cast<long long>(has_prefix)) - 1 : kWave - 1;
-    Out contrib = (lane <= stop) ? v : Out(0);
-    contrib = wave_sum(contrib);
-    running += contrib;
-    if (has_prefix) break;
-    base -= kWave;
+    const uint64_t w = t >= 0 ? load_word(&st.words[t]) : kPrefix;  // before tile 0: empty prefix
+    const uint64_t state = w & kStateMask;
+    const uint64_t pending = __ballot(state == kNone);
+    const uint64_t has_prefix = __ballot(state == kPrefix);
+    // First lane (nearest predecessor) that is unpublished or an inclusive prefix.
+    const uint64_t stops = pending | has_prefix;
+    const int first = stops ? __ffsll(static_cast<long long>(stops)) - 1 : kWindow;
+    const bool take_first = first < kWindow && ((has_prefix >> first) & 1ull);
+    const int take = first + (take_first ? 1 : 0);  // lanes [0, take) are consumed
+    int64_t contrib = (lane < take && t >= 0) ? word_value(w) : 0;
+    running += wave_sum(contrib);
+    if (take_first) break;
+    base -= take;
   }
   return running;
 }
@@ -115,7 +116,7 @@ constexpr int kPadded = kTile + kTile / 16;
 
 template <typename In, typename Out, bool kInclusive>
 __global__ void __launch_bounds__(kBlock) scan_kernel(const In* in, Out* out, int64_t n,
-                                                      ScanState<Out> st) {
+                                                      ScanState st) {
   __shared__ union {
     In in[kPadded];
     Out out[kPadded];
@@ -144,24 +145,15 @@ __global__ void __launch_bounds__(kBlock) scan_kernel(const In* in, Out* out, in
   const Out thread_prefix = block_exclusive<Out>(local, s_wave, &block_total);
 
   if (threadIdx.x < kWave) {
-    Out prefix = 0;
+    int64_t prefix = 0;
     if (tile == 0) {
-      if (threadIdx.x == 0) {
-        st.prefix[0] = block_total;
-        store_flag(&st.flags[0], kPrefix);
-      }
+      if (threadIdx.x == 0) store_word(&st.words[0], kPrefix, block_total);
     } else {
-      if (threadIdx.x == 0) {
-        st.aggregate[tile] = block_total;
-        store_flag(&st.flags[tile], kAggregate);
-      }
+      if (threadIdx.x == 0) store_word(&st.words[tile], kAggregate, block_total);
       prefix = look_back(st, tile);
-      if (threadIdx.x == 0) {
-        st.prefix[tile] = prefix + block_total;
-        store_flag(&st.flags[tile], kPrefix);
-      }
+      if (threadIdx.x == 0) store_word(&st.words[tile], kPrefix, prefix + int64_t(block_total));
     }
-    if (threadIdx.x == 0) s_prefix = prefix;
+    if (threadIdx.x == 0) s_prefix = static_cast<Out>(prefix);
   }
   __syncthreads();  // also orders the LDS reads above before the writes below
   Out run = s_prefix + thread_prefix;
@@ -185,21 +177,21 @@ int run_scan(const In* in, Out* out, int64_t n, void* ws, size_t ws_bytes, hipSt
   MT_REQUIRE(n >= 0, "%s: negative length", name);
   if (n == 0) return MT_OK;
   MT_REQUIRE(in != nullptr && out != nullptr, "%s: null data pointer", name);
-  const size_t need = scan_state_bytes<Out>(n);
+  const size_t need = scan_state_bytes(n);
   if (ws == nullptr || ws_bytes < need) {
     set_error("%s: workspace of %zu bytes is smaller than the %zu required", name, ws_bytes, need);
     return MT_ERR_WORKSPACE_TOO_SMALL;
   }
   const int64_t tiles = (n + kTile - 1) / kTile;
-  ScanState<Out> st = scan_state<Out>(ws, n);
-  reset_scan_state<<<blocks_for(tiles, 256), 256, 0, stream>>>(st.ticket, st.flags, tiles);
+  ScanState st = scan_state(ws);
+  reset_scan_state<<<blocks_for(tiles, 256), 256, 0, stream>>>(st.ticket, st.words, tiles);
   scan_kernel<In, Out, kInclusive><<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(in, out, n, st);
   return check_launch(name);
 }
 
 }  // namespace
 
-size_t scan_workspace_bytes_i64(int64_t n) { return scan_state_bytes<int64_t>(n); }
+size_t scan_workspace_bytes_i64(int64_t n) { return scan_state_bytes(n); }
 
 int exclusive_scan_i32(int32_t* data, int64_t n, void* ws, size_t ws_bytes, hipStream_t stream) {
   return run_scan<int32_t, int32_t, false>(data, data, n, ws, ws_bytes, stream,

@@ -182,11 +182,13 @@ __global__ void __launch_bounds__(kBlock) prepare_kernel(SortWorkspace w, int64_
   }
 }
 
+// A look-back word carries its own count, so relaxed agent-scope atomics suffice: no
+// release/acquire fences (cache write-back / invalidation on gfx950) on the tile chain.
 __device__ __forceinline__ uint64_t lb_load(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void lb_store(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys,

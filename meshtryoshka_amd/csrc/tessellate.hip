@@ -58,49 +58,72 @@ __global__ void __launch_bounds__(kBlock) count_vertices_kernel(const float* sdf
   }
 }
 
+// A block owns 256 consecutive columns. For each level set their vertices occupy one
+// contiguous output range (the offsets are a scan in (level set, column) order), so the
+// block first lists its crossings in LDS in output order, then writes positions, SDF,
+// types and every SH plane of that range with consecutive threads on consecutive words.
 __global__ void __launch_bounds__(kBlock) create_vertices_kernel(
     const float* gv, const float* gsdf, const float* gsh, int64_t ng, int nsh, const int32_t* lower,
     int64_t n, const float* level_sets, int S, const int32_t* offsets, int64_t nt, float* ov,
     float* od, float* osh, uint8_t* otype, int32_t* overflow) {
-  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const ColumnSdf col = load_column(gsdf, lower, n, c);
-  const float x0 = gv[3 * int64_t(col.v0) + 0];
-  const float y0 = gv[3 * int64_t(col.v0) + 1];
-  const float z0 = gv[3 * int64_t(col.v0) + 2];
+  __shared__ int32_t s_v0[3 * kBlock], s_v1[3 * kBlock];
+  __shared__ float s_t[3 * kBlock], s_d0[3 * kBlock], s_d1[3 * kBlock];
+  __shared__ uint8_t s_axis[3 * kBlock];
+  __shared__ int s_total;
+  const int64_t c_first = int64_t(blockIdx.x) * kBlock;
+  const int64_t c = c_first + threadIdx.x;
+  const int64_t c_last = (c_first + kBlock < n ? c_first + kBlock : n) - 1;
+  const bool valid = c < n;
+  ColumnSdf col{};
+  if (valid) col = load_column(gsdf, lower, n, c);
   for (int s = 0; s < S; ++s) {
     const float iso = level_sets[s];
-    bool cr[3];
+    const int64_t base = offsets[int64_t(s) * n + c_first];
+    int k = 0;
+    if (valid) {
+      int pos = static_cast<int>(offsets[int64_t(s) * n + c] - base);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) cr[a] = crosses(col.d0, col.d[a], iso);
-    if (!(cr[0] || cr[1] || cr[2])) continue;
-    int64_t idx = offsets[s * n + c];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      if (!cr[a]) continue;
-      if (idx >= nt) {  // capacity exceeded: drop the vertex, report it
-        if (overflow) atomicOr(overflow, 1);
-        ++idx;
-        continue;
+      for (int a = 0; a < 3; ++a) {
+        if (!crosses(col.d0, col.d[a], iso)) continue;
+        s_v0[pos] = col.v0;
+        s_v1[pos] = col.v[a];
+        s_t[pos] = edge_t(col.d0, col.d[a], iso);
+        s_d0[pos] = col.d0;
+        s_d1[pos] = col.d[a];
+        s_axis[pos] = static_cast<uint8_t>(a);
+        ++pos;
+        ++k;
       }
-      const int64_t g1 = col.v[a];
-      const float d1 = col.d[a];
-      const float t = edge_t(col.d0, d1, iso);
+      if (c == c_last) s_total = pos;
+    }
+    __syncthreads();
+    const int total = s_total;
+    if (base + total > nt && threadIdx.x == 0 && overflow) atomicOr(overflow, 1);
+    const int live = static_cast<int>(base >= nt ? 0 : (base + total <= nt ? total : nt - base));
+    for (int v = threadIdx.x; v < live; v += kBlock) {
+      const int64_t idx = base + v;
+      const int64_t g0 = s_v0[v], g1 = s_v1[v];
+      const float t = s_t[v];
+      const float x0 = gv[3 * g0 + 0], y0 = gv[3 * g0 + 1], z0 = gv[3 * g0 + 2];
       const float x1 = gv[3 * g1 + 0], y1 = gv[3 * g1 + 1], z1 = gv[3 * g1 + 2];
       ov[3 * idx + 0] = x0 + t * (x1 - x0);
       ov[3 * idx + 1] = y0 + t * (y1 - y0);
       ov[3 * idx + 2] = z0 + t * (z1 - z0);
-      od[idx] = col.d0 + t * (d1 - col.d0);
-      otype[idx] = static_cast<uint8_t>(a);
-      for (int k = 0; k < nsh; ++k) {
-        const float* s0 = gsh + (int64_t(k) * ng + col.v0) * 3;
-        const float* s1 = gsh + (int64_t(k) * ng + g1) * 3;
-        float* o = osh + (int64_t(k) * nt + idx) * 3;
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch) o[ch] = s0[ch] + t * (s1[ch] - s0[ch]);
-      }
-      ++idx;
+      od[idx] = s_d0[v] + t * (s_d1[v] - s_d0[v]);
+      otype[idx] = s_axis[v];
     }
+    for (int q = 0; q < nsh; ++q) {
+      const float* plane = gsh + int64_t(q) * ng * 3;
+      float* out = osh + (int64_t(q) * nt + base) * 3;
+      for (int i = threadIdx.x; i < 3 * live; i += kBlock) {
+        const int v = i / 3, ch = i - 3 * v;
+        const float t = s_t[v];
+        const float a = plane[3 * int64_t(s_v0[v]) + ch];
+        const float b = plane[3 * int64_t(s_v1[v]) + ch];
+        out[i] = a + t * (b - a);
+      }
+    }
+    __syncthreads();  // the next level set reuses the lists
   }
 }
 
