@@ -41,6 +41,8 @@ def parse_args():
     p.add_argument("--width", type=int, default=1237)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-tiles", type=int, default=24)
+    p.add_argument("--cpu-slab", type=float, default=1 / 64,
+                   help="fraction of the grid's z layers the CPU baseline runs on")
     p.add_argument("--breakdown", action="store_true", help="print the per-kernel table to stderr")
     p.add_argument("--no-autotune", action="store_true",
                    help="skip the autotuner and use --tile / --chunk")
@@ -49,6 +51,14 @@ def parse_args():
     p.add_argument("--only-batched", action="store_true",
                    help="time only the batched step with --tile / --chunk (no plain path, no "
                         "autotuner, no CPU baseline): the run profiled per kernel")
+    p.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
+    p.add_argument("--serve-scenes", type=int, default=8)
+    p.add_argument("--serve-min-spheres", type=int, default=8)
+    p.add_argument("--serve-max-spheres", type=int, default=256)
+    p.add_argument("--serve-rates", type=float, nargs="+", default=[50.0, 200.0, 500.0, 2000.0])
+    p.add_argument("--serve-seconds", type=float, default=4.0)
+    p.add_argument("--serve-budget-gib", type=int, default=48)
+    p.add_argument("--serve-max-batch", type=int, default=8)
     p.add_argument("--probe", action="store_true",
                    help="build the scene, print its sizes and memory, and exit")
     return p.parse_args()
@@ -190,8 +200,9 @@ def algorithmic_bytes(z):
         "mt_create_triangle_faces": n_vox * (28 + 4) + s * n_vox * (1 + 8) + 4 * s * n_lc + vt + f * 16,
         "mt_project_vertices": vt * 12 + b * vt * 12,
         "mt_bound_triangles": f * 12 + b * vt * 12 + b * f * 20,
-        "mt_inclusive_scan_i32_to_i64": 12 * b * f,
-        "mt_generate_keys": b * f * (4 + 8 + 8) + f * 12 + b * vt * 4 + k * 12,
+        "mt_inclusive_scan_i32_to_i64": 12 * b * -(-f // 256),  # over (camera, 256-face block)
+        "mt_key_block_counts": f * 12 + b * vt * 12 + 4 * b * -(-f // 256),
+        "mt_generate_keys_from_geometry": f * 12 + b * vt * 12 + k * 12 + 8 * b * -(-f // 256),
         "mt_sort_keys": 8 * k + 24 * k * z.get("sort_passes", 8),
         "mt_delineate_tiles": 8 * k + 16 * b * z["T"],
         "mt_evaluate_sh_forward": vt * (12 + nb) + b * vt * 12,
@@ -201,18 +212,28 @@ def algorithmic_bytes(z):
         + 2 * (12 * b * vt + 4 * vt),
         "mt_evaluate_sh_backward": vt * (12 + nb) + b * vt * 12 + vt * nb,
         "mt_create_triangle_vertices_backward": n_lc * 16 + vt * (12 + 4 + nb) + 2 * n_lc * (4 + nb),
-        "zero_fill": 4 * (3 * b * vt + vt + vt * z["NSH"] * 3 + n_lc * (1 + 3 * z["NSH"])),
+        # Fused vertex + colour kernels of the batched step (grid-vertex count ~ N_lc).
+        "mt_create_triangle_vertices_colors_forward": n_lc * (16 + 4 + 12 + nb) + 4 * s * n_lc
+        + vt * 17 + b * vt * 12,
+        "mt_create_triangle_vertices_colors_backward": n_lc * (16 + 4) + 4 * s * n_lc
+        + vt * (4 + 12 + 24 * b) + n_lc * (4 + nb),
+        "zero_fill": 4 * (3 * b * vt + vt) + 16 * b * z["T"],
     }
 
 
 def sort_passes(keys):
-    """Digit passes the radix sort executes: 8-bit digits that are not constant."""
-    passes = 0
-    for p in range(8):
-        d = (keys >> (8 * p)) & 0xFF
-        if keys.numel() and int(d.min()) != int(d.max()):
-            passes += 1
-    return passes
+    """Digit passes the radix sort executes: 8-bit digits over the key bits that vary."""
+    import torch
+
+    if keys.numel() == 0:
+        return 0
+    u = keys.view(torch.int64)
+    varying = 0
+    for b in range(64):
+        bit = (u >> b) & 1
+        if int(bit.max()) != int(bit.min()):
+            varying += 1
+    return (varying + 7) // 8
 
 
 # ------------------------------------------------------------------------------------
@@ -220,32 +241,47 @@ def sort_passes(keys):
 # ------------------------------------------------------------------------------------
 
 
-def cpu_baseline(sc, args):
-    import numpy as np
+def cpu_baseline(sc, args, live):
+    """The oracle (NumPy, single thread) on a bounded spatial sample of the same step.
 
+    The sample is a slab of the occupancy grid (a fraction of its z layers, centred) with
+    fields drawn by the scene's own generator recipe. The oracle runs the full step on the
+    slab: tessellation, projection .. delineation, SH, fused composite forward + backward
+    (on a seeded set of non-empty tiles, scaled by key count), SH adjoint and vertex
+    adjoint. Each stage is then scaled to the full workload by its own unit: voxels for
+    tessellation and the vertex adjoint, faces for the front end and SH, keys for the
+    composite."""
+    import numpy as np
+    import torch
+
+    from meshtryoshka_amd import triangle_extraction as te
     from oracle import extraction as ox
     from oracle import rasterization as orr
 
-    v = sc.voxels
-    t0 = time.perf_counter()
-    sdf = sc.sdf.detach().cpu().numpy()
-    world = sc.world.cpu().numpy()
-    sh = sc.sh.detach().cpu().numpy()
-    lower = v.lower_corners.cpu().numpy()
-    upper = v.upper_corners.cpu().numpy()
-    nbr = v.neighbors.cpu().numpy()
-    idx = v.indices.cpu().numpy()
+    packed = sc.spec.occupancy
+    res = packed.shape[0]
+    layers = max(2, int(round(res * args.cpu_slab)))
+    i0 = (res - layers) // 2
+    crop = torch.zeros_like(packed)
+    crop[i0 : i0 + layers] = packed[i0 : i0 + layers]
+    v = te.extract_voxels(crop.to(sc.world.device))
+    gen = torch.Generator("cpu").manual_seed(args.seed + 1)
+    world_t, sdf_t, sh_t = sc.spec.field_fn(v.vertices.cpu(), gen)
+    world, sdf, sh = world_t.numpy(), sdf_t.numpy(), sh_t.numpy()
+    lower, upper = v.lower_corners.cpu().numpy(), v.upper_corners.cpu().numpy()
+    nbr, idx = v.neighbors.cpu().numpy(), v.indices.cpu().numpy()
     ls = sc.level_sets.cpu().numpy()
-    t_copy = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    tri = ox.tessellate_voxels(sdf, world, sh, nbr, lower, upper, idx, ls)
-    t_tess = time.perf_counter() - t0
     E = sc.cams.extrinsics.cpu().numpy()
     K = sc.cams.intrinsics.cpu().numpy()
     h, w = sc.cams.image_shape
     b = E.shape[0]
     th, tw = sc.spec.tile_shape
     rows, cols = -(-h // th), -(-w // tw)
+    deg = sc.spec.sh_degree
+
+    t0 = time.perf_counter()
+    tri = ox.tessellate_voxels(sdf, world, sh, nbr, lower, upper, idx, ls)
+    t_tess = time.perf_counter() - t0
     t0 = time.perf_counter()
     xy, depth = orr.project_vertices(tri.vertices, E, K)
     tmin, tmax, cnt = orr.bound_triangles(xy, depth, tri.faces, (th, tw), (rows, cols), sc.spec.near_plane)
@@ -253,17 +289,15 @@ def cpu_baseline(sc, args):
     keys, vals, _ = orr.generate_keys(depth, tri.faces, tri.face_boundaries, tmin, tmax, cnt, cols, lay)
     sk, sv = orr.sort_keys(keys, vals)
     ranges = orr.delineate_tiles(sk, lay, b, ls.shape[0], rows * cols, False)
-    col, _ = orr.evaluate_spherical_harmonics(tri.vertices, tri.spherical_harmonics, E, sc.spec.sh_degree)
+    col, _ = orr.evaluate_spherical_harmonics(tri.vertices, tri.spherical_harmonics, E, deg)
     col = col.astype(np.float32)
     t_front = time.perf_counter() - t0
-    # Composite forward + backward on a seeded sample of non-empty tiles, scaled by keys.
     rng = np.random.default_rng(0)
     nonempty = np.argwhere(ranges[:, 0, :, 1] > ranges[:, 0, :, 0])
     pick = nonempty[rng.choice(len(nonempty), size=min(args.cpu_sample_tiles, len(nonempty)), replace=False)]
     sub = np.zeros_like(ranges[:, 0])
     sub[pick[:, 0], pick[:, 1]] = ranges[pick[:, 0], 0, pick[:, 1]]
     keys_sampled = int((sub[..., 1] - sub[..., 0]).sum())
-    keys_total = int((ranges[:, 0, :, 1] - ranges[:, 0, :, 0]).sum())
     t0 = time.perf_counter()
     hits = orr.fused_hits(xy, tri.faces, sk, sv, sub, lay, ls.shape[0], (th, tw), (h, w))
     img = orr.fused_forward_image(hits, col, tri.signed_distances, tri.faces, sc.spec.sharpness, b, (h, w))
@@ -271,18 +305,114 @@ def cpu_baseline(sc, args):
     orr.fused_backward(hits, col.astype(np.float64), tri.signed_distances.astype(np.float64), tri.faces,
                        sc.spec.sharpness, img, g)
     t_comp_sample = time.perf_counter() - t0
-    t_comp = t_comp_sample * keys_total / max(keys_sampled, 1)
-    step_s = t_tess + t_front + t_comp
+    nv = tri.vertices.shape[0]
+    t0 = time.perf_counter()
+    gcol = rng.standard_normal((b, nv, 3)).astype(np.float32)
+    gsh = orr.evaluate_spherical_harmonics_backward(tri.vertices, tri.spherical_harmonics, E, deg, gcol)
+    t_shb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ox.create_triangle_vertices_backward(tri.record, world.shape[0], sh.shape[0], None,
+                                         rng.standard_normal(nv).astype(np.float32), gsh)
+    t_vb = time.perf_counter() - t0
+
+    vox_ratio = sc.voxels.num_voxels / max(v.num_voxels, 1)
+    face_ratio = live["F"] / max(tri.faces.shape[0], 1)
+    t_comp = t_comp_sample * live["K"] / max(keys_sampled, 1)
+    step_s = (t_tess + t_vb) * vox_ratio + (t_front + t_shb) * face_ratio + t_comp
+    sample_s = t_tess + t_front + t_comp_sample + t_shb + t_vb
     return {
-        "value": 1.0 / step_s,
+        "value": round(1.0 / step_s, 6),
         "unit": "steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"NumPy oracle, FP32: full tessellation ({t_tess:.1f} s) and front end "
-                   f"(projection..SH, {t_front:.1f} s) of the step, plus fused composite fwd+bwd "
-                   f"on {len(pick)} random non-empty tiles ({keys_sampled} of {keys_total} keys, "
-                   f"{t_comp_sample:.1f} s) scaled by key count to {t_comp:.0f} s; "
-                   f"excludes {t_copy:.1f} s of device-to-host copies"),
+        "sample": (f"NumPy oracle (1 thread) on a slab of {layers}/{res} z layers of the same scene "
+                   f"({v.num_voxels} voxels, {tri.faces.shape[0]} faces, {len(sk)} keys; "
+                   f"{sample_s:.1f} s of CPU work): tessellation {t_tess:.1f} s and vertex adjoint "
+                   f"{t_vb:.1f} s scaled x{vox_ratio:.1f} by voxels; projection..delineation + SH "
+                   f"{t_front:.1f} s and SH adjoint {t_shb:.1f} s scaled x{face_ratio:.1f} by faces; "
+                   f"fused composite fwd+bwd on {len(pick)} random non-empty tiles "
+                   f"({keys_sampled} keys, {t_comp_sample:.1f} s) scaled to {live['K']} keys"),
+        "seconds_per_step": round(step_s, 1),
+    }
+
+
+# ------------------------------------------------------------------------------------
+# Serving: device-resident scenes, Poisson request stream, continuous batching
+# ------------------------------------------------------------------------------------
+
+SERVE_SHAPES = ((256, 256), (512, 512), (768, 1024), (1080, 1920))
+
+
+def serving_benchmark(args, device, log):
+    """Resident scenes of the sphere-field generator with spheres log-spaced over
+    [args.serve_min_spheres, args.serve_max_spheres]; requests = (scene, random hemisphere
+    pose, resolution from SERVE_SHAPES), forward only. For each offered rate a Poisson
+    stream runs for args.serve_seconds; reports completed req/s and p50/p99 latency."""
+    import math
+    import random
+
+    import torch
+
+    from meshtryoshka_amd import serving
+    from meshtryoshka_amd import triangle_extraction as te
+    from meshtryoshka_amd import workloads
+
+    n = args.serve_scenes
+    lo, hi = math.log(args.serve_min_spheres), math.log(args.serve_max_spheres)
+    spheres = [int(round(math.exp(lo + (hi - lo) * i / max(1, n - 1)))) for i in range(n)]
+    scenes, cells = {}, []
+    t0 = time.time()
+    for sid, ns in enumerate(spheres):
+        spec = workloads.sphere_field(seed=100 + sid, num_spheres=ns, resolution=args.resolution,
+                                      num_cameras=4, image_shape=(512, 512), device=device)
+        vox = te.extract_voxels(spec.occupancy.to(device))
+        world, sdf, sh = workloads.finish_fields(spec, vox.vertices)
+        del spec.occupancy
+        sc = serving.tessellate_scene(vox, world.to(device), sdf.to(device), sh.to(device),
+                                      spec.level_sets.to(device), spec.sh_degree, spec.sharpness,
+                                      spec.background)
+        cams = spec.cameras.to(device)
+        sc.keys_per_pixel = serving.calibrate_key_density(sc, cams.extrinsics, cams.intrinsics, (512, 512))
+        cells.append(int(vox.num_voxels))
+        scenes[sid] = sc
+        del vox, world, sdf, sh
+    torch.cuda.synchronize()
+    resident = torch.cuda.memory_allocated(device)
+    log(f"[serve] {n} scenes ({min(cells)}..{max(cells)} cells) resident, "
+        f"{resident / 2**30:.1f} GiB, built in {time.time() - t0:.1f} s")
+    poses = {}
+    gen = torch.Generator("cpu").manual_seed(args.seed + 7)
+    for shape in SERVE_SHAPES:
+        poses[shape] = workloads.hemisphere_cameras(gen, 64, shape)
+    pick = random.Random(args.seed)
+
+    def make_request(rid, arrival):
+        shape = SERVE_SHAPES[pick.randrange(len(SERVE_SHAPES))]
+        e, k, _ = poses[shape][pick.randrange(64)]
+        return serving.Request(arrival=arrival, rid=rid, scene_id=pick.randrange(n),
+                               extrinsics=e, intrinsics=k, image_shape=shape)
+
+    # Warm-up: one request per (scene, shape) so first-use costs stay out of the sweep.
+    server = serving.RenderServer(scenes, hbm_budget_bytes=args.serve_budget_gib << 30,
+                                  max_batch=args.serve_max_batch)
+    for sid in scenes:
+        for shape in SERVE_SHAPES:
+            e, k, _ = poses[shape][0]
+            server.submit(serving.Request(0.0, 0, sid, e, k, shape))
+    server.drain()
+    torch.cuda.synchronize()
+    sweep = []
+    for rate in args.serve_rates:
+        server = serving.RenderServer(scenes, hbm_budget_bytes=args.serve_budget_gib << 30,
+                                      max_batch=args.serve_max_batch)
+        st = serving.poisson_load(server, make_request, rate, args.serve_seconds, seed=int(rate))
+        log(f"[serve] offered {rate} req/s: {st}")
+        sweep.append(st)
+    return {
+        "scenes": n, "spheres": spheres, "cells": cells, "resolution": args.resolution,
+        "shapes": [list(x) for x in SERVE_SHAPES], "hbm_budget_gib": args.serve_budget_gib,
+        "max_batch": args.serve_max_batch, "resident_gib": round(resident / 2**30, 2),
+        "sweep": sweep,
     }
 
 
@@ -364,6 +494,18 @@ def main():
     live = step.live_sizes()
     log(f"[bench] batched graph step: {ms_per_step:.2f} ms/step, loss {float(step.loss):.6f}, {live}")
 
+    if args.only_batched:
+        # Reference transfer for HBM counters (FETCH_SIZE / WRITE_SIZE): one elementwise
+        # kernel that reads exactly 1 GiB and writes exactly 1 GiB (torch's MulFunctor
+        # kernel, the last one of the run), so counter readings can be scaled to bytes.
+        src = torch.ones(1 << 28, dtype=torch.float32, device=device)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log("[bench] calibration copy: 1 GiB read + 1 GiB written")
+
     # 4. Per-kernel durations: the same kernels, eager, bracketed by HIP events.
     timer = LaunchTimer()
     stats = torch.zeros(2, dtype=torch.int64, device=device)
@@ -421,7 +563,21 @@ def main():
     cpu = None
     if not (args.no_cpu_baseline or args.only_batched):
         log("[bench] CPU baseline (oracle on a bounded sample)")
-        cpu = cpu_baseline(sc, args)
+        from threadpoolctl import threadpool_limits
+
+        with threadpool_limits(limits=1):  # the baseline is a single-thread port
+            cpu = cpu_baseline(sc, args, live)
+
+    serve = None
+    if not (args.no_serving or args.only_batched):
+        # Free the training step's arenas and the scene before loading the served scenes.
+        del step, step_args, with_stats, inputs
+        sc = None
+        timer = None
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        log("[bench] serving sweep")
+        serve = serving_benchmark(args, device, log)
 
     out = {
         "metric": "train steps/s fwd+bwd (batched graph-captured step; plain per-camera path beside it)",
@@ -449,6 +605,7 @@ def main():
                      "candidates": [[list(c.tile_shape), c.chunk, round(c.ms, 3)] for c in tuned]},
         "roofline": roof,
         "kernels": table,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

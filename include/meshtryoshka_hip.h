@@ -137,6 +137,38 @@ int mt_create_triangle_vertices_backward(
     float* grid_sdf_grad, float* grid_sh_grad, void* workspace, size_t workspace_bytes,
     mt_stream_t stream);
 
+/* Batched training step (no single counterpart in the reference: fuses
+ * create_triangle_vertices.slang:145-199 with evaluate_spherical_harmonics.slang:187-226).
+ * Writes out_vertices / out_sdf / out_types exactly like
+ * mt_create_triangle_vertices_forward and out_colors f32[B, num_triangle_vertices, 3] for
+ * the B cameras centred at camera_positions f32[B, 3]; each triangle vertex's SH
+ * coefficients are interpolated from grid_sh in registers and never stored. Colours are
+ * bit-identical to mt_create_triangle_vertices_forward + mt_evaluate_sh_forward. */
+int mt_create_triangle_vertices_colors_forward(
+    const float* grid_vertices, const float* grid_sdf, const float* grid_sh,
+    int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree, const int32_t* lower_corners,
+    int64_t num_columns, const float* level_sets, int32_t S, const int32_t* vertex_offsets,
+    int64_t num_triangle_vertices, const float* camera_positions, int32_t B, float* out_vertices,
+    float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, mt_stream_t stream);
+
+/* Adjoint of the fused forward above with respect to grid_sdf and grid_sh (fuses
+ * evaluate_spherical_harmonics.slang:228-270 with create_triangle_vertices.slang:201-291).
+ * Overwrites grid_sdf_grad f32[num_grid_vertices] and grid_sh_grad
+ * f32[num_sh, num_grid_vertices, 3] (no zero-fill needed) from grad_sdf f32[V] and
+ * grad_colors f32[B, V, 3], using the forward's vertices f32[V, 3] and colors (relu is
+ * active where the colour is > 0). Every grid vertex gathers from the columns that
+ * reference it (workspace: mt_create_triangle_vertices_backward_workspace_bytes); this
+ * needs each grid vertex to be the origin / +axis end of at most one column, as
+ * extract_voxels guarantees; otherwise nothing is written and bit 1 (value 2) of *status
+ * is set. num_sh in {1, 4, 9, 16}. */
+int mt_create_triangle_vertices_colors_backward(
+    const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree,
+    const int32_t* lower_corners, int64_t num_columns, const float* level_sets, int32_t S,
+    const int32_t* vertex_offsets, int64_t num_triangle_vertices, const float* vertices,
+    const float* colors, const float* camera_positions, int32_t B, const float* grad_sdf,
+    const float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
+    size_t workspace_bytes, int32_t* status, mt_stream_t stream);
+
 /* face_counts int32[S * num_voxels] (3 x triangles) and cell_codes uint8[S, num_voxels].
  * Replaces count_triangle_faces.slang:13-50 (.py:28-63). upper_corners
  * int32[4, num_voxels]. */
@@ -189,6 +221,32 @@ int mt_generate_keys(const float* depths, int64_t V, const int32_t* faces, int64
                      const int32_t* tile_max, const int64_t* offsets, int32_t B,
                      int32_t grid_cols, mt_key_layout layout, int64_t* keys, int32_t* values,
                      int64_t key_capacity, int32_t* overflow, mt_stream_t stream);
+
+/* Batched training step (no single counterpart in the reference: fuses
+ * bound_triangles.slang:144-192 with generate_keys.slang:4-68 and the tile-count cumsum of
+ * generate_keys.py:40-47). Faces are taken in blocks of 256 per camera. block_counts
+ * int32[B, ceil(F / 256)] = number of keys of every (camera, face block), computed with
+ * mt_bound_triangles' arithmetic (same culling, same num_faces_x3 semantics). */
+int mt_key_block_counts(const float* xy, const float* depths, int64_t V, const int32_t* faces,
+                        int64_t F, const int32_t* num_faces_x3, int32_t B, int32_t tile_h,
+                        int32_t tile_w, int32_t row_min, int32_t col_min, int32_t row_max,
+                        int32_t col_max, float near_plane, int32_t* block_counts,
+                        mt_stream_t stream);
+
+/* Keys of every (camera, face, covered tile) like mt_generate_keys, with each face's
+ * tile rectangle recomputed from the projected geometry (no stored bounds) and each
+ * block's first key at block_offsets[q - 1] (0 for q = 0), block_offsets = the inclusive
+ * int64 scan of mt_key_block_counts' output (mt_inclusive_scan_i32_to_i64). The keys and
+ * values are identical to mt_bound_triangles + inclusive scan + mt_generate_keys. */
+int mt_generate_keys_from_geometry(const float* xy, const float* depths, int64_t V,
+                                   const int32_t* faces, int64_t F, const int32_t* num_faces_x3,
+                                   const int32_t* shell_face_boundaries, int32_t S, int32_t B,
+                                   int32_t tile_h, int32_t tile_w, int32_t row_min,
+                                   int32_t col_min, int32_t row_max, int32_t col_max,
+                                   float near_plane, const int64_t* block_offsets,
+                                   int32_t grid_cols, mt_key_layout layout, int64_t* keys,
+                                   int32_t* values, int64_t key_capacity, int32_t* overflow,
+                                   mt_stream_t stream);
 
 /* Workspace for mt_sort_keys on n pairs. */
 size_t mt_sort_workspace_bytes(int64_t n);

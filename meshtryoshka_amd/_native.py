@@ -86,6 +86,16 @@ SIGNATURES = {
         [c_p, c_i64, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
          c_sz, c_p],
     ),
+    "mt_create_triangle_vertices_colors_forward": (
+        c_i32,
+        [c_p, c_p, c_p, c_i64, c_i32, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_i32, c_p,
+         c_p, c_p, c_p, c_p, c_p],
+    ),
+    "mt_create_triangle_vertices_colors_backward": (
+        c_i32,
+        [c_p, c_i64, c_i32, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_i32, c_p,
+         c_p, c_p, c_p, c_p, c_sz, c_p, c_p],
+    ),
     "mt_count_triangle_faces": (c_i32, [c_p, c_p, c_i64, c_p, c_i64, c_p, c_i32, c_p, c_p, c_p]),
     "mt_create_triangle_faces": (
         c_i32,
@@ -101,6 +111,16 @@ SIGNATURES = {
         c_i32,
         [c_p, c_i64, c_p, c_i64, c_p, c_i32, c_p, c_p, c_p, c_i32, c_i32, KeyLayout, c_p, c_p,
          c_i64, c_p, c_p],
+    ),
+    "mt_key_block_counts": (
+        c_i32,
+        [c_p, c_p, c_i64, c_p, c_i64, c_p, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_f32,
+         c_p, c_p],
+    ),
+    "mt_generate_keys_from_geometry": (
+        c_i32,
+        [c_p, c_p, c_i64, c_p, c_i64, c_p, c_p, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32,
+         c_i32, c_f32, c_p, c_i32, KeyLayout, c_p, c_p, c_i64, c_p, c_p],
     ),
     "mt_sort_workspace_bytes": (c_sz, [c_i64]),
     "mt_sort_keys": (c_i32, [c_p, c_p, c_i64, c_p, c_p, c_p, c_sz, c_p, c_p]),

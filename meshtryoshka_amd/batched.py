@@ -9,7 +9,13 @@ capacity, kernels read the live counts from device memory (``num_faces_x3`` /
 ``n_device`` arguments of the C ABI) and write nothing past a capacity (they raise a
 device overflow flag instead), so the whole step can be captured once into a hipGraph
 (``torch.cuda.CUDAGraph``) and replayed. Results are bit-identical to the plain path:
-the same kernels run on the same data, only the sizes come from device memory.
+the same arithmetic runs on the same data, only the sizes come from device memory.
+
+Unlike the plain path, the per-triangle-vertex SH array ([NSH, V, 3], 192 bytes per vertex
+at degree 3) is never stored: vertices and their colours for all cameras come out of one
+kernel that interpolates the grid SH in registers, and the SH adjoint is recomputed per
+triangle vertex inside the gather of the vertex adjoint. Colours stay bit-identical to the
+plain path's; gradients equal it up to the sign of zero terms.
 
 Capacities come from one eager step (``measure_capacities``) times a margin; after a
 replay, ``overflowed()`` (one host read) tells whether any capacity was exceeded, in
@@ -81,16 +87,16 @@ class BatchedTrainingStep:
         self.codes = torch.empty((S, self.n_vox), dtype=torch.uint8, device=dev)
         self.tv = torch.zeros((V, 3), **f32)
         self.td = torch.zeros(V, **f32)
-        self.tsh = torch.zeros((num_sh, V, 3), **f32)
         self.ttype = torch.zeros(V, dtype=torch.uint8, device=dev)
         self.faces = torch.zeros((F, 3), **i32)
         self.voxel_indices = torch.zeros(F, **i32)
         self.xy = torch.zeros((B, V, 2), **f32)
         self.depth = torch.zeros((B, V), **f32)
-        self.tmin = torch.zeros((B, F, 2), **i32)
-        self.tmax = torch.zeros((B, F, 2), **i32)
-        self.counts = torch.zeros((B, F), **i32)
-        self.key_offsets = torch.zeros(B * F, **i64)
+        # Keys per (camera, 256-face block) and their inclusive scan (the key kernels need
+        # no per-face tile bounds, counts or offsets).
+        self.n_blk = (F + 255) // 256
+        self.block_counts = torch.zeros(B * self.n_blk, **i32)
+        self.block_offsets = torch.zeros(B * self.n_blk, **i64)
         self.keys = torch.zeros(K, **i64)
         self.values = torch.zeros(K, **i32)
         self.sorted_keys = torch.zeros(K, **i64)
@@ -104,20 +110,19 @@ class BatchedTrainingStep:
         self.colors_grad = torch.zeros_like(self.colors)
         self.td_grad = torch.zeros_like(self.td)
         self.sharp_grad = torch.zeros(1, **f32)
-        self.tsh_grad = torch.zeros_like(self.tsh)
         self.grid_sdf_grad = torch.zeros(self.n_grid, **f32)
         self.grid_sh_grad = torch.zeros((num_sh, self.n_grid, 3), **f32)
         self.overflow = torch.zeros(1, **i32)
         self.sharpness = torch.tensor([float(sharpness)], **f32)
         self.background = torch.as_tensor(background, dtype=torch.float32, device=dev).contiguous()
-        scan_n = max(S * self.n_col + 1, S * self.n_vox + 1, B * F)
+        scan_n = max(S * self.n_col + 1, S * self.n_vox + 1, B * self.n_blk)
         self.scan_ws = _native.workspace(_native.query("mt_scan_workspace_bytes", scan_n), dev)
         self.sort_ws = _native.workspace(_native.query("mt_sort_workspace_bytes", K), dev)
         self.vbwd_ws = _native.workspace(
             _native.query("mt_create_triangle_vertices_backward_workspace_bytes", self.n_grid), dev)
         # Live counts, read by kernels from device memory.
         self.num_faces_x3 = self.face_offsets[S * self.n_vox :]
-        self.num_keys = self.key_offsets[B * F - 1 :]
+        self.num_keys = self.block_offsets[B * self.n_blk - 1 :]
         self.graph = None
         self.static_inputs = None
 
@@ -152,9 +157,6 @@ class BatchedTrainingStep:
         self.colors_grad.zero_()
         self.td_grad.zero_()
         self.sharp_grad.zero_()
-        self.tsh_grad.zero_()
-        self.grid_sdf_grad.zero_()
-        self.grid_sh_grad.zero_()
         if hook is not None:
             hook("zero_fill")
         # --- tessellation
@@ -162,10 +164,13 @@ class BatchedTrainingStep:
              self.n_col, self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), st)
         call("mt_exclusive_scan_i32", self.vertex_offsets.data_ptr(), self.vertex_offsets.numel(),
              self.scan_ws.data_ptr(), self.scan_ws.numel(), st)
-        call("mt_create_triangle_vertices_forward", grid_vertices.data_ptr(), grid_sdf.data_ptr(),
-             grid_sh.data_ptr(), self.n_grid, self.num_sh, v.lower_corners.data_ptr(), self.n_col,
-             self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V, self.tv.data_ptr(),
-             self.td.data_ptr(), self.tsh.data_ptr(), self.ttype.data_ptr(),
+        call("mt_camera_positions", self.extrinsics.data_ptr(), B, self.campos.data_ptr(), st)
+        # Vertices and their colours in one pass; the per-vertex SH array is never stored.
+        call("mt_create_triangle_vertices_colors_forward", grid_vertices.data_ptr(),
+             grid_sdf.data_ptr(), grid_sh.data_ptr(), self.n_grid, self.num_sh, self.sh_degree,
+             v.lower_corners.data_ptr(), self.n_col, self.level_sets.data_ptr(), S,
+             self.vertex_offsets.data_ptr(), V, self.campos.data_ptr(), B, self.tv.data_ptr(),
+             self.td.data_ptr(), self.ttype.data_ptr(), self.colors.data_ptr(),
              self.overflow.data_ptr(), st)
         call("mt_count_triangle_faces", grid_sdf.data_ptr(), v.lower_corners.data_ptr(), self.n_col,
              v.upper_corners.data_ptr(), self.n_vox, self.level_sets.data_ptr(), S,
@@ -183,24 +188,22 @@ class BatchedTrainingStep:
         call("mt_project_vertices", self.tv.data_ptr(), V, self.extrinsics.data_ptr(),
              self.intrinsics.data_ptr(), B, self.xy.data_ptr(), self.depth.data_ptr(), st)
         th, tw = self.tile_shape
-        call("mt_bound_triangles", self.xy.data_ptr(), self.depth.data_ptr(), V,
-             self.faces.data_ptr(), F, B, th, tw, 0, 0, self.rows, self.cols, self.near_plane,
-             self.tmin.data_ptr(), self.tmax.data_ptr(), self.counts.data_ptr(),
-             self.num_faces_x3.data_ptr(), st)
-        call("mt_inclusive_scan_i32_to_i64", self.counts.data_ptr(), self.key_offsets.data_ptr(),
-             B * F, self.scan_ws.data_ptr(), self.scan_ws.numel(), st)
-        call("mt_generate_keys", self.depth.data_ptr(), V, self.faces.data_ptr(), F,
-             face_boundaries.data_ptr(), S, self.tmin.data_ptr(), self.tmax.data_ptr(),
-             self.key_offsets.data_ptr(), B, self.cols, self.layout, self.keys.data_ptr(),
-             self.values.data_ptr(), K, self.overflow.data_ptr(), st)
+        call("mt_key_block_counts", self.xy.data_ptr(), self.depth.data_ptr(), V,
+             self.faces.data_ptr(), F, self.num_faces_x3.data_ptr(), B, th, tw, 0, 0, self.rows,
+             self.cols, self.near_plane, self.block_counts.data_ptr(), st)
+        call("mt_inclusive_scan_i32_to_i64", self.block_counts.data_ptr(),
+             self.block_offsets.data_ptr(), B * self.n_blk, self.scan_ws.data_ptr(),
+             self.scan_ws.numel(), st)
+        call("mt_generate_keys_from_geometry", self.xy.data_ptr(), self.depth.data_ptr(), V,
+             self.faces.data_ptr(), F, self.num_faces_x3.data_ptr(), face_boundaries.data_ptr(), S,
+             B, th, tw, 0, 0, self.rows, self.cols, self.near_plane, self.block_offsets.data_ptr(),
+             self.cols, self.layout, self.keys.data_ptr(), self.values.data_ptr(), K,
+             self.overflow.data_ptr(), st)
         call("mt_sort_keys", self.keys.data_ptr(), self.values.data_ptr(), K,
              self.sorted_keys.data_ptr(), self.sorted_values.data_ptr(), self.sort_ws.data_ptr(),
              self.sort_ws.numel(), self.num_keys.data_ptr(), st)
         call("mt_delineate_tiles", self.sorted_keys.data_ptr(), K, self.layout, 0, B, 1, self.T,
              self.ranges.data_ptr(), self.num_keys.data_ptr(), st)
-        call("mt_camera_positions", self.extrinsics.data_ptr(), B, self.campos.data_ptr(), st)
-        call("mt_evaluate_sh_forward", self.tv.data_ptr(), V, self.tsh.data_ptr(), self.num_sh,
-             self.sh_degree, self.campos.data_ptr(), B, self.colors.data_ptr(), st)
         args = self._composite_args()
         call("mt_intersect_and_composite_forward", args, self.out.data_ptr(), st)
         # --- loss and backward chain
@@ -209,14 +212,14 @@ class BatchedTrainingStep:
         call("mt_intersect_and_composite_backward", args, self.out.data_ptr(),
              self.grad_out.data_ptr(), self.colors_grad.data_ptr(), self.td_grad.data_ptr(),
              self.sharp_grad.data_ptr(), self.quirk, st)
-        call("mt_evaluate_sh_backward", self.tv.data_ptr(), V, self.tsh.data_ptr(), self.num_sh,
-             self.sh_degree, self.campos.data_ptr(), B, self.colors_grad.data_ptr(),
-             self.tsh_grad.data_ptr(), st)
-        call("mt_create_triangle_vertices_backward", grid_sdf.data_ptr(), self.n_grid, self.num_sh,
-             v.lower_corners.data_ptr(), self.n_col, self.level_sets.data_ptr(), S,
-             self.vertex_offsets.data_ptr(), V, None, self.td_grad.data_ptr(),
-             self.tsh_grad.data_ptr(), None, self.grid_sdf_grad.data_ptr(),
-             self.grid_sh_grad.data_ptr(), self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(), st)
+        # SH adjoint fused into the gather of the vertex adjoint (overwrites the grid grads).
+        call("mt_create_triangle_vertices_colors_backward", grid_sdf.data_ptr(), self.n_grid,
+             self.num_sh, self.sh_degree, v.lower_corners.data_ptr(), self.n_col,
+             self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V, self.tv.data_ptr(),
+             self.colors.data_ptr(), self.campos.data_ptr(), B, self.td_grad.data_ptr(),
+             self.colors_grad.data_ptr(), self.grid_sdf_grad.data_ptr(),
+             self.grid_sh_grad.data_ptr(), self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(),
+             self.overflow.data_ptr(), st)
 
     # ------------------------------------------------------------------------------
     def capture(self, grid_vertices, grid_sdf, grid_sh, target) -> None:

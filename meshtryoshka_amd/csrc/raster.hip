@@ -3,6 +3,7 @@
 // library is compiled without FP contraction so that keys (which embed depth bits) and
 // tile bounds are reproducible bit for bit.
 #include "common.h"
+#include "sh.h"
 
 namespace mt {
 namespace {
@@ -48,6 +49,46 @@ __device__ __forceinline__ int32_t f2i_rz(float x) {
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
+// Tile rows [y0, y1) x columns [x0, x1) overlapped by one face in one camera
+// (bound_triangles.slang:144-192): empty for back faces (negative signed area) and faces
+// with a corner in front of the near plane.
+struct TileRect {
+  int y0, y1, x0, x1;
+};
+
+__device__ __forceinline__ TileRect face_tile_rect(const float* xy, const float* depths, int64_t o,
+                                                   int64_t ia, int64_t ib, int64_t ic, int th,
+                                                   int tw, int rmin, int cmin, int rmax, int cmax,
+                                                   float near_plane) {
+  const float2* pxy = reinterpret_cast<const float2*>(xy);
+  const float2 a = pxy[o + ia], bb = pxy[o + ib], c = pxy[o + ic];
+  const float da = depths[o + ia], db = depths[o + ib], dc = depths[o + ic];
+  TileRect r{0, 0, 0, 0};
+  const float check = (bb.x - a.x) * (c.y - a.y) - (bb.y - a.y) * (c.x - a.x);
+  const float min_depth = fminf(fminf(da, db), dc);
+  if (!(check < 0.0f) && !(min_depth < near_plane)) {
+    const float ylo = fminf(fminf(a.y, bb.y), c.y), yhi = fmaxf(fmaxf(a.y, bb.y), c.y);
+    const float xlo = fminf(fminf(a.x, bb.x), c.x), xhi = fmaxf(fmaxf(a.x, bb.x), c.x);
+    r.y0 = clampi(f2i_rz(ylo) / th, rmin, rmax);
+    r.y1 = clampi(f2i_rz(ceilf(yhi / float(th))), rmin, rmax);
+    r.x0 = clampi(f2i_rz(xlo) / tw, cmin, cmax);
+    r.x1 = clampi(f2i_rz(ceilf(xhi / float(tw))), cmin, cmax);
+  }
+  return r;
+}
+
+// Face whose keys are generated: live (below the device-side face count) and with all
+// three corners inside the vertex array.
+__device__ __forceinline__ bool keyed_face(const int32_t* faces, int64_t f, int64_t V,
+                                           const int32_t* live_x3, int64_t& ia, int64_t& ib,
+                                           int64_t& ic) {
+  ia = faces[3 * f + 0];
+  ib = faces[3 * f + 1];
+  ic = faces[3 * f + 2];
+  const bool bad_corner = ia < 0 || ib < 0 || ic < 0 || ia >= V || ib >= V || ic >= V;
+  return !((live_x3 != nullptr && f >= int64_t(*live_x3) / 3) || bad_corner);
+}
+
 __global__ void __launch_bounds__(kBlock) bound_kernel(const float* xy, const float* depths,
                                                        int64_t V, const int32_t* faces, int64_t F,
                                                        int B, int th, int tw, int rmin, int cmin,
@@ -67,26 +108,13 @@ __global__ void __launch_bounds__(kBlock) bound_kernel(const float* xy, const fl
     }
     return;
   }
-  const float2* pxy = reinterpret_cast<const float2*>(xy);
   for (int b = 0; b < B; ++b) {
-    const int64_t o = int64_t(b) * V;
-    const float2 a = pxy[o + ia], bb = pxy[o + ib], c = pxy[o + ic];
-    const float da = depths[o + ia], db = depths[o + ib], dc = depths[o + ic];
-    int y0 = 0, y1 = 0, x0 = 0, x1 = 0;
-    const float check = (bb.x - a.x) * (c.y - a.y) - (bb.y - a.y) * (c.x - a.x);
-    const float min_depth = fminf(fminf(da, db), dc);
-    if (!(check < 0.0f) && !(min_depth < near_plane)) {
-      const float ylo = fminf(fminf(a.y, bb.y), c.y), yhi = fmaxf(fmaxf(a.y, bb.y), c.y);
-      const float xlo = fminf(fminf(a.x, bb.x), c.x), xhi = fmaxf(fmaxf(a.x, bb.x), c.x);
-      y0 = clampi(f2i_rz(ylo) / th, rmin, rmax);
-      y1 = clampi(f2i_rz(ceilf(yhi / float(th))), rmin, rmax);
-      x0 = clampi(f2i_rz(xlo) / tw, cmin, cmax);
-      x1 = clampi(f2i_rz(ceilf(xhi / float(tw))), cmin, cmax);
-    }
+    const TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic, th, tw, rmin, cmin,
+                                      rmax, cmax, near_plane);
     const int64_t q = int64_t(b) * F + f;
-    reinterpret_cast<int2*>(tmin)[q] = make_int2(y0, x0);
-    reinterpret_cast<int2*>(tmax)[q] = make_int2(y1, x1);
-    counts[q] = (y1 - y0) * (x1 - x0);
+    reinterpret_cast<int2*>(tmin)[q] = make_int2(r.y0, r.x0);
+    reinterpret_cast<int2*>(tmax)[q] = make_int2(r.y1, r.x1);
+    counts[q] = (r.y1 - r.y0) * (r.x1 - r.x0);
   }
 }
 
@@ -178,6 +206,112 @@ __global__ void __launch_bounds__(kBlock) keys_kernel(const float* depths, int64
 }
 
 // ------------------------------------------------------------------------------------
+// Batched step: keys straight from the projected geometry. The per-(camera, face) tile
+// bounds, counts and the inclusive scan over all of them are never stored: one kernel
+// counts the keys of each (camera, 256-face block), a scan over the block counts gives
+// each block its output position, and the key kernel recomputes each face's tile
+// rectangle (the same face_tile_rect arithmetic) while writing the block's keys.
+// ------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) key_block_counts_kernel(
+    const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F, int B,
+    int th, int tw, int rmin, int cmin, int rmax, int cmax, float near_plane,
+    const int32_t* live_x3, int32_t* block_counts) {
+  __shared__ int32_t s_wave[kBlock / kWave];
+  const int64_t f = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t nblk = gridDim.x;
+  int64_t ia = 0, ib = 0, ic = 0;
+  const bool keyed = f < F && keyed_face(faces, f, V, live_x3, ia, ib, ic);
+  for (int b = 0; b < B; ++b) {
+    int32_t count = 0;
+    if (keyed) {
+      const TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic, th, tw, rmin, cmin,
+                                        rmax, cmax, near_plane);
+      count = (r.y1 - r.y0) * (r.x1 - r.x0);
+    }
+    const int32_t w = wave_sum(count);
+    if (__lane_id() == 0) s_wave[threadIdx.x / kWave] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int32_t total = 0;
+#pragma unroll
+      for (int q = 0; q < kBlock / kWave; ++q) total += s_wave[q];
+      block_counts[int64_t(b) * nblk + blockIdx.x] = total;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
+    const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
+    const int32_t* live_x3, const int32_t* boundaries, int S, int th, int tw, int rmin, int cmin,
+    int rmax, int cmax, float near_plane, const int64_t* block_incl, int grid_cols, KeyFields kf,
+    int64_t* keys, int32_t* values, int64_t kcap, int32_t* overflow) {
+  __shared__ int32_t s_prefix[kBlock + 1];
+  __shared__ int32_t s_r0[kBlock], s_c0[kBlock], s_w[kBlock];
+  __shared__ uint64_t s_key_hi[kBlock];
+  __shared__ int32_t s_wave[kBlock / kWave];
+  __shared__ int64_t s_block_start;
+  const int b = blockIdx.y;
+  const int64_t f = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  int32_t count = 0;
+  int64_t ia = 0, ib = 0, ic = 0;
+  if (f < F && keyed_face(faces, f, V, live_x3, ia, ib, ic)) {
+    const int64_t o = int64_t(b) * V;
+    const TileRect r = face_tile_rect(xy, depths, o, ia, ib, ic, th, tw, rmin, cmin, rmax, cmax,
+                                      near_plane);
+    count = (r.y1 - r.y0) * (r.x1 - r.x0);
+    s_r0[threadIdx.x] = r.y0;
+    s_c0[threadIdx.x] = r.x0;
+    s_w[threadIdx.x] = r.x1 - r.x0;
+    if (count > 0) {
+      uint32_t shell = 0;
+      while (shell < uint32_t(S) && !(f < boundaries[shell])) ++shell;
+      const float mean_depth = ((depths[o + ia] + depths[o + ib]) + depths[o + ic]) / 3.f;
+      s_key_hi[threadIdx.x] = pack_key(kf, uint32_t(b), shell, 0u, mean_depth);
+    }
+  }
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / kWave;
+  const int32_t inc = wave_inclusive_sum(count);
+  if (lane == kWave - 1) s_wave[wave] = inc;
+  if (threadIdx.x == 0) {
+    const int64_t q = int64_t(b) * gridDim.x + blockIdx.x;
+    s_block_start = q == 0 ? 0 : block_incl[q - 1];
+  }
+  __syncthreads();
+  int32_t before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    if (w < wave) before += s_wave[w];
+    total += s_wave[w];
+  }
+  s_prefix[threadIdx.x] = before + inc - count;
+  __syncthreads();
+  const int64_t base = s_block_start;
+  const uint64_t tile_mask = field_mask(kf.tile_bits);
+  for (int32_t j = threadIdx.x; j < total; j += kBlock) {
+    int lo = 0, hi = kBlock - 1;  // last face whose prefix is <= j (it has keys)
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_prefix[mid] <= j) lo = mid; else hi = mid - 1;
+    }
+    const int32_t local = j - s_prefix[lo];
+    const int32_t w = s_w[lo];
+    const uint32_t row = uint32_t(s_r0[lo] + local / w);
+    const uint32_t col = uint32_t(s_c0[lo] + local % w);
+    const uint32_t tile = row * uint32_t(grid_cols) + col;
+    const uint64_t key = s_key_hi[lo] | ((uint64_t(tile) & tile_mask) << kf.tile_offset);
+    if (base + j >= kcap) {
+      if (overflow) atomicOr(overflow, 1);
+      continue;
+    }
+    keys[base + j] = static_cast<int64_t>(key);
+    values[base + j] = static_cast<int32_t>(int64_t(blockIdx.x) * kBlock + lo);
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // Delineation (delineate_tiles.slang:12-58)
 // ------------------------------------------------------------------------------------
 
@@ -246,49 +380,6 @@ __global__ void camera_positions_kernel(const float* E, int B, float* out) {
 // Spherical harmonics (evaluate_spherical_harmonics.slang:124-270)
 // ------------------------------------------------------------------------------------
 
-constexpr float kC0 = 0.28209479177387814f;
-constexpr float kC1 = 0.4886025119029199f;
-constexpr float kC2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f,
-                          -1.0925484305920792f, 0.5462742152960396f};
-constexpr float kC3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
-                          0.3731763325901154f,  -0.4570457994644658f, 1.445305721320277f,
-                          -0.5900435899266435f};
-
-// Basis values with the reference's signs; only the first (deg+1)^2 are written.
-__device__ __forceinline__ void sh_basis(float x, float y, float z, int deg, float* out) {
-  out[0] = kC0;
-  if (deg > 0) {
-    out[1] = -kC1 * y;
-    out[2] = kC1 * z;
-    out[3] = -kC1 * x;
-  }
-  if (deg > 1) {
-    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
-    out[4] = kC2[0] * xy;
-    out[5] = kC2[1] * yz;
-    out[6] = kC2[2] * (2.0f * zz - xx - yy);
-    out[7] = kC2[3] * xz;
-    out[8] = kC2[4] * (xx - yy);
-    if (deg > 2) {
-      out[9] = kC3[0] * y * (3.0f * xx - yy);
-      out[10] = kC3[1] * xy * z;
-      out[11] = kC3[2] * y * (4.0f * zz - xx - yy);
-      out[12] = kC3[3] * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
-      out[13] = kC3[4] * x * (4.0f * zz - xx - yy);
-      out[14] = kC3[5] * z * (xx - yy);
-      out[15] = kC3[6] * x * (xx - 3.0f * yy);
-    }
-  }
-}
-
-__device__ __forceinline__ float3 view_dir(const float* verts, int64_t v, const float* cam) {
-  float dx = verts[3 * v + 0] - cam[0];
-  float dy = verts[3 * v + 1] - cam[1];
-  float dz = verts[3 * v + 2] - cam[2];
-  const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
-  return make_float3(dx * inv, dy * inv, dz * inv);
-}
-
 template <int kDeg>
 __global__ void __launch_bounds__(kBlock) sh_forward_kernel(const float* verts, int64_t V,
                                                             const float* sh, const float* cams,
@@ -302,16 +393,12 @@ __global__ void __launch_bounds__(kBlock) sh_forward_kernel(const float* verts, 
 #pragma unroll
     for (int c = 0; c < 3; ++c) coef[k][c] = sh[(int64_t(k) * V + v) * 3 + c];
   for (int b = 0; b < B; ++b) {
-    const float3 d = view_dir(verts, v, cams + 3 * b);
+    const float3 d = view_dir(vertex_at(verts, v), cams + 3 * b);
     float basis[16];
     sh_basis(d.x, d.y, d.z, kDeg, basis);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      float acc = basis[0] * coef[0][c];
-#pragma unroll
-      for (int k = 1; k < kN; ++k) acc += basis[k] * coef[k][c];
-      colors[(int64_t(b) * V + v) * 3 + c] = fmaxf(acc + 0.5f, 0.0f);
-    }
+    for (int c = 0; c < 3; ++c)
+      colors[(int64_t(b) * V + v) * 3 + c] = fmaxf(sh_channel_pre<kN>(basis, coef, c), 0.0f);
   }
 }
 
@@ -333,16 +420,13 @@ __global__ void __launch_bounds__(kBlock) sh_backward_kernel(const float* verts,
       grad[k][c] = 0.f;
     }
   for (int b = 0; b < B; ++b) {
-    const float3 d = view_dir(verts, v, cams + 3 * b);
+    const float3 d = view_dir(vertex_at(verts, v), cams + 3 * b);
     float basis[16];
     sh_basis(d.x, d.y, d.z, kDeg, basis);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      float acc = basis[0] * coef[0][c];
-#pragma unroll
-      for (int k = 1; k < kN; ++k) acc += basis[k] * coef[k][c];
       // d max(x, 0) / dx passes only for x > 0.
-      const float g = (acc + 0.5f > 0.0f) ? gcol[(int64_t(b) * V + v) * 3 + c] : 0.0f;
+      const float g = (sh_channel_pre<kN>(basis, coef, c) > 0.0f) ? gcol[(int64_t(b) * V + v) * 3 + c] : 0.0f;
 #pragma unroll
       for (int k = 0; k < kN; ++k) grad[k][c] += basis[k] * g;
     }
@@ -393,6 +477,38 @@ extern "C" int mt_generate_keys(const float* depths, int64_t V, const int32_t* f
       depths, V, faces, F, shell_face_boundaries, S, tile_min, tile_max, offsets, grid_cols,
       mt::to_fields(layout), keys, values, key_capacity, overflow);
   return mt::check_launch("mt_generate_keys");
+}
+
+extern "C" int mt_key_block_counts(const float* xy, const float* depths, int64_t V,
+                                   const int32_t* faces, int64_t F, const int32_t* num_faces_x3,
+                                   int32_t B, int32_t tile_h, int32_t tile_w, int32_t row_min,
+                                   int32_t col_min, int32_t row_max, int32_t col_max,
+                                   float near_plane, int32_t* block_counts, mt_stream_t stream) {
+  MT_REQUIRE(F >= 0 && B >= 0 && tile_h > 0 && tile_w > 0, "mt_key_block_counts: bad sizes");
+  if (F == 0 || B == 0) return MT_OK;
+  mt::key_block_counts_kernel<<<mt::blocks_for(F, mt::kBlock), mt::kBlock, 0,
+                                static_cast<hipStream_t>(stream)>>>(
+      xy, depths, V, faces, F, B, tile_h, tile_w, row_min, col_min, row_max, col_max, near_plane,
+      num_faces_x3, block_counts);
+  return mt::check_launch("mt_key_block_counts");
+}
+
+extern "C" int mt_generate_keys_from_geometry(
+    const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
+    const int32_t* num_faces_x3, const int32_t* shell_face_boundaries, int32_t S, int32_t B,
+    int32_t tile_h, int32_t tile_w, int32_t row_min, int32_t col_min, int32_t row_max,
+    int32_t col_max, float near_plane, const int64_t* block_offsets, int32_t grid_cols,
+    mt_key_layout layout, int64_t* keys, int32_t* values, int64_t key_capacity, int32_t* overflow,
+    mt_stream_t stream) {
+  MT_REQUIRE(F >= 0 && B >= 0 && B <= 65535 && key_capacity >= 0 && tile_h > 0 && tile_w > 0,
+             "mt_generate_keys_from_geometry: bad sizes");
+  if (F == 0 || B == 0) return MT_OK;
+  dim3 grid(mt::blocks_for(F, mt::kBlock), static_cast<unsigned>(B));
+  mt::keys_from_geometry_kernel<<<grid, mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+      xy, depths, V, faces, F, num_faces_x3, shell_face_boundaries, S, tile_h, tile_w, row_min,
+      col_min, row_max, col_max, near_plane, block_offsets, grid_cols, mt::to_fields(layout), keys,
+      values, key_capacity, overflow);
+  return mt::check_launch("mt_generate_keys_from_geometry");
 }
 
 extern "C" int mt_delineate_tiles(const int64_t* sorted_keys, int64_t n, mt_key_layout layout,

@@ -1,8 +1,10 @@
 // Stable LSD radix sort of (int64 key, int32 value) pairs in signed key order:
-// onesweep, 8-bit digits, wave64 multi-split ranking, decoupled look-back across tiles.
+// onesweep, 8-bit digits over the key bits that vary, wave64 multi-split ranking,
+// decoupled look-back across tiles.
 //
 //   init       zero the digit histograms, tickets and look-back words
-//   histogram  one read of all keys -> counts of every digit position (8 x 256)
+//   mask       one read of all keys -> OR and AND of every key -> the bits that vary
+//   histogram  one read of all keys -> counts of every digit of every pass
 //   prepare    exclusive digit offsets per pass; a pass whose digit is the same for every
 //              key is marked inactive; buffer routing so that the last active pass lands
 //              in the output (input -> [tmp <-> out] ... -> out)
@@ -11,16 +13,19 @@
 //              look-back array, reordered in LDS by digit and written as contiguous runs.
 //   copy       only if no pass was active (all keys equal): out = in
 //
-// All decisions are taken on the device, so the sequence is graph-capturable and never
-// synchronises the host.
+// Constant key bits cannot change the order, so the digits are taken from the varying
+// bits only, packed from the least significant one up (pass p sorts varying bits
+// 8p .. 8p + 7). Typical fused keys (camera | tile | depth | shell) have ~48 varying bits:
+// 6 passes instead of 8. All decisions are taken on the device, so the sequence is
+// graph-capturable and never synchronises the host.
 #include "common.h"
 
 namespace mt {
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 512;
 constexpr int kWaves = kBlock / kWave;
-constexpr int kItems = 16;
+constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems;
 constexpr int kRadix = 256;
 constexpr int kPasses = 8;
@@ -32,13 +37,14 @@ constexpr uint64_t kStateMask = 3ull << 62;
 constexpr uint64_t kValueMask = (1ull << 59) - 1;
 constexpr uint64_t kTagMask = ~kStateMask & ~kValueMask;
 // Predecessor look-back words loaded at once per digit (independent loads in flight).
-constexpr int kLookback = 16;
+constexpr int kLookback = 8;
 
 struct SortWorkspace {
   unsigned long long* hist;  // [8][256]
   int64_t* digit_base;       // [8][256]
   int32_t* info;             // [0..7] active, [8..15] src, [16..23] dst, [24] num_active
   uint32_t* tickets;         // [8]
+  unsigned long long* bits;  // [0] OR of all keys, [1] AND of all keys
   uint64_t* lookback;        // [tiles][256]
   int64_t* tmp_keys;
   int32_t* tmp_values;
@@ -51,6 +57,7 @@ size_t workspace_bytes(int64_t n) {
   size_t b = 0;
   b += align_up(kPasses * kRadix * 8, 256);
   b += align_up(kPasses * kRadix * 8, 256);
+  b += 256;
   b += 256;
   b += 256;
   b += align_up(size_t(tiles) * kRadix * 8, 256);
@@ -71,6 +78,8 @@ SortWorkspace carve(void* ws, int64_t n) {
   p += 256;
   w.tickets = reinterpret_cast<uint32_t*>(p);
   p += 256;
+  w.bits = reinterpret_cast<unsigned long long*>(p);
+  p += 256;
   w.lookback = reinterpret_cast<uint64_t*>(p);
   p += align_up(size_t(tiles) * kRadix * 8, 256);
   w.tmp_keys = reinterpret_cast<int64_t*>(p);
@@ -79,16 +88,66 @@ SortWorkspace carve(void* ws, int64_t n) {
   return w;
 }
 
-__device__ __forceinline__ uint32_t digit_of(uint64_t key, int pass) {
-  // Flipping the sign bit turns signed order into unsigned order.
-  return static_cast<uint32_t>(((key ^ (1ull << 63)) >> (8 * pass)) & 0xFFull);
+// The varying key bits (set = the bit differs between some keys) as runs of contiguous
+// bits: run r moves (key >> shift) & (2^width - 1) to bit dst of the compacted key, whose
+// bit i is the i-th varying bit counted from the least significant one.
+struct Runs {
+  int count;
+  uint32_t desc[32];  // shift | width << 8 | dst << 16
+};
+
+// Written by one thread straight into LDS (dynamic indexing stays out of registers).
+__device__ void make_runs(uint64_t v, Runs* r) {
+  int count = 0;
+  int dst = 0;
+  int b = 0;
+  while (b < 64) {
+    if (!((v >> b) & 1ull)) {
+      ++b;
+      continue;
+    }
+    int e = b;
+    while (e < 64 && ((v >> e) & 1ull)) ++e;
+    r->desc[count++] = uint32_t(b) | (uint32_t(e - b) << 8) | (uint32_t(dst) << 16);
+    dst += e - b;
+    b = e;
+  }
+  r->count = count;
 }
+
+// Compacted key (varying bits only, packed low). Flipping the sign bit first turns signed
+// order into unsigned order. Pass p sorts by bits 8p .. 8p + 7 of this value.
+__device__ __forceinline__ uint64_t compact_key(uint64_t key, const Runs& r) {
+  const uint64_t k = key ^ (1ull << 63);
+  uint64_t out = 0;
+  for (int i = 0; i < r.count; ++i) {
+    const uint32_t dsc = r.desc[i];
+    const uint32_t shift = dsc & 0xFFu, width = (dsc >> 8) & 0xFFu, dst = dsc >> 16;
+    const uint64_t m = width >= 64 ? ~0ull : ((1ull << width) - 1ull);
+    out |= ((k >> shift) & m) << dst;
+  }
+  return out;
+}
+
+__device__ __forceinline__ uint32_t digit_of(uint64_t key, const Runs& r, int pass) {
+  return static_cast<uint32_t>((compact_key(key, r) >> (8 * pass)) & 0xFFull);
+}
+
+__device__ __forceinline__ uint64_t varying_bits(const SortWorkspace& w) {
+  return w.bits[0] & ~w.bits[1];
+}
+
+__device__ __forceinline__ int passes_for(uint64_t v) { return (__popcll(v) + 7) / 8; }
 
 __global__ void init_kernel(SortWorkspace w, int64_t lookback_words) {
   const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = tid; i < kPasses * kRadix; i += stride) w.hist[i] = 0ull;
   if (tid < kPasses) w.tickets[tid] = 0u;
+  if (tid == 0) {
+    w.bits[0] = 0ull;
+    w.bits[1] = ~0ull;
+  }
   for (int64_t i = tid; i < lookback_words; i += stride) w.lookback[i] = 0ull;
 }
 
@@ -98,15 +157,40 @@ __device__ __forceinline__ int64_t live_count(int64_t n, const int64_t* n_device
   return d < n ? (d < 0 ? 0 : d) : n;
 }
 
-__global__ void __launch_bounds__(kBlock) histogram_kernel(const int64_t* keys, int64_t n,
-                                                           SortWorkspace w, const int64_t* n_device) {
+__global__ void __launch_bounds__(256) mask_kernel(const int64_t* keys, int64_t n, SortWorkspace w,
+                                                   const int64_t* n_device) {
+  n = live_count(n, n_device);
+  uint64_t o = 0ull, a = ~0ull;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t k = static_cast<uint64_t>(keys[i]);
+    o |= k;
+    a &= k;
+  }
+#pragma unroll
+  for (int d = kWave / 2; d >= 1; d >>= 1) {
+    o |= __shfl_xor(o, d, kWave);
+    a &= __shfl_xor(a, d, kWave);
+  }
+  if (__lane_id() == 0) {
+    atomicOr(&w.bits[0], static_cast<unsigned long long>(o));
+    atomicAnd(&w.bits[1], static_cast<unsigned long long>(a));
+  }
+}
+
+__global__ void __launch_bounds__(256) histogram_kernel(const int64_t* keys, int64_t n,
+                                                        SortWorkspace w, const int64_t* n_device) {
   n = live_count(n, n_device);
   __shared__ uint32_t s_hist[kPasses][kRadix];
-  for (int i = threadIdx.x; i < kPasses * kRadix; i += kBlock) (&s_hist[0][0])[i] = 0u;
+  __shared__ Runs s_runs;
+  const uint64_t v = varying_bits(w);
+  const int passes = passes_for(v);
+  for (int i = threadIdx.x; i < kPasses * kRadix; i += blockDim.x) (&s_hist[0][0])[i] = 0u;
+  if (threadIdx.x == 0) make_runs(v, &s_runs);
   __syncthreads();
-  // Each thread walks its keys keeping, per digit position, the current digit and its run
-  // length, and flushes a run when the digit changes: uniform high digits cost one
-  // atomic per run instead of one per key.
+  // Each thread walks its keys keeping, per pass, the current digit and its run length,
+  // and flushes a run when the digit changes: slowly varying digits cost one LDS atomic
+  // per run instead of one per key.
   uint32_t cur[kPasses];
   uint32_t run[kPasses];
 #pragma unroll
@@ -114,35 +198,38 @@ __global__ void __launch_bounds__(kBlock) histogram_kernel(const int64_t* keys, 
     cur[p] = 0u;
     run[p] = 0u;
   }
-  const int64_t stride = int64_t(gridDim.x) * kBlock;
-  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
-    const uint64_t k = static_cast<uint64_t>(keys[i]);
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t ck = compact_key(static_cast<uint64_t>(keys[i]), s_runs);
 #pragma unroll
     for (int p = 0; p < kPasses; ++p) {
-      const uint32_t d = digit_of(k, p);
-      if (run[p] != 0u && d != cur[p]) {
-        atomicAdd(&s_hist[p][cur[p]], run[p]);
-        run[p] = 0u;
+      if (p < passes) {
+        const uint32_t d = static_cast<uint32_t>((ck >> (8 * p)) & 0xFFull);
+        if (run[p] != 0u && d != cur[p]) {
+          atomicAdd(&s_hist[p][cur[p]], run[p]);
+          run[p] = 0u;
+        }
+        cur[p] = d;
+        ++run[p];
       }
-      cur[p] = d;
-      ++run[p];
     }
   }
 #pragma unroll
   for (int p = 0; p < kPasses; ++p)
     if (run[p]) atomicAdd(&s_hist[p][cur[p]], run[p]);
   __syncthreads();
-  for (int i = threadIdx.x; i < kPasses * kRadix; i += kBlock) {
+  for (int i = threadIdx.x; i < passes * kRadix; i += blockDim.x) {
     const uint32_t c = (&s_hist[0][0])[i];
     if (c) atomicAdd(&w.hist[i], static_cast<unsigned long long>(c));
   }
 }
 
-__global__ void __launch_bounds__(kBlock) prepare_kernel(SortWorkspace w, int64_t n,
+__global__ void __launch_bounds__(kRadix) prepare_kernel(SortWorkspace w, int64_t n,
                                                          const int64_t* n_device) {
   n = live_count(n, n_device);
   __shared__ int32_t s_active[kPasses];
-  __shared__ int64_t s_wave[kWaves];
+  __shared__ int64_t s_wave[kRadix / kWave];
+  const int passes = passes_for(varying_bits(w));
   const int d = threadIdx.x;
   for (int p = 0; p < kPasses; ++p) {
     const int64_t c = static_cast<int64_t>(w.hist[p * kRadix + d]);
@@ -156,7 +243,7 @@ __global__ void __launch_bounds__(kBlock) prepare_kernel(SortWorkspace w, int64_
     for (int q = 0; q < wave; ++q) before += s_wave[q];
     w.digit_base[p * kRadix + d] = before + inc - c;
     const int full = __syncthreads_or(c == n);
-    if (d == 0) s_active[p] = full ? 0 : 1;
+    if (d == 0) s_active[p] = (p < passes && !full) ? 1 : 0;
     __syncthreads();
   }
   if (d == 0) {
@@ -206,17 +293,19 @@ __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys
   int32_t* dst_v = dst_sel == 1 ? out_values : w.tmp_values;
 
   __shared__ uint32_t s_wave_count[kWaves][kRadix];
-  __shared__ uint32_t s_hist[kRadix];
   __shared__ uint32_t s_local_start[kRadix];
   __shared__ int64_t s_global_base[kRadix];
-  __shared__ int64_t s_keys[kTile];
-  __shared__ int32_t s_vals[kTile];
+  __shared__ int64_t s_buf[kTile];  // keys, then (as int32) values, in digit order
+  __shared__ uint8_t s_dig[kTile];
   __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_scan_wave[kWaves];
+  __shared__ uint32_t s_scan_wave[kRadix / kWave];
+  __shared__ Runs s_runs;
 
   for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&s_wave_count[0][0])[i] = 0u;
-  s_hist[threadIdx.x] = 0u;
-  if (threadIdx.x == 0) s_tile = atomicAdd(&w.tickets[pass], 1u);
+  if (threadIdx.x == 0) {
+    s_tile = atomicAdd(&w.tickets[pass], 1u);
+    make_runs(varying_bits(w), &s_runs);
+  }
   __syncthreads();
   const int64_t tile = s_tile;
   const int64_t base = tile * kTile;
@@ -225,7 +314,7 @@ __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys
   const int lane = __lane_id();
   const uint64_t lt = lanemask_lt();
 
-  // Items: wave w owns [base + w * 1024, base + (w + 1) * 1024), item i of lane l is at
+  // Items: wave w owns [base + w * 512, base + (w + 1) * 512), item i of lane l is at
   // offset i * 64 + l, so (wave, item, lane) order is the input order.
   int64_t keys[kItems];
   int32_t vals[kItems];
@@ -238,21 +327,11 @@ __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys
     const bool valid = idx < n;
     keys[i] = valid ? src_k[idx] : 0;
     vals[i] = valid ? src_v[idx] : 0;
-    digits[i] = valid ? digit_of(static_cast<uint64_t>(keys[i]), pass) : kRadix;
+    digits[i] = valid ? digit_of(static_cast<uint64_t>(keys[i]), s_runs, pass) : kRadix;
   }
-  // 1. Tile histogram first, published before the (longer) ranking so that successors'
-  //    look-back finds this tile's aggregate early.
-#pragma unroll
-  for (int i = 0; i < kItems; ++i)
-    if (digits[i] < kRadix) atomicAdd(&s_hist[digits[i]], 1u);
-  __syncthreads();
-  const int d = threadIdx.x;
-  const uint32_t tile_count = s_hist[d];
-  const uint64_t tag = uint64_t(pass) << 59;
-  uint64_t* lb = w.lookback;
-  lb_store(&lb[tile * kRadix + d], (tile == 0 ? kStatePrefix : kStateAggregate) | tag | tile_count);
-
-  // 2. Stable rank within each wave: peers with the same digit via 8 ballots.
+  // 1. Stable rank within each wave: peers with the same digit via 8 ballots; the first
+  //    lane of each peer group adds the group's size to the wave's digit count (one LDS
+  //    update per distinct digit, no atomics).
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     const bool valid = digits[i] < kRadix;
@@ -271,77 +350,101 @@ __global__ void __launch_bounds__(kBlock) onesweep_kernel(const int64_t* in_keys
   }
   __syncthreads();
 
-  // Per digit: offsets of each wave inside the tile's run of that digit.
-  uint32_t running = 0;
+  // 2. Tile histogram = sum of the wave counts; publish it at once for the successors'
+  //    look-back, then turn the wave counts into each wave's offset inside the digit run.
+  const int d = threadIdx.x;  // digit owned by threads < kRadix
+  const uint64_t tag = uint64_t(pass) << 59;
+  uint64_t* lb = w.lookback;
+  uint32_t tile_count = 0;
+  if (d < kRadix) {
 #pragma unroll
-  for (int q = 0; q < kWaves; ++q) {
-    const uint32_t c = s_wave_count[q][d];
-    s_wave_count[q][d] = running;
-    running += c;
+    for (int q = 0; q < kWaves; ++q) {
+      const uint32_t c = s_wave_count[q][d];
+      s_wave_count[q][d] = tile_count;
+      tile_count += c;
+    }
+    lb_store(&lb[tile * kRadix + d], (tile == 0 ? kStatePrefix : kStateAggregate) | tag | tile_count);
   }
-  // Exclusive scan of the tile counts over digits: start of each digit's run in LDS.
-  uint32_t inc = wave_inclusive_sum(tile_count);
-  if (lane == kWave - 1) s_scan_wave[wave] = inc;
 
-  // 3. Look-back over preceding tiles for this digit, kLookback words per round.
   int64_t prefix = 0;
-  if (tile > 0) {
-    int64_t t = tile - 1;
-    bool done = false;
-    while (!done) {
-      uint64_t words[kLookback];
+  uint32_t inc = 0;
+  if (d < kRadix) {
+    // Exclusive scan of the tile counts over digits: start of each digit's run in LDS.
+    inc = wave_inclusive_sum(tile_count);
+    if (lane == kWave - 1) s_scan_wave[wave] = inc;
+
+    // 3. Look-back over preceding tiles for this digit, kLookback words per round.
+    if (tile > 0) {
+      int64_t t = tile - 1;
+      bool done = false;
+      while (!done) {
+        uint64_t words[kLookback];
 #pragma unroll
-      for (int j = 0; j < kLookback; ++j)
-        words[j] = (t - j >= 0) ? lb_load(&lb[(t - j) * kRadix + d]) : (kStatePrefix | tag);
-      bool stop = false;
-      int consumed = 0;
+        for (int j = 0; j < kLookback; ++j)
+          words[j] = (t - j >= 0) ? lb_load(&lb[(t - j) * kRadix + d]) : (kStatePrefix | tag);
+        bool stop = false;
+        int consumed = 0;
 #pragma unroll
-      for (int j = 0; j < kLookback; ++j) {
-        const uint64_t wd = words[j];
-        const bool valid = (wd & kStateMask) != 0 && (wd & kTagMask) == tag;
-        if (!stop) {
-          if (!valid) {
-            stop = true;  // not published yet: poll again from this predecessor
-          } else {
-            prefix += static_cast<int64_t>(wd & kValueMask);
-            ++consumed;
-            if ((wd & kStateMask) == kStatePrefix) {
-              stop = true;
-              done = true;
+        for (int j = 0; j < kLookback; ++j) {
+          const uint64_t wd = words[j];
+          const bool valid = (wd & kStateMask) != 0 && (wd & kTagMask) == tag;
+          if (!stop) {
+            if (!valid) {
+              stop = true;  // not published yet: poll again from this predecessor
+            } else {
+              prefix += static_cast<int64_t>(wd & kValueMask);
+              ++consumed;
+              if ((wd & kStateMask) == kStatePrefix) {
+                stop = true;
+                done = true;
+              }
             }
           }
         }
+        t -= consumed;
       }
-      t -= consumed;
+      lb_store(&lb[tile * kRadix + d], kStatePrefix | tag | uint64_t(prefix + tile_count));
     }
-    lb_store(&lb[tile * kRadix + d], kStatePrefix | tag | uint64_t(prefix + tile_count));
   }
   __syncthreads();
-  uint32_t wave_before = 0;
-  for (int q = 0; q < wave; ++q) wave_before += s_scan_wave[q];
-  const uint32_t local_start = wave_before + inc - tile_count;
-  s_local_start[d] = local_start;
-  s_global_base[d] = w.digit_base[pass * kRadix + d] + prefix;
+  if (d < kRadix) {
+    uint32_t wave_before = 0;
+    for (int q = 0; q < wave; ++q) wave_before += s_scan_wave[q];
+    s_local_start[d] = wave_before + inc - tile_count;
+    s_global_base[d] = w.digit_base[pass * kRadix + d] + prefix;
+  }
   __syncthreads();
 
-  // Reorder the tile by digit in LDS.
+  // 4. Reorder the keys by digit in LDS and write them as contiguous runs; then the same
+  //    for the values through the same buffer.
+  const int valid_count = static_cast<int>((n - base) < kTile ? (n - base) : kTile);
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     if (digits[i] < kRadix) {
       const uint32_t dd = digits[i];
       const uint32_t pos = s_local_start[dd] + s_wave_count[wave][dd] + ranks[i];
-      s_keys[pos] = keys[i];
-      s_vals[pos] = vals[i];
+      s_buf[pos] = keys[i];
+      s_dig[pos] = static_cast<uint8_t>(dd);
     }
   }
   __syncthreads();
-  const int64_t valid_count = (n - base) < kTile ? (n - base) : kTile;
   for (int j = threadIdx.x; j < valid_count; j += kBlock) {
-    const int64_t k = s_keys[j];
-    const uint32_t dd = digit_of(static_cast<uint64_t>(k), pass);
-    const int64_t g = s_global_base[dd] + (j - static_cast<int64_t>(s_local_start[dd]));
-    dst_k[g] = k;
-    dst_v[g] = s_vals[j];
+    const uint32_t dd = s_dig[j];
+    dst_k[s_global_base[dd] + (j - static_cast<int64_t>(s_local_start[dd]))] = s_buf[j];
+  }
+  __syncthreads();
+  int32_t* s_vbuf = reinterpret_cast<int32_t*>(s_buf);
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    if (digits[i] < kRadix) {
+      const uint32_t dd = digits[i];
+      s_vbuf[s_local_start[dd] + s_wave_count[wave][dd] + ranks[i]] = vals[i];
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < valid_count; j += kBlock) {
+    const uint32_t dd = s_dig[j];
+    dst_v[s_global_base[dd] + (j - static_cast<int64_t>(s_local_start[dd]))] = s_vbuf[j];
   }
 }
 
@@ -355,6 +458,12 @@ __global__ void copy_if_idle_kernel(const int64_t* in_keys, const int32_t* in_va
     out_keys[i] = in_keys[i];
     out_values[i] = in_values[i];
   }
+}
+
+unsigned capped_blocks(int64_t n, int64_t per_block, int64_t cap) {
+  int64_t b = (n + per_block - 1) / per_block;
+  b = b < 1 ? 1 : (b > cap ? cap : b);
+  return static_cast<unsigned>(b);
 }
 
 }  // namespace
@@ -380,18 +489,15 @@ extern "C" int mt_sort_keys(const int64_t* keys_in, const int32_t* values_in, in
   SortWorkspace w = carve(workspace, n);
   const int64_t tiles = (n + kTile - 1) / kTile;
   const int64_t lb_words = tiles * kRadix;
-  init_kernel<<<blocks_for(lb_words < 4096 ? 4096 : lb_words, 256 * 8) , 256, 0, s>>>(w, lb_words);
-  const unsigned hist_blocks = static_cast<unsigned>(
-      (n + int64_t(kBlock) * 64 - 1) / (int64_t(kBlock) * 64) < 2048
-          ? (n + int64_t(kBlock) * 64 - 1) / (int64_t(kBlock) * 64)
-          : 2048);
-  histogram_kernel<<<hist_blocks, kBlock, 0, s>>>(keys_in, n, w, n_device);
-  prepare_kernel<<<1, kBlock, 0, s>>>(w, n, n_device);
+  init_kernel<<<capped_blocks(lb_words, 256 * 8, 1 << 20), 256, 0, s>>>(w, lb_words);
+  mask_kernel<<<capped_blocks(n, 256 * 16, 4096), 256, 0, s>>>(keys_in, n, w, n_device);
+  histogram_kernel<<<capped_blocks(n, 256 * 32, 4096), 256, 0, s>>>(keys_in, n, w, n_device);
+  prepare_kernel<<<1, kRadix, 0, s>>>(w, n, n_device);
   for (int p = 0; p < kPasses; ++p) {
     onesweep_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, s>>>(keys_in, values_in, keys_out,
                                                                    values_out, n, p, w, n_device);
   }
-  copy_if_idle_kernel<<<blocks_for(n < 65536 * 4 ? n : 65536 * 4, 256 * 4), 256, 0, s>>>(
+  copy_if_idle_kernel<<<capped_blocks(n, 256 * 4, 65536), 256, 0, s>>>(
       keys_in, values_in, keys_out, values_out, n, w.info, n_device);
   return check_launch("mt_sort_keys");
 }

@@ -6,6 +6,7 @@
 // t = clamp((iso - d0) / (d1 - d0), 0, 1) and written at
 // vertex_offsets[s * N + c] + (number of crossing axes before this one).
 #include "common.h"
+#include "sh.h"
 
 #define MT_MC_TABLE_QUALIFIER static __device__ const
 #include "mc_tables.h"
@@ -298,6 +299,164 @@ __global__ void __launch_bounds__(kBlock) gather_vertices_backward_kernel(
   }
 }
 
+// ------------------------------------------------------------------------------------
+// Batched training step: triangle vertices fused with their colours, and the adjoint of
+// both. The per-vertex SH array [NSH, V, 3] is never stored: the forward interpolates each
+// vertex's coefficients from the grid while it has the vertex in registers and writes only
+// the colours of the B cameras; the adjoint recomputes the SH gradient of each triangle
+// vertex from its colour gradient while gathering it into the grid. The operations and
+// their order are those of the separate kernels (create_vertices_kernel, then
+// sh_forward_kernel / sh_backward_kernel, then gather_vertices_backward_kernel), so
+// colours are bit-identical and gradients equal up to the sign of zero terms.
+// ------------------------------------------------------------------------------------
+
+template <int kDeg>
+__global__ void __launch_bounds__(kBlock) create_vertices_colors_kernel(
+    const float* gv, const float* gsdf, const float* gsh, int64_t ng, const int32_t* lower,
+    int64_t n, const float* level_sets, int S, const int32_t* offsets, int64_t nt,
+    const float* cams, int B, float* ov, float* od, uint8_t* otype, float* colors,
+    int32_t* overflow) {
+  constexpr int kN = (kDeg + 1) * (kDeg + 1);
+  __shared__ int32_t s_v0[3 * kBlock], s_v1[3 * kBlock];
+  __shared__ float s_t[3 * kBlock], s_d0[3 * kBlock], s_d1[3 * kBlock];
+  __shared__ uint8_t s_axis[3 * kBlock];
+  __shared__ int s_total;
+  const int64_t c_first = int64_t(blockIdx.x) * kBlock;
+  const int64_t c = c_first + threadIdx.x;
+  const int64_t c_last = (c_first + kBlock < n ? c_first + kBlock : n) - 1;
+  const bool valid = c < n;
+  ColumnSdf col{};
+  if (valid) col = load_column(gsdf, lower, n, c);
+  for (int s = 0; s < S; ++s) {
+    const float iso = level_sets[s];
+    const int64_t base = offsets[int64_t(s) * n + c_first];
+    if (valid) {
+      int pos = static_cast<int>(offsets[int64_t(s) * n + c] - base);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (!crosses(col.d0, col.d[a], iso)) continue;
+        s_v0[pos] = col.v0;
+        s_v1[pos] = col.v[a];
+        s_t[pos] = edge_t(col.d0, col.d[a], iso);
+        s_d0[pos] = col.d0;
+        s_d1[pos] = col.d[a];
+        s_axis[pos] = static_cast<uint8_t>(a);
+        ++pos;
+      }
+      if (c == c_last) s_total = pos;
+    }
+    __syncthreads();
+    const int total = s_total;
+    if (base + total > nt && threadIdx.x == 0 && overflow) atomicOr(overflow, 1);
+    const int live = static_cast<int>(base >= nt ? 0 : (base + total <= nt ? total : nt - base));
+    for (int v = threadIdx.x; v < live; v += kBlock) {
+      const int64_t idx = base + v;
+      const int64_t g0 = s_v0[v], g1 = s_v1[v];
+      const float t = s_t[v];
+      const float x0 = gv[3 * g0 + 0], y0 = gv[3 * g0 + 1], z0 = gv[3 * g0 + 2];
+      const float x1 = gv[3 * g1 + 0], y1 = gv[3 * g1 + 1], z1 = gv[3 * g1 + 2];
+      const float3 p = make_float3(x0 + t * (x1 - x0), y0 + t * (y1 - y0), z0 + t * (z1 - z0));
+      ov[3 * idx + 0] = p.x;
+      ov[3 * idx + 1] = p.y;
+      ov[3 * idx + 2] = p.z;
+      od[idx] = s_d0[v] + t * (s_d1[v] - s_d0[v]);
+      otype[idx] = s_axis[v];
+      float coef[kN][3];
+#pragma unroll
+      for (int k = 0; k < kN; ++k) {
+        const float* plane = gsh + int64_t(k) * ng * 3;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+          const float a = plane[3 * g0 + ch], b = plane[3 * g1 + ch];
+          coef[k][ch] = a + t * (b - a);
+        }
+      }
+      for (int b = 0; b < B; ++b) {
+        const float3 d = view_dir(p, cams + 3 * b);
+        float basis[16];
+        sh_basis(d.x, d.y, d.z, kDeg, basis);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch)
+          colors[(int64_t(b) * nt + idx) * 3 + ch] = fmaxf(sh_channel_pre<kN>(basis, coef, ch), 0.0f);
+      }
+    }
+    __syncthreads();  // the next level set reuses the lists
+  }
+}
+
+template <int kNsh, int kDeg>
+__global__ void __launch_bounds__(kBlock) gather_colors_backward_kernel(
+    const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
+    int S, const int32_t* offsets, int64_t nt, const float* tv, const float* colors,
+    const float* cams, int B, const float* g_d, const float* g_col, float* gg_d, float* gg_sh,
+    const int32_t* maps, int32_t* status) {
+  constexpr int kN = (kDeg + 1) * (kDeg + 1);
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (maps[4 * ng] != 0) {  // the columns do not map one-to-one onto grid vertices
+    if (g == 0 && status) atomicOr(status, 2);
+    return;
+  }
+  if (g >= ng) return;
+  float acc_d = 0.f;
+  float acc_sh[kN][3];
+#pragma unroll
+  for (int k = 0; k < kN; ++k) acc_sh[k][0] = acc_sh[k][1] = acc_sh[k][2] = 0.f;
+  for (int role = 0; role < 4; ++role) {
+    const int32_t c = maps[role * ng + g];
+    if (c < 0) continue;
+    const ColumnSdf col = load_column(gsdf, lower, n, c);
+    for (int s = 0; s < S; ++s) {
+      const float iso = level_sets[s];
+      bool cr[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) cr[a] = crosses(col.d0, col.d[a], iso);
+      int64_t idx = offsets[s * n + c];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (!cr[a]) continue;
+        const int64_t here = idx++;
+        if (role != 0 && role != a + 1) continue;
+        if (here >= nt) continue;
+        const float t = edge_t(col.d0, col.d[a], iso);
+        const float wgt = (role == 0) ? 1.0f - t : t;
+        acc_d += wgt * g_d[here];
+        // SH gradient of this triangle vertex (sh_backward_kernel's sum over cameras).
+        float gt[kN][3];
+#pragma unroll
+        for (int k = 0; k < kN; ++k) gt[k][0] = gt[k][1] = gt[k][2] = 0.f;
+        const float3 p = vertex_at(tv, here);
+        for (int b = 0; b < B; ++b) {
+          const float3 d = view_dir(p, cams + 3 * b);
+          float basis[16];
+          sh_basis(d.x, d.y, d.z, kDeg, basis);
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) {
+            const int64_t q = (int64_t(b) * nt + here) * 3 + ch;
+            // relu passes gradient where the colour is > 0 (pre-activation > 0).
+            const float gc = colors[q] > 0.0f ? g_col[q] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < kN; ++k) gt[k][ch] += basis[k] * gc;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < kN; ++k) {
+          acc_sh[k][0] += wgt * gt[k][0];
+          acc_sh[k][1] += wgt * gt[k][1];
+          acc_sh[k][2] += wgt * gt[k][2];
+        }
+      }
+    }
+  }
+  gg_d[g] = acc_d;
+#pragma unroll
+  for (int k = 0; k < kNsh; ++k) {
+    float* o = gg_sh + (int64_t(k) * ng + g) * 3;
+    o[0] = k < kN ? acc_sh[k < kN ? k : 0][0] : 0.f;
+    o[1] = k < kN ? acc_sh[k < kN ? k : 0][1] : 0.f;
+    o[2] = k < kN ? acc_sh[k < kN ? k : 0][2] : 0.f;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) count_faces_kernel(const float* sdf, const int32_t* lower,
                                                              int64_t ncol, const int32_t* upper,
                                                              int64_t n, const float* level_sets,
@@ -325,25 +484,47 @@ __global__ void __launch_bounds__(kBlock) count_faces_kernel(const float* sdf, c
   }
 }
 
+// A block owns 256 consecutive voxels. For each level set their face corners occupy one
+// contiguous output range (face offsets are a scan in (level set, voxel) order), so the
+// corners are spread evenly over the block's threads: each thread finds its corner's
+// voxel by binary search over the voxels' corner offsets in LDS, resolves the corner's
+// triangle vertex, and consecutive threads write consecutive corners.
 __global__ void __launch_bounds__(kBlock) create_faces_kernel(
     const int32_t* neighbors, const int32_t* indices, int64_t n, int64_t ncol,
     const int32_t* voffsets, const uint8_t* vtypes, const uint8_t* codes, int S,
     const int32_t* foffsets, int32_t* faces, int32_t* voxel_indices, int64_t fcap,
     int64_t vcap, int32_t* overflow) {
-  const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  const int32_t linear = indices[v];
+  __shared__ int32_t s_start[kBlock];
+  __shared__ uint8_t s_code[kBlock];
+  __shared__ int s_total;
+  const int64_t v_first = int64_t(blockIdx.x) * kBlock;
+  const int last = static_cast<int>((v_first + kBlock < n ? v_first + kBlock : n) - 1 - v_first);
+  const int64_t v = v_first + threadIdx.x;
+  const bool valid = int(threadIdx.x) <= last;
   for (int s = 0; s < S; ++s) {
-    const int code = codes[s * n + v];
-    const int first = MT_MC_CASE_START[code];
-    const int num = MT_MC_CASE_START[code + 1] - first;
-    if (num == 0) continue;
-    const int64_t base = foffsets[s * n + v];
-    for (int q = 0; q < num; ++q) {
-      const int e = MT_MC_CASE_EDGES[first + q];
+    const int64_t base = foffsets[int64_t(s) * n + v_first];
+    if (valid) {
+      const int code = codes[int64_t(s) * n + v];
+      const int start = static_cast<int>(foffsets[int64_t(s) * n + v] - base);
+      s_code[threadIdx.x] = static_cast<uint8_t>(code);
+      s_start[threadIdx.x] = start;
+      if (int(threadIdx.x) == last) s_total = start + (MT_MC_CASE_START[code + 1] - MT_MC_CASE_START[code]);
+    }
+    __syncthreads();
+    const int total = s_total;
+    for (int i = threadIdx.x; i < total; i += kBlock) {
+      int lo = 0, hi = last;  // last voxel whose first corner is <= i (it has corners)
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_start[mid] <= i) lo = mid; else hi = mid - 1;
+      }
+      const int q = i - s_start[lo];
+      const int code = s_code[lo];
+      const int e = MT_MC_CASE_EDGES[MT_MC_CASE_START[code] + q];
       const int slot = MT_MC_EDGE_OWNER[e][0];
       const int axis = MT_MC_EDGE_OWNER[e][1];
-      const int64_t owner = (slot < 0) ? v : neighbors[slot * n + v];
+      const int64_t vv = v_first + lo;
+      const int64_t owner = (slot < 0) ? vv : neighbors[slot * n + vv];
       const int64_t vo = s * ncol + owner;
       const int32_t start = voffsets[vo];
       int32_t cnt = voffsets[vo + 1] - start;
@@ -358,7 +539,7 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
           break;
         }
       }
-      const int64_t pos = base + q;
+      const int64_t pos = base + i;
       const int64_t tri = pos / 3;
       const int corner = 2 - static_cast<int>(pos % 3);
       if (tri >= fcap) {
@@ -366,8 +547,9 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
         continue;
       }
       faces[3 * tri + corner] = target;
-      if (corner == 0) voxel_indices[tri] = linear;
+      if (corner == 0) voxel_indices[tri] = indices[vv];
     }
+    __syncthreads();  // the next level set reuses the LDS lists
   }
 }
 
@@ -488,4 +670,85 @@ extern "C" int mt_create_triangle_faces(const int32_t* neighbors, const int32_t*
       neighbors, indices, num_voxels, num_columns, vertex_offsets, vertex_types, cell_codes, S,
       face_offsets, faces, voxel_indices, face_capacity, vertex_capacity, overflow);
   return mt::check_launch("mt_create_triangle_faces");
+}
+
+extern "C" int mt_create_triangle_vertices_colors_forward(
+    const float* grid_vertices, const float* grid_sdf, const float* grid_sh,
+    int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree, const int32_t* lower_corners,
+    int64_t num_columns, const float* level_sets, int32_t S, const int32_t* vertex_offsets,
+    int64_t num_triangle_vertices, const float* camera_positions, int32_t B, float* out_vertices,
+    float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, mt_stream_t stream) {
+  MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0 && B >= 0,
+             "mt_create_triangle_vertices_colors_forward: negative size");
+  MT_REQUIRE(sh_degree >= 0 && sh_degree <= 3 && (sh_degree + 1) * (sh_degree + 1) <= num_sh,
+             "mt_create_triangle_vertices_colors_forward: degree %d needs (deg+1)^2 <= %d coefficients",
+             sh_degree, num_sh);
+  if (num_columns == 0 || S == 0 || num_triangle_vertices == 0) return MT_OK;
+  const dim3 grid(mt::blocks_for(num_columns, mt::kBlock));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+#define MT_VCOL(D)                                                                              \
+  mt::create_vertices_colors_kernel<D><<<grid, mt::kBlock, 0, st>>>(                           \
+      grid_vertices, grid_sdf, grid_sh, num_grid_vertices, lower_corners, num_columns,         \
+      level_sets, S, vertex_offsets, num_triangle_vertices, camera_positions, B, out_vertices, \
+      out_sdf, out_types, out_colors, overflow)
+  switch (sh_degree) {
+    case 0: MT_VCOL(0); break;
+    case 1: MT_VCOL(1); break;
+    case 2: MT_VCOL(2); break;
+    default: MT_VCOL(3); break;
+  }
+#undef MT_VCOL
+  return mt::check_launch("mt_create_triangle_vertices_colors_forward");
+}
+
+extern "C" int mt_create_triangle_vertices_colors_backward(
+    const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree,
+    const int32_t* lower_corners, int64_t num_columns, const float* level_sets, int32_t S,
+    const int32_t* vertex_offsets, int64_t num_triangle_vertices, const float* vertices,
+    const float* colors, const float* camera_positions, int32_t B, const float* grad_sdf,
+    const float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
+    size_t workspace_bytes, int32_t* status, mt_stream_t stream) {
+  MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0 && B >= 0,
+             "mt_create_triangle_vertices_colors_backward: negative size");
+  MT_REQUIRE(num_sh == 1 || num_sh == 4 || num_sh == 9 || num_sh == 16,
+             "mt_create_triangle_vertices_colors_backward: %d SH coefficients (need 1, 4, 9 or 16)",
+             num_sh);
+  MT_REQUIRE(sh_degree >= 0 && (sh_degree + 1) * (sh_degree + 1) <= num_sh,
+             "mt_create_triangle_vertices_colors_backward: degree %d needs (deg+1)^2 <= %d",
+             sh_degree, num_sh);
+  const size_t need = mt_create_triangle_vertices_backward_workspace_bytes(num_grid_vertices);
+  if (workspace == nullptr || workspace_bytes < need) {
+    mt::set_error("mt_create_triangle_vertices_colors_backward: workspace of %zu bytes is smaller than the %zu required",
+                  workspace_bytes, need);
+    return MT_ERR_WORKSPACE_TOO_SMALL;
+  }
+  if (num_grid_vertices == 0) return MT_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int32_t* maps = static_cast<int32_t*>(workspace);
+  const int64_t words = 4 * num_grid_vertices;
+  mt::fill_maps_kernel<<<mt::blocks_for(words, 256), 256, 0, st>>>(maps, words);
+  if (num_columns > 0)
+    mt::build_maps_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
+        lower_corners, num_columns, num_grid_vertices, maps);
+  const dim3 grid(mt::blocks_for(num_grid_vertices, mt::kBlock));
+#define MT_GCOL(N, D)                                                                          \
+  mt::gather_colors_backward_kernel<N, D><<<grid, mt::kBlock, 0, st>>>(                       \
+      grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S, vertex_offsets, \
+      num_triangle_vertices, vertices, colors, camera_positions, B, grad_sdf, grad_colors,    \
+      grid_sdf_grad, grid_sh_grad, maps, status)
+  const int key = num_sh * 4 + sh_degree;
+  switch (key) {
+    case 1 * 4 + 0: MT_GCOL(1, 0); break;
+    case 4 * 4 + 0: MT_GCOL(4, 0); break;
+    case 4 * 4 + 1: MT_GCOL(4, 1); break;
+    case 9 * 4 + 0: MT_GCOL(9, 0); break;
+    case 9 * 4 + 1: MT_GCOL(9, 1); break;
+    case 9 * 4 + 2: MT_GCOL(9, 2); break;
+    case 16 * 4 + 0: MT_GCOL(16, 0); break;
+    case 16 * 4 + 1: MT_GCOL(16, 1); break;
+    case 16 * 4 + 2: MT_GCOL(16, 2); break;
+    default: MT_GCOL(16, 3); break;
+  }
+#undef MT_GCOL
+  return mt::check_launch("mt_create_triangle_vertices_colors_backward");
 }

@@ -1,0 +1,146 @@
+"""Fused vertex + colour kernels of the batched step against the separate kernels they
+replace: mt_create_triangle_vertices_colors_forward == mt_create_triangle_vertices_forward
+followed by mt_evaluate_sh_forward (bit for bit), and the fused adjoint == mt_evaluate_sh_backward
+followed by mt_create_triangle_vertices_backward (equal up to the sign of zero terms), plus the
+FP64 oracle adjoint within 1e-3 relative."""
+
+import numpy as np
+import pytest
+import torch
+
+from meshtryoshka_amd import _native
+from meshtryoshka_amd import triangle_extraction as te
+from meshtryoshka_amd import workloads
+from oracle import extraction as ox
+from oracle import rasterization as orr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", params=[3, 1])
+def case(request):
+    dev = torch.device("cuda")
+    spec = workloads.sphere_field(seed=5, num_spheres=6, resolution=64, num_cameras=3,
+                                  image_shape=(48, 64), sh_degree=request.param, device=dev)
+    vox = te.extract_voxels(spec.occupancy.to(dev))
+    world, sdf, sh = workloads.finish_fields(spec, vox.vertices)
+    ls = spec.level_sets.to(dev)
+    cams = spec.cameras.to(dev)
+    return spec, vox, world.to(dev), sdf.to(dev), sh.to(dev), ls, cams
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _offsets(vox, sdf, ls):
+    n = vox.lower_corners.shape[1]
+    s = ls.shape[0]
+    off = torch.zeros(s * n + 1, dtype=torch.int32, device=sdf.device)
+    _native.call("mt_count_triangle_vertices", sdf.data_ptr(), vox.lower_corners.data_ptr(), n,
+                 ls.data_ptr(), s, off.data_ptr(), _stream())
+    ws = _native.workspace(_native.query("mt_scan_workspace_bytes", off.numel()), sdf.device)
+    _native.call("mt_exclusive_scan_i32", off.data_ptr(), off.numel(), ws.data_ptr(), ws.numel(),
+                 _stream())
+    return off
+
+
+def test_fused_forward_and_adjoint_match_separate_kernels(case):
+    spec, vox, world, sdf, sh, ls, cams = case
+    dev = sdf.device
+    n, s, ng, nsh, deg = vox.lower_corners.shape[1], ls.shape[0], world.shape[0], sh.shape[0], spec.sh_degree
+    b = cams.extrinsics.shape[0]
+    off = _offsets(vox, sdf, ls)
+    V = int(off[-1])
+    f32 = dict(dtype=torch.float32, device=dev)
+    campos = torch.empty((b, 3), **f32)
+    _native.call("mt_camera_positions", cams.extrinsics.contiguous().data_ptr(), b, campos.data_ptr(), _stream())
+    # Separate kernels.
+    tv, td = torch.empty((V, 3), **f32), torch.empty(V, **f32)
+    tsh, tt = torch.empty((nsh, V, 3), **f32), torch.empty(V, dtype=torch.uint8, device=dev)
+    ov = torch.zeros(1, dtype=torch.int32, device=dev)
+    _native.call("mt_create_triangle_vertices_forward", world.data_ptr(), sdf.data_ptr(), sh.data_ptr(),
+                 ng, nsh, vox.lower_corners.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V,
+                 tv.data_ptr(), td.data_ptr(), tsh.data_ptr(), tt.data_ptr(), ov.data_ptr(), _stream())
+    col = torch.empty((b, V, 3), **f32)
+    _native.call("mt_evaluate_sh_forward", tv.data_ptr(), V, tsh.data_ptr(), nsh, deg,
+                 campos.data_ptr(), b, col.data_ptr(), _stream())
+    # Fused kernel.
+    tv2, td2 = torch.empty((V, 3), **f32), torch.empty(V, **f32)
+    tt2, col2 = torch.empty(V, dtype=torch.uint8, device=dev), torch.empty((b, V, 3), **f32)
+    _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(), sdf.data_ptr(),
+                 sh.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(), n, ls.data_ptr(), s,
+                 off.data_ptr(), V, campos.data_ptr(), b, tv2.data_ptr(), td2.data_ptr(),
+                 tt2.data_ptr(), col2.data_ptr(), ov.data_ptr(), _stream())
+    assert int(ov) == 0
+    assert torch.equal(tv, tv2) and torch.equal(td, td2) and torch.equal(tt, tt2)
+    assert torch.equal(col, col2)
+
+    # Adjoint: random upstream gradients for SDF and colours.
+    gen = torch.Generator("cpu").manual_seed(11)
+    g_d = torch.randn(V, generator=gen).to(dev)
+    g_col = torch.randn((b, V, 3), generator=gen).to(dev)
+    g_tsh = torch.zeros((nsh, V, 3), **f32)
+    _native.call("mt_evaluate_sh_backward", tv.data_ptr(), V, tsh.data_ptr(), nsh, deg,
+                 campos.data_ptr(), b, g_col.data_ptr(), g_tsh.data_ptr(), _stream())
+    ws = _native.workspace(_native.query("mt_create_triangle_vertices_backward_workspace_bytes", ng), dev)
+    gg_d, gg_sh = torch.zeros(ng, **f32), torch.zeros((nsh, ng, 3), **f32)
+    _native.call("mt_create_triangle_vertices_backward", sdf.data_ptr(), ng, nsh,
+                 vox.lower_corners.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V, None,
+                 g_d.data_ptr(), g_tsh.data_ptr(), None, gg_d.data_ptr(), gg_sh.data_ptr(),
+                 ws.data_ptr(), ws.numel(), _stream())
+    gg_d2 = torch.full((ng,), float("nan"), **f32)  # fully overwritten
+    gg_sh2 = torch.full((nsh, ng, 3), float("nan"), **f32)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    _native.call("mt_create_triangle_vertices_colors_backward", sdf.data_ptr(), ng, nsh, deg,
+                 vox.lower_corners.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V, tv2.data_ptr(),
+                 col2.data_ptr(), campos.data_ptr(), b, g_d.data_ptr(), g_col.data_ptr(),
+                 gg_d2.data_ptr(), gg_sh2.data_ptr(), ws.data_ptr(), ws.numel(), status.data_ptr(),
+                 _stream())
+    assert int(status) == 0
+    torch.testing.assert_close(gg_d2, gg_d, rtol=0, atol=0)
+    torch.testing.assert_close(gg_sh2, gg_sh, rtol=0, atol=0)
+
+    # FP64 oracle adjoint of the same frozen pipeline (vertex adjoint of the SH adjoint).
+    vnp = ox.extract_voxels(spec.occupancy.cpu().numpy())
+    tri = ox.tessellate_voxels(sdf.cpu().numpy(), world.cpu().numpy(), sh.cpu().numpy(),
+                               vnp.neighbors, vnp.lower_corners, vnp.upper_corners, vnp.indices,
+                               ls.cpu().numpy())
+    g_sh_ref = orr.evaluate_spherical_harmonics_backward(tri.vertices, tri.spherical_harmonics,
+                                                         cams.extrinsics.cpu().numpy(), deg,
+                                                         g_col.cpu().numpy())
+    _, gd_ref, gs_ref = ox.create_triangle_vertices_backward(tri.record, ng, nsh, None,
+                                                             g_d.cpu().numpy(), g_sh_ref)
+    got_sh = gg_sh2.cpu().numpy().astype(np.float64)
+    assert np.abs(got_sh - gs_ref).max() <= 1e-3 * np.abs(gs_ref).max()
+    got_d = gg_d2.cpu().numpy().astype(np.float64)
+    assert np.abs(got_d - gd_ref).max() <= 1e-3 * np.abs(gd_ref).max()
+
+
+def test_fused_adjoint_reports_shared_origins(case):
+    """Two columns sharing an origin cannot be gathered: status bit 2, outputs untouched."""
+    spec, vox, world, sdf, sh, ls, cams = case
+    dev = sdf.device
+    lower = vox.lower_corners.clone()
+    lower[0, 1] = lower[0, 0]
+    n, s, ng, nsh = lower.shape[1], ls.shape[0], world.shape[0], sh.shape[0]
+    off = _offsets(vox, sdf, ls)
+    V = int(off[-1])
+    b = cams.extrinsics.shape[0]
+    f32 = dict(dtype=torch.float32, device=dev)
+    campos = torch.zeros((b, 3), **f32)
+    ws = _native.workspace(_native.query("mt_create_triangle_vertices_backward_workspace_bytes", ng), dev)
+    gg_d = torch.full((ng,), 7.0, **f32)
+    gg_sh = torch.full((nsh, ng, 3), 7.0, **f32)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    zeros_v = torch.zeros(V, **f32)
+    zeros_c = torch.zeros((b, V, 3), **f32)
+    tv = torch.zeros((V, 3), **f32)
+    _native.call("mt_create_triangle_vertices_colors_backward", sdf.data_ptr(), ng, nsh,
+                 spec.sh_degree, lower.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V,
+                 tv.data_ptr(), zeros_c.data_ptr(), campos.data_ptr(), b, zeros_v.data_ptr(),
+                 zeros_c.data_ptr(), gg_d.data_ptr(), gg_sh.data_ptr(), ws.data_ptr(), ws.numel(),
+                 status.data_ptr(), _stream())
+    torch.cuda.synchronize()
+    assert int(status) == 2
+    assert bool((gg_d == 7.0).all()) and bool((gg_sh == 7.0).all())

@@ -69,6 +69,53 @@ def test_front_end_bit_exact(scene):
     np.testing.assert_array_equal(ranges.cpu().numpy(), scene.ranges)
 
 
+def test_keys_from_geometry_match_oracle(scene):
+    """The batched step's key path (block counts -> scan -> keys recomputing each face's
+    tile rectangle) writes exactly the oracle's keys and values; the device face count
+    cuts the faces off like mt_bound_triangles does."""
+    from meshtryoshka_amd import _native
+    from meshtryoshka_amd.triangle_rasterization.keys import to_layout
+
+    t = scene.tri
+    xy, depth = to_dev(scene.xy), to_dev(scene.depth)
+    faces = to_dev(t.faces)
+    v, f = xy.shape[1], faces.shape[0]
+    b = scene.B
+    th, tw = scene.tile_shape
+    st = torch.cuda.current_stream().cuda_stream
+    nblk = -(-f // 256)
+    layout = to_layout(_key_defines())
+    for live in (f, f // 2):
+        live_x3 = torch.tensor([3 * live], dtype=torch.int32, device="cuda")
+        counts = torch.empty(b * nblk, dtype=torch.int32, device="cuda")
+        _native.call("mt_key_block_counts", xy.data_ptr(), depth.data_ptr(), v, faces.data_ptr(), f,
+                     live_x3.data_ptr(), b, th, tw, 0, 0, scene.rows, scene.cols,
+                     float(scene.spec.near_plane), counts.data_ptr(), st)
+        offs = torch.empty(b * nblk, dtype=torch.int64, device="cuda")
+        ws = _native.workspace(_native.query("mt_scan_workspace_bytes", b * nblk), xy.device)
+        _native.call("mt_inclusive_scan_i32_to_i64", counts.data_ptr(), offs.data_ptr(), b * nblk,
+                     ws.data_ptr(), ws.numel(), st)
+        cnt = scene.counts.reshape(b, f).copy()
+        cnt[:, live:] = 0
+        want_n = int(cnt.sum())
+        assert int(offs[-1]) == want_n
+        keys = torch.full((want_n + 7,), -1, dtype=torch.int64, device="cuda")
+        vals = torch.full((want_n + 7,), -1, dtype=torch.int32, device="cuda")
+        over = torch.zeros(1, dtype=torch.int32, device="cuda")
+        _native.call("mt_generate_keys_from_geometry", xy.data_ptr(), depth.data_ptr(), v,
+                     faces.data_ptr(), f, live_x3.data_ptr(), to_dev(t.face_boundaries).data_ptr(),
+                     scene.S, b, th, tw, 0, 0, scene.rows, scene.cols, float(scene.spec.near_plane),
+                     offs.data_ptr(), scene.cols, layout, keys.data_ptr(), vals.data_ptr(),
+                     keys.numel(), over.data_ptr(), st)
+        assert int(over) == 0
+        # Oracle keys of the live faces, in (camera, face, tile) order.
+        alive = np.broadcast_to(np.arange(f)[None, :] < live, (b, f)).reshape(-1)
+        keep = np.repeat(alive, scene.counts.reshape(-1))
+        np.testing.assert_array_equal(keys[:want_n].cpu().numpy(), scene.keys[keep])
+        np.testing.assert_array_equal(vals[:want_n].cpu().numpy(), scene.values[keep])
+        assert bool((keys[want_n:] == -1).all())
+
+
 @pytest.mark.parametrize("n", [1, 2, 100, 4096, 4097, 65_537, 1_000_003])
 def test_sort_matches_stable_argsort(n):
     rng = np.random.default_rng(n)
@@ -94,6 +141,21 @@ def test_sort_all_equal_and_structured_keys():
     lay = orr.fused_layout()
     depth = rng.uniform(0.5, 4.0, n).astype(np.float32)
     keys = orr.pack_keys(lay, rng.integers(0, 4, n), rng.integers(0, 5, n), rng.integers(0, 4056, n), depth)
+    out = tr.sort_keys(tr.PairedKeys(to_dev(keys), to_dev(values)), None)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(out.keys.cpu().numpy(), keys[order])
+    np.testing.assert_array_equal(out.triangle_indices.cpu().numpy(), values[order])
+
+
+@pytest.mark.parametrize("mask", [0x5555_5555_5555_5555, -0x5555_5555_5555_5556,
+                                  (1 << 0) | (1 << 9) | (1 << 33) | (1 << 62), -(2**63) | 0xF0])
+def test_sort_scattered_varying_bits(mask):
+    """Digits are taken from the varying key bits only; scattered and alternating bit
+    patterns (up to 32 separate runs, with and without the sign bit) keep exact order."""
+    n = 300_001
+    rng = np.random.default_rng(abs(mask) % 1000)
+    keys = rng.integers(-(2**63), 2**63 - 1, size=n, dtype=np.int64) & np.int64(mask)
+    values = rng.permutation(n).astype(np.int32)
     out = tr.sort_keys(tr.PairedKeys(to_dev(keys), to_dev(values)), None)
     order = np.argsort(keys, kind="stable")
     np.testing.assert_array_equal(out.keys.cpu().numpy(), keys[order])
