@@ -52,11 +52,11 @@ def parse_args():
                    help="time only the batched step with --tile / --chunk (no plain path, no "
                         "autotuner, no CPU baseline): the run profiled per kernel")
     p.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
-    p.add_argument("--serve-scenes", type=int, default=8)
-    p.add_argument("--serve-min-spheres", type=int, default=8)
-    p.add_argument("--serve-max-spheres", type=int, default=256)
-    p.add_argument("--serve-rates", type=float, nargs="+", default=[50.0, 200.0, 500.0, 2000.0])
-    p.add_argument("--serve-seconds", type=float, default=4.0)
+    p.add_argument("--serve-spheres", type=int, nargs="+", default=[8, 16, 32, 64, 96, 128, 192, 256],
+                   help="sphere count of each resident scene")
+    p.add_argument("--serve-rates", type=float, nargs="+",
+                   default=[50.0, 100.0, 200.0, 500.0, 1000.0, 2000.0])
+    p.add_argument("--serve-seconds", type=float, default=3.0)
     p.add_argument("--serve-budget-gib", type=int, default=48)
     p.add_argument("--serve-max-batch", type=int, default=8)
     p.add_argument("--probe", action="store_true",
@@ -344,11 +344,10 @@ SERVE_SHAPES = ((256, 256), (512, 512), (768, 1024), (1080, 1920))
 
 
 def serving_benchmark(args, device, log):
-    """Resident scenes of the sphere-field generator with spheres log-spaced over
-    [args.serve_min_spheres, args.serve_max_spheres]; requests = (scene, random hemisphere
-    pose, resolution from SERVE_SHAPES), forward only. For each offered rate a Poisson
-    stream runs for args.serve_seconds; reports completed req/s and p50/p99 latency."""
-    import math
+    """Resident scenes of the sphere-field generator (one per entry of args.serve_spheres);
+    requests = (scene, random hemisphere pose, resolution from SERVE_SHAPES), forward only.
+    For each offered rate a Poisson stream runs for args.serve_seconds; reports completed
+    req/s and p50/p99 latency."""
     import random
 
     import torch
@@ -357,9 +356,8 @@ def serving_benchmark(args, device, log):
     from meshtryoshka_amd import triangle_extraction as te
     from meshtryoshka_amd import workloads
 
-    n = args.serve_scenes
-    lo, hi = math.log(args.serve_min_spheres), math.log(args.serve_max_spheres)
-    spheres = [int(round(math.exp(lo + (hi - lo) * i / max(1, n - 1)))) for i in range(n)]
+    spheres = list(args.serve_spheres)
+    n = len(spheres)
     scenes, cells = {}, []
     t0 = time.time()
     for sid, ns in enumerate(spheres):

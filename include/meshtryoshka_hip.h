@@ -151,16 +151,21 @@ int mt_create_triangle_vertices_colors_forward(
     int64_t num_triangle_vertices, const float* camera_positions, int32_t B, float* out_vertices,
     float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, mt_stream_t stream);
 
+/* Workspace for mt_create_triangle_vertices_colors_backward. */
+size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(int64_t num_grid_vertices,
+                                                                   int64_t num_columns,
+                                                                   int32_t sh_degree);
+
 /* Adjoint of the fused forward above with respect to grid_sdf and grid_sh (fuses
  * evaluate_spherical_harmonics.slang:228-270 with create_triangle_vertices.slang:201-291).
  * Overwrites grid_sdf_grad f32[num_grid_vertices] and grid_sh_grad
  * f32[num_sh, num_grid_vertices, 3] (no zero-fill needed) from grad_sdf f32[V] and
  * grad_colors f32[B, V, 3], using the forward's vertices f32[V, 3] and colors (relu is
- * active where the colour is > 0). Every grid vertex gathers from the columns that
- * reference it (workspace: mt_create_triangle_vertices_backward_workspace_bytes); this
- * needs each grid vertex to be the origin / +axis end of at most one column, as
- * extract_voxels guarantees; otherwise nothing is written and bit 1 (value 2) of *status
- * is set. num_sh in {1, 4, 9, 16}. */
+ * active where the colour is > 0). Each column visits its crossings once: the origin's
+ * share goes straight to the grid gradient, the axis ends' shares through the workspace
+ * to the grid vertices they end at. This needs each grid vertex to be the origin / +axis
+ * end of at most one column, as extract_voxels guarantees; otherwise nothing is written
+ * and bit 1 (value 2) of *status is set. num_sh in {1, 4, 9, 16}. */
 int mt_create_triangle_vertices_colors_backward(
     const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree,
     const int32_t* lower_corners, int64_t num_columns, const float* level_sets, int32_t S,

@@ -15,7 +15,7 @@ Unlike the plain path, the per-triangle-vertex SH array ([NSH, V, 3], 192 bytes 
 at degree 3) is never stored: vertices and their colours for all cameras come out of one
 kernel that interpolates the grid SH in registers, and the SH adjoint is recomputed per
 triangle vertex inside the gather of the vertex adjoint. Colours stay bit-identical to the
-plain path's; gradients equal it up to the sign of zero terms.
+plain path's; gradients equal it up to float summation order.
 
 Capacities come from one eager step (``measure_capacities``) times a margin; after a
 replay, ``overflowed()`` (one host read) tells whether any capacity was exceeded, in
@@ -119,7 +119,8 @@ class BatchedTrainingStep:
         self.scan_ws = _native.workspace(_native.query("mt_scan_workspace_bytes", scan_n), dev)
         self.sort_ws = _native.workspace(_native.query("mt_sort_workspace_bytes", K), dev)
         self.vbwd_ws = _native.workspace(
-            _native.query("mt_create_triangle_vertices_backward_workspace_bytes", self.n_grid), dev)
+            _native.query("mt_create_triangle_vertices_colors_backward_workspace_bytes",
+                          self.n_grid, self.n_col, sh_degree), dev)
         # Live counts, read by kernels from device memory.
         self.num_faces_x3 = self.face_offsets[S * self.n_vox :]
         self.num_keys = self.block_offsets[B * self.n_blk - 1 :]

@@ -304,12 +304,14 @@ __global__ void __launch_bounds__(kBlock) gather_vertices_backward_kernel(
 // both. The per-vertex SH array [NSH, V, 3] is never stored: the forward interpolates each
 // vertex's coefficients from the grid while it has the vertex in registers and writes only
 // the colours of the B cameras; the adjoint recomputes the SH gradient of each triangle
-// vertex from its colour gradient while gathering it into the grid. The operations and
-// their order are those of the separate kernels (create_vertices_kernel, then
-// sh_forward_kernel / sh_backward_kernel, then gather_vertices_backward_kernel), so
-// colours are bit-identical and gradients equal up to the sign of zero terms.
+// vertex from its colour gradient (sh_backward_kernel's arithmetic) and distributes it to
+// the two grid vertices of its edge. Colours are bit-identical to the separate kernels;
+// gradients equal them up to float summation order.
 // ------------------------------------------------------------------------------------
 
+// The block's 256 column origins' SH rows are staged once in LDS (read by every crossing
+// of the column, over all level sets, and by the +x crossing of the previous column);
+// only the other axis ends' rows are read from global memory per vertex.
 template <int kDeg>
 __global__ void __launch_bounds__(kBlock) create_vertices_colors_kernel(
     const float* gv, const float* gsdf, const float* gsh, int64_t ng, const int32_t* lower,
@@ -317,16 +319,30 @@ __global__ void __launch_bounds__(kBlock) create_vertices_colors_kernel(
     const float* cams, int B, float* ov, float* od, uint8_t* otype, float* colors,
     int32_t* overflow) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
-  __shared__ int32_t s_v0[3 * kBlock], s_v1[3 * kBlock];
-  __shared__ float s_t[3 * kBlock], s_d0[3 * kBlock], s_d1[3 * kBlock];
-  __shared__ uint8_t s_axis[3 * kBlock];
+  constexpr int kRow = 3 * kN + 1;  // padded: consecutive rows start in different banks
+  __shared__ float s_sh[kBlock * kRow];
+  __shared__ int32_t s_g0[kBlock];
+  __shared__ float s_c0[kBlock];  // the column's origin SDF
+  __shared__ int32_t s_v1[3 * kBlock];
+  __shared__ float s_t[3 * kBlock], s_d1[3 * kBlock];
+  __shared__ uint8_t s_axis[3 * kBlock], s_col[3 * kBlock];
   __shared__ int s_total;
   const int64_t c_first = int64_t(blockIdx.x) * kBlock;
   const int64_t c = c_first + threadIdx.x;
   const int64_t c_last = (c_first + kBlock < n ? c_first + kBlock : n) - 1;
   const bool valid = c < n;
   ColumnSdf col{};
-  if (valid) col = load_column(gsdf, lower, n, c);
+  if (valid) {
+    col = load_column(gsdf, lower, n, c);
+#pragma unroll
+    for (int k = 0; k < kN; ++k) {
+      const float* plane = gsh + int64_t(k) * ng * 3;
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) s_sh[threadIdx.x * kRow + 3 * k + ch] = plane[3 * int64_t(col.v0) + ch];
+    }
+  }
+  s_g0[threadIdx.x] = valid ? col.v0 : -1;
+  s_c0[threadIdx.x] = col.d0;
   for (int s = 0; s < S; ++s) {
     const float iso = level_sets[s];
     const int64_t base = offsets[int64_t(s) * n + c_first];
@@ -335,12 +351,11 @@ __global__ void __launch_bounds__(kBlock) create_vertices_colors_kernel(
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         if (!crosses(col.d0, col.d[a], iso)) continue;
-        s_v0[pos] = col.v0;
         s_v1[pos] = col.v[a];
         s_t[pos] = edge_t(col.d0, col.d[a], iso);
-        s_d0[pos] = col.d0;
         s_d1[pos] = col.d[a];
         s_axis[pos] = static_cast<uint8_t>(a);
+        s_col[pos] = static_cast<uint8_t>(threadIdx.x);
         ++pos;
       }
       if (c == c_last) s_total = pos;
@@ -351,7 +366,8 @@ __global__ void __launch_bounds__(kBlock) create_vertices_colors_kernel(
     const int live = static_cast<int>(base >= nt ? 0 : (base + total <= nt ? total : nt - base));
     for (int v = threadIdx.x; v < live; v += kBlock) {
       const int64_t idx = base + v;
-      const int64_t g0 = s_v0[v], g1 = s_v1[v];
+      const int lc = s_col[v];
+      const int64_t g0 = s_g0[lc], g1 = s_v1[v];
       const float t = s_t[v];
       const float x0 = gv[3 * g0 + 0], y0 = gv[3 * g0 + 1], z0 = gv[3 * g0 + 2];
       const float x1 = gv[3 * g1 + 0], y1 = gv[3 * g1 + 1], z1 = gv[3 * g1 + 2];
@@ -359,15 +375,18 @@ __global__ void __launch_bounds__(kBlock) create_vertices_colors_kernel(
       ov[3 * idx + 0] = p.x;
       ov[3 * idx + 1] = p.y;
       ov[3 * idx + 2] = p.z;
-      od[idx] = s_d0[v] + t * (s_d1[v] - s_d0[v]);
+      const float d0 = s_c0[lc];
+      od[idx] = d0 + t * (s_d1[v] - d0);
       otype[idx] = s_axis[v];
+      // The axis end's row: the next column's staged origin when that is the end vertex.
+      const int nb = (lc + 1 < kBlock && s_g0[lc + 1] == g1) ? lc + 1 : -1;
       float coef[kN][3];
 #pragma unroll
       for (int k = 0; k < kN; ++k) {
-        const float* plane = gsh + int64_t(k) * ng * 3;
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-          const float a = plane[3 * g0 + ch], b = plane[3 * g1 + ch];
+          const float a = s_sh[lc * kRow + 3 * k + ch];
+          const float b = nb >= 0 ? s_sh[nb * kRow + 3 * k + ch] : gsh[(int64_t(k) * ng + g1) * 3 + ch];
           coef[k][ch] = a + t * (b - a);
         }
       }
@@ -384,77 +403,132 @@ __global__ void __launch_bounds__(kBlock) create_vertices_colors_kernel(
   }
 }
 
+// SH gradient of one triangle vertex: sum over cameras of basis x colour gradient where the
+// colour's relu is active (sh_backward_kernel's loop, same order).
+template <int kN, int kDeg>
+__device__ __forceinline__ void vertex_sh_grad(const float* tv, const float* colors,
+                                               const float* g_col, const float* cams, int B,
+                                               int64_t nt, int64_t here, float (*gt)[3]) {
+#pragma unroll
+  for (int k = 0; k < kN; ++k) gt[k][0] = gt[k][1] = gt[k][2] = 0.f;
+  const float3 p = vertex_at(tv, here);
+  for (int b = 0; b < B; ++b) {
+    const float3 d = view_dir(p, cams + 3 * b);
+    float basis[16];
+    sh_basis(d.x, d.y, d.z, kDeg, basis);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      const int64_t q = (int64_t(b) * nt + here) * 3 + ch;
+      const float gc = colors[q] > 0.0f ? g_col[q] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < kN; ++k) gt[k][ch] += basis[k] * gc;
+    }
+  }
+}
+
+// Phase 1, one thread per column: the column's crossings are visited once (axis by axis,
+// level sets in order); (1 - t) g goes to the column's origin, which no other column has
+// (written straight into the grid gradient), and t g to the axis end, summed per axis into
+// part[a][j][c] (plane-major: j = 0 SDF, 1 + 3k + ch SH) for phase 2.
 template <int kNsh, int kDeg>
-__global__ void __launch_bounds__(kBlock) gather_colors_backward_kernel(
+__global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
     const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
     int S, const int32_t* offsets, int64_t nt, const float* tv, const float* colors,
     const float* cams, int B, const float* g_d, const float* g_col, float* gg_d, float* gg_sh,
-    const int32_t* maps, int32_t* status) {
+    float* part, const int32_t* maps) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
+  constexpr int kRow = 1 + 3 * kN;
+  if (maps[4 * ng] != 0) return;  // not one-to-one: reported by phase 2
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const ColumnSdf col = load_column(gsdf, lower, n, c);
+  float acc0_d = 0.f;
+  float acc0[kN][3];
+#pragma unroll
+  for (int k = 0; k < kN; ++k) acc0[k][0] = acc0[k][1] = acc0[k][2] = 0.f;
+  for (int a = 0; a < 3; ++a) {
+    if (col.v[a] < 0) continue;
+    float acce_d = 0.f;
+    float acce[kN][3];
+#pragma unroll
+    for (int k = 0; k < kN; ++k) acce[k][0] = acce[k][1] = acce[k][2] = 0.f;
+    for (int s = 0; s < S; ++s) {
+      const float iso = level_sets[s];
+      if (!crosses(col.d0, col.d[a], iso)) continue;
+      int64_t here = offsets[s * n + c];
+      for (int q = 0; q < a; ++q) here += crosses(col.d0, col.d[q], iso) ? 1 : 0;
+      if (here >= nt) continue;
+      const float t = edge_t(col.d0, col.d[a], iso);
+      const float w0 = 1.0f - t;
+      const float gd = g_d[here];
+      acc0_d += w0 * gd;
+      acce_d += t * gd;
+      float gt[kN][3];
+      vertex_sh_grad<kN, kDeg>(tv, colors, g_col, cams, B, nt, here, gt);
+#pragma unroll
+      for (int k = 0; k < kN; ++k)
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+          acc0[k][ch] += w0 * gt[k][ch];
+          acce[k][ch] += t * gt[k][ch];
+        }
+    }
+    float* pa = part + int64_t(a) * kRow * n;
+    pa[c] = acce_d;
+#pragma unroll
+    for (int k = 0; k < kN; ++k)
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) pa[(1 + 3 * k + ch) * n + c] = acce[k][ch];
+  }
+  const int64_t g = col.v0;
+  gg_d[g] = acc0_d;
+#pragma unroll
+  for (int k = 0; k < kNsh; ++k)
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) gg_sh[(int64_t(k) * ng + g) * 3 + ch] = k < kN ? acc0[k < kN ? k : 0][ch] : 0.f;
+}
+
+// Phase 2, one thread per grid vertex: add the axis-end sums of the (at most three)
+// columns that end at it; a grid vertex that is no column's origin starts from zero.
+template <int kNsh, int kDeg>
+__global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
+    int64_t ng, int64_t n, const float* part, float* gg_d, float* gg_sh, const int32_t* maps,
+    int32_t* status) {
+  constexpr int kN = (kDeg + 1) * (kDeg + 1);
+  constexpr int kRow = 1 + 3 * kN;
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (maps[4 * ng] != 0) {  // the columns do not map one-to-one onto grid vertices
     if (g == 0 && status) atomicOr(status, 2);
     return;
   }
   if (g >= ng) return;
-  float acc_d = 0.f;
-  float acc_sh[kN][3];
+  const bool origin = maps[g] >= 0;
+  int32_t ends[3];
+  bool any = false;
 #pragma unroll
-  for (int k = 0; k < kN; ++k) acc_sh[k][0] = acc_sh[k][1] = acc_sh[k][2] = 0.f;
-  for (int role = 0; role < 4; ++role) {
-    const int32_t c = maps[role * ng + g];
-    if (c < 0) continue;
-    const ColumnSdf col = load_column(gsdf, lower, n, c);
-    for (int s = 0; s < S; ++s) {
-      const float iso = level_sets[s];
-      bool cr[3];
+  for (int a = 0; a < 3; ++a) {
+    ends[a] = maps[(a + 1) * ng + g];
+    any = any || ends[a] >= 0;
+  }
+  if (origin && !any) return;  // phase 1 wrote the final value
+  float sd = origin ? gg_d[g] : 0.f;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) cr[a] = crosses(col.d0, col.d[a], iso);
-      int64_t idx = offsets[s * n + c];
+  for (int a = 0; a < 3; ++a)
+    if (ends[a] >= 0) sd += part[int64_t(a) * kRow * n + ends[a]];
+  gg_d[g] = sd;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        if (!cr[a]) continue;
-        const int64_t here = idx++;
-        if (role != 0 && role != a + 1) continue;
-        if (here >= nt) continue;
-        const float t = edge_t(col.d0, col.d[a], iso);
-        const float wgt = (role == 0) ? 1.0f - t : t;
-        acc_d += wgt * g_d[here];
-        // SH gradient of this triangle vertex (sh_backward_kernel's sum over cameras).
-        float gt[kN][3];
+  for (int k = 0; k < kNsh; ++k)
 #pragma unroll
-        for (int k = 0; k < kN; ++k) gt[k][0] = gt[k][1] = gt[k][2] = 0.f;
-        const float3 p = vertex_at(tv, here);
-        for (int b = 0; b < B; ++b) {
-          const float3 d = view_dir(p, cams + 3 * b);
-          float basis[16];
-          sh_basis(d.x, d.y, d.z, kDeg, basis);
+    for (int ch = 0; ch < 3; ++ch) {
+      float* o = gg_sh + (int64_t(k) * ng + g) * 3 + ch;
+      float v = origin ? *o : 0.f;
+      if (k < kN) {
 #pragma unroll
-          for (int ch = 0; ch < 3; ++ch) {
-            const int64_t q = (int64_t(b) * nt + here) * 3 + ch;
-            // relu passes gradient where the colour is > 0 (pre-activation > 0).
-            const float gc = colors[q] > 0.0f ? g_col[q] : 0.0f;
-#pragma unroll
-            for (int k = 0; k < kN; ++k) gt[k][ch] += basis[k] * gc;
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < kN; ++k) {
-          acc_sh[k][0] += wgt * gt[k][0];
-          acc_sh[k][1] += wgt * gt[k][1];
-          acc_sh[k][2] += wgt * gt[k][2];
-        }
+        for (int a = 0; a < 3; ++a)
+          if (ends[a] >= 0) v += part[(int64_t(a) * kRow + 1 + 3 * k + ch) * n + ends[a]];
       }
+      *o = v;
     }
-  }
-  gg_d[g] = acc_d;
-#pragma unroll
-  for (int k = 0; k < kNsh; ++k) {
-    float* o = gg_sh + (int64_t(k) * ng + g) * 3;
-    o[0] = k < kN ? acc_sh[k < kN ? k : 0][0] : 0.f;
-    o[1] = k < kN ? acc_sh[k < kN ? k : 0][1] : 0.f;
-    o[2] = k < kN ? acc_sh[k < kN ? k : 0][2] : 0.f;
-  }
 }
 
 __global__ void __launch_bounds__(kBlock) count_faces_kernel(const float* sdf, const int32_t* lower,
@@ -701,6 +775,13 @@ extern "C" int mt_create_triangle_vertices_colors_forward(
   return mt::check_launch("mt_create_triangle_vertices_colors_forward");
 }
 
+extern "C" size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(
+    int64_t num_grid_vertices, int64_t num_columns, int32_t sh_degree) {
+  const int64_t row = 1 + 3 * (int64_t(sh_degree) + 1) * (int64_t(sh_degree) + 1);
+  const size_t maps = static_cast<size_t>(4 * num_grid_vertices + 1) * sizeof(int32_t);
+  return (maps + 255) / 256 * 256 + static_cast<size_t>(3 * row * num_columns) * sizeof(float);
+}
+
 extern "C" int mt_create_triangle_vertices_colors_backward(
     const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree,
     const int32_t* lower_corners, int64_t num_columns, const float* level_sets, int32_t S,
@@ -716,7 +797,8 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
   MT_REQUIRE(sh_degree >= 0 && (sh_degree + 1) * (sh_degree + 1) <= num_sh,
              "mt_create_triangle_vertices_colors_backward: degree %d needs (deg+1)^2 <= %d",
              sh_degree, num_sh);
-  const size_t need = mt_create_triangle_vertices_backward_workspace_bytes(num_grid_vertices);
+  const size_t need = mt_create_triangle_vertices_colors_backward_workspace_bytes(
+      num_grid_vertices, num_columns, sh_degree);
   if (workspace == nullptr || workspace_bytes < need) {
     mt::set_error("mt_create_triangle_vertices_colors_backward: workspace of %zu bytes is smaller than the %zu required",
                   workspace_bytes, need);
@@ -726,29 +808,37 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
   hipStream_t st = static_cast<hipStream_t>(stream);
   int32_t* maps = static_cast<int32_t*>(workspace);
   const int64_t words = 4 * num_grid_vertices;
+  float* part = reinterpret_cast<float*>(static_cast<char*>(workspace) +
+                                         (static_cast<size_t>(words + 1) * 4 + 255) / 256 * 256);
   mt::fill_maps_kernel<<<mt::blocks_for(words, 256), 256, 0, st>>>(maps, words);
   if (num_columns > 0)
     mt::build_maps_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
         lower_corners, num_columns, num_grid_vertices, maps);
-  const dim3 grid(mt::blocks_for(num_grid_vertices, mt::kBlock));
-#define MT_GCOL(N, D)                                                                          \
-  mt::gather_colors_backward_kernel<N, D><<<grid, mt::kBlock, 0, st>>>(                       \
-      grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S, vertex_offsets, \
-      num_triangle_vertices, vertices, colors, camera_positions, B, grad_sdf, grad_colors,    \
-      grid_sdf_grad, grid_sh_grad, maps, status)
+  const dim3 cgrid(mt::blocks_for(num_columns, mt::kBlock));
+  const dim3 ggrid(mt::blocks_for(num_grid_vertices, mt::kBlock));
+#define MT_ADJ(N, D)                                                                             \
+  do {                                                                                           \
+    if (num_columns > 0)                                                                         \
+      mt::colors_adjoint_columns_kernel<N, D><<<cgrid, mt::kBlock, 0, st>>>(                     \
+          grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S,                \
+          vertex_offsets, num_triangle_vertices, vertices, colors, camera_positions, B, grad_sdf, \
+          grad_colors, grid_sdf_grad, grid_sh_grad, part, maps);                                 \
+    mt::colors_adjoint_ends_kernel<N, D><<<ggrid, mt::kBlock, 0, st>>>(                          \
+        num_grid_vertices, num_columns, part, grid_sdf_grad, grid_sh_grad, maps, status);        \
+  } while (0)
   const int key = num_sh * 4 + sh_degree;
   switch (key) {
-    case 1 * 4 + 0: MT_GCOL(1, 0); break;
-    case 4 * 4 + 0: MT_GCOL(4, 0); break;
-    case 4 * 4 + 1: MT_GCOL(4, 1); break;
-    case 9 * 4 + 0: MT_GCOL(9, 0); break;
-    case 9 * 4 + 1: MT_GCOL(9, 1); break;
-    case 9 * 4 + 2: MT_GCOL(9, 2); break;
-    case 16 * 4 + 0: MT_GCOL(16, 0); break;
-    case 16 * 4 + 1: MT_GCOL(16, 1); break;
-    case 16 * 4 + 2: MT_GCOL(16, 2); break;
-    default: MT_GCOL(16, 3); break;
+    case 1 * 4 + 0: MT_ADJ(1, 0); break;
+    case 4 * 4 + 0: MT_ADJ(4, 0); break;
+    case 4 * 4 + 1: MT_ADJ(4, 1); break;
+    case 9 * 4 + 0: MT_ADJ(9, 0); break;
+    case 9 * 4 + 1: MT_ADJ(9, 1); break;
+    case 9 * 4 + 2: MT_ADJ(9, 2); break;
+    case 16 * 4 + 0: MT_ADJ(16, 0); break;
+    case 16 * 4 + 1: MT_ADJ(16, 1); break;
+    case 16 * 4 + 2: MT_ADJ(16, 2); break;
+    default: MT_ADJ(16, 3); break;
   }
-#undef MT_GCOL
+#undef MT_ADJ
   return mt::check_launch("mt_create_triangle_vertices_colors_backward");
 }
