@@ -80,6 +80,11 @@ __device__ __forceinline__ float wave_sum_f(float x) {
   return x;
 }
 
+// Single-pass scans (scan.hip), callable from other kernels' host code.
+size_t scan_workspace_bytes_i64(int64_t n);
+int inclusive_scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, void* ws,
+                              size_t ws_bytes, hipStream_t stream);
+
 // Key bit fields (keys.slang:1-34). A depth always has 32 bits.
 struct KeyFields {
   int camera_offset, camera_bits, shell_offset, shell_bits, tile_offset, tile_bits, depth_offset;
