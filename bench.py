@@ -1,16 +1,21 @@
 #!/usr/bin/env python
 """Training-step benchmark of the extract -> rasterize -> composite path on one MI355X.
 
-Workload (sphere-field training step): a sparse 1024^3 virtual grid whose occupied cells
-lie within 3 cells of the zero set of min(64 seeded spheres, ground plane y = -0.6);
-corner SDF = clean SDF + 0.01 N(0, 1); SH degree 3; 5 shells spaced 1/512 apart;
-4 cameras at 1237x822 on the radius-2 upper hemisphere; 16x16 tiles; sharpness 50.
-One step = tessellate_voxels (5 level sets) + fused render of the 4 cameras + mean
-squared error against a seeded U(0, 1) target + backward to the grid SDF and SH.
-Inputs are resident in HBM before timing starts.
+Default workload (sphere-field training step, 4 cameras): a sparse 1024^3 virtual grid
+whose occupied cells lie within 3 cells of the zero set of min(64 seeded spheres, ground
+plane y = -0.6); corner SDF = clean SDF + 0.01 N(0, 1); SH degree 3; 5 shells spaced 1/512
+apart; 4 cameras at 1237x822 on the radius-2 upper hemisphere; sharpness 50. One step =
+tessellate_voxels (5 level sets) + fused render of the 4 cameras + mean squared error
+against a seeded U(0, 1) target + backward to the grid SDF and SH. Inputs are resident in
+HBM before timing starts.
+
+``--workload sphere-field-32cam-mixed``: 256 spheres, 7 shells, 32 cameras whose
+resolutions are drawn from {320x240, 640x480, 1237x822, 1920x1080, 2473x1643}, rendered in
+camera groups inside one captured step (a 512^3 virtual grid: ~16M occupied cells).
 
 Prints one JSON line (metric, value = steps/s, roofline of the dominant kernel measured
-with HIP events over the timed steps, CPU baseline of the oracle on a bounded sample).
+with HIP events over the instrumented steps, CPU baseline of the oracle on a bounded
+sample).
 """
 
 from __future__ import annotations
@@ -28,7 +33,18 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
-def parse_args():
+# Preset scenes: option values that replace the defaults.
+WORKLOADS = {
+    "sphere-field-4cam": {},
+    "sphere-field-32cam-mixed": {
+        "spheres": 256, "resolution": 512, "shells": 7, "cameras": 32, "camera_group": 8,
+        "resolutions": ["320x240", "640x480", "1237x822", "1920x1080", "2473x1643"],
+        "steps": 3, "warmup": 1,
+    },
+}
+
+
+def parse_args(argv=None):
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     p.add_argument("--steps", type=int, default=10)
@@ -39,6 +55,13 @@ def parse_args():
     p.add_argument("--cameras", type=int, default=4)
     p.add_argument("--height", type=int, default=822)
     p.add_argument("--width", type=int, default=1237)
+    p.add_argument("--shells", type=int, default=5)
+    p.add_argument("--resolutions", type=str, nargs="+", default=None, metavar="WxH",
+                   help="draw each camera's resolution uniformly from these (mixed resolutions)")
+    p.add_argument("--camera-group", type=int, default=0,
+                   help="cameras rendered together inside the batched step (0 = all)")
+    p.add_argument("--workload", choices=sorted(WORKLOADS), default="sphere-field-4cam",
+                   help="preset scene sizes (explicit options given on the command line win)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-tiles", type=int, default=24)
     p.add_argument("--cpu-slab", type=float, default=1 / 64,
@@ -54,6 +77,9 @@ def parse_args():
     p.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
     p.add_argument("--serve-spheres", type=int, nargs="+", default=[8, 16, 32, 64, 96, 128, 192, 256],
                    help="sphere count of each resident scene")
+    p.add_argument("--serve-resolution", type=int, default=512,
+                   help="virtual grid of the served scenes (512^3 gives ~1M..25M occupied cells "
+                        "for 8..256 spheres)")
     p.add_argument("--serve-rates", type=float, nargs="+",
                    default=[50.0, 100.0, 200.0, 500.0, 1000.0, 2000.0])
     p.add_argument("--serve-seconds", type=float, default=3.0)
@@ -61,7 +87,21 @@ def parse_args():
     p.add_argument("--serve-max-batch", type=int, default=8)
     p.add_argument("--probe", action="store_true",
                    help="build the scene, print its sizes and memory, and exit")
-    return p.parse_args()
+    args = p.parse_args(argv)
+    given = {a.split("=")[0] for a in (sys.argv[1:] if argv is None else argv) if a.startswith("--")}
+    for key, value in WORKLOADS[args.workload].items():
+        if "--" + key.replace("_", "-") not in given:
+            setattr(args, key, value)
+    return args
+
+
+def parse_resolutions(items):
+    """'WxH' strings -> (H, W) tuples."""
+    out = []
+    for it in items:
+        w, h = it.lower().split("x")
+        out.append((int(h), int(w)))
+    return out
 
 
 # ------------------------------------------------------------------------------------
@@ -76,12 +116,15 @@ class Scene:
 def build_scene(args, device):
     import torch
 
+    from meshtryoshka_amd import batched
     from meshtryoshka_amd import triangle_extraction as te
     from meshtryoshka_amd import workloads
 
+    res = parse_resolutions(args.resolutions) if args.resolutions else None
     spec = workloads.sphere_field(seed=args.seed, num_spheres=args.spheres,
                                   resolution=args.resolution, num_cameras=args.cameras,
-                                  image_shape=(args.height, args.width), device=device)
+                                  num_shells=args.shells, image_shape=(args.height, args.width),
+                                  device=device, resolutions=res)
     sc = Scene()
     sc.spec = spec
     sc.voxels = te.extract_voxels(spec.occupancy.to(device))
@@ -91,8 +134,14 @@ def build_scene(args, device):
     sc.sh = sh.to(device).requires_grad_(True)
     sc.cams = spec.cameras.to(device)
     b = sc.cams.extrinsics.shape[0]
-    h, w = sc.cams.image_shape
-    sc.target = workloads.target_image(spec, (b, 3, h, w)).to(device)
+    sc.shapes = batched.camera_shapes(sc.cams.image_shape, b)
+    sc.uniform = len(set(sc.shapes)) == 1
+    sc.num_pixels = sum(h * w for h, w in sc.shapes)
+    if sc.uniform:
+        h, w = sc.shapes[0]
+        sc.target = workloads.target_image(spec, (b, 3, h, w)).to(device)
+    else:  # flat concatenation of the per-camera [3, H_b, W_b] planes
+        sc.target = workloads.target_image(spec, (3 * sc.num_pixels,)).to(device)
     sc.level_sets = spec.level_sets.to(device)
     sc.sharpness = torch.tensor(spec.sharpness, device=device)
     sc.background = torch.tensor(spec.background, device=device)
@@ -118,13 +167,21 @@ def plain_per_camera_step(sc):
                                v.upper_corners, v.indices, sc.level_sets)
     b = sc.cams.extrinsics.shape[0]
     loss = 0.0
+    p = 0
     for c in range(b):
+        h, w = sc.shapes[c]
         img = tr.render(sc.cams.extrinsics[c : c + 1], sc.cams.intrinsics[c : c + 1],
-                        sc.cams.image_shape, tri.vertices.detach(), tri.faces, tri.face_boundaries,
+                        (h, w), tri.vertices.detach(), tri.faces, tri.face_boundaries,
                         tri.signed_distances, tri.spherical_harmonics, sc.spec.sh_degree,
                         sc.sharpness, sc.background, tile_shape=sc.spec.tile_shape,
                         near_plane=sc.spec.near_plane)
-        loss = loss + ((img - sc.target[c : c + 1]) ** 2).mean() / b
+        if sc.uniform:
+            tgt = sc.target[c : c + 1]
+        else:
+            tgt = sc.target[3 * p : 3 * (p + h * w)].view(1, 3, h, w)
+        p += h * w
+        # One mean over every camera's pixels.
+        loss = loss + ((img - tgt) ** 2).sum() / (3 * sc.num_pixels)
     loss.backward()
     return loss
 
@@ -165,20 +222,26 @@ class LaunchTimer:
         self.marks = []
 
     def mean_ms(self, name):
+        """Average duration of one launch."""
         return self.totals[name] / self.counts[name]
 
+    def step_ms(self, name, steps):
+        """Duration per step (all launches of the stage in one step)."""
+        return self.totals[name] / steps
 
-def make_batched(sc, tile_shape, chunk, margin=1.15):
+
+def make_batched(sc, tile_shape, chunk, camera_group=None, margin=1.15):
     from meshtryoshka_amd import batched
 
     cap = batched.measure_capacities(sc.voxels, sc.level_sets, sc.cams.extrinsics, sc.cams.intrinsics,
-                                     sc.cams.image_shape, sc.world, sc.sdf.detach(), sc.sh.detach(),
-                                     tile_shape=tile_shape, near_plane=sc.spec.near_plane).scaled(margin)
+                                     sc.shapes, sc.world, sc.sdf.detach(), sc.sh.detach(),
+                                     tile_shape=tile_shape, near_plane=sc.spec.near_plane,
+                                     camera_group=camera_group).scaled(margin)
     return batched.BatchedTrainingStep(sc.voxels, sc.level_sets, sc.cams.extrinsics,
-                                       sc.cams.intrinsics, sc.cams.image_shape, sc.sh.shape[0],
+                                       sc.cams.intrinsics, sc.shapes, sc.sh.shape[0],
                                        sc.spec.sh_degree, sc.spec.sharpness, sc.spec.background,
                                        cap, tile_shape=tile_shape, chunk=chunk,
-                                       near_plane=sc.spec.near_plane)
+                                       near_plane=sc.spec.near_plane, camera_group=camera_group)
 
 
 # ------------------------------------------------------------------------------------
@@ -187,10 +250,15 @@ def make_batched(sc, tile_shape, chunk, margin=1.15):
 
 
 def algorithmic_bytes(z):
-    """Unique bytes each kernel stage must move at minimum (assumes perfect on-chip reuse)."""
+    """Unique bytes each kernel stage must move per step at minimum (perfect on-chip reuse).
+
+    ``G`` camera groups run one after another: stages that read per-vertex or per-face
+    data once per group count it G times; ``P`` is the pixel count and ``TT`` the tile
+    count summed over all cameras; ``K`` is the key count summed over all groups."""
     nb = z["NSH"] * 12
     n_lc, n_vox, vt, f, k, b, s = z["N_lc"], z["N_vox"], z["V_t"], z["F"], z["K"], z["B"], z["S"]
-    hw = z["H"] * z["W"]
+    g, px, tt = z.get("G", 1), z["P"], z["TT"]
+    blocks = b * -(-f // 256)
     return {
         "mt_count_triangle_vertices": n_lc * 16 + n_lc * 4 + 4 * s * n_lc,
         "mt_exclusive_scan_i32": 8 * (s * n_lc + 1) + 8 * (s * n_vox + 1),  # both scans
@@ -198,26 +266,27 @@ def algorithmic_bytes(z):
         + vt * (12 + 4 + nb + 1),
         "mt_count_triangle_faces": n_vox * 32 + n_lc * 4 + s * n_vox * (4 + 1),
         "mt_create_triangle_faces": n_vox * (28 + 4) + s * n_vox * (1 + 8) + 4 * s * n_lc + vt + f * 16,
-        "mt_project_vertices": vt * 12 + b * vt * 12,
+        "mt_project_vertices": g * vt * 12 + b * vt * 12,
         "mt_bound_triangles": f * 12 + b * vt * 12 + b * f * 20,
-        "mt_inclusive_scan_i32_to_i64": 12 * b * -(-f // 256),  # over (camera, 256-face block)
-        "mt_key_block_counts": f * 12 + b * vt * 12 + 4 * b * -(-f // 256),
-        "mt_generate_keys_from_geometry": f * 12 + b * vt * 12 + k * 12 + 8 * b * -(-f // 256),
+        "mt_inclusive_scan_i32_to_i64": 12 * blocks,  # over (camera, 256-face block)
+        "mt_key_block_counts": g * f * 12 + b * vt * 12 + 4 * blocks,
+        "mt_generate_keys_from_geometry": g * f * 12 + b * vt * 12 + k * 12 + 8 * blocks,
         "mt_sort_keys": 8 * k + 24 * k * z.get("sort_passes", 8),
-        "mt_delineate_tiles": 8 * k + 16 * b * z["T"],
+        "mt_delineate_tiles": 8 * k + 16 * tt,
         "mt_evaluate_sh_forward": vt * (12 + nb) + b * vt * 12,
-        "mt_intersect_and_composite_forward": 96 * k + 16 * b * hw + 16 * b * z["T"],
-        "mt_image_l2_loss": 16 * b * hw + 12 * b * hw + 16 * b * hw,
-        "mt_intersect_and_composite_backward": 96 * k + 32 * b * hw + 16 * b * z["T"]
-        + 2 * (12 * b * vt + 4 * vt),
+        "mt_intersect_and_composite_forward": 96 * k + 16 * px + 16 * tt,
+        "mt_image_l2_loss": 16 * px + 12 * px + 16 * px,
+        "mt_intersect_and_composite_backward": 96 * k + 32 * px + 16 * tt
+        + 2 * (12 * b * vt + 4 * g * vt),
         "mt_evaluate_sh_backward": vt * (12 + nb) + b * vt * 12 + vt * nb,
         "mt_create_triangle_vertices_backward": n_lc * 16 + vt * (12 + 4 + nb) + 2 * n_lc * (4 + nb),
         # Fused vertex + colour kernels of the batched step (grid-vertex count ~ N_lc).
-        "mt_create_triangle_vertices_colors_forward": n_lc * (16 + 4 + 12 + nb) + 4 * s * n_lc
-        + vt * 17 + b * vt * 12,
-        "mt_create_triangle_vertices_colors_backward": n_lc * (16 + 4) + 4 * s * n_lc
-        + vt * (4 + 12 + 24 * b) + n_lc * (4 + nb),
-        "zero_fill": 4 * (3 * b * vt + vt) + 16 * b * z["T"],
+        "mt_create_triangle_vertices_colors_forward": g * (n_lc * (16 + 4 + 12 + nb) + 4 * s * n_lc
+                                                           + vt * 17) + b * vt * 12,
+        "mt_create_triangle_vertices_colors_backward": g * (n_lc * (16 + 4) + 4 * s * n_lc
+                                                            + vt * (4 + 12) + n_lc * (4 + nb))
+        + 24 * b * vt,
+        "zero_fill": 4 * (3 * b * vt + g * vt) + 16 * tt,
     }
 
 
@@ -234,6 +303,44 @@ def sort_passes(keys):
         if int(bit.max()) != int(bit.min()):
             varying += 1
     return (varying + 7) // 8
+
+
+def workload_description(args, sc):
+    if sc.uniform:
+        h, w = sc.shapes[0]
+        cams = f"{args.cameras} cameras {w}x{h}"
+    else:
+        counts = {}
+        for h, w in sc.shapes:
+            counts[f"{w}x{h}"] = counts.get(f"{w}x{h}", 0) + 1
+        cams = f"{args.cameras} cameras of mixed resolution (" + ", ".join(
+            f"{n}x {k}" for k, n in sorted(counts.items())) + ")"
+    return (f"sphere-field training step: {args.resolution}^3 virtual grid, {args.spheres} spheres "
+            f"+ ground plane, band 3 cells, SH deg 3, {args.shells} shells, {cams}")
+
+
+def workload_key(args, best):
+    """Names the committed profile summary of this workload and tile choice."""
+    th, tw = best.tile_shape
+    return f"{args.workload}_tile{th}x{tw}_chunk{best.chunk}"
+
+
+def profile_traffic(entry_point, key):
+    """HBM bytes per launch of ``entry_point`` from the committed rocprofv3 --pmc summary
+    (profiles/batched_step_<key>.md, FETCH_SIZE and WRITE_SIZE scaled by the calibration
+    kernel), or None when no summary of this workload exists."""
+    path = os.path.join(ROOT, "profiles", f"batched_step_{key}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if cells and cells[0] == entry_point and len(cells) >= 6:
+                try:
+                    return int(float(cells[5]) * 2**30)  # "HBM GiB per launch" column
+                except ValueError:
+                    return None
+    return None
 
 
 # ------------------------------------------------------------------------------------
@@ -361,8 +468,9 @@ def serving_benchmark(args, device, log):
     scenes, cells = {}, []
     t0 = time.time()
     for sid, ns in enumerate(spheres):
-        spec = workloads.sphere_field(seed=100 + sid, num_spheres=ns, resolution=args.resolution,
-                                      num_cameras=4, image_shape=(512, 512), device=device)
+        spec = workloads.sphere_field(seed=100 + sid, num_spheres=ns,
+                                      resolution=args.serve_resolution, num_cameras=4,
+                                      image_shape=(512, 512), device=device)
         vox = te.extract_voxels(spec.occupancy.to(device))
         world, sdf, sh = workloads.finish_fields(spec, vox.vertices)
         del spec.occupancy
@@ -407,7 +515,7 @@ def serving_benchmark(args, device, log):
         log(f"[serve] offered {rate} req/s: {st}")
         sweep.append(st)
     return {
-        "scenes": n, "spheres": spheres, "cells": cells, "resolution": args.resolution,
+        "scenes": n, "spheres": spheres, "cells": cells, "resolution": args.serve_resolution,
         "shapes": [list(x) for x in SERVE_SHAPES], "hbm_budget_gib": args.serve_budget_gib,
         "max_batch": args.serve_max_batch, "resident_gib": round(resident / 2**30, 2),
         "sweep": sweep,
@@ -421,7 +529,8 @@ def main():
     args = parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by camera, see DESIGN.md)", file=sys.stderr)
+              "(the path would shard by camera, with one all-reduce of the grid gradients)",
+              file=sys.stderr)
         sys.exit(2)
     import torch
 
@@ -438,14 +547,14 @@ def main():
         f"{sc.voxels.lower_corners.shape[1]} columns")
     inputs = (sc.world, sc.sdf.detach(), sc.sh.detach(), sc.target)
     b = sc.cams.extrinsics.shape[0]
-    h, w = sc.cams.image_shape
+    group = args.camera_group or None
     if args.probe:
         from meshtryoshka_amd import batched
 
         for ts in ((16, 16), (8, 8)):
             cap = batched.measure_capacities(sc.voxels, sc.level_sets, sc.cams.extrinsics,
-                                             sc.cams.intrinsics, (h, w), sc.world, sc.sdf.detach(),
-                                             sc.sh.detach(), tile_shape=ts)
+                                             sc.cams.intrinsics, sc.shapes, sc.world, sc.sdf.detach(),
+                                             sc.sh.detach(), tile_shape=ts, camera_group=group)
             log(f"[probe] tiles {ts}: {cap}")
         log(f"[probe] voxels {sc.voxels.num_voxels} columns {sc.voxels.lower_corners.shape[1]} "
             f"peak memory {torch.cuda.max_memory_allocated() / 2**30:.1f} GiB")
@@ -471,10 +580,11 @@ def main():
         tuned = []
     else:
         best, tuned = autotune.autotune(sc.voxels, sc.level_sets, sc.cams.extrinsics, sc.cams.intrinsics,
-                                        (h, w), sc.sh.shape[0], sc.spec.sh_degree, sc.spec.sharpness,
-                                        sc.spec.background, inputs, near_plane=sc.spec.near_plane, log=log)
+                                        sc.shapes, sc.sh.shape[0], sc.spec.sh_degree, sc.spec.sharpness,
+                                        sc.spec.background, inputs, near_plane=sc.spec.near_plane, log=log,
+                                        camera_group=group)
     log(f"[bench] autotune choice: tile {best.tile_shape} chunk {best.chunk}")
-    step = make_batched(sc, best.tile_shape, best.chunk)
+    step = make_batched(sc, best.tile_shape, best.chunk, camera_group=group)
 
     # 3. Graph-captured batched step: the timed region.
     step.capture(*inputs)
@@ -510,8 +620,8 @@ def main():
     inst_steps = 2
     for _ in range(inst_steps):
         step_args = step._composite_args
-        def with_stats(_orig=step_args):
-            a = _orig()
+        def with_stats(g=None, _orig=step_args):
+            a = _orig(g)
             a.stats = stats.data_ptr()
             return a
         step._composite_args = with_stats
@@ -523,15 +633,18 @@ def main():
     sizes = {
         "N_lc": int(sc.voxels.lower_corners.shape[1]), "N_vox": int(sc.voxels.upper_corners.shape[1]),
         "V_t": live["V_t"], "F": live["F"], "K": live["K"], "B": b, "S": int(sc.level_sets.shape[0]),
-        "H": h, "W": w, "T": step.T, "NSH": int(sc.sh.shape[0]),
+        "G": len(step.groups), "P": sc.num_pixels, "TT": step.total_tiles, "NSH": int(sc.sh.shape[0]),
         "tile_shape": list(best.tile_shape), "chunk": best.chunk,
     }
-    sizes["sort_passes"] = sort_passes(step.sorted_keys[: live["K"]])
+    if sc.uniform:
+        sizes["H"], sizes["W"] = sc.shapes[0]
+        sizes["T"] = step.T
+    sizes["sort_passes"] = sort_passes(step.sorted_keys[: live["K_last_group"]])
     alg = algorithmic_bytes(sizes)
     table = {}
     for name in timer.totals:
-        ms = timer.mean_ms(name)
-        entry = {"ms": round(ms, 4)}
+        ms = timer.step_ms(name, inst_steps)
+        entry = {"ms": round(ms, 4), "launches": timer.counts[name] // inst_steps}
         if alg.get(name):
             entry["GB/s"] = round(alg[name] / (ms * 1e-3) / 1e9, 1)
             entry["hbm_frac"] = round(entry["GB/s"] / HBM_PEAK_GBS, 4)
@@ -539,27 +652,31 @@ def main():
     s = stats.tolist()
     tests = s[0] / (2 * inst_steps)
     hits = s[1] / (2 * inst_steps)
-    sizes["composite_tests_per_launch"] = tests
-    sizes["composite_hits_per_launch"] = hits
+    sizes["composite_tests_per_step"] = tests
+    sizes["composite_hits_per_step"] = hits
     if args.breakdown:
         for name, e in sorted(table.items(), key=lambda kv: -kv[1]["ms"]):
             log(f"  {name:40s} {e}")
+    # Roofline of the dominant stage: algorithmic bytes of one launch over the average
+    # launch duration (HIP events on the launch stream).
     dom = max((k for k in table if k.startswith("mt_")), key=lambda k: table[k]["ms"])
+    launches = table[dom]["launches"]
+    ms_launch = timer.mean_ms(dom)
+    bytes_launch = alg.get(dom, 0) / max(launches, 1)
+    achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": profile_traffic(dom, workload_key(args, best)),
+            "ms_per_launch": round(ms_launch, 4), "launches_per_step": launches,
+            "algorithmic_bytes_per_launch": int(bytes_launch)}
     if dom.startswith("mt_intersect_and_composite"):
-        ms = timer.mean_ms(dom)
-        fl = 58 * tests + (25 if dom.endswith("forward") else 60) * hits
-        ach = fl / (ms * 1e-3) / 1e12
-        roof = {"bound": "valu", "kernel": dom, "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "ms": round(ms, 4), "flops_per_launch": fl,
-                "note": "FP32 vector ops: 58 per (pixel, triangle) test + 25 (fwd) / 60 (bwd) per hit"}
-    else:
-        e = table[dom]
-        roof = {"bound": "hbm", "kernel": dom, "achieved": e.get("GB/s"), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": e.get("hbm_frac"), "traffic": None, "ms": e["ms"]}
+        fl = (58 * tests + (25 if dom.endswith("forward") else 60) * hits) / max(launches, 1)
+        roof["valu"] = {"achieved": round(fl / (ms_launch * 1e-3) / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "note": "FP32 vector ops: 58 per (pixel, triangle) "
+                        "test + 25 (fwd) / 60 (bwd) per hit"}
 
     cpu = None
-    if not (args.no_cpu_baseline or args.only_batched):
+    if not (args.no_cpu_baseline or args.only_batched or not sc.uniform):
         log("[bench] CPU baseline (oracle on a bounded sample)")
         from threadpoolctl import threadpool_limits
 
@@ -569,11 +686,16 @@ def main():
     serve = None
     if not (args.no_serving or args.only_batched):
         # Free the training step's arenas and the scene before loading the served scenes.
+        import gc
+
         del step, step_args, with_stats, inputs
         sc = None
         timer = None
+        gc.collect()
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
+        log(f"[bench] device memory after freeing the training step: "
+            f"{torch.cuda.memory_allocated(device) / 2**30:.1f} GiB")
         log("[bench] serving sweep")
         serve = serving_benchmark(args, device, log)
 
@@ -591,9 +713,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": (f"sphere-field training step: {args.resolution}^3 virtual grid, "
-                         f"{args.spheres} spheres + ground plane, band 3 cells, SH deg 3, 5 shells, "
-                         f"{args.cameras} cameras {args.width}x{args.height}"),
+            "workload": workload_description(args, sc),
+            "name": args.workload,
+            "camera_group": args.camera_group or args.cameras,
             "sizes": sizes,
         },
         "plain_per_camera": None if plain_s is None else {

@@ -158,8 +158,9 @@ size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(int64_t num_g
 
 /* Adjoint of the fused forward above with respect to grid_sdf and grid_sh (fuses
  * evaluate_spherical_harmonics.slang:228-270 with create_triangle_vertices.slang:201-291).
- * Overwrites grid_sdf_grad f32[num_grid_vertices] and grid_sh_grad
- * f32[num_sh, num_grid_vertices, 3] (no zero-fill needed) from grad_sdf f32[V] and
+ * Overwrites (accumulate = 0; no zero-fill needed) or adds to (accumulate = 1)
+ * grid_sdf_grad f32[num_grid_vertices] and grid_sh_grad f32[num_sh, num_grid_vertices, 3]
+ * from grad_sdf f32[V] and
  * grad_colors f32[B, V, 3], using the forward's vertices f32[V, 3] and colors (relu is
  * active where the colour is > 0). Each column visits its crossings once: the origin's
  * share goes straight to the grid gradient, the axis ends' shares through the workspace
@@ -172,7 +173,7 @@ int mt_create_triangle_vertices_colors_backward(
     const int32_t* vertex_offsets, int64_t num_triangle_vertices, const float* vertices,
     const float* colors, const float* camera_positions, int32_t B, const float* grad_sdf,
     const float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
-    size_t workspace_bytes, int32_t* status, mt_stream_t stream);
+    size_t workspace_bytes, int32_t accumulate, int32_t* status, mt_stream_t stream);
 
 /* face_counts int32[S * num_voxels] (3 x triangles) and cell_codes uint8[S, num_voxels].
  * Replaces count_triangle_faces.slang:13-50 (.py:28-63). upper_corners
@@ -231,25 +232,29 @@ int mt_generate_keys(const float* depths, int64_t V, const int32_t* faces, int64
  * bound_triangles.slang:144-192 with generate_keys.slang:4-68 and the tile-count cumsum of
  * generate_keys.py:40-47). Faces are taken in blocks of 256 per camera. block_counts
  * int32[B, ceil(F / 256)] = number of keys of every (camera, face block), computed with
- * mt_bound_triangles' arithmetic (same culling, same num_faces_x3 semantics). */
+ * mt_bound_triangles' arithmetic (same culling, same num_faces_x3 semantics). cam_dims
+ * int32[B, 4] (height, width, grid_rows, grid_cols), or NULL: camera b's tile grid is
+ * [0, grid_rows_b) x [0, grid_cols_b) instead of [row_min, row_max) x [col_min, col_max). */
 int mt_key_block_counts(const float* xy, const float* depths, int64_t V, const int32_t* faces,
                         int64_t F, const int32_t* num_faces_x3, int32_t B, int32_t tile_h,
                         int32_t tile_w, int32_t row_min, int32_t col_min, int32_t row_max,
-                        int32_t col_max, float near_plane, int32_t* block_counts,
-                        mt_stream_t stream);
+                        int32_t col_max, float near_plane, const int32_t* cam_dims,
+                        int32_t* block_counts, mt_stream_t stream);
 
 /* Keys of every (camera, face, covered tile) like mt_generate_keys, with each face's
  * tile rectangle recomputed from the projected geometry (no stored bounds) and each
  * block's first key at block_offsets[q - 1] (0 for q = 0), block_offsets = the inclusive
  * int64 scan of mt_key_block_counts' output (mt_inclusive_scan_i32_to_i64). The keys and
- * values are identical to mt_bound_triangles + inclusive scan + mt_generate_keys. */
+ * values are identical to mt_bound_triangles + inclusive scan + mt_generate_keys. cam_dims
+ * as in mt_key_block_counts (tile ids then use each camera's grid_cols). */
 int mt_generate_keys_from_geometry(const float* xy, const float* depths, int64_t V,
                                    const int32_t* faces, int64_t F, const int32_t* num_faces_x3,
                                    const int32_t* shell_face_boundaries, int32_t S, int32_t B,
                                    int32_t tile_h, int32_t tile_w, int32_t row_min,
                                    int32_t col_min, int32_t row_max, int32_t col_max,
-                                   float near_plane, const int64_t* block_offsets,
-                                   int32_t grid_cols, mt_key_layout layout, int64_t* keys,
+                                   float near_plane, const int32_t* cam_dims,
+                                   const int64_t* block_offsets, int32_t grid_cols,
+                                   mt_key_layout layout, int64_t* keys,
                                    int32_t* values, int64_t key_capacity, int32_t* overflow,
                                    mt_stream_t stream);
 
@@ -267,11 +272,13 @@ int mt_sort_keys(const int64_t* keys_in, const int32_t* values_in, int64_t n, in
 
 /* [start, end) of every (camera[, shell], tile) run of the sorted keys into the
  * zero-initialised int64[B, S', T, 2] (S' = S if separate_shells else 1). If n_device
- * is not NULL, min(*n_device, n) keys are live. Replaces delineate_tiles.slang:12-58
- * (.py:24-55). */
+ * is not NULL, min(*n_device, n) keys are live. With tile_base int64[B + 1] (cameras of
+ * different resolutions, S' = 1 only) camera b's tiles start at pair tile_base[b] and T is
+ * ignored. Replaces delineate_tiles.slang:12-58 (.py:24-55). */
 int mt_delineate_tiles(const int64_t* sorted_keys, int64_t n, mt_key_layout layout,
                        int32_t separate_shells, int32_t B, int32_t S_out, int32_t T,
-                       int64_t* tile_ranges, const int64_t* n_device, mt_stream_t stream);
+                       int64_t* tile_ranges, const int64_t* n_device, const int64_t* tile_base,
+                       mt_stream_t stream);
 
 /* out f32[B, 3] = inverse(extrinsics[b])[:3, 3] (computed in double precision).
  * Replaces torch.inverse(extrinsics)[:, :3, 3] (evaluate_spherical_harmonics.py:38). */
@@ -308,6 +315,16 @@ typedef struct mt_composite_args {
   /* Optional (may be NULL): uint64[2] incremented by the number of (pixel, triangle)
    * barycentric tests executed and of accepted hits. */
   unsigned long long* stats;
+  /* Optional camera table for cameras of different resolutions (all NULL / 0: every
+   * camera is height x width on a grid_rows x grid_cols tile grid). cam_dims int32[B, 4] =
+   * (height, width, grid_rows, grid_cols) per camera; tile_base int64[B + 1] = first tile
+   * of each camera in tile_ranges (pairs, S' = 1); pixel_base int64[B + 1] = first pixel of
+   * each camera: its planes [4, H_b, W_b] start at out + 4 * pixel_base[b];
+   * total_tiles = tile_base[B] (host copy). */
+  const int32_t* cam_dims;
+  const int64_t* tile_base;
+  const int64_t* pixel_base;
+  int64_t total_tiles;
 } mt_composite_args;
 
 /* out f32[B, 4, H, W] (rgb, transmittance). Replaces intersect_and_composite_forward
@@ -329,9 +346,14 @@ int mt_intersect_and_composite_backward(const mt_composite_args* args, const flo
  * target)^2) over B x 3 x H x W, and grad_out f32[B, 4, H, W] = its gradient with
  * respect to out f32[B, 4, H, W] (rgb, transmittance). The reference computes this loss
  * with torch ops in the model (model_meshtryoshka.py:249-255, L2 part). background f32[3],
- * target f32[B, 3, H, W]; *loss must be zeroed by the caller. */
+ * target f32[B, 3, H, W]; *loss must be zeroed by the caller. With pixel_base int64[B + 1]
+ * (cameras of different resolutions; H, W ignored) camera b's planes start at
+ * out + 4 * pixel_base[b] and target + 3 * pixel_base[b], total_pixels = pixel_base[B].
+ * mean_pixels > 0 sets the pixel count of the mean (a batch rendered in camera groups adds
+ * each group's share of one mean); 0 means the pixels of this call. */
 int mt_image_l2_loss(const float* out, const float* background, const float* target, int32_t B,
-                     int32_t H, int32_t W, float* loss, float* grad_out, mt_stream_t stream);
+                     int32_t H, int32_t W, const int64_t* pixel_base, int64_t total_pixels,
+                     int64_t mean_pixels, float* loss, float* grad_out, mt_stream_t stream);
 
 /* Explicit mode: first hit per (camera, shell, pixel) in depth order. tile_ranges
  * int64[B, S, T, 2]; out_uv f32[B*S, H, W, 2]; out_index int32[B*S, H, W] (-1 = miss).

@@ -58,6 +58,11 @@ class CompositeArgs(ctypes.Structure):
         ("layout", KeyLayout),
         ("chunk", c_i32),
         ("stats", c_p),
+        # Camera table (cameras of different resolutions); all None / 0 when uniform.
+        ("cam_dims", c_p),
+        ("tile_base", c_p),
+        ("pixel_base", c_p),
+        ("total_tiles", c_i64),
     ]
 
 
@@ -95,7 +100,7 @@ SIGNATURES = {
     "mt_create_triangle_vertices_colors_backward": (
         c_i32,
         [c_p, c_i64, c_i32, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_i32, c_p,
-         c_p, c_p, c_p, c_p, c_sz, c_p, c_p],
+         c_p, c_p, c_p, c_p, c_sz, c_i32, c_p, c_p],
     ),
     "mt_count_triangle_faces": (c_i32, [c_p, c_p, c_i64, c_p, c_i64, c_p, c_i32, c_p, c_p, c_p]),
     "mt_create_triangle_faces": (
@@ -116,16 +121,17 @@ SIGNATURES = {
     "mt_key_block_counts": (
         c_i32,
         [c_p, c_p, c_i64, c_p, c_i64, c_p, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_f32,
-         c_p, c_p],
+         c_p, c_p, c_p],
     ),
     "mt_generate_keys_from_geometry": (
         c_i32,
         [c_p, c_p, c_i64, c_p, c_i64, c_p, c_p, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32,
-         c_i32, c_f32, c_p, c_i32, KeyLayout, c_p, c_p, c_i64, c_p, c_p],
+         c_i32, c_f32, c_p, c_p, c_i32, KeyLayout, c_p, c_p, c_i64, c_p, c_p],
     ),
     "mt_sort_workspace_bytes": (c_sz, [c_i64]),
     "mt_sort_keys": (c_i32, [c_p, c_p, c_i64, c_p, c_p, c_p, c_sz, c_p, c_p]),
-    "mt_delineate_tiles": (c_i32, [c_p, c_i64, KeyLayout, c_i32, c_i32, c_i32, c_i32, c_p, c_p, c_p]),
+    "mt_delineate_tiles": (c_i32, [c_p, c_i64, KeyLayout, c_i32, c_i32, c_i32, c_i32, c_p, c_p, c_p,
+                                   c_p]),
     "mt_camera_positions": (c_i32, [c_p, c_i32, c_p, c_p]),
     "mt_evaluate_sh_forward": (c_i32, [c_p, c_i64, c_p, c_i32, c_i32, c_p, c_i32, c_p, c_p]),
     "mt_evaluate_sh_backward": (c_i32, [c_p, c_i64, c_p, c_i32, c_i32, c_p, c_i32, c_p, c_p, c_p]),
@@ -134,7 +140,8 @@ SIGNATURES = {
         c_i32,
         [ctypes.POINTER(CompositeArgs), c_p, c_p, c_p, c_p, c_p, c_i32, c_p],
     ),
-    "mt_image_l2_loss": (c_i32, [c_p, c_p, c_p, c_i32, c_i32, c_i32, c_p, c_p, c_p]),
+    "mt_image_l2_loss": (c_i32, [c_p, c_p, c_p, c_i32, c_i32, c_i32, c_p, c_i64, c_i64, c_p, c_p,
+                                 c_p]),
     "mt_intersect": (
         c_i32,
         [c_p, c_p, c_p, c_i64, c_p, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_p,

@@ -45,7 +45,8 @@ def _time_step(step, inputs, warmup: int, reps: int) -> float:
 
 def autotune(voxels, level_sets, extrinsics, intrinsics, image_shape, num_sh, sh_degree, sharpness,
              background, inputs, tile_shapes=TILE_SHAPES, chunks=CHUNKS, margin=1.15, warmup=1,
-             reps=3, near_plane=0.2, log=None) -> tuple[Choice, list[Choice]]:
+             reps=3, near_plane=0.2, log=None,
+             camera_group: int | None = None) -> tuple[Choice, list[Choice]]:
     """Returns (best, all measured candidates). ``inputs`` = (grid_vertices, grid_sdf,
     grid_sh, target) as passed to ``BatchedTrainingStep.run``."""
     grid_vertices, grid_sdf, grid_sh, _ = inputs
@@ -53,12 +54,13 @@ def autotune(voxels, level_sets, extrinsics, intrinsics, image_shape, num_sh, sh
     for ts in tile_shapes:
         cap = batched.measure_capacities(voxels, level_sets, extrinsics, intrinsics, image_shape,
                                          grid_vertices, grid_sdf, grid_sh, tile_shape=ts,
-                                         near_plane=near_plane).scaled(margin)
+                                         near_plane=near_plane,
+                                         camera_group=camera_group).scaled(margin)
         for ch in chunks:
             step = batched.BatchedTrainingStep(voxels, level_sets, extrinsics, intrinsics,
                                                image_shape, num_sh, sh_degree, sharpness,
                                                background, cap, tile_shape=ts, chunk=ch,
-                                               near_plane=near_plane)
+                                               near_plane=near_plane, camera_group=camera_group)
             ms = _time_step(step, inputs, warmup, reps)
             if step.overflowed():
                 raise RuntimeError(f"capacity overflow while tuning tile {ts} chunk {ch}")

@@ -8,14 +8,21 @@ key counts) to allocate exactly-sized tensors. Here every buffer is preallocated
 capacity, kernels read the live counts from device memory (``num_faces_x3`` /
 ``n_device`` arguments of the C ABI) and write nothing past a capacity (they raise a
 device overflow flag instead), so the whole step can be captured once into a hipGraph
-(``torch.cuda.CUDAGraph``) and replayed. Results are bit-identical to the plain path:
-the same arithmetic runs on the same data, only the sizes come from device memory.
+(``torch.cuda.CUDAGraph``) and replayed.
 
 Unlike the plain path, the per-triangle-vertex SH array ([NSH, V, 3], 192 bytes per vertex
-at degree 3) is never stored: vertices and their colours for all cameras come out of one
-kernel that interpolates the grid SH in registers, and the SH adjoint is recomputed per
-triangle vertex inside the gather of the vertex adjoint. Colours stay bit-identical to the
-plain path's; gradients equal it up to float summation order.
+at degree 3) is never stored: vertices and their colours come out of one kernel that
+interpolates the grid SH in registers, and the SH adjoint is recomputed per triangle
+vertex and distributed to the grid. Keys are generated straight from the projected
+geometry (no per-face tile bounds, counts or offsets are stored). Images are
+bit-identical to the plain path's; gradients equal it up to float summation order.
+
+Cameras may have different resolutions (a per-camera table of image and tile-grid sizes
+drives the key, delineation, compositing and loss kernels) and may be rendered in groups
+of ``camera_group`` cameras one after another inside the same captured step, which bounds
+the per-camera buffers (projected vertices, colours and their gradients, keys) to one
+group: the tessellation runs once, each group adds its share of the loss and of the grid
+gradients.
 
 Capacities come from one eager step (``measure_capacities``) times a margin; after a
 replay, ``overflowed()`` (one host read) tells whether any capacity was exceeded, in
@@ -30,30 +37,43 @@ import torch
 
 from . import _native
 from .triangle_extraction import Voxels
-from .triangle_rasterization.common import TileGrid
 from .triangle_rasterization.keys import create_key_defines, to_layout
-
-_ptr = _native.ptr
 
 
 @dataclass(frozen=True)
 class Capacities:
     vertices: int
     faces: int
-    keys: int
+    keys: int  # per camera group
 
     def scaled(self, margin: float) -> "Capacities":
         return Capacities(int(self.vertices * margin) + 1024, int(self.faces * margin) + 1024,
                           int(self.keys * margin) + 4096)
 
 
+def camera_shapes(image_shape, b: int) -> list[tuple[int, int]]:
+    """(H, W) of every camera from one shared (H, W) or a per-camera list of them."""
+    if len(image_shape) == 2 and all(isinstance(x, int) for x in image_shape):
+        return [(int(image_shape[0]), int(image_shape[1]))] * b
+    shapes = [(int(s[0]), int(s[1])) for s in image_shape]
+    if len(shapes) != b:
+        raise ValueError(f"{len(shapes)} image shapes for {b} cameras")
+    return shapes
+
+
+def camera_groups(b: int, camera_group: int | None) -> list[tuple[int, int]]:
+    g = b if not camera_group else max(1, min(int(camera_group), b))
+    return [(c, min(b, c + g)) for c in range(0, b, g)]
+
+
 class BatchedTrainingStep:
-    """One fused forward + backward step over B same-resolution cameras and S shells."""
+    """One fused forward + backward step over B cameras (any resolutions) and S shells."""
 
     def __init__(self, voxels: Voxels, level_sets: torch.Tensor, extrinsics: torch.Tensor,
-                 intrinsics: torch.Tensor, image_shape: tuple[int, int], num_sh: int, sh_degree: int,
+                 intrinsics: torch.Tensor, image_shape, num_sh: int, sh_degree: int,
                  sharpness: float, background, capacities: Capacities, tile_shape=(16, 16),
-                 chunk: int = 0, near_plane: float = 0.2, reference_quirk: bool = False):
+                 chunk: int = 0, near_plane: float = 0.2, reference_quirk: bool = False,
+                 camera_group: int | None = None):
         dev = voxels.lower_corners.device
         self.device = dev
         self.voxels = voxels
@@ -61,7 +81,9 @@ class BatchedTrainingStep:
         self.extrinsics = extrinsics.contiguous().float().to(dev)
         self.intrinsics = intrinsics.contiguous().float().to(dev)
         self.B = self.extrinsics.shape[0]
-        self.H, self.W = image_shape
+        self.shapes = camera_shapes(image_shape, self.B)
+        self.uniform = len(set(self.shapes)) == 1
+        self.H, self.W = self.shapes[0]
         self.S = self.level_sets.shape[0]
         self.num_sh = num_sh
         self.sh_degree = sh_degree
@@ -70,9 +92,6 @@ class BatchedTrainingStep:
         self.near_plane = float(near_plane)
         self.quirk = int(reference_quirk)
         self.cap = capacities
-        self.grid = TileGrid(self.tile_shape, (self.H, self.W), self.B, self.S)
-        self.rows, self.cols = self.grid.grid_shape
-        self.T = self.rows * self.cols
         self.layout = to_layout(create_key_defines(("camera", 6), ("tile", 20), ("depth", 32),
                                                    ("shell", 6)))
         self.n_col = voxels.lower_corners.shape[1]
@@ -81,7 +100,35 @@ class BatchedTrainingStep:
         f32 = dict(dtype=torch.float32, device=dev)
         i32 = dict(dtype=torch.int32, device=dev)
         i64 = dict(dtype=torch.int64, device=dev)
-        V, F, K, B, S = self.cap.vertices, self.cap.faces, self.cap.keys, self.B, self.S
+        V, F, K, S = self.cap.vertices, self.cap.faces, self.cap.keys, self.S
+        th, tw = self.tile_shape
+
+        # Camera table: image and tile-grid sizes; first pixel of every camera.
+        dims = [(h, w, -(-h // th), -(-w // tw)) for h, w in self.shapes]
+        first = [0]
+        for h, w in self.shapes:
+            first.append(first[-1] + h * w)
+        self.pixel_first = first
+        self.num_pixels = first[-1]
+        self.rows, self.cols = dims[0][2], dims[0][3]
+        self.T = self.rows * self.cols
+        # Camera groups, each with its device-side table (camera indices local to the group).
+        self.groups = []
+        for c0, c1 in camera_groups(self.B, camera_group):
+            tb, pb = [0], [0]
+            for c in range(c0, c1):
+                tb.append(tb[-1] + dims[c][2] * dims[c][3])
+                pb.append(pb[-1] + dims[c][0] * dims[c][1])
+            self.groups.append({
+                "c0": c0, "c1": c1, "n": c1 - c0, "tiles": tb[-1], "pixels": pb[-1],
+                "cam_dims": torch.tensor([list(d) for d in dims[c0:c1]], **i32),
+                "tile_base": torch.tensor(tb, **i64),
+                "pixel_base": torch.tensor(pb, **i64),
+            })
+        self.G = max(g["n"] for g in self.groups)
+        self.total_tiles = sum(g["tiles"] for g in self.groups)
+        G = self.G
+
         self.vertex_offsets = torch.zeros(S * self.n_col + 1, **i32)
         self.face_offsets = torch.zeros(S * self.n_vox + 1, **i32)
         self.codes = torch.empty((S, self.n_vox), dtype=torch.uint8, device=dev)
@@ -90,22 +137,21 @@ class BatchedTrainingStep:
         self.ttype = torch.zeros(V, dtype=torch.uint8, device=dev)
         self.faces = torch.zeros((F, 3), **i32)
         self.voxel_indices = torch.zeros(F, **i32)
-        self.xy = torch.zeros((B, V, 2), **f32)
-        self.depth = torch.zeros((B, V), **f32)
-        # Keys per (camera, 256-face block) and their inclusive scan (the key kernels need
-        # no per-face tile bounds, counts or offsets).
+        self.xy = torch.zeros((G, V, 2), **f32)
+        self.depth = torch.zeros((G, V), **f32)
+        # Keys per (camera, 256-face block) and their inclusive scan.
         self.n_blk = (F + 255) // 256
-        self.block_counts = torch.zeros(B * self.n_blk, **i32)
-        self.block_offsets = torch.zeros(B * self.n_blk, **i64)
+        self.block_counts = torch.zeros(G * self.n_blk, **i32)
+        self.block_offsets = torch.zeros(G * self.n_blk, **i64)
         self.keys = torch.zeros(K, **i64)
         self.values = torch.zeros(K, **i32)
         self.sorted_keys = torch.zeros(K, **i64)
         self.sorted_values = torch.zeros(K, **i32)
-        self.ranges = torch.zeros((B, 1, self.T, 2), **i64)
-        self.campos = torch.zeros((B, 3), **f32)
-        self.colors = torch.zeros((B, V, 3), **f32)
-        self.out = torch.zeros((B, 4, self.H, self.W), **f32)
-        self.grad_out = torch.zeros_like(self.out)
+        self.ranges = torch.zeros((max(g["tiles"] for g in self.groups), 2), **i64)
+        self.campos = torch.zeros((self.B, 3), **f32)
+        self.colors = torch.zeros((G, V, 3), **f32)
+        self.out_flat = torch.zeros(4 * self.num_pixels, **f32)
+        self.grad_flat = torch.zeros_like(self.out_flat)
         self.loss = torch.zeros(1, **f32)
         self.colors_grad = torch.zeros_like(self.colors)
         self.td_grad = torch.zeros_like(self.td)
@@ -113,9 +159,10 @@ class BatchedTrainingStep:
         self.grid_sdf_grad = torch.zeros(self.n_grid, **f32)
         self.grid_sh_grad = torch.zeros((num_sh, self.n_grid, 3), **f32)
         self.overflow = torch.zeros(1, **i32)
+        self.group_keys = torch.zeros(len(self.groups), **i64)  # key count of each group
         self.sharpness = torch.tensor([float(sharpness)], **f32)
         self.background = torch.as_tensor(background, dtype=torch.float32, device=dev).contiguous()
-        scan_n = max(S * self.n_col + 1, S * self.n_vox + 1, B * self.n_blk)
+        scan_n = max(S * self.n_col + 1, S * self.n_vox + 1, G * self.n_blk)
         self.scan_ws = _native.workspace(_native.query("mt_scan_workspace_bytes", scan_n), dev)
         self.sort_ws = _native.workspace(_native.query("mt_sort_workspace_bytes", K), dev)
         self.vbwd_ws = _native.workspace(
@@ -123,26 +170,47 @@ class BatchedTrainingStep:
                           self.n_grid, self.n_col, sh_degree), dev)
         # Live counts, read by kernels from device memory.
         self.num_faces_x3 = self.face_offsets[S * self.n_vox :]
-        self.num_keys = self.block_offsets[B * self.n_blk - 1 :]
         self.graph = None
         self.static_inputs = None
 
     # ------------------------------------------------------------------------------
-    def _composite_args(self):
+    @property
+    def out(self) -> torch.Tensor:
+        """The rendered (rgb, transmittance) planes: [B, 4, H, W] when every camera has the
+        same resolution, else the flat per-camera concatenation (camera b's [4, H_b, W_b]
+        at 4 * pixel_first[b])."""
+        if self.uniform:
+            return self.out_flat.view(self.B, 4, self.H, self.W)
+        return self.out_flat
+
+    def image(self, b: int) -> torch.Tensor:
+        """Camera b's (rgb, transmittance) planes [4, H_b, W_b]."""
+        h, w = self.shapes[b]
+        p = self.pixel_first[b]
+        return self.out_flat[4 * p : 4 * (p + h * w)].view(4, h, w)
+
+    def _composite_args(self, g=None):
+        g = self.groups[0] if g is None else g
         th, tw = self.tile_shape
+        table = not self.uniform
+        h, w = self.shapes[g["c0"]]
         return _native.CompositeArgs(
             self.xy.data_ptr(), self.colors.data_ptr(), self.td.data_ptr(), self.cap.vertices,
             self.faces.data_ptr(), self.sharpness.data_ptr(), self.sorted_keys.data_ptr(),
-            self.sorted_values.data_ptr(), self.ranges.data_ptr(), self.B, self.S, self.rows,
-            self.cols, th, tw, self.H, self.W, self.layout, self.chunk, None,
+            self.sorted_values.data_ptr(), self.ranges.data_ptr(), g["n"], self.S, self.rows,
+            self.cols, th, tw, h, w, self.layout, self.chunk, None,
+            g["cam_dims"].data_ptr() if table else None,
+            g["tile_base"].data_ptr() if table else None,
+            g["pixel_base"].data_ptr() if table else None, g["tiles"] if table else 0,
         )
 
     def run(self, grid_vertices: torch.Tensor, grid_sdf: torch.Tensor, grid_sh: torch.Tensor,
             target: torch.Tensor, hook=None) -> None:
         """Enqueue one forward + backward on the current stream (no host sync).
 
-        ``hook(name)``, if given, is called after each library launch (the benchmark
-        records a HIP event there to time every kernel stage)."""
+        ``target`` is [B, 3, H, W] (one resolution) or the flat per-camera concatenation of
+        the [3, H_b, W_b] planes. ``hook(name)``, if given, is called after each library
+        launch (the benchmark records a HIP event there to time every kernel stage)."""
         if hook is None:
             call = _native.call
         else:
@@ -151,76 +219,95 @@ class BatchedTrainingStep:
                 hook(name)
         st = torch.cuda.current_stream(self.device).cuda_stream
         v = self.voxels
-        S, B = self.S, self.B
+        S = self.S
         V, F, K = self.cap.vertices, self.cap.faces, self.cap.keys
+        th, tw = self.tile_shape
+        target = target.contiguous().view(-1)
+        table = not self.uniform
         self.loss.zero_()
-        self.ranges.zero_()
-        self.colors_grad.zero_()
-        self.td_grad.zero_()
         self.sharp_grad.zero_()
-        if hook is not None:
-            hook("zero_fill")
-        # --- tessellation
+        # --- tessellation counts (once for all camera groups)
         call("mt_count_triangle_vertices", grid_sdf.data_ptr(), v.lower_corners.data_ptr(),
              self.n_col, self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), st)
         call("mt_exclusive_scan_i32", self.vertex_offsets.data_ptr(), self.vertex_offsets.numel(),
              self.scan_ws.data_ptr(), self.scan_ws.numel(), st)
-        call("mt_camera_positions", self.extrinsics.data_ptr(), B, self.campos.data_ptr(), st)
-        # Vertices and their colours in one pass; the per-vertex SH array is never stored.
-        call("mt_create_triangle_vertices_colors_forward", grid_vertices.data_ptr(),
-             grid_sdf.data_ptr(), grid_sh.data_ptr(), self.n_grid, self.num_sh, self.sh_degree,
-             v.lower_corners.data_ptr(), self.n_col, self.level_sets.data_ptr(), S,
-             self.vertex_offsets.data_ptr(), V, self.campos.data_ptr(), B, self.tv.data_ptr(),
-             self.td.data_ptr(), self.ttype.data_ptr(), self.colors.data_ptr(),
-             self.overflow.data_ptr(), st)
         call("mt_count_triangle_faces", grid_sdf.data_ptr(), v.lower_corners.data_ptr(), self.n_col,
              v.upper_corners.data_ptr(), self.n_vox, self.level_sets.data_ptr(), S,
              self.face_offsets.data_ptr(), self.codes.data_ptr(), st)
         call("mt_exclusive_scan_i32", self.face_offsets.data_ptr(), self.face_offsets.numel(),
              self.scan_ws.data_ptr(), self.scan_ws.numel(), st)
-        call("mt_create_triangle_faces", v.neighbors.data_ptr(), v.indices.data_ptr(), self.n_vox,
-             self.n_col, self.vertex_offsets.data_ptr(), self.ttype.data_ptr(),
-             self.codes.data_ptr(), S, self.face_offsets.data_ptr(), self.faces.data_ptr(),
-             self.voxel_indices.data_ptr(), F, V, self.overflow.data_ptr(), st)
-        face_boundaries = (self.face_offsets[self.n_vox :: self.n_vox][:S] // 3).contiguous()
-        if hook is not None:
-            hook("face_boundaries")
-        # --- rasterization
-        call("mt_project_vertices", self.tv.data_ptr(), V, self.extrinsics.data_ptr(),
-             self.intrinsics.data_ptr(), B, self.xy.data_ptr(), self.depth.data_ptr(), st)
-        th, tw = self.tile_shape
-        call("mt_key_block_counts", self.xy.data_ptr(), self.depth.data_ptr(), V,
-             self.faces.data_ptr(), F, self.num_faces_x3.data_ptr(), B, th, tw, 0, 0, self.rows,
-             self.cols, self.near_plane, self.block_counts.data_ptr(), st)
-        call("mt_inclusive_scan_i32_to_i64", self.block_counts.data_ptr(),
-             self.block_offsets.data_ptr(), B * self.n_blk, self.scan_ws.data_ptr(),
-             self.scan_ws.numel(), st)
-        call("mt_generate_keys_from_geometry", self.xy.data_ptr(), self.depth.data_ptr(), V,
-             self.faces.data_ptr(), F, self.num_faces_x3.data_ptr(), face_boundaries.data_ptr(), S,
-             B, th, tw, 0, 0, self.rows, self.cols, self.near_plane, self.block_offsets.data_ptr(),
-             self.cols, self.layout, self.keys.data_ptr(), self.values.data_ptr(), K,
-             self.overflow.data_ptr(), st)
-        call("mt_sort_keys", self.keys.data_ptr(), self.values.data_ptr(), K,
-             self.sorted_keys.data_ptr(), self.sorted_values.data_ptr(), self.sort_ws.data_ptr(),
-             self.sort_ws.numel(), self.num_keys.data_ptr(), st)
-        call("mt_delineate_tiles", self.sorted_keys.data_ptr(), K, self.layout, 0, B, 1, self.T,
-             self.ranges.data_ptr(), self.num_keys.data_ptr(), st)
-        args = self._composite_args()
-        call("mt_intersect_and_composite_forward", args, self.out.data_ptr(), st)
-        # --- loss and backward chain
-        call("mt_image_l2_loss", self.out.data_ptr(), self.background.data_ptr(), target.data_ptr(),
-             B, self.H, self.W, self.loss.data_ptr(), self.grad_out.data_ptr(), st)
-        call("mt_intersect_and_composite_backward", args, self.out.data_ptr(),
-             self.grad_out.data_ptr(), self.colors_grad.data_ptr(), self.td_grad.data_ptr(),
-             self.sharp_grad.data_ptr(), self.quirk, st)
-        # SH adjoint fused into the gather of the vertex adjoint (overwrites the grid grads).
-        call("mt_create_triangle_vertices_colors_backward", grid_sdf.data_ptr(), self.n_grid,
-             self.num_sh, self.sh_degree, v.lower_corners.data_ptr(), self.n_col,
-             self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V, self.tv.data_ptr(),
-             self.colors.data_ptr(), self.campos.data_ptr(), B, self.td_grad.data_ptr(),
-             self.colors_grad.data_ptr(), self.grid_sdf_grad.data_ptr(),
-             self.grid_sh_grad.data_ptr(), self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(),
-             self.overflow.data_ptr(), st)
+        call("mt_camera_positions", self.extrinsics.data_ptr(), self.B, self.campos.data_ptr(), st)
+        face_boundaries = None
+        for gi, g in enumerate(self.groups):
+            c0, n = g["c0"], g["n"]
+            cam_dims = g["cam_dims"].data_ptr() if table else None
+            self.colors_grad[:n].zero_()
+            self.td_grad.zero_()
+            self.ranges[: g["tiles"]].zero_()
+            if hook is not None:
+                hook("zero_fill")
+            # Vertices and the group's colours in one pass (later groups rewrite the same
+            # positions); the per-vertex SH array is never stored.
+            call("mt_create_triangle_vertices_colors_forward", grid_vertices.data_ptr(),
+                 grid_sdf.data_ptr(), grid_sh.data_ptr(), self.n_grid, self.num_sh,
+                 self.sh_degree, v.lower_corners.data_ptr(), self.n_col,
+                 self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V,
+                 self.campos[c0:].data_ptr(), n, self.tv.data_ptr(), self.td.data_ptr(),
+                 self.ttype.data_ptr(), self.colors.data_ptr(), self.overflow.data_ptr(), st)
+            if gi == 0:
+                call("mt_create_triangle_faces", v.neighbors.data_ptr(), v.indices.data_ptr(),
+                     self.n_vox, self.n_col, self.vertex_offsets.data_ptr(), self.ttype.data_ptr(),
+                     self.codes.data_ptr(), S, self.face_offsets.data_ptr(), self.faces.data_ptr(),
+                     self.voxel_indices.data_ptr(), F, V, self.overflow.data_ptr(), st)
+                face_boundaries = (self.face_offsets[self.n_vox :: self.n_vox][:S] // 3).contiguous()
+                if hook is not None:
+                    hook("face_boundaries")
+            # --- rasterization of the group's cameras
+            call("mt_project_vertices", self.tv.data_ptr(), V, self.extrinsics[c0:].data_ptr(),
+                 self.intrinsics[c0:].data_ptr(), n, self.xy.data_ptr(), self.depth.data_ptr(), st)
+            call("mt_key_block_counts", self.xy.data_ptr(), self.depth.data_ptr(), V,
+                 self.faces.data_ptr(), F, self.num_faces_x3.data_ptr(), n, th, tw, 0, 0,
+                 self.rows, self.cols, self.near_plane, cam_dims, self.block_counts.data_ptr(), st)
+            call("mt_inclusive_scan_i32_to_i64", self.block_counts.data_ptr(),
+                 self.block_offsets.data_ptr(), n * self.n_blk, self.scan_ws.data_ptr(),
+                 self.scan_ws.numel(), st)
+            n_keys = self.block_offsets[n * self.n_blk - 1 :]
+            self.group_keys[gi : gi + 1].copy_(n_keys[:1])
+            call("mt_generate_keys_from_geometry", self.xy.data_ptr(), self.depth.data_ptr(), V,
+                 self.faces.data_ptr(), F, self.num_faces_x3.data_ptr(), face_boundaries.data_ptr(),
+                 S, n, th, tw, 0, 0, self.rows, self.cols, self.near_plane, cam_dims,
+                 self.block_offsets.data_ptr(), self.cols, self.layout, self.keys.data_ptr(),
+                 self.values.data_ptr(), K, self.overflow.data_ptr(), st)
+            call("mt_sort_keys", self.keys.data_ptr(), self.values.data_ptr(), K,
+                 self.sorted_keys.data_ptr(), self.sorted_values.data_ptr(), self.sort_ws.data_ptr(),
+                 self.sort_ws.numel(), n_keys.data_ptr(), st)
+            call("mt_delineate_tiles", self.sorted_keys.data_ptr(), K, self.layout, 0, n, 1, self.T,
+                 self.ranges.data_ptr(), n_keys.data_ptr(),
+                 g["tile_base"].data_ptr() if table else None, st)
+            args = self._composite_args(g)
+            p0 = self.pixel_first[c0]
+            out = self.out_flat[4 * p0 :]
+            grad = self.grad_flat[4 * p0 :]
+            call("mt_intersect_and_composite_forward", args, out.data_ptr(), st)
+            # --- loss (one mean over every camera's pixels) and backward chain
+            h, w = self.shapes[c0]
+            call("mt_image_l2_loss", out.data_ptr(), self.background.data_ptr(),
+                 target[3 * p0 :].data_ptr(), n, h, w,
+                 g["pixel_base"].data_ptr() if table else None, g["pixels"], self.num_pixels,
+                 self.loss.data_ptr(), grad.data_ptr(), st)
+            call("mt_intersect_and_composite_backward", args, out.data_ptr(), grad.data_ptr(),
+                 self.colors_grad.data_ptr(), self.td_grad.data_ptr(), self.sharp_grad.data_ptr(),
+                 self.quirk, st)
+            # SH adjoint fused into the vertex adjoint: the first group overwrites the grid
+            # gradients, later groups add to them.
+            call("mt_create_triangle_vertices_colors_backward", grid_sdf.data_ptr(), self.n_grid,
+                 self.num_sh, self.sh_degree, v.lower_corners.data_ptr(), self.n_col,
+                 self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V,
+                 self.tv.data_ptr(), self.colors.data_ptr(), self.campos[c0:].data_ptr(), n,
+                 self.td_grad.data_ptr(), self.colors_grad.data_ptr(),
+                 self.grid_sdf_grad.data_ptr(), self.grid_sh_grad.data_ptr(),
+                 self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(), int(gi > 0),
+                 self.overflow.data_ptr(), st)
 
     # ------------------------------------------------------------------------------
     def capture(self, grid_vertices, grid_sdf, grid_sh, target) -> None:
@@ -243,26 +330,38 @@ class BatchedTrainingStep:
         return bool(self.overflow.item())
 
     def live_sizes(self) -> dict:
+        """Vertex and face counts, the key count summed over camera groups, and the
+        largest and the last group's key counts."""
+        keys = self.group_keys.tolist()
         return {
             "V_t": int(self.vertex_offsets[-1]),
             "F": int(self.face_offsets[-1]) // 3,
-            "K": int(self.num_keys[0]),
+            "K": int(sum(keys)),
+            "K_max_group": int(max(keys)),
+            "K_last_group": int(keys[-1]),
         }
 
 
 def measure_capacities(voxels: Voxels, level_sets, extrinsics, intrinsics, image_shape, grid_vertices,
-                       grid_sdf, grid_sh, tile_shape=(16, 16), near_plane=0.2) -> Capacities:
-    """Exact vertex / face / key counts of one eager forward (reads three sizes back)."""
+                       grid_sdf, grid_sh, tile_shape=(16, 16), near_plane=0.2,
+                       camera_group: int | None = None) -> Capacities:
+    """Exact vertex and face counts of one eager forward and the largest key count of a
+    camera group (reads the sizes back)."""
     from . import triangle_extraction as te
     from . import triangle_rasterization as tr
+    from .triangle_rasterization.common import TileGrid
 
+    b = extrinsics.shape[0]
+    shapes = camera_shapes(image_shape, b)
     with torch.no_grad():
         tri = te.tessellate_voxels(grid_sdf, grid_vertices, grid_sh, voxels.neighbors,
                                    voxels.lower_corners, voxels.upper_corners, voxels.indices,
                                    level_sets)
-        b = extrinsics.shape[0]
-        grid = TileGrid(tuple(tile_shape), tuple(image_shape), b, level_sets.shape[0])
-        xy, depth = tr.project_vertices(tri.vertices, extrinsics, intrinsics)
-        bounds = tr.bound_triangles(xy, depth, tri.faces, grid, near_plane)
-        k = int(bounds.num_tiles_overlapped.to(torch.int64).sum())
+        per_camera = []
+        for c in range(b):
+            grid = TileGrid(tuple(tile_shape), shapes[c], 1, level_sets.shape[0])
+            xy, depth = tr.project_vertices(tri.vertices, extrinsics[c : c + 1], intrinsics[c : c + 1])
+            bounds = tr.bound_triangles(xy, depth, tri.faces, grid, near_plane)
+            per_camera.append(int(bounds.num_tiles_overlapped.to(torch.int64).sum()))
+    k = max(sum(per_camera[c0:c1]) for c0, c1 in camera_groups(b, camera_group))
     return Capacities(int(tri.vertices.shape[0]), int(tri.faces.shape[0]), k)

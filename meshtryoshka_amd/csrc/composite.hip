@@ -175,6 +175,9 @@ struct TileCoord {
   int pix;                // this thread's pixel (tile-local), -1 for padding threads
   int x, y;
   bool in_image;
+  int height, width;      // the camera's image
+  int64_t slot;           // the tile's pair in tile_ranges
+  int64_t pixel_base;     // the camera's first pixel (planes of height x width)
 };
 
 __device__ __forceinline__ TileCoord tile_coord(int bid_linear, int grid_cols, int T, int th,
@@ -182,6 +185,10 @@ __device__ __forceinline__ TileCoord tile_coord(int bid_linear, int grid_cols, i
   TileCoord c;
   c.cam = bid_linear / T;
   c.tile = bid_linear % T;
+  c.slot = bid_linear;
+  c.height = height;
+  c.width = width;
+  c.pixel_base = int64_t(c.cam) * height * width;
   const int ty = c.tile / grid_cols, tx = c.tile % grid_cols;
   c.x0 = tx * tw;
   c.y0 = ty * th;
@@ -191,6 +198,27 @@ __device__ __forceinline__ TileCoord tile_coord(int bid_linear, int grid_cols, i
   c.x = c.x0 + lx;
   c.y = c.y0 + ly;
   c.in_image = c.pix >= 0 && c.x < width && c.y < height;
+  return c;
+}
+
+// Workgroup bid of a fused launch: its camera and tile, with the camera table when the
+// cameras have different resolutions (block ids run over every camera's tiles in order).
+__device__ __forceinline__ TileCoord locate_tile(const mt_composite_args& a, int64_t bid) {
+  if (a.tile_base == nullptr)
+    return tile_coord(int(bid), a.grid_cols, a.grid_rows * a.grid_cols, a.tile_h, a.tile_w,
+                      a.height, a.width);
+  int lo = 0, hi = a.num_cameras - 1;  // last camera whose first tile is <= bid
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.tile_base[mid] <= bid) lo = mid; else hi = mid - 1;
+  }
+  const int h = a.cam_dims[4 * lo + 0], w = a.cam_dims[4 * lo + 1];
+  const int rows = a.cam_dims[4 * lo + 2], cols = a.cam_dims[4 * lo + 3];
+  const int64_t local = bid - a.tile_base[lo];
+  TileCoord c = tile_coord(int(local), cols, rows * cols, a.tile_h, a.tile_w, h, w);
+  c.cam = lo;
+  c.slot = bid;
+  c.pixel_base = a.pixel_base[lo];
   return c;
 }
 
@@ -329,11 +357,9 @@ __global__ void __launch_bounds__(512) fused_forward_kernel(mt_composite_args a,
   const int npix = a.tile_h * a.tile_w;
   const ChunkLds L = carve_chunk(a.chunk, npix);
   __shared__ int s_wave[kMaxTilePixels / kWave];
-  const int T = a.grid_rows * a.grid_cols;
-  const TileCoord tc = tile_coord(int(xcd_swizzle(blockIdx.x, gridDim.x)), a.grid_cols, T,
-                                  a.tile_h, a.tile_w, a.height, a.width);
-  const int64_t start = a.tile_ranges[2 * (int64_t(tc.cam) * T + tc.tile) + 0];
-  const int64_t end = a.tile_ranges[2 * (int64_t(tc.cam) * T + tc.tile) + 1];
+  const TileCoord tc = locate_tile(a, xcd_swizzle(blockIdx.x, gridDim.x));
+  const int64_t start = a.tile_ranges[2 * tc.slot + 0];
+  const int64_t end = a.tile_ranges[2 * tc.slot + 1];
   const float k = a.sharpness[0];
   const float px = float(tc.x) + 0.5f, py = float(tc.y) + 0.5f;
   const float2* xy = reinterpret_cast<const float2*>(a.xy) + int64_t(tc.cam) * a.num_vertices;
@@ -390,8 +416,8 @@ __global__ void __launch_bounds__(512) fused_forward_kernel(mt_composite_args a,
     if (!__syncthreads_or(active)) break;
   }
   if (tc.in_image) {
-    const int64_t plane = int64_t(a.height) * a.width;
-    float* o = out + int64_t(tc.cam) * 4 * plane + int64_t(tc.y) * a.width + tc.x;
+    const int64_t plane = int64_t(tc.height) * tc.width;
+    float* o = out + 4 * tc.pixel_base + int64_t(tc.y) * tc.width + tc.x;
     o[0] = r;
     o[plane] = g;
     o[2 * plane] = bl;
@@ -429,11 +455,9 @@ __global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a
   const ChunkLds L = carve_chunk(a.chunk, npix);
   __shared__ int s_wave[kMaxTilePixels / kWave];
   __shared__ float s_red[kMaxTilePixels / kWave];
-  const int T = a.grid_rows * a.grid_cols;
-  const TileCoord tc = tile_coord(int(xcd_swizzle(blockIdx.x, gridDim.x)), a.grid_cols, T,
-                                  a.tile_h, a.tile_w, a.height, a.width);
-  const int64_t start = a.tile_ranges[2 * (int64_t(tc.cam) * T + tc.tile) + 0];
-  const int64_t end = a.tile_ranges[2 * (int64_t(tc.cam) * T + tc.tile) + 1];
+  const TileCoord tc = locate_tile(a, xcd_swizzle(blockIdx.x, gridDim.x));
+  const int64_t start = a.tile_ranges[2 * tc.slot + 0];
+  const int64_t end = a.tile_ranges[2 * tc.slot + 1];
   const float k = a.sharpness[0];
   const float px = float(tc.x) + 0.5f, py = float(tc.y) + 0.5f;
   const float2* xy = reinterpret_cast<const float2*>(a.xy) + int64_t(tc.cam) * a.num_vertices;
@@ -442,8 +466,8 @@ __global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a
   float fr = 0.f, fg = 0.f, fb = 0.f, ft = 0.f;    // final accumulator
   float gr = 0.f, gg = 0.f, gb = 0.f, gt = 0.f;    // its gradient
   if (tc.in_image) {
-    const int64_t plane = int64_t(a.height) * a.width;
-    const int64_t o = int64_t(tc.cam) * 4 * plane + int64_t(tc.y) * a.width + tc.x;
+    const int64_t plane = int64_t(tc.height) * tc.width;
+    const int64_t o = 4 * tc.pixel_base + int64_t(tc.y) * tc.width + tc.x;
     fr = out[o];
     fg = out[o + plane];
     fb = out[o + 2 * plane];
@@ -775,16 +799,31 @@ __global__ void __launch_bounds__(256) composite_backward_kernel(
 // gradient where 0 <= x <= 1, like torch.clamp.
 __global__ void __launch_bounds__(256) image_l2_loss_kernel(const float* out, const float* bg,
                                                             const float* target, int64_t B,
-                                                            int64_t HW, float inv_n, float* loss,
-                                                            float* grad) {
+                                                            int64_t HW_uniform,
+                                                            const int64_t* pixel_base,
+                                                            int64_t total, float inv_n,
+                                                            float* loss, float* grad) {
   __shared__ float s_red[256 / kWave];
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   float acc = 0.f;
-  if (p < B * HW) {
-    const int64_t b = p / HW, q = p % HW;
-    const float* o = out + b * 4 * HW + q;
-    const float* t = target + b * 3 * HW + q;
-    float* g = grad + b * 4 * HW + q;
+  if (p < total) {
+    int64_t first, HW;  // the camera's first pixel and pixel count
+    if (pixel_base) {
+      int lo = 0, hi = int(B) - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pixel_base[mid] <= p) lo = mid; else hi = mid - 1;
+      }
+      first = pixel_base[lo];
+      HW = pixel_base[lo + 1] - first;
+    } else {
+      HW = HW_uniform;
+      first = (p / HW) * HW;
+    }
+    const int64_t q = p - first;
+    const float* o = out + 4 * first + q;
+    const float* t = target + 3 * first + q;
+    float* g = grad + 4 * first + q;
     const float trans = o[3 * HW];
     float gt = 0.f;
 #pragma unroll
@@ -814,6 +853,11 @@ int check_composite_args(const mt_composite_args* a, const char* name) {
   MT_REQUIRE(a->num_cameras >= 0 && a->grid_rows >= 0 && a->grid_cols >= 0, "%s: bad grid", name);
   MT_REQUIRE(int64_t(a->num_cameras) * a->grid_rows * a->grid_cols < (int64_t(1) << 31),
              "%s: too many tiles", name);
+  MT_REQUIRE((a->cam_dims == nullptr) == (a->tile_base == nullptr) &&
+                 (a->tile_base == nullptr) == (a->pixel_base == nullptr),
+             "%s: cam_dims, tile_base and pixel_base go together", name);
+  MT_REQUIRE(a->tile_base == nullptr || (a->total_tiles >= 0 && a->total_tiles < (int64_t(1) << 31)),
+             "%s: bad total_tiles", name);
   return MT_OK;
 }
 
@@ -837,7 +881,8 @@ extern "C" int mt_intersect_and_composite_forward(const mt_composite_args* args,
   mt_composite_args a = *args;
   const int npix = a.tile_h * a.tile_w;
   a.chunk = resolve_chunk(args->chunk, npix);
-  const int64_t blocks = int64_t(a.num_cameras) * a.grid_rows * a.grid_cols;
+  const int64_t blocks =
+      a.tile_base ? a.total_tiles : int64_t(a.num_cameras) * a.grid_rows * a.grid_cols;
   if (blocks == 0) return MT_OK;
   mt::fused_forward_kernel<<<static_cast<unsigned>(blocks), block_threads(npix),
                              mt::chunk_lds_bytes(a.chunk, npix), static_cast<hipStream_t>(stream)>>>(
@@ -853,7 +898,8 @@ extern "C" int mt_intersect_and_composite_backward(const mt_composite_args* args
   mt_composite_args a = *args;
   const int npix = a.tile_h * a.tile_w;
   a.chunk = resolve_chunk(args->chunk, npix);
-  const int64_t blocks = int64_t(a.num_cameras) * a.grid_rows * a.grid_cols;
+  const int64_t blocks =
+      a.tile_base ? a.total_tiles : int64_t(a.num_cameras) * a.grid_rows * a.grid_cols;
   if (blocks == 0) return MT_OK;
   mt::fused_backward_kernel<<<static_cast<unsigned>(blocks), block_threads(npix),
                               mt::chunk_lds_bytes(a.chunk, npix), static_cast<hipStream_t>(stream)>>>(
@@ -862,13 +908,17 @@ extern "C" int mt_intersect_and_composite_backward(const mt_composite_args* args
 }
 
 extern "C" int mt_image_l2_loss(const float* out, const float* background, const float* target,
-                                int32_t B, int32_t H, int32_t W, float* loss, float* grad_out,
-                                mt_stream_t stream) {
-  MT_REQUIRE(B >= 0 && H >= 0 && W >= 0, "mt_image_l2_loss: negative size");
-  const int64_t n = int64_t(B) * H * W;
+                                int32_t B, int32_t H, int32_t W, const int64_t* pixel_base,
+                                int64_t total_pixels, int64_t mean_pixels, float* loss,
+                                float* grad_out, mt_stream_t stream) {
+  MT_REQUIRE(B >= 0 && H >= 0 && W >= 0 && total_pixels >= 0 && mean_pixels >= 0,
+             "mt_image_l2_loss: negative size");
+  const int64_t n = pixel_base ? total_pixels : int64_t(B) * H * W;
   if (n == 0) return MT_OK;
+  const int64_t m = mean_pixels > 0 ? mean_pixels : n;
   mt::image_l2_loss_kernel<<<mt::blocks_for(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(
-      out, background, target, B, int64_t(H) * W, 1.0f / float(3 * n), loss, grad_out);
+      out, background, target, B, int64_t(H) * W, pixel_base, n, 1.0f / float(3 * m), loss,
+      grad_out);
   return mt::check_launch("mt_image_l2_loss");
 }
 

@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(kBlock) keys_kernel(const float* depths, int64
 __global__ void __launch_bounds__(kBlock) key_block_counts_kernel(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F, int B,
     int th, int tw, int rmin, int cmin, int rmax, int cmax, float near_plane,
-    const int32_t* live_x3, int32_t* block_counts) {
+    const int32_t* cam_dims, const int32_t* live_x3, int32_t* block_counts) {
   __shared__ int32_t s_wave[kBlock / kWave];
   const int64_t f = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   const int64_t nblk = gridDim.x;
@@ -225,8 +225,10 @@ __global__ void __launch_bounds__(kBlock) key_block_counts_kernel(
   for (int b = 0; b < B; ++b) {
     int32_t count = 0;
     if (keyed) {
-      const TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic, th, tw, rmin, cmin,
-                                        rmax, cmax, near_plane);
+      const int r0 = cam_dims ? 0 : rmin, c0 = cam_dims ? 0 : cmin;
+      const int r1 = cam_dims ? cam_dims[4 * b + 2] : rmax, c1 = cam_dims ? cam_dims[4 * b + 3] : cmax;
+      const TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic, th, tw, r0, c0,
+                                        r1, c1, near_plane);
       count = (r.y1 - r.y0) * (r.x1 - r.x0);
     }
     const int32_t w = wave_sum(count);
@@ -245,14 +247,21 @@ __global__ void __launch_bounds__(kBlock) key_block_counts_kernel(
 __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
     const int32_t* live_x3, const int32_t* boundaries, int S, int th, int tw, int rmin, int cmin,
-    int rmax, int cmax, float near_plane, const int64_t* block_incl, int grid_cols, KeyFields kf,
-    int64_t* keys, int32_t* values, int64_t kcap, int32_t* overflow) {
+    int rmax, int cmax, float near_plane, const int32_t* cam_dims, const int64_t* block_incl,
+    int grid_cols, KeyFields kf, int64_t* keys, int32_t* values, int64_t kcap, int32_t* overflow) {
   __shared__ int32_t s_prefix[kBlock + 1];
   __shared__ int32_t s_r0[kBlock], s_c0[kBlock], s_w[kBlock];
   __shared__ uint64_t s_key_hi[kBlock];
   __shared__ int32_t s_wave[kBlock / kWave];
   __shared__ int64_t s_block_start;
   const int b = blockIdx.y;
+  if (cam_dims) {  // this camera's own tile grid
+    rmin = 0;
+    cmin = 0;
+    rmax = cam_dims[4 * b + 2];
+    cmax = cam_dims[4 * b + 3];
+    grid_cols = cmax;
+  }
   const int64_t f = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   int32_t count = 0;
   int64_t ia = 0, ib = 0, ic = 0;
@@ -325,13 +334,15 @@ __device__ __forceinline__ void run_index(uint64_t key, const KeyFields& kf, boo
 __global__ void __launch_bounds__(kBlock) delineate_kernel(const int64_t* keys, int64_t n,
                                                            KeyFields kf, int separate, int B,
                                                            int S_out, int T, int64_t* ranges,
-                                                           const int64_t* n_device) {
+                                                           const int64_t* n_device,
+                                                           const int64_t* tile_base) {
   if (n_device != nullptr && *n_device < n) n = *n_device;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t c, s, t;
   run_index(static_cast<uint64_t>(keys[i]), kf, separate != 0, c, s, t);
   auto slot = [&](uint32_t cc, uint32_t ss, uint32_t tt) -> int64_t {
+    if (tile_base) return (tile_base[cc] + tt) * 2;  // per-camera tile grids, S' = 1
     return ((int64_t(cc) * S_out + ss) * T + tt) * 2;
   };
   if (i > 0) {
@@ -346,34 +357,34 @@ __global__ void __launch_bounds__(kBlock) delineate_kernel(const int64_t* keys, 
 }
 
 // ------------------------------------------------------------------------------------
-// Camera positions: inverse(E)[:3, 3] in double precision (Gauss-Jordan, partial pivot).
+// Camera positions: inverse(E)[:3, 3] in double precision, from the adjugate: column 3 of
+// the inverse is the cofactors of E's row 3 over det(E). Every index is a compile-time
+// constant, so the matrix stays in registers (no private memory).
 // ------------------------------------------------------------------------------------
+
+__device__ __forceinline__ double det3_rows012(const double (&m)[4][4], int c0, int c1, int c2) {
+  return m[0][c0] * (m[1][c1] * m[2][c2] - m[1][c2] * m[2][c1]) -
+         m[0][c1] * (m[1][c0] * m[2][c2] - m[1][c2] * m[2][c0]) +
+         m[0][c2] * (m[1][c0] * m[2][c1] - m[1][c1] * m[2][c0]);
+}
 
 __global__ void camera_positions_kernel(const float* E, int B, float* out) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
-  double m[4][8];
+  double m[4][4];
+#pragma unroll
   for (int r = 0; r < 4; ++r)
-    for (int c = 0; c < 8; ++c) m[r][c] = (c < 4) ? double(E[16 * b + 4 * r + c]) : (c - 4 == r ? 1.0 : 0.0);
-  for (int c = 0; c < 4; ++c) {
-    int p = c;
-    for (int r = c + 1; r < 4; ++r)
-      if (fabs(m[r][c]) > fabs(m[p][c])) p = r;
-    if (p != c)
-      for (int q = 0; q < 8; ++q) {
-        const double tmp = m[c][q];
-        m[c][q] = m[p][q];
-        m[p][q] = tmp;
-      }
-    const double inv = 1.0 / m[c][c];
-    for (int q = 0; q < 8; ++q) m[c][q] *= inv;
-    for (int r = 0; r < 4; ++r) {
-      if (r == c) continue;
-      const double fct = m[r][c];
-      for (int q = 0; q < 8; ++q) m[r][q] -= fct * m[c][q];
-    }
-  }
-  for (int r = 0; r < 3; ++r) out[3 * b + r] = static_cast<float>(m[r][4 + 3]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) m[r][c] = double(E[16 * b + 4 * r + c]);
+  // Cofactor C(3, c) = (-1)^(3 + c) * minor(3, c), the minor over rows 0..2 without column c.
+  double cof[4];
+  cof[0] = -det3_rows012(m, 1, 2, 3);
+  cof[1] = det3_rows012(m, 0, 2, 3);
+  cof[2] = -det3_rows012(m, 0, 1, 3);
+  cof[3] = det3_rows012(m, 0, 1, 2);
+  const double det = m[3][0] * cof[0] + m[3][1] * cof[1] + m[3][2] * cof[2] + m[3][3] * cof[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) out[3 * b + r] = static_cast<float>(cof[r] / det);
 }
 
 // ------------------------------------------------------------------------------------
@@ -483,13 +494,14 @@ extern "C" int mt_key_block_counts(const float* xy, const float* depths, int64_t
                                    const int32_t* faces, int64_t F, const int32_t* num_faces_x3,
                                    int32_t B, int32_t tile_h, int32_t tile_w, int32_t row_min,
                                    int32_t col_min, int32_t row_max, int32_t col_max,
-                                   float near_plane, int32_t* block_counts, mt_stream_t stream) {
+                                   float near_plane, const int32_t* cam_dims,
+                                   int32_t* block_counts, mt_stream_t stream) {
   MT_REQUIRE(F >= 0 && B >= 0 && tile_h > 0 && tile_w > 0, "mt_key_block_counts: bad sizes");
   if (F == 0 || B == 0) return MT_OK;
   mt::key_block_counts_kernel<<<mt::blocks_for(F, mt::kBlock), mt::kBlock, 0,
                                 static_cast<hipStream_t>(stream)>>>(
       xy, depths, V, faces, F, B, tile_h, tile_w, row_min, col_min, row_max, col_max, near_plane,
-      num_faces_x3, block_counts);
+      cam_dims, num_faces_x3, block_counts);
   return mt::check_launch("mt_key_block_counts");
 }
 
@@ -497,28 +509,31 @@ extern "C" int mt_generate_keys_from_geometry(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
     const int32_t* num_faces_x3, const int32_t* shell_face_boundaries, int32_t S, int32_t B,
     int32_t tile_h, int32_t tile_w, int32_t row_min, int32_t col_min, int32_t row_max,
-    int32_t col_max, float near_plane, const int64_t* block_offsets, int32_t grid_cols,
-    mt_key_layout layout, int64_t* keys, int32_t* values, int64_t key_capacity, int32_t* overflow,
-    mt_stream_t stream) {
+    int32_t col_max, float near_plane, const int32_t* cam_dims, const int64_t* block_offsets,
+    int32_t grid_cols, mt_key_layout layout, int64_t* keys, int32_t* values, int64_t key_capacity,
+    int32_t* overflow, mt_stream_t stream) {
   MT_REQUIRE(F >= 0 && B >= 0 && B <= 65535 && key_capacity >= 0 && tile_h > 0 && tile_w > 0,
              "mt_generate_keys_from_geometry: bad sizes");
   if (F == 0 || B == 0) return MT_OK;
   dim3 grid(mt::blocks_for(F, mt::kBlock), static_cast<unsigned>(B));
   mt::keys_from_geometry_kernel<<<grid, mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
       xy, depths, V, faces, F, num_faces_x3, shell_face_boundaries, S, tile_h, tile_w, row_min,
-      col_min, row_max, col_max, near_plane, block_offsets, grid_cols, mt::to_fields(layout), keys,
-      values, key_capacity, overflow);
+      col_min, row_max, col_max, near_plane, cam_dims, block_offsets, grid_cols,
+      mt::to_fields(layout), keys, values, key_capacity, overflow);
   return mt::check_launch("mt_generate_keys_from_geometry");
 }
 
 extern "C" int mt_delineate_tiles(const int64_t* sorted_keys, int64_t n, mt_key_layout layout,
                                   int32_t separate_shells, int32_t B, int32_t S_out, int32_t T,
                                   int64_t* tile_ranges, const int64_t* n_device,
-                                  mt_stream_t stream) {
+                                  const int64_t* tile_base, mt_stream_t stream) {
   MT_REQUIRE(n >= 0, "mt_delineate_tiles: negative length");
+  MT_REQUIRE(tile_base == nullptr || (separate_shells == 0 && S_out == 1),
+             "mt_delineate_tiles: per-camera tile grids need one range per (camera, tile)");
   if (n == 0) return MT_OK;
   mt::delineate_kernel<<<mt::blocks_for(n, mt::kBlock), mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
-      sorted_keys, n, mt::to_fields(layout), separate_shells, B, S_out, T, tile_ranges, n_device);
+      sorted_keys, n, mt::to_fields(layout), separate_shells, B, S_out, T, tile_ranges, n_device,
+      tile_base);
   return mt::check_launch("mt_delineate_tiles");
 }
 

@@ -66,4 +66,13 @@ __device__ __forceinline__ float sh_channel_pre(const float* basis, const float 
   return acc + 0.5f;
 }
 
+// The same sum for one channel whose coefficients are in coef[0..kN).
+template <int kN>
+__device__ __forceinline__ float sh_pre(const float* basis, const float* coef) {
+  float acc = basis[0] * coef[0];
+#pragma unroll
+  for (int k = 1; k < kN; ++k) acc += basis[k] * coef[k];
+  return acc + 0.5f;
+}
+
 }  // namespace mt

@@ -435,7 +435,7 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
     const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
     int S, const int32_t* offsets, int64_t nt, const float* tv, const float* colors,
     const float* cams, int B, const float* g_d, const float* g_col, float* gg_d, float* gg_sh,
-    float* part, const int32_t* maps) {
+    float* part, const int32_t* maps, bool accumulate) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 1 + 3 * kN;
   if (maps[4 * ng] != 0) return;  // not one-to-one: reported by phase 2
@@ -481,11 +481,15 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
       for (int ch = 0; ch < 3; ++ch) pa[(1 + 3 * k + ch) * n + c] = acce[k][ch];
   }
   const int64_t g = col.v0;
-  gg_d[g] = acc0_d;
+  gg_d[g] = accumulate ? gg_d[g] + acc0_d : acc0_d;
 #pragma unroll
   for (int k = 0; k < kNsh; ++k)
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch) gg_sh[(int64_t(k) * ng + g) * 3 + ch] = k < kN ? acc0[k < kN ? k : 0][ch] : 0.f;
+    for (int ch = 0; ch < 3; ++ch) {
+      const float v = k < kN ? acc0[k < kN ? k : 0][ch] : 0.f;
+      float* o = gg_sh + (int64_t(k) * ng + g) * 3 + ch;
+      *o = accumulate ? *o + v : v;
+    }
 }
 
 // Phase 2, one thread per grid vertex: add the axis-end sums of the (at most three)
@@ -493,7 +497,7 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
 template <int kNsh, int kDeg>
 __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
     int64_t ng, int64_t n, const float* part, float* gg_d, float* gg_sh, const int32_t* maps,
-    int32_t* status) {
+    bool accumulate, int32_t* status) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 1 + 3 * kN;
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -502,7 +506,9 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
     return;
   }
   if (g >= ng) return;
-  const bool origin = maps[g] >= 0;
+  // Phase 1 wrote (or added to) every column origin; other grid vertices start from zero
+  // unless the call accumulates.
+  const bool origin = maps[g] >= 0 || accumulate;
   int32_t ends[3];
   bool any = false;
 #pragma unroll
@@ -788,7 +794,7 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
     const int32_t* vertex_offsets, int64_t num_triangle_vertices, const float* vertices,
     const float* colors, const float* camera_positions, int32_t B, const float* grad_sdf,
     const float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
-    size_t workspace_bytes, int32_t* status, mt_stream_t stream) {
+    size_t workspace_bytes, int32_t accumulate, int32_t* status, mt_stream_t stream) {
   MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0 && B >= 0,
              "mt_create_triangle_vertices_colors_backward: negative size");
   MT_REQUIRE(num_sh == 1 || num_sh == 4 || num_sh == 9 || num_sh == 16,
@@ -822,9 +828,10 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
       mt::colors_adjoint_columns_kernel<N, D><<<cgrid, mt::kBlock, 0, st>>>(                     \
           grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S,                \
           vertex_offsets, num_triangle_vertices, vertices, colors, camera_positions, B, grad_sdf, \
-          grad_colors, grid_sdf_grad, grid_sh_grad, part, maps);                                 \
+          grad_colors, grid_sdf_grad, grid_sh_grad, part, maps, accumulate != 0);                \
     mt::colors_adjoint_ends_kernel<N, D><<<ggrid, mt::kBlock, 0, st>>>(                          \
-        num_grid_vertices, num_columns, part, grid_sdf_grad, grid_sh_grad, maps, status);        \
+        num_grid_vertices, num_columns, part, grid_sdf_grad, grid_sh_grad, maps, accumulate != 0, \
+        status);                                                                                 \
   } while (0)
   const int key = num_sh * 4 + sh_degree;
   switch (key) {

@@ -136,7 +136,7 @@ def render_batch(scene: ResidentScene, extrinsics: torch.Tensor, intrinsics: tor
     call("mt_sort_keys", keys.data_ptr(), vals.data_ptr(), k_cap, skeys.data_ptr(), svals.data_ptr(),
          sort_ws.data_ptr(), sort_ws.numel(), n_dev.data_ptr(), st)
     call("mt_delineate_tiles", skeys.data_ptr(), k_cap, _LAYOUT, 0, b, 1, t, ranges.data_ptr(),
-         n_dev.data_ptr(), st)
+         n_dev.data_ptr(), None, st)
     call("mt_camera_positions", E.data_ptr(), b, campos.data_ptr(), st)
     call("mt_evaluate_sh_forward", scene.vertices.data_ptr(), v, scene.spherical_harmonics.data_ptr(),
          scene.spherical_harmonics.shape[0], scene.sh_degree, campos.data_ptr(), b,

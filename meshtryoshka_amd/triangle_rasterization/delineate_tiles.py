@@ -25,5 +25,5 @@ def delineate_tiles(sorted_paired_keys: PairedKeys, grid: TileGrid,
     if n > 0:
         _native.call("mt_delineate_tiles", keys.contiguous().data_ptr(), n, to_layout(key_defines),
                      int(separate_shells), shape[0], shape[1], shape[2],
-                     tile_boundaries.data_ptr(), None, _native.stream(keys))
+                     tile_boundaries.data_ptr(), None, None, _native.stream(keys))
     return tile_boundaries

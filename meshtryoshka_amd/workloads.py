@@ -11,7 +11,10 @@ then the target image), so the same seed gives the same tensors on any device.
   ``band`` cells of the zero set of ``min(spheres, ground plane y = -0.6)`` are occupied;
   corner SDF = clean SDF + ``noise`` N(0, 1); SH degree 3 with DC ``U(0, 1)`` and higher
   bands ``N(0, 0.05)``; shells spaced ``1/512`` apart around 0; cameras on the upper
-  hemisphere of radius 2 looking at the origin (+y up, 60 deg vertical FOV).
+  hemisphere of radius 2 looking at the origin (+y up, 60 deg vertical FOV). With
+  ``resolutions``, each camera's (H, W) is drawn uniformly from that list (after the
+  SH coefficients, before the camera poses), and ``Cameras.image_shape`` is the per-camera
+  list.
 """
 
 from __future__ import annotations
@@ -28,7 +31,7 @@ from .triangle_extraction.common import pack_occupancy
 class Cameras:
     extrinsics: torch.Tensor  # f32[B, 4, 4] OpenCV world-to-camera
     intrinsics: torch.Tensor  # f32[B, 3, 3] pixels
-    image_shape: tuple[int, int]  # (H, W)
+    image_shape: tuple  # (H, W) shared by every camera, or a list of per-camera (H, W)
 
     def to(self, device):
         return Cameras(self.extrinsics.to(device), self.intrinsics.to(device), self.image_shape)
@@ -195,6 +198,7 @@ def sphere_field(
     image_shape=(822, 1237),
     sharpness: float = 50.0,
     device="cpu",
+    resolutions=None,
 ) -> SceneSpec:
     gen = torch.Generator("cpu").manual_seed(seed)
     centres = (torch.rand((num_spheres, 3), generator=gen) * 1.6 - 0.8).float()
@@ -224,7 +228,8 @@ def sphere_field(
         near_plane=0.2,
         seed=seed,
         field_fn=field_fn,
-        extra={"generator": gen, "num_cameras": num_cameras, "image_shape": image_shape},
+        extra={"generator": gen, "num_cameras": num_cameras, "image_shape": image_shape,
+               "resolutions": None if resolutions is None else [tuple(r) for r in resolutions]},
     )
     return spec
 
@@ -241,9 +246,15 @@ def finish_fields(spec: SceneSpec, unit_vertices: torch.Tensor):
     world, sdf, sh = spec.field_fn(unit_vertices, gen)
     if spec.cameras is None:
         nc = spec.extra["num_cameras"]
-        cams = hemisphere_cameras(gen, nc, spec.extra["image_shape"])
+        choices = spec.extra.get("resolutions")
+        shapes = None
+        if choices:
+            pick = torch.randint(len(choices), (nc,), generator=gen).tolist()
+            shapes = [choices[i] for i in pick]
+        cams = hemisphere_cameras(gen, nc, spec.extra["image_shape"], shapes=shapes)
         spec.cameras = Cameras(
-            torch.stack([c[0] for c in cams]), torch.stack([c[1] for c in cams]), cams[0][2]
+            torch.stack([c[0] for c in cams]), torch.stack([c[1] for c in cams]),
+            cams[0][2] if shapes is None else [c[2] for c in cams],
         )
     return world, sdf, sh
 

@@ -99,3 +99,83 @@ def test_overflow_is_reported_not_written(scene):
     step.run(world, sdf, sh, target)
     torch.cuda.synchronize()
     assert step.overflowed()
+
+
+def test_camera_groups_match_one_group(scene):
+    """Rendering the cameras in groups (one after another inside the step) gives the same
+    images and loss, and grid gradients equal up to float summation order."""
+    spec, vox, world, sdf, sh, cams, target = scene
+    one = make_step(scene)
+    one.run(world, sdf, sh, target)
+    for group in (1, 2):
+        step = make_step(scene, camera_group=group)
+        assert len(step.groups) == -(-3 // group)
+        step.run(world, sdf, sh, target)
+        assert not step.overflowed()
+        assert torch.equal(step.out, one.out), group
+        assert abs(float(step.loss) - float(one.loss)) <= 1e-6 * float(one.loss)
+        assert _rel(step.grid_sdf_grad, one.grid_sdf_grad) < 1e-5
+        assert _rel(step.grid_sh_grad, one.grid_sh_grad) < 1e-5
+
+
+@pytest.mark.parametrize("camera_group", [None, 2])
+@pytest.mark.parametrize("tile_shape", [(16, 16), (8, 8), (32, 8)])
+def test_mixed_resolutions_match_plain_per_camera(scene, tile_shape, camera_group):
+    """Cameras of different resolutions (odd sizes, partial tiles) in one step: every
+    camera's image equals the plain path's render of that camera alone, the loss is the mean
+    over all cameras' pixels, and the grid gradients match its autograd (summation order
+    only). Replays of the captured step reproduce the eager step."""
+    spec, vox, world, sdf, sh, cams, _ = scene
+    dev = sdf.device
+    shapes = [(96, 128), (61, 83), (130, 47)]
+    intr = torch.stack([workloads.pinhole(s) for s in shapes]).to(dev)
+    gen = torch.Generator("cpu").manual_seed(3)
+    targets = [torch.rand((3, h, w), generator=gen).to(dev) for h, w in shapes]
+    flat_target = torch.cat([t.reshape(-1) for t in targets])
+    ls = spec.level_sets.to(dev)
+
+    # Plain path, camera by camera, one mean over all pixels.
+    sdf_p = sdf.clone().requires_grad_(True)
+    sh_p = sh.clone().requires_grad_(True)
+    tri = te.tessellate_voxels(sdf_p, world, sh_p, vox.neighbors, vox.lower_corners,
+                               vox.upper_corners, vox.indices, ls)
+    total = sum(h * w for h, w in shapes)
+    loss = 0.0
+    imgs = []
+    for c, (h, w) in enumerate(shapes):
+        img = tr.render(cams.extrinsics[c : c + 1], intr[c : c + 1], (h, w), tri.vertices.detach(),
+                        tri.faces, tri.face_boundaries, tri.signed_distances,
+                        tri.spherical_harmonics, spec.sh_degree,
+                        torch.tensor(spec.sharpness, device=dev),
+                        torch.tensor(spec.background, device=dev), tile_shape=tile_shape)
+        imgs.append(img.detach()[0])
+        loss = loss + ((img[0] - targets[c]) ** 2).sum()
+    loss = loss / (3 * total)
+    loss.backward()
+
+    cap = batched.measure_capacities(vox, ls, cams.extrinsics, intr, shapes, world, sdf, sh,
+                                     tile_shape=tile_shape, camera_group=camera_group).scaled(1.2)
+    step = batched.BatchedTrainingStep(vox, ls, cams.extrinsics, intr, shapes, sh.shape[0],
+                                       spec.sh_degree, spec.sharpness, spec.background, cap,
+                                       tile_shape=tile_shape, camera_group=camera_group)
+    step.run(world, sdf, sh, flat_target)
+    assert not step.overflowed()
+    bg = step.background[:, None, None]
+    for c in range(len(shapes)):
+        o = step.image(c)
+        rgb = (o[:3] + o[3:4] * bg).clip(0, 1)
+        assert torch.equal(rgb, imgs[c]), c
+    assert abs(float(step.loss) - float(loss)) <= 1e-6 * float(loss)
+    assert _rel(step.grid_sdf_grad, sdf_p.grad) < 1e-5
+    assert _rel(step.grid_sh_grad, sh_p.grad) < 1e-5
+
+    ref_out = step.out.clone()
+    ref_sdf = step.grid_sdf_grad.clone()
+    static = [t.clone() for t in (world, sdf, sh, flat_target)]
+    step.capture(*static)
+    for i in range(3):
+        step.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(step.out, ref_out), i
+        assert _rel(step.grid_sdf_grad, ref_sdf) < 1e-5, i
+        _ = (static[1] * 2).sum().item()

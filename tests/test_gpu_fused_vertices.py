@@ -97,12 +97,27 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
     _native.call("mt_create_triangle_vertices_colors_backward", sdf.data_ptr(), ng, nsh, deg,
                  vox.lower_corners.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V, tv2.data_ptr(),
                  col2.data_ptr(), campos.data_ptr(), b, g_d.data_ptr(), g_col.data_ptr(),
-                 gg_d2.data_ptr(), gg_sh2.data_ptr(), ws2.data_ptr(), ws2.numel(), status.data_ptr(),
-                 _stream())
+                 gg_d2.data_ptr(), gg_sh2.data_ptr(), ws2.data_ptr(), ws2.numel(), 0,
+                 status.data_ptr(), _stream())
     assert int(status) == 0
     # Same terms, summed per column and axis first: equal up to float summation order.
     torch.testing.assert_close(gg_d2, gg_d, rtol=1e-5, atol=1e-6 * float(gg_d.abs().max()))
     torch.testing.assert_close(gg_sh2, gg_sh, rtol=1e-5, atol=1e-6 * float(gg_sh.abs().max()))
+
+    # Accumulating mode adds to what the grid gradients hold: splitting the cameras into
+    # two calls (the second accumulating) gives the one-call result.
+    gg_d3 = torch.full((ng,), float("nan"), **f32)
+    gg_sh3 = torch.full((nsh, ng, 3), float("nan"), **f32)
+    for i, (c0, c1) in enumerate([(0, 1), (1, b)]):
+        gd_part = g_d if i == 0 else torch.zeros_like(g_d)
+        _native.call("mt_create_triangle_vertices_colors_backward", sdf.data_ptr(), ng, nsh, deg,
+                     vox.lower_corners.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V,
+                     tv2.data_ptr(), col2[c0:].data_ptr(), campos[c0:].data_ptr(), c1 - c0,
+                     gd_part.data_ptr(), g_col[c0:].data_ptr(), gg_d3.data_ptr(), gg_sh3.data_ptr(),
+                     ws2.data_ptr(), ws2.numel(), i, status.data_ptr(), _stream())
+    assert int(status) == 0
+    torch.testing.assert_close(gg_d3, gg_d2, rtol=1e-5, atol=1e-6 * float(gg_d.abs().max()))
+    torch.testing.assert_close(gg_sh3, gg_sh2, rtol=1e-5, atol=1e-6 * float(gg_sh.abs().max()))
 
     # FP64 oracle adjoint of the same frozen pipeline (vertex adjoint of the SH adjoint).
     vnp = ox.extract_voxels(spec.occupancy.cpu().numpy())
@@ -143,7 +158,7 @@ def test_fused_adjoint_reports_shared_origins(case):
     _native.call("mt_create_triangle_vertices_colors_backward", sdf.data_ptr(), ng, nsh,
                  spec.sh_degree, lower.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V,
                  tv.data_ptr(), zeros_c.data_ptr(), campos.data_ptr(), b, zeros_v.data_ptr(),
-                 zeros_c.data_ptr(), gg_d.data_ptr(), gg_sh.data_ptr(), ws.data_ptr(), ws.numel(),
+                 zeros_c.data_ptr(), gg_d.data_ptr(), gg_sh.data_ptr(), ws.data_ptr(), ws.numel(), 0,
                  status.data_ptr(), _stream())
     torch.cuda.synchronize()
     assert int(status) == 2

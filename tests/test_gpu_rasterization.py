@@ -90,7 +90,7 @@ def test_keys_from_geometry_match_oracle(scene):
         counts = torch.empty(b * nblk, dtype=torch.int32, device="cuda")
         _native.call("mt_key_block_counts", xy.data_ptr(), depth.data_ptr(), v, faces.data_ptr(), f,
                      live_x3.data_ptr(), b, th, tw, 0, 0, scene.rows, scene.cols,
-                     float(scene.spec.near_plane), counts.data_ptr(), st)
+                     float(scene.spec.near_plane), None, counts.data_ptr(), st)
         offs = torch.empty(b * nblk, dtype=torch.int64, device="cuda")
         ws = _native.workspace(_native.query("mt_scan_workspace_bytes", b * nblk), xy.device)
         _native.call("mt_inclusive_scan_i32_to_i64", counts.data_ptr(), offs.data_ptr(), b * nblk,
@@ -105,7 +105,7 @@ def test_keys_from_geometry_match_oracle(scene):
         _native.call("mt_generate_keys_from_geometry", xy.data_ptr(), depth.data_ptr(), v,
                      faces.data_ptr(), f, live_x3.data_ptr(), to_dev(t.face_boundaries).data_ptr(),
                      scene.S, b, th, tw, 0, 0, scene.rows, scene.cols, float(scene.spec.near_plane),
-                     offs.data_ptr(), scene.cols, layout, keys.data_ptr(), vals.data_ptr(),
+                     None, offs.data_ptr(), scene.cols, layout, keys.data_ptr(), vals.data_ptr(),
                      keys.numel(), over.data_ptr(), st)
         assert int(over) == 0
         # Oracle keys of the live faces, in (camera, face, tile) order.
