@@ -37,7 +37,7 @@ FP32_PEAK_TFLOPS = 157.3
 WORKLOADS = {
     "sphere-field-4cam": {},
     "sphere-field-32cam-mixed": {
-        "spheres": 256, "resolution": 512, "shells": 7, "cameras": 32, "camera_group": 8,
+        "spheres": 256, "resolution": 512, "shells": 7, "cameras": 32, "camera_group": 16,
         "resolutions": ["320x240", "640x480", "1237x822", "1920x1080", "2473x1643"],
         "steps": 3, "warmup": 1,
     },
@@ -75,6 +75,10 @@ def parse_args(argv=None):
                    help="time only the batched step with --tile / --chunk (no plain path, no "
                         "autotuner, no CPU baseline): the run profiled per kernel")
     p.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
+    p.add_argument("--only-serving", action="store_true",
+                   help="run only the serving sweep (prints its JSON object)")
+    p.add_argument("--no-mode-compare", action="store_true",
+                   help="skip the fused-vs-explicit peak memory and gradient comparison")
     p.add_argument("--serve-spheres", type=int, nargs="+", default=[8, 16, 32, 64, 96, 128, 192, 256],
                    help="sphere count of each resident scene")
     p.add_argument("--serve-resolution", type=int, default=512,
@@ -83,8 +87,10 @@ def parse_args(argv=None):
     p.add_argument("--serve-rates", type=float, nargs="+",
                    default=[50.0, 100.0, 200.0, 500.0, 1000.0, 2000.0])
     p.add_argument("--serve-seconds", type=float, default=3.0)
-    p.add_argument("--serve-budget-gib", type=int, default=48)
-    p.add_argument("--serve-max-batch", type=int, default=8)
+    p.add_argument("--serve-budget-gib", type=int, default=128)
+    p.add_argument("--serve-max-batch", type=int, default=16)
+    p.add_argument("--serve-deadline", type=float, default=1.0,
+                   help="requests not admitted within this many seconds of arrival are shed")
     p.add_argument("--probe", action="store_true",
                    help="build the scene, print its sizes and memory, and exit")
     args = p.parse_args(argv)
@@ -154,7 +160,7 @@ def build_scene(args, device):
 # ------------------------------------------------------------------------------------
 
 
-def plain_per_camera_step(sc):
+def plain_per_camera_step(sc, mode="fused"):
     import torch
 
     from meshtryoshka_amd import triangle_extraction as te
@@ -174,7 +180,7 @@ def plain_per_camera_step(sc):
                         (h, w), tri.vertices.detach(), tri.faces, tri.face_boundaries,
                         tri.signed_distances, tri.spherical_harmonics, sc.spec.sh_degree,
                         sc.sharpness, sc.background, tile_shape=sc.spec.tile_shape,
-                        near_plane=sc.spec.near_plane)
+                        near_plane=sc.spec.near_plane, mode=mode)
         if sc.uniform:
             tgt = sc.target[c : c + 1]
         else:
@@ -184,6 +190,38 @@ def plain_per_camera_step(sc):
         loss = loss + ((img - tgt) ** 2).sum() / (3 * sc.num_pixels)
     loss.backward()
     return loss
+
+
+def compare_render_modes(sc, log):
+    """Plain per-camera step in fused mode (backward recomputes the hits) and in explicit
+    mode (stores 12 B per (camera, shell, pixel) of hit data): peak device memory of each
+    step, its duration, and how far the explicit gradients are from the fused ones."""
+    import torch
+
+    out = {}
+    grads = {}
+    for mode in ("fused", "explicit"):
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        t0 = time.perf_counter()
+        loss = plain_per_camera_step(sc, mode=mode)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        peak = torch.cuda.max_memory_allocated() - base
+        grads[mode] = (sc.sdf.grad.clone(), sc.sh.grad.clone())
+        out[mode] = {"peak_step_gib": round(peak / 2**30, 3), "ms_per_step": round(1000 * dt, 1),
+                     "loss": float(loss)}
+        log(f"[modes] {mode}: peak {peak / 2**30:.2f} GiB above the resident scene, {1000 * dt:.1f} ms")
+    (fs, fh), (es, eh) = grads["fused"], grads["explicit"]
+    out["grad_sdf_rel_diff"] = float((es - fs).abs().max() / fs.abs().max())
+    out["grad_sh_rel_diff"] = float((eh - fh).abs().max() / fh.abs().max())
+    out["loss_abs_diff"] = abs(out["fused"]["loss"] - out["explicit"]["loss"])
+    log(f"[modes] explicit vs fused gradients: sdf {out['grad_sdf_rel_diff']:.2e}, "
+        f"sh {out['grad_sh_rel_diff']:.2e} (max-abs / max)")
+    sc.sdf.grad = None
+    sc.sh.grad = None
+    return out
 
 
 # ------------------------------------------------------------------------------------
@@ -453,8 +491,9 @@ SERVE_SHAPES = ((256, 256), (512, 512), (768, 1024), (1080, 1920))
 def serving_benchmark(args, device, log):
     """Resident scenes of the sphere-field generator (one per entry of args.serve_spheres);
     requests = (scene, random hemisphere pose, resolution from SERVE_SHAPES), forward only.
-    For each offered rate a Poisson stream runs for args.serve_seconds; reports completed
-    req/s and p50/p99 latency."""
+    For each offered rate a Poisson stream runs for args.serve_seconds and then drains;
+    requests not admitted within args.serve_deadline seconds are shed. Reports completed
+    req/s, p50/p99 latency of the completed requests and the number shed."""
     import random
 
     import torch
@@ -466,6 +505,7 @@ def serving_benchmark(args, device, log):
     spheres = list(args.serve_spheres)
     n = len(spheres)
     scenes, cells = {}, []
+    cal_gen = torch.Generator("cpu").manual_seed(args.seed + 5)
     t0 = time.time()
     for sid, ns in enumerate(spheres):
         spec = workloads.sphere_field(seed=100 + sid, num_spheres=ns,
@@ -477,8 +517,12 @@ def serving_benchmark(args, device, log):
         sc = serving.tessellate_scene(vox, world.to(device), sdf.to(device), sh.to(device),
                                       spec.level_sets.to(device), spec.sh_degree, spec.sharpness,
                                       spec.background)
-        cams = spec.cameras.to(device)
-        sc.keys_per_pixel = serving.calibrate_key_density(sc, cams.extrinsics, cams.intrinsics, (512, 512))
+        cal = {}
+        for shape in SERVE_SHAPES:
+            poses = workloads.hemisphere_cameras(cal_gen, 4, shape)
+            cal[shape] = (torch.stack([p[0] for p in poses]).to(device),
+                          torch.stack([p[1] for p in poses]).to(device))
+        serving.calibrate_keys(sc, cal)
         cells.append(int(vox.num_voxels))
         scenes[sid] = sc
         del vox, world, sdf, sh
@@ -500,24 +544,24 @@ def serving_benchmark(args, device, log):
 
     # Warm-up: one request per (scene, shape) so first-use costs stay out of the sweep.
     server = serving.RenderServer(scenes, hbm_budget_bytes=args.serve_budget_gib << 30,
-                                  max_batch=args.serve_max_batch)
+                                  max_batch=args.serve_max_batch, deadline_s=args.serve_deadline)
     for sid in scenes:
         for shape in SERVE_SHAPES:
             e, k, _ = poses[shape][0]
-            server.submit(serving.Request(0.0, 0, sid, e, k, shape))
+            server.submit(serving.Request(time.perf_counter(), 0, sid, e, k, shape))
     server.drain()
     torch.cuda.synchronize()
     sweep = []
     for rate in args.serve_rates:
-        server = serving.RenderServer(scenes, hbm_budget_bytes=args.serve_budget_gib << 30,
-                                      max_batch=args.serve_max_batch)
+        server.reset_stats()
         st = serving.poisson_load(server, make_request, rate, args.serve_seconds, seed=int(rate))
         log(f"[serve] offered {rate} req/s: {st}")
         sweep.append(st)
     return {
         "scenes": n, "spheres": spheres, "cells": cells, "resolution": args.serve_resolution,
         "shapes": [list(x) for x in SERVE_SHAPES], "hbm_budget_gib": args.serve_budget_gib,
-        "max_batch": args.serve_max_batch, "resident_gib": round(resident / 2**30, 2),
+        "max_batch": args.serve_max_batch, "deadline_s": args.serve_deadline,
+        "resident_gib": round(resident / 2**30, 2),
         "sweep": sweep,
     }
 
@@ -539,6 +583,9 @@ def main():
     device = torch.device("cuda:0")
     torch.cuda.set_device(device)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.only_serving:
+        print(json.dumps({"serving": serving_benchmark(args, device, log)}), flush=True)
+        return
     log("[bench] building scene")
     t0 = time.time()
     sc = build_scene(args, device)
@@ -573,6 +620,9 @@ def main():
         torch.cuda.synchronize()
         plain_s = (time.perf_counter() - t0) / plain_steps
         log(f"[bench] plain per-camera path: {1000 * plain_s:.2f} ms/step (loss {float(plain_loss):.6f})")
+    modes = None
+    if not args.only_batched and sc.uniform and not args.no_mode_compare:
+        modes = compare_render_modes(sc, log)
 
     # 2. Autotune the batched step (tile shape x chunk).
     if args.no_autotune or args.only_batched:
@@ -683,6 +733,7 @@ def main():
         with threadpool_limits(limits=1):  # the baseline is a single-thread port
             cpu = cpu_baseline(sc, args, live)
 
+    workload = workload_description(args, sc)
     serve = None
     if not (args.no_serving or args.only_batched):
         # Free the training step's arenas and the scene before loading the served scenes.
@@ -713,7 +764,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": workload_description(args, sc),
+            "workload": workload,
             "name": args.workload,
             "camera_group": args.camera_group or args.cameras,
             "sizes": sizes,
@@ -725,6 +776,7 @@ def main():
                      "candidates": [[list(c.tile_shape), c.chunk, round(c.ms, 3)] for c in tuned]},
         "roofline": roof,
         "kernels": table,
+        "render_modes": modes,
         "serving": serve,
         "cpu_baseline": cpu,
     }

@@ -222,37 +222,42 @@ __device__ __forceinline__ void rank_tile(const uint32_t* digits, uint32_t* rank
 }
 
 // Pass step 1: keys per (digit, tile), digit-major. Tiles past the live keys write zeros.
+// A plain LDS histogram (one sub-histogram per wave): counting needs no ranks, and every
+// load instruction reads 64 consecutive keys.
 __global__ void __launch_bounds__(kBlock) count_kernel(const int64_t* in_k, const int32_t* in_v,
                                                        int64_t* out_k, int32_t* out_v, int64_t n,
                                                        int pass, SortWorkspace w,
                                                        const int64_t* n_device) {
   if (w.info[pass] == 0) return;
   n = live_count(n, n_device);
-  const Buffers bf = route(w, pass, in_k, in_v, out_k, out_v);
-  __shared__ uint32_t s_wave_count[kWaves][kRadix];
+  __shared__ uint32_t s_hist[kWaves][kRadix];
   __shared__ Runs s_runs;
-  for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&s_wave_count[0][0])[i] = 0u;
+  for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&s_hist[0][0])[i] = 0u;
   if (threadIdx.x == 0) make_runs(varying_bits(w), &s_runs);
   __syncthreads();
   const int64_t tile = xcd_swizzle(blockIdx.x, gridDim.x);
   const int64_t tiles = gridDim.x;
   const int64_t base = tile * kTile;
-  const int wave = threadIdx.x / kWave;
-  const int lane = __lane_id();
-  uint32_t digits[kItems];
-  uint32_t ranks[kItems];
-  const int64_t wbase = base + int64_t(wave) * (kItems * kWave);
+  if (base < n) {  // uniform per block
+    const Buffers bf = route(w, pass, in_k, in_v, out_k, out_v);
+    const int wave = threadIdx.x / kWave;
+    uint64_t keys[kItems];
 #pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    const int64_t idx = wbase + i * kWave + lane;
-    digits[i] = idx < n ? digit_of(static_cast<uint64_t>(bf.src_k[idx]), s_runs, pass) : kRadix;
+    for (int i = 0; i < kItems; ++i) {
+      const int64_t idx = base + i * kBlock + threadIdx.x;
+      keys[i] = idx < n ? static_cast<uint64_t>(bf.src_k[idx]) : 0ull;
+    }
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const int64_t idx = base + i * kBlock + threadIdx.x;
+      if (idx < n) atomicAdd(&s_hist[wave][digit_of(keys[i], s_runs, pass)], 1u);
+    }
   }
-  rank_tile(digits, ranks, s_wave_count);
   __syncthreads();
   for (int d = threadIdx.x; d < kRadix; d += kBlock) {
     uint32_t total = 0;
 #pragma unroll
-    for (int q = 0; q < kWaves; ++q) total += s_wave_count[q][d];
+    for (int q = 0; q < kWaves; ++q) total += s_hist[q][d];
     w.counts[int64_t(d) * tiles + tile] = static_cast<int32_t>(total);
   }
 }

@@ -2,17 +2,22 @@
 
 * ``ResidentScene``: a scene's extracted shells (vertices, faces, shell boundaries, SDF,
   SH) kept on the device; tessellation happens once, at load.
-* ``render_batch``: forward-only fused render of B cameras of one scene and one
-  resolution, launched on the current stream with no host synchronisation: the key
-  buffers are sized from the scene's measured keys-per-pixel density (with margin) and a
-  device overflow flag reports the rare batch that needs a re-render with more room.
-* ``RenderServer``: requests (scene, pose, resolution) wait in a FIFO queue; every
-  scheduling round admits queued requests while the estimated HBM footprint of the
-  in-flight work stays within the budget, groups the admitted ones by (scene,
-  resolution) into batches of up to ``max_batch`` cameras, and launches each batch on
-  its own stream, so batches of different scenes and resolutions overlap on the device.
-  Completed batches (event query, no blocking) release their memory, which admits more
-  requests: requests that arrive while others run join at the next round.
+* ``render_batch``: forward-only fused render of B cameras of one scene, each at its own
+  resolution (a per-camera table of image and tile-grid sizes drives the key,
+  delineation and compositing kernels), launched on the current stream with no host
+  synchronisation: the key buffers are sized from the scene's calibrated keys-per-pixel
+  density (with margin) and a device overflow flag reports the rare batch that needs a
+  re-render with more room.
+* ``RenderServer``: requests (scene, pose, resolution) wait in one FIFO per scene. Each of
+  ``num_streams`` streams owns a device arena of budget / num_streams bytes, allocated
+  once; every scheduling round, while a stream is free, the server takes the oldest
+  request and gathers the next queued requests of the same scene (any resolution) into a
+  batch of up to ``max_batch`` cameras whose buffers fit the stream's arena (the exact
+  byte plan of ``render_batch``), and launches the batch on that stream, so batches of
+  different scenes overlap on the device and no batch allocates device memory. Completed
+  batches (event query, no blocking) free their stream and arena; requests that arrive
+  meanwhile join the next batches. Requests that waited longer than ``deadline_s``
+  without being admitted are shed (reported, never rendered).
 
 Each camera's pixels depend only on that camera's keys (the sort is stable and keys carry
 the camera in their top bits), so a request's image is the same alone and in any batch.
@@ -46,18 +51,31 @@ class ResidentScene:
     sharpness: torch.Tensor  # f32[1]
     background: torch.Tensor  # f32[3]
     keys_per_pixel: float = 8.0  # measured density of (face, tile) keys per image pixel
+    # (pixels, keys) of calibrated resolutions: the most keys one camera needed at that size
+    key_table: list = field(default_factory=list)
 
     @property
     def num_shells(self) -> int:
         return int(self.face_boundaries.shape[0])
 
-    def bytes_per_camera(self, h: int, w: int, margin: float) -> int:
-        v = self.vertices.shape[0]
-        f = self.faces.shape[0]
-        keys = int(self.keys_per_pixel * h * w * margin) + 4096
-        # xy + depth + colours per vertex, bounds + counts + offsets per face,
-        # keys / values twice + sort workspace, output image.
-        return v * 24 + f * 28 + keys * 37 + h * w * 16
+    def keys_for(self, h: int, w: int, margin: float) -> int:
+        """Key capacity for one camera. Keys grow with resolution but not in proportion
+        (at low resolution nearly every visible face still needs one key), so a calibrated
+        resolution at least as large bounds smaller ones; above the largest, keys are
+        scaled by pixels. Without calibration: the keys-per-pixel density."""
+        px = h * w
+        est = None
+        for cal_px, cal_keys in sorted(self.key_table):
+            if cal_px >= px:
+                est = cal_keys
+                break
+        if est is None:
+            if self.key_table:
+                cal_px, cal_keys = max(self.key_table)
+                est = cal_keys * px / cal_px
+            else:
+                est = self.keys_per_pixel * px
+        return int(est * margin) + 4096
 
 
 def tessellate_scene(voxels, grid_vertices, grid_sdf, grid_sh, level_sets, sh_degree, sharpness,
@@ -79,92 +97,189 @@ def tessellate_scene(voxels, grid_vertices, grid_sdf, grid_sh, level_sets, sh_de
 
 @dataclass
 class BatchResult:
-    image: torch.Tensor  # f32[B, 3, H, W] (background composited, clipped)
+    images: list  # per camera f32[3, H_b, W_b] (background composited, clipped)
     overflow: torch.Tensor  # int32[1]
+    keys: torch.Tensor  # int64[1]: keys the batch needed
     nbytes: int
 
 
+class ArenaFull(RuntimeError):
+    pass
+
+
+class _Buffers:
+    """Hands out a batch's buffers: from torch's allocator, or carved (256-byte aligned)
+    out of a preallocated byte arena, or (``arena`` = "count") only counts the bytes."""
+
+    def __init__(self, device, arena=None):
+        self.device = device
+        self.arena = arena
+        self.offset = 0
+
+    def take(self, shape, dtype):
+        shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        numel = 1
+        for x in shape:
+            numel *= int(x)
+        nbytes = numel * torch.empty((), dtype=dtype).element_size()
+        start = (self.offset + 255) // 256 * 256
+        self.offset = start + nbytes
+        if self.arena is None:
+            return torch.empty(shape, dtype=dtype, device=self.device)
+        if isinstance(self.arena, str):
+            return None
+        if self.offset > self.arena.numel():
+            raise ArenaFull(f"batch needs {self.offset} bytes, arena holds {self.arena.numel()}")
+        return self.arena[start : start + nbytes].view(dtype).view(shape)
+
+
+def _batch_plan(scene: ResidentScene, shapes, tile_shape, k_cap):
+    th, tw = tile_shape
+    dims = [(h, w, -(-h // th), -(-w // tw)) for h, w in shapes]
+    tile_base, pixel_base = [0], [0]
+    for h, w, r, c in dims:
+        tile_base.append(tile_base[-1] + r * c)
+        pixel_base.append(pixel_base[-1] + h * w)
+    b = len(shapes)
+    v = scene.vertices.shape[0]
+    n_blk = (scene.faces.shape[0] + 255) // 256
+    f32, i32, i64, u8 = torch.float32, torch.int32, torch.int64, torch.uint8
+    plan = [
+        ("xy", (b, v, 2), f32), ("depth", (b, v), f32), ("colors", (b, v, 3), f32),
+        ("block_counts", b * n_blk, i32), ("block_offsets", b * n_blk, i64),
+        ("keys", k_cap, i64), ("vals", k_cap, i32), ("skeys", k_cap, i64), ("svals", k_cap, i32),
+        ("ranges", (tile_base[-1], 2), i64), ("out", 4 * pixel_base[-1], f32),
+        ("campos", (b, 3), f32), ("overflow", 1, i32), ("live_x3", 1, i32),
+        ("scan_ws", _native.query("mt_scan_workspace_bytes", b * n_blk), u8),
+        ("sort_ws", _native.query("mt_sort_workspace_bytes", k_cap), u8),
+    ]
+    return dims, tile_base, pixel_base, plan
+
+
+def batch_bytes(scene: ResidentScene, image_shapes, tile_shape=(16, 16), key_margin=1.25,
+                key_capacity=None) -> int:
+    """Device bytes ``render_batch`` takes for these cameras (what an arena must hold)."""
+    shapes = [tuple(x) for x in image_shapes]
+    k_cap = key_capacity or sum(scene.keys_for(h, w, key_margin) for h, w in shapes)
+    _, _, _, plan = _batch_plan(scene, shapes, tile_shape, k_cap)
+    bufs = _Buffers(None, "count")
+    for _, shape, dtype in plan:
+        bufs.take(shape, dtype)
+    return bufs.offset
+
+
 def render_batch(scene: ResidentScene, extrinsics: torch.Tensor, intrinsics: torch.Tensor,
-                 image_shape, tile_shape=(16, 16), chunk=0, near_plane=0.2,
-                 key_margin=1.5, key_capacity=None) -> BatchResult:
-    """Forward-only fused render of B cameras of one scene, no host sync."""
+                 image_shapes, tile_shape=(16, 16), chunk=0, near_plane=0.2,
+                 key_margin=1.25, key_capacity=None, arena=None) -> BatchResult:
+    """Forward-only fused render of B cameras of one scene, no host sync.
+
+    ``image_shapes`` is one (H, W) for every camera or a per-camera list. With ``arena``
+    (a uint8 device tensor), every working buffer is carved out of it (the images
+    are fresh tensors); ``ArenaFull`` if it is too small."""
+    from .batched import camera_shapes
+
     call = _native.call
     dev = scene.vertices.device
     st = torch.cuda.current_stream(dev).cuda_stream
     b = extrinsics.shape[0]
-    h, w = image_shape
+    if not 0 < b <= 64:
+        raise ValueError(f"render_batch: {b} cameras (1..64 fit the key's camera field)")
+    shapes = camera_shapes(image_shapes, b)
+    table = len(set(shapes)) > 1
+    th, tw = tile_shape
     s = scene.num_shells
-    grid = TileGrid(tuple(tile_shape), (h, w), b, s)
-    rows, cols = grid.grid_shape
-    t = rows * cols
     v = scene.vertices.shape[0]
     f = scene.faces.shape[0]
-    k_cap = key_capacity or int(scene.keys_per_pixel * h * w * b * key_margin) + 4096
+    n_blk = (f + 255) // 256
+    k_cap = key_capacity or sum(scene.keys_for(h, w, key_margin) for h, w in shapes)
+    dims, tile_base, pixel_base, plan = _batch_plan(scene, shapes, tile_shape, k_cap)
+    rows, cols = dims[0][2], dims[0][3]
+    bufs = _Buffers(dev, arena)
+    t = {name: bufs.take(shape, dtype) for name, shape, dtype in plan}
     E = extrinsics.contiguous().float()
     K = intrinsics.contiguous().float()
-    f32 = dict(dtype=torch.float32, device=dev)
-    i32 = dict(dtype=torch.int32, device=dev)
-    xy = torch.empty((b, v, 2), **f32)
-    depth = torch.empty((b, v), **f32)
-    tmin = torch.empty((b, f, 2), **i32)
-    tmax = torch.empty((b, f, 2), **i32)
-    counts = torch.empty((b, f), **i32)
-    offsets = torch.empty(b * f, dtype=torch.int64, device=dev)
-    keys = torch.empty(k_cap, dtype=torch.int64, device=dev)
-    vals = torch.empty(k_cap, **i32)
-    skeys = torch.empty_like(keys)
-    svals = torch.empty_like(vals)
-    ranges = torch.zeros((b, 1, t, 2), dtype=torch.int64, device=dev)
-    campos = torch.empty((b, 3), **f32)
-    colors = torch.empty((b, v, 3), **f32)
-    out = torch.empty((b, 4, h, w), **f32)
-    overflow = torch.zeros(1, **i32)
-    scan_ws = _native.workspace(_native.query("mt_scan_workspace_bytes", b * f), dev)
-    sort_ws = _native.workspace(_native.query("mt_sort_workspace_bytes", k_cap), dev)
-    th, tw = tile_shape
+    # Small per-batch tables travel with the launch (host -> device copies on the stream).
+    cam_dims = torch.tensor([list(d) for d in dims], dtype=torch.int32).to(dev, non_blocking=True)
+    tbase = torch.tensor(tile_base, dtype=torch.int64).to(dev, non_blocking=True)
+    pbase = torch.tensor(pixel_base, dtype=torch.int64).to(dev, non_blocking=True)
+    t["live_x3"].fill_(3 * f)
+    t["overflow"].zero_()
+    t["ranges"].zero_()
+    xy, depth, colors, out = t["xy"], t["depth"], t["colors"], t["out"]
+    block_counts, block_offsets = t["block_counts"], t["block_offsets"]
+    keys, vals, skeys, svals = t["keys"], t["vals"], t["skeys"], t["svals"]
+    scan_ws, sort_ws = t["scan_ws"], t["sort_ws"]
+    live_x3, overflow, ranges, campos = t["live_x3"], t["overflow"], t["ranges"], t["campos"]
+    dims_p = cam_dims.data_ptr() if table else None
     call("mt_project_vertices", scene.vertices.data_ptr(), v, E.data_ptr(), K.data_ptr(), b,
          xy.data_ptr(), depth.data_ptr(), st)
-    call("mt_bound_triangles", xy.data_ptr(), depth.data_ptr(), v, scene.faces.data_ptr(), f, b,
-         th, tw, 0, 0, rows, cols, float(near_plane), tmin.data_ptr(), tmax.data_ptr(),
-         counts.data_ptr(), None, st)
-    call("mt_inclusive_scan_i32_to_i64", counts.data_ptr(), offsets.data_ptr(), b * f,
+    call("mt_key_block_counts", xy.data_ptr(), depth.data_ptr(), v, scene.faces.data_ptr(), f,
+         live_x3.data_ptr(), b, th, tw, 0, 0, rows, cols, float(near_plane), dims_p,
+         block_counts.data_ptr(), st)
+    call("mt_inclusive_scan_i32_to_i64", block_counts.data_ptr(), block_offsets.data_ptr(), b * n_blk,
          scan_ws.data_ptr(), scan_ws.numel(), st)
-    n_dev = offsets[b * f - 1 :]
-    call("mt_generate_keys", depth.data_ptr(), v, scene.faces.data_ptr(), f,
-         scene.face_boundaries.data_ptr(), s, tmin.data_ptr(), tmax.data_ptr(), offsets.data_ptr(),
-         b, cols, _LAYOUT, keys.data_ptr(), vals.data_ptr(), k_cap, overflow.data_ptr(), st)
+    n_dev = block_offsets[b * n_blk - 1 :]
+    call("mt_generate_keys_from_geometry", xy.data_ptr(), depth.data_ptr(), v, scene.faces.data_ptr(),
+         f, live_x3.data_ptr(), scene.face_boundaries.data_ptr(), s, b, th, tw, 0, 0, rows, cols,
+         float(near_plane), dims_p, block_offsets.data_ptr(), cols, _LAYOUT, keys.data_ptr(),
+         vals.data_ptr(), k_cap, overflow.data_ptr(), st)
     call("mt_sort_keys", keys.data_ptr(), vals.data_ptr(), k_cap, skeys.data_ptr(), svals.data_ptr(),
          sort_ws.data_ptr(), sort_ws.numel(), n_dev.data_ptr(), st)
-    call("mt_delineate_tiles", skeys.data_ptr(), k_cap, _LAYOUT, 0, b, 1, t, ranges.data_ptr(),
-         n_dev.data_ptr(), None, st)
+    call("mt_delineate_tiles", skeys.data_ptr(), k_cap, _LAYOUT, 0, b, 1, rows * cols,
+         ranges.data_ptr(), n_dev.data_ptr(), tbase.data_ptr() if table else None, st)
     call("mt_camera_positions", E.data_ptr(), b, campos.data_ptr(), st)
     call("mt_evaluate_sh_forward", scene.vertices.data_ptr(), v, scene.spherical_harmonics.data_ptr(),
          scene.spherical_harmonics.shape[0], scene.sh_degree, campos.data_ptr(), b,
          colors.data_ptr(), st)
+    h0, w0 = shapes[0]
     args = _native.CompositeArgs(
         xy.data_ptr(), colors.data_ptr(), scene.signed_distances.data_ptr(), v,
         scene.faces.data_ptr(), scene.sharpness.data_ptr(), skeys.data_ptr(), svals.data_ptr(),
-        ranges.data_ptr(), b, s, rows, cols, th, tw, h, w, _LAYOUT, chunk, None,
+        ranges.data_ptr(), b, s, rows, cols, th, tw, h0, w0, _LAYOUT, chunk, None,
+        dims_p, tbase.data_ptr() if table else None, pbase.data_ptr() if table else None,
+        tile_base[-1] if table else 0,
     )
     call("mt_intersect_and_composite_forward", args, out.data_ptr(), st)
-    image = (out[:, :3] + out[:, 3:4] * scene.background[:, None, None]).clip(0, 1)
-    nbytes = b * scene.bytes_per_camera(h, w, key_margin)
-    return BatchResult(image, overflow, nbytes)
+    # Background and clip (the plain render's expression); the images are the batch's only
+    # allocations outside the arena.
+    bg = scene.background[:, None, None]
+    images = []
+    for (h, w), p in zip(shapes, pixel_base):
+        o = out[4 * p : 4 * (p + h * w)].view(4, h, w)
+        images.append((o[:3] + o[3:4] * bg).clip(0, 1))
+    return BatchResult(images, overflow, n_dev, bufs.offset)
+
+
+def _max_keys(scene, extrinsics, intrinsics, image_shape, tile_shape, near_plane) -> int:
+    from . import triangle_rasterization as tr
+
+    best = 0
+    with torch.no_grad():
+        for c in range(extrinsics.shape[0]):
+            grid = TileGrid(tuple(tile_shape), tuple(image_shape), 1, scene.num_shells)
+            xy, depth = tr.project_vertices(scene.vertices, extrinsics[c : c + 1], intrinsics[c : c + 1])
+            bounds = tr.bound_triangles(xy, depth, scene.faces, grid, near_plane)
+            best = max(best, int(bounds.num_tiles_overlapped.to(torch.int64).sum()))
+    return best
 
 
 def calibrate_key_density(scene: ResidentScene, extrinsics, intrinsics, image_shape,
                           tile_shape=(16, 16), near_plane=0.2) -> float:
-    """Keys per pixel of a sample of poses (reads one size back; done at scene load)."""
-    from . import triangle_rasterization as tr
-
-    b = extrinsics.shape[0]
-    grid = TileGrid(tuple(tile_shape), tuple(image_shape), b, scene.num_shells)
-    with torch.no_grad():
-        xy, depth = tr.project_vertices(scene.vertices, extrinsics, intrinsics)
-        bounds = tr.bound_triangles(xy, depth, scene.faces, grid, near_plane)
-        k = int(bounds.num_tiles_overlapped.to(torch.int64).sum())
+    """The largest keys-per-pixel density over a sample of poses (reads sizes back; done
+    at scene load)."""
     h, w = image_shape
-    return k / (b * h * w)
+    return _max_keys(scene, extrinsics, intrinsics, image_shape, tile_shape, near_plane) / (h * w)
+
+
+def calibrate_keys(scene: ResidentScene, poses_by_shape: dict, tile_shape=(16, 16),
+                   near_plane=0.2) -> list:
+    """(pixels, most keys of one camera) per resolution over sample poses
+    ``{(H, W): (extrinsics[N, 4, 4], intrinsics[N, 3, 3])}``; sets ``scene.key_table``."""
+    table = []
+    for (h, w), (E, K) in poses_by_shape.items():
+        table.append((h * w, _max_keys(scene, E, K, (h, w), tile_shape, near_plane)))
+    scene.key_table = sorted(table)
+    return scene.key_table
 
 
 # ------------------------------------------------------------------------------------
@@ -192,63 +307,112 @@ class _InFlight:
 
 class RenderServer:
     def __init__(self, scenes: dict, hbm_budget_bytes: int, max_batch: int = 8, num_streams: int = 4,
-                 tile_shape=(16, 16), chunk=0, key_margin=1.5, keep_images=False):
+                 tile_shape=(16, 16), chunk=0, key_margin=1.25, keep_images=False,
+                 deadline_s: float | None = None):
         self.scenes = scenes
         self.budget = int(hbm_budget_bytes)
-        self.max_batch = max_batch
+        self.max_batch = min(int(max_batch), 64)
         self.tile_shape = tile_shape
         self.chunk = chunk
         self.key_margin = key_margin
         self.keep_images = keep_images
+        self.deadline_s = deadline_s
         dev = next(iter(scenes.values())).vertices.device
         self.device = dev
+        # Each stream owns an arena of budget / num_streams bytes, allocated once: a batch's
+        # buffers are carved out of its stream's arena, so serving never allocates device
+        # memory per batch (only the small output images).
+        self.arena_bytes = self.budget // num_streams // 256 * 256
         self.streams = [torch.cuda.Stream(device=dev) for _ in range(num_streams)]
-        self._stream_cycle = itertools.cycle(self.streams)
-        self.queue: deque[Request] = deque()
+        self.arenas = {st: torch.empty(self.arena_bytes, dtype=torch.uint8, device=dev)
+                       for st in self.streams}
+        self.free_streams = list(self.streams)
+        # One FIFO per scene; submission order across scenes by sequence number.
+        self.queues: dict = {sid: deque() for sid in scenes}
+        self._seq = itertools.count()
         self.inflight: list[_InFlight] = []
-        self.in_use = 0
         self.completed: list[Request] = []
+        self.shed: list[Request] = []
         self.rerenders = 0
+        self.batches = 0
+        self.cameras = 0
+        self.oversize = 0
+
+    def reset_stats(self) -> None:
+        """Clears completed / shed lists and counters (the queues must be empty)."""
+        if self.queued or self.inflight:
+            raise RuntimeError("reset_stats: requests still queued or in flight")
+        self.completed, self.shed = [], []
+        self.rerenders = self.batches = self.cameras = self.oversize = 0
+
+    @property
+    def queued(self) -> int:
+        return sum(len(q) for q in self.queues.values())
 
     def submit(self, req: Request) -> None:
-        self.queue.append(req)
+        self.queues[req.scene_id].append((next(self._seq), req))
 
-    def _footprint(self, req: Request) -> int:
-        h, w = req.image_shape
-        return self.scenes[req.scene_id].bytes_per_camera(h, w, self.key_margin)
+    def _bytes(self, sid, shapes, key_capacity=None) -> int:
+        return batch_bytes(self.scenes[sid], shapes, self.tile_shape, self.key_margin, key_capacity)
+
+    def _shed_late(self) -> None:
+        if self.deadline_s is None:
+            return
+        cutoff = time.perf_counter() - self.deadline_s
+        for q in self.queues.values():
+            while q and q[0][1].arrival < cutoff:
+                self.shed.append(q.popleft()[1])
 
     def schedule(self) -> int:
-        """Admit queued requests that fit the budget and launch them; returns batches launched."""
-        admitted = []
-        budget_left = self.budget - self.in_use
-        while self.queue:
-            need = self._footprint(self.queue[0])
-            if need > budget_left and (self.inflight or admitted):
-                break  # wait for memory to free up (FIFO: no overtaking)
-            req = self.queue.popleft()
-            admitted.append(req)
-            budget_left -= need
-        groups: dict = {}
-        for r in admitted:
-            groups.setdefault((r.scene_id, tuple(r.image_shape)), []).append(r)
+        """Launch batches while a stream is free; returns batches launched. A batch = the
+        oldest queued request plus the next queued requests of the same scene (any
+        resolution, in order), up to max_batch cameras and what the stream's arena holds.
+        A single request larger than an arena runs alone, from the general allocator."""
+        self._shed_late()
         launched = 0
-        for (sid, shape), reqs in groups.items():
-            for i in range(0, len(reqs), self.max_batch):
-                self._launch(sid, shape, reqs[i : i + self.max_batch])
+        while self.free_streams:
+            heads = [(q[0][0], sid) for sid, q in self.queues.items() if q]
+            if not heads:
+                break
+            _, sid = min(heads)
+            q = self.queues[sid]
+            shapes = [tuple(q[0][1].image_shape)]
+            if self._bytes(sid, shapes) > self.arena_bytes:
+                if self.inflight:
+                    break  # runs alone once the device is idle (never overtaken)
+                self._launch(sid, [q.popleft()[1]], use_arena=False)
                 launched += 1
+                continue
+            batch = [q.popleft()[1]]
+            while q and len(batch) < self.max_batch:
+                more = shapes + [tuple(q[0][1].image_shape)]
+                if self._bytes(sid, more) > self.arena_bytes:
+                    break
+                shapes = more
+                batch.append(q.popleft()[1])
+            self._launch(sid, batch)
+            launched += 1
         return launched
 
-    def _launch(self, sid, shape, reqs, key_capacity=None) -> None:
+    def _launch(self, sid, reqs, key_capacity=None, use_arena=True) -> None:
         scene = self.scenes[sid]
-        stream = next(self._stream_cycle)
+        stream = self.free_streams.pop()
+        arena = self.arenas[stream] if use_arena else None
+        if arena is not None and key_capacity is not None:
+            if self._bytes(sid, [tuple(r.image_shape) for r in reqs], key_capacity) > self.arena_bytes:
+                arena = None
+        if arena is None:
+            self.oversize += 1
         with torch.cuda.stream(stream):
             E = torch.stack([r.extrinsics for r in reqs]).to(self.device, non_blocking=True)
             K = torch.stack([r.intrinsics for r in reqs]).to(self.device, non_blocking=True)
-            res = render_batch(scene, E, K, shape, self.tile_shape, self.chunk,
-                               key_margin=self.key_margin, key_capacity=key_capacity)
+            res = render_batch(scene, E, K, [tuple(r.image_shape) for r in reqs], self.tile_shape,
+                               self.chunk, key_margin=self.key_margin, key_capacity=key_capacity,
+                               arena=arena)
             ev = torch.cuda.Event()
             ev.record(stream)
-        self.in_use += res.nbytes
+        self.batches += 1
+        self.cameras += len(reqs)
         self.inflight.append(_InFlight(reqs, res, ev, stream))
 
     def poll(self) -> int:
@@ -260,25 +424,22 @@ class RenderServer:
         self.inflight = still
         done = 0
         for job in finished:
-            self.in_use -= job.result.nbytes
-            if int(job.result.overflow.item()):  # the batch has completed: this read does not wait
+            self.free_streams.append(job.stream)
+            if int(job.result.overflow.item()):  # the batch has completed: these reads do not wait
                 self.rerenders += 1
-                b = len(job.requests)
-                h, w = job.requests[0].image_shape
-                sid = job.requests[0].scene_id
-                cap = int(self.scenes[sid].keys_per_pixel * h * w * b * self.key_margin * 4) + 4096
-                self._launch(sid, (h, w), job.requests, key_capacity=cap)
+                need = int(job.result.keys.item()) + 4096
+                self._launch(job.requests[0].scene_id, job.requests, key_capacity=need)
                 continue
             for i, r in enumerate(job.requests):
                 r.done = now
                 if self.keep_images:
-                    r.image = job.result.image[i]
+                    r.image = job.result.images[i]
                 self.completed.append(r)
                 done += 1
         return done
 
     def drain(self) -> None:
-        while self.queue or self.inflight:
+        while self.queued or self.inflight:
             self.schedule()
             self.poll()
 
@@ -299,20 +460,23 @@ def latency_stats(reqs) -> dict:
 def poisson_load(server: RenderServer, make_request, rate_per_s: float, duration_s: float,
                  seed: int = 0) -> dict:
     """Submit a Poisson stream of requests for ``duration_s`` seconds (wall clock) while
-    scheduling and polling continuously; returns throughput and latency percentiles."""
+    scheduling and polling continuously, then drain; returns completed throughput, the
+    latency percentiles of completed requests and the number shed past the deadline."""
     import random
 
     rng = random.Random(seed)
     t0 = time.perf_counter()
     arrivals = []
     t = 0.0
-    while t < duration_s:
+    while True:
         t += rng.expovariate(rate_per_s)
+        if t >= duration_s:
+            break
         arrivals.append(t)
     heapq.heapify(arrivals)
     submitted = []
     rid = 0
-    while arrivals or server.queue or server.inflight:
+    while arrivals or server.queued or server.inflight:
         now = time.perf_counter() - t0
         while arrivals and arrivals[0] <= now:
             a = heapq.heappop(arrivals)
@@ -325,7 +489,10 @@ def poisson_load(server: RenderServer, make_request, rate_per_s: float, duration
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     stats = latency_stats(submitted)
-    stats.update({"offered_req_per_s": rate_per_s, "completed": len(server.completed),
-                  "req_per_s": round(len(submitted) / wall, 2), "wall_s": round(wall, 3),
-                  "rerenders": server.rerenders})
+    stats.update({"offered_req_per_s": rate_per_s, "submitted": len(submitted),
+                  "completed": len(server.completed), "shed": len(server.shed),
+                  "req_per_s": round(len(server.completed) / wall, 2), "wall_s": round(wall, 3),
+                  "batches": server.batches,
+                  "mean_batch": round(server.cameras / max(server.batches, 1), 2),
+                  "rerenders": server.rerenders, "oversize": server.oversize})
     return stats

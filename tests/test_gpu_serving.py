@@ -48,16 +48,30 @@ def test_same_image_alone_and_in_mix(scenes):
     server.submit(alone)
     server.drain()
     torch.cuda.synchronize()
-    mix = [_request(1, 1, p) for p in _poses(3, 8, (64, 96))]
+    # Scene 1 requests at two other resolutions and the same one: batched together with
+    # the target (per-camera resolution table); scene 0 requests in their own batches.
+    mix = [_request(1, 1, p) for p in _poses(2, 8, (64, 96))]
     mix += [_request(10, 0, p) for p in _poses(3, 9, (48, 48))]
+    mix += [_request(20, 1, p) for p in _poses(2, 10, (40, 72))]
+    mix += [_request(30, 1, p) for p in _poses(1, 12, (97, 61))]
     mixed = _request(99, 1, target_pose)
-    mix.insert(2, mixed)
-    server2 = serving.RenderServer(scenes, hbm_budget_bytes=8 << 30, max_batch=4, keep_images=True)
+    mix.insert(4, mixed)
+    server2 = serving.RenderServer(scenes, hbm_budget_bytes=8 << 30, max_batch=8, keep_images=True)
     for r in mix:
         server2.submit(r)
     server2.drain()
     torch.cuda.synchronize()
+    # Scene 1's six cameras in one mixed-resolution batch (a batch whose keys overflowed the
+    # calibrated capacity is rendered again with the exact count, not split).
+    assert server2.batches - server2.rerenders == 2
     assert torch.equal(alone.image, mixed.image)
+    # Every request of the mix equals its own render alone.
+    for r in mix:
+        solo = serving.RenderServer(scenes, hbm_budget_bytes=8 << 30, max_batch=1, keep_images=True)
+        again = _request(r.rid, r.scene_id, (r.extrinsics, r.intrinsics, r.image_shape))
+        solo.submit(again)
+        solo.drain()
+        assert torch.equal(again.image, r.image), (r.rid, r.image_shape)
     # And both equal the plain API render of that one camera.
     sc = scenes[1]
     e, k, shape = target_pose
@@ -73,6 +87,6 @@ def test_budget_limits_inflight_work(scenes):
         server.submit(_request(i, 0, p))
     server.schedule()
     # With no memory to spare, only the head request is admitted.
-    assert len(server.inflight) == 1 and len(server.queue) == 4
+    assert len(server.inflight) == 1 and server.queued == 4
     server.drain()
     assert len(server.completed) == 5
