@@ -309,6 +309,8 @@ def algorithmic_bytes(z):
         "mt_inclusive_scan_i32_to_i64": 12 * blocks,  # over (camera, 256-face block)
         "mt_key_block_counts": g * f * 12 + b * vt * 12 + 4 * blocks,
         "mt_generate_keys_from_geometry": g * f * 12 + b * vt * 12 + k * 12 + 8 * blocks,
+        # One pass: faces once per group, projected vertices once, keys written once.
+        "mt_generate_keys_single_pass": g * f * 12 + b * vt * 12 + k * 12,
         "mt_sort_keys": 8 * k + 24 * k * z.get("sort_passes", 8),
         "mt_delineate_tiles": 8 * k + 16 * tt,
         "mt_evaluate_sh_forward": vt * (12 + nb) + b * vt * 12,

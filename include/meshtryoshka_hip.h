@@ -234,29 +234,54 @@ int mt_generate_keys(const float* depths, int64_t V, const int32_t* faces, int64
  * int32[B, ceil(F / 256)] = number of keys of every (camera, face block), computed with
  * mt_bound_triangles' arithmetic (same culling, same num_faces_x3 semantics). cam_dims
  * int32[B, 4] (height, width, grid_rows, grid_cols), or NULL: camera b's tile grid is
- * [0, grid_rows_b) x [0, grid_cols_b) instead of [row_min, row_max) x [col_min, col_max). */
+ * [0, grid_rows_b) x [0, grid_cols_b) instead of [row_min, row_max) x [col_min, col_max).
+ * cull = 1 narrows each face's tiles to those that meet its candidate pixel rectangle
+ * (the bound the composites use per tile, here over the face's whole tile box): keys of
+ * tiles where no pixel can pass the barycentric test are dropped, which changes no image
+ * or gradient of the fused composites; cull = 0 keeps every tile of the bounding box. */
 int mt_key_block_counts(const float* xy, const float* depths, int64_t V, const int32_t* faces,
                         int64_t F, const int32_t* num_faces_x3, int32_t B, int32_t tile_h,
                         int32_t tile_w, int32_t row_min, int32_t col_min, int32_t row_max,
                         int32_t col_max, float near_plane, const int32_t* cam_dims,
-                        int32_t* block_counts, mt_stream_t stream);
+                        int32_t cull, int32_t* block_counts, mt_stream_t stream);
 
 /* Keys of every (camera, face, covered tile) like mt_generate_keys, with each face's
  * tile rectangle recomputed from the projected geometry (no stored bounds) and each
  * block's first key at block_offsets[q - 1] (0 for q = 0), block_offsets = the inclusive
- * int64 scan of mt_key_block_counts' output (mt_inclusive_scan_i32_to_i64). The keys and
- * values are identical to mt_bound_triangles + inclusive scan + mt_generate_keys. cam_dims
- * as in mt_key_block_counts (tile ids then use each camera's grid_cols). */
+ * int64 scan of mt_key_block_counts' output (mt_inclusive_scan_i32_to_i64). With cull = 0
+ * the keys and values are identical to mt_bound_triangles + inclusive scan +
+ * mt_generate_keys; cull (the same value as for mt_key_block_counts) and cam_dims as in
+ * mt_key_block_counts (tile ids then use each camera's grid_cols). */
 int mt_generate_keys_from_geometry(const float* xy, const float* depths, int64_t V,
                                    const int32_t* faces, int64_t F, const int32_t* num_faces_x3,
                                    const int32_t* shell_face_boundaries, int32_t S, int32_t B,
                                    int32_t tile_h, int32_t tile_w, int32_t row_min,
                                    int32_t col_min, int32_t row_max, int32_t col_max,
-                                   float near_plane, const int32_t* cam_dims,
+                                   float near_plane, const int32_t* cam_dims, int32_t cull,
                                    const int64_t* block_offsets, int32_t grid_cols,
                                    mt_key_layout layout, int64_t* keys,
                                    int32_t* values, int64_t key_capacity, int32_t* overflow,
                                    mt_stream_t stream);
+
+/* The three calls above in one kernel (batched training step and serving; no single
+ * counterpart in the reference: fuses bound_triangles.slang:144-192, the tile-count cumsum
+ * of generate_keys.py:40-47 and generate_keys.slang:4-68). Blocks of 256 faces of one
+ * camera take their index from a ticket, count their keys, find their first output
+ * position by decoupled look-back over the earlier blocks, and write their keys, so each
+ * face's tile rectangle is computed once. Keys and values equal mt_key_block_counts +
+ * scan + mt_generate_keys_from_geometry with the same arguments; *n_keys (device) = the
+ * total key count (also when it exceeds key_capacity, which sets *overflow). Workspace:
+ * mt_generate_keys_single_pass_workspace_bytes(F, B). */
+size_t mt_generate_keys_single_pass_workspace_bytes(int64_t F, int32_t B);
+int mt_generate_keys_single_pass(const float* xy, const float* depths, int64_t V,
+                                 const int32_t* faces, int64_t F, const int32_t* num_faces_x3,
+                                 const int32_t* shell_face_boundaries, int32_t S, int32_t B,
+                                 int32_t tile_h, int32_t tile_w, int32_t row_min, int32_t col_min,
+                                 int32_t row_max, int32_t col_max, float near_plane,
+                                 const int32_t* cam_dims, int32_t cull, int32_t grid_cols,
+                                 mt_key_layout layout, int64_t* keys, int32_t* values,
+                                 int64_t key_capacity, int64_t* n_keys, int32_t* overflow,
+                                 void* workspace, size_t workspace_bytes, mt_stream_t stream);
 
 /* Workspace for mt_sort_keys on n pairs. */
 size_t mt_sort_workspace_bytes(int64_t n);

@@ -14,8 +14,12 @@ Unlike the plain path, the per-triangle-vertex SH array ([NSH, V, 3], 192 bytes 
 at degree 3) is never stored: vertices and their colours come out of one kernel that
 interpolates the grid SH in registers, and the SH adjoint is recomputed per triangle
 vertex and distributed to the grid. Keys are generated straight from the projected
-geometry (no per-face tile bounds, counts or offsets are stored). Images are
-bit-identical to the plain path's; gradients equal it up to float summation order.
+geometry (no per-face tile bounds, counts or offsets are stored), and only for tiles that
+meet the face's candidate pixel rectangle (the bound the composites apply per tile): a
+face that covers no pixel centre of a tile cannot change that tile, and most sub-pixel
+faces cover none, so the sort and the composites see a fraction of the reference's keys.
+Images are bit-identical to the plain path's; gradients equal it up to float summation
+order.
 
 Cameras may have different resolutions (a per-camera table of image and tile-grid sizes
 drives the key, delineation, compositing and loss kernels) and may be rendered in groups
@@ -73,7 +77,7 @@ class BatchedTrainingStep:
                  intrinsics: torch.Tensor, image_shape, num_sh: int, sh_degree: int,
                  sharpness: float, background, capacities: Capacities, tile_shape=(16, 16),
                  chunk: int = 0, near_plane: float = 0.2, reference_quirk: bool = False,
-                 camera_group: int | None = None):
+                 camera_group: int | None = None, cull_keys: bool = True):
         dev = voxels.lower_corners.device
         self.device = dev
         self.voxels = voxels
@@ -91,6 +95,7 @@ class BatchedTrainingStep:
         self.chunk = chunk
         self.near_plane = float(near_plane)
         self.quirk = int(reference_quirk)
+        self.cull = int(cull_keys)
         self.cap = capacities
         self.layout = to_layout(create_key_defines(("camera", 6), ("tile", 20), ("depth", 32),
                                                    ("shell", 6)))
@@ -139,10 +144,7 @@ class BatchedTrainingStep:
         self.voxel_indices = torch.zeros(F, **i32)
         self.xy = torch.zeros((G, V, 2), **f32)
         self.depth = torch.zeros((G, V), **f32)
-        # Keys per (camera, 256-face block) and their inclusive scan.
-        self.n_blk = (F + 255) // 256
-        self.block_counts = torch.zeros(G * self.n_blk, **i32)
-        self.block_offsets = torch.zeros(G * self.n_blk, **i64)
+        self.n_keys = torch.zeros(1, **i64)  # key count of the current camera group
         self.keys = torch.zeros(K, **i64)
         self.values = torch.zeros(K, **i32)
         self.sorted_keys = torch.zeros(K, **i64)
@@ -162,8 +164,10 @@ class BatchedTrainingStep:
         self.group_keys = torch.zeros(len(self.groups), **i64)  # key count of each group
         self.sharpness = torch.tensor([float(sharpness)], **f32)
         self.background = torch.as_tensor(background, dtype=torch.float32, device=dev).contiguous()
-        scan_n = max(S * self.n_col + 1, S * self.n_vox + 1, G * self.n_blk)
+        scan_n = max(S * self.n_col + 1, S * self.n_vox + 1)
         self.scan_ws = _native.workspace(_native.query("mt_scan_workspace_bytes", scan_n), dev)
+        self.keys_ws = _native.workspace(
+            _native.query("mt_generate_keys_single_pass_workspace_bytes", F, G), dev)
         self.sort_ws = _native.workspace(_native.query("mt_sort_workspace_bytes", K), dev)
         self.vbwd_ws = _native.workspace(
             _native.query("mt_create_triangle_vertices_colors_backward_workspace_bytes",
@@ -265,19 +269,16 @@ class BatchedTrainingStep:
             # --- rasterization of the group's cameras
             call("mt_project_vertices", self.tv.data_ptr(), V, self.extrinsics[c0:].data_ptr(),
                  self.intrinsics[c0:].data_ptr(), n, self.xy.data_ptr(), self.depth.data_ptr(), st)
-            call("mt_key_block_counts", self.xy.data_ptr(), self.depth.data_ptr(), V,
-                 self.faces.data_ptr(), F, self.num_faces_x3.data_ptr(), n, th, tw, 0, 0,
-                 self.rows, self.cols, self.near_plane, cam_dims, self.block_counts.data_ptr(), st)
-            call("mt_inclusive_scan_i32_to_i64", self.block_counts.data_ptr(),
-                 self.block_offsets.data_ptr(), n * self.n_blk, self.scan_ws.data_ptr(),
-                 self.scan_ws.numel(), st)
-            n_keys = self.block_offsets[n * self.n_blk - 1 :]
-            self.group_keys[gi : gi + 1].copy_(n_keys[:1])
-            call("mt_generate_keys_from_geometry", self.xy.data_ptr(), self.depth.data_ptr(), V,
+            # Keys of the group's (camera, face, tile) triples in one pass (counts, their scan
+            # by look-back, and the keys); the total lands in n_keys on the device.
+            n_keys = self.n_keys
+            call("mt_generate_keys_single_pass", self.xy.data_ptr(), self.depth.data_ptr(), V,
                  self.faces.data_ptr(), F, self.num_faces_x3.data_ptr(), face_boundaries.data_ptr(),
-                 S, n, th, tw, 0, 0, self.rows, self.cols, self.near_plane, cam_dims,
-                 self.block_offsets.data_ptr(), self.cols, self.layout, self.keys.data_ptr(),
-                 self.values.data_ptr(), K, self.overflow.data_ptr(), st)
+                 S, n, th, tw, 0, 0, self.rows, self.cols, self.near_plane, cam_dims, self.cull,
+                 self.cols, self.layout, self.keys.data_ptr(), self.values.data_ptr(), K,
+                 n_keys.data_ptr(), self.overflow.data_ptr(), self.keys_ws.data_ptr(),
+                 self.keys_ws.numel(), st)
+            self.group_keys[gi : gi + 1].copy_(n_keys)
             call("mt_sort_keys", self.keys.data_ptr(), self.values.data_ptr(), K,
                  self.sorted_keys.data_ptr(), self.sorted_values.data_ptr(), self.sort_ws.data_ptr(),
                  self.sort_ws.numel(), n_keys.data_ptr(), st)
@@ -342,14 +343,34 @@ class BatchedTrainingStep:
         }
 
 
+def count_keys(vertices: torch.Tensor, faces: torch.Tensor, extrinsics: torch.Tensor,
+               intrinsics: torch.Tensor, image_shape, tile_shape=(16, 16), near_plane=0.2,
+               cull: bool = True) -> int:
+    """Keys one camera needs (reads the count back): ``mt_key_block_counts`` on its
+    projected vertices, with the batched step's culling."""
+    from . import triangle_rasterization as tr
+
+    th, tw = tile_shape
+    h, w = image_shape
+    f = faces.shape[0]
+    if f == 0:
+        return 0
+    xy, depth = tr.project_vertices(vertices, extrinsics, intrinsics)
+    n_blk = (f + 255) // 256
+    counts = torch.empty(n_blk, dtype=torch.int32, device=vertices.device)
+    st = torch.cuda.current_stream(vertices.device).cuda_stream
+    _native.call("mt_key_block_counts", xy.contiguous().data_ptr(), depth.contiguous().data_ptr(),
+                 vertices.shape[0], faces.contiguous().data_ptr(), f, None, 1, th, tw, 0, 0,
+                 -(-h // th), -(-w // tw), float(near_plane), None, int(cull), counts.data_ptr(), st)
+    return int(counts.to(torch.int64).sum())
+
+
 def measure_capacities(voxels: Voxels, level_sets, extrinsics, intrinsics, image_shape, grid_vertices,
                        grid_sdf, grid_sh, tile_shape=(16, 16), near_plane=0.2,
-                       camera_group: int | None = None) -> Capacities:
+                       camera_group: int | None = None, cull_keys: bool = True) -> Capacities:
     """Exact vertex and face counts of one eager forward and the largest key count of a
     camera group (reads the sizes back)."""
     from . import triangle_extraction as te
-    from . import triangle_rasterization as tr
-    from .triangle_rasterization.common import TileGrid
 
     b = extrinsics.shape[0]
     shapes = camera_shapes(image_shape, b)
@@ -357,11 +378,8 @@ def measure_capacities(voxels: Voxels, level_sets, extrinsics, intrinsics, image
         tri = te.tessellate_voxels(grid_sdf, grid_vertices, grid_sh, voxels.neighbors,
                                    voxels.lower_corners, voxels.upper_corners, voxels.indices,
                                    level_sets)
-        per_camera = []
-        for c in range(b):
-            grid = TileGrid(tuple(tile_shape), shapes[c], 1, level_sets.shape[0])
-            xy, depth = tr.project_vertices(tri.vertices, extrinsics[c : c + 1], intrinsics[c : c + 1])
-            bounds = tr.bound_triangles(xy, depth, tri.faces, grid, near_plane)
-            per_camera.append(int(bounds.num_tiles_overlapped.to(torch.int64).sum()))
+        per_camera = [count_keys(tri.vertices, tri.faces, extrinsics[c : c + 1],
+                                 intrinsics[c : c + 1], shapes[c], tile_shape, near_plane,
+                                 cull_keys) for c in range(b)]
     k = max(sum(per_camera[c0:c1]) for c0, c1 in camera_groups(b, camera_group))
     return Capacities(int(tri.vertices.shape[0]), int(tri.faces.shape[0]), k)

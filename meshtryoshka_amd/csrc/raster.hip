@@ -3,6 +3,8 @@
 // library is compiled without FP contraction so that keys (which embed depth bits) and
 // tile bounds are reproducible bit for bit.
 #include "common.h"
+#include "geometry.h"
+#include "lookback.h"
 #include "sh.h"
 
 namespace mt {
@@ -75,6 +77,31 @@ __device__ __forceinline__ TileRect face_tile_rect(const float* xy, const float*
     r.x1 = clampi(f2i_rz(ceilf(xhi / float(tw))), cmin, cmax);
   }
   return r;
+}
+
+// Narrows a face's tile rectangle to the tiles that meet its candidate pixel rectangle
+// over the whole tile-aligned box (geometry.h, the bound the composites use per tile): no
+// pixel of the other tiles can pass the barycentric test for this face, so their keys
+// cannot change any pixel or gradient. Most sub-pixel faces cover no pixel centre at all
+// and get no key.
+__device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, int64_t o, int64_t ia,
+                                               int64_t ib, int64_t ic, int th, int tw) {
+  if (r.y1 <= r.y0 || r.x1 <= r.x0) return r;
+  const float2* pxy = reinterpret_cast<const float2*>(xy);
+  const float2 a = pxy[o + ia], bb = pxy[o + ib], c = pxy[o + ic];
+  const TriTerms t = tri_terms(a.x, a.y, bb.x, bb.y, c.x, c.y);
+  const Rect q = candidate_rect(t, r.x0 * tw, r.y0 * th, (r.x1 - r.x0) * tw, (r.y1 - r.y0) * th);
+  TileRect n;
+  if (q.w <= 0 || q.h <= 0) {
+    n.y0 = n.y1 = r.y0;
+    n.x0 = n.x1 = r.x0;
+    return n;
+  }
+  n.y0 = r.y0 + q.y / th;
+  n.y1 = r.y0 + (q.y + q.h - 1) / th + 1;
+  n.x0 = r.x0 + q.x / tw;
+  n.x1 = r.x0 + (q.x + q.w - 1) / tw + 1;
+  return n;
 }
 
 // Face whose keys are generated: live (below the device-side face count) and with all
@@ -216,7 +243,7 @@ __global__ void __launch_bounds__(kBlock) keys_kernel(const float* depths, int64
 __global__ void __launch_bounds__(kBlock) key_block_counts_kernel(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F, int B,
     int th, int tw, int rmin, int cmin, int rmax, int cmax, float near_plane,
-    const int32_t* cam_dims, const int32_t* live_x3, int32_t* block_counts) {
+    const int32_t* cam_dims, int cull, const int32_t* live_x3, int32_t* block_counts) {
   __shared__ int32_t s_wave[kBlock / kWave];
   const int64_t f = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   const int64_t nblk = gridDim.x;
@@ -227,8 +254,9 @@ __global__ void __launch_bounds__(kBlock) key_block_counts_kernel(
     if (keyed) {
       const int r0 = cam_dims ? 0 : rmin, c0 = cam_dims ? 0 : cmin;
       const int r1 = cam_dims ? cam_dims[4 * b + 2] : rmax, c1 = cam_dims ? cam_dims[4 * b + 3] : cmax;
-      const TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic, th, tw, r0, c0,
-                                        r1, c1, near_plane);
+      TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic, th, tw, r0, c0, r1, c1,
+                                  near_plane);
+      if (cull) r = cull_tiles(r, xy, int64_t(b) * V, ia, ib, ic, th, tw);
       count = (r.y1 - r.y0) * (r.x1 - r.x0);
     }
     const int32_t w = wave_sum(count);
@@ -247,7 +275,8 @@ __global__ void __launch_bounds__(kBlock) key_block_counts_kernel(
 __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
     const int32_t* live_x3, const int32_t* boundaries, int S, int th, int tw, int rmin, int cmin,
-    int rmax, int cmax, float near_plane, const int32_t* cam_dims, const int64_t* block_incl,
+    int rmax, int cmax, float near_plane, const int32_t* cam_dims, int cull,
+    const int64_t* block_incl,
     int grid_cols, KeyFields kf, int64_t* keys, int32_t* values, int64_t kcap, int32_t* overflow) {
   __shared__ int32_t s_prefix[kBlock + 1];
   __shared__ int32_t s_r0[kBlock], s_c0[kBlock], s_w[kBlock];
@@ -267,8 +296,9 @@ __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
   int64_t ia = 0, ib = 0, ic = 0;
   if (f < F && keyed_face(faces, f, V, live_x3, ia, ib, ic)) {
     const int64_t o = int64_t(b) * V;
-    const TileRect r = face_tile_rect(xy, depths, o, ia, ib, ic, th, tw, rmin, cmin, rmax, cmax,
-                                      near_plane);
+    TileRect r = face_tile_rect(xy, depths, o, ia, ib, ic, th, tw, rmin, cmin, rmax, cmax,
+                                near_plane);
+    if (cull) r = cull_tiles(r, xy, o, ia, ib, ic, th, tw);
     count = (r.y1 - r.y0) * (r.x1 - r.x0);
     s_r0[threadIdx.x] = r.y0;
     s_c0[threadIdx.x] = r.x0;
@@ -318,6 +348,125 @@ __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
     keys[base + j] = static_cast<int64_t>(key);
     values[base + j] = static_cast<int32_t>(int64_t(blockIdx.x) * kBlock + lo);
   }
+}
+
+// Single pass: the counts, their scan and the keys in one kernel. Blocks take their
+// (camera, block of kPassFaces faces) index from a ticket, count their faces' keys, find
+// their first output position by decoupled look-back over the earlier blocks' published
+// counts, and write their keys; each face's (culled) tile rectangle is computed once. A
+// block covers kPassFaces = 4 x 256 faces so that the look-back is paid once per 1024
+// faces. The last block also stores the total key count.
+constexpr int kPassPer = 4;
+constexpr int kPassFaces = kPassPer * kBlock;
+
+__global__ void __launch_bounds__(kBlock) keys_single_pass_kernel(
+    const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
+    const int32_t* live_x3, const int32_t* boundaries, int S, int B, int th, int tw, int rmin,
+    int cmin, int rmax, int cmax, float near_plane, const int32_t* cam_dims, int cull,
+    int grid_cols, KeyFields kf, int64_t* keys, int32_t* values, int64_t kcap, int64_t* n_keys,
+    int32_t* overflow, ScanState st) {
+  __shared__ int32_t s_prefix[kPassFaces];
+  __shared__ int32_t s_r0[kPassFaces], s_c0[kPassFaces], s_w[kPassFaces];
+  __shared__ uint64_t s_key_hi[kPassFaces];
+  __shared__ int32_t s_wave[kBlock / kWave];
+  __shared__ int64_t s_q, s_block_start;
+  if (threadIdx.x == 0) s_q = atomicAdd(st.ticket, 1u);
+  __syncthreads();
+  const int64_t q = s_q;
+  const int64_t nblk = (F + kPassFaces - 1) / kPassFaces;
+  const int b = static_cast<int>(q / nblk);
+  const int64_t fb = q % nblk;
+  if (cam_dims) {  // this camera's own tile grid
+    rmin = 0;
+    cmin = 0;
+    rmax = cam_dims[4 * b + 2];
+    cmax = cam_dims[4 * b + 3];
+    grid_cols = cmax;
+  }
+  const int64_t o = int64_t(b) * V;
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / kWave;
+  int32_t carry = 0;
+  for (int i = 0; i < kPassPer; ++i) {
+    const int l = i * kBlock + threadIdx.x;
+    const int64_t f = fb * kPassFaces + l;
+    int32_t count = 0;
+    int64_t ia = 0, ib = 0, ic = 0;
+    if (f < F && keyed_face(faces, f, V, live_x3, ia, ib, ic)) {
+      TileRect r = face_tile_rect(xy, depths, o, ia, ib, ic, th, tw, rmin, cmin, rmax, cmax,
+                                  near_plane);
+      if (cull) r = cull_tiles(r, xy, o, ia, ib, ic, th, tw);
+      count = (r.y1 - r.y0) * (r.x1 - r.x0);
+      s_r0[l] = r.y0;
+      s_c0[l] = r.x0;
+      s_w[l] = r.x1 - r.x0;
+      if (count > 0) {
+        uint32_t shell = 0;
+        while (shell < uint32_t(S) && !(f < boundaries[shell])) ++shell;
+        const float mean_depth = ((depths[o + ia] + depths[o + ib]) + depths[o + ic]) / 3.f;
+        s_key_hi[l] = pack_key(kf, uint32_t(b), shell, 0u, mean_depth);
+      }
+    }
+    const int32_t inc = wave_inclusive_sum(count);
+    if (lane == kWave - 1) s_wave[wave] = inc;
+    __syncthreads();
+    int32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) {
+      if (w < wave) before += s_wave[w];
+      total += s_wave[w];
+    }
+    s_prefix[l] = carry + before + inc - count;
+    carry += total;
+    __syncthreads();  // s_wave is reused by the next round
+  }
+  const int32_t total = carry;
+  if (threadIdx.x < kWave) {
+    int64_t prefix = 0;
+    if (q == 0) {
+      if (threadIdx.x == 0) store_word(&st.words[0], kPrefix, total);
+    } else {
+      if (threadIdx.x == 0) store_word(&st.words[q], kAggregate, total);
+      prefix = look_back(st, q);
+      if (threadIdx.x == 0) store_word(&st.words[q], kPrefix, prefix + int64_t(total));
+    }
+    if (threadIdx.x == 0) {
+      s_block_start = prefix;
+      if (q == int64_t(B) * nblk - 1 && n_keys) *n_keys = prefix + int64_t(total);
+    }
+  }
+  __syncthreads();
+  const int64_t base = s_block_start;
+  const uint64_t tile_mask = field_mask(kf.tile_bits);
+  for (int32_t j = threadIdx.x; j < total; j += kBlock) {
+    int lo = 0, hi = kPassFaces - 1;  // last face whose prefix is <= j (it has keys)
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_prefix[mid] <= j) lo = mid; else hi = mid - 1;
+    }
+    const int32_t local = j - s_prefix[lo];
+    const int32_t w = s_w[lo];
+    const uint32_t row = uint32_t(s_r0[lo] + local / w);
+    const uint32_t col = uint32_t(s_c0[lo] + local % w);
+    const uint32_t tile = row * uint32_t(grid_cols) + col;
+    const uint64_t key = s_key_hi[lo] | ((uint64_t(tile) & tile_mask) << kf.tile_offset);
+    if (base + j >= kcap) {
+      if (overflow) atomicOr(overflow, 1);
+      continue;
+    }
+    keys[base + j] = static_cast<int64_t>(key);
+    values[base + j] = static_cast<int32_t>(fb * kPassFaces + lo);
+  }
+}
+
+__global__ void fill_i64_kernel(int64_t* p, int64_t n, int64_t value) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
+size_t single_pass_state_bytes(int64_t F, int B) {
+  const int64_t blocks = int64_t(B) * ((F + kPassFaces - 1) / kPassFaces);
+  return 256 + (static_cast<size_t>(blocks) * 8 + 255) / 256 * 256;
 }
 
 // ------------------------------------------------------------------------------------
@@ -494,14 +643,14 @@ extern "C" int mt_key_block_counts(const float* xy, const float* depths, int64_t
                                    const int32_t* faces, int64_t F, const int32_t* num_faces_x3,
                                    int32_t B, int32_t tile_h, int32_t tile_w, int32_t row_min,
                                    int32_t col_min, int32_t row_max, int32_t col_max,
-                                   float near_plane, const int32_t* cam_dims,
+                                   float near_plane, const int32_t* cam_dims, int32_t cull,
                                    int32_t* block_counts, mt_stream_t stream) {
   MT_REQUIRE(F >= 0 && B >= 0 && tile_h > 0 && tile_w > 0, "mt_key_block_counts: bad sizes");
   if (F == 0 || B == 0) return MT_OK;
   mt::key_block_counts_kernel<<<mt::blocks_for(F, mt::kBlock), mt::kBlock, 0,
                                 static_cast<hipStream_t>(stream)>>>(
       xy, depths, V, faces, F, B, tile_h, tile_w, row_min, col_min, row_max, col_max, near_plane,
-      cam_dims, num_faces_x3, block_counts);
+      cam_dims, cull, num_faces_x3, block_counts);
   return mt::check_launch("mt_key_block_counts");
 }
 
@@ -509,18 +658,57 @@ extern "C" int mt_generate_keys_from_geometry(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
     const int32_t* num_faces_x3, const int32_t* shell_face_boundaries, int32_t S, int32_t B,
     int32_t tile_h, int32_t tile_w, int32_t row_min, int32_t col_min, int32_t row_max,
-    int32_t col_max, float near_plane, const int32_t* cam_dims, const int64_t* block_offsets,
-    int32_t grid_cols, mt_key_layout layout, int64_t* keys, int32_t* values, int64_t key_capacity,
-    int32_t* overflow, mt_stream_t stream) {
+    int32_t col_max, float near_plane, const int32_t* cam_dims, int32_t cull,
+    const int64_t* block_offsets, int32_t grid_cols, mt_key_layout layout, int64_t* keys,
+    int32_t* values, int64_t key_capacity, int32_t* overflow, mt_stream_t stream) {
   MT_REQUIRE(F >= 0 && B >= 0 && B <= 65535 && key_capacity >= 0 && tile_h > 0 && tile_w > 0,
              "mt_generate_keys_from_geometry: bad sizes");
   if (F == 0 || B == 0) return MT_OK;
   dim3 grid(mt::blocks_for(F, mt::kBlock), static_cast<unsigned>(B));
   mt::keys_from_geometry_kernel<<<grid, mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
       xy, depths, V, faces, F, num_faces_x3, shell_face_boundaries, S, tile_h, tile_w, row_min,
-      col_min, row_max, col_max, near_plane, cam_dims, block_offsets, grid_cols,
+      col_min, row_max, col_max, near_plane, cam_dims, cull, block_offsets, grid_cols,
       mt::to_fields(layout), keys, values, key_capacity, overflow);
   return mt::check_launch("mt_generate_keys_from_geometry");
+}
+
+extern "C" size_t mt_generate_keys_single_pass_workspace_bytes(int64_t F, int32_t B) {
+  return mt::single_pass_state_bytes(F < 0 ? 0 : F, B < 0 ? 0 : B);
+}
+
+extern "C" int mt_generate_keys_single_pass(
+    const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
+    const int32_t* num_faces_x3, const int32_t* shell_face_boundaries, int32_t S, int32_t B,
+    int32_t tile_h, int32_t tile_w, int32_t row_min, int32_t col_min, int32_t row_max,
+    int32_t col_max, float near_plane, const int32_t* cam_dims, int32_t cull, int32_t grid_cols,
+    mt_key_layout layout, int64_t* keys, int32_t* values, int64_t key_capacity, int64_t* n_keys,
+    int32_t* overflow, void* workspace, size_t workspace_bytes, mt_stream_t stream) {
+  MT_REQUIRE(F >= 0 && B >= 0 && B <= 65535 && key_capacity >= 0 && tile_h > 0 && tile_w > 0,
+             "mt_generate_keys_single_pass: bad sizes");
+  MT_REQUIRE(n_keys != nullptr, "mt_generate_keys_single_pass: null key count");
+  const size_t need = mt::single_pass_state_bytes(F, B);
+  if (workspace == nullptr || workspace_bytes < need) {
+    mt::set_error("mt_generate_keys_single_pass: workspace of %zu bytes is smaller than the %zu required",
+                  workspace_bytes, need);
+    return MT_ERR_WORKSPACE_TOO_SMALL;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t blocks = int64_t(B) * ((F + mt::kPassFaces - 1) / mt::kPassFaces);
+  mt::ScanState state;
+  state.ticket = static_cast<uint32_t*>(workspace);
+  state.words = reinterpret_cast<uint64_t*>(static_cast<char*>(workspace) + 256);
+  mt::reset_scan_state<<<mt::blocks_for(blocks > 0 ? blocks : 1, 256), 256, 0, st>>>(
+      state.ticket, state.words, blocks);
+  if (blocks == 0) {
+    mt::fill_i64_kernel<<<1, 64, 0, st>>>(n_keys, 1, 0);
+    return mt::check_launch("mt_generate_keys_single_pass");
+  }
+  MT_REQUIRE(blocks < (int64_t(1) << 31), "mt_generate_keys_single_pass: too many face blocks");
+  mt::keys_single_pass_kernel<<<static_cast<unsigned>(blocks), mt::kBlock, 0, st>>>(
+      xy, depths, V, faces, F, num_faces_x3, shell_face_boundaries, S, B, tile_h, tile_w, row_min,
+      col_min, row_max, col_max, near_plane, cam_dims, cull, grid_cols, mt::to_fields(layout),
+      keys, values, key_capacity, n_keys, overflow, state);
+  return mt::check_launch("mt_generate_keys_single_pass");
 }
 
 extern "C" int mt_delineate_tiles(const int64_t* sorted_keys, int64_t n, mt_key_layout layout,

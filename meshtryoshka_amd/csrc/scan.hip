@@ -6,6 +6,7 @@
 // inclusive prefixes up to 64 tiles at a time. Every element is read once and written once:
 // 8 bytes per element for the int32 scan, 12 for int32 -> int64.
 #include "common.h"
+#include "lookback.h"
 
 namespace mt {
 namespace {
@@ -13,23 +14,6 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;
-
-// Look-back word per tile: [63:62] state, [61:0] value as 62-bit two's complement. The
-// word carries its own value, so publishing and polling need only relaxed agent-scope
-// atomics: no release/acquire fences (which on gfx950 write back / invalidate caches) sit
-// on the serial chain between neighbouring tiles.
-constexpr uint64_t kNone = 0;
-constexpr uint64_t kAggregate = 1ull << 62;
-constexpr uint64_t kPrefix = 2ull << 62;
-constexpr uint64_t kStateMask = 3ull << 62;
-constexpr uint64_t kValueMask = (1ull << 62) - 1;
-// Predecessor words loaded at once per look-back round (independent loads in flight).
-constexpr int kWindow = 64;
-
-struct ScanState {
-  uint32_t* ticket;
-  uint64_t* words;
-};
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -46,22 +30,6 @@ ScanState scan_state(void* ws) {
   return s;
 }
 
-__global__ void reset_scan_state(uint32_t* ticket, uint64_t* words, int64_t tiles) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i == 0) *ticket = 0;
-  if (i < tiles) words[i] = kNone;
-}
-
-__device__ __forceinline__ uint64_t load_word(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void store_word(uint64_t* p, uint64_t state, int64_t value) {
-  __hip_atomic_store(p, state | (static_cast<uint64_t>(value) & kValueMask), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ int64_t word_value(uint64_t w) {
-  return static_cast<int64_t>(w << 2) >> 2;  // sign-extend the 62-bit field
-}
 
 // Block-wide exclusive scan of one value per thread; returns the exclusive prefix and
 // writes the block total into *total (all threads).
@@ -81,32 +49,6 @@ __device__ __forceinline__ T block_exclusive(T x, T* s_wave, T* total) {
   }
   *total = sum;
   return before + inc - x;
-}
-
-// Look-back by the first wave: returns the sum of all tiles before `tile`. Lane l polls
-// tile (base - l); the wave consumes the published words up to the nearest inclusive
-// prefix, or up to the first unpublished one, and polls again from there.
-__device__ int64_t look_back(const ScanState& st, int64_t tile) {
-  const int lane = __lane_id();
-  int64_t running = 0;
-  int64_t base = tile - 1;
-  while (base >= 0) {
-    const int64_t t = base - lane;
-    const uint64_t w = t >= 0 ? load_word(&st.words[t]) : kPrefix;  // before tile 0: empty prefix
-    const uint64_t state = w & kStateMask;
-    const uint64_t pending = __ballot(state == kNone);
-    const uint64_t has_prefix = __ballot(state == kPrefix);
-    // First lane (nearest predecessor) that is unpublished or an inclusive prefix.
-    const uint64_t stops = pending | has_prefix;
-    const int first = stops ? __ffsll(static_cast<long long>(stops)) - 1 : kWindow;
-    const bool take_first = first < kWindow && ((has_prefix >> first) & 1ull);
-    const int take = first + (take_first ? 1 : 0);  // lanes [0, take) are consumed
-    int64_t contrib = (lane < take && t >= 0) ? word_value(w) : 0;
-    running += wave_sum(contrib);
-    if (take_first) break;
-    base -= take;
-  }
-  return running;
 }
 
 // LDS index with one pad slot per 16 elements: thread t's 16 consecutive items start at

@@ -34,7 +34,6 @@ from dataclasses import dataclass, field
 import torch
 
 from . import _native
-from .triangle_rasterization.common import TileGrid
 from .triangle_rasterization.keys import create_key_defines, to_layout
 
 _LAYOUT = to_layout(create_key_defines(("camera", 6), ("tile", 20), ("depth", 32), ("shell", 6)))
@@ -142,15 +141,15 @@ def _batch_plan(scene: ResidentScene, shapes, tile_shape, k_cap):
         pixel_base.append(pixel_base[-1] + h * w)
     b = len(shapes)
     v = scene.vertices.shape[0]
-    n_blk = (scene.faces.shape[0] + 255) // 256
     f32, i32, i64, u8 = torch.float32, torch.int32, torch.int64, torch.uint8
     plan = [
         ("xy", (b, v, 2), f32), ("depth", (b, v), f32), ("colors", (b, v, 3), f32),
-        ("block_counts", b * n_blk, i32), ("block_offsets", b * n_blk, i64),
+        ("n_keys", 1, i64),
         ("keys", k_cap, i64), ("vals", k_cap, i32), ("skeys", k_cap, i64), ("svals", k_cap, i32),
         ("ranges", (tile_base[-1], 2), i64), ("out", 4 * pixel_base[-1], f32),
         ("campos", (b, 3), f32), ("overflow", 1, i32), ("live_x3", 1, i32),
-        ("scan_ws", _native.query("mt_scan_workspace_bytes", b * n_blk), u8),
+        ("keys_ws", _native.query("mt_generate_keys_single_pass_workspace_bytes",
+                                  scene.faces.shape[0], b), u8),
         ("sort_ws", _native.query("mt_sort_workspace_bytes", k_cap), u8),
     ]
     return dims, tile_base, pixel_base, plan
@@ -190,7 +189,6 @@ def render_batch(scene: ResidentScene, extrinsics: torch.Tensor, intrinsics: tor
     s = scene.num_shells
     v = scene.vertices.shape[0]
     f = scene.faces.shape[0]
-    n_blk = (f + 255) // 256
     k_cap = key_capacity or sum(scene.keys_for(h, w, key_margin) for h, w in shapes)
     dims, tile_base, pixel_base, plan = _batch_plan(scene, shapes, tile_shape, k_cap)
     rows, cols = dims[0][2], dims[0][3]
@@ -206,23 +204,16 @@ def render_batch(scene: ResidentScene, extrinsics: torch.Tensor, intrinsics: tor
     t["overflow"].zero_()
     t["ranges"].zero_()
     xy, depth, colors, out = t["xy"], t["depth"], t["colors"], t["out"]
-    block_counts, block_offsets = t["block_counts"], t["block_offsets"]
     keys, vals, skeys, svals = t["keys"], t["vals"], t["skeys"], t["svals"]
-    scan_ws, sort_ws = t["scan_ws"], t["sort_ws"]
+    keys_ws, sort_ws, n_dev = t["keys_ws"], t["sort_ws"], t["n_keys"]
     live_x3, overflow, ranges, campos = t["live_x3"], t["overflow"], t["ranges"], t["campos"]
     dims_p = cam_dims.data_ptr() if table else None
     call("mt_project_vertices", scene.vertices.data_ptr(), v, E.data_ptr(), K.data_ptr(), b,
          xy.data_ptr(), depth.data_ptr(), st)
-    call("mt_key_block_counts", xy.data_ptr(), depth.data_ptr(), v, scene.faces.data_ptr(), f,
-         live_x3.data_ptr(), b, th, tw, 0, 0, rows, cols, float(near_plane), dims_p,
-         block_counts.data_ptr(), st)
-    call("mt_inclusive_scan_i32_to_i64", block_counts.data_ptr(), block_offsets.data_ptr(), b * n_blk,
-         scan_ws.data_ptr(), scan_ws.numel(), st)
-    n_dev = block_offsets[b * n_blk - 1 :]
-    call("mt_generate_keys_from_geometry", xy.data_ptr(), depth.data_ptr(), v, scene.faces.data_ptr(),
+    call("mt_generate_keys_single_pass", xy.data_ptr(), depth.data_ptr(), v, scene.faces.data_ptr(),
          f, live_x3.data_ptr(), scene.face_boundaries.data_ptr(), s, b, th, tw, 0, 0, rows, cols,
-         float(near_plane), dims_p, block_offsets.data_ptr(), cols, _LAYOUT, keys.data_ptr(),
-         vals.data_ptr(), k_cap, overflow.data_ptr(), st)
+         float(near_plane), dims_p, 1, cols, _LAYOUT, keys.data_ptr(), vals.data_ptr(), k_cap,
+         n_dev.data_ptr(), overflow.data_ptr(), keys_ws.data_ptr(), keys_ws.numel(), st)
     call("mt_sort_keys", keys.data_ptr(), vals.data_ptr(), k_cap, skeys.data_ptr(), svals.data_ptr(),
          sort_ws.data_ptr(), sort_ws.numel(), n_dev.data_ptr(), st)
     call("mt_delineate_tiles", skeys.data_ptr(), k_cap, _LAYOUT, 0, b, 1, rows * cols,
@@ -251,15 +242,13 @@ def render_batch(scene: ResidentScene, extrinsics: torch.Tensor, intrinsics: tor
 
 
 def _max_keys(scene, extrinsics, intrinsics, image_shape, tile_shape, near_plane) -> int:
-    from . import triangle_rasterization as tr
+    from .batched import count_keys
 
     best = 0
     with torch.no_grad():
         for c in range(extrinsics.shape[0]):
-            grid = TileGrid(tuple(tile_shape), tuple(image_shape), 1, scene.num_shells)
-            xy, depth = tr.project_vertices(scene.vertices, extrinsics[c : c + 1], intrinsics[c : c + 1])
-            bounds = tr.bound_triangles(xy, depth, scene.faces, grid, near_plane)
-            best = max(best, int(bounds.num_tiles_overlapped.to(torch.int64).sum()))
+            best = max(best, count_keys(scene.vertices, scene.faces, extrinsics[c : c + 1],
+                                        intrinsics[c : c + 1], image_shape, tile_shape, near_plane))
     return best
 
 

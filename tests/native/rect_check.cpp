@@ -5,7 +5,8 @@
 // with -ffp-contract=off evaluates the same expressions as the GPU kernels.
 //
 // Prints "pairs=<tested> hits=<accepted> outside=<violations> area=<sum of rect areas>
-// full=<whole-tile fallbacks>" and exits 1 on any violation.
+// full=<whole-tile fallbacks> pixels=<sum of tile / region pixel counts>" and exits 1 on
+// any violation.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -103,10 +104,13 @@ Tri random_triangle(int kind, double cx, double cy) {
 int main(int argc, char** argv) {
   const long n = argc > 1 ? std::atol(argv[1]) : 200000;
   if (argc > 2) g_state = std::strtoull(argv[2], nullptr, 10) | 1ull;
-  static const int shapes[4][2] = {{8, 8}, {16, 16}, {32, 8}, {16, 32}};  // (h, w)
-  long long pairs = 0, hits = 0, outside = 0, area = 0, full = 0;
+  // (h, w): the autotuner's tile shapes, and multi-tile regions (the key kernels bound a
+  // face's whole tile-aligned bounding box at once).
+  static const int shapes[8][2] = {{8, 8},   {16, 16}, {32, 8}, {16, 32},
+                                   {24, 40}, {40, 72}, {8, 96}, {64, 16}};
+  long long pairs = 0, hits = 0, outside = 0, area = 0, full = 0, pixels = 0;
   for (long it = 0; it < n; ++it) {
-    const int* sh = shapes[next_u64() % 4];
+    const int* sh = shapes[next_u64() % 8];
     const int th = sh[0], tw = sh[1];
     const int x0 = int(next_u64() % 160) * tw, y0 = int(next_u64() % 100) * th;
     const double cx = x0 + uniform(-4.0, tw + 4.0), cy = y0 + uniform(-4.0, th + 4.0);
@@ -114,6 +118,7 @@ int main(int argc, char** argv) {
     const mt::TriTerms terms = mt::tri_terms(t.x[0], t.y[0], t.x[1], t.y[1], t.x[2], t.y[2]);
     const mt::Rect r = mt::candidate_rect(terms, x0, y0, tw, th);
     area += (long long)r.w * r.h;
+    pixels += (long long)tw * th;
     if (r.w == tw && r.h == th) ++full;
     if (r.x < 0 || r.y < 0 || r.w < 0 || r.h < 0 || r.x + r.w > tw || r.y + r.h > th) {
       std::printf("bad rectangle %d %d %d %d\n", r.x, r.y, r.w, r.h);
@@ -139,7 +144,7 @@ int main(int argc, char** argv) {
       }
     }
   }
-  std::printf("pairs=%lld hits=%lld outside=%lld area=%lld full=%lld\n", pairs, hits, outside, area,
-              full);
+  std::printf("pairs=%lld hits=%lld outside=%lld area=%lld full=%lld pixels=%lld\n", pairs, hits, outside,
+              area, full, pixels);
   return outside == 0 ? 0 : 1;
 }

@@ -45,7 +45,8 @@ def make_step(scene, margin=1.2, **kw):
     spec, vox, world, sdf, sh, cams, target = scene
     ls = spec.level_sets.to(sdf.device)
     cap = batched.measure_capacities(vox, ls, cams.extrinsics, cams.intrinsics, cams.image_shape,
-                                     world, sdf, sh).scaled(margin)
+                                     world, sdf, sh, cull_keys=kw.get("cull_keys", True)
+                                     ).scaled(margin)
     return batched.BatchedTrainingStep(vox, ls, cams.extrinsics, cams.intrinsics, cams.image_shape,
                                        sh.shape[0], spec.sh_degree, spec.sharpness, spec.background,
                                        cap, **kw)
@@ -99,6 +100,23 @@ def test_overflow_is_reported_not_written(scene):
     step.run(world, sdf, sh, target)
     torch.cuda.synchronize()
     assert step.overflowed()
+
+
+def test_culled_keys_change_nothing(scene):
+    """Keys only for tiles that meet a face's candidate pixel rectangle: far fewer keys,
+    the same images, the same loss, the same gradients (summation order aside)."""
+    spec, vox, world, sdf, sh, cams, target = scene
+    full = make_step(scene, cull_keys=False)
+    full.run(world, sdf, sh, target)
+    culled = make_step(scene)
+    culled.run(world, sdf, sh, target)
+    assert not full.overflowed() and not culled.overflowed()
+    k_full, k_culled = full.live_sizes()["K"], culled.live_sizes()["K"]
+    assert 0 < k_culled < k_full
+    assert torch.equal(culled.out, full.out)
+    assert abs(float(culled.loss) - float(full.loss)) <= 1e-6 * float(full.loss)
+    assert _rel(culled.grid_sdf_grad, full.grid_sdf_grad) < 1e-5
+    assert _rel(culled.grid_sh_grad, full.grid_sh_grad) < 1e-5
 
 
 def test_camera_groups_match_one_group(scene):
