@@ -235,10 +235,12 @@ int mt_generate_keys(const float* depths, int64_t V, const int32_t* faces, int64
  * mt_bound_triangles' arithmetic (same culling, same num_faces_x3 semantics). cam_dims
  * int32[B, 4] (height, width, grid_rows, grid_cols), or NULL: camera b's tile grid is
  * [0, grid_rows_b) x [0, grid_cols_b) instead of [row_min, row_max) x [col_min, col_max).
- * cull = 1 narrows each face's tiles to those that meet its candidate pixel rectangle
- * (the bound the composites use per tile, here over the face's whole tile box): keys of
- * tiles where no pixel can pass the barycentric test are dropped, which changes no image
- * or gradient of the fused composites; cull = 0 keeps every tile of the bounding box. */
+ * cull = 1 narrows each face's tiles to those where some pixel can pass the barycentric
+ * test: the candidate pixel rectangle (the bound the composites use per tile, here over the
+ * face's whole tile box) excludes the others, and when it holds at most 16 pixels each is
+ * tested with the reference's float operations. Keys of tiles where no pixel can pass are
+ * dropped, which changes no image or gradient of the fused composites; cull = 0 keeps every
+ * tile of the bounding box. */
 int mt_key_block_counts(const float* xy, const float* depths, int64_t V, const int32_t* faces,
                         int64_t F, const int32_t* num_faces_x3, int32_t B, int32_t tile_h,
                         int32_t tile_w, int32_t row_min, int32_t col_min, int32_t row_max,

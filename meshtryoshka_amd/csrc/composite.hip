@@ -71,24 +71,15 @@ struct Bary {
 };
 
 // The reference's barycentric test (intersect_and_composite.slang:65-145) at pixel centre
-// (px, py). Miss proof without division: with r = rcp(D) (relative error <= 2^-22) the
-// product q * r is within a factor (1 +- 2^-21) of q / D. If it is < -2^-100 the exactly
-// rounded q / D is a nonzero negative number; if it is > 1 + 2^-19 the rounded quotient
-// exceeds 1. Either way the reference rejects the pixel, so the division is skipped.
-// r == 0 (D outside [2^-60, 2^60] or NaN) disables the shortcut.
+// (px, py), with the divisions skipped where rcp(D) already proves a miss (geometry.h;
+// s.g3.z = rcp(D), or 0 when D is outside [2^-60, 2^60] or NaN).
 __device__ __forceinline__ Bary bary_test(const Staged& s, float px, float py) {
   Bary out;
   const BaryNum n = bary_numerators(to_terms(s), px, py);
-  const float r = s.g3.z;
-  if (r != 0.0f) {
-    const float ua = n.un * r, va = n.vn * r, wa = n.wn * r;
-    constexpr float kNeg = -0x1p-100f;
-    constexpr float kBig = 1.0f + 0x1p-19f;
-    if (ua < kNeg || va < kNeg || wa < kNeg || ua > kBig || va > kBig || wa > kBig) {
-      out.hit = false;
-      out.u = out.v = out.w = 0.0f;
-      return out;
-    }
+  if (rcp_proves_miss(n, s.g3.z)) {
+    out.hit = false;
+    out.u = out.v = out.w = 0.0f;
+    return out;
   }
   const float D = s.g3.y;
   out.v = n.vn / D;

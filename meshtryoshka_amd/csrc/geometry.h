@@ -93,6 +93,20 @@ MT_HD bool bary_accept(float u, float v, float w) {
   return !miss;
 }
 
+// Miss proof without division: with r an approximate reciprocal of D (relative error
+// <= 2^-22, e.g. the hardware rcp) the product q * r is within a factor (1 +- 2^-21) of
+// q / D. If it is < -2^-100 the exactly rounded q / D is a nonzero negative number; if it
+// is > 1 + 2^-19 the rounded quotient exceeds 1. Either way the reference rejects the
+// pixel, so the divisions can be skipped. r == 0 disables the shortcut (callers pass 0
+// unless |D| is in [2^-60, 2^60]).
+MT_HD bool rcp_proves_miss(const BaryNum& n, float r) {
+  if (r == 0.0f) return false;
+  const float ua = n.un * r, va = n.vn * r, wa = n.wn * r;
+  constexpr float kNeg = -0x1p-100f;
+  constexpr float kBig = 1.0f + 0x1p-19f;
+  return ua < kNeg || va < kNeg || wa < kNeg || ua > kBig || va > kBig || wa > kBig;
+}
+
 // Candidate rectangle of a triangle inside one tile.
 //
 // The tile's pixel centres are (x0 + i + 0.5, y0 + j + 0.5), 0 <= i < tw, 0 <= j < th.
@@ -200,9 +214,11 @@ MT_HD Rect candidate_rect(const TriTerms& t, int tile_x0, int tile_y0, int tw, i
   if (!spread || !(pos || neg)) return full;
 
   // Vertices: line i meets line j at ((c_j ny_i - c_i ny_j), (c_i nx_j - c_j nx_i)) / cross.
-  const double q0x = (c[1] * ny[0] - c[0] * ny[1]) / c01, q0y = (c[0] * nx[1] - c[1] * nx[0]) / c01;
-  const double q1x = (c[2] * ny[1] - c[1] * ny[2]) / c12, q1y = (c[1] * nx[2] - c[2] * nx[1]) / c12;
-  const double q2x = (c[0] * ny[2] - c[2] * ny[0]) / c20, q2y = (c[2] * nx[0] - c[0] * nx[2]) / c20;
+  // One division per pair of coordinates (the extra rounding is far inside the slack).
+  const double i01 = 1.0 / c01, i12 = 1.0 / c12, i20 = 1.0 / c20;
+  const double q0x = (c[1] * ny[0] - c[0] * ny[1]) * i01, q0y = (c[0] * nx[1] - c[1] * nx[0]) * i01;
+  const double q1x = (c[2] * ny[1] - c[1] * ny[2]) * i12, q1y = (c[1] * nx[2] - c[2] * nx[1]) * i12;
+  const double q2x = (c[0] * ny[2] - c[2] * ny[0]) * i20, q2y = (c[2] * nx[0] - c[0] * nx[2]) * i20;
   double xmin = q0x < q1x ? q0x : q1x, xmax = q0x > q1x ? q0x : q1x;
   double ymin = q0y < q1y ? q0y : q1y, ymax = q0y > q1y ? q0y : q1y;
   xmin = xmin < q2x ? xmin : q2x;

@@ -79,28 +79,56 @@ __device__ __forceinline__ TileRect face_tile_rect(const float* xy, const float*
   return r;
 }
 
-// Narrows a face's tile rectangle to the tiles that meet its candidate pixel rectangle
-// over the whole tile-aligned box (geometry.h, the bound the composites use per tile): no
-// pixel of the other tiles can pass the barycentric test for this face, so their keys
-// cannot change any pixel or gradient. Most sub-pixel faces cover no pixel centre at all
-// and get no key.
+// Narrows a face's tile rectangle to the tiles where some pixel can pass the barycentric
+// test. The candidate pixel rectangle over the whole tile-aligned box (geometry.h, the
+// bound the composites use per tile) excludes the other pixels; when it holds at most
+// kExactPixels pixels, each of them is put through the test itself (the reference's float
+// operations, as the composites evaluate them), and only tiles with an accepted pixel
+// keep a key. Keys dropped this way cannot change any pixel or gradient; most sub-pixel
+// faces cover no pixel centre at all and get no key.
+constexpr int kExactPixels = 16;
+
 __device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, int64_t o, int64_t ia,
                                                int64_t ib, int64_t ic, int th, int tw) {
   if (r.y1 <= r.y0 || r.x1 <= r.x0) return r;
   const float2* pxy = reinterpret_cast<const float2*>(xy);
   const float2 a = pxy[o + ia], bb = pxy[o + ib], c = pxy[o + ic];
   const TriTerms t = tri_terms(a.x, a.y, bb.x, bb.y, c.x, c.y);
-  const Rect q = candidate_rect(t, r.x0 * tw, r.y0 * th, (r.x1 - r.x0) * tw, (r.y1 - r.y0) * th);
+  const int px0 = r.x0 * tw, py0 = r.y0 * th;
+  const Rect q = candidate_rect(t, px0, py0, (r.x1 - r.x0) * tw, (r.y1 - r.y0) * th);
   TileRect n;
-  if (q.w <= 0 || q.h <= 0) {
-    n.y0 = n.y1 = r.y0;
-    n.x0 = n.x1 = r.x0;
+  n.y0 = n.y1 = r.y0;
+  n.x0 = n.x1 = r.x0;
+  if (q.w <= 0 || q.h <= 0) return n;
+  if (q.w * q.h > kExactPixels) {
+    n.y0 = r.y0 + q.y / th;
+    n.y1 = r.y0 + (q.y + q.h - 1) / th + 1;
+    n.x0 = r.x0 + q.x / tw;
+    n.x1 = r.x0 + (q.x + q.w - 1) / tw + 1;
     return n;
   }
-  n.y0 = r.y0 + q.y / th;
-  n.y1 = r.y0 + (q.y + q.h - 1) / th + 1;
-  n.x0 = r.x0 + q.x / tw;
-  n.x1 = r.x0 + (q.x + q.w - 1) / tw + 1;
+  const float ad = fabsf(t.D);
+  const float rd = (ad >= 0x1p-60f && ad <= 0x1p60f) ? __builtin_amdgcn_rcpf(t.D) : 0.0f;
+  int ax0 = 0x7fffffff, ay0 = 0x7fffffff, ax1 = -1, ay1 = -1;  // accepted pixels' box
+  for (int j = 0; j < q.h; ++j) {
+    for (int i = 0; i < q.w; ++i) {
+      const int X = px0 + q.x + i, Y = py0 + q.y + j;
+      const BaryNum m = bary_numerators(t, float(X) + 0.5f, float(Y) + 0.5f);
+      if (rcp_proves_miss(m, rd)) continue;
+      const float v = m.vn / t.D, w = m.wn / t.D, u = m.un / t.D;
+      if (bary_accept(u, v, w)) {
+        ax0 = X < ax0 ? X : ax0;
+        ax1 = X > ax1 ? X : ax1;
+        ay0 = Y < ay0 ? Y : ay0;
+        ay1 = Y > ay1 ? Y : ay1;
+      }
+    }
+  }
+  if (ax1 < 0) return n;
+  n.y0 = ay0 / th;
+  n.y1 = ay1 / th + 1;
+  n.x0 = ax0 / tw;
+  n.x1 = ax1 / tw + 1;
   return n;
 }
 
@@ -351,11 +379,16 @@ __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
 }
 
 // Single pass: the counts, their scan and the keys in one kernel. Blocks take their
-// (camera, block of kPassFaces faces) index from a ticket, count their faces' keys, find
-// their first output position by decoupled look-back over the earlier blocks' published
-// counts, and write their keys; each face's (culled) tile rectangle is computed once. A
-// block covers kPassFaces = 4 x 256 faces so that the look-back is paid once per 1024
-// faces. The last block also stores the total key count.
+// (camera, block of kPassFaces faces) index from a ticket and
+//   1. bound each face (culling, tile-aligned box) and list the faces whose box needs the
+//      pixel-level narrowing (cull_tiles) in LDS,
+//   2. narrow the listed faces with every thread busy (back-facing, clipped and keyless
+//      faces take no thread),
+//   3. scan the key counts in face order, find the block's first output position by
+//      decoupled look-back over the earlier blocks' published counts, and write the keys.
+// Each face's tile rectangle is computed once; a block covers kPassFaces = 4 x 256 faces
+// so that the look-back is paid once per 1024 faces. The last block also stores the total
+// key count.
 constexpr int kPassPer = 4;
 constexpr int kPassFaces = kPassPer * kBlock;
 
@@ -365,12 +398,17 @@ __global__ void __launch_bounds__(kBlock) keys_single_pass_kernel(
     int cmin, int rmax, int cmax, float near_plane, const int32_t* cam_dims, int cull,
     int grid_cols, KeyFields kf, int64_t* keys, int32_t* values, int64_t kcap, int64_t* n_keys,
     int32_t* overflow, ScanState st) {
-  __shared__ int32_t s_prefix[kPassFaces];
-  __shared__ int32_t s_r0[kPassFaces], s_c0[kPassFaces], s_w[kPassFaces];
+  __shared__ int32_t s_prefix[kPassFaces];  // key count, then its exclusive scan
+  __shared__ int32_t s_r0[kPassFaces], s_c0[kPassFaces], s_w[kPassFaces], s_h[kPassFaces];
   __shared__ uint64_t s_key_hi[kPassFaces];
+  __shared__ int16_t s_work[kPassFaces];
   __shared__ int32_t s_wave[kBlock / kWave];
+  __shared__ int32_t s_nwork;
   __shared__ int64_t s_q, s_block_start;
-  if (threadIdx.x == 0) s_q = atomicAdd(st.ticket, 1u);
+  if (threadIdx.x == 0) {
+    s_q = atomicAdd(st.ticket, 1u);
+    s_nwork = 0;
+  }
   __syncthreads();
   const int64_t q = s_q;
   const int64_t nblk = (F + kPassFaces - 1) / kPassFaces;
@@ -384,29 +422,63 @@ __global__ void __launch_bounds__(kBlock) keys_single_pass_kernel(
     grid_cols = cmax;
   }
   const int64_t o = int64_t(b) * V;
-  const int lane = __lane_id();
-  const int wave = threadIdx.x / kWave;
-  int32_t carry = 0;
+  auto key_hi = [&](int64_t f, int64_t ia, int64_t ib, int64_t ic) {
+    uint32_t shell = 0;
+    while (shell < uint32_t(S) && !(f < boundaries[shell])) ++shell;
+    const float mean_depth = ((depths[o + ia] + depths[o + ib]) + depths[o + ic]) / 3.f;
+    return pack_key(kf, uint32_t(b), shell, 0u, mean_depth);
+  };
+  // 1. Bounds.
   for (int i = 0; i < kPassPer; ++i) {
     const int l = i * kBlock + threadIdx.x;
     const int64_t f = fb * kPassFaces + l;
     int32_t count = 0;
     int64_t ia = 0, ib = 0, ic = 0;
     if (f < F && keyed_face(faces, f, V, live_x3, ia, ib, ic)) {
-      TileRect r = face_tile_rect(xy, depths, o, ia, ib, ic, th, tw, rmin, cmin, rmax, cmax,
-                                  near_plane);
-      if (cull) r = cull_tiles(r, xy, o, ia, ib, ic, th, tw);
-      count = (r.y1 - r.y0) * (r.x1 - r.x0);
+      const TileRect r = face_tile_rect(xy, depths, o, ia, ib, ic, th, tw, rmin, cmin, rmax, cmax,
+                                        near_plane);
       s_r0[l] = r.y0;
       s_c0[l] = r.x0;
       s_w[l] = r.x1 - r.x0;
-      if (count > 0) {
-        uint32_t shell = 0;
-        while (shell < uint32_t(S) && !(f < boundaries[shell])) ++shell;
-        const float mean_depth = ((depths[o + ia] + depths[o + ib]) + depths[o + ic]) / 3.f;
-        s_key_hi[l] = pack_key(kf, uint32_t(b), shell, 0u, mean_depth);
+      s_h[l] = r.y1 - r.y0;
+      count = (r.y1 - r.y0) * (r.x1 - r.x0);
+      if (count > 0 && cull) {
+        s_work[atomicAdd(&s_nwork, 1)] = static_cast<int16_t>(l);
+        count = 0;  // set by step 2
+      } else if (count > 0) {
+        s_key_hi[l] = key_hi(f, ia, ib, ic);
       }
     }
+    s_prefix[l] = count;
+  }
+  __syncthreads();
+  // 2. Pixel-level narrowing of the listed faces.
+  const int nwork = s_nwork;
+  for (int k = threadIdx.x; k < nwork; k += kBlock) {
+    const int l = s_work[k];
+    const int64_t f = fb * kPassFaces + l;
+    const int64_t ia = faces[3 * f + 0], ib = faces[3 * f + 1], ic = faces[3 * f + 2];
+    TileRect r;
+    r.y0 = s_r0[l];
+    r.x0 = s_c0[l];
+    r.y1 = r.y0 + s_h[l];
+    r.x1 = r.x0 + s_w[l];
+    r = cull_tiles(r, xy, o, ia, ib, ic, th, tw);
+    const int32_t count = (r.y1 - r.y0) * (r.x1 - r.x0);
+    s_r0[l] = r.y0;
+    s_c0[l] = r.x0;
+    s_w[l] = r.x1 - r.x0;
+    s_prefix[l] = count;
+    if (count > 0) s_key_hi[l] = key_hi(f, ia, ib, ic);
+  }
+  __syncthreads();
+  // 3. Scan in face order, look-back, keys.
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / kWave;
+  int32_t carry = 0;
+  for (int i = 0; i < kPassPer; ++i) {
+    const int l = i * kBlock + threadIdx.x;
+    const int32_t count = s_prefix[l];
     const int32_t inc = wave_inclusive_sum(count);
     if (lane == kWave - 1) s_wave[wave] = inc;
     __syncthreads();
