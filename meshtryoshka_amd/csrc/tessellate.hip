@@ -443,19 +443,27 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
       acc0_d += w0 * gd;
       acce_d += t * gd;
       const float3 p = vertex_at(tv, here);
-      for (int b = 0; b < B; ++b) {
-        const float3 d = view_dir(p, cams + 3 * b);
+      const float* cb = colors + 3 * here;  // camera b's colour of this vertex: cb + b * 3 nt
+      const float* gb = g_col + 3 * here;
+      const int64_t stride = 3 * nt;
+      for (int b = 0; b < B; ++b, cb += stride, gb += stride) {
+        // The view direction with the hardware reciprocal square root (~1 ulp): the
+        // gradient tolerates it, and it saves the correctly rounded sqrt and division that
+        // the forward needs for bit-exact colours.
+        const float dx = p.x - cams[3 * b + 0], dy = p.y - cams[3 * b + 1], dz = p.z - cams[3 * b + 2];
+        const float inv = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
         float basis[16];
-        sh_basis(d.x, d.y, d.z, kDeg, basis);
+        sh_basis(dx * inv, dy * inv, dz * inv, kDeg, basis);
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-          const int64_t q = (int64_t(b) * nt + here) * 3 + ch;
-          const float gc = colors[q] > 0.0f ? g_col[q] : 0.0f;
+          const float gc = cb[ch] > 0.0f ? gb[ch] : 0.0f;
           const float g0 = w0 * gc, g1 = t * gc;
+          // Fused multiply-adds (the build keeps separate roundings elsewhere, for bit-exact
+          // forward values; a gradient sum only needs the tolerance).
 #pragma unroll
           for (int k = 0; k < kN; ++k) {
-            acc0[k][ch] += basis[k] * g0;
-            acce[k][ch] += basis[k] * g1;
+            acc0[k][ch] = __builtin_fmaf(basis[k], g0, acc0[k][ch]);
+            acce[k][ch] = __builtin_fmaf(basis[k], g1, acce[k][ch]);
           }
         }
       }
