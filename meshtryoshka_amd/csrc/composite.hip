@@ -433,9 +433,62 @@ __device__ __forceinline__ void scatter_sample(const mt_composite_args& a, int c
   }
 }
 
+// Per staged triangle of the current chunk, its three vertices' colour (3) and SDF (1)
+// gradient sums in LDS: the pixels of a tile that hit one triangle add into LDS, and one
+// set of global atomics per triangle and chunk replaces one per hit.
+constexpr int kGradSlots = 12;
+
+__host__ __device__ inline size_t grad_lds_bytes(int chunk) {
+  return align16(sizeof(float) * kGradSlots * size_t(chunk));
+}
+
+__device__ __forceinline__ void scatter_sample_lds(float* acc, float bu, float bv, float bw,
+                                                   float dr, float dg, float db, float dsdf) {
+  const float wgt[3] = {bu, bv, bw};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    atomicAdd(acc + 4 * q + 0, wgt[q] * dr);
+    atomicAdd(acc + 4 * q + 1, wgt[q] * dg);
+    atomicAdd(acc + 4 * q + 2, wgt[q] * db);
+    atomicAdd(acc + 4 * q + 3, wgt[q] * dsdf);
+  }
+}
+
+// Adds the chunk's nonzero per-triangle sums to the global gradients and clears them for
+// the next chunk (the sums start at zero and only hit triangles' sums become nonzero).
+__device__ __forceinline__ void flush_chunk_grads(const ChunkLds& L, float* acc, int count,
+                                                  const int32_t* faces, int64_t num_vertices,
+                                                  int cam, float* colors_grad, float* sdf_grad) {
+  float* cgrad = colors_grad + int64_t(cam) * num_vertices * 3;
+  for (int j = threadIdx.x; j < count; j += blockDim.x) {
+    float* g = acc + kGradSlots * j;
+    float e[kGradSlots];
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < kGradSlots; ++i) {
+      e[i] = g[i];
+      any = any || e[i] != 0.0f;
+    }
+    if (!any) continue;
+    const int64_t face = L.face[j];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int64_t v = faces[3 * face + q];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (e[4 * q + c] != 0.0f) atomicAdd(cgrad + 3 * v + c, e[4 * q + c]);
+      if (e[4 * q + 3] != 0.0f) atomicAdd(sdf_grad + v, e[4 * q + 3]);
+    }
+#pragma unroll
+    for (int i = 0; i < kGradSlots; ++i) g[i] = 0.0f;
+  }
+}
+
 // ------------------------------------------------------------------------------------
 // Fused backward (intersect_and_composite.slang:341-564): the same traversal, recomputed,
-// with the suffix trick for the accumulated colour.
+// with the suffix trick for the accumulated colour. A sample's gradient is added to its
+// triangle's LDS sums when the triangle is still staged (the usual case), otherwise
+// straight to global memory.
 // ------------------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a, const float* out,
@@ -444,6 +497,8 @@ __global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a
                                                               int quirk) {
   const int npix = a.tile_h * a.tile_w;
   const ChunkLds L = carve_chunk(a.chunk, npix);
+  float* gacc = reinterpret_cast<float*>(g_dynamic_lds + chunk_lds_bytes(a.chunk, npix));
+  for (int j = threadIdx.x; j < kGradSlots * a.chunk; j += blockDim.x) gacc[j] = 0.0f;
   __shared__ int s_wave[kMaxTilePixels / kWave];
   __shared__ float s_red[kMaxTilePixels / kWave];
   const TileCoord tc = locate_tile(a, xcd_swizzle(blockIdx.x, gridDim.x));
@@ -477,6 +532,7 @@ __global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a
   uint32_t n_tests = 0, n_hits = 0;
   // Sample waiting for its outer-role SDF gradient (set by the next accepted hit).
   bool pending = false;
+  int p_slot = -1;  // the pending sample's triangle in the staged chunk, -1 once restaged
   int64_t p_v[3] = {0, 0, 0};
   float p_u = 0.f, p_v_ = 0.f, p_w = 0.f, p_dr = 0.f, p_dg = 0.f, p_db = 0.f, p_dsdf = 0.f;
 
@@ -486,6 +542,7 @@ __global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a
     __syncthreads();
     stage_chunk(L, xy, a.faces, a.sorted_values, a.sorted_keys, a.layout.shell_offset,
                 a.layout.shell_bits, first, count, tc.x0, tc.y0, a.tile_w, a.tile_h);
+    p_slot = -1;
     __syncthreads();
     const int total = scan_areas(L, count, s_wave);
     __syncthreads();
@@ -533,10 +590,15 @@ __global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a
           if (pending) {
             p_dsdf += d_out;
             const bool q = quirk != 0;
-            scatter_sample(a, tc.cam, p_v, q ? bc.u : p_u, q ? bc.v : p_v_, q ? bc.w : p_w, p_dr,
-                           p_dg, p_db, p_dsdf, colors_grad, sdf_grad);
+            const float su = q ? bc.u : p_u, sv = q ? bc.v : p_v_, sw = q ? bc.w : p_w;
+            if (p_slot >= 0)
+              scatter_sample_lds(gacc + kGradSlots * p_slot, su, sv, sw, p_dr, p_dg, p_db, p_dsdf);
+            else
+              scatter_sample(a, tc.cam, p_v, su, sv, sw, p_dr, p_dg, p_db, p_dsdf, colors_grad,
+                             sdf_grad);
           }
           pending = true;
+          p_slot = i;
           p_v[0] = sm.v[0];
           p_v[1] = sm.v[1];
           p_v[2] = sm.v[2];
@@ -559,9 +621,11 @@ __global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a
       }
       L.live[tc.pix] = active ? 1 : 0;
     }
-    if (!__syncthreads_or(active)) break;
+    const bool more = __syncthreads_or(active);  // also: every LDS sum of the chunk is in
+    flush_chunk_grads(L, gacc, count, a.faces, a.num_vertices, tc.cam, colors_grad, sdf_grad);
+    if (!more) break;
   }
-  if (pending) {
+  if (pending) {  // its triangle's chunk has been flushed
     scatter_sample(a, tc.cam, p_v, p_u, p_v_, p_w, p_dr, p_dg, p_db, p_dsdf, colors_grad,
                    sdf_grad);
   }
@@ -857,10 +921,13 @@ int check_composite_args(const mt_composite_args* a, const char* name) {
 
 // Chunk actually staged: the requested one (0 = one key per pixel), at most kMaxChunk and
 // small enough that the stage buffers and hit masks fit the LDS budget.
-static int resolve_chunk(int requested, int npix) {
+static int resolve_chunk(int requested, int npix, bool grads = false) {
   int c = requested > 0 ? requested : npix;
   if (c > mt::kMaxChunk) c = mt::kMaxChunk;
-  while (c > 32 && mt::chunk_lds_bytes(c, npix) > size_t(mt::kMaxLds)) c -= 32;
+  auto bytes = [&](int cc) {
+    return mt::chunk_lds_bytes(cc, npix) + (grads ? mt::grad_lds_bytes(cc) : size_t(0));
+  };
+  while (c > 32 && bytes(c) > size_t(mt::kMaxLds)) c -= 32;
   return c;
 }
 
@@ -888,12 +955,13 @@ extern "C" int mt_intersect_and_composite_backward(const mt_composite_args* args
   if (int rc = mt::check_composite_args(args, "mt_intersect_and_composite_backward")) return rc;
   mt_composite_args a = *args;
   const int npix = a.tile_h * a.tile_w;
-  a.chunk = resolve_chunk(args->chunk, npix);
+  a.chunk = resolve_chunk(args->chunk, npix, true);
   const int64_t blocks =
       a.tile_base ? a.total_tiles : int64_t(a.num_cameras) * a.grid_rows * a.grid_cols;
   if (blocks == 0) return MT_OK;
   mt::fused_backward_kernel<<<static_cast<unsigned>(blocks), block_threads(npix),
-                              mt::chunk_lds_bytes(a.chunk, npix), static_cast<hipStream_t>(stream)>>>(
+                              mt::chunk_lds_bytes(a.chunk, npix) + mt::grad_lds_bytes(a.chunk),
+                              static_cast<hipStream_t>(stream)>>>(
       a, out, grad_out, colors_grad, sdf_grad, sharpness_grad, reference_quirk);
   return mt::check_launch("mt_intersect_and_composite_backward");
 }
