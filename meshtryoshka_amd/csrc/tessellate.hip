@@ -312,24 +312,28 @@ __global__ void __launch_bounds__(kBlock) gather_vertices_backward_kernel(
 // The block's 256 column origins' SH rows are staged once in LDS (read by every crossing
 // of the column, over all level sets, and by the +x crossing of the previous column);
 // only the other axis ends' rows are read from global memory per vertex.
+// 128 columns per block: the staged origin rows take 25 KB of LDS, so four or five blocks
+// share a CU (256 columns: two).
+constexpr int kVcBlock = 128;
+
 template <int kDeg>
-__global__ void __launch_bounds__(kBlock) create_vertices_colors_kernel(
+__global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
     const float* gv, const float* gsdf, const float* gsh, int64_t ng, const int32_t* lower,
     int64_t n, const float* level_sets, int S, const int32_t* offsets, int64_t nt,
     const float* cams, int B, float* ov, float* od, uint8_t* otype, float* colors,
     int32_t* overflow) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 3 * kN + 1;  // padded: consecutive rows start in different banks
-  __shared__ float s_sh[kBlock * kRow];
-  __shared__ int32_t s_g0[kBlock];
-  __shared__ float s_c0[kBlock];  // the column's origin SDF
-  __shared__ int32_t s_v1[3 * kBlock];
-  __shared__ float s_t[3 * kBlock], s_d1[3 * kBlock];
-  __shared__ uint8_t s_axis[3 * kBlock], s_col[3 * kBlock];
+  __shared__ float s_sh[kVcBlock * kRow];
+  __shared__ int32_t s_g0[kVcBlock];
+  __shared__ float s_c0[kVcBlock];  // the column's origin SDF
+  __shared__ int32_t s_v1[3 * kVcBlock];
+  __shared__ float s_t[3 * kVcBlock], s_d1[3 * kVcBlock];
+  __shared__ uint8_t s_axis[3 * kVcBlock], s_col[3 * kVcBlock];
   __shared__ int s_total;
-  const int64_t c_first = int64_t(blockIdx.x) * kBlock;
+  const int64_t c_first = int64_t(blockIdx.x) * kVcBlock;
   const int64_t c = c_first + threadIdx.x;
-  const int64_t c_last = (c_first + kBlock < n ? c_first + kBlock : n) - 1;
+  const int64_t c_last = (c_first + kVcBlock < n ? c_first + kVcBlock : n) - 1;
   const bool valid = c < n;
   ColumnSdf col{};
   if (valid) {
@@ -364,7 +368,7 @@ __global__ void __launch_bounds__(kBlock) create_vertices_colors_kernel(
     const int total = s_total;
     if (base + total > nt && threadIdx.x == 0 && overflow) atomicOr(overflow, 1);
     const int live = static_cast<int>(base >= nt ? 0 : (base + total <= nt ? total : nt - base));
-    for (int v = threadIdx.x; v < live; v += kBlock) {
+    for (int v = threadIdx.x; v < live; v += kVcBlock) {
       const int64_t idx = base + v;
       const int lc = s_col[v];
       const int64_t g0 = s_g0[lc], g1 = s_v1[v];
@@ -379,7 +383,7 @@ __global__ void __launch_bounds__(kBlock) create_vertices_colors_kernel(
       od[idx] = d0 + t * (s_d1[v] - d0);
       otype[idx] = s_axis[v];
       // The axis end's row: the next column's staged origin when that is the end vertex.
-      const int nb = (lc + 1 < kBlock && s_g0[lc + 1] == g1) ? lc + 1 : -1;
+      const int nb = (lc + 1 < kVcBlock && s_g0[lc + 1] == g1) ? lc + 1 : -1;
       float coef[kN][3];
 #pragma unroll
       for (int k = 0; k < kN; ++k) {
@@ -759,10 +763,10 @@ extern "C" int mt_create_triangle_vertices_colors_forward(
              "mt_create_triangle_vertices_colors_forward: degree %d needs (deg+1)^2 <= %d coefficients",
              sh_degree, num_sh);
   if (num_columns == 0 || S == 0 || num_triangle_vertices == 0) return MT_OK;
-  const dim3 grid(mt::blocks_for(num_columns, mt::kBlock));
+  const dim3 grid(mt::blocks_for(num_columns, mt::kVcBlock));
   hipStream_t st = static_cast<hipStream_t>(stream);
 #define MT_VCOL(D)                                                                              \
-  mt::create_vertices_colors_kernel<D><<<grid, mt::kBlock, 0, st>>>(                           \
+  mt::create_vertices_colors_kernel<D><<<grid, mt::kVcBlock, 0, st>>>(                           \
       grid_vertices, grid_sdf, grid_sh, num_grid_vertices, lower_corners, num_columns,         \
       level_sets, S, vertex_offsets, num_triangle_vertices, camera_positions, B, out_vertices, \
       out_sdf, out_types, out_colors, overflow)
