@@ -143,13 +143,17 @@ int mt_create_triangle_vertices_backward(
  * mt_create_triangle_vertices_forward and out_colors f32[B, num_triangle_vertices, 3] for
  * the B cameras centred at camera_positions f32[B, 3]; each triangle vertex's SH
  * coefficients are interpolated from grid_sh in registers and never stored. Colours are
- * bit-identical to mt_create_triangle_vertices_forward + mt_evaluate_sh_forward. */
+ * bit-identical to mt_create_triangle_vertices_forward + mt_evaluate_sh_forward. parts: 1 =
+ * the vertices only, 2 = the colours only (the vertices are recomputed, not written), 3 =
+ * both; split, the colours can run on another stream beside the work that needs only the
+ * vertices. */
 int mt_create_triangle_vertices_colors_forward(
     const float* grid_vertices, const float* grid_sdf, const float* grid_sh,
     int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree, const int32_t* lower_corners,
     int64_t num_columns, const float* level_sets, int32_t S, const int32_t* vertex_offsets,
     int64_t num_triangle_vertices, const float* camera_positions, int32_t B, float* out_vertices,
-    float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, mt_stream_t stream);
+    float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, int32_t parts,
+    mt_stream_t stream);
 
 /* Workspace for mt_create_triangle_vertices_colors_backward. */
 size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(int64_t num_grid_vertices,

@@ -94,7 +94,7 @@ SIGNATURES = {
     "mt_create_triangle_vertices_colors_forward": (
         c_i32,
         [c_p, c_p, c_p, c_i64, c_i32, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_i32, c_p,
-         c_p, c_p, c_p, c_p, c_p],
+         c_p, c_p, c_p, c_p, c_i32, c_p],
     ),
     "mt_create_triangle_vertices_colors_backward_workspace_bytes": (c_sz, [c_i64, c_i64, c_i32]),
     "mt_create_triangle_vertices_colors_backward": (

@@ -258,7 +258,7 @@ class BatchedTrainingStep:
                  self.sh_degree, v.lower_corners.data_ptr(), self.n_col,
                  self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V,
                  self.campos[c0:].data_ptr(), n, self.tv.data_ptr(), self.td.data_ptr(),
-                 self.ttype.data_ptr(), self.colors.data_ptr(), self.overflow.data_ptr(), st)
+                 self.ttype.data_ptr(), self.colors.data_ptr(), self.overflow.data_ptr(), 3, st)
             # Faces (first group only) and the group's projection both need only the vertices:
             # without per-launch timing they run side by side on two streams.
             overlap = gi == 0 and hook is None

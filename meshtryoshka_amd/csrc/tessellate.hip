@@ -321,9 +321,11 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
     const float* gv, const float* gsdf, const float* gsh, int64_t ng, const int32_t* lower,
     int64_t n, const float* level_sets, int S, const int32_t* offsets, int64_t nt,
     const float* cams, int B, float* ov, float* od, uint8_t* otype, float* colors,
-    int32_t* overflow) {
+    int32_t* overflow, int parts) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 3 * kN + 1;  // padded: consecutive rows start in different banks
+  const bool write_vertices = (parts & 1) != 0;
+  const bool write_colors = (parts & 2) != 0 && B > 0;
   __shared__ float s_sh[kVcBlock * kRow];
   __shared__ int32_t s_g0[kVcBlock];
   __shared__ float s_c0[kVcBlock];  // the column's origin SDF
@@ -336,8 +338,8 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
   const int64_t c_last = (c_first + kVcBlock < n ? c_first + kVcBlock : n) - 1;
   const bool valid = c < n;
   ColumnSdf col{};
-  if (valid) {
-    col = load_column(gsdf, lower, n, c);
+  if (valid) col = load_column(gsdf, lower, n, c);
+  if (valid && write_colors) {
 #pragma unroll
     for (int k = 0; k < kN; ++k) {
       const float* plane = gsh + int64_t(k) * ng * 3;
@@ -376,12 +378,15 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
       const float x0 = gv[3 * g0 + 0], y0 = gv[3 * g0 + 1], z0 = gv[3 * g0 + 2];
       const float x1 = gv[3 * g1 + 0], y1 = gv[3 * g1 + 1], z1 = gv[3 * g1 + 2];
       const float3 p = make_float3(x0 + t * (x1 - x0), y0 + t * (y1 - y0), z0 + t * (z1 - z0));
-      ov[3 * idx + 0] = p.x;
-      ov[3 * idx + 1] = p.y;
-      ov[3 * idx + 2] = p.z;
-      const float d0 = s_c0[lc];
-      od[idx] = d0 + t * (s_d1[v] - d0);
-      otype[idx] = s_axis[v];
+      if (write_vertices) {
+        ov[3 * idx + 0] = p.x;
+        ov[3 * idx + 1] = p.y;
+        ov[3 * idx + 2] = p.z;
+        const float d0 = s_c0[lc];
+        od[idx] = d0 + t * (s_d1[v] - d0);
+        otype[idx] = s_axis[v];
+      }
+      if (!write_colors) continue;
       // The axis end's row: the next column's staged origin when that is the end vertex.
       const int nb = (lc + 1 < kVcBlock && s_g0[lc + 1] == g1) ? lc + 1 : -1;
       float coef[kN][3];
@@ -756,7 +761,9 @@ extern "C" int mt_create_triangle_vertices_colors_forward(
     int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree, const int32_t* lower_corners,
     int64_t num_columns, const float* level_sets, int32_t S, const int32_t* vertex_offsets,
     int64_t num_triangle_vertices, const float* camera_positions, int32_t B, float* out_vertices,
-    float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, mt_stream_t stream) {
+    float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, int32_t parts,
+    mt_stream_t stream) {
+  MT_REQUIRE(parts >= 1 && parts <= 3, "mt_create_triangle_vertices_colors_forward: parts %d", parts);
   MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0 && B >= 0,
              "mt_create_triangle_vertices_colors_forward: negative size");
   MT_REQUIRE(sh_degree >= 0 && sh_degree <= 3 && (sh_degree + 1) * (sh_degree + 1) <= num_sh,
@@ -769,7 +776,7 @@ extern "C" int mt_create_triangle_vertices_colors_forward(
   mt::create_vertices_colors_kernel<D><<<grid, mt::kVcBlock, 0, st>>>(                           \
       grid_vertices, grid_sdf, grid_sh, num_grid_vertices, lower_corners, num_columns,         \
       level_sets, S, vertex_offsets, num_triangle_vertices, camera_positions, B, out_vertices, \
-      out_sdf, out_types, out_colors, overflow)
+      out_sdf, out_types, out_colors, overflow, parts)
   switch (sh_degree) {
     case 0: MT_VCOL(0); break;
     case 1: MT_VCOL(1); break;

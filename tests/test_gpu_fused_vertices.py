@@ -71,10 +71,27 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
     _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(), sdf.data_ptr(),
                  sh.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(), n, ls.data_ptr(), s,
                  off.data_ptr(), V, campos.data_ptr(), b, tv2.data_ptr(), td2.data_ptr(),
-                 tt2.data_ptr(), col2.data_ptr(), ov.data_ptr(), _stream())
+                 tt2.data_ptr(), col2.data_ptr(), ov.data_ptr(), 3, _stream())
     assert int(ov) == 0
     assert torch.equal(tv, tv2) and torch.equal(td, td2) and torch.equal(tt, tt2)
     assert torch.equal(col, col2)
+    # Split in two calls (vertices, then colours): the same outputs, and each call leaves the
+    # other's outputs untouched.
+    tv3, td3 = torch.full((V, 3), 7.0, **f32), torch.full((V,), 7.0, **f32)
+    tt3 = torch.full((V,), 7, dtype=torch.uint8, device=dev)
+    col3 = torch.full((b, V, 3), 7.0, **f32)
+    _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(), sdf.data_ptr(),
+                 sh.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(), n, ls.data_ptr(), s,
+                 off.data_ptr(), V, campos.data_ptr(), b, tv3.data_ptr(), td3.data_ptr(),
+                 tt3.data_ptr(), col3.data_ptr(), ov.data_ptr(), 1, _stream())
+    assert torch.equal(tv3, tv) and torch.equal(td3, td) and torch.equal(tt3, tt)
+    assert bool((col3 == 7.0).all())
+    tv4 = torch.full((V, 3), 7.0, **f32)
+    _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(), sdf.data_ptr(),
+                 sh.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(), n, ls.data_ptr(), s,
+                 off.data_ptr(), V, campos.data_ptr(), b, tv4.data_ptr(), td3.data_ptr(),
+                 tt3.data_ptr(), col3.data_ptr(), ov.data_ptr(), 2, _stream())
+    assert torch.equal(col3, col) and bool((tv4 == 7.0).all())
 
     # Adjoint: random upstream gradients for SDF and colours.
     gen = torch.Generator("cpu").manual_seed(11)
