@@ -170,14 +170,17 @@ size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(int64_t num_g
  * share goes straight to the grid gradient, the axis ends' shares through the workspace
  * to the grid vertices they end at. This needs each grid vertex to be the origin / +axis
  * end of at most one column, as extract_voxels guarantees; otherwise nothing is written
- * and bit 1 (value 2) of *status is set. num_sh in {1, 4, 9, 16}. */
+ * and bit 1 (value 2) of *status is set. num_sh in {1, 4, 9, 16}. vertex_touched
+ * (uint8[V] or NULL): the flags mt_intersect_and_composite_backward set; vertices without
+ * the flag have zero gradients and are skipped. */
 int mt_create_triangle_vertices_colors_backward(
     const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree,
     const int32_t* lower_corners, int64_t num_columns, const float* level_sets, int32_t S,
     const int32_t* vertex_offsets, int64_t num_triangle_vertices, const float* vertices,
     const float* colors, const float* camera_positions, int32_t B, const float* grad_sdf,
     const float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
-    size_t workspace_bytes, int32_t accumulate, int32_t* status, mt_stream_t stream);
+    size_t workspace_bytes, int32_t accumulate, const uint8_t* vertex_touched, int32_t* status,
+    mt_stream_t stream);
 
 /* face_counts int32[S * num_voxels] (3 x triangles) and cell_codes uint8[S, num_voxels].
  * Replaces count_triangle_faces.slang:13-50 (.py:28-63). upper_corners
@@ -356,6 +359,9 @@ typedef struct mt_composite_args {
   const int64_t* tile_base;
   const int64_t* pixel_base;
   int64_t total_tiles;
+  /* Optional (may be NULL), backward only: uint8[V], set to 1 for every vertex that
+   * receives a gradient (the caller zeroes it); lets the colour adjoint skip the others. */
+  uint8_t* vertex_touched;
 } mt_composite_args;
 
 /* out f32[B, 4, H, W] (rgb, transmittance). Replaces intersect_and_composite_forward

@@ -63,6 +63,7 @@ class CompositeArgs(ctypes.Structure):
         ("tile_base", c_p),
         ("pixel_base", c_p),
         ("total_tiles", c_i64),
+        ("vertex_touched", c_p),
     ]
 
 
@@ -100,7 +101,7 @@ SIGNATURES = {
     "mt_create_triangle_vertices_colors_backward": (
         c_i32,
         [c_p, c_i64, c_i32, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_i32, c_p,
-         c_p, c_p, c_p, c_p, c_sz, c_i32, c_p, c_p],
+         c_p, c_p, c_p, c_p, c_sz, c_i32, c_p, c_p, c_p],
     ),
     "mt_count_triangle_faces": (c_i32, [c_p, c_p, c_i64, c_p, c_i64, c_p, c_i32, c_p, c_p, c_p]),
     "mt_create_triangle_faces": (

@@ -157,6 +157,9 @@ class BatchedTrainingStep:
         self.loss = torch.zeros(1, **f32)
         self.colors_grad = torch.zeros_like(self.colors)
         self.td_grad = torch.zeros_like(self.td)
+        # Vertices that received a gradient in the current group (set by the composite
+        # backward, read by the colour adjoint to skip the rest).
+        self.touched = torch.zeros(V, dtype=torch.uint8, device=dev)
         self.sharp_grad = torch.zeros(1, **f32)
         self.grid_sdf_grad = torch.zeros(self.n_grid, **f32)
         self.grid_sh_grad = torch.zeros((num_sh, self.n_grid, 3), **f32)
@@ -248,6 +251,7 @@ class BatchedTrainingStep:
             cam_dims = g["cam_dims"].data_ptr() if table else None
             self.colors_grad[:n].zero_()
             self.td_grad.zero_()
+            self.touched.zero_()
             self.ranges[: g["tiles"]].zero_()
             if hook is not None:
                 hook("zero_fill")
@@ -310,6 +314,7 @@ class BatchedTrainingStep:
                  target[3 * p0 :].data_ptr(), n, h, w,
                  g["pixel_base"].data_ptr() if table else None, g["pixels"], self.num_pixels,
                  self.loss.data_ptr(), grad.data_ptr(), st)
+            args.vertex_touched = self.touched.data_ptr()
             call("mt_intersect_and_composite_backward", args, out.data_ptr(), grad.data_ptr(),
                  self.colors_grad.data_ptr(), self.td_grad.data_ptr(), self.sharp_grad.data_ptr(),
                  self.quirk, st)
@@ -322,6 +327,7 @@ class BatchedTrainingStep:
                  self.td_grad.data_ptr(), self.colors_grad.data_ptr(),
                  self.grid_sdf_grad.data_ptr(), self.grid_sh_grad.data_ptr(),
                  self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(), int(gi > 0),
+                 self.touched.data_ptr(),
                  self.overflow.data_ptr(), st)
 
     # ------------------------------------------------------------------------------

@@ -426,6 +426,7 @@ __device__ __forceinline__ void scatter_sample(const mt_composite_args& a, int c
   float* cgrad = colors_grad + int64_t(cam) * a.num_vertices * 3;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
+    if (a.vertex_touched) a.vertex_touched[v[q]] = 1;
     atomicAdd(cgrad + 3 * v[q] + 0, wgt[q] * dr);
     atomicAdd(cgrad + 3 * v[q] + 1, wgt[q] * dg);
     atomicAdd(cgrad + 3 * v[q] + 2, wgt[q] * db);
@@ -458,7 +459,8 @@ __device__ __forceinline__ void scatter_sample_lds(float* acc, float bu, float b
 // the next chunk (the sums start at zero and only hit triangles' sums become nonzero).
 __device__ __forceinline__ void flush_chunk_grads(const ChunkLds& L, float* acc, int count,
                                                   const int32_t* faces, int64_t num_vertices,
-                                                  int cam, float* colors_grad, float* sdf_grad) {
+                                                  int cam, float* colors_grad, float* sdf_grad,
+                                                  uint8_t* touched) {
   float* cgrad = colors_grad + int64_t(cam) * num_vertices * 3;
   for (int j = threadIdx.x; j < count; j += blockDim.x) {
     float* g = acc + kGradSlots * j;
@@ -474,6 +476,7 @@ __device__ __forceinline__ void flush_chunk_grads(const ChunkLds& L, float* acc,
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const int64_t v = faces[3 * face + q];
+      if (touched) touched[v] = 1;
 #pragma unroll
       for (int c = 0; c < 3; ++c)
         if (e[4 * q + c] != 0.0f) atomicAdd(cgrad + 3 * v + c, e[4 * q + c]);
@@ -622,7 +625,8 @@ __global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a
       L.live[tc.pix] = active ? 1 : 0;
     }
     const bool more = __syncthreads_or(active);  // also: every LDS sum of the chunk is in
-    flush_chunk_grads(L, gacc, count, a.faces, a.num_vertices, tc.cam, colors_grad, sdf_grad);
+    flush_chunk_grads(L, gacc, count, a.faces, a.num_vertices, tc.cam, colors_grad, sdf_grad,
+                      a.vertex_touched);
     if (!more) break;
   }
   if (pending) {  // its triangle's chunk has been flushed

@@ -423,7 +423,7 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
     const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
     int S, const int32_t* offsets, int64_t nt, const float* tv, const float* colors,
     const float* cams, int B, const float* g_d, const float* g_col, float* gg_d, float* gg_sh,
-    float* part, const int32_t* maps, bool accumulate) {
+    float* part, const int32_t* maps, bool accumulate, const uint8_t* touched) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 1 + 3 * kN;
   if (maps[4 * ng] != 0) return;
@@ -446,6 +446,7 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
       int64_t here = offsets[s * n + c];
       for (int q = 0; q < a; ++q) here += crosses(col.d0, col.d[q], iso) ? 1 : 0;
       if (here >= nt) continue;
+      if (touched && !touched[here]) continue;  // no gradient reached this vertex
       const float t = edge_t(col.d0, col.d[a], iso);
       const float w0 = 1.0f - t;
       const float gd = g_d[here];
@@ -800,7 +801,8 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
     const int32_t* vertex_offsets, int64_t num_triangle_vertices, const float* vertices,
     const float* colors, const float* camera_positions, int32_t B, const float* grad_sdf,
     const float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
-    size_t workspace_bytes, int32_t accumulate, int32_t* status, mt_stream_t stream) {
+    size_t workspace_bytes, int32_t accumulate, const uint8_t* vertex_touched, int32_t* status,
+    mt_stream_t stream) {
   MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0 && B >= 0,
              "mt_create_triangle_vertices_colors_backward: negative size");
   MT_REQUIRE(num_sh == 1 || num_sh == 4 || num_sh == 9 || num_sh == 16,
@@ -834,7 +836,8 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
       mt::colors_adjoint_columns_kernel<N, D><<<cgrid, mt::kBlock, 0, st>>>(                     \
           grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S,                \
           vertex_offsets, num_triangle_vertices, vertices, colors, camera_positions, B, grad_sdf, \
-          grad_colors, grid_sdf_grad, grid_sh_grad, part, maps, accumulate != 0);                \
+          grad_colors, grid_sdf_grad, grid_sh_grad, part, maps, accumulate != 0,                \
+          vertex_touched);                                                                      \
     mt::colors_adjoint_ends_kernel<N, D><<<ggrid, mt::kBlock, 0, st>>>(                          \
         num_grid_vertices, num_columns, part, grid_sdf_grad, grid_sh_grad, maps, accumulate != 0, \
         status);                                                                                 \
