@@ -274,11 +274,14 @@ int mt_generate_keys_from_geometry(const float* xy, const float* depths, int64_t
 
 /* The three calls above in one kernel (batched training step and serving; no single
  * counterpart in the reference: fuses bound_triangles.slang:144-192, the tile-count cumsum
- * of generate_keys.py:40-47 and generate_keys.slang:4-68). Blocks of 256 faces of one
- * camera take their index from a ticket, count their keys, find their first output
+ * of generate_keys.py:40-47 and generate_keys.slang:4-68). Blocks of faces (for all B
+ * cameras) take their index from a ticket, count their keys, find their first output
  * position by decoupled look-back over the earlier blocks, and write their keys, so each
- * face's tile rectangle is computed once. Keys and values equal mt_key_block_counts +
- * scan + mt_generate_keys_from_geometry with the same arguments; *n_keys (device) = the
+ * face's tile rectangle is computed once. For every camera, the keys and values are those
+ * of mt_key_block_counts + scan + mt_generate_keys_from_geometry with the same arguments,
+ * in the same order; the cameras' sequences are interleaved block by block, which a
+ * stable sort cannot tell apart because keys of different cameras differ in the camera
+ * field (B > 1 therefore needs 2^camera_bits >= B; B <= 1024). *n_keys (device) = the
  * total key count (also when it exceeds key_capacity, which sets *overflow). Workspace:
  * mt_generate_keys_single_pass_workspace_bytes(F, B). */
 size_t mt_generate_keys_single_pass_workspace_bytes(int64_t F, int32_t B);

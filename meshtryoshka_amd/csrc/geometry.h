@@ -131,7 +131,29 @@ struct Rect {
   int x, y, w, h;
 };
 
+// 1 / x in double for the rectangle's vertices and distance scale: on the GPU the hardware
+// reciprocal refined by one Newton step (relative error near 2^-50, far inside the 1e-8
+// relative slack), on the host the division. Callers keep |x| within [2^-1000, 2^1000].
+MT_HD double rcp_refined(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r = __builtin_amdgcn_rcp(x);
+  return __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+#else
+  return 1.0 / x;
+#endif
+}
+
+MT_HD bool rcp_range(double x) {
+  const double a = __builtin_fabs(x);
+  return a >= 0x1p-1000 && a <= 0x1p1000;
+}
+
 MT_HD Rect candidate_rect(const TriTerms& t, int tile_x0, int tile_y0, int tw, int th) {
+#if defined(__clang__)
+  // Only double arithmetic below (the float inputs are converted exactly): fused
+  // multiply-adds round less than the separate operations the bound allows for.
+#pragma clang fp contract(fast)
+#endif
   Rect full{0, 0, tw, th};
   const float vals[14] = {t.p1x, t.p1y, t.p2x, t.p2y, t.e12x, t.e12y, t.e13x,
                           t.e13y, t.d1313, t.d1213, t.d1212, t.d1223, t.d1323, t.D};
@@ -212,10 +234,11 @@ MT_HD Rect candidate_rect(const TriTerms& t, int tile_x0, int tile_y0, int tw, i
   const bool pos = c01 > 0.0 && c12 > 0.0 && c20 > 0.0;
   const bool neg = c01 < 0.0 && c12 < 0.0 && c20 < 0.0;
   if (!spread || !(pos || neg)) return full;
+  if (!rcp_range(c01) || !rcp_range(c12) || !rcp_range(c20)) return full;
 
   // Vertices: line i meets line j at ((c_j ny_i - c_i ny_j), (c_i nx_j - c_j nx_i)) / cross.
-  // One division per pair of coordinates (the extra rounding is far inside the slack).
-  const double i01 = 1.0 / c01, i12 = 1.0 / c12, i20 = 1.0 / c20;
+  // One reciprocal per pair of coordinates (the extra rounding is far inside the slack).
+  const double i01 = rcp_refined(c01), i12 = rcp_refined(c12), i20 = rcp_refined(c20);
   const double q0x = (c[1] * ny[0] - c[0] * ny[1]) * i01, q0y = (c[0] * nx[1] - c[1] * nx[0]) * i01;
   const double q1x = (c[2] * ny[1] - c[1] * ny[2]) * i12, q1y = (c[1] * nx[2] - c[2] * nx[1]) * i12;
   const double q2x = (c[0] * ny[2] - c[2] * ny[0]) * i20, q2y = (c[2] * nx[0] - c[0] * nx[2]) * i20;
@@ -225,13 +248,15 @@ MT_HD Rect candidate_rect(const TriTerms& t, int tile_x0, int tile_y0, int tw, i
   xmax = xmax > q2x ? xmax : q2x;
   ymin = ymin < q2y ? ymin : q2y;
   ymax = ymax > q2y ? ymax : q2y;
-  // |c_i| / |n_i| <= |c_i| / max(|nx_i|, |ny_i|): distance scale of each line.
+  // |c_i| / |n_i| <= |c_i| / max(|nx_i|, |ny_i|): distance scale of each line (over-
+  // estimated by 1.001 against the reciprocal's rounding).
   double dist = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const double m = __builtin_fabs(nx[i]) > __builtin_fabs(ny[i]) ? __builtin_fabs(nx[i])
                                                                     : __builtin_fabs(ny[i]);
-    dist += __builtin_fabs(c[i]) / m;
+    if (!rcp_range(m)) return full;
+    dist += 1.001 * __builtin_fabs(c[i]) * rcp_refined(m);
   }
   const double qmag = __builtin_fabs(xmin) + __builtin_fabs(xmax) + __builtin_fabs(ymin) +
                       __builtin_fabs(ymax);

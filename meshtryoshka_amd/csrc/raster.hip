@@ -58,25 +58,64 @@ struct TileRect {
   int y0, y1, x0, x1;
 };
 
-__device__ __forceinline__ TileRect face_tile_rect(const float* xy, const float* depths, int64_t o,
-                                                   int64_t ia, int64_t ib, int64_t ic, int th,
-                                                   int tw, int rmin, int cmin, int rmax, int cmax,
-                                                   float near_plane) {
-  const float2* pxy = reinterpret_cast<const float2*>(xy);
-  const float2 a = pxy[o + ia], bb = pxy[o + ib], c = pxy[o + ic];
-  const float da = depths[o + ia], db = depths[o + ib], dc = depths[o + ic];
+// Tile-side arithmetic: truncating integer quotients and correctly rounded float quotients
+// by the tile height (rows) and width (columns). With power-of-two sides (kPow2) they are
+// shifts and exact scalings: x * 2^-k is the same correctly rounded number as x / 2^k
+// (f32 denormals are preserved), and the truncated quotient of a negative x is minus the
+// shifted |x|.
+template <bool kPow2>
+struct TileSize {
+  int th, tw, lh, lw;
+  float ih, iw;  // 2^-lh, 2^-lw
+
+  __device__ __forceinline__ TileSize(int h, int w) : th(h), tw(w), lh(0), lw(0), ih(1.f), iw(1.f) {
+    if (kPow2) {
+      lh = __builtin_ctz(unsigned(h));
+      lw = __builtin_ctz(unsigned(w));
+      ih = __int_as_float((127 - lh) << 23);
+      iw = __int_as_float((127 - lw) << 23);
+    }
+  }
+  __device__ __forceinline__ static int shift_div(int x, int l) {
+    return x >= 0 ? (x >> l) : -static_cast<int>((-static_cast<int64_t>(x)) >> l);
+  }
+  __device__ __forceinline__ int row(int x) const { return kPow2 ? shift_div(x, lh) : x / th; }
+  __device__ __forceinline__ int col(int x) const { return kPow2 ? shift_div(x, lw) : x / tw; }
+  __device__ __forceinline__ float rowf(float x) const { return kPow2 ? x * ih : x / float(th); }
+  __device__ __forceinline__ float colf(float x) const { return kPow2 ? x * iw : x / float(tw); }
+};
+
+__host__ __forceinline__ bool pow2_tiles(int th, int tw) {
+  return th > 0 && tw > 0 && (th & (th - 1)) == 0 && (tw & (tw - 1)) == 0;
+}
+
+template <bool kPow2>
+__device__ __forceinline__ TileRect corner_tile_rect(float2 a, float2 bb, float2 c, float da,
+                                                     float db, float dc, const TileSize<kPow2>& ts,
+                                                     int rmin, int cmin, int rmax, int cmax,
+                                                     float near_plane) {
   TileRect r{0, 0, 0, 0};
   const float check = (bb.x - a.x) * (c.y - a.y) - (bb.y - a.y) * (c.x - a.x);
   const float min_depth = fminf(fminf(da, db), dc);
   if (!(check < 0.0f) && !(min_depth < near_plane)) {
     const float ylo = fminf(fminf(a.y, bb.y), c.y), yhi = fmaxf(fmaxf(a.y, bb.y), c.y);
     const float xlo = fminf(fminf(a.x, bb.x), c.x), xhi = fmaxf(fmaxf(a.x, bb.x), c.x);
-    r.y0 = clampi(f2i_rz(ylo) / th, rmin, rmax);
-    r.y1 = clampi(f2i_rz(ceilf(yhi / float(th))), rmin, rmax);
-    r.x0 = clampi(f2i_rz(xlo) / tw, cmin, cmax);
-    r.x1 = clampi(f2i_rz(ceilf(xhi / float(tw))), cmin, cmax);
+    r.y0 = clampi(ts.row(f2i_rz(ylo)), rmin, rmax);
+    r.y1 = clampi(f2i_rz(ceilf(ts.rowf(yhi))), rmin, rmax);
+    r.x0 = clampi(ts.col(f2i_rz(xlo)), cmin, cmax);
+    r.x1 = clampi(f2i_rz(ceilf(ts.colf(xhi))), cmin, cmax);
   }
   return r;
+}
+
+template <bool kPow2>
+__device__ __forceinline__ TileRect face_tile_rect(const float* xy, const float* depths, int64_t o,
+                                                   int64_t ia, int64_t ib, int64_t ic,
+                                                   const TileSize<kPow2>& ts, int rmin, int cmin,
+                                                   int rmax, int cmax, float near_plane) {
+  const float2* pxy = reinterpret_cast<const float2*>(xy);
+  return corner_tile_rect(pxy[o + ia], pxy[o + ib], pxy[o + ic], depths[o + ia], depths[o + ib],
+                          depths[o + ic], ts, rmin, cmin, rmax, cmax, near_plane);
 }
 
 // Narrows a face's tile rectangle to the tiles where some pixel can pass the barycentric
@@ -88,9 +127,11 @@ __device__ __forceinline__ TileRect face_tile_rect(const float* xy, const float*
 // faces cover no pixel centre at all and get no key.
 constexpr int kExactPixels = 16;
 
+template <bool kPow2>
 __device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, int64_t o, int64_t ia,
-                                               int64_t ib, int64_t ic, int th, int tw) {
+                                               int64_t ib, int64_t ic, const TileSize<kPow2>& ts) {
   if (r.y1 <= r.y0 || r.x1 <= r.x0) return r;
+  const int th = ts.th, tw = ts.tw;
   const float2* pxy = reinterpret_cast<const float2*>(xy);
   const float2 a = pxy[o + ia], bb = pxy[o + ib], c = pxy[o + ic];
   const TriTerms t = tri_terms(a.x, a.y, bb.x, bb.y, c.x, c.y);
@@ -101,10 +142,10 @@ __device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, int6
   n.x0 = n.x1 = r.x0;
   if (q.w <= 0 || q.h <= 0) return n;
   if (q.w * q.h > kExactPixels) {
-    n.y0 = r.y0 + q.y / th;
-    n.y1 = r.y0 + (q.y + q.h - 1) / th + 1;
-    n.x0 = r.x0 + q.x / tw;
-    n.x1 = r.x0 + (q.x + q.w - 1) / tw + 1;
+    n.y0 = r.y0 + ts.row(q.y);
+    n.y1 = r.y0 + ts.row(q.y + q.h - 1) + 1;
+    n.x0 = r.x0 + ts.col(q.x);
+    n.x1 = r.x0 + ts.col(q.x + q.w - 1) + 1;
     return n;
   }
   const float ad = fabsf(t.D);
@@ -125,10 +166,10 @@ __device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, int6
     }
   }
   if (ax1 < 0) return n;
-  n.y0 = ay0 / th;
-  n.y1 = ay1 / th + 1;
-  n.x0 = ax0 / tw;
-  n.x1 = ax1 / tw + 1;
+  n.y0 = ts.row(ay0);
+  n.y1 = ts.row(ay1) + 1;
+  n.x0 = ts.col(ax0);
+  n.x1 = ts.col(ax1) + 1;
   return n;
 }
 
@@ -164,8 +205,8 @@ __global__ void __launch_bounds__(kBlock) bound_kernel(const float* xy, const fl
     return;
   }
   for (int b = 0; b < B; ++b) {
-    const TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic, th, tw, rmin, cmin,
-                                      rmax, cmax, near_plane);
+    const TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic,
+                                      TileSize<false>(th, tw), rmin, cmin, rmax, cmax, near_plane);
     const int64_t q = int64_t(b) * F + f;
     reinterpret_cast<int2*>(tmin)[q] = make_int2(r.y0, r.x0);
     reinterpret_cast<int2*>(tmax)[q] = make_int2(r.y1, r.x1);
@@ -282,9 +323,10 @@ __global__ void __launch_bounds__(kBlock) key_block_counts_kernel(
     if (keyed) {
       const int r0 = cam_dims ? 0 : rmin, c0 = cam_dims ? 0 : cmin;
       const int r1 = cam_dims ? cam_dims[4 * b + 2] : rmax, c1 = cam_dims ? cam_dims[4 * b + 3] : cmax;
-      TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic, th, tw, r0, c0, r1, c1,
+      const TileSize<false> ts(th, tw);
+      TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic, ts, r0, c0, r1, c1,
                                   near_plane);
-      if (cull) r = cull_tiles(r, xy, int64_t(b) * V, ia, ib, ic, th, tw);
+      if (cull) r = cull_tiles(r, xy, int64_t(b) * V, ia, ib, ic, ts);
       count = (r.y1 - r.y0) * (r.x1 - r.x0);
     }
     const int32_t w = wave_sum(count);
@@ -324,9 +366,10 @@ __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
   int64_t ia = 0, ib = 0, ic = 0;
   if (f < F && keyed_face(faces, f, V, live_x3, ia, ib, ic)) {
     const int64_t o = int64_t(b) * V;
-    TileRect r = face_tile_rect(xy, depths, o, ia, ib, ic, th, tw, rmin, cmin, rmax, cmax,
+    const TileSize<false> ts(th, tw);
+    TileRect r = face_tile_rect(xy, depths, o, ia, ib, ic, ts, rmin, cmin, rmax, cmax,
                                 near_plane);
-    if (cull) r = cull_tiles(r, xy, o, ia, ib, ic, th, tw);
+    if (cull) r = cull_tiles(r, xy, o, ia, ib, ic, ts);
     count = (r.y1 - r.y0) * (r.x1 - r.x0);
     s_r0[threadIdx.x] = r.y0;
     s_c0[threadIdx.x] = r.x0;
@@ -378,24 +421,30 @@ __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
   }
 }
 
-// Single pass: the counts, their scan and the keys in one kernel. Blocks take their
-// (camera, block of kPassFaces faces) index from a ticket and
-//   1. bound each face (culling, tile-aligned box) and list the faces whose box needs the
-//      pixel-level narrowing (cull_tiles) in LDS,
-//   2. narrow the listed faces with every thread busy (back-facing, clipped and keyless
-//      faces take no thread),
-//   3. scan the key counts in face order, find the block's first output position by
-//      decoupled look-back over the earlier blocks' published counts, and write the keys.
-// Each face's tile rectangle is computed once; a block covers kPassFaces = 4 x 256 faces
-// so that the look-back is paid once per 1024 faces. The last block also stores the total
-// key count.
+// Single pass: the counts, their scan and the keys in one kernel. A block takes its camera
+// slice (kPassFaces / 2^lfpb cameras) and face block (2^lfpb faces; pairs are camera-major
+// inside the block) from a ticket, slice-major, and
+//   1. bounds each pair (culling, tile-aligned box) and lists the pairs whose box needs
+//      the pixel-level narrowing (cull_tiles) in LDS,
+//   2. narrows the listed pairs with every thread busy (back-facing, clipped and keyless
+//      pairs take no thread),
+//   3. scans the key counts in pair order, finds the block's first output position by
+//      decoupled look-back over the earlier blocks' published counts, and writes the keys.
+// A face's corner indices are read from memory once per camera slice. The keys of one
+// camera come out in (face, tile) order, the order of mt_generate_keys_from_geometry; the
+// cameras of a slice are interleaved block by block. Keys of different cameras differ in their camera
+// field, so a stable sort orders both arrangements identically. The last block also stores
+// the total key count.
 constexpr int kPassPer = 4;
 constexpr int kPassFaces = kPassPer * kBlock;
 
-__global__ void __launch_bounds__(kBlock) keys_single_pass_kernel(
+template <bool kPow2>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+keys_single_pass_kernel(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
-    const int32_t* live_x3, const int32_t* boundaries, int S, int B, int th, int tw, int rmin,
-    int cmin, int rmax, int cmax, float near_plane, const int32_t* cam_dims, int cull,
+    const int32_t* live_x3, const int32_t* boundaries, int S, int B, int lfpb, int nslices,
+    int th, int tw,
+    int rmin, int cmin, int rmax, int cmax, float near_plane, const int32_t* cam_dims, int cull,
     int grid_cols, KeyFields kf, int64_t* keys, int32_t* values, int64_t kcap, int64_t* n_keys,
     int32_t* overflow, ScanState st) {
   __shared__ int32_t s_prefix[kPassFaces];  // key count, then its exclusive scan
@@ -411,18 +460,19 @@ __global__ void __launch_bounds__(kBlock) keys_single_pass_kernel(
   }
   __syncthreads();
   const int64_t q = s_q;
-  const int64_t nblk = (F + kPassFaces - 1) / kPassFaces;
-  const int b = static_cast<int>(q / nblk);
-  const int64_t fb = q % nblk;
-  if (cam_dims) {  // this camera's own tile grid
-    rmin = 0;
-    cmin = 0;
-    rmax = cam_dims[4 * b + 2];
-    cmax = cam_dims[4 * b + 3];
-    grid_cols = cmax;
-  }
-  const int64_t o = int64_t(b) * V;
-  auto key_hi = [&](int64_t f, int64_t ia, int64_t ib, int64_t ic) {
+  const int fpb = 1 << lfpb;
+  const int64_t nfb = (F + fpb - 1) >> lfpb;  // face blocks
+  const int64_t nblk = nfb * nslices;
+  const int64_t f0 = (q % nfb) << lfpb;  // first face of the block
+  const int bfirst = int(q / nfb) * (kPassFaces >> lfpb);  // first camera of the slice
+  const TileSize<kPow2> ts(th, tw);
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / kWave;
+  const int64_t live = live_x3 != nullptr ? int64_t(*live_x3) / 3 : F;
+  const int64_t fend = live < F ? live : F;  // faces below it are keyed if their corners are valid
+  // Pair l: camera bfirst + (l >> lfpb), face f0 + (l & (fpb - 1)).
+  auto key_hi = [&](int b, int64_t f, int64_t ia, int64_t ib, int64_t ic) {
+    const int64_t o = int64_t(b) * V;
     uint32_t shell = 0;
     while (shell < uint32_t(S) && !(f < boundaries[shell])) ++shell;
     const float mean_depth = ((depths[o + ia] + depths[o + ib]) + depths[o + ic]) / 3.f;
@@ -431,11 +481,20 @@ __global__ void __launch_bounds__(kBlock) keys_single_pass_kernel(
   // 1. Bounds.
   for (int i = 0; i < kPassPer; ++i) {
     const int l = i * kBlock + threadIdx.x;
-    const int64_t f = fb * kPassFaces + l;
+    const int b = bfirst + (l >> lfpb);
+    const int64_t f = f0 + (l & (fpb - 1));
     int32_t count = 0;
+    bool listed = false;
     int64_t ia = 0, ib = 0, ic = 0;
-    if (f < F && keyed_face(faces, f, V, live_x3, ia, ib, ic)) {
-      const TileRect r = face_tile_rect(xy, depths, o, ia, ib, ic, th, tw, rmin, cmin, rmax, cmax,
+    if (b < B && f < fend && keyed_face(faces, f, V, nullptr, ia, ib, ic)) {
+      int r0 = rmin, c0 = cmin, r1 = rmax, c1 = cmax;
+      if (cam_dims) {  // this camera's own tile grid
+        r0 = 0;
+        c0 = 0;
+        r1 = cam_dims[4 * b + 2];
+        c1 = cam_dims[4 * b + 3];
+      }
+      const TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic, ts, r0, c0, r1, c1,
                                         near_plane);
       s_r0[l] = r.y0;
       s_c0[l] = r.x0;
@@ -443,38 +502,43 @@ __global__ void __launch_bounds__(kBlock) keys_single_pass_kernel(
       s_h[l] = r.y1 - r.y0;
       count = (r.y1 - r.y0) * (r.x1 - r.x0);
       if (count > 0 && cull) {
-        s_work[atomicAdd(&s_nwork, 1)] = static_cast<int16_t>(l);
+        listed = true;
         count = 0;  // set by step 2
       } else if (count > 0) {
-        s_key_hi[l] = key_hi(f, ia, ib, ic);
+        s_key_hi[l] = key_hi(b, f, ia, ib, ic);
       }
     }
     s_prefix[l] = count;
+    // Append the listed pairs: one LDS atomic per wave.
+    const uint64_t m = __ballot(listed);
+    int at = 0;
+    if (lane == 0 && m != 0) at = atomicAdd(&s_nwork, __popcll(m));
+    at = __shfl(at, 0, kWave);
+    if (listed) s_work[at + __popcll(m & lanemask_lt())] = static_cast<int16_t>(l);
   }
   __syncthreads();
-  // 2. Pixel-level narrowing of the listed faces.
+  // 2. Pixel-level narrowing of the listed pairs.
   const int nwork = s_nwork;
   for (int k = threadIdx.x; k < nwork; k += kBlock) {
     const int l = s_work[k];
-    const int64_t f = fb * kPassFaces + l;
+    const int b = bfirst + (l >> lfpb);
+    const int64_t f = f0 + (l & (fpb - 1));
     const int64_t ia = faces[3 * f + 0], ib = faces[3 * f + 1], ic = faces[3 * f + 2];
     TileRect r;
     r.y0 = s_r0[l];
     r.x0 = s_c0[l];
     r.y1 = r.y0 + s_h[l];
     r.x1 = r.x0 + s_w[l];
-    r = cull_tiles(r, xy, o, ia, ib, ic, th, tw);
+    r = cull_tiles(r, xy, int64_t(b) * V, ia, ib, ic, ts);
     const int32_t count = (r.y1 - r.y0) * (r.x1 - r.x0);
     s_r0[l] = r.y0;
     s_c0[l] = r.x0;
     s_w[l] = r.x1 - r.x0;
     s_prefix[l] = count;
-    if (count > 0) s_key_hi[l] = key_hi(f, ia, ib, ic);
+    if (count > 0) s_key_hi[l] = key_hi(b, f, ia, ib, ic);
   }
   __syncthreads();
-  // 3. Scan in face order, look-back, keys.
-  const int lane = __lane_id();
-  const int wave = threadIdx.x / kWave;
+  // 3. Scan in pair order, look-back, keys.
   int32_t carry = 0;
   for (int i = 0; i < kPassPer; ++i) {
     const int l = i * kBlock + threadIdx.x;
@@ -504,31 +568,48 @@ __global__ void __launch_bounds__(kBlock) keys_single_pass_kernel(
     }
     if (threadIdx.x == 0) {
       s_block_start = prefix;
-      if (q == int64_t(B) * nblk - 1 && n_keys) *n_keys = prefix + int64_t(total);
+      if (q == nblk - 1 && n_keys) *n_keys = prefix + int64_t(total);
     }
   }
   __syncthreads();
   const int64_t base = s_block_start;
   const uint64_t tile_mask = field_mask(kf.tile_bits);
   for (int32_t j = threadIdx.x; j < total; j += kBlock) {
-    int lo = 0, hi = kPassFaces - 1;  // last face whose prefix is <= j (it has keys)
+    int lo = 0, hi = kPassFaces - 1;  // last pair whose prefix is <= j (it has keys)
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (s_prefix[mid] <= j) lo = mid; else hi = mid - 1;
     }
     const int32_t local = j - s_prefix[lo];
     const int32_t w = s_w[lo];
+    const int b = bfirst + (lo >> lfpb);
+    const int cols = cam_dims ? cam_dims[4 * b + 3] : grid_cols;
     const uint32_t row = uint32_t(s_r0[lo] + local / w);
     const uint32_t col = uint32_t(s_c0[lo] + local % w);
-    const uint32_t tile = row * uint32_t(grid_cols) + col;
+    const uint32_t tile = row * uint32_t(cols) + col;
     const uint64_t key = s_key_hi[lo] | ((uint64_t(tile) & tile_mask) << kf.tile_offset);
     if (base + j >= kcap) {
       if (overflow) atomicOr(overflow, 1);
       continue;
     }
     keys[base + j] = static_cast<int64_t>(key);
-    values[base + j] = static_cast<int32_t>(fb * kPassFaces + lo);
+    values[base + j] = static_cast<int32_t>(f0 + (lo & (fpb - 1)));
   }
+}
+
+// log2 of the faces per block of the single-pass kernel: the largest power of two with
+// (faces x B) <= kPassFaces, at least kBlock faces (then the cameras come in slices of
+// kPassFaces / kBlock).
+__host__ inline int single_pass_lfpb(int B) {
+  int l = 0;
+  while ((int64_t(2) << l) * (B > 0 ? B : 1) <= kPassFaces) ++l;
+  constexpr int kMin = __builtin_ctz(kBlock);
+  return l > kMin ? l : kMin;
+}
+
+__host__ inline int single_pass_slices(int B, int lfpb) {
+  const int per = kPassFaces >> lfpb;
+  return (B + per - 1) / per;
 }
 
 __global__ void fill_i64_kernel(int64_t* p, int64_t n, int64_t value) {
@@ -536,8 +617,14 @@ __global__ void fill_i64_kernel(int64_t* p, int64_t n, int64_t value) {
   if (i < n) p[i] = value;
 }
 
+int64_t single_pass_blocks(int64_t F, int B) {
+  if (B <= 0) return 0;
+  const int lfpb = single_pass_lfpb(B);
+  return ((F + (int64_t(1) << lfpb) - 1) >> lfpb) * single_pass_slices(B, lfpb);
+}
+
 size_t single_pass_state_bytes(int64_t F, int B) {
-  const int64_t blocks = int64_t(B) * ((F + kPassFaces - 1) / kPassFaces);
+  const int64_t blocks = single_pass_blocks(F, B);
   return 256 + (static_cast<size_t>(blocks) * 8 + 255) / 256 * 256;
 }
 
@@ -765,7 +852,11 @@ extern "C" int mt_generate_keys_single_pass(
     return MT_ERR_WORKSPACE_TOO_SMALL;
   }
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const int64_t blocks = int64_t(B) * ((F + mt::kPassFaces - 1) / mt::kPassFaces);
+  MT_REQUIRE(B <= mt::kPassFaces, "mt_generate_keys_single_pass: more than %d cameras", mt::kPassFaces);
+  MT_REQUIRE(B <= 1 || (layout.camera_bits < 31 && (int64_t(1) << layout.camera_bits) >= B),
+             "mt_generate_keys_single_pass: the key layout's camera field cannot tell %d cameras apart", B);
+  const int64_t blocks = mt::single_pass_blocks(F, B);
+  const int lfpb = mt::single_pass_lfpb(B);
   mt::ScanState state;
   state.ticket = static_cast<uint32_t*>(workspace);
   state.words = reinterpret_cast<uint64_t*>(static_cast<char*>(workspace) + 256);
@@ -776,10 +867,12 @@ extern "C" int mt_generate_keys_single_pass(
     return mt::check_launch("mt_generate_keys_single_pass");
   }
   MT_REQUIRE(blocks < (int64_t(1) << 31), "mt_generate_keys_single_pass: too many face blocks");
-  mt::keys_single_pass_kernel<<<static_cast<unsigned>(blocks), mt::kBlock, 0, st>>>(
-      xy, depths, V, faces, F, num_faces_x3, shell_face_boundaries, S, B, tile_h, tile_w, row_min,
-      col_min, row_max, col_max, near_plane, cam_dims, cull, grid_cols, mt::to_fields(layout),
-      keys, values, key_capacity, n_keys, overflow, state);
+  auto kernel = mt::pow2_tiles(tile_h, tile_w) ? mt::keys_single_pass_kernel<true>
+                                                : mt::keys_single_pass_kernel<false>;
+  kernel<<<static_cast<unsigned>(blocks), mt::kBlock, 0, st>>>(
+      xy, depths, V, faces, F, num_faces_x3, shell_face_boundaries, S, B, lfpb,
+      mt::single_pass_slices(B, lfpb), tile_h, tile_w, row_min, col_min, row_max, col_max, near_plane, cam_dims, cull, grid_cols,
+      mt::to_fields(layout), keys, values, key_capacity, n_keys, overflow, state);
   return mt::check_launch("mt_generate_keys_single_pass");
 }
 

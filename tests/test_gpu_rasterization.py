@@ -316,9 +316,10 @@ def test_tile_shape_invariance(scene):
 @pytest.mark.parametrize("cull", [0, 1])
 def test_single_pass_keys_match_three_pass(cull):
     """mt_generate_keys_single_pass (counts, look-back scan and keys in one kernel) writes
-    the same keys, values and total as mt_key_block_counts + scan +
-    mt_generate_keys_from_geometry, with and without culling, for several cameras
-    (thousands of face blocks chained by look-back)."""
+    the same total as mt_key_block_counts + scan + mt_generate_keys_from_geometry and, per
+    camera, the same keys and values in the same order (cameras are interleaved by face
+    block), so that the stable sort gives identical results; with and without culling, for
+    several cameras (thousands of face blocks chained by look-back)."""
     dev = torch.device("cuda")
     spec = workloads.sphere_field(seed=4, num_spheres=10, resolution=192, num_cameras=5,
                                   image_shape=(150, 230), device=dev)
@@ -367,7 +368,14 @@ def test_single_pass_keys_match_three_pass(cull):
     torch.cuda.synchronize()
     assert int(over) == 0
     assert int(n_keys) == k
-    assert torch.equal(keys2[:k], keys) and torch.equal(vals2[:k], vals)
+    cam, cam2 = (keys >> 58) & 63, (keys2[:k] >> 58) & 63
+    assert int(cam.max()) == b - 1
+    for c in range(b):
+        assert torch.equal(keys2[:k][cam2 == c], keys[cam == c])
+        assert torch.equal(vals2[:k][cam2 == c], vals[cam == c])
+    # The stable sort of both arrangements is the same.
+    order, order2 = torch.sort(keys, stable=True).indices, torch.sort(keys2[:k], stable=True).indices
+    assert torch.equal(keys[order], keys2[:k][order2]) and torch.equal(vals[order], vals2[:k][order2])
     assert bool((keys2[k:] == -1).all())
     # Too small a capacity: the count is still exact, the overflow flag is raised and
     # nothing is written past the capacity.
@@ -379,5 +387,5 @@ def test_single_pass_keys_match_three_pass(cull):
                  over.data_ptr(), ws2.data_ptr(), ws2.numel(), st)
     torch.cuda.synchronize()
     assert int(over) == 1 and int(n_keys) == k
-    assert torch.equal(small[: k // 2], keys[: k // 2])
+    assert torch.equal(small[: k // 2], keys2[: k // 2])
     assert int(small[k // 2]) == -1
