@@ -171,14 +171,11 @@ MT_HD Rect candidate_rect(const TriTerms& t, int tile_x0, int tile_y0, int tw, i
   const double o1x = double(t.p1x) - tile_x0, o1y = double(t.p1y) - tile_y0;
   const double o2x = double(t.p2x) - tile_x0, o2y = double(t.p2y) - tile_y0;
   auto reach = [](double o, double lo, double hi) {
-    const double a = __builtin_fabs(lo - o), b = __builtin_fabs(hi - o);
-    return a > b ? a : b;
+    return __builtin_fmax(__builtin_fabs(lo - o), __builtin_fabs(hi - o));
   };
   const double r1x = reach(o1x, xlo, xhi), r1y = reach(o1y, ylo, yhi);
   const double r2x = reach(o2x, xlo, xhi), r2y = reach(o2y, ylo, yhi);
-  double rmax = r1x > r1y ? r1x : r1y;
-  rmax = rmax > r2x ? rmax : r2x;
-  rmax = rmax > r2y ? rmax : r2y;
+  const double rmax = __builtin_fmax(__builtin_fmax(r1x, r1y), __builtin_fmax(r2x, r2y));
   const double e12x = t.e12x, e12y = t.e12y, e13x = t.e13x, e13y = t.e13y;
   const double esum = __builtin_fabs(e12x) + __builtin_fabs(e12y) + __builtin_fabs(e13x) +
                       __builtin_fabs(e13y);
@@ -202,22 +199,19 @@ MT_HD Rect candidate_rect(const TriTerms& t, int tile_x0, int tile_y0, int tw, i
   plane(1, t.d1313, t.d1213, e12x, e12y, e13x, e13y, o1x, o1y, r1x, r1y);  // v
   plane(2, t.d1212, t.d1213, e13x, e13y, e12x, e12y, o1x, o1y, r1x, r1y);  // w
 
-  // Whole-tile rejection: |numerator| <= M_i over the tile (affine, so its extreme is at a
-  // corner of the pixel-centre range, plus the rounding allowance). If the coordinates
-  // cannot sum to 0.999 anywhere (degenerate or distant triangles), no pixel can hit.
+  // Whole-tile rejection: |numerator| <= M_i over the tile (affine, so its largest |value|
+  // over the pixel-centre box is |value at the centre| + |nx| hx + |ny| hy with hx, hy the
+  // half extents, plus the rounding allowance). If the coordinates cannot sum to 0.999
+  // anywhere (degenerate or distant triangles), no pixel can hit.
+  const double cx = 0.5 * (xlo + xhi), cy = 0.5 * (ylo + yhi);
+  const double hx = 0.5 * (xhi - xlo), hy = 0.5 * (yhi - ylo);
   double msum = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const double delta = c[i] + (nx[i] * (i == 0 ? o2x : o1x) + ny[i] * (i == 0 ? o2y : o1y));
-    const double l0 = nx[i] * xlo + ny[i] * ylo + c[i] - delta;
-    const double l1 = nx[i] * xhi + ny[i] * ylo + c[i] - delta;
-    const double l2 = nx[i] * xlo + ny[i] * yhi + c[i] - delta;
-    const double l3 = nx[i] * xhi + ny[i] * yhi + c[i] - delta;
-    double m = __builtin_fabs(l0);
-    m = m > __builtin_fabs(l1) ? m : __builtin_fabs(l1);
-    m = m > __builtin_fabs(l2) ? m : __builtin_fabs(l2);
-    m = m > __builtin_fabs(l3) ? m : __builtin_fabs(l3);
-    msum += m + 2.0 * __builtin_fabs(delta);
+    const double centre = nx[i] * cx + ny[i] * cy + (c[i] - delta);
+    msum += __builtin_fabs(centre) + __builtin_fabs(nx[i]) * hx + __builtin_fabs(ny[i]) * hy +
+            2.0 * __builtin_fabs(delta);
   }
   if (msum < 0.998 * __builtin_fabs(double(t.D))) return Rect{0, 0, 0, 0};
 
@@ -242,33 +236,30 @@ MT_HD Rect candidate_rect(const TriTerms& t, int tile_x0, int tile_y0, int tw, i
   const double q0x = (c[1] * ny[0] - c[0] * ny[1]) * i01, q0y = (c[0] * nx[1] - c[1] * nx[0]) * i01;
   const double q1x = (c[2] * ny[1] - c[1] * ny[2]) * i12, q1y = (c[1] * nx[2] - c[2] * nx[1]) * i12;
   const double q2x = (c[0] * ny[2] - c[2] * ny[0]) * i20, q2y = (c[2] * nx[0] - c[0] * nx[2]) * i20;
-  double xmin = q0x < q1x ? q0x : q1x, xmax = q0x > q1x ? q0x : q1x;
-  double ymin = q0y < q1y ? q0y : q1y, ymax = q0y > q1y ? q0y : q1y;
-  xmin = xmin < q2x ? xmin : q2x;
-  xmax = xmax > q2x ? xmax : q2x;
-  ymin = ymin < q2y ? ymin : q2y;
-  ymax = ymax > q2y ? ymax : q2y;
+  // A NaN anywhere makes the sum NaN (fmin / fmax below would drop it).
+  const double qsum = ((q0x + q0y) + (q1x + q1y)) + (q2x + q2y);
+  if (!(qsum == qsum)) return full;
+  const double xmin = __builtin_fmin(__builtin_fmin(q0x, q1x), q2x);
+  const double xmax = __builtin_fmax(__builtin_fmax(q0x, q1x), q2x);
+  const double ymin = __builtin_fmin(__builtin_fmin(q0y, q1y), q2y);
+  const double ymax = __builtin_fmax(__builtin_fmax(q0y, q1y), q2y);
   // |c_i| / |n_i| <= |c_i| / max(|nx_i|, |ny_i|): distance scale of each line (over-
   // estimated by 1.001 against the reciprocal's rounding).
   double dist = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    const double m = __builtin_fabs(nx[i]) > __builtin_fabs(ny[i]) ? __builtin_fabs(nx[i])
-                                                                    : __builtin_fabs(ny[i]);
+    const double m = __builtin_fmax(__builtin_fabs(nx[i]), __builtin_fabs(ny[i]));
     if (!rcp_range(m)) return full;
     dist += 1.001 * __builtin_fabs(c[i]) * rcp_refined(m);
   }
   const double qmag = __builtin_fabs(xmin) + __builtin_fabs(xmax) + __builtin_fabs(ymin) +
                       __builtin_fabs(ymax);
   const double slack = 1e-3 + 1e-8 * (dist + qmag);
-  if (!(xmin == xmin) || !(ymin == ymin) || !(xmax == xmax) || !(ymax == ymax)) return full;
   // Pixel i has its centre at i + 0.5.
-  double ix0 = __builtin_ceil(xmin - slack - 0.5), ix1 = __builtin_floor(xmax + slack - 0.5);
-  double iy0 = __builtin_ceil(ymin - slack - 0.5), iy1 = __builtin_floor(ymax + slack - 0.5);
-  ix0 = ix0 < 0.0 ? 0.0 : ix0;
-  iy0 = iy0 < 0.0 ? 0.0 : iy0;
-  ix1 = ix1 > double(tw - 1) ? double(tw - 1) : ix1;
-  iy1 = iy1 > double(th - 1) ? double(th - 1) : iy1;
+  const double ix0 = __builtin_fmax(__builtin_ceil(xmin - slack - 0.5), 0.0);
+  const double ix1 = __builtin_fmin(__builtin_floor(xmax + slack - 0.5), double(tw - 1));
+  const double iy0 = __builtin_fmax(__builtin_ceil(ymin - slack - 0.5), 0.0);
+  const double iy1 = __builtin_fmin(__builtin_floor(ymax + slack - 0.5), double(th - 1));
   Rect r;
   if (ix1 < ix0 || iy1 < iy0) {
     r.x = 0;

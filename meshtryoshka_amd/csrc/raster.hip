@@ -478,15 +478,46 @@ keys_single_pass_kernel(
     const float mean_depth = ((depths[o + ia] + depths[o + ib]) + depths[o + ic]) / 3.f;
     return pack_key(kf, uint32_t(b), shell, 0u, mean_depth);
   };
-  // 1. Bounds.
+  // 1. Bounds (loads of all kPassPer pairs first).
+  const float2* pxy = reinterpret_cast<const float2*>(xy);
+  int32_t ca[kPassPer], cb[kPassPer], cc[kPassPer];
+#pragma unroll
+  for (int i = 0; i < kPassPer; ++i) {
+    const int l = i * kBlock + threadIdx.x;
+    const int b = bfirst + (l >> lfpb);
+    const int64_t f = f0 + (l & (fpb - 1));
+    ca[i] = cb[i] = cc[i] = -1;
+    if (b < B && f < fend) {
+      ca[i] = faces[3 * f + 0];
+      cb[i] = faces[3 * f + 1];
+      cc[i] = faces[3 * f + 2];
+    }
+  }
+  float2 pa[kPassPer], pb[kPassPer], pc[kPassPer];
+  float za[kPassPer], zb[kPassPer], zc[kPassPer];
+  bool valid[kPassPer];
+#pragma unroll
+  for (int i = 0; i < kPassPer; ++i) {
+    const int l = i * kBlock + threadIdx.x;
+    const int64_t o = int64_t(bfirst + (l >> lfpb)) * V;
+    valid[i] = ca[i] >= 0 && cb[i] >= 0 && cc[i] >= 0 && ca[i] < V && cb[i] < V && cc[i] < V;
+    if (valid[i]) {
+      pa[i] = pxy[o + ca[i]];
+      pb[i] = pxy[o + cb[i]];
+      pc[i] = pxy[o + cc[i]];
+      za[i] = depths[o + ca[i]];
+      zb[i] = depths[o + cb[i]];
+      zc[i] = depths[o + cc[i]];
+    }
+  }
+#pragma unroll
   for (int i = 0; i < kPassPer; ++i) {
     const int l = i * kBlock + threadIdx.x;
     const int b = bfirst + (l >> lfpb);
     const int64_t f = f0 + (l & (fpb - 1));
     int32_t count = 0;
     bool listed = false;
-    int64_t ia = 0, ib = 0, ic = 0;
-    if (b < B && f < fend && keyed_face(faces, f, V, nullptr, ia, ib, ic)) {
+    if (valid[i]) {
       int r0 = rmin, c0 = cmin, r1 = rmax, c1 = cmax;
       if (cam_dims) {  // this camera's own tile grid
         r0 = 0;
@@ -494,8 +525,8 @@ keys_single_pass_kernel(
         r1 = cam_dims[4 * b + 2];
         c1 = cam_dims[4 * b + 3];
       }
-      const TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic, ts, r0, c0, r1, c1,
-                                        near_plane);
+      const TileRect r = corner_tile_rect(pa[i], pb[i], pc[i], za[i], zb[i], zc[i], ts, r0, c0,
+                                          r1, c1, near_plane);
       s_r0[l] = r.y0;
       s_c0[l] = r.x0;
       s_w[l] = r.x1 - r.x0;
@@ -505,7 +536,7 @@ keys_single_pass_kernel(
         listed = true;
         count = 0;  // set by step 2
       } else if (count > 0) {
-        s_key_hi[l] = key_hi(b, f, ia, ib, ic);
+        s_key_hi[l] = key_hi(b, f, ca[i], cb[i], cc[i]);
       }
     }
     s_prefix[l] = count;
