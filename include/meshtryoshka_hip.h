@@ -281,7 +281,9 @@ int mt_generate_keys_from_geometry(const float* xy, const float* depths, int64_t
  * of mt_key_block_counts + scan + mt_generate_keys_from_geometry with the same arguments,
  * in the same order; the cameras' sequences are interleaved block by block, which a
  * stable sort cannot tell apart because keys of different cameras differ in the camera
- * field (B > 1 therefore needs 2^camera_bits >= B; B <= 1024). *n_keys (device) = the
+ * field (B > 1 therefore needs 2^camera_bits >= B). Tile grids: at most 65535 rows and
+ * columns (row_min, col_min >= 0 and row_max, col_max <= 65535; the same for the cam_dims
+ * grids, which the caller checks). *n_keys (device) = the
  * total key count (also when it exceeds key_capacity, which sets *overflow). Workspace:
  * mt_generate_keys_single_pass_workspace_bytes(F, B). */
 size_t mt_generate_keys_single_pass_workspace_bytes(int64_t F, int32_t B);

@@ -110,6 +110,8 @@ class BatchedTrainingStep:
 
         # Camera table: image and tile-grid sizes; first pixel of every camera.
         dims = [(h, w, -(-h // th), -(-w // tw)) for h, w in self.shapes]
+        if max(max(d[2], d[3]) for d in dims) > 65535:
+            raise ValueError("tile grids are limited to 65535 rows and columns")
         first = [0]
         for h, w in self.shapes:
             first.append(first[-1] + h * w)

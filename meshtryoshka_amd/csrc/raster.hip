@@ -127,27 +127,24 @@ __device__ __forceinline__ TileRect face_tile_rect(const float* xy, const float*
 // faces cover no pixel centre at all and get no key.
 constexpr int kExactPixels = 16;
 
+// The pixel tests of cull_tiles (below) for a candidate rectangle q of at most kExactPixels
+// pixels. The face's corners and the triangle's terms are loaded and computed again (behind
+// a compiler memory barrier), so that they do not stay in registers across the candidate
+// rectangle.
 template <bool kPow2>
-__device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, int64_t o, int64_t ia,
-                                               int64_t ib, int64_t ic, const TileSize<kPow2>& ts) {
-  if (r.y1 <= r.y0 || r.x1 <= r.x0) return r;
+__device__ __forceinline__ TileRect exact_tiles(TileRect r, Rect q, const float* xy,
+                                                const int32_t* faces, int64_t f, int64_t o,
+                                                const TileSize<kPow2>& ts) {
   const int th = ts.th, tw = ts.tw;
-  const float2* pxy = reinterpret_cast<const float2*>(xy);
-  const float2 a = pxy[o + ia], bb = pxy[o + ib], c = pxy[o + ic];
-  const TriTerms t = tri_terms(a.x, a.y, bb.x, bb.y, c.x, c.y);
-  const int px0 = r.x0 * tw, py0 = r.y0 * th;
-  const Rect q = candidate_rect(t, px0, py0, (r.x1 - r.x0) * tw, (r.y1 - r.y0) * th);
   TileRect n;
   n.y0 = n.y1 = r.y0;
   n.x0 = n.x1 = r.x0;
-  if (q.w <= 0 || q.h <= 0) return n;
-  if (q.w * q.h > kExactPixels) {
-    n.y0 = r.y0 + ts.row(q.y);
-    n.y1 = r.y0 + ts.row(q.y + q.h - 1) + 1;
-    n.x0 = r.x0 + ts.col(q.x);
-    n.x1 = r.x0 + ts.col(q.x + q.w - 1) + 1;
-    return n;
-  }
+  const int px0 = r.x0 * tw, py0 = r.y0 * th;
+  asm volatile("" ::: "memory");
+  const float2* pxy = reinterpret_cast<const float2*>(xy);
+  const float2 a = pxy[o + faces[3 * f + 0]], bb = pxy[o + faces[3 * f + 1]];
+  const float2 c = pxy[o + faces[3 * f + 2]];
+  const TriTerms t = tri_terms(a.x, a.y, bb.x, bb.y, c.x, c.y);
   const float ad = fabsf(t.D);
   const float rd = (ad >= 0x1p-60f && ad <= 0x1p60f) ? __builtin_amdgcn_rcpf(t.D) : 0.0f;
   int ax0 = 0x7fffffff, ay0 = 0x7fffffff, ax1 = -1, ay1 = -1;  // accepted pixels' box
@@ -171,6 +168,32 @@ __device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, int6
   n.x0 = ts.col(ax0);
   n.x1 = ts.col(ax1) + 1;
   return n;
+}
+
+// Face f (whose three corners the caller has checked), camera offset o into xy.
+template <bool kPow2>
+__device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, const int32_t* faces,
+                                               int64_t f, int64_t o, const TileSize<kPow2>& ts) {
+  if (r.y1 <= r.y0 || r.x1 <= r.x0) return r;
+  const int th = ts.th, tw = ts.tw;
+  const float2* pxy = reinterpret_cast<const float2*>(xy);
+  const float2 a = pxy[o + faces[3 * f + 0]], bb = pxy[o + faces[3 * f + 1]];
+  const float2 c = pxy[o + faces[3 * f + 2]];
+  const TriTerms t = tri_terms(a.x, a.y, bb.x, bb.y, c.x, c.y);
+  const int px0 = r.x0 * tw, py0 = r.y0 * th;
+  const Rect q = candidate_rect(t, px0, py0, (r.x1 - r.x0) * tw, (r.y1 - r.y0) * th);
+  TileRect n;
+  n.y0 = n.y1 = r.y0;
+  n.x0 = n.x1 = r.x0;
+  if (q.w <= 0 || q.h <= 0) return n;
+  if (q.w * q.h > kExactPixels) {
+    n.y0 = r.y0 + ts.row(q.y);
+    n.y1 = r.y0 + ts.row(q.y + q.h - 1) + 1;
+    n.x0 = r.x0 + ts.col(q.x);
+    n.x1 = r.x0 + ts.col(q.x + q.w - 1) + 1;
+    return n;
+  }
+  return exact_tiles(r, q, xy, faces, f, o, ts);
 }
 
 // Face whose keys are generated: live (below the device-side face count) and with all
@@ -326,7 +349,7 @@ __global__ void __launch_bounds__(kBlock) key_block_counts_kernel(
       const TileSize<false> ts(th, tw);
       TileRect r = face_tile_rect(xy, depths, int64_t(b) * V, ia, ib, ic, ts, r0, c0, r1, c1,
                                   near_plane);
-      if (cull) r = cull_tiles(r, xy, int64_t(b) * V, ia, ib, ic, ts);
+      if (cull) r = cull_tiles(r, xy, faces, f, int64_t(b) * V, ts);
       count = (r.y1 - r.y0) * (r.x1 - r.x0);
     }
     const int32_t w = wave_sum(count);
@@ -369,7 +392,7 @@ __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
     const TileSize<false> ts(th, tw);
     TileRect r = face_tile_rect(xy, depths, o, ia, ib, ic, ts, rmin, cmin, rmax, cmax,
                                 near_plane);
-    if (cull) r = cull_tiles(r, xy, o, ia, ib, ic, ts);
+    if (cull) r = cull_tiles(r, xy, faces, f, o, ts);
     count = (r.y1 - r.y0) * (r.x1 - r.x0);
     s_r0[threadIdx.x] = r.y0;
     s_c0[threadIdx.x] = r.x0;
@@ -439,7 +462,7 @@ constexpr int kPassPer = 4;
 constexpr int kPassFaces = kPassPer * kBlock;
 
 template <bool kPow2>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kPow2 ? 6 : 5)))
 keys_single_pass_kernel(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
     const int32_t* live_x3, const int32_t* boundaries, int S, int B, int lfpb, int nslices,
@@ -448,7 +471,9 @@ keys_single_pass_kernel(
     int grid_cols, KeyFields kf, int64_t* keys, int32_t* values, int64_t kcap, int64_t* n_keys,
     int32_t* overflow, ScanState st) {
   __shared__ int32_t s_prefix[kPassFaces];  // key count, then its exclusive scan
-  __shared__ int32_t s_r0[kPassFaces], s_c0[kPassFaces], s_w[kPassFaces], s_h[kPassFaces];
+  // Tile rectangles packed 16 + 16 bits: origin (row << 16 | column), extent (height << 16 |
+  // width); grids have at most 65535 tile rows and columns.
+  __shared__ uint32_t s_org[kPassFaces], s_ext[kPassFaces];
   __shared__ uint64_t s_key_hi[kPassFaces];
   __shared__ int16_t s_work[kPassFaces];
   __shared__ int32_t s_wave[kBlock / kWave];
@@ -478,74 +503,77 @@ keys_single_pass_kernel(
     const float mean_depth = ((depths[o + ia] + depths[o + ib]) + depths[o + ic]) / 3.f;
     return pack_key(kf, uint32_t(b), shell, 0u, mean_depth);
   };
-  // 1. Bounds (loads of all kPassPer pairs first).
+  // 1. Bounds, kHoist pairs at a time: their corner indices, then their corners, are all
+  // loaded before any is used, so that the loads overlap.
   const float2* pxy = reinterpret_cast<const float2*>(xy);
-  int32_t ca[kPassPer], cb[kPassPer], cc[kPassPer];
+  constexpr int kHoist = 2;
 #pragma unroll
-  for (int i = 0; i < kPassPer; ++i) {
-    const int l = i * kBlock + threadIdx.x;
-    const int b = bfirst + (l >> lfpb);
-    const int64_t f = f0 + (l & (fpb - 1));
-    ca[i] = cb[i] = cc[i] = -1;
-    if (b < B && f < fend) {
-      ca[i] = faces[3 * f + 0];
-      cb[i] = faces[3 * f + 1];
-      cc[i] = faces[3 * f + 2];
-    }
-  }
-  float2 pa[kPassPer], pb[kPassPer], pc[kPassPer];
-  float za[kPassPer], zb[kPassPer], zc[kPassPer];
-  bool valid[kPassPer];
+  for (int i0 = 0; i0 < kPassPer; i0 += kHoist) {
+    int32_t ca[kHoist], cb[kHoist], cc[kHoist];
 #pragma unroll
-  for (int i = 0; i < kPassPer; ++i) {
-    const int l = i * kBlock + threadIdx.x;
-    const int64_t o = int64_t(bfirst + (l >> lfpb)) * V;
-    valid[i] = ca[i] >= 0 && cb[i] >= 0 && cc[i] >= 0 && ca[i] < V && cb[i] < V && cc[i] < V;
-    if (valid[i]) {
-      pa[i] = pxy[o + ca[i]];
-      pb[i] = pxy[o + cb[i]];
-      pc[i] = pxy[o + cc[i]];
-      za[i] = depths[o + ca[i]];
-      zb[i] = depths[o + cb[i]];
-      zc[i] = depths[o + cc[i]];
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < kPassPer; ++i) {
-    const int l = i * kBlock + threadIdx.x;
-    const int b = bfirst + (l >> lfpb);
-    const int64_t f = f0 + (l & (fpb - 1));
-    int32_t count = 0;
-    bool listed = false;
-    if (valid[i]) {
-      int r0 = rmin, c0 = cmin, r1 = rmax, c1 = cmax;
-      if (cam_dims) {  // this camera's own tile grid
-        r0 = 0;
-        c0 = 0;
-        r1 = cam_dims[4 * b + 2];
-        c1 = cam_dims[4 * b + 3];
-      }
-      const TileRect r = corner_tile_rect(pa[i], pb[i], pc[i], za[i], zb[i], zc[i], ts, r0, c0,
-                                          r1, c1, near_plane);
-      s_r0[l] = r.y0;
-      s_c0[l] = r.x0;
-      s_w[l] = r.x1 - r.x0;
-      s_h[l] = r.y1 - r.y0;
-      count = (r.y1 - r.y0) * (r.x1 - r.x0);
-      if (count > 0 && cull) {
-        listed = true;
-        count = 0;  // set by step 2
-      } else if (count > 0) {
-        s_key_hi[l] = key_hi(b, f, ca[i], cb[i], cc[i]);
+    for (int h = 0; h < kHoist; ++h) {
+      const int l = (i0 + h) * kBlock + threadIdx.x;
+      const int b = bfirst + (l >> lfpb);
+      const int64_t f = f0 + (l & (fpb - 1));
+      ca[h] = cb[h] = cc[h] = -1;
+      if (b < B && f < fend) {
+        ca[h] = faces[3 * f + 0];
+        cb[h] = faces[3 * f + 1];
+        cc[h] = faces[3 * f + 2];
       }
     }
-    s_prefix[l] = count;
-    // Append the listed pairs: one LDS atomic per wave.
-    const uint64_t m = __ballot(listed);
-    int at = 0;
-    if (lane == 0 && m != 0) at = atomicAdd(&s_nwork, __popcll(m));
-    at = __shfl(at, 0, kWave);
-    if (listed) s_work[at + __popcll(m & lanemask_lt())] = static_cast<int16_t>(l);
+    float2 pa[kHoist], pb[kHoist], pc[kHoist];
+    float za[kHoist], zb[kHoist], zc[kHoist];
+    bool valid[kHoist];
+#pragma unroll
+    for (int h = 0; h < kHoist; ++h) {
+      const int l = (i0 + h) * kBlock + threadIdx.x;
+      const int64_t o = int64_t(bfirst + (l >> lfpb)) * V;
+      valid[h] = ca[h] >= 0 && cb[h] >= 0 && cc[h] >= 0 && ca[h] < V && cb[h] < V && cc[h] < V;
+      if (valid[h]) {
+        pa[h] = pxy[o + ca[h]];
+        pb[h] = pxy[o + cb[h]];
+        pc[h] = pxy[o + cc[h]];
+        za[h] = depths[o + ca[h]];
+        zb[h] = depths[o + cb[h]];
+        zc[h] = depths[o + cc[h]];
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < kHoist; ++h) {
+      const int l = (i0 + h) * kBlock + threadIdx.x;
+      const int b = bfirst + (l >> lfpb);
+      const int64_t f = f0 + (l & (fpb - 1));
+      int32_t count = 0;
+      bool listed = false;
+      if (valid[h]) {
+        int r0 = rmin, c0 = cmin, r1 = rmax, c1 = cmax;
+        if (cam_dims) {  // this camera's own tile grid
+          r0 = 0;
+          c0 = 0;
+          r1 = cam_dims[4 * b + 2];
+          c1 = cam_dims[4 * b + 3];
+        }
+        const TileRect r = corner_tile_rect(pa[h], pb[h], pc[h], za[h], zb[h], zc[h], ts, r0, c0,
+                                            r1, c1, near_plane);
+        s_org[l] = (uint32_t(r.y0) << 16) | uint32_t(r.x0);
+        s_ext[l] = (uint32_t(r.y1 - r.y0) << 16) | uint32_t(r.x1 - r.x0);
+        count = (r.y1 - r.y0) * (r.x1 - r.x0);
+        if (count > 0 && cull) {
+          listed = true;
+          count = 0;  // set by step 2
+        } else if (count > 0) {
+          s_key_hi[l] = key_hi(b, f, ca[h], cb[h], cc[h]);
+        }
+      }
+      s_prefix[l] = count;
+      // Append the listed pairs: one LDS atomic per wave.
+      const uint64_t m = __ballot(listed);
+      int at = 0;
+      if (lane == 0 && m != 0) at = atomicAdd(&s_nwork, __popcll(m));
+      at = __shfl(at, 0, kWave);
+      if (listed) s_work[at + __popcll(m & lanemask_lt())] = static_cast<int16_t>(l);
+    }
   }
   __syncthreads();
   // 2. Pixel-level narrowing of the listed pairs.
@@ -554,19 +582,17 @@ keys_single_pass_kernel(
     const int l = s_work[k];
     const int b = bfirst + (l >> lfpb);
     const int64_t f = f0 + (l & (fpb - 1));
-    const int64_t ia = faces[3 * f + 0], ib = faces[3 * f + 1], ic = faces[3 * f + 2];
     TileRect r;
-    r.y0 = s_r0[l];
-    r.x0 = s_c0[l];
-    r.y1 = r.y0 + s_h[l];
-    r.x1 = r.x0 + s_w[l];
-    r = cull_tiles(r, xy, int64_t(b) * V, ia, ib, ic, ts);
+    r.y0 = int(s_org[l] >> 16);
+    r.x0 = int(s_org[l] & 0xffffu);
+    r.y1 = r.y0 + int(s_ext[l] >> 16);
+    r.x1 = r.x0 + int(s_ext[l] & 0xffffu);
+    r = cull_tiles(r, xy, faces, f, int64_t(b) * V, ts);
     const int32_t count = (r.y1 - r.y0) * (r.x1 - r.x0);
-    s_r0[l] = r.y0;
-    s_c0[l] = r.x0;
-    s_w[l] = r.x1 - r.x0;
+    s_org[l] = (uint32_t(r.y0) << 16) | uint32_t(r.x0);
+    s_ext[l] = (uint32_t(r.y1 - r.y0) << 16) | uint32_t(r.x1 - r.x0);
     s_prefix[l] = count;
-    if (count > 0) s_key_hi[l] = key_hi(b, f, ia, ib, ic);
+    if (count > 0) s_key_hi[l] = key_hi(b, f, faces[3 * f + 0], faces[3 * f + 1], faces[3 * f + 2]);
   }
   __syncthreads();
   // 3. Scan in pair order, look-back, keys.
@@ -612,11 +638,11 @@ keys_single_pass_kernel(
       if (s_prefix[mid] <= j) lo = mid; else hi = mid - 1;
     }
     const int32_t local = j - s_prefix[lo];
-    const int32_t w = s_w[lo];
+    const int32_t w = int32_t(s_ext[lo] & 0xffffu);
     const int b = bfirst + (lo >> lfpb);
     const int cols = cam_dims ? cam_dims[4 * b + 3] : grid_cols;
-    const uint32_t row = uint32_t(s_r0[lo] + local / w);
-    const uint32_t col = uint32_t(s_c0[lo] + local % w);
+    const uint32_t row = (s_org[lo] >> 16) + uint32_t(local / w);
+    const uint32_t col = (s_org[lo] & 0xffffu) + uint32_t(local % w);
     const uint32_t tile = row * uint32_t(cols) + col;
     const uint64_t key = s_key_hi[lo] | ((uint64_t(tile) & tile_mask) << kf.tile_offset);
     if (base + j >= kcap) {
@@ -876,6 +902,9 @@ extern "C" int mt_generate_keys_single_pass(
   MT_REQUIRE(F >= 0 && B >= 0 && B <= 65535 && key_capacity >= 0 && tile_h > 0 && tile_w > 0,
              "mt_generate_keys_single_pass: bad sizes");
   MT_REQUIRE(n_keys != nullptr, "mt_generate_keys_single_pass: null key count");
+  MT_REQUIRE(cam_dims != nullptr ||
+                 (row_min >= 0 && col_min >= 0 && row_max <= 65535 && col_max <= 65535),
+             "mt_generate_keys_single_pass: the tile grid must lie in [0, 65535]^2");
   const size_t need = mt::single_pass_state_bytes(F, B);
   if (workspace == nullptr || workspace_bytes < need) {
     mt::set_error("mt_generate_keys_single_pass: workspace of %zu bytes is smaller than the %zu required",

@@ -135,6 +135,8 @@ class _Buffers:
 def _batch_plan(scene: ResidentScene, shapes, tile_shape, k_cap):
     th, tw = tile_shape
     dims = [(h, w, -(-h // th), -(-w // tw)) for h, w in shapes]
+    if max(max(d[2], d[3]) for d in dims) > 65535:
+        raise ValueError("tile grids are limited to 65535 rows and columns")
     tile_base, pixel_base = [0], [0]
     for h, w, r, c in dims:
         tile_base.append(tile_base[-1] + r * c)
