@@ -423,7 +423,7 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
     const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
     int S, const int32_t* offsets, int64_t nt, const float* tv, const float* colors,
     const float* cams, int B, const float* g_d, const float* g_col, float* gg_d, float* gg_sh,
-    float* part, const int32_t* maps, bool accumulate, const uint8_t* touched) {
+    float* part, uint8_t* part_live, const int32_t* maps, bool accumulate, const uint8_t* touched) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 1 + 3 * kN;
   if (maps[4 * ng] != 0) return;
@@ -434,8 +434,10 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
   float acc0[kN][3];
 #pragma unroll
   for (int k = 0; k < kN; ++k) acc0[k][0] = acc0[k][1] = acc0[k][2] = 0.f;
+  bool live0 = false;  // some vertex of the column takes part
   for (int a = 0; a < 3; ++a) {
     if (col.v[a] < 0) continue;
+    bool live = false;  // some vertex of this edge takes part
     float acce_d = 0.f;
     float acce[kN][3];
 #pragma unroll
@@ -447,6 +449,7 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
       for (int q = 0; q < a; ++q) here += crosses(col.d0, col.d[q], iso) ? 1 : 0;
       if (here >= nt) continue;
       if (touched && !touched[here]) continue;  // no gradient reached this vertex
+      live = true;
       const float t = edge_t(col.d0, col.d[a], iso);
       const float w0 = 1.0f - t;
       const float gd = g_d[here];
@@ -478,6 +481,10 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
         }
       }
     }
+    // Rows of edges without a contributing vertex are left unwritten (phase 2 skips them).
+    part_live[int64_t(a) * n + c] = live ? 1 : 0;
+    if (!live) continue;
+    live0 = true;
     float* pa = part + int64_t(a) * kRow * n;
     pa[c] = acce_d;
 #pragma unroll
@@ -486,6 +493,7 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
       for (int ch = 0; ch < 3; ++ch) pa[(1 + 3 * k + ch) * n + c] = acce[k][ch];
   }
   const int64_t g = col.v0;
+  if (accumulate && !live0) return;  // nothing to add
   gg_d[g] = accumulate ? gg_d[g] + acc0_d : acc0_d;
 #pragma unroll
   for (int k = 0; k < kNsh; ++k)
@@ -501,8 +509,8 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
 // columns that end at it; a grid vertex that is no column's origin starts from zero.
 template <int kNsh, int kDeg>
 __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
-    int64_t ng, int64_t n, const float* part, float* gg_d, float* gg_sh, const int32_t* maps,
-    bool accumulate, int32_t* status) {
+    int64_t ng, int64_t n, const float* part, const uint8_t* part_live, float* gg_d, float* gg_sh,
+    const int32_t* maps, bool accumulate, int32_t* status) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 1 + 3 * kN;
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -519,6 +527,7 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     ends[a] = maps[(a + 1) * ng + g];
+    if (ends[a] >= 0 && !part_live[int64_t(a) * n + ends[a]]) ends[a] = -1;  // a zero row
     any = any || ends[a] >= 0;
   }
   if (origin && !any) return;  // phase 1 wrote the final value
@@ -792,7 +801,8 @@ extern "C" size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(
     int64_t num_grid_vertices, int64_t num_columns, int32_t sh_degree) {
   const int64_t row = 1 + 3 * (int64_t(sh_degree) + 1) * (int64_t(sh_degree) + 1);
   const size_t maps = static_cast<size_t>(4 * num_grid_vertices + 1) * sizeof(int32_t);
-  return (maps + 255) / 256 * 256 + static_cast<size_t>(3 * row * num_columns) * sizeof(float);
+  const size_t part = static_cast<size_t>(3 * row * num_columns) * sizeof(float);
+  return (maps + 255) / 256 * 256 + (part + 255) / 256 * 256 + static_cast<size_t>(3 * num_columns);
 }
 
 extern "C" int mt_create_triangle_vertices_colors_backward(
@@ -824,6 +834,9 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
   const int64_t words = 4 * num_grid_vertices;
   float* part = reinterpret_cast<float*>(static_cast<char*>(workspace) +
                                          (static_cast<size_t>(words + 1) * 4 + 255) / 256 * 256);
+  const int64_t row = 1 + 3 * (int64_t(sh_degree) + 1) * (int64_t(sh_degree) + 1);
+  uint8_t* part_live = reinterpret_cast<uint8_t*>(part) +
+                       (static_cast<size_t>(3 * row * num_columns) * sizeof(float) + 255) / 256 * 256;
   mt::fill_maps_kernel<<<mt::blocks_for(words, 256), 256, 0, st>>>(maps, words);
   if (num_columns > 0)
     mt::build_maps_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
@@ -836,11 +849,11 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
       mt::colors_adjoint_columns_kernel<N, D><<<cgrid, mt::kBlock, 0, st>>>(                     \
           grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S,                \
           vertex_offsets, num_triangle_vertices, vertices, colors, camera_positions, B, grad_sdf, \
-          grad_colors, grid_sdf_grad, grid_sh_grad, part, maps, accumulate != 0,                \
+          grad_colors, grid_sdf_grad, grid_sh_grad, part, part_live, maps, accumulate != 0,     \
           vertex_touched);                                                                      \
     mt::colors_adjoint_ends_kernel<N, D><<<ggrid, mt::kBlock, 0, st>>>(                          \
-        num_grid_vertices, num_columns, part, grid_sdf_grad, grid_sh_grad, maps, accumulate != 0, \
-        status);                                                                                 \
+        num_grid_vertices, num_columns, part, part_live, grid_sdf_grad, grid_sh_grad, maps,       \
+        accumulate != 0, status);                                                                \
   } while (0)
   const int key = num_sh * 4 + sh_degree;
   switch (key) {
