@@ -412,6 +412,40 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
   }
 }
 
+// Phase 0, one thread per column: bit a of col_live[c] is set when some vertex of the
+// column's axis-a edge takes part (below the vertex count and, with touched flags, flagged);
+// the columns with any set bit are appended to list (one atomic per wave).
+__global__ void __launch_bounds__(kBlock) colors_adjoint_mask_kernel(
+    const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
+    int S, const int32_t* offsets, int64_t nt, const uint8_t* touched, const int32_t* maps,
+    uint8_t* col_live, int32_t* list, int32_t* count) {
+  if (maps[4 * ng] != 0) return;
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint32_t bits = 0;
+  if (c < n) {
+    const ColumnSdf col = load_column(gsdf, lower, n, c);
+    for (int s = 0; s < S; ++s) {
+      const float iso = level_sets[s];
+      int64_t here = offsets[s * n + c];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (!crosses(col.d0, col.d[a], iso)) continue;
+        if (col.v[a] >= 0 && here < nt && (touched == nullptr || touched[here])) bits |= 1u << a;
+        ++here;
+      }
+    }
+    col_live[c] = static_cast<uint8_t>(bits);
+  }
+  const uint64_t m = __ballot(bits != 0);
+  const int lane = __lane_id();
+  int at = 0;
+  if (lane == 0 && m != 0) at = atomicAdd(count, __popcll(m));
+  at = __shfl(at, 0, kWave);
+  if (bits != 0) list[at + __popcll(m & lanemask_lt())] = static_cast<int32_t>(c);
+}
+
+__global__ void zero_count_kernel(int32_t* count) { *count = 0; }
+
 // Phase 1, one thread per column: the column's crossings are visited once (axis by axis,
 // level sets in order); (1 - t) g goes to the column's origin, which no other column has
 // (written straight into the grid gradient), and t g to the axis end, summed per axis into
@@ -423,12 +457,14 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
     const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
     int S, const int32_t* offsets, int64_t nt, const float* tv, const float* colors,
     const float* cams, int B, const float* g_d, const float* g_col, float* gg_d, float* gg_sh,
-    float* part, uint8_t* part_live, const int32_t* maps, bool accumulate, const uint8_t* touched) {
+    float* part, const int32_t* list, const int32_t* count, const int32_t* maps, bool accumulate,
+    const uint8_t* touched) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 1 + 3 * kN;
   if (maps[4 * ng] != 0) return;
-  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= *count) return;
+  const int64_t c = list[i];  // a column with some contributing vertex
   const ColumnSdf col = load_column(gsdf, lower, n, c);
   float acc0_d = 0.f;
   float acc0[kN][3];
@@ -481,8 +517,7 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
         }
       }
     }
-    // Rows of edges without a contributing vertex are left unwritten (phase 2 skips them).
-    part_live[int64_t(a) * n + c] = live ? 1 : 0;
+    // Rows of edges without a contributing vertex (col_live bit clear) stay unwritten.
     if (!live) continue;
     live0 = true;
     float* pa = part + int64_t(a) * kRow * n;
@@ -509,7 +544,7 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
 // columns that end at it; a grid vertex that is no column's origin starts from zero.
 template <int kNsh, int kDeg>
 __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
-    int64_t ng, int64_t n, const float* part, const uint8_t* part_live, float* gg_d, float* gg_sh,
+    int64_t ng, int64_t n, const float* part, const uint8_t* col_live, float* gg_d, float* gg_sh,
     const int32_t* maps, bool accumulate, int32_t* status) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 1 + 3 * kN;
@@ -519,15 +554,16 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
     return;
   }
   if (g >= ng) return;
-  // Phase 1 wrote (or added to) every column origin; other grid vertices start from zero
-  // unless the call accumulates.
-  const bool origin = maps[g] >= 0 || accumulate;
+  // Phase 1 wrote (or added to) the origins of the live columns; other grid vertices start
+  // from zero unless the call accumulates.
+  const int32_t own = maps[g];
+  const bool origin = (own >= 0 && col_live[own] != 0) || accumulate;
   int32_t ends[3];
   bool any = false;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     ends[a] = maps[(a + 1) * ng + g];
-    if (ends[a] >= 0 && !part_live[int64_t(a) * n + ends[a]]) ends[a] = -1;  // a zero row
+    if (ends[a] >= 0 && !((col_live[ends[a]] >> a) & 1u)) ends[a] = -1;  // an unwritten zero row
     any = any || ends[a] >= 0;
   }
   if (origin && !any) return;  // phase 1 wrote the final value
@@ -802,7 +838,9 @@ extern "C" size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(
   const int64_t row = 1 + 3 * (int64_t(sh_degree) + 1) * (int64_t(sh_degree) + 1);
   const size_t maps = static_cast<size_t>(4 * num_grid_vertices + 1) * sizeof(int32_t);
   const size_t part = static_cast<size_t>(3 * row * num_columns) * sizeof(float);
-  return (maps + 255) / 256 * 256 + (part + 255) / 256 * 256 + static_cast<size_t>(3 * num_columns);
+  const size_t list = static_cast<size_t>(num_columns + 1) * sizeof(int32_t);
+  return (maps + 255) / 256 * 256 + (part + 255) / 256 * 256 + (list + 255) / 256 * 256 +
+         static_cast<size_t>(num_columns);
 }
 
 extern "C" int mt_create_triangle_vertices_colors_backward(
@@ -835,12 +873,21 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
   float* part = reinterpret_cast<float*>(static_cast<char*>(workspace) +
                                          (static_cast<size_t>(words + 1) * 4 + 255) / 256 * 256);
   const int64_t row = 1 + 3 * (int64_t(sh_degree) + 1) * (int64_t(sh_degree) + 1);
-  uint8_t* part_live = reinterpret_cast<uint8_t*>(part) +
-                       (static_cast<size_t>(3 * row * num_columns) * sizeof(float) + 255) / 256 * 256;
+  int32_t* list = reinterpret_cast<int32_t*>(
+      reinterpret_cast<char*>(part) +
+      (static_cast<size_t>(3 * row * num_columns) * sizeof(float) + 255) / 256 * 256);
+  int32_t* live_count = list + num_columns;
+  uint8_t* col_live = reinterpret_cast<uint8_t*>(list) +
+                      (static_cast<size_t>(num_columns + 1) * sizeof(int32_t) + 255) / 256 * 256;
   mt::fill_maps_kernel<<<mt::blocks_for(words, 256), 256, 0, st>>>(maps, words);
   if (num_columns > 0)
     mt::build_maps_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
         lower_corners, num_columns, num_grid_vertices, maps);
+  mt::zero_count_kernel<<<1, 1, 0, st>>>(live_count);
+  if (num_columns > 0)
+    mt::colors_adjoint_mask_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
+        grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S, vertex_offsets,
+        num_triangle_vertices, vertex_touched, maps, col_live, list, live_count);
   const dim3 cgrid(mt::blocks_for(num_columns, mt::kBlock));
   const dim3 ggrid(mt::blocks_for(num_grid_vertices, mt::kBlock));
 #define MT_ADJ(N, D)                                                                             \
@@ -849,10 +896,10 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
       mt::colors_adjoint_columns_kernel<N, D><<<cgrid, mt::kBlock, 0, st>>>(                     \
           grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S,                \
           vertex_offsets, num_triangle_vertices, vertices, colors, camera_positions, B, grad_sdf, \
-          grad_colors, grid_sdf_grad, grid_sh_grad, part, part_live, maps, accumulate != 0,     \
-          vertex_touched);                                                                      \
+          grad_colors, grid_sdf_grad, grid_sh_grad, part, list, live_count, maps,               \
+          accumulate != 0, vertex_touched);                                                     \
     mt::colors_adjoint_ends_kernel<N, D><<<ggrid, mt::kBlock, 0, st>>>(                          \
-        num_grid_vertices, num_columns, part, part_live, grid_sdf_grad, grid_sh_grad, maps,       \
+        num_grid_vertices, num_columns, part, col_live, grid_sdf_grad, grid_sh_grad, maps,        \
         accumulate != 0, status);                                                                \
   } while (0)
   const int key = num_sh * 4 + sh_degree;
