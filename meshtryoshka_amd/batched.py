@@ -248,6 +248,10 @@ class BatchedTrainingStep:
         call("mt_exclusive_scan_i32", self.face_offsets.data_ptr(), self.face_offsets.numel(),
              self.scan_ws.data_ptr(), self.scan_ws.numel(), st)
         call("mt_camera_positions", self.extrinsics.data_ptr(), self.B, self.campos.data_ptr(), st)
+        # The grid gradients start from zero and every camera group adds to them, so the
+        # adjoint touches only the grid vertices that receive a gradient.
+        self.grid_sdf_grad.zero_()
+        self.grid_sh_grad.zero_()
         for gi, g in enumerate(self.groups):
             c0, n = g["c0"], g["n"]
             cam_dims = g["cam_dims"].data_ptr() if table else None
@@ -320,16 +324,14 @@ class BatchedTrainingStep:
             call("mt_intersect_and_composite_backward", args, out.data_ptr(), grad.data_ptr(),
                  self.colors_grad.data_ptr(), self.td_grad.data_ptr(), self.sharp_grad.data_ptr(),
                  self.quirk, st)
-            # SH adjoint fused into the vertex adjoint: the first group overwrites the grid
-            # gradients, later groups add to them.
+            # SH adjoint fused into the vertex adjoint, adding to the grid gradients.
             call("mt_create_triangle_vertices_colors_backward", grid_sdf.data_ptr(), self.n_grid,
                  self.num_sh, self.sh_degree, v.lower_corners.data_ptr(), self.n_col,
                  self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V,
                  self.tv.data_ptr(), self.colors.data_ptr(), self.campos[c0:].data_ptr(), n,
                  self.td_grad.data_ptr(), self.colors_grad.data_ptr(),
                  self.grid_sdf_grad.data_ptr(), self.grid_sh_grad.data_ptr(),
-                 self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(), int(gi > 0),
-                 self.touched.data_ptr(),
+                 self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(), 1, self.touched.data_ptr(),
                  self.overflow.data_ptr(), st)
 
     # ------------------------------------------------------------------------------

@@ -414,14 +414,17 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
 
 // Phase 0, one thread per column: bit a of col_live[c] is set when some vertex of the
 // column's axis-a edge takes part (below the vertex count and, with touched flags, flagged);
-// the columns with any set bit are appended to list (one atomic per wave).
+// bit 3 s + a of col_slots[c] when the one of level set s does (S <= kSlotLevels). The
+// columns with any set bit are appended to list (one atomic per wave).
+constexpr int kSlotLevels = 10;
+
 __global__ void __launch_bounds__(kBlock) colors_adjoint_mask_kernel(
     const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
     int S, const int32_t* offsets, int64_t nt, const uint8_t* touched, const int32_t* maps,
-    uint8_t* col_live, int32_t* list, int32_t* count) {
+    uint8_t* col_live, uint32_t* col_slots, int32_t* list, int32_t* count) {
   if (maps[4 * ng] != 0) return;
   const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  uint32_t bits = 0;
+  uint32_t bits = 0, slots = 0;
   if (c < n) {
     const ColumnSdf col = load_column(gsdf, lower, n, c);
     for (int s = 0; s < S; ++s) {
@@ -430,11 +433,15 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_mask_kernel(
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         if (!crosses(col.d0, col.d[a], iso)) continue;
-        if (col.v[a] >= 0 && here < nt && (touched == nullptr || touched[here])) bits |= 1u << a;
+        if (col.v[a] >= 0 && here < nt && (touched == nullptr || touched[here])) {
+          bits |= 1u << a;
+          if (s < kSlotLevels) slots |= 1u << (3 * s + a);
+        }
         ++here;
       }
     }
     col_live[c] = static_cast<uint8_t>(bits);
+    col_slots[c] = slots;
   }
   const uint64_t m = __ballot(bits != 0);
   const int lane = __lane_id();
@@ -457,14 +464,15 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
     const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
     int S, const int32_t* offsets, int64_t nt, const float* tv, const float* colors,
     const float* cams, int B, const float* g_d, const float* g_col, float* gg_d, float* gg_sh,
-    float* part, const int32_t* list, const int32_t* count, const int32_t* maps, bool accumulate,
-    const uint8_t* touched) {
+    float* part, const int32_t* list, const int32_t* count, const uint8_t* col_live,
+    const uint32_t* col_slots, const int32_t* maps, bool accumulate, const uint8_t* touched) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 1 + 3 * kN;
   if (maps[4 * ng] != 0) return;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= *count) return;
   const int64_t c = list[i];  // a column with some contributing vertex
+  const uint32_t axes = col_live[c], slots = col_slots[c];
   const ColumnSdf col = load_column(gsdf, lower, n, c);
   float acc0_d = 0.f;
   float acc0[kN][3];
@@ -472,13 +480,14 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
   for (int k = 0; k < kN; ++k) acc0[k][0] = acc0[k][1] = acc0[k][2] = 0.f;
   bool live0 = false;  // some vertex of the column takes part
   for (int a = 0; a < 3; ++a) {
-    if (col.v[a] < 0) continue;
+    if (col.v[a] < 0 || !((axes >> a) & 1u)) continue;
     bool live = false;  // some vertex of this edge takes part
     float acce_d = 0.f;
     float acce[kN][3];
 #pragma unroll
     for (int k = 0; k < kN; ++k) acce[k][0] = acce[k][1] = acce[k][2] = 0.f;
     for (int s = 0; s < S; ++s) {
+      if (s < kSlotLevels && !((slots >> (3 * s + a)) & 1u)) continue;  // nothing to add
       const float iso = level_sets[s];
       if (!crosses(col.d0, col.d[a], iso)) continue;
       int64_t here = offsets[s * n + c];
@@ -839,8 +848,9 @@ extern "C" size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(
   const size_t maps = static_cast<size_t>(4 * num_grid_vertices + 1) * sizeof(int32_t);
   const size_t part = static_cast<size_t>(3 * row * num_columns) * sizeof(float);
   const size_t list = static_cast<size_t>(num_columns + 1) * sizeof(int32_t);
+  const size_t slots = static_cast<size_t>(num_columns) * sizeof(uint32_t);
   return (maps + 255) / 256 * 256 + (part + 255) / 256 * 256 + (list + 255) / 256 * 256 +
-         static_cast<size_t>(num_columns);
+         (slots + 255) / 256 * 256 + static_cast<size_t>(num_columns);
 }
 
 extern "C" int mt_create_triangle_vertices_colors_backward(
@@ -877,8 +887,11 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
       reinterpret_cast<char*>(part) +
       (static_cast<size_t>(3 * row * num_columns) * sizeof(float) + 255) / 256 * 256);
   int32_t* live_count = list + num_columns;
-  uint8_t* col_live = reinterpret_cast<uint8_t*>(list) +
-                      (static_cast<size_t>(num_columns + 1) * sizeof(int32_t) + 255) / 256 * 256;
+  uint32_t* col_slots = reinterpret_cast<uint32_t*>(
+      reinterpret_cast<char*>(list) +
+      (static_cast<size_t>(num_columns + 1) * sizeof(int32_t) + 255) / 256 * 256);
+  uint8_t* col_live = reinterpret_cast<uint8_t*>(col_slots) +
+                      (static_cast<size_t>(num_columns) * sizeof(uint32_t) + 255) / 256 * 256;
   mt::fill_maps_kernel<<<mt::blocks_for(words, 256), 256, 0, st>>>(maps, words);
   if (num_columns > 0)
     mt::build_maps_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
@@ -887,7 +900,7 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
   if (num_columns > 0)
     mt::colors_adjoint_mask_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
         grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S, vertex_offsets,
-        num_triangle_vertices, vertex_touched, maps, col_live, list, live_count);
+        num_triangle_vertices, vertex_touched, maps, col_live, col_slots, list, live_count);
   const dim3 cgrid(mt::blocks_for(num_columns, mt::kBlock));
   const dim3 ggrid(mt::blocks_for(num_grid_vertices, mt::kBlock));
 #define MT_ADJ(N, D)                                                                             \
@@ -896,8 +909,8 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
       mt::colors_adjoint_columns_kernel<N, D><<<cgrid, mt::kBlock, 0, st>>>(                     \
           grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S,                \
           vertex_offsets, num_triangle_vertices, vertices, colors, camera_positions, B, grad_sdf, \
-          grad_colors, grid_sdf_grad, grid_sh_grad, part, list, live_count, maps,               \
-          accumulate != 0, vertex_touched);                                                     \
+          grad_colors, grid_sdf_grad, grid_sh_grad, part, list, live_count, col_live, col_slots, \
+          maps, accumulate != 0, vertex_touched);                                               \
     mt::colors_adjoint_ends_kernel<N, D><<<ggrid, mt::kBlock, 0, st>>>(                          \
         num_grid_vertices, num_columns, part, col_live, grid_sdf_grad, grid_sh_grad, maps,        \
         accumulate != 0, status);                                                                \
