@@ -427,7 +427,36 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_mask_kernel(
   uint32_t bits = 0, slots = 0;
   if (c < n) {
     const ColumnSdf col = load_column(gsdf, lower, n, c);
-    for (int s = 0; s < S; ++s) {
+    int s0 = 0;
+    if (S <= kSlotLevels && touched != nullptr) {
+      // All level sets' first vertices, then every flag, are loaded before any is used.
+      int32_t first[kSlotLevels];
+#pragma unroll
+      for (int s = 0; s < kSlotLevels; ++s) first[s] = s < S ? offsets[s * n + c] : 0;
+      uint8_t flag[kSlotLevels][3];
+#pragma unroll
+      for (int s = 0; s < kSlotLevels; ++s) {
+        const float iso = s < S ? level_sets[s] : 0.f;
+        int64_t here = first[s];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          flag[s][a] = 0;
+          if (s >= S || !crosses(col.d0, col.d[a], iso)) continue;
+          if (col.v[a] >= 0 && here < nt) flag[s][a] = touched[here];
+          ++here;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < kSlotLevels; ++s)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          if (flag[s][a]) {
+            bits |= 1u << a;
+            slots |= 1u << (3 * s + a);
+          }
+      s0 = S;
+    }
+    for (int s = s0; s < S; ++s) {
       const float iso = level_sets[s];
       int64_t here = offsets[s * n + c];
 #pragma unroll
@@ -443,15 +472,32 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_mask_kernel(
     col_live[c] = static_cast<uint8_t>(bits);
     col_slots[c] = slots;
   }
+  // One atomic per block: waves count their live columns, thread 0 reserves the block's
+  // range, each wave writes its part in lane order.
+  __shared__ int s_count[kBlock / kWave];
+  __shared__ int s_base;
   const uint64_t m = __ballot(bits != 0);
   const int lane = __lane_id();
-  int at = 0;
-  if (lane == 0 && m != 0) at = atomicAdd(count, __popcll(m));
-  at = __shfl(at, 0, kWave);
+  const int wave = threadIdx.x / kWave;
+  if (lane == 0) s_count[wave] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int total = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) total += s_count[w];
+    s_base = total > 0 ? atomicAdd(count, total) : 0;
+  }
+  __syncthreads();
+  int at = s_base;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) at += w < wave ? s_count[w] : 0;
   if (bits != 0) list[at + __popcll(m & lanemask_lt())] = static_cast<int32_t>(c);
 }
 
-__global__ void zero_count_kernel(int32_t* count) { *count = 0; }
+__global__ void zero_counts_kernel(int32_t* a, int32_t* b) {
+  *a = 0;
+  *b = 0;
+}
 
 // Phase 1, one thread per column: the column's crossings are visited once (axis by axis,
 // level sets in order); (1 - t) g goes to the column's origin, which no other column has
@@ -549,19 +595,57 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
     }
 }
 
+// For accumulating calls: the grid vertices that some live row ends at (one atomic per
+// block), so that phase 2 runs over them only.
+__global__ void __launch_bounds__(kBlock) colors_adjoint_ends_mask_kernel(
+    int64_t ng, const uint8_t* col_live, const int32_t* maps, int32_t* list, int32_t* count) {
+  if (maps[4 * ng] != 0) return;
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool live = false;
+  if (g < ng) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int32_t e = maps[(a + 1) * ng + g];
+      live = live || (e >= 0 && ((col_live[e] >> a) & 1u));
+    }
+  }
+  __shared__ int s_count[kBlock / kWave];
+  __shared__ int s_base;
+  const uint64_t m = __ballot(live);
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / kWave;
+  if (lane == 0) s_count[wave] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int total = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) total += s_count[w];
+    s_base = total > 0 ? atomicAdd(count, total) : 0;
+  }
+  __syncthreads();
+  int at = s_base;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) at += w < wave ? s_count[w] : 0;
+  if (live) list[at + __popcll(m & lanemask_lt())] = static_cast<int32_t>(g);
+}
+
 // Phase 2, one thread per grid vertex: add the axis-end sums of the (at most three)
 // columns that end at it; a grid vertex that is no column's origin starts from zero.
 template <int kNsh, int kDeg>
 __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
     int64_t ng, int64_t n, const float* part, const uint8_t* col_live, float* gg_d, float* gg_sh,
-    const int32_t* maps, bool accumulate, int32_t* status) {
+    const int32_t* maps, bool accumulate, int32_t* status, const int32_t* list,
+    const int32_t* count) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 1 + 3 * kN;
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (maps[4 * ng] != 0) {  // the columns do not map one-to-one onto grid vertices
-    if (g == 0 && status) atomicOr(status, 2);
+    if (i == 0 && status) atomicOr(status, 2);
     return;
   }
+  // With a list (accumulating calls): only the listed grid vertices, which have live ends.
+  if (list != nullptr && i >= *count) return;
+  const int64_t g = list != nullptr ? int64_t(list[i]) : i;
   if (g >= ng) return;
   // Phase 1 wrote (or added to) the origins of the live columns; other grid vertices start
   // from zero unless the call accumulates.
@@ -849,8 +933,10 @@ extern "C" size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(
   const size_t part = static_cast<size_t>(3 * row * num_columns) * sizeof(float);
   const size_t list = static_cast<size_t>(num_columns + 1) * sizeof(int32_t);
   const size_t slots = static_cast<size_t>(num_columns) * sizeof(uint32_t);
+  const size_t live = static_cast<size_t>(num_columns);
+  const size_t ends = static_cast<size_t>(num_grid_vertices + 1) * sizeof(int32_t);
   return (maps + 255) / 256 * 256 + (part + 255) / 256 * 256 + (list + 255) / 256 * 256 +
-         (slots + 255) / 256 * 256 + static_cast<size_t>(num_columns);
+         (slots + 255) / 256 * 256 + (live + 255) / 256 * 256 + ends;
 }
 
 extern "C" int mt_create_triangle_vertices_colors_backward(
@@ -892,17 +978,25 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
       (static_cast<size_t>(num_columns + 1) * sizeof(int32_t) + 255) / 256 * 256);
   uint8_t* col_live = reinterpret_cast<uint8_t*>(col_slots) +
                       (static_cast<size_t>(num_columns) * sizeof(uint32_t) + 255) / 256 * 256;
+  int32_t* ends_list = reinterpret_cast<int32_t*>(
+      col_live + (static_cast<size_t>(num_columns) + 255) / 256 * 256);
+  int32_t* ends_count = ends_list + num_grid_vertices;
   mt::fill_maps_kernel<<<mt::blocks_for(words, 256), 256, 0, st>>>(maps, words);
   if (num_columns > 0)
     mt::build_maps_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
         lower_corners, num_columns, num_grid_vertices, maps);
-  mt::zero_count_kernel<<<1, 1, 0, st>>>(live_count);
+  mt::zero_counts_kernel<<<1, 1, 0, st>>>(live_count, ends_count);
   if (num_columns > 0)
     mt::colors_adjoint_mask_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
         grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S, vertex_offsets,
         num_triangle_vertices, vertex_touched, maps, col_live, col_slots, list, live_count);
   const dim3 cgrid(mt::blocks_for(num_columns, mt::kBlock));
   const dim3 ggrid(mt::blocks_for(num_grid_vertices, mt::kBlock));
+  // An accumulating call leaves the grid vertices without live ends alone: phase 2 runs
+  // over the list of the others.
+  if (accumulate != 0)
+    mt::colors_adjoint_ends_mask_kernel<<<ggrid, mt::kBlock, 0, st>>>(
+        num_grid_vertices, col_live, maps, ends_list, ends_count);
 #define MT_ADJ(N, D)                                                                             \
   do {                                                                                           \
     if (num_columns > 0)                                                                         \
@@ -913,7 +1007,7 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
           maps, accumulate != 0, vertex_touched);                                               \
     mt::colors_adjoint_ends_kernel<N, D><<<ggrid, mt::kBlock, 0, st>>>(                          \
         num_grid_vertices, num_columns, part, col_live, grid_sdf_grad, grid_sh_grad, maps,        \
-        accumulate != 0, status);                                                                \
+        accumulate != 0, status, accumulate != 0 ? ends_list : nullptr, ends_count);              \
   } while (0)
   const int key = num_sh * 4 + sh_degree;
   switch (key) {
