@@ -412,6 +412,47 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
   }
 }
 
+// Rows of the colour adjoint's workspace: [SDF sum, SH sums (coefficient-major, then
+// channel)], padded to whole float4s. A live column's four rows (axis ends x, y, z, then
+// its origin) are contiguous at its position in the live-column list.
+__host__ __device__ constexpr int adjoint_row_floats(int n_coef) { return (1 + 3 * n_coef + 3) / 4 * 4; }
+
+template <int kN>
+__device__ __forceinline__ void store_row(float* row, float d, const float (&acc)[kN][3]) {
+  constexpr int kPad = adjoint_row_floats(kN);
+  float r[kPad];
+  r[0] = d;
+#pragma unroll
+  for (int k = 0; k < kN; ++k)
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) r[1 + 3 * k + ch] = acc[k][ch];
+#pragma unroll
+  for (int j = 1 + 3 * kN; j < kPad; ++j) r[j] = 0.f;
+  float4* r4 = reinterpret_cast<float4*>(row);
+#pragma unroll
+  for (int j = 0; j < kPad / 4; ++j) r4[j] = make_float4(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]);
+}
+
+template <int kN>
+__device__ __forceinline__ void add_row(const float* row, float& d, float (&acc)[kN][3]) {
+  constexpr int kPad = adjoint_row_floats(kN);
+  float r[kPad];
+  const float4* r4 = reinterpret_cast<const float4*>(row);
+#pragma unroll
+  for (int j = 0; j < kPad / 4; ++j) {
+    const float4 q = r4[j];
+    r[4 * j] = q.x;
+    r[4 * j + 1] = q.y;
+    r[4 * j + 2] = q.z;
+    r[4 * j + 3] = q.w;
+  }
+  d += r[0];
+#pragma unroll
+  for (int k = 0; k < kN; ++k)
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) acc[k][ch] += r[1 + 3 * k + ch];
+}
+
 // Phase 0, one thread per column: bit a of col_live[c] is set when some vertex of the
 // column's axis-a edge takes part (below the vertex count and, with touched flags, flagged);
 // bit 3 s + a of col_slots[c] when the one of level set s does (S <= kSlotLevels). The
@@ -421,7 +462,7 @@ constexpr int kSlotLevels = 10;
 __global__ void __launch_bounds__(kBlock) colors_adjoint_mask_kernel(
     const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
     int S, const int32_t* offsets, int64_t nt, const uint8_t* touched, const int32_t* maps,
-    uint8_t* col_live, uint32_t* col_slots, int32_t* list, int32_t* count) {
+    uint8_t* col_live, uint32_t* col_slots, int32_t* list, int32_t* col_pos, int32_t* count) {
   if (maps[4 * ng] != 0) return;
   const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   uint32_t bits = 0, slots = 0;
@@ -491,7 +532,11 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_mask_kernel(
   int at = s_base;
 #pragma unroll
   for (int w = 0; w < kBlock / kWave; ++w) at += w < wave ? s_count[w] : 0;
-  if (bits != 0) list[at + __popcll(m & lanemask_lt())] = static_cast<int32_t>(c);
+  if (bits != 0) {
+    const int32_t i = at + __popcll(m & lanemask_lt());
+    list[i] = static_cast<int32_t>(c);
+    col_pos[c] = i;
+  }
 }
 
 __global__ void zero_counts_kernel(int32_t* a, int32_t* b) {
@@ -509,11 +554,10 @@ template <int kNsh, int kDeg>
 __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
     const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
     int S, const int32_t* offsets, int64_t nt, const float* tv, const float* colors,
-    const float* cams, int B, const float* g_d, const float* g_col, float* gg_d, float* gg_sh,
-    float* part, const int32_t* list, const int32_t* count, const uint8_t* col_live,
-    const uint32_t* col_slots, const int32_t* maps, bool accumulate, const uint8_t* touched) {
+    const float* cams, int B, const float* g_d, const float* g_col, float* part,
+    const int32_t* list, const int32_t* count, const uint8_t* col_live, const uint32_t* col_slots,
+    const int32_t* maps, const uint8_t* touched) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
-  constexpr int kRow = 1 + 3 * kN;
   if (maps[4 * ng] != 0) return;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= *count) return;
@@ -575,24 +619,9 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
     // Rows of edges without a contributing vertex (col_live bit clear) stay unwritten.
     if (!live) continue;
     live0 = true;
-    float* pa = part + int64_t(a) * kRow * n;
-    pa[c] = acce_d;
-#pragma unroll
-    for (int k = 0; k < kN; ++k)
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) pa[(1 + 3 * k + ch) * n + c] = acce[k][ch];
+    store_row<kN>(part + (i * 4 + a) * adjoint_row_floats(kN), acce_d, acce);
   }
-  const int64_t g = col.v0;
-  if (accumulate && !live0) return;  // nothing to add
-  gg_d[g] = accumulate ? gg_d[g] + acc0_d : acc0_d;
-#pragma unroll
-  for (int k = 0; k < kNsh; ++k)
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-      const float v = k < kN ? acc0[k < kN ? k : 0][ch] : 0.f;
-      float* o = gg_sh + (int64_t(k) * ng + g) * 3 + ch;
-      *o = accumulate ? *o + v : v;
-    }
+  store_row<kN>(part + (i * 4 + 3) * adjoint_row_floats(kN), acc0_d, acc0);  // the origin's
 }
 
 // For accumulating calls: the grid vertices that some live row ends at (one atomic per
@@ -603,6 +632,8 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_mask_kernel(
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   bool live = false;
   if (g < ng) {
+    const int32_t own = maps[g];
+    live = own >= 0 && col_live[own] != 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const int32_t e = maps[(a + 1) * ng + g];
@@ -633,11 +664,11 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_mask_kernel(
 // columns that end at it; a grid vertex that is no column's origin starts from zero.
 template <int kNsh, int kDeg>
 __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
-    int64_t ng, int64_t n, const float* part, const uint8_t* col_live, float* gg_d, float* gg_sh,
-    const int32_t* maps, bool accumulate, int32_t* status, const int32_t* list,
-    const int32_t* count) {
+    int64_t ng, int64_t n, const float* part, const uint8_t* col_live, const int32_t* col_pos,
+    float* gg_d, float* gg_sh, const int32_t* maps, bool accumulate, int32_t* status,
+    const int32_t* list, const int32_t* count) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
-  constexpr int kRow = 1 + 3 * kN;
+  constexpr int kPad = adjoint_row_floats(kN);
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (maps[4 * ng] != 0) {  // the columns do not map one-to-one onto grid vertices
     if (i == 0 && status) atomicOr(status, 2);
@@ -647,36 +678,29 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
   if (list != nullptr && i >= *count) return;
   const int64_t g = list != nullptr ? int64_t(list[i]) : i;
   if (g >= ng) return;
-  // Phase 1 wrote (or added to) the origins of the live columns; other grid vertices start
-  // from zero unless the call accumulates.
+  // The sum: the origin row of the grid vertex's own column, then the axis-end rows of the
+  // columns that end at it (axis order); rows of dead columns and edges are zero.
+  float sd = 0.f;
+  float v[kN][3];
+#pragma unroll
+  for (int k = 0; k < kN; ++k) v[k][0] = v[k][1] = v[k][2] = 0.f;
   const int32_t own = maps[g];
-  const bool origin = (own >= 0 && col_live[own] != 0) || accumulate;
-  int32_t ends[3];
-  bool any = false;
+  if (own >= 0 && col_live[own] != 0)
+    add_row<kN>(part + (int64_t(col_pos[own]) * 4 + 3) * kPad, sd, v);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    ends[a] = maps[(a + 1) * ng + g];
-    if (ends[a] >= 0 && !((col_live[ends[a]] >> a) & 1u)) ends[a] = -1;  // an unwritten zero row
-    any = any || ends[a] >= 0;
+    const int32_t e = maps[(a + 1) * ng + g];
+    if (e >= 0 && ((col_live[e] >> a) & 1u))
+      add_row<kN>(part + (int64_t(col_pos[e]) * 4 + a) * kPad, sd, v);
   }
-  if (origin && !any) return;  // phase 1 wrote the final value
-  float sd = origin ? gg_d[g] : 0.f;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-    if (ends[a] >= 0) sd += part[int64_t(a) * kRow * n + ends[a]];
-  gg_d[g] = sd;
+  gg_d[g] = accumulate ? gg_d[g] + sd : sd;
 #pragma unroll
   for (int k = 0; k < kNsh; ++k)
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
       float* o = gg_sh + (int64_t(k) * ng + g) * 3 + ch;
-      float v = origin ? *o : 0.f;
-      if (k < kN) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-          if (ends[a] >= 0) v += part[(int64_t(a) * kRow + 1 + 3 * k + ch) * n + ends[a]];
-      }
-      *o = v;
+      const float x = k < kN ? v[k < kN ? k : 0][ch] : 0.f;
+      *o = accumulate ? *o + x : x;
     }
 }
 
@@ -928,15 +952,16 @@ extern "C" int mt_create_triangle_vertices_colors_forward(
 
 extern "C" size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(
     int64_t num_grid_vertices, int64_t num_columns, int32_t sh_degree) {
-  const int64_t row = 1 + 3 * (int64_t(sh_degree) + 1) * (int64_t(sh_degree) + 1);
+  const int64_t row = mt::adjoint_row_floats((sh_degree + 1) * (sh_degree + 1));
   const size_t maps = static_cast<size_t>(4 * num_grid_vertices + 1) * sizeof(int32_t);
-  const size_t part = static_cast<size_t>(3 * row * num_columns) * sizeof(float);
+  const size_t part = static_cast<size_t>(4 * row * num_columns) * sizeof(float);
   const size_t list = static_cast<size_t>(num_columns + 1) * sizeof(int32_t);
   const size_t slots = static_cast<size_t>(num_columns) * sizeof(uint32_t);
   const size_t live = static_cast<size_t>(num_columns);
   const size_t ends = static_cast<size_t>(num_grid_vertices + 1) * sizeof(int32_t);
-  return (maps + 255) / 256 * 256 + (part + 255) / 256 * 256 + (list + 255) / 256 * 256 +
-         (slots + 255) / 256 * 256 + (live + 255) / 256 * 256 + ends;
+  const size_t pos = static_cast<size_t>(num_columns) * sizeof(int32_t);
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  return up(maps) + up(part) + up(list) + up(slots) + up(live) + up(ends) + pos;
 }
 
 extern "C" int mt_create_triangle_vertices_colors_backward(
@@ -968,10 +993,10 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
   const int64_t words = 4 * num_grid_vertices;
   float* part = reinterpret_cast<float*>(static_cast<char*>(workspace) +
                                          (static_cast<size_t>(words + 1) * 4 + 255) / 256 * 256);
-  const int64_t row = 1 + 3 * (int64_t(sh_degree) + 1) * (int64_t(sh_degree) + 1);
+  const int64_t row = mt::adjoint_row_floats((sh_degree + 1) * (sh_degree + 1));
   int32_t* list = reinterpret_cast<int32_t*>(
       reinterpret_cast<char*>(part) +
-      (static_cast<size_t>(3 * row * num_columns) * sizeof(float) + 255) / 256 * 256);
+      (static_cast<size_t>(4 * row * num_columns) * sizeof(float) + 255) / 256 * 256);
   int32_t* live_count = list + num_columns;
   uint32_t* col_slots = reinterpret_cast<uint32_t*>(
       reinterpret_cast<char*>(list) +
@@ -981,6 +1006,9 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
   int32_t* ends_list = reinterpret_cast<int32_t*>(
       col_live + (static_cast<size_t>(num_columns) + 255) / 256 * 256);
   int32_t* ends_count = ends_list + num_grid_vertices;
+  int32_t* col_pos = reinterpret_cast<int32_t*>(
+      reinterpret_cast<char*>(ends_list) +
+      (static_cast<size_t>(num_grid_vertices + 1) * sizeof(int32_t) + 255) / 256 * 256);
   mt::fill_maps_kernel<<<mt::blocks_for(words, 256), 256, 0, st>>>(maps, words);
   if (num_columns > 0)
     mt::build_maps_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
@@ -989,7 +1017,8 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
   if (num_columns > 0)
     mt::colors_adjoint_mask_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
         grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S, vertex_offsets,
-        num_triangle_vertices, vertex_touched, maps, col_live, col_slots, list, live_count);
+        num_triangle_vertices, vertex_touched, maps, col_live, col_slots, list, col_pos,
+        live_count);
   const dim3 cgrid(mt::blocks_for(num_columns, mt::kBlock));
   const dim3 ggrid(mt::blocks_for(num_grid_vertices, mt::kBlock));
   // An accumulating call leaves the grid vertices without live ends alone: phase 2 runs
@@ -1003,11 +1032,10 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
       mt::colors_adjoint_columns_kernel<N, D><<<cgrid, mt::kBlock, 0, st>>>(                     \
           grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S,                \
           vertex_offsets, num_triangle_vertices, vertices, colors, camera_positions, B, grad_sdf, \
-          grad_colors, grid_sdf_grad, grid_sh_grad, part, list, live_count, col_live, col_slots, \
-          maps, accumulate != 0, vertex_touched);                                               \
+          grad_colors, part, list, live_count, col_live, col_slots, maps, vertex_touched);      \
     mt::colors_adjoint_ends_kernel<N, D><<<ggrid, mt::kBlock, 0, st>>>(                          \
-        num_grid_vertices, num_columns, part, col_live, grid_sdf_grad, grid_sh_grad, maps,        \
-        accumulate != 0, status, accumulate != 0 ? ends_list : nullptr, ends_count);              \
+        num_grid_vertices, num_columns, part, col_live, col_pos, grid_sdf_grad, grid_sh_grad,     \
+        maps, accumulate != 0, status, accumulate != 0 ? ends_list : nullptr, ends_count);        \
   } while (0)
   const int key = num_sh * 4 + sh_degree;
   switch (key) {
