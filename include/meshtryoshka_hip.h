@@ -166,21 +166,23 @@ size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(int64_t num_g
  * grid_sdf_grad f32[num_grid_vertices] and grid_sh_grad f32[num_sh, num_grid_vertices, 3]
  * from grad_sdf f32[V] and
  * grad_colors f32[B, V, 3], using the forward's vertices f32[V, 3] and colors (relu is
- * active where the colour is > 0). Each column visits its crossings once: the origin's
- * share goes straight to the grid gradient, the axis ends' shares through the workspace
- * to the grid vertices they end at. This needs each grid vertex to be the origin / +axis
- * end of at most one column, as extract_voxels guarantees; otherwise nothing is written
- * and bit 1 (value 2) of *status is set. num_sh in {1, 4, 9, 16}. vertex_touched
- * (uint8[V] or NULL): the flags mt_intersect_and_composite_backward set; vertices without
- * the flag have zero gradients and are skipped. */
+ * active where the colour is > 0). Each column visits its crossings once; the origin's
+ * and the axis ends' shares go through workspace rows to the grid vertices, each of which
+ * is written once. This needs each grid vertex to be the origin / +axis end of at most one
+ * column, as extract_voxels guarantees; otherwise nothing is written and bit 1 (value 2)
+ * of *status is set. num_sh in {1, 4, 9, 16}. vertex_touched (uint8[V] or NULL): the flags
+ * mt_intersect_and_composite_backward set; vertices without the flag have zero gradients
+ * and are skipped. clear_gradients = 1 zeroes the grad_sdf and grad_colors entries it
+ * reads (every nonzero entry when the flags are given), so that a caller that keeps them
+ * zero between steps needs no zero-fill (no counterpart in the reference). */
 int mt_create_triangle_vertices_colors_backward(
     const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree,
     const int32_t* lower_corners, int64_t num_columns, const float* level_sets, int32_t S,
     const int32_t* vertex_offsets, int64_t num_triangle_vertices, const float* vertices,
-    const float* colors, const float* camera_positions, int32_t B, const float* grad_sdf,
-    const float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
-    size_t workspace_bytes, int32_t accumulate, const uint8_t* vertex_touched, int32_t* status,
-    mt_stream_t stream);
+    const float* colors, const float* camera_positions, int32_t B, float* grad_sdf,
+    float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
+    size_t workspace_bytes, int32_t accumulate, const uint8_t* vertex_touched,
+    int32_t clear_gradients, int32_t* status, mt_stream_t stream);
 
 /* face_counts int32[S * num_voxels] (3 x triangles) and cell_codes uint8[S, num_voxels].
  * Replaces count_triangle_faces.slang:13-50 (.py:28-63). upper_corners

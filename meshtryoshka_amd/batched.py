@@ -255,8 +255,8 @@ class BatchedTrainingStep:
         for gi, g in enumerate(self.groups):
             c0, n = g["c0"], g["n"]
             cam_dims = g["cam_dims"].data_ptr() if table else None
-            self.colors_grad[:n].zero_()
-            self.td_grad.zero_()
+            # colors_grad and td_grad hold zeros here: they start as zeros and the adjoint
+            # clears every entry the composite backward wrote (the flagged vertices).
             self.touched.zero_()
             self.ranges[: g["tiles"]].zero_()
             if hook is not None:
@@ -331,7 +331,7 @@ class BatchedTrainingStep:
                  self.tv.data_ptr(), self.colors.data_ptr(), self.campos[c0:].data_ptr(), n,
                  self.td_grad.data_ptr(), self.colors_grad.data_ptr(),
                  self.grid_sdf_grad.data_ptr(), self.grid_sh_grad.data_ptr(),
-                 self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(), 1, self.touched.data_ptr(),
+                 self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(), 1, self.touched.data_ptr(), 1,
                  self.overflow.data_ptr(), st)
 
     # ------------------------------------------------------------------------------

@@ -554,9 +554,9 @@ template <int kNsh, int kDeg>
 __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
     const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
     int S, const int32_t* offsets, int64_t nt, const float* tv, const float* colors,
-    const float* cams, int B, const float* g_d, const float* g_col, float* part,
-    const int32_t* list, const int32_t* count, const uint8_t* col_live, const uint32_t* col_slots,
-    const int32_t* maps, const uint8_t* touched) {
+    const float* cams, int B, float* g_d, float* g_col, float* part, const int32_t* list,
+    const int32_t* count, const uint8_t* col_live, const uint32_t* col_slots, const int32_t* maps,
+    const uint8_t* touched, bool clear) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   if (maps[4 * ng] != 0) return;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -588,11 +588,12 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
       const float t = edge_t(col.d0, col.d[a], iso);
       const float w0 = 1.0f - t;
       const float gd = g_d[here];
+      if (clear) g_d[here] = 0.f;
       acc0_d += w0 * gd;
       acce_d += t * gd;
       const float3 p = vertex_at(tv, here);
       const float* cb = colors + 3 * here;  // camera b's colour of this vertex: cb + b * 3 nt
-      const float* gb = g_col + 3 * here;
+      float* gb = g_col + 3 * here;
       const int64_t stride = 3 * nt;
       for (int b = 0; b < B; ++b, cb += stride, gb += stride) {
         // The view direction with the hardware reciprocal square root (~1 ulp): the
@@ -602,9 +603,11 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
         const float inv = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
         float basis[16];
         sh_basis(dx * inv, dy * inv, dz * inv, kDeg, basis);
+        const float gr[3] = {gb[0], gb[1], gb[2]};
+        if (clear) gb[0] = gb[1] = gb[2] = 0.f;
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-          const float gc = cb[ch] > 0.0f ? gb[ch] : 0.0f;
+          const float gc = cb[ch] > 0.0f ? gr[ch] : 0.0f;
           const float g0 = w0 * gc, g1 = t * gc;
           // Fused multiply-adds (the build keeps separate roundings elsewhere, for bit-exact
           // forward values; a gradient sum only needs the tolerance).
@@ -968,10 +971,10 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
     const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree,
     const int32_t* lower_corners, int64_t num_columns, const float* level_sets, int32_t S,
     const int32_t* vertex_offsets, int64_t num_triangle_vertices, const float* vertices,
-    const float* colors, const float* camera_positions, int32_t B, const float* grad_sdf,
-    const float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
-    size_t workspace_bytes, int32_t accumulate, const uint8_t* vertex_touched, int32_t* status,
-    mt_stream_t stream) {
+    const float* colors, const float* camera_positions, int32_t B, float* grad_sdf,
+    float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
+    size_t workspace_bytes, int32_t accumulate, const uint8_t* vertex_touched,
+    int32_t clear_gradients, int32_t* status, mt_stream_t stream) {
   MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0 && B >= 0,
              "mt_create_triangle_vertices_colors_backward: negative size");
   MT_REQUIRE(num_sh == 1 || num_sh == 4 || num_sh == 9 || num_sh == 16,
@@ -1032,7 +1035,8 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
       mt::colors_adjoint_columns_kernel<N, D><<<cgrid, mt::kBlock, 0, st>>>(                     \
           grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S,                \
           vertex_offsets, num_triangle_vertices, vertices, colors, camera_positions, B, grad_sdf, \
-          grad_colors, part, list, live_count, col_live, col_slots, maps, vertex_touched);      \
+          grad_colors, part, list, live_count, col_live, col_slots, maps, vertex_touched,       \
+          clear_gradients != 0);                                                                \
     mt::colors_adjoint_ends_kernel<N, D><<<ggrid, mt::kBlock, 0, st>>>(                          \
         num_grid_vertices, num_columns, part, col_live, col_pos, grid_sdf_grad, grid_sh_grad,     \
         maps, accumulate != 0, status, accumulate != 0 ? ends_list : nullptr, ends_count);        \

@@ -75,6 +75,7 @@ def test_graph_replays_match_eager(scene):
     step.run(world, sdf, sh, target)
     ref_out = step.out.clone()
     ref_sdf = step.grid_sdf_grad.clone()
+    ref_sh = step.grid_sh_grad.clone()
     ref_loss = step.loss.clone()
     static = [t.clone() for t in (world, sdf, sh, target)]
     step.capture(*static)
@@ -83,6 +84,8 @@ def test_graph_replays_match_eager(scene):
         torch.cuda.synchronize()
         assert torch.equal(step.out, ref_out), i
         assert _rel(step.grid_sdf_grad, ref_sdf) < 1e-5, i
+        # The vertex gradient buffers are cleared by each step's adjoint, not refilled.
+        assert _rel(step.grid_sh_grad, ref_sh) < 1e-5, i
         assert abs(float(step.loss) - float(ref_loss)) <= 1e-6 * abs(float(ref_loss)), i
         # Eager work between replays.
         _ = (static[1] * 2).sum().item()
