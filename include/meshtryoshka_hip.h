@@ -184,6 +184,28 @@ int mt_create_triangle_vertices_colors_backward(
     size_t workspace_bytes, int32_t accumulate, const uint8_t* vertex_touched,
     int32_t clear_gradients, int32_t* status, mt_stream_t stream);
 
+/* SDF regularizers (replace compute_sdf_regularizers.slang:18-128, .py:26-139). Per voxel:
+ * the Eikonal loss at the voxel centre, (|(voxel_x dx, voxel_y dy, voxel_z dz)| - 1)^2 with
+ * (dx, dy, dz) the trilinear gradient there, split into eikonal_loss_pos / _neg by the sign
+ * of |.| - 1, and the curvature loss (the 7-point Laplacian at the voxel's +xyz corner over
+ * its present neighbours)^2; each f32[num_voxels]. voxel_neighbors int32[7, num_voxels],
+ * voxel_lower_corners int32[4, num_columns], voxel_upper_corners int32[4, num_voxels];
+ * (voxel_x, voxel_y, voxel_z) = (k, j, i) of the grid size. The backward adds
+ * d(sum of grad_* x loss) / d grid_sdf into grid_sdf_grad (float atomics; zero it first
+ * for a plain gradient). */
+int mt_compute_sdf_regularizers_forward(const float* grid_sdf, const int32_t* voxel_neighbors,
+                                        const int32_t* voxel_lower_corners, int64_t num_columns,
+                                        const int32_t* voxel_upper_corners, int64_t num_voxels,
+                                        int32_t voxel_x, int32_t voxel_y, int32_t voxel_z,
+                                        float* eikonal_loss_pos, float* eikonal_loss_neg,
+                                        float* curvature_loss, mt_stream_t stream);
+int mt_compute_sdf_regularizers_backward(
+    const float* grid_sdf, const int32_t* voxel_neighbors, const int32_t* voxel_lower_corners,
+    int64_t num_columns, const int32_t* voxel_upper_corners, int64_t num_voxels, int32_t voxel_x,
+    int32_t voxel_y, int32_t voxel_z, const float* grad_eikonal_loss_pos,
+    const float* grad_eikonal_loss_neg, const float* grad_curvature_loss, float* grid_sdf_grad,
+    mt_stream_t stream);
+
 /* face_counts int32[S * num_voxels] (3 x triangles) and cell_codes uint8[S, num_voxels].
  * Replaces count_triangle_faces.slang:13-50 (.py:28-63). upper_corners
  * int32[4, num_voxels]. */

@@ -2,10 +2,10 @@
 (triangle_extraction/__init__.py:1-157), running on hand-written gfx950 HIP kernels.
 
 Implemented: the voxel-structure builder (``extract_voxels`` and its three kernels), the
-multi-level-set marching cubes (``tessellate_voxels`` and its kernels), the exclusive
-scan and occupancy packing. The reference's coarse-to-fine subdivision and regularizer
-ops are exported under their names but raise ``NotImplementedError``: they are not part
-of the extract -> rasterize -> composite path.
+multi-level-set marching cubes (``tessellate_voxels`` and its kernels), the SDF
+regularizers (``compute_sdf_regularizers``), the exclusive scan and occupancy packing.
+The reference's coarse-to-fine subdivision ops are exported under their names but raise
+``NotImplementedError``: they are not part of the extract -> rasterize -> composite path.
 """
 
 import logging
@@ -18,6 +18,8 @@ from .common import pack_occupancy as pack_occupancy
 from .common import record_function
 from .common import unpack_occupancy as unpack_occupancy
 from .compute_exclusive_cumsum import compute_exclusive_cumsum
+from .compute_sdf_regularizers import SDFRegularizerLosses as SDFRegularizerLosses
+from .compute_sdf_regularizers import compute_sdf_regularizers as compute_sdf_regularizers
 from .compute_vertex_occupancy import compute_vertex_occupancy
 from .count_occupancy import count_occupancy
 from .count_triangle_faces import count_triangle_faces
@@ -38,7 +40,6 @@ def _not_in_this_build(name: str):
     return fn
 
 
-compute_sdf_regularizers = _not_in_this_build("compute_sdf_regularizers")
 dilate_occupancy = _not_in_this_build("dilate_occupancy")
 get_occupancy_mesh = _not_in_this_build("get_occupancy_mesh")
 index_vertices = _not_in_this_build("index_vertices")
