@@ -384,6 +384,91 @@ def profile_traffic(entry_point, key):
 
 
 # ------------------------------------------------------------------------------------
+# SDF regularizers (the other per-step differentiable kernel of the reference's training
+# step): forward and adjoint on the scene's voxels, HIP events on the launch stream
+# ------------------------------------------------------------------------------------
+
+
+def regularizer_benchmark(sc, cpu_sample: int = 0):
+    """ms per call of mt_compute_sdf_regularizers_{forward,backward} over all voxels, their
+    algorithmic GB/s (per voxel: 7 neighbour + 4 lower + 4 upper + 3 subvoxel corner indices
+    read, 3 losses written / 3 upstream gradients read; per grid vertex: its SDF read once,
+    and for the adjoint its gradient read and written once), and the oracle's single-thread
+    time on a sample of cpu_sample voxels scaled to all of them (0: not timed)."""
+    import torch
+
+    from meshtryoshka_amd import _native
+
+    v = sc.voxels
+    n = int(v.upper_corners.shape[1])
+    ncol = int(v.lower_corners.shape[1])
+    ng = int(sc.sdf.shape[0])
+    i, j, kp = sc.spec.occupancy.shape
+    grid = (int(i), int(j), 32 * int(kp))
+    sdf = sc.sdf.detach().contiguous()
+    dev = sdf.device
+    outs = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(3)]
+    gen = torch.Generator("cpu").manual_seed(3)
+    ups = [torch.rand(n, generator=gen).to(dev) for _ in range(3)]
+    grad = torch.zeros_like(sdf)
+    common = (sdf.data_ptr(), v.neighbors.data_ptr(), v.lower_corners.data_ptr(), ncol,
+              v.upper_corners.data_ptr(), n, grid[2], grid[1], grid[0])
+    st = torch.cuda.current_stream(dev).cuda_stream
+
+    def fwd():
+        _native.call("mt_compute_sdf_regularizers_forward", *common,
+                     *(o.data_ptr() for o in outs), st)
+
+    def bwd():
+        _native.call("mt_compute_sdf_regularizers_backward", *common,
+                     *(u.data_ptr() for u in ups), grad.data_ptr(), st)
+
+    def timed(fn, reps=10):
+        for _ in range(2):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_f, ms_b = timed(fwd), timed(bwd)
+    per_voxel = 4 * (7 + 4 + 4 + 3) + 12
+    bytes_f = n * per_voxel + 4 * ng
+    bytes_b = n * per_voxel + 12 * ng
+    res = {"voxels": n, "grid_vertices": ng,
+           "forward_ms": round(ms_f, 4), "backward_ms": round(ms_b, 4),
+           "forward_GB/s": round(bytes_f / (ms_f * 1e-3) / 1e9, 1),
+           "backward_GB/s": round(bytes_b / (ms_b * 1e-3) / 1e9, 1),
+           "algorithmic_bytes": {"forward": int(bytes_f), "backward": int(bytes_b)}}
+    res["forward_hbm_frac"] = round(res["forward_GB/s"] / HBM_PEAK_GBS, 4)
+    res["backward_hbm_frac"] = round(res["backward_GB/s"] / HBM_PEAK_GBS, 4)
+    if cpu_sample > 0:
+        import numpy as np
+        from threadpoolctl import threadpool_limits
+
+        from oracle import regularizers as orz
+
+        m = min(cpu_sample, n)
+        nb = v.neighbors[:, :m].cpu().numpy()
+        lo = v.lower_corners.cpu().numpy()
+        up = v.upper_corners[:, :m].cpu().numpy()
+        s_np = sdf.cpu().numpy()
+        g_np = [u[:m].cpu().numpy() for u in ups]
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            orz.compute_sdf_regularizers(s_np, nb, lo, up, grid)
+            orz.compute_sdf_regularizers_backward(s_np, nb, lo, up, grid, *g_np)
+            sec = time.perf_counter() - t0
+        res["cpu_oracle"] = {"seconds_per_fwd_bwd": round(sec * n / m, 3), "cores": 1,
+                             "sample": f"NumPy oracle forward + adjoint on the first {m} of {n} "
+                             f"voxels ({sec:.1f} s), scaled by voxels"}
+    return res
+
+
+# ------------------------------------------------------------------------------------
 # CPU baseline: the oracle on a bounded sample of the same step
 # ------------------------------------------------------------------------------------
 
@@ -731,6 +816,10 @@ def main():
                         "unit": "TFLOP/s", "note": "FP32 vector ops: 58 per (pixel, triangle) "
                         "test + 25 (fwd) / 60 (bwd) per hit"}
 
+    log("[bench] SDF regularizers")
+    reg = regularizer_benchmark(sc, 0 if (args.only_batched or args.no_cpu_baseline) else 2_000_000)
+    log(f"[bench] regularizers: {reg}")
+
     cpu = None
     if not (args.no_cpu_baseline or args.only_batched or not sc.uniform):
         log("[bench] CPU baseline (oracle on a bounded sample)")
@@ -782,6 +871,7 @@ def main():
                      "candidates": [[list(c.tile_shape), c.chunk, round(c.ms, 3)] for c in tuned]},
         "roofline": roof,
         "kernels": table,
+        "regularizers": reg,
         "render_modes": modes,
         "serving": serve,
         "cpu_baseline": cpu,
