@@ -206,6 +206,37 @@ int mt_compute_sdf_regularizers_backward(
     const float* grad_eikonal_loss_neg, const float* grad_curvature_loss, float* grid_sdf_grad,
     mt_stream_t stream);
 
+/* Coarse-to-fine subdivision. Packed grids are int32[I, J, KP], bit b of word (i, j, kp)
+ * being cell k = 32 kp + b.
+ * mt_write_occupancy: sets the cells of voxel_indices int32[n] (linear cell indices) in
+ * occupancy (num_words words, zeroed by the caller); indices outside the grid are ignored;
+ * min_shell / max_shell are unused, as in the reference (write_occupancy.slang:3-20,
+ * .py:25-46).
+ * mt_dilate_occupancy: box dilation by 0 < dilation < 32 cells along every axis into out
+ * (dilate_occupancy.slang:9-94, .py:27-48).
+ * mt_upscale_occupancy: cell (i, j, k) of the finer grid int32[I2, J2, KP2] (I2 >= I, ...)
+ * is occupied where the coarse cell under any of its 8 corners is, corners mapped with
+ * integer arithmetic (upscale_occupancy.slang:1-74, .py:27-53).
+ * mt_upscale_parameters: for every occupied vertex of the new vertex-occupancy grid
+ * (int32[NI, NJ, NKP] with its exclusive offsets), the old vertices (vertex-occupancy grid
+ * int32[OI, OJ, OKP], offsets, SDF f32[old_num_vertices], SH f32[num_sh, old_num_vertices,
+ * 3]) around its position, trilinearly weighted over the occupied ones and normalised,
+ * into new_sdf f32[new_num_vertices] and new_sh f32[num_sh, new_num_vertices, 3]
+ * (upscale_parameters.slang:54-198, .py:34-99). num_sh in {1, 4, 9, 16}. */
+int mt_write_occupancy(const int32_t* voxel_indices, int64_t n, int32_t* occupancy,
+                       int64_t num_words, int32_t min_shell, int32_t max_shell,
+                       mt_stream_t stream);
+int mt_dilate_occupancy(const int32_t* occupancy, int32_t I, int32_t J, int32_t KP,
+                        int32_t dilation, int32_t* out, mt_stream_t stream);
+int mt_upscale_occupancy(const int32_t* occupancy, int32_t I, int32_t J, int32_t KP,
+                         int32_t* out, int32_t I2, int32_t J2, int32_t KP2, mt_stream_t stream);
+int mt_upscale_parameters(const int32_t* old_vertex_occupancy, const int32_t* old_vertex_offsets,
+                          int32_t OI, int32_t OJ, int32_t OKP, const float* old_sdf,
+                          const float* old_sh, int64_t old_num_vertices, int32_t num_sh,
+                          const int32_t* new_vertex_occupancy, const int32_t* new_vertex_offsets,
+                          int32_t NI, int32_t NJ, int32_t NKP, float* new_sdf, float* new_sh,
+                          int64_t new_num_vertices, mt_stream_t stream);
+
 /* face_counts int32[S * num_voxels] (3 x triangles) and cell_codes uint8[S, num_voxels].
  * Replaces count_triangle_faces.slang:13-50 (.py:28-63). upper_corners
  * int32[4, num_voxels]. */

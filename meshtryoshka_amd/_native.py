@@ -104,6 +104,12 @@ SIGNATURES = {
          c_p, c_p, c_p, c_p, c_sz, c_i32, c_p, c_i32, c_p, c_p],
     ),
     "mt_count_triangle_faces": (c_i32, [c_p, c_p, c_i64, c_p, c_i64, c_p, c_i32, c_p, c_p, c_p]),
+    "mt_write_occupancy": (c_i32, [c_p, c_i64, c_p, c_i64, c_i32, c_i32, c_p]),
+    "mt_dilate_occupancy": (c_i32, [c_p, c_i32, c_i32, c_i32, c_i32, c_p, c_p]),
+    "mt_upscale_occupancy": (c_i32, [c_p, c_i32, c_i32, c_i32, c_p, c_i32, c_i32, c_i32, c_p]),
+    "mt_upscale_parameters": (
+        c_i32, [c_p, c_p, c_i32, c_i32, c_i32, c_p, c_p, c_i64, c_i32, c_p, c_p, c_i32, c_i32,
+                c_i32, c_p, c_p, c_i64, c_p]),
     "mt_compute_sdf_regularizers_forward": (
         c_i32, [c_p, c_p, c_p, c_i64, c_p, c_i64, c_i32, c_i32, c_i32, c_p, c_p, c_p, c_p]),
     "mt_compute_sdf_regularizers_backward": (

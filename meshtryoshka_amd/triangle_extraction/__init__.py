@@ -3,9 +3,10 @@
 
 Implemented: the voxel-structure builder (``extract_voxels`` and its three kernels), the
 multi-level-set marching cubes (``tessellate_voxels`` and its kernels), the SDF
-regularizers (``compute_sdf_regularizers``), the exclusive scan and occupancy packing.
-The reference's coarse-to-fine subdivision ops are exported under their names but raise
-``NotImplementedError``: they are not part of the extract -> rasterize -> composite path.
+regularizers (``compute_sdf_regularizers``), the coarse-to-fine subdivision ops
+(``write_occupancy``, ``dilate_occupancy``, ``upscale_occupancy``, ``upscale_parameters``,
+``index_vertices``), the exclusive scan and occupancy packing. ``get_occupancy_mesh`` (a
+visualisation helper) is exported under its name but raises ``NotImplementedError``.
 """
 
 import logging
@@ -27,6 +28,12 @@ from .count_triangle_vertices import count_triangle_vertices
 from .create_triangle_faces import create_triangle_faces
 from .create_triangle_vertices import CreateTriangleVertices, create_triangle_vertices
 from .create_voxels import Voxels, create_voxels
+from .subdivision import UpscaledParameters as UpscaledParameters
+from .subdivision import dilate_occupancy as dilate_occupancy
+from .subdivision import index_vertices as index_vertices
+from .subdivision import upscale_occupancy as upscale_occupancy
+from .subdivision import upscale_parameters as upscale_parameters
+from .subdivision import write_occupancy as write_occupancy
 
 
 def _not_in_this_build(name: str):
@@ -40,17 +47,7 @@ def _not_in_this_build(name: str):
     return fn
 
 
-dilate_occupancy = _not_in_this_build("dilate_occupancy")
 get_occupancy_mesh = _not_in_this_build("get_occupancy_mesh")
-index_vertices = _not_in_this_build("index_vertices")
-upscale_occupancy = _not_in_this_build("upscale_occupancy")
-upscale_parameters = _not_in_this_build("upscale_parameters")
-write_occupancy = _not_in_this_build("write_occupancy")
-
-
-class UpscaledParameters(NamedTuple):
-    signed_distances: Tensor
-    spherical_harmonics: Tensor
 
 
 @record_function("extract_voxels")
