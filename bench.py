@@ -469,6 +469,65 @@ def regularizer_benchmark(sc, cpu_sample: int = 0):
 
 
 # ------------------------------------------------------------------------------------
+# Coarse-to-fine subdivision (the reference's schedule step, tesseract.py:436-481): mark the
+# cells that hold triangles, dilate by 4, upscale the parameters to twice the resolution
+# ------------------------------------------------------------------------------------
+
+
+def subdivision_benchmark(device, resolution: int = 512, spheres: int = 64):
+    """ms per call of write_occupancy / dilate_occupancy / upscale_occupancy /
+    upscale_parameters on a sphere-field scene at ``resolution``^3 upscaled to twice that,
+    with their algorithmic GB/s."""
+    import torch
+
+    from meshtryoshka_amd import triangle_extraction as te
+    from meshtryoshka_amd import workloads
+
+    spec = workloads.sphere_field(seed=11, num_spheres=spheres, resolution=resolution,
+                                  num_cameras=1, image_shape=(64, 64), device=device)
+    occ = spec.occupancy.to(device)
+    vox = te.extract_voxels(occ)
+    _, sdf, sh = workloads.finish_fields(spec, vox.vertices)
+    sdf, sh = sdf.to(device), sh.to(device)
+    with torch.no_grad():
+        tri = te.tessellate_voxels(sdf, vox.vertices, sh, vox.neighbors, vox.lower_corners,
+                                   vox.upper_corners, vox.indices, spec.level_sets.to(device))
+    i, j, kp = occ.shape
+    target = (2 * i, 2 * j, 64 * kp)
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            out = fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps, out
+
+    faces_voxels = tri.voxel_indices
+    ms_w, marked = timed(lambda: te.write_occupancy(faces_voxels, occ.shape, 0, 1))
+    ms_d, dilated = timed(lambda: te.dilate_occupancy(marked, 4))
+    ms_u, _ = timed(lambda: te.upscale_occupancy(occ & dilated, target))
+    ms_p, up = timed(lambda: te.upscale_parameters(sdf, sh, occ, dilated, target), reps=3)
+    words, words2 = occ.numel(), 8 * occ.numel()
+    nsh = int(sh.shape[0])
+    v_new = int(up.signed_distances.shape[0])
+    bytes_ = {"write": 4 * faces_voxels.numel() + 4 * words, "dilate": 8 * words,
+              "upscale_occupancy": 4 * words + 4 * words2,
+              # vertex grids, old and new parameters; the vertex-occupancy and offset
+              # passes of both grids are inside the call
+              "upscale_parameters": 8 * words + 8 * words2 + 4 * (1 + 3 * nsh) * (
+                  int(sdf.shape[0]) + v_new)}
+    ms = {"write": ms_w, "dilate": ms_d, "upscale_occupancy": ms_u, "upscale_parameters": ms_p}
+    return {"grid": [int(i), int(j), 32 * int(kp)], "target": list(target),
+            "cells_marked": int(faces_voxels.numel()), "new_vertices": v_new,
+            "ops": {k: {"ms": round(v, 4), "GB/s": round(bytes_[k] / (v * 1e-3) / 1e9, 1)}
+                    for k, v in ms.items()}}
+
+
+# ------------------------------------------------------------------------------------
 # CPU baseline: the oracle on a bounded sample of the same step
 # ------------------------------------------------------------------------------------
 
@@ -829,6 +888,11 @@ def main():
             cpu = cpu_baseline(sc, args, live)
 
     workload = workload_description(args, sc)
+    sub = None
+    if not args.only_batched:
+        log("[bench] subdivision")
+        sub = subdivision_benchmark(device)
+        log(f"[bench] subdivision: {sub}")
     serve = None
     if not (args.no_serving or args.only_batched):
         # Free the training step's arenas and the scene before loading the served scenes.
@@ -872,6 +936,7 @@ def main():
         "roofline": roof,
         "kernels": table,
         "regularizers": reg,
+        "subdivision": sub,
         "render_modes": modes,
         "serving": serve,
         "cpu_baseline": cpu,

@@ -18,7 +18,11 @@ __global__ void __launch_bounds__(kBlock) write_occupancy_kernel(const int32_t* 
   if (t >= n) return;
   const int32_t v = voxel_indices[t];
   if (v < 0 || (int64_t(v) >> 5) >= num_words) return;  // not a cell of this grid
-  atomicOr(occupancy + (v >> 5), 1u << (v & 31));
+  // Many triangles share a cell: an atomic only where the bit is not yet visible.
+  const uint32_t bit = 1u << (v & 31);
+  uint32_t* word = occupancy + (v >> 5);
+  if ((__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit) == 0)
+    atomicOr(word, bit);
 }
 
 // OR of the (2d + 1)^2 words around (i, j) in word column kp (dilate_occupancy.slang:28-36).
@@ -121,17 +125,22 @@ __device__ __forceinline__ int64_t vertex_index(const uint32_t* vocc, const int3
   return int64_t(voff[w]) + __popc(chunk & ((1u << bit) - 1u));
 }
 
-// One thread per word of the new vertex-occupancy grid: every occupied new vertex gets the
-// occupancy-weighted trilinear interpolation of the old vertices around its position
-// (upscale_parameters.slang:54-198), weights and sums in the reference's order.
+// One thread per bit of the new vertex-occupancy grid (neighbouring lanes, neighbouring
+// vertices along k): every occupied new vertex gets the occupancy-weighted trilinear
+// interpolation of the old vertices around its position (upscale_parameters.slang:54-198),
+// weights and sums in the reference's order.
 template <int kNsh>
 __global__ void __launch_bounds__(kBlock) upscale_parameters_kernel(
     const uint32_t* old_vocc, const int32_t* old_voff, int OI, int OJ, int OKP,
     const float* old_sdf, const float* old_sh, int64_t old_v, const uint32_t* new_vocc,
     const int32_t* new_voff, int NI, int NJ, int NKP, float* new_sdf, float* new_sh,
     int64_t new_v) {
-  const int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t w = tid >> 5;
+  const int b = static_cast<int>(tid & 31);
   if (w >= int64_t(NI) * NJ * NKP) return;
+  const uint32_t chunk = new_vocc[w];
+  if (((chunk >> b) & 1u) == 0) return;
   const int kp = static_cast<int>(w % NKP);
   const int j = static_cast<int>((w / NKP) % NJ);
   const int i = static_cast<int>(w / (int64_t(NKP) * NJ));
@@ -139,10 +148,8 @@ __global__ void __launch_bounds__(kBlock) upscale_parameters_kernel(
   const float inv_j = 1.0f / float(NJ - 1);
   const float inv_k = 1.0f / float(32 * (NKP - 1));
   const float scale_x = float((OKP - 1) * 32), scale_y = float(OJ - 1), scale_z = float(OI - 1);
-  int64_t out = new_voff[w];
-  uint32_t chunk = new_vocc[w];
-  for (int b = 0; b < 32; ++b, chunk >>= 1) {
-    if ((chunk & 1u) == 0) continue;
+  const int64_t out = int64_t(new_voff[w]) + __popc(chunk & ((1u << b) - 1u));
+  {
     const int k = 32 * kp + b;
     const float x = float(k) * inv_k, y = float(j) * inv_j, z = float(i) * inv_i;
     const float xu = x * scale_x, yu = y * scale_y, zu = z * scale_z;
@@ -174,7 +181,6 @@ __global__ void __launch_bounds__(kBlock) upscale_parameters_kernel(
         for (int c = 0; c < 3; ++c) new_sh[(int64_t(s) * new_v + out) * 3 + c] = sh[s][c] * m;
       new_sdf[out] = sd * m;
     }
-    ++out;
   }
 }
 
@@ -241,7 +247,7 @@ extern "C" int mt_upscale_parameters(const int32_t* old_vertex_occupancy,
   const auto* ov = reinterpret_cast<const uint32_t*>(old_vertex_occupancy);
   const auto* nv = reinterpret_cast<const uint32_t*>(new_vertex_occupancy);
 #define MT_UPS(N)                                                                              \
-  mt::upscale_parameters_kernel<N><<<mt::blocks_for(words, mt::kBlock), mt::kBlock, 0, st>>>( \
+  mt::upscale_parameters_kernel<N><<<mt::blocks_for(32 * words, mt::kBlock), mt::kBlock, 0, st>>>( \
       ov, old_vertex_offsets, OI, OJ, OKP, old_sdf, old_sh, old_num_vertices, nv,             \
       new_vertex_offsets, NI, NJ, NKP, new_sdf, new_sh, new_num_vertices)
   switch (num_sh) {
