@@ -762,42 +762,50 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
     }
     __syncthreads();
     const int total = s_total;
-    for (int i = threadIdx.x; i < total; i += kBlock) {
+    // One thread per triangle: its three corners are consecutive corners of one voxel (the
+    // block's range starts on a triangle, as every voxel contributes 3 per triangle).
+    for (int t = threadIdx.x; 3 * t < total; t += kBlock) {
+      const int i = 3 * t;
       int lo = 0, hi = last;  // last voxel whose first corner is <= i (it has corners)
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (s_start[mid] <= i) lo = mid; else hi = mid - 1;
       }
-      const int q = i - s_start[lo];
+      const int q0 = i - s_start[lo];
       const int code = s_code[lo];
-      const int e = MT_MC_CASE_EDGES[MT_MC_CASE_START[code] + q];
-      const int slot = MT_MC_EDGE_OWNER[e][0];
-      const int axis = MT_MC_EDGE_OWNER[e][1];
       const int64_t vv = v_first + lo;
-      const int64_t owner = (slot < 0) ? vv : neighbors[slot * n + vv];
-      const int64_t vo = s * ncol + owner;
-      const int32_t start = voffsets[vo];
-      int32_t cnt = voffsets[vo + 1] - start;
-      int32_t target = -1;
-      if (int64_t(start) + cnt > vcap) {  // vertices past the capacity were never written
-        if (overflow) atomicOr(overflow, 1);
-        cnt = 0;
-      }
-      for (int k = 0; k < cnt; ++k) {
-        if (vtypes[start + k] == axis) {
-          target = start + k;
-          break;
+      int32_t target[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int e = MT_MC_CASE_EDGES[MT_MC_CASE_START[code] + q0 + c];
+        const int slot = MT_MC_EDGE_OWNER[e][0];
+        const int axis = MT_MC_EDGE_OWNER[e][1];
+        const int64_t owner = (slot < 0) ? vv : neighbors[slot * n + vv];
+        const int64_t vo = s * ncol + owner;
+        const int32_t start = voffsets[vo];
+        int32_t cnt = voffsets[vo + 1] - start;
+        target[c] = -1;
+        if (int64_t(start) + cnt > vcap) {  // vertices past the capacity were never written
+          if (overflow) atomicOr(overflow, 1);
+          cnt = 0;
+        }
+        for (int k = 0; k < cnt; ++k) {
+          if (vtypes[start + k] == axis) {
+            target[c] = start + k;
+            break;
+          }
         }
       }
-      const int64_t pos = base + i;
-      const int64_t tri = pos / 3;
-      const int corner = 2 - static_cast<int>(pos % 3);
+      const int64_t tri = (base + i) / 3;
       if (tri >= fcap) {
         if (overflow) atomicOr(overflow, 1);
         continue;
       }
-      faces[3 * tri + corner] = target;
-      if (corner == 0) voxel_indices[tri] = indices[vv];
+      // Corner c of the case lands at position 2 - c (the reference's reversed winding).
+      faces[3 * tri + 0] = target[2];
+      faces[3 * tri + 1] = target[1];
+      faces[3 * tri + 2] = target[0];
+      voxel_indices[tri] = indices[vv];
     }
     __syncthreads();  // the next level set reuses the LDS lists
   }
