@@ -323,10 +323,13 @@ def algorithmic_bytes(z):
         # Fused vertex + colour kernels of the batched step (grid-vertex count ~ N_lc).
         "mt_create_triangle_vertices_colors_forward": g * (n_lc * (16 + 4 + 12 + nb) + 4 * s * n_lc
                                                            + vt * 17) + b * vt * 12,
-        "mt_create_triangle_vertices_colors_backward": g * (n_lc * (16 + 4) + 4 * s * n_lc
-                                                            + vt * (4 + 12) + n_lc * (4 + nb))
-        + 24 * b * vt,
-        "zero_fill": 4 * (3 * b * vt + g * vt) + 16 * tt,
+        # Columns, offsets and flags once per group; the flagged vertices' position, SDF and
+        # per-camera colour / colour-gradient reads plus the clearing writes; the grid
+        # gradients written once.
+        "mt_create_triangle_vertices_colors_backward": g * (n_lc * (16 + 4) + 4 * s * n_lc + vt)
+        + z.get("V_touched", vt) * (12 + 8 + 36 * b) + n_lc * (4 + nb),
+        # Per group the vertex flags and tile ranges; once per step the grid gradients.
+        "zero_fill": g * vt + 16 * tt + n_lc * (4 + nb),
     }
 
 
@@ -840,6 +843,8 @@ def main():
         sizes["H"], sizes["W"] = sc.shapes[0]
         sizes["T"] = step.T
     sizes["sort_passes"] = sort_passes(step.sorted_keys[: live["K_last_group"]])
+    # Vertices that received a gradient (last group; the colour adjoint's work unit).
+    sizes["V_touched"] = int(step.touched[: live["V_t"]].to(torch.int64).sum())
     alg = algorithmic_bytes(sizes)
     table = {}
     for name in timer.totals:
