@@ -94,6 +94,92 @@ def tessellate_scene(voxels, grid_vertices, grid_sdf, grid_sh, level_sets, sh_de
     )
 
 
+# ------------------------------------------------------------------------------------
+# Scenes on disk: the reference trainer's checkpoint layout (trainer.py:111-137): a dict with
+# "model" = the model's state dict, whose tesseract entries hold the packed occupancy, the
+# raw per-vertex SDF and the SH split into the DC band and the rest (tesseract.py:161-214).
+# The reference keeps the grid shape, level sets and sharpness in its config; a checkpoint
+# written here carries them in an extra "scene" entry, and for a reference checkpoint they
+# are passed in. Single-component tesseracts (one cubic grid) only.
+# ------------------------------------------------------------------------------------
+
+
+def save_scene_checkpoint(path, occupancy, signed_distances, spherical_harmonics, level_sets,
+                          sh_degree, sharpness, background, step: int = 0) -> None:
+    """Writes a scene (packed occupancy int32[i, j, k/32], grid SDF f32[V], grid SH
+    f32[NSH, V, 3]) in the checkpoint layout above."""
+    i, j, kp = occupancy.shape
+    sh = spherical_harmonics.detach().cpu()
+    model = {
+        "tesseract.step": torch.tensor(step, dtype=torch.int32),
+        "tesseract.occupancy": occupancy.detach().cpu().reshape(-1).to(torch.int32),
+        "tesseract.signed_distances_raw": signed_distances.detach().cpu().to(torch.float32),
+        "tesseract.sh_rgb": sh[:1].to(torch.float32),
+        "tesseract.sh_rest": sh[1:].to(torch.float32) if sh.shape[0] > 1
+        else torch.empty([], dtype=torch.float32),
+        "background": torch.as_tensor(background, dtype=torch.float32).cpu(),
+        "step": torch.tensor([step], dtype=torch.int32),
+    }
+    scene = {"grid_shape": [int(i), int(j), 32 * int(kp)],
+             "level_sets": [float(x) for x in torch.as_tensor(level_sets).cpu().tolist()],
+             "sh_degree": int(sh_degree), "sharpness": float(sharpness)}
+    torch.save({"model": model, "step": step, "scene": scene}, path)
+
+
+def read_scene_checkpoint(path, grid_shape=None, level_sets=None, sh_degree=None,
+                          sharpness=None) -> dict:
+    """The scene tensors of a checkpoint (loaded with ``weights_only=True``: nothing in the
+    file is executed), on the CPU. Arguments override or supply what the file lacks."""
+    content = torch.load(path, map_location="cpu", weights_only=True)
+    model = content["model"]
+    extra = content.get("scene", {})
+    grid_shape = tuple(grid_shape or extra.get("grid_shape") or ())
+    if len(grid_shape) != 3 or grid_shape[2] % 32:
+        raise ValueError("the grid shape (i, j, k with k % 32 == 0) is needed: reference "
+                         "checkpoints keep it in their config")
+    i, j, k = grid_shape
+    occupancy = model["tesseract.occupancy"].to(torch.int32)
+    if occupancy.numel() != i * j * (k // 32):
+        raise ValueError(f"the occupancy holds {occupancy.numel()} words, a {grid_shape} grid "
+                         f"{i * j * (k // 32)}: multi-component tesseracts are not supported")
+    sdf = model["tesseract.signed_distances_raw"].to(torch.float32)
+    sh = model["tesseract.sh_rgb"].to(torch.float32)
+    rest = model.get("tesseract.sh_rest")
+    if rest is not None and rest.dim() == 3 and rest.shape[0] > 0:
+        sh = torch.cat([sh, rest.to(torch.float32)], dim=0)
+    if sh_degree is None:
+        sh_degree = extra.get("sh_degree", int(round(sh.shape[0] ** 0.5)) - 1)
+    if level_sets is None:
+        level_sets = extra.get("level_sets", [0.0])
+    if sharpness is None:
+        sharpness = extra.get("sharpness")
+        if sharpness is None:
+            raise ValueError("the sharpness is needed: reference checkpoints keep it in their config")
+    return {"occupancy": occupancy.view(i, j, k // 32), "signed_distances": sdf,
+            "spherical_harmonics": sh, "level_sets": torch.as_tensor(level_sets, dtype=torch.float32),
+            "sh_degree": int(sh_degree), "sharpness": float(sharpness),
+            "background": model.get("background", torch.zeros(3)).to(torch.float32)}
+
+
+def load_scene_checkpoint(path, device, vertex_transform=None, **overrides) -> ResidentScene:
+    """A resident scene from a checkpoint: voxels extracted on ``device``, shells tessellated
+    at the level sets. ``vertex_transform`` maps the tessellated vertices (unit-cube grid
+    coordinates) to world space, as the reference's contraction does (tessellator.py:82-83)."""
+    from . import triangle_extraction as te
+
+    c = read_scene_checkpoint(path, **overrides)
+    vox = te.extract_voxels(c["occupancy"].to(device))
+    if vox.vertices.shape[0] != c["signed_distances"].shape[0]:
+        raise ValueError(f"{c['signed_distances'].shape[0]} SDF values for "
+                         f"{vox.vertices.shape[0]} grid vertices")
+    scene = tessellate_scene(vox, vox.vertices, c["signed_distances"].to(device),
+                             c["spherical_harmonics"].to(device), c["level_sets"].to(device),
+                             c["sh_degree"], c["sharpness"], c["background"].to(device))
+    if vertex_transform is not None:
+        scene.vertices = vertex_transform(scene.vertices).contiguous()
+    return scene
+
+
 @dataclass
 class BatchResult:
     images: list  # per camera f32[3, H_b, W_b] (background composited, clipped)

@@ -90,3 +90,31 @@ def test_budget_limits_inflight_work(scenes):
     assert len(server.inflight) == 1 and server.queued == 4
     server.drain()
     assert len(server.completed) == 5
+
+
+def test_scene_checkpoint_renders_the_same(tmp_path):
+    """A scene written with save_scene_checkpoint and loaded back (voxels extracted and shells
+    tessellated on the GPU, unit-cube vertices mapped to world space by the same affine map)
+    renders the same image as the scene built in memory."""
+    dev = torch.device("cuda")
+    spec = workloads.sphere_field(seed=3, num_spheres=5, resolution=64, num_cameras=1,
+                                  image_shape=(48, 64), device=dev)
+    occ = spec.occupancy.to(dev)
+    vox = te.extract_voxels(occ)
+    _, sdf, sh = workloads.finish_fields(spec, vox.vertices)
+    to_world = lambda v: 2 * v - 1  # noqa: E731 (the generator's world frame)
+    mem = serving.tessellate_scene(vox, vox.vertices, sdf.to(dev), sh.to(dev),
+                                   spec.level_sets.to(dev), spec.sh_degree, spec.sharpness,
+                                   spec.background)
+    mem.vertices = to_world(mem.vertices).contiguous()
+    path = tmp_path / "scene.ckpt"
+    serving.save_scene_checkpoint(path, occ, sdf, sh, spec.level_sets, spec.sh_degree,
+                                  spec.sharpness, spec.background)
+    disk = serving.load_scene_checkpoint(path, dev, vertex_transform=to_world)
+    assert torch.equal(disk.faces, mem.faces) and torch.equal(disk.vertices, mem.vertices)
+    e, k, shape = _poses(1, 21, (48, 64))[0]
+    imgs = [serving.render_batch(s, e[None].cuda(), k[None].cuda(), [shape]).images[0]
+            for s in (mem, disk)]
+    torch.cuda.synchronize()
+    assert torch.equal(imgs[0], imgs[1])
+    assert float((imgs[0] < 0.999).float().mean()) > 0.01  # the scene is in view
