@@ -71,3 +71,15 @@ def test_model_weighting_and_unused_outputs(scene):
                                                 np.zeros(n))
     err = np.abs(x.grad.cpu().numpy().astype(np.float64) - ref).max()
     assert err <= 1e-5 * np.abs(ref).max(), err
+
+
+def test_no_voxels():
+    dev = torch.device("cuda")
+    sdf = torch.zeros(5, device=dev, requires_grad=True)
+    nb = torch.zeros((7, 0), dtype=torch.int32, device=dev)
+    lo = torch.zeros((4, 0), dtype=torch.int32, device=dev)
+    up = torch.zeros((4, 0), dtype=torch.int32, device=dev)
+    out = te.compute_sdf_regularizers(sdf, nb, lo, up, (1, 1, 32))
+    assert all(o.shape == (0,) for o in out)
+    sum(o.sum() for o in out).backward()
+    assert torch.equal(sdf.grad, torch.zeros(5, device=dev))

@@ -69,3 +69,18 @@ def test_index_vertices_on_device():
     pts = base[rng.integers(0, 300, 2000)]
     got = te.index_vertices(torch.from_numpy(pts).cuda(), 1e-6)
     assert np.array_equal(got.cpu().numpy(), osd.index_vertices(pts, 1e-6))
+
+
+def test_empty_inputs():
+    dev = torch.device("cuda")
+    empty_idx = torch.empty(0, dtype=torch.int32, device=dev)
+    assert not bool(te.write_occupancy(empty_idx, (2, 3, 1), 0, 1).any())
+    zero = torch.zeros((4, 4, 2), dtype=torch.int32, device=dev)
+    assert not bool(te.dilate_occupancy(zero, 3).any())
+    assert not bool(te.upscale_occupancy(zero, (8, 8, 128)).any())
+    full = torch.full((4, 4, 2), -1, dtype=torch.int32, device=dev)
+    v = (5 * 5 * 65)
+    up = te.upscale_parameters(torch.zeros(v, device=dev), torch.zeros((1, v, 3), device=dev),
+                               full, zero, (8, 8, 128))
+    assert up.signed_distances.shape == (0,) and up.spherical_harmonics.shape == (1, 0, 3)
+    assert not bool(up.occupancy.any())
