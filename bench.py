@@ -321,8 +321,9 @@ def algorithmic_bytes(z):
         "mt_evaluate_sh_backward": vt * (12 + nb) + b * vt * 12 + vt * nb,
         "mt_create_triangle_vertices_backward": n_lc * 16 + vt * (12 + 4 + nb) + 2 * n_lc * (4 + nb),
         # Fused vertex + colour kernels of the batched step (grid-vertex count ~ N_lc).
-        "mt_create_triangle_vertices_colors_forward": g * (n_lc * (16 + 4 + 12 + nb) + 4 * s * n_lc
-                                                           + vt * 17) + b * vt * 12,
+        # The first group also writes the vertices; later groups write colours only.
+        "mt_create_triangle_vertices_colors_forward": g * (n_lc * (16 + 4 + 12 + nb) + 4 * s * n_lc)
+        + vt * 17 + b * vt * 12,
         # Columns, offsets and flags once per group; the flagged vertices' position, SDF and
         # per-camera colour / colour-gradient reads plus the clearing writes; the grid
         # gradients written once.

@@ -261,14 +261,15 @@ class BatchedTrainingStep:
             self.ranges[: g["tiles"]].zero_()
             if hook is not None:
                 hook("zero_fill")
-            # Vertices and the group's colours in one pass (later groups rewrite the same
-            # positions); the per-vertex SH array is never stored.
+            # Vertices and the first group's colours in one pass, later groups' colours alone
+            # (the positions are the same); the per-vertex SH array is never stored.
             call("mt_create_triangle_vertices_colors_forward", grid_vertices.data_ptr(),
                  grid_sdf.data_ptr(), grid_sh.data_ptr(), self.n_grid, self.num_sh,
                  self.sh_degree, v.lower_corners.data_ptr(), self.n_col,
                  self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V,
                  self.campos[c0:].data_ptr(), n, self.tv.data_ptr(), self.td.data_ptr(),
-                 self.ttype.data_ptr(), self.colors.data_ptr(), self.overflow.data_ptr(), 3, st)
+                 self.ttype.data_ptr(), self.colors.data_ptr(), self.overflow.data_ptr(),
+                 3 if gi == 0 else 2, st)
             # Faces (first group only) and the group's projection both need only the vertices:
             # without per-launch timing they run side by side on two streams.
             overlap = gi == 0 and hook is None

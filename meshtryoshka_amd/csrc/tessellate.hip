@@ -951,7 +951,8 @@ extern "C" int mt_create_triangle_vertices_colors_forward(
       grid_vertices, grid_sdf, grid_sh, num_grid_vertices, lower_corners, num_columns,         \
       level_sets, S, vertex_offsets, num_triangle_vertices, camera_positions, B, out_vertices, \
       out_sdf, out_types, out_colors, overflow, parts)
-  switch (sh_degree) {
+  // Without colours the degree-0 instance stages no SH rows (less LDS, more blocks per CU).
+  switch ((parts & 2) != 0 && B > 0 ? sh_degree : 0) {
     case 0: MT_VCOL(0); break;
     case 1: MT_VCOL(1); break;
     case 2: MT_VCOL(2); break;
