@@ -321,6 +321,8 @@ def algorithmic_bytes(z):
         "mt_evaluate_sh_backward": vt * (12 + nb) + b * vt * 12 + vt * nb,
         "mt_create_triangle_vertices_backward": n_lc * 16 + vt * (12 + 4 + nb) + 2 * n_lc * (4 + nb),
         # Fused vertex + colour kernels of the batched step (grid-vertex count ~ N_lc).
+        # grid_sh read once, its SH rows written once (the row holds the nb used bytes).
+        "mt_pack_sh_rows": 2 * n_lc * nb,
         # The first group also writes the vertices; later groups write colours only.
         "mt_create_triangle_vertices_colors_forward": g * (n_lc * (16 + 4 + 12 + nb) + 4 * s * n_lc)
         + vt * 17 + b * vt * 12,

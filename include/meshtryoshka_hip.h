@@ -146,7 +146,8 @@ int mt_create_triangle_vertices_backward(
  * bit-identical to mt_create_triangle_vertices_forward + mt_evaluate_sh_forward. parts: 1 =
  * the vertices only, 2 = the colours only (the vertices are recomputed, not written), 3 =
  * both; split, the colours can run on another stream beside the work that needs only the
- * vertices. */
+ * vertices. Adding 4 to parts reads grid_sh as SH rows (mt_pack_sh_rows' layout, 16-byte
+ * aligned) instead of [num_sh, num_grid_vertices, 3]: the colours are the same bits. */
 int mt_create_triangle_vertices_colors_forward(
     const float* grid_vertices, const float* grid_sdf, const float* grid_sh,
     int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree, const int32_t* lower_corners,
@@ -154,6 +155,19 @@ int mt_create_triangle_vertices_colors_forward(
     int64_t num_triangle_vertices, const float* camera_positions, int32_t B, float* out_vertices,
     float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, int32_t parts,
     mt_stream_t stream);
+
+/* Floats per SH row for an SH degree (0..3; -1 otherwise): 3 (deg+1)^2 rounded up to a
+ * multiple of 4. No counterpart in the reference (a layout of this library). */
+int64_t mt_sh_row_floats(int32_t sh_degree);
+
+/* grid_sh f32[num_sh, num_grid_vertices, 3] (the reference's spherical_harmonics layout,
+ * meshtryoshka/model/tesseract.py:239) -> rows f32[num_grid_vertices, mt_sh_row_floats]:
+ * each grid vertex's first (sh_degree+1)^2 coefficients x 3 channels contiguous
+ * (coefficient-major, then channel), zero-padded. No counterpart in the reference: the
+ * fused colour pass (parts | 4) gathers one row per edge end instead of one entry from
+ * each coefficient plane. */
+int mt_pack_sh_rows(const float* grid_sh, int64_t num_grid_vertices, int32_t num_sh,
+                    int32_t sh_degree, float* rows, mt_stream_t stream);
 
 /* Workspace for mt_create_triangle_vertices_colors_backward. */
 size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(int64_t num_grid_vertices,

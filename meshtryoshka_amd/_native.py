@@ -97,6 +97,8 @@ SIGNATURES = {
         [c_p, c_p, c_p, c_i64, c_i32, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_i32, c_p,
          c_p, c_p, c_p, c_p, c_i32, c_p],
     ),
+    "mt_sh_row_floats": (c_i64, [c_i32]),
+    "mt_pack_sh_rows": (c_i32, [c_p, c_i64, c_i32, c_i32, c_p, c_p]),
     "mt_create_triangle_vertices_colors_backward_workspace_bytes": (c_sz, [c_i64, c_i64, c_i32]),
     "mt_create_triangle_vertices_colors_backward": (
         c_i32,

@@ -154,6 +154,9 @@ class BatchedTrainingStep:
         self.ranges = torch.zeros((max(g["tiles"] for g in self.groups), 2), **i64)
         self.campos = torch.zeros((self.B, 3), **f32)
         self.colors = torch.zeros((G, V, 3), **f32)
+        # Each grid vertex's SH coefficients as one contiguous row, repacked every step.
+        row = _native.query("mt_sh_row_floats", self.sh_degree)
+        self.sh_rows = torch.empty((self.n_grid, row), **f32)
         self.out_flat = torch.zeros(4 * self.num_pixels, **f32)
         self.grad_flat = torch.zeros_like(self.out_flat)
         self.loss = torch.zeros(1, **f32)
@@ -248,6 +251,8 @@ class BatchedTrainingStep:
         call("mt_exclusive_scan_i32", self.face_offsets.data_ptr(), self.face_offsets.numel(),
              self.scan_ws.data_ptr(), self.scan_ws.numel(), st)
         call("mt_camera_positions", self.extrinsics.data_ptr(), self.B, self.campos.data_ptr(), st)
+        call("mt_pack_sh_rows", grid_sh.data_ptr(), self.n_grid, self.num_sh, self.sh_degree,
+             self.sh_rows.data_ptr(), st)
         # The grid gradients start from zero and every camera group adds to them, so the
         # adjoint touches only the grid vertices that receive a gradient.
         self.grid_sdf_grad.zero_()
@@ -264,12 +269,12 @@ class BatchedTrainingStep:
             # Vertices and the first group's colours in one pass, later groups' colours alone
             # (the positions are the same); the per-vertex SH array is never stored.
             call("mt_create_triangle_vertices_colors_forward", grid_vertices.data_ptr(),
-                 grid_sdf.data_ptr(), grid_sh.data_ptr(), self.n_grid, self.num_sh,
+                 grid_sdf.data_ptr(), self.sh_rows.data_ptr(), self.n_grid, self.num_sh,
                  self.sh_degree, v.lower_corners.data_ptr(), self.n_col,
                  self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V,
                  self.campos[c0:].data_ptr(), n, self.tv.data_ptr(), self.td.data_ptr(),
                  self.ttype.data_ptr(), self.colors.data_ptr(), self.overflow.data_ptr(),
-                 3 if gi == 0 else 2, st)
+                 (3 if gi == 0 else 2) | 4, st)
             # Faces (first group only) and the group's projection both need only the vertices:
             # without per-launch timing they run side by side on two streams.
             overlap = gi == 0 and hook is None

@@ -316,7 +316,44 @@ __global__ void __launch_bounds__(kBlock) gather_vertices_backward_kernel(
 // share a CU (256 columns: two).
 constexpr int kVcBlock = 128;
 
+// SH rows: a grid vertex's (deg + 1)^2 coefficients x 3 channels contiguous (coefficient-
+// major, then channel), zero-padded to whole float4s. The fused forward gathers one row
+// per axis-end vertex (one or two cache lines) instead of one 12-byte entry from each of
+// the (deg + 1)^2 coefficient planes of grid_sh [NSH, V, 3].
+__host__ __device__ constexpr int sh_row_floats(int deg) { return (3 * (deg + 1) * (deg + 1) + 3) / 4 * 4; }
+
+constexpr int kPackBlock = 256;
+constexpr int kPackVerts = 64;  // grid vertices per block
+
+// grid_sh [NSH, V, 3] -> rows [V, sh_row_floats(deg)] through LDS: each coefficient plane's
+// 64 x 3 floats are read contiguously, the block's 64 rows written as contiguous float4s.
 template <int kDeg>
+__global__ void __launch_bounds__(kPackBlock) pack_sh_rows_kernel(const float* gsh, int64_t ng,
+                                                                 float* rows) {
+  constexpr int kN = (kDeg + 1) * (kDeg + 1);
+  constexpr int kR = sh_row_floats(kDeg);
+  __shared__ float s[kPackVerts * kR];
+  const int64_t g0 = int64_t(blockIdx.x) * kPackVerts;
+  const int nv = static_cast<int>(ng - g0 < kPackVerts ? ng - g0 : kPackVerts);
+  for (int i = threadIdx.x; i < kN * 3 * nv; i += kPackBlock) {
+    const int k = i / (3 * nv), r = i - k * 3 * nv;  // plane k, float r of the block's range
+    s[(r / 3) * kR + 3 * k + r % 3] = gsh[(int64_t(k) * ng + g0) * 3 + r];
+  }
+  __syncthreads();
+  float4* out = reinterpret_cast<float4*>(rows + g0 * kR);
+  for (int q = threadIdx.x; q < nv * kR / 4; q += kPackBlock) {
+    const int j = 4 * q, col = j % kR;  // a float4 never spans two rows (kR % 4 == 0)
+    float4 x;
+    x.x = col + 0 < 3 * kN ? s[j + 0] : 0.0f;
+    x.y = col + 1 < 3 * kN ? s[j + 1] : 0.0f;
+    x.z = col + 2 < 3 * kN ? s[j + 2] : 0.0f;
+    x.w = col + 3 < 3 * kN ? s[j + 3] : 0.0f;
+    out[q] = x;
+  }
+}
+
+// kRows: gsh holds SH rows (pack_sh_rows_kernel's layout) instead of [NSH, V, 3] planes.
+template <int kDeg, bool kRows>
 __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
     const float* gv, const float* gsdf, const float* gsh, int64_t ng, const int32_t* lower,
     int64_t n, const float* level_sets, int S, const int32_t* offsets, int64_t nt,
@@ -324,6 +361,7 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
     int32_t* overflow, int parts) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 3 * kN + 1;  // padded: consecutive rows start in different banks
+  constexpr int kR = sh_row_floats(kDeg);
   const bool write_vertices = (parts & 1) != 0;
   const bool write_colors = (parts & 2) != 0 && B > 0;
   __shared__ float s_sh[kVcBlock * kRow];
@@ -340,11 +378,23 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
   ColumnSdf col{};
   if (valid) col = load_column(gsdf, lower, n, c);
   if (valid && write_colors) {
+    if constexpr (kRows) {
+      const float4* src = reinterpret_cast<const float4*>(gsh + int64_t(col.v0) * kR);
 #pragma unroll
-    for (int k = 0; k < kN; ++k) {
-      const float* plane = gsh + int64_t(k) * ng * 3;
+      for (int q = 0; q < kR / 4; ++q) {
+        const float4 x = src[q];
+        const float e[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-      for (int ch = 0; ch < 3; ++ch) s_sh[threadIdx.x * kRow + 3 * k + ch] = plane[3 * int64_t(col.v0) + ch];
+        for (int i = 0; i < 4; ++i)
+          if (4 * q + i < 3 * kN) s_sh[threadIdx.x * kRow + 4 * q + i] = e[i];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kN; ++k) {
+        const float* plane = gsh + int64_t(k) * ng * 3;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) s_sh[threadIdx.x * kRow + 3 * k + ch] = plane[3 * int64_t(col.v0) + ch];
+      }
     }
   }
   s_g0[threadIdx.x] = valid ? col.v0 : -1;
@@ -390,13 +440,39 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
       // The axis end's row: the next column's staged origin when that is the end vertex.
       const int nb = (lc + 1 < kVcBlock && s_g0[lc + 1] == g1) ? lc + 1 : -1;
       float coef[kN][3];
+      if constexpr (kRows) {
+        float e[kR];
+        if (nb >= 0) {
 #pragma unroll
-      for (int k = 0; k < kN; ++k) {
+          for (int i = 0; i < 3 * kN; ++i) e[i] = s_sh[nb * kRow + i];
+        } else {
+          const float4* src = reinterpret_cast<const float4*>(gsh + g1 * kR);
 #pragma unroll
-        for (int ch = 0; ch < 3; ++ch) {
-          const float a = s_sh[lc * kRow + 3 * k + ch];
-          const float b = nb >= 0 ? s_sh[nb * kRow + 3 * k + ch] : gsh[(int64_t(k) * ng + g1) * 3 + ch];
-          coef[k][ch] = a + t * (b - a);
+          for (int q = 0; q < kR / 4; ++q) {
+            const float4 x = src[q];
+            e[4 * q + 0] = x.x;
+            e[4 * q + 1] = x.y;
+            e[4 * q + 2] = x.z;
+            e[4 * q + 3] = x.w;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < kN; ++k) {
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) {
+            const float a = s_sh[lc * kRow + 3 * k + ch];
+            coef[k][ch] = a + t * (e[3 * k + ch] - a);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < kN; ++k) {
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) {
+            const float a = s_sh[lc * kRow + 3 * k + ch];
+            const float b = nb >= 0 ? s_sh[nb * kRow + 3 * k + ch] : gsh[(int64_t(k) * ng + g1) * 3 + ch];
+            coef[k][ch] = a + t * (b - a);
+          }
         }
       }
       for (int b = 0; b < B; ++b) {
@@ -937,20 +1013,32 @@ extern "C" int mt_create_triangle_vertices_colors_forward(
     int64_t num_triangle_vertices, const float* camera_positions, int32_t B, float* out_vertices,
     float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, int32_t parts,
     mt_stream_t stream) {
-  MT_REQUIRE(parts >= 1 && parts <= 3, "mt_create_triangle_vertices_colors_forward: parts %d", parts);
+  MT_REQUIRE((parts & 3) != 0 && (parts & ~7) == 0,
+             "mt_create_triangle_vertices_colors_forward: parts %d", parts);
   MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0 && B >= 0,
              "mt_create_triangle_vertices_colors_forward: negative size");
   MT_REQUIRE(sh_degree >= 0 && sh_degree <= 3 && (sh_degree + 1) * (sh_degree + 1) <= num_sh,
              "mt_create_triangle_vertices_colors_forward: degree %d needs (deg+1)^2 <= %d coefficients",
              sh_degree, num_sh);
+  MT_REQUIRE((parts & 4) == 0 || reinterpret_cast<uintptr_t>(grid_sh) % 16 == 0,
+             "mt_create_triangle_vertices_colors_forward: SH rows not 16-byte aligned");
   if (num_columns == 0 || S == 0 || num_triangle_vertices == 0) return MT_OK;
   const dim3 grid(mt::blocks_for(num_columns, mt::kVcBlock));
   hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool rows = (parts & 4) != 0;
 #define MT_VCOL(D)                                                                              \
-  mt::create_vertices_colors_kernel<D><<<grid, mt::kVcBlock, 0, st>>>(                           \
-      grid_vertices, grid_sdf, grid_sh, num_grid_vertices, lower_corners, num_columns,         \
-      level_sets, S, vertex_offsets, num_triangle_vertices, camera_positions, B, out_vertices, \
-      out_sdf, out_types, out_colors, overflow, parts)
+  do {                                                                                          \
+    if (rows)                                                                                   \
+      mt::create_vertices_colors_kernel<D, true><<<grid, mt::kVcBlock, 0, st>>>(                \
+          grid_vertices, grid_sdf, grid_sh, num_grid_vertices, lower_corners, num_columns,     \
+          level_sets, S, vertex_offsets, num_triangle_vertices, camera_positions, B,           \
+          out_vertices, out_sdf, out_types, out_colors, overflow, parts);                      \
+    else                                                                                        \
+      mt::create_vertices_colors_kernel<D, false><<<grid, mt::kVcBlock, 0, st>>>(               \
+          grid_vertices, grid_sdf, grid_sh, num_grid_vertices, lower_corners, num_columns,     \
+          level_sets, S, vertex_offsets, num_triangle_vertices, camera_positions, B,           \
+          out_vertices, out_sdf, out_types, out_colors, overflow, parts);                      \
+  } while (0)
   // Without colours the degree-0 instance stages no SH rows (less LDS, more blocks per CU).
   switch ((parts & 2) != 0 && B > 0 ? sh_degree : 0) {
     case 0: MT_VCOL(0); break;
@@ -960,6 +1048,28 @@ extern "C" int mt_create_triangle_vertices_colors_forward(
   }
 #undef MT_VCOL
   return mt::check_launch("mt_create_triangle_vertices_colors_forward");
+}
+
+extern "C" int64_t mt_sh_row_floats(int32_t sh_degree) {
+  return sh_degree < 0 || sh_degree > 3 ? -1 : mt::sh_row_floats(sh_degree);
+}
+
+extern "C" int mt_pack_sh_rows(const float* grid_sh, int64_t num_grid_vertices, int32_t num_sh,
+                               int32_t sh_degree, float* rows, mt_stream_t stream) {
+  MT_REQUIRE(num_grid_vertices >= 0, "mt_pack_sh_rows: negative size");
+  MT_REQUIRE(sh_degree >= 0 && sh_degree <= 3 && (sh_degree + 1) * (sh_degree + 1) <= num_sh,
+             "mt_pack_sh_rows: degree %d needs (deg+1)^2 <= %d coefficients", sh_degree, num_sh);
+  MT_REQUIRE(reinterpret_cast<uintptr_t>(rows) % 16 == 0, "mt_pack_sh_rows: rows not 16-byte aligned");
+  if (num_grid_vertices == 0) return MT_OK;
+  const dim3 grid(mt::blocks_for(num_grid_vertices, mt::kPackVerts));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (sh_degree) {
+    case 0: mt::pack_sh_rows_kernel<0><<<grid, mt::kPackBlock, 0, st>>>(grid_sh, num_grid_vertices, rows); break;
+    case 1: mt::pack_sh_rows_kernel<1><<<grid, mt::kPackBlock, 0, st>>>(grid_sh, num_grid_vertices, rows); break;
+    case 2: mt::pack_sh_rows_kernel<2><<<grid, mt::kPackBlock, 0, st>>>(grid_sh, num_grid_vertices, rows); break;
+    default: mt::pack_sh_rows_kernel<3><<<grid, mt::kPackBlock, 0, st>>>(grid_sh, num_grid_vertices, rows); break;
+  }
+  return mt::check_launch("mt_pack_sh_rows");
 }
 
 extern "C" size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(

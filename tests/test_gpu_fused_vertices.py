@@ -92,6 +92,29 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
                  off.data_ptr(), V, campos.data_ptr(), b, tv4.data_ptr(), td3.data_ptr(),
                  tt3.data_ptr(), col3.data_ptr(), ov.data_ptr(), 2, _stream())
     assert torch.equal(col3, col) and bool((tv4 == 7.0).all())
+    # SH rows: a permutation of grid_sh's first (deg+1)^2 planes, zero-padded; the colour
+    # pass reading them gives the same bits.
+    kn = (deg + 1) ** 2
+    rf = _native.query("mt_sh_row_floats", deg)
+    assert rf % 4 == 0 and 3 * kn <= rf < 3 * kn + 4
+    rows = torch.full((ng, rf), 7.0, **f32)
+    _native.call("mt_pack_sh_rows", sh.data_ptr(), ng, nsh, deg, rows.data_ptr(), _stream())
+    assert torch.equal(rows[:, : 3 * kn], sh[:kn].permute(1, 0, 2).reshape(ng, 3 * kn))
+    assert bool((rows[:, 3 * kn :] == 0).all())
+    for parts in (3 | 4, 2 | 4):
+        col5 = torch.full((b, V, 3), 7.0, **f32)
+        tv5, td5 = torch.full((V, 3), 7.0, **f32), torch.full((V,), 7.0, **f32)
+        tt5 = torch.full((V,), 7, dtype=torch.uint8, device=dev)
+        _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(),
+                     sdf.data_ptr(), rows.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(),
+                     n, ls.data_ptr(), s, off.data_ptr(), V, campos.data_ptr(), b, tv5.data_ptr(),
+                     td5.data_ptr(), tt5.data_ptr(), col5.data_ptr(), ov.data_ptr(), parts,
+                     _stream())
+        assert torch.equal(col5, col)
+        if parts & 1:
+            assert torch.equal(tv5, tv) and torch.equal(td5, td) and torch.equal(tt5, tt)
+        else:
+            assert bool((tv5 == 7.0).all())
 
     # Adjoint: random upstream gradients for SDF and colours.
     gen = torch.Generator("cpu").manual_seed(11)
