@@ -322,32 +322,36 @@ constexpr int kVcBlock = 128;
 // the (deg + 1)^2 coefficient planes of grid_sh [NSH, V, 3].
 __host__ __device__ constexpr int sh_row_floats(int deg) { return (3 * (deg + 1) * (deg + 1) + 3) / 4 * 4; }
 
-constexpr int kPackBlock = 256;
-constexpr int kPackVerts = 64;  // grid vertices per block
+constexpr int kPackVerts = 64;               // grid vertices per block
+constexpr int kPackBlock = 3 * kPackVerts;  // one thread per float of a plane's range
 
 // grid_sh [NSH, V, 3] -> rows [V, sh_row_floats(deg)] through LDS: each coefficient plane's
 // 64 x 3 floats are read contiguously, the block's 64 rows written as contiguous float4s.
+// The staged rows are one float longer than the rows written, so that the plane reads'
+// LDS stores spread over the banks.
 template <int kDeg>
 __global__ void __launch_bounds__(kPackBlock) pack_sh_rows_kernel(const float* gsh, int64_t ng,
                                                                  float* rows) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kR = sh_row_floats(kDeg);
-  __shared__ float s[kPackVerts * kR];
+  constexpr int kS = kR + 1;
+  __shared__ float s[kPackVerts * kS];
   const int64_t g0 = int64_t(blockIdx.x) * kPackVerts;
   const int nv = static_cast<int>(ng - g0 < kPackVerts ? ng - g0 : kPackVerts);
-  for (int i = threadIdx.x; i < kN * 3 * nv; i += kPackBlock) {
-    const int k = i / (3 * nv), r = i - k * 3 * nv;  // plane k, float r of the block's range
-    s[(r / 3) * kR + 3 * k + r % 3] = gsh[(int64_t(k) * ng + g0) * 3 + r];
-  }
+  const int r = threadIdx.x;  // float r of the block's range of each plane
+#pragma unroll
+  for (int k = 0; k < kN; ++k)
+    if (r < 3 * nv) s[(r / 3) * kS + 3 * k + r % 3] = gsh[(int64_t(k) * ng + g0) * 3 + r];
   __syncthreads();
   float4* out = reinterpret_cast<float4*>(rows + g0 * kR);
   for (int q = threadIdx.x; q < nv * kR / 4; q += kPackBlock) {
-    const int j = 4 * q, col = j % kR;  // a float4 never spans two rows (kR % 4 == 0)
+    const int row = (4 * q) / kR, col = 4 * q - row * kR;  // a float4 never spans two rows
+    const float* src = s + row * kS + col;
     float4 x;
-    x.x = col + 0 < 3 * kN ? s[j + 0] : 0.0f;
-    x.y = col + 1 < 3 * kN ? s[j + 1] : 0.0f;
-    x.z = col + 2 < 3 * kN ? s[j + 2] : 0.0f;
-    x.w = col + 3 < 3 * kN ? s[j + 3] : 0.0f;
+    x.x = col + 0 < 3 * kN ? src[0] : 0.0f;
+    x.y = col + 1 < 3 * kN ? src[1] : 0.0f;
+    x.z = col + 2 < 3 * kN ? src[2] : 0.0f;
+    x.w = col + 3 < 3 * kN ? src[3] : 0.0f;
     out[q] = x;
   }
 }
