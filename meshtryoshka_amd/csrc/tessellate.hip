@@ -869,12 +869,15 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
           if (overflow) atomicOr(overflow, 1);
           cnt = 0;
         }
-        for (int k = 0; k < cnt; ++k) {
-          if (vtypes[start + k] == axis) {
-            target[c] = start + k;
-            break;
-          }
-        }
+        // A column has at most one vertex per axis, written in axis order: its (at most
+        // three) types are read together, so the three corners' lookups overlap instead of
+        // waiting on each other load by load. Same result as a first-match search.
+        const int t0 = cnt > 0 ? vtypes[start + 0] : -1;
+        const int t1 = cnt > 1 ? vtypes[start + 1] : -1;
+        const int t2 = cnt > 2 ? vtypes[start + 2] : -1;
+        target[c] = t0 == axis ? start : t1 == axis ? start + 1 : t2 == axis ? start + 2 : -1;
+        for (int k = 3; k < cnt && target[c] < 0; ++k)  // never taken: counts are <= 3
+          if (vtypes[start + k] == axis) target[c] = start + k;
       }
       const int64_t tri = (base + i) / 3;
       if (tri >= fcap) {
