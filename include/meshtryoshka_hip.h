@@ -188,7 +188,10 @@ size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(int64_t num_g
  * mt_intersect_and_composite_backward set; vertices without the flag have zero gradients
  * and are skipped. clear_gradients = 1 zeroes the grad_sdf and grad_colors entries it
  * reads (every nonzero entry when the flags are given), so that a caller that keeps them
- * zero between steps needs no zero-fill (no counterpart in the reference). */
+ * zero between steps needs no zero-fill (no counterpart in the reference). maps_ready = 1
+ * reuses the grid-vertex maps (the first part of the workspace) that an earlier call built
+ * over the same lower_corners, num_columns and num_grid_vertices, with the workspace not
+ * written since except by such calls; 0 rebuilds them. */
 int mt_create_triangle_vertices_colors_backward(
     const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree,
     const int32_t* lower_corners, int64_t num_columns, const float* level_sets, int32_t S,
@@ -196,7 +199,7 @@ int mt_create_triangle_vertices_colors_backward(
     const float* colors, const float* camera_positions, int32_t B, float* grad_sdf,
     float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
     size_t workspace_bytes, int32_t accumulate, const uint8_t* vertex_touched,
-    int32_t clear_gradients, int32_t* status, mt_stream_t stream);
+    int32_t clear_gradients, int32_t maps_ready, int32_t* status, mt_stream_t stream);
 
 /* SDF regularizers (replace compute_sdf_regularizers.slang:18-128, .py:26-139). Per voxel:
  * the Eikonal loss at the voxel centre, (|(voxel_x dx, voxel_y dy, voxel_z dz)| - 1)^2 with

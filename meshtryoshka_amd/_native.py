@@ -103,7 +103,7 @@ SIGNATURES = {
     "mt_create_triangle_vertices_colors_backward": (
         c_i32,
         [c_p, c_i64, c_i32, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_i32, c_p,
-         c_p, c_p, c_p, c_p, c_sz, c_i32, c_p, c_i32, c_p, c_p],
+         c_p, c_p, c_p, c_p, c_sz, c_i32, c_p, c_i32, c_i32, c_p, c_p],
     ),
     "mt_count_triangle_faces": (c_i32, [c_p, c_p, c_i64, c_p, c_i64, c_p, c_i32, c_p, c_p, c_p]),
     "mt_write_occupancy": (c_i32, [c_p, c_i64, c_p, c_i64, c_i32, c_i32, c_p]),

@@ -157,6 +157,7 @@ class BatchedTrainingStep:
         # Each grid vertex's SH coefficients as one contiguous row, repacked every step.
         row = _native.query("mt_sh_row_floats", self.sh_degree)
         self.sh_rows = torch.empty((self.n_grid, row), **f32)
+        self.maps_built = False
         self.out_flat = torch.zeros(4 * self.num_pixels, **f32)
         self.grad_flat = torch.zeros_like(self.out_flat)
         self.loss = torch.zeros(1, **f32)
@@ -338,7 +339,11 @@ class BatchedTrainingStep:
                  self.td_grad.data_ptr(), self.colors_grad.data_ptr(),
                  self.grid_sdf_grad.data_ptr(), self.grid_sh_grad.data_ptr(),
                  self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(), 1, self.touched.data_ptr(), 1,
+                 int(self.maps_built),
                  self.overflow.data_ptr(), st)
+            # The adjoint's grid-vertex maps stay in its workspace (they depend only on the
+            # columns): every later call reuses them, captured replays included.
+            self.maps_built = True
 
     # ------------------------------------------------------------------------------
     def capture(self, grid_vertices, grid_sdf, grid_sh, target) -> None:

@@ -1100,7 +1100,7 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
     const float* colors, const float* camera_positions, int32_t B, float* grad_sdf,
     float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
     size_t workspace_bytes, int32_t accumulate, const uint8_t* vertex_touched,
-    int32_t clear_gradients, int32_t* status, mt_stream_t stream) {
+    int32_t clear_gradients, int32_t maps_ready, int32_t* status, mt_stream_t stream) {
   MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0 && B >= 0,
              "mt_create_triangle_vertices_colors_backward: negative size");
   MT_REQUIRE(num_sh == 1 || num_sh == 4 || num_sh == 9 || num_sh == 16,
@@ -1138,10 +1138,14 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
   int32_t* col_pos = reinterpret_cast<int32_t*>(
       reinterpret_cast<char*>(ends_list) +
       (static_cast<size_t>(num_grid_vertices + 1) * sizeof(int32_t) + 255) / 256 * 256);
-  mt::fill_maps_kernel<<<mt::blocks_for(words, 256), 256, 0, st>>>(maps, words);
-  if (num_columns > 0)
-    mt::build_maps_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
-        lower_corners, num_columns, num_grid_vertices, maps);
+  // The grid-vertex maps depend only on the columns: a caller whose workspace holds them
+  // from an earlier call over the same columns skips rebuilding them.
+  if (maps_ready == 0) {
+    mt::fill_maps_kernel<<<mt::blocks_for(words, 256), 256, 0, st>>>(maps, words);
+    if (num_columns > 0)
+      mt::build_maps_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
+          lower_corners, num_columns, num_grid_vertices, maps);
+  }
   mt::zero_counts_kernel<<<1, 1, 0, st>>>(live_count, ends_count);
   if (num_columns > 0)
     mt::colors_adjoint_mask_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(

@@ -137,7 +137,7 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
     _native.call("mt_create_triangle_vertices_colors_backward", sdf.data_ptr(), ng, nsh, deg,
                  vox.lower_corners.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V, tv2.data_ptr(),
                  col2.data_ptr(), campos.data_ptr(), b, g_d.data_ptr(), g_col.data_ptr(),
-                 gg_d2.data_ptr(), gg_sh2.data_ptr(), ws2.data_ptr(), ws2.numel(), 0, None, 0,
+                 gg_d2.data_ptr(), gg_sh2.data_ptr(), ws2.data_ptr(), ws2.numel(), 0, None, 0, 0,
                  status.data_ptr(), _stream())
     assert int(status) == 0
     # Same terms, summed per column and axis first: equal up to float summation order.
@@ -146,7 +146,8 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
 
     # Gradients only at some vertices, flagged as the composite backward flags them: the
     # adjoint that skips unflagged vertices gives the same sums; with clear_gradients it
-    # also leaves the vertex gradients all zero.
+    # also leaves the vertex gradients all zero. The flagged calls reuse the grid-vertex maps
+    # that the workspace holds from the call before (maps_ready = 1).
     flags = (torch.rand(V, generator=gen) < 0.1).to(dev)
     g_d_s = torch.where(flags, g_d, torch.zeros_like(g_d))
     g_col_s = torch.where(flags[None, :, None], g_col, torch.zeros_like(g_col))
@@ -160,7 +161,7 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
                      tv2.data_ptr(), col2.data_ptr(), campos.data_ptr(), b, gd_in.data_ptr(),
                      gc_in.data_ptr(), gd_o.data_ptr(), gs_o.data_ptr(), ws2.data_ptr(),
                      ws2.numel(), 0, None if touched is None else touched.data_ptr(), clear,
-                     status.data_ptr(), _stream())
+                     int(touched is not None), status.data_ptr(), _stream())
         outs.append((gd_o, gs_o))
         if clear:
             assert not bool(gd_in.any()) and not bool(gc_in.any())
@@ -180,7 +181,7 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
                      vox.lower_corners.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V,
                      tv2.data_ptr(), col2[c0:].data_ptr(), campos[c0:].data_ptr(), c1 - c0,
                      gd_part.data_ptr(), g_col[c0:].data_ptr(), gg_d3.data_ptr(), gg_sh3.data_ptr(),
-                     ws2.data_ptr(), ws2.numel(), i, None, 0, status.data_ptr(), _stream())
+                     ws2.data_ptr(), ws2.numel(), i, None, 0, i, status.data_ptr(), _stream())
     assert int(status) == 0
     torch.testing.assert_close(gg_d3, gg_d2, rtol=1e-5, atol=1e-6 * float(gg_d.abs().max()))
     torch.testing.assert_close(gg_sh3, gg_sh2, rtol=1e-5, atol=1e-6 * float(gg_sh.abs().max()))
@@ -225,7 +226,7 @@ def test_fused_adjoint_reports_shared_origins(case):
                  spec.sh_degree, lower.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V,
                  tv.data_ptr(), zeros_c.data_ptr(), campos.data_ptr(), b, zeros_v.data_ptr(),
                  zeros_c.data_ptr(), gg_d.data_ptr(), gg_sh.data_ptr(), ws.data_ptr(), ws.numel(), 0,
-                 None, 0, status.data_ptr(), _stream())
+                 None, 0, 0, status.data_ptr(), _stream())
     torch.cuda.synchronize()
     assert int(status) == 2
     assert bool((gg_d == 7.0).all()) and bool((gg_sh == 7.0).all())
