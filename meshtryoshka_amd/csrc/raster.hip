@@ -16,13 +16,16 @@ constexpr int kBlock = 256;
 // Projection (project_vertices.slang:51-92)
 // ------------------------------------------------------------------------------------
 
+// One thread per (vertex, camera); the B blocks of a vertex range are consecutive, so the
+// vertices are read from HBM once and from L2 by the other cameras' blocks.
 __global__ void __launch_bounds__(kBlock) project_kernel(const float* verts, int64_t V,
                                                          const float* E, const float* K, int B,
                                                          float* out_xy, float* out_d) {
-  const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int b = static_cast<int>(blockIdx.x % static_cast<unsigned>(B));
+  const int64_t v = int64_t(blockIdx.x / static_cast<unsigned>(B)) * blockDim.x + threadIdx.x;
   if (v >= V) return;
   const float x = verts[3 * v + 0], y = verts[3 * v + 1], z = verts[3 * v + 2];
-  for (int b = 0; b < B; ++b) {
+  {
     const float* e = E + 16 * b;
     const float* k = K + 9 * b;
     float cs[3];
@@ -822,7 +825,10 @@ extern "C" int mt_project_vertices(const float* vertices, int64_t V, const float
                                    float* out_depth, mt_stream_t stream) {
   MT_REQUIRE(V >= 0 && B >= 0, "mt_project_vertices: negative size");
   if (V == 0 || B == 0) return MT_OK;
-  mt::project_kernel<<<mt::blocks_for(V, mt::kBlock), mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+  const int64_t blocks = int64_t(mt::blocks_for(V, mt::kBlock)) * B;
+  MT_REQUIRE(blocks <= INT32_MAX, "mt_project_vertices: %lld vertices x %d cameras is too many for one launch",
+             static_cast<long long>(V), B);
+  mt::project_kernel<<<static_cast<unsigned>(blocks), mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
       vertices, V, extrinsics, intrinsics, B, out_xy, out_depth);
   return mt::check_launch("mt_project_vertices");
 }
