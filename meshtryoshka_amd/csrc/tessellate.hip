@@ -824,8 +824,10 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
     const int32_t* voffsets, const uint8_t* vtypes, const uint8_t* codes, int S,
     const int32_t* foffsets, int32_t* faces, int32_t* voxel_indices, int64_t fcap,
     int64_t vcap, int32_t* overflow) {
+  constexpr int kMaxTri = 5 * kBlock;  // a marching-cubes case has at most 5 triangles
   __shared__ int32_t s_start[kBlock];
   __shared__ uint8_t s_code[kBlock];
+  __shared__ uint8_t s_owner[kMaxTri];  // the block-local voxel of each of the block's triangles
   __shared__ int s_total;
   const int64_t v_first = int64_t(blockIdx.x) * kBlock;
   const int last = static_cast<int>((v_first + kBlock < n ? v_first + kBlock : n) - 1 - v_first);
@@ -836,9 +838,12 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
     if (valid) {
       const int code = codes[int64_t(s) * n + v];
       const int start = static_cast<int>(foffsets[int64_t(s) * n + v] - base);
+      const int corners = MT_MC_CASE_START[code + 1] - MT_MC_CASE_START[code];
       s_code[threadIdx.x] = static_cast<uint8_t>(code);
       s_start[threadIdx.x] = start;
-      if (int(threadIdx.x) == last) s_total = start + (MT_MC_CASE_START[code + 1] - MT_MC_CASE_START[code]);
+      for (int t = start / 3; t < (start + corners) / 3; ++t)
+        if (t < kMaxTri) s_owner[t] = static_cast<uint8_t>(threadIdx.x);
+      if (int(threadIdx.x) == last) s_total = start + corners;
     }
     __syncthreads();
     const int total = s_total;
@@ -846,10 +851,16 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
     // block's range starts on a triangle, as every voxel contributes 3 per triangle).
     for (int t = threadIdx.x; 3 * t < total; t += kBlock) {
       const int i = 3 * t;
-      int lo = 0, hi = last;  // last voxel whose first corner is <= i (it has corners)
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_start[mid] <= i) lo = mid; else hi = mid - 1;
+      int lo;  // the voxel whose corners hold corner i
+      if (t < kMaxTri) {
+        lo = s_owner[t];
+      } else {  // never taken (at most 5 triangles per voxel): binary search
+        lo = 0;
+        int hi = last;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_start[mid] <= i) lo = mid; else hi = mid - 1;
+        }
       }
       const int q0 = i - s_start[lo];
       const int code = s_code[lo];
