@@ -373,8 +373,9 @@ int mt_generate_keys_single_pass(const float* xy, const float* depths, int64_t V
 size_t mt_sort_workspace_bytes(int64_t n);
 
 /* Stable ascending sort of (signed int64 key, int32 value) pairs over all 64 key bits
- * (hand-written onesweep LSD radix sort; passes whose digit is constant are skipped on
- * the device). Inputs are not modified. If n_device is not NULL the device value
+ * (hand-written reduce-then-scan LSD radix sort: per 8-bit digit a count pass, a scan of
+ * the (digit, tile) counts and a scatter pass; passes whose digit is constant over all
+ * keys are skipped on the device). Inputs are not modified. If n_device is not NULL the device value
  * min(*n_device, n) is the number of pairs and n only sizes the launch (capacity).
  * Replaces sort_keys (sort_keys.cu:215-270, sort_keys.py:18-29). */
 int mt_sort_keys(const int64_t* keys_in, const int32_t* values_in, int64_t n, int64_t* keys_out,
@@ -471,7 +472,7 @@ int mt_image_l2_loss(const float* out, const float* background, const float* tar
 
 /* Explicit mode: first hit per (camera, shell, pixel) in depth order. tile_ranges
  * int64[B, S, T, 2]; out_uv f32[B*S, H, W, 2]; out_index int32[B*S, H, W] (-1 = miss).
- * Replaces intersect.slang:157-252 (.py:33-70). */
+ * Replaces intersect.slang:85-182 (.py:33-70). */
 int mt_intersect(const int32_t* sorted_values, const int64_t* tile_ranges, const float* xy,
                  int64_t V, const int32_t* faces, int32_t B, int32_t S, int32_t grid_rows,
                  int32_t grid_cols, int32_t tile_h, int32_t tile_w, int32_t height,
@@ -479,7 +480,7 @@ int mt_intersect(const int32_t* sorted_values, const int64_t* tile_ranges, const
 
 /* (rgb, sdf) = u a + v b + (1 - u - v) c per pixel; misses write 0.
  * uv f32[B*S, H, W, 2]; index int32[B*S, H, W]; colors f32[B, V, 3].
- * Replaces interpolate.slang:156-210 (.py:21-75). */
+ * Replaces interpolate.slang:19-76 (.py:21-75). */
 int mt_interpolate_forward(const float* uv, const int32_t* index, const int32_t* faces,
                            const float* sdf, const float* colors, int64_t V, int32_t B,
                            int32_t S, int32_t H, int32_t W, float* out_sdf, float* out_colors,
@@ -494,7 +495,7 @@ int mt_interpolate_backward(const float* uv, const int32_t* index, const int32_t
 
 /* Per-pixel front-to-back composite over S masked layers. sdf f32[P, S],
  * colors f32[P, S, 3], mask uint8[P, S]; out_colors f32[P, 3], out_trans f32[P].
- * Replaces composite_forward (composite.slang:179-221, .py:22-61). */
+ * Replaces composite_forward (composite.slang:60-104, .py:22-61). */
 int mt_composite_forward(const float* sdf, const float* colors, const uint8_t* mask, int64_t P,
                          int32_t S, const float* sharpness, float* out_colors, float* out_trans,
                          mt_stream_t stream);
@@ -503,7 +504,7 @@ int mt_composite_forward(const float* sdf, const float* colors, const uint8_t* m
  * sharpness_grad f32[1]. reference_quirk = 1 sends each delayed SDF gradient to layer
  * s - 1 and the last one to layer S - 1 as the reference does; 0 sends them to the
  * previous and the last unmasked layer. Replaces composite_backward (composite.slang:
- * 225-335, .py:63-101). */
+ * 106-218, .py:63-101). */
 int mt_composite_backward(const float* sdf, const float* colors, const uint8_t* mask, int64_t P,
                           int32_t S, const float* sharpness, const float* out_colors,
                           const float* out_trans, const float* grad_colors,

@@ -3,7 +3,7 @@
 * ``reference_gradient_quirks`` (env ``MESHTRYOSHKA_REFERENCE_QUIRKS=1``): the fused
   backward scatters each delayed sample gradient with the *next* triangle's barycentrics
   (intersect_and_composite.slang:452, 497-513) and the explicit composite backward writes
-  delayed SDF gradients to layer s - 1 / S - 1 (composite.slang:314, 330), exactly as the
+  delayed SDF gradients to layer s - 1 / S - 1 (composite.slang:197, 213), exactly as the
   reference does. Off by default: the kernels then compute the true adjoint.
 * ``composite_chunk``: triangles staged in LDS per round by the tile kernels
   (0 = one per tile pixel, the reference's coupling). Set by the autotuner.

@@ -646,7 +646,7 @@ __global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a
 }
 
 // ------------------------------------------------------------------------------------
-// Explicit mode: first hit per (camera, shell, pixel) (intersect.slang:157-252)
+// Explicit mode: first hit per (camera, shell, pixel) (intersect.slang:85-182)
 // ------------------------------------------------------------------------------------
 
 constexpr int kIntersectChunk = 256;

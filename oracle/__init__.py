@@ -18,9 +18,9 @@ it follows). It has two modes:
   float64 with torch autograd. This is the target for the image and gradient
   tolerances.
 
-How each function is pinned (closed forms, invariants, brute force) is listed in
-``oracle/PINNING.md``; every function there has at least one independent check
-in ``tests/``.
+How each function is pinned (closed forms, invariants, brute force, known answers,
+autograd) is listed in ``oracle/PINNING.md``; every function there has at least one
+independent check in ``tests/``.
 """
 
 from . import extraction, rasterization, tables  # noqa: F401

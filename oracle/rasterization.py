@@ -602,7 +602,7 @@ def fused_backward(hits: HitList, colors, sdf, faces, sharpness, out_image, grad
 
 
 def explicit_intersect(xy, faces, sorted_values, tile_ranges, tile_shape, image_shape):
-    """First hit in depth order per (camera, shell, pixel) (intersect.slang:157-252).
+    """First hit in depth order per (camera, shell, pixel) (intersect.slang:85-182).
     Returns uv f32[B, S, H, W, 2] and index i32[B, S, H, W] (-1 = miss)."""
     b, s, t, _ = tile_ranges.shape
     h, w = image_shape
@@ -638,7 +638,7 @@ def explicit_intersect(xy, faces, sorted_values, tile_ranges, tile_shape, image_
 
 
 def interpolate(uv, index, faces, sdf, colors, dtype=F32):
-    """interpolate.slang:156-210: (rgb, sdf) = u a + v b + (1 - u - v) c, miss -> 0."""
+    """interpolate.slang:19-76: (rgb, sdf) = u a + v b + (1 - u - v) c, miss -> 0."""
     b, s, h, w, _ = uv.shape
     dt = dtype
     u = uv[..., 0].astype(dt)
@@ -677,7 +677,7 @@ def interpolate_backward(uv, index, faces, num_vertices, num_cameras, grad_sdf, 
 
 
 def composite(sdf_pl, rgb_pl, mask_pl, sharpness, dtype=F32):
-    """composite.slang:179-221 per pixel over shells in order, masked."""
+    """composite.slang:60-104 per pixel over shells in order, masked."""
     p, s = sdf_pl.shape
     dt = dtype
     k = dt(sharpness)
@@ -698,11 +698,11 @@ def composite(sdf_pl, rgb_pl, mask_pl, sharpness, dtype=F32):
 
 def composite_backward(sdf_pl, rgb_pl, mask_pl, sharpness, out_color, out_trans, grad_color,
                        grad_trans, reference_quirk=False):
-    """Suffix-trick adjoint (composite.slang:225-335), float64.
+    """Suffix-trick adjoint (composite.slang:106-218), float64.
 
     The delayed SDF gradient of a shell goes to the previous *unmasked* shell; the
     reference writes it to shell_index - 1 and the last one to num_shells - 1
-    (composite.slang:314, 330), which differs when a masked shell sits in between.
+    (composite.slang:197, 213), which differs when a masked shell sits in between.
     ``reference_quirk=True`` reproduces that."""
     p, s = sdf_pl.shape
     k = float(sharpness)

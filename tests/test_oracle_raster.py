@@ -136,17 +136,22 @@ def _single_quad_scene(sdf_values, colors, k):
     for s in range(S):
         z = 1.0 + s
         base = len(verts)
+        # With identity extrinsics the corners land at screen top-left, top-right,
+        # bottom-right and bottom-left (screen y grows downwards). (TL, TR, BR) and
+        # (TL, BR, BL) have check = (b.x - a.x)(c.y - a.y) - (b.y - a.y)(c.x - a.x) > 0,
+        # which bound_triangles.slang:95-98 keeps.
         verts += [(-10, -10, z), (10, -10, z), (10, 10, z), (-10, 10, z)]
-        faces += [(base, base + 2, base + 1), (base, base + 3, base + 2)]
+        faces += [(base, base + 1, base + 2), (base, base + 2, base + 3)]
     verts = np.array(verts, F32)
     faces = np.array(faces, np.int32)
     E = np.eye(4, dtype=F32)[None]
     K = np.array([[[8, 0, 8], [0, 8, 8], [0, 0, 1]]], F32)
     xy, d = orr.project_vertices(verts, E, K)
     tmin, tmax, cnt = orr.bound_triangles(xy, d, faces, (16, 16), (1, 1), 0.2)
-    if (cnt == 0).any():  # winding faces the other way: flip
-        faces = faces[:, [0, 2, 1]].copy()
-        tmin, tmax, cnt = orr.bound_triangles(xy, d, faces, (16, 16), (1, 1), 0.2)
+    assert (cnt == 1).all()
+    # The opposite winding is back-facing: every triangle culled.
+    _, _, cnt_rev = orr.bound_triangles(xy, d, faces[:, [0, 2, 1]].copy(), (16, 16), (1, 1), 0.2)
+    assert (cnt_rev == 0).all()
     bounds = np.arange(2, 2 * S + 1, 2, dtype=np.int32)
     lay = orr.fused_layout()
     keys, vals, _ = orr.generate_keys(d, faces, bounds, tmin, tmax, cnt, 1, lay)
