@@ -65,6 +65,20 @@ size_t mt_scan_workspace_bytes(int64_t n);
 int mt_exclusive_scan_i32(int32_t* data, int64_t n, void* workspace, size_t workspace_bytes,
                           mt_stream_t stream);
 
+/* Largest n accepted by mt_exclusive_scan_i32_noalloc (2^28 elements = 65536 tiles). */
+#define MT_SCAN_NOALLOC_MAX_ELEMENTS (1LL << 28)
+
+/* mt_exclusive_scan_i32 without a workspace and without any device allocation: the
+ * look-back state (512 KiB + 256 B) lives in a device array inside the library, one copy
+ * per GPU, held until the process ends. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). n <= MT_SCAN_NOALLOC_MAX_ELEMENTS, else
+ * MT_ERR_INVALID_ARGUMENT. `stream` must belong to the current device. Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (concurrent streams or host
+ * threads) race and must be serialised by the caller. Each call clears the state it uses
+ * with a fill kernel before scanning. */
+int mt_exclusive_scan_i32_noalloc(int32_t* data, int64_t n, mt_stream_t stream);
+
 /* out[i] = sum_{j <= i} in[j] as int64. Replaces the int64 torch.cumsum of per-(camera,
  * face) tile counts in generate_keys (triangle_rasterization/generate_keys.py:48). */
 int mt_inclusive_scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, void* workspace,

@@ -73,6 +73,7 @@ SIGNATURES = {
     "mt_version": (c_i32, []),
     "mt_scan_workspace_bytes": (c_sz, [c_i64]),
     "mt_exclusive_scan_i32": (c_i32, [c_p, c_i64, c_p, c_sz, c_p]),
+    "mt_exclusive_scan_i32_noalloc": (c_i32, [c_p, c_i64, c_p]),
     "mt_inclusive_scan_i32_to_i64": (c_i32, [c_p, c_p, c_i64, c_p, c_sz, c_p]),
     "mt_compute_vertex_occupancy": (c_i32, [c_p, c_i32, c_i32, c_i32, c_p, c_p, c_p]),
     "mt_count_occupancy": (c_i32, [c_p, c_i32, c_i32, c_i32, c_p, c_p]),

@@ -5,6 +5,8 @@
 // its aggregate, then one wave walks back over the predecessors' published aggregates /
 // inclusive prefixes up to 64 tiles at a time. Every element is read once and written once:
 // 8 bytes per element for the int32 scan, 12 for int32 -> int64.
+#include <mutex>
+
 #include "common.h"
 #include "lookback.h"
 
@@ -131,6 +133,47 @@ int run_scan(const In* in, Out* out, int64_t n, void* ws, size_t ws_bytes, hipSt
   return check_launch(name);
 }
 
+// Look-back state of mt_exclusive_scan_i32_noalloc: a ticket word and one look-back word
+// per tile, in a file-scope device array instead of a caller workspace. Every GPU the
+// library runs on holds its own copy (512 KiB + 256 B) until the process ends.
+constexpr int64_t kNoAllocMaxTiles = MT_SCAN_NOALLOC_MAX_ELEMENTS / kTile;
+static_assert(kNoAllocMaxTiles * kTile == MT_SCAN_NOALLOC_MAX_ELEMENTS, "whole tiles");
+constexpr size_t kScratchBytes = 256 + kNoAllocMaxTiles * sizeof(uint64_t);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// In host code `scratch` names a host-side stand-in, not device memory: kernels must get
+// the device address, which differs per GPU. It is looked up once per device, with that
+// device current, and kept here.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_address[kMaxDevices] = {};
+
+int scratch_address(void** out, const char* name) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("%s: hipGetDevice failed: %s", name, hipGetErrorString(e));
+    return MT_ERR_HIP;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("%s: device %d is beyond the %d devices this library tracks", name, dev, kMaxDevices);
+    return MT_ERR_INVALID_ARGUMENT;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_address[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error("%s: hipGetSymbolAddress of the scan scratch array failed on device %d: %s", name,
+                dev, hipGetErrorString(e));
+      return MT_ERR_HIP;
+    }
+    g_scratch_address[dev] = p;
+  }
+  *out = g_scratch_address[dev];
+  return MT_OK;
+}
+
 }  // namespace
 
 size_t scan_workspace_bytes_i64(int64_t n) { return scan_state_bytes(n); }
@@ -139,6 +182,13 @@ int exclusive_scan_i32(int32_t* data, int64_t n, void* ws, size_t ws_bytes, hipS
   return run_scan<int32_t, int32_t, false>(data, data, n, ws, ws_bytes, stream,
                                            "mt_exclusive_scan_i32");
 }
+
+int exclusive_scan_i32_state(int32_t* data, int64_t n, void* state, size_t state_bytes,
+                             hipStream_t stream, const char* name) {
+  return run_scan<int32_t, int32_t, false>(data, data, n, state, state_bytes, stream, name);
+}
+
+size_t scratch_bytes() { return kScratchBytes; }
 
 int inclusive_scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, void* ws,
                               size_t ws_bytes, hipStream_t stream) {
@@ -157,6 +207,27 @@ extern "C" int mt_exclusive_scan_i32(int32_t* data, int64_t n, void* workspace,
                                      size_t workspace_bytes, mt_stream_t stream) {
   return mt::exclusive_scan_i32(data, n, workspace, workspace_bytes,
                                 static_cast<hipStream_t>(stream));
+}
+
+extern "C" int mt_exclusive_scan_i32_noalloc(int32_t* data, int64_t n, mt_stream_t stream) {
+  const char* name = "mt_exclusive_scan_i32_noalloc";
+  MT_REQUIRE(n >= 0, "%s: negative length", name);
+  if (n > MT_SCAN_NOALLOC_MAX_ELEMENTS) {
+    mt::set_error("%s: %lld elements exceed the no-allocation limit of %lld; use "
+                  "mt_exclusive_scan_i32 with a workspace", name, static_cast<long long>(n),
+                  static_cast<long long>(MT_SCAN_NOALLOC_MAX_ELEMENTS));
+    return MT_ERR_INVALID_ARGUMENT;
+  }
+  if (n == 0) return MT_OK;
+  MT_REQUIRE(data != nullptr, "%s: null data pointer", name);
+  void* state = nullptr;
+  const int rc = mt::scratch_address(&state, name);
+  if (rc != MT_OK) return rc;
+  // The same kernels as mt_exclusive_scan_i32; the reset kernel clears the ticket and the
+  // look-back words of this call's tiles before the scan, so a previous call's contents
+  // never leak into this one.
+  return mt::exclusive_scan_i32_state(data, n, state, mt::scratch_bytes(),
+                                      static_cast<hipStream_t>(stream), name);
 }
 
 extern "C" int mt_inclusive_scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n,
