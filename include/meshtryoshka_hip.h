@@ -288,6 +288,19 @@ int mt_create_triangle_faces(const int32_t* neighbors, const int32_t* indices,
                              int32_t* faces, int32_t* voxel_indices, int64_t face_capacity,
                              int64_t vertex_capacity, int32_t* overflow, mt_stream_t stream);
 
+/* mt_create_triangle_faces reading int64 exclusive face offsets int64[S * num_voxels + 1]
+ * (from mt_inclusive_scan_i32_to_i64 of the face counts), for meshes with 2^31 or more
+ * face corners, where the reference's int32 compute_exclusive_cumsum
+ * (compute_exclusive_cumsum.cu:8-39) wraps around; face ids stay int32
+ * (face_capacity < 2^31). Same outputs as mt_create_triangle_faces wherever the int32
+ * offsets do not wrap. Replaces create_triangle_faces.slang:3-78 (.py:28-65). */
+int mt_create_triangle_faces_i64(const int32_t* neighbors, const int32_t* indices,
+                                 int64_t num_voxels, int64_t num_columns,
+                                 const int32_t* vertex_offsets, const uint8_t* vertex_types,
+                                 const uint8_t* cell_codes, int32_t S, const int64_t* face_offsets,
+                                 int32_t* faces, int32_t* voxel_indices, int64_t face_capacity,
+                                 int64_t vertex_capacity, int32_t* overflow, mt_stream_t stream);
+
 /* ---------------------------------------------------------------------------------
  * Rasterization (triangle_rasterization/__init__.py:22-329)
  * ------------------------------------------------------------------------------- */
@@ -309,7 +322,7 @@ int mt_bound_triangles(const float* xy, const float* depths, int64_t V, const in
                        int64_t F, int32_t B, int32_t tile_h, int32_t tile_w, int32_t row_min,
                        int32_t col_min, int32_t row_max, int32_t col_max, float near_plane,
                        int32_t* tile_min, int32_t* tile_max, int32_t* counts,
-                       const int32_t* num_faces_x3, mt_stream_t stream);
+                       const int64_t* num_faces_x3, mt_stream_t stream);
 
 /* One key per (camera, face, covered tile), written at the inclusive offsets
  * int64[B * F] of the tile counts. keys int64[K], values int32[K] (face ids) with
@@ -335,7 +348,7 @@ int mt_generate_keys(const float* depths, int64_t V, const int32_t* faces, int64
  * dropped, which changes no image or gradient of the fused composites; cull = 0 keeps every
  * tile of the bounding box. */
 int mt_key_block_counts(const float* xy, const float* depths, int64_t V, const int32_t* faces,
-                        int64_t F, const int32_t* num_faces_x3, int32_t B, int32_t tile_h,
+                        int64_t F, const int64_t* num_faces_x3, int32_t B, int32_t tile_h,
                         int32_t tile_w, int32_t row_min, int32_t col_min, int32_t row_max,
                         int32_t col_max, float near_plane, const int32_t* cam_dims,
                         int32_t cull, int32_t* block_counts, mt_stream_t stream);
@@ -348,7 +361,7 @@ int mt_key_block_counts(const float* xy, const float* depths, int64_t V, const i
  * mt_generate_keys; cull (the same value as for mt_key_block_counts) and cam_dims as in
  * mt_key_block_counts (tile ids then use each camera's grid_cols). */
 int mt_generate_keys_from_geometry(const float* xy, const float* depths, int64_t V,
-                                   const int32_t* faces, int64_t F, const int32_t* num_faces_x3,
+                                   const int32_t* faces, int64_t F, const int64_t* num_faces_x3,
                                    const int32_t* shell_face_boundaries, int32_t S, int32_t B,
                                    int32_t tile_h, int32_t tile_w, int32_t row_min,
                                    int32_t col_min, int32_t row_max, int32_t col_max,
@@ -374,7 +387,7 @@ int mt_generate_keys_from_geometry(const float* xy, const float* depths, int64_t
  * mt_generate_keys_single_pass_workspace_bytes(F, B). */
 size_t mt_generate_keys_single_pass_workspace_bytes(int64_t F, int32_t B);
 int mt_generate_keys_single_pass(const float* xy, const float* depths, int64_t V,
-                                 const int32_t* faces, int64_t F, const int32_t* num_faces_x3,
+                                 const int32_t* faces, int64_t F, const int64_t* num_faces_x3,
                                  const int32_t* shell_face_boundaries, int32_t S, int32_t B,
                                  int32_t tile_h, int32_t tile_w, int32_t row_min, int32_t col_min,
                                  int32_t row_max, int32_t col_max, float near_plane,

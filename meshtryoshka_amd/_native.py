@@ -121,6 +121,10 @@ SIGNATURES = {
         c_i32,
         [c_p, c_p, c_i64, c_i64, c_p, c_p, c_p, c_i32, c_p, c_p, c_p, c_i64, c_i64, c_p, c_p],
     ),
+    "mt_create_triangle_faces_i64": (
+        c_i32,
+        [c_p, c_p, c_i64, c_i64, c_p, c_p, c_p, c_i32, c_p, c_p, c_p, c_i64, c_i64, c_p, c_p],
+    ),
     "mt_project_vertices": (c_i32, [c_p, c_i64, c_p, c_p, c_i32, c_p, c_p, c_p]),
     "mt_bound_triangles": (
         c_i32,

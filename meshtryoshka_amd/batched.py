@@ -137,7 +137,10 @@ class BatchedTrainingStep:
         G = self.G
 
         self.vertex_offsets = torch.zeros(S * self.n_col + 1, **i32)
-        self.face_offsets = torch.zeros(S * self.n_vox + 1, **i32)
+        # Face-corner counts per (level set, cell) and their exclusive offsets in int64: a
+        # mesh may hold 2^31 face corners or more (716M faces) before face ids (int32) run out.
+        self.face_counts = torch.zeros(S * self.n_vox + 1, **i32)
+        self.face_offsets = torch.zeros(S * self.n_vox + 1, **i64)
         self.codes = torch.empty((S, self.n_vox), dtype=torch.uint8, device=dev)
         self.tv = torch.zeros((V, 3), **f32)
         self.td = torch.zeros(V, **f32)
@@ -171,11 +174,12 @@ class BatchedTrainingStep:
         self.grid_sh_grad = torch.zeros((num_sh, self.n_grid, 3), **f32)
         self.overflow = torch.zeros(1, **i32)
         self.face_boundaries = torch.zeros(S, **i32)  # last face (exclusive) of each shell
+        self.face_bound64 = torch.zeros(S, **i64)
         self.side = torch.cuda.Stream(device=dev)  # default priority, like every stream here
         self.group_keys = torch.zeros(len(self.groups), **i64)  # key count of each group
         self.sharpness = torch.tensor([float(sharpness)], **f32)
         self.background = torch.as_tensor(background, dtype=torch.float32, device=dev).contiguous()
-        scan_n = max(S * self.n_col + 1, S * self.n_vox + 1)
+        scan_n = max(S * self.n_col + 1, S * self.n_vox)
         self.scan_ws = _native.workspace(_native.query("mt_scan_workspace_bytes", scan_n), dev)
         self.keys_ws = _native.workspace(
             _native.query("mt_generate_keys_single_pass_workspace_bytes", F, G), dev)
@@ -183,7 +187,7 @@ class BatchedTrainingStep:
         self.vbwd_ws = _native.workspace(
             _native.query("mt_create_triangle_vertices_colors_backward_workspace_bytes",
                           self.n_grid, self.n_col, sh_degree), dev)
-        # Live counts, read by kernels from device memory.
+        # Live counts, read by kernels from device memory (int64 face corners).
         self.num_faces_x3 = self.face_offsets[S * self.n_vox :]
         self.graph = None
         self.static_inputs = None
@@ -248,9 +252,11 @@ class BatchedTrainingStep:
              self.scan_ws.data_ptr(), self.scan_ws.numel(), st)
         call("mt_count_triangle_faces", grid_sdf.data_ptr(), v.lower_corners.data_ptr(), self.n_col,
              v.upper_corners.data_ptr(), self.n_vox, self.level_sets.data_ptr(), S,
-             self.face_offsets.data_ptr(), self.codes.data_ptr(), st)
-        call("mt_exclusive_scan_i32", self.face_offsets.data_ptr(), self.face_offsets.numel(),
-             self.scan_ws.data_ptr(), self.scan_ws.numel(), st)
+             self.face_counts.data_ptr(), self.codes.data_ptr(), st)
+        # face_offsets[0] stays 0; entries 1.. are the inclusive sums of the counts.
+        call("mt_inclusive_scan_i32_to_i64", self.face_counts.data_ptr(),
+             self.face_offsets[1:].data_ptr(), S * self.n_vox, self.scan_ws.data_ptr(),
+             self.scan_ws.numel(), st)
         call("mt_camera_positions", self.extrinsics.data_ptr(), self.B, self.campos.data_ptr(), st)
         call("mt_pack_sh_rows", grid_sh.data_ptr(), self.n_grid, self.num_sh, self.sh_degree,
              self.sh_rows.data_ptr(), st)
@@ -285,13 +291,15 @@ class BatchedTrainingStep:
                     self.side.wait_stream(main)
                 with torch.cuda.stream(self.side if overlap else main):
                     fst = torch.cuda.current_stream(self.device).cuda_stream
-                    call("mt_create_triangle_faces", v.neighbors.data_ptr(), v.indices.data_ptr(),
-                         self.n_vox, self.n_col, self.vertex_offsets.data_ptr(),
-                         self.ttype.data_ptr(), self.codes.data_ptr(), S,
-                         self.face_offsets.data_ptr(), self.faces.data_ptr(),
-                         self.voxel_indices.data_ptr(), F, V, self.overflow.data_ptr(), fst)
+                    call("mt_create_triangle_faces_i64", v.neighbors.data_ptr(),
+                         v.indices.data_ptr(), self.n_vox, self.n_col,
+                         self.vertex_offsets.data_ptr(), self.ttype.data_ptr(),
+                         self.codes.data_ptr(), S, self.face_offsets.data_ptr(),
+                         self.faces.data_ptr(), self.voxel_indices.data_ptr(), F, V,
+                         self.overflow.data_ptr(), fst)
                     torch.floor_divide(self.face_offsets[self.n_vox :: self.n_vox][:S], 3,
-                                       out=self.face_boundaries)
+                                       out=self.face_bound64)
+                    self.face_boundaries.copy_(self.face_bound64)
                 if hook is not None:
                     hook("face_boundaries")
             face_boundaries = self.face_boundaries
@@ -410,7 +418,8 @@ def measure_capacities(voxels: Voxels, level_sets, extrinsics, intrinsics, image
     b = extrinsics.shape[0]
     shapes = camera_shapes(image_shape, b)
     with torch.no_grad():
-        tri = te.tessellate_voxels(grid_sdf, grid_vertices, grid_sh, voxels.neighbors,
+        # Positions and faces do not depend on the SH: tessellate with the DC band only.
+        tri = te.tessellate_voxels(grid_sdf, grid_vertices, grid_sh[:1], voxels.neighbors,
                                    voxels.lower_corners, voxels.upper_corners, voxels.indices,
                                    level_sets)
         per_camera = [count_keys(tri.vertices, tri.faces, extrinsics[c : c + 1],

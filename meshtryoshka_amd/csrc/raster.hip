@@ -203,7 +203,7 @@ __device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, cons
 // Face whose keys are generated: live (below the device-side face count) and with all
 // three corners inside the vertex array.
 __device__ __forceinline__ bool keyed_face(const int32_t* faces, int64_t f, int64_t V,
-                                           const int32_t* live_x3, int64_t& ia, int64_t& ib,
+                                           const int64_t* live_x3, int64_t& ia, int64_t& ib,
                                            int64_t& ic) {
   ia = faces[3 * f + 0];
   ib = faces[3 * f + 1];
@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(kBlock) bound_kernel(const float* xy, const fl
                                                        int B, int th, int tw, int rmin, int cmin,
                                                        int rmax, int cmax, float near_plane,
                                                        int32_t* tmin, int32_t* tmax,
-                                                       int32_t* counts, const int32_t* live_x3) {
+                                                       int32_t* counts, const int64_t* live_x3) {
   const int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (f >= F) return;
   const int64_t ia = faces[3 * f + 0], ib = faces[3 * f + 1], ic = faces[3 * f + 2];
@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(kBlock) keys_kernel(const float* depths, int64
 __global__ void __launch_bounds__(kBlock) key_block_counts_kernel(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F, int B,
     int th, int tw, int rmin, int cmin, int rmax, int cmax, float near_plane,
-    const int32_t* cam_dims, int cull, const int32_t* live_x3, int32_t* block_counts) {
+    const int32_t* cam_dims, int cull, const int64_t* live_x3, int32_t* block_counts) {
   __shared__ int32_t s_wave[kBlock / kWave];
   const int64_t f = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   const int64_t nblk = gridDim.x;
@@ -371,7 +371,7 @@ __global__ void __launch_bounds__(kBlock) key_block_counts_kernel(
 
 __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
-    const int32_t* live_x3, const int32_t* boundaries, int S, int th, int tw, int rmin, int cmin,
+    const int64_t* live_x3, const int32_t* boundaries, int S, int th, int tw, int rmin, int cmin,
     int rmax, int cmax, float near_plane, const int32_t* cam_dims, int cull,
     const int64_t* block_incl,
     int grid_cols, KeyFields kf, int64_t* keys, int32_t* values, int64_t kcap, int32_t* overflow) {
@@ -469,7 +469,7 @@ template <bool kPow2>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kPow2 ? 6 : 5)))
 keys_single_pass_kernel(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
-    const int32_t* live_x3, const int32_t* boundaries, int S, int B, int lfpb, int nslices,
+    const int64_t* live_x3, const int32_t* boundaries, int S, int B, int lfpb, int nslices,
     int th, int tw,
     int rmin, int cmin, int rmax, int cmax, float near_plane, const int32_t* cam_dims, int cull,
     int grid_cols, KeyFields kf, int64_t* keys, int32_t* values, int64_t kcap, int64_t* n_keys,
@@ -838,7 +838,7 @@ extern "C" int mt_bound_triangles(const float* xy, const float* depths, int64_t 
                                   int32_t tile_w, int32_t row_min, int32_t col_min,
                                   int32_t row_max, int32_t col_max, float near_plane,
                                   int32_t* tile_min, int32_t* tile_max, int32_t* counts,
-                                  const int32_t* num_faces_x3, mt_stream_t stream) {
+                                  const int64_t* num_faces_x3, mt_stream_t stream) {
   MT_REQUIRE(F >= 0 && B >= 0 && tile_h > 0 && tile_w > 0, "mt_bound_triangles: bad sizes");
   if (F == 0 || B == 0) return MT_OK;
   mt::bound_kernel<<<mt::blocks_for(F, mt::kBlock), mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
@@ -863,7 +863,7 @@ extern "C" int mt_generate_keys(const float* depths, int64_t V, const int32_t* f
 }
 
 extern "C" int mt_key_block_counts(const float* xy, const float* depths, int64_t V,
-                                   const int32_t* faces, int64_t F, const int32_t* num_faces_x3,
+                                   const int32_t* faces, int64_t F, const int64_t* num_faces_x3,
                                    int32_t B, int32_t tile_h, int32_t tile_w, int32_t row_min,
                                    int32_t col_min, int32_t row_max, int32_t col_max,
                                    float near_plane, const int32_t* cam_dims, int32_t cull,
@@ -879,7 +879,7 @@ extern "C" int mt_key_block_counts(const float* xy, const float* depths, int64_t
 
 extern "C" int mt_generate_keys_from_geometry(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
-    const int32_t* num_faces_x3, const int32_t* shell_face_boundaries, int32_t S, int32_t B,
+    const int64_t* num_faces_x3, const int32_t* shell_face_boundaries, int32_t S, int32_t B,
     int32_t tile_h, int32_t tile_w, int32_t row_min, int32_t col_min, int32_t row_max,
     int32_t col_max, float near_plane, const int32_t* cam_dims, int32_t cull,
     const int64_t* block_offsets, int32_t grid_cols, mt_key_layout layout, int64_t* keys,
@@ -901,7 +901,7 @@ extern "C" size_t mt_generate_keys_single_pass_workspace_bytes(int64_t F, int32_
 
 extern "C" int mt_generate_keys_single_pass(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
-    const int32_t* num_faces_x3, const int32_t* shell_face_boundaries, int32_t S, int32_t B,
+    const int64_t* num_faces_x3, const int32_t* shell_face_boundaries, int32_t S, int32_t B,
     int32_t tile_h, int32_t tile_w, int32_t row_min, int32_t col_min, int32_t row_max,
     int32_t col_max, float near_plane, const int32_t* cam_dims, int32_t cull, int32_t grid_cols,
     mt_key_layout layout, int64_t* keys, int32_t* values, int64_t key_capacity, int64_t* n_keys,

@@ -819,10 +819,13 @@ __global__ void __launch_bounds__(kBlock) count_faces_kernel(const float* sdf, c
 // corners are spread evenly over the block's threads: each thread finds its corner's
 // voxel by binary search over the voxels' corner offsets in LDS, resolves the corner's
 // triangle vertex, and consecutive threads write consecutive corners.
+// Face offsets are int32 (the reference's layout) or int64 (more than 2^31 - 1 face
+// corners); within one block they span at most 256 voxels x 15 corners.
+template <typename FOff>
 __global__ void __launch_bounds__(kBlock) create_faces_kernel(
     const int32_t* neighbors, const int32_t* indices, int64_t n, int64_t ncol,
     const int32_t* voffsets, const uint8_t* vtypes, const uint8_t* codes, int S,
-    const int32_t* foffsets, int32_t* faces, int32_t* voxel_indices, int64_t fcap,
+    const FOff* foffsets, int32_t* faces, int32_t* voxel_indices, int64_t fcap,
     int64_t vcap, int32_t* overflow) {
   constexpr int kMaxTri = 5 * kBlock;  // a marching-cubes case has at most 5 triangles
   __shared__ int32_t s_start[kBlock];
@@ -1017,11 +1020,30 @@ extern "C" int mt_create_triangle_faces(const int32_t* neighbors, const int32_t*
                                         mt_stream_t stream) {
   MT_REQUIRE(num_voxels >= 0 && S >= 0 && face_capacity >= 0, "mt_create_triangle_faces: bad sizes");
   if (num_voxels == 0 || S == 0) return MT_OK;
-  mt::create_faces_kernel<<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
-                            static_cast<hipStream_t>(stream)>>>(
+  mt::create_faces_kernel<int32_t><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
+                                     static_cast<hipStream_t>(stream)>>>(
       neighbors, indices, num_voxels, num_columns, vertex_offsets, vertex_types, cell_codes, S,
       face_offsets, faces, voxel_indices, face_capacity, vertex_capacity, overflow);
   return mt::check_launch("mt_create_triangle_faces");
+}
+
+extern "C" int mt_create_triangle_faces_i64(const int32_t* neighbors, const int32_t* indices,
+                                            int64_t num_voxels, int64_t num_columns,
+                                            const int32_t* vertex_offsets,
+                                            const uint8_t* vertex_types, const uint8_t* cell_codes,
+                                            int32_t S, const int64_t* face_offsets, int32_t* faces,
+                                            int32_t* voxel_indices, int64_t face_capacity,
+                                            int64_t vertex_capacity, int32_t* overflow,
+                                            mt_stream_t stream) {
+  MT_REQUIRE(num_voxels >= 0 && S >= 0 && face_capacity >= 0,
+             "mt_create_triangle_faces_i64: bad sizes");
+  MT_REQUIRE(face_capacity < (int64_t(1) << 31), "mt_create_triangle_faces_i64: face ids are int32");
+  if (num_voxels == 0 || S == 0) return MT_OK;
+  mt::create_faces_kernel<int64_t><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
+                                     static_cast<hipStream_t>(stream)>>>(
+      neighbors, indices, num_voxels, num_columns, vertex_offsets, vertex_types, cell_codes, S,
+      face_offsets, faces, voxel_indices, face_capacity, vertex_capacity, overflow);
+  return mt::check_launch("mt_create_triangle_faces_i64");
 }
 
 extern "C" int mt_create_triangle_vertices_colors_forward(

@@ -235,7 +235,7 @@ def _batch_plan(scene: ResidentScene, shapes, tile_shape, k_cap):
         ("n_keys", 1, i64),
         ("keys", k_cap, i64), ("vals", k_cap, i32), ("skeys", k_cap, i64), ("svals", k_cap, i32),
         ("ranges", (tile_base[-1], 2), i64), ("out", 4 * pixel_base[-1], f32),
-        ("campos", (b, 3), f32), ("overflow", 1, i32), ("live_x3", 1, i32),
+        ("campos", (b, 3), f32), ("overflow", 1, i32), ("live_x3", 1, i64),
         ("keys_ws", _native.query("mt_generate_keys_single_pass_workspace_bytes",
                                   scene.faces.shape[0], b), u8),
         ("sort_ws", _native.query("mt_sort_workspace_bytes", k_cap), u8),

@@ -18,7 +18,7 @@ from torch import Tensor
 from .common import pack_occupancy as pack_occupancy
 from .common import record_function
 from .common import unpack_occupancy as unpack_occupancy
-from .compute_exclusive_cumsum import compute_exclusive_cumsum
+from .compute_exclusive_cumsum import compute_exclusive_cumsum, exclusive_offsets_i64
 from .compute_sdf_regularizers import SDFRegularizerLosses as SDFRegularizerLosses
 from .compute_sdf_regularizers import compute_sdf_regularizers as compute_sdf_regularizers
 from .compute_vertex_occupancy import compute_vertex_occupancy
@@ -54,7 +54,7 @@ get_occupancy_mesh = _not_in_this_build("get_occupancy_mesh")
 def extract_voxels(occupancy: Tensor) -> Voxels:
     """Packed occupancy int32[i, j, k/32] -> Voxels (triangle_extraction/__init__.py:31-62)."""
     i, j, k_packed = occupancy.shape
-    logging.debug(f"Extracting voxels for grid with shape ({i}, {j}, {k_packed * 32}).")
+    logging.debug("extract_voxels: %d x %d x %d cell grid", i, j, k_packed * 32)
     vertex_occupancy, vertex_offsets = compute_vertex_occupancy(occupancy)
     compute_exclusive_cumsum(vertex_offsets)
     num_vertices = vertex_offsets[-1].item()
@@ -63,7 +63,7 @@ def extract_voxels(occupancy: Tensor) -> Voxels:
     compute_exclusive_cumsum(voxel_offsets)
     num_voxels = voxel_offsets[-1].item()
     voxel_offsets = voxel_offsets[:-1].view(vertex_occupancy.shape)
-    logging.debug(f"Sampled {num_vertices} vertices and {num_voxels} voxels.")
+    logging.debug("extract_voxels: %d corners, %d occupied cells", num_vertices, num_voxels)
     return create_voxels(occupancy, voxel_offsets, num_voxels, vertex_occupancy, vertex_offsets,
                          num_vertices)
 
@@ -88,7 +88,7 @@ def tessellate_voxels(signed_distances: Tensor, vertices: Tensor, spherical_harm
     vertices and faces occupy one contiguous block, delimited by the cumulative
     ``vertex_boundaries`` / ``face_boundaries``."""
     (num_level_sets,) = level_sets.shape
-    logging.debug(f"Tessellating {neighbors.shape[1]} voxels with {num_level_sets} level sets.")
+    logging.debug("tessellate_voxels: %d cells, %d level sets", neighbors.shape[1], num_level_sets)
     with torch.no_grad():
         vertex_offsets = count_triangle_vertices(signed_distances, lower_corners, level_sets)
         compute_exclusive_cumsum(vertex_offsets)
@@ -100,14 +100,17 @@ def tessellate_voxels(signed_distances: Tensor, vertices: Tensor, spherical_harm
     ) = create_triangle_vertices(vertices, signed_distances, spherical_harmonics, lower_corners,
                                  level_sets, vertex_offsets)
     with torch.no_grad():
-        face_offsets, voxel_cell_codes = count_triangle_faces(signed_distances, lower_corners,
-                                                              upper_corners, level_sets)
-        compute_exclusive_cumsum(face_offsets)
+        face_counts, voxel_cell_codes = count_triangle_faces(signed_distances, lower_corners,
+                                                             upper_corners, level_sets)
+        # Face offsets in int64: the reference scans them in int32 in place
+        # (compute_exclusive_cumsum), which wraps once a mesh has 2^31 face corners
+        # (716M faces); below that the offsets, and so every output, are the same.
+        face_offsets = exclusive_offsets_i64(face_counts)
         faces, voxel_indices = create_triangle_faces(signed_distances, neighbors, indices,
                                                      vertex_offsets, triangle_vertex_types,
                                                      voxel_cell_codes, level_sets, face_offsets)
     vertex_boundaries = vertex_offsets[1:].view(num_level_sets, -1)[:, -1].clone()
-    face_boundaries = face_offsets[1:].view(num_level_sets, -1)[:, -1].clone() // 3
+    face_boundaries = (face_offsets[1:].view(num_level_sets, -1)[:, -1] // 3).to(torch.int32)
     return Triangles(triangle_vertices, triangle_signed_distances, triangle_spherical_harmonics,
                      faces, voxel_indices, vertex_boundaries, face_boundaries)
 

@@ -1,4 +1,7 @@
-"""Marching-cubes faces (create_triangle_faces.py:23-65)."""
+"""Marching-cubes faces (create_triangle_faces.py:23-65).
+
+``face_offsets`` may be the reference's int32 exclusive offsets or int64 ones (meshes
+with 2^31 face corners or more, where an int32 scan wraps)."""
 
 from typing import NamedTuple
 
@@ -28,8 +31,12 @@ def create_triangle_faces(signed_distances: Tensor, neighbors: Tensor, indices: 
     num_columns = (vertex_offsets.shape[0] - 1) // num_level_sets
     faces = torch.empty((num_faces, 3), dtype=torch.int32, device=device)
     voxel_indices = torch.empty((num_faces,), dtype=torch.int32, device=device)
+    if face_offsets.dtype not in (torch.int32, torch.int64):
+        raise RuntimeError("face_offsets must be int32 (the reference's layout) or int64")
+    entry = "mt_create_triangle_faces" if face_offsets.dtype == torch.int32 else \
+        "mt_create_triangle_faces_i64"
     _native.call(
-        "mt_create_triangle_faces", neighbors.contiguous().data_ptr(), indices.contiguous().data_ptr(),
+        entry, neighbors.contiguous().data_ptr(), indices.contiguous().data_ptr(),
         num_voxels, num_columns, vertex_offsets.contiguous().data_ptr(),
         triangle_vertex_types.contiguous().data_ptr(), voxel_cell_codes.contiguous().data_ptr(),
         num_level_sets, face_offsets.contiguous().data_ptr(), faces.data_ptr(),

@@ -88,7 +88,7 @@ def test_keys_from_geometry_match_oracle(scene):
     nblk = -(-f // 256)
     layout = to_layout(_key_defines())
     for live in (f, f // 2):
-        live_x3 = torch.tensor([3 * live], dtype=torch.int32, device="cuda")
+        live_x3 = torch.tensor([3 * live], dtype=torch.int64, device="cuda")
         counts = torch.empty(b * nblk, dtype=torch.int32, device="cuda")
         _native.call("mt_key_block_counts", xy.data_ptr(), depth.data_ptr(), v, faces.data_ptr(), f,
                      live_x3.data_ptr(), b, th, tw, 0, 0, scene.rows, scene.cols,
