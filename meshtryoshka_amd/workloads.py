@@ -4,7 +4,8 @@ Every scene draws from one ``torch.Generator("cpu")`` seeded with ``seed`` in a 
 order (scene primitives, then corner SDF noise, then SH coefficients, then cameras,
 then the target image), so the same seed gives the same tensors on any device.
 
-* ``dense_sphere``: a dense 64^3 cell grid on [-1, 1]^3 (every cell occupied), corner
+* ``dense_sphere``: a dense 64^3 cell grid on [-1, 1]^3 (every cell occupied; other
+  resolutions are padded along k to whole 32-cell words), corner
   SDF ``|p| - 0.5 + 0.02 N(0, 1)``, SH degree 0 colour ``U(0, 1)``, three level sets
   {+0.02, 0, -0.02}, one 128x128 pinhole camera (60 deg FOV) at distance 2.5 on +z.
 * ``sphere_field``: a sparse grid at ``resolution^3`` virtual cells; cells within
@@ -82,10 +83,16 @@ def pinhole(image_shape, fov_y_deg=60.0) -> torch.Tensor:
 
 
 def dense_sphere(seed: int = 0, resolution: int = 64, image_size: int = 128) -> SceneSpec:
-    occ = torch.ones((resolution, resolution, resolution), dtype=torch.bool)
+    # Packed rows hold whole 32-cell words: a resolution that is not a multiple of 32 is
+    # padded with unoccupied cells along k (x), and x is rescaled so that the occupied
+    # cells still span [-1, 1] (the scale is exactly 1 without padding).
+    kpad = -(-resolution // 32) * 32
+    occ = torch.zeros((resolution, resolution, kpad), dtype=torch.bool)
+    occ[:, :, :resolution] = True
+    scale = torch.tensor([kpad / resolution, 1.0, 1.0], dtype=torch.float32)
 
     def field_fn(unit_vertices: torch.Tensor, gen: torch.Generator):
-        p = 2 * unit_vertices - 1
+        p = 2 * (unit_vertices * scale.to(unit_vertices.device)) - 1
         noise = torch.randn(p.shape[0], generator=gen, dtype=torch.float32).to(p.device)
         sdf = p.norm(dim=-1) - 0.5 + 0.02 * noise
         sh = torch.rand((1, p.shape[0], 3), generator=gen, dtype=torch.float32).to(p.device)
