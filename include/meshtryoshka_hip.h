@@ -488,14 +488,15 @@ int mt_intersect_and_composite_backward(const mt_composite_args* args, const flo
  * target)^2) over B x 3 x H x W, and grad_out f32[B, 4, H, W] = its gradient with
  * respect to out f32[B, 4, H, W] (rgb, transmittance). The reference computes this loss
  * with torch ops in the model (model_meshtryoshka.py:249-255, L2 part). background f32[3],
- * target f32[B, 3, H, W]; *loss must be zeroed by the caller. With pixel_base int64[B + 1]
+ * target f32[B, 3, H, W]; *loss (float64, accumulated over the blocks' partial sums) must
+ * be zeroed by the caller. With pixel_base int64[B + 1]
  * (cameras of different resolutions; H, W ignored) camera b's planes start at
  * out + 4 * pixel_base[b] and target + 3 * pixel_base[b], total_pixels = pixel_base[B].
  * mean_pixels > 0 sets the pixel count of the mean (a batch rendered in camera groups adds
  * each group's share of one mean); 0 means the pixels of this call. */
 int mt_image_l2_loss(const float* out, const float* background, const float* target, int32_t B,
                      int32_t H, int32_t W, const int64_t* pixel_base, int64_t total_pixels,
-                     int64_t mean_pixels, float* loss, float* grad_out, mt_stream_t stream);
+                     int64_t mean_pixels, double* loss, float* grad_out, mt_stream_t stream);
 
 /* Explicit mode: first hit per (camera, shell, pixel) in depth order. tile_ranges
  * int64[B, S, T, 2]; out_uv f32[B*S, H, W, 2]; out_index int32[B*S, H, W] (-1 = miss).

@@ -163,7 +163,7 @@ class BatchedTrainingStep:
         self.maps_built = False
         self.out_flat = torch.zeros(4 * self.num_pixels, **f32)
         self.grad_flat = torch.zeros_like(self.out_flat)
-        self.loss = torch.zeros(1, **f32)
+        self.loss = torch.zeros(1, dtype=torch.float64, device=dev)
         self.colors_grad = torch.zeros_like(self.colors)
         self.td_grad = torch.zeros_like(self.td)
         # Vertices that received a gradient in the current group (set by the composite

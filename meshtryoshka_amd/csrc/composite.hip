@@ -861,7 +861,7 @@ __global__ void __launch_bounds__(256) image_l2_loss_kernel(const float* out, co
                                                             int64_t HW_uniform,
                                                             const int64_t* pixel_base,
                                                             int64_t total, float inv_n,
-                                                            float* loss, float* grad) {
+                                                            double* loss, float* grad) {
   __shared__ float s_red[256 / kWave];
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   float acc = 0.f;
@@ -900,7 +900,12 @@ __global__ void __launch_bounds__(256) image_l2_loss_kernel(const float* out, co
   const float s = wave_sum_f(acc);
   if (__lane_id() == 0) s_red[threadIdx.x / kWave] = s;
   __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(loss, (s_red[0] + s_red[1] + s_red[2] + s_red[3]) * inv_n);
+  // Block partials are summed in float64: tens of thousands of FP32 atomic additions to
+  // one float would leave the mean ~1e-6 (relative) from the exact sum of the same terms.
+  if (threadIdx.x == 0) {
+    const double part = double(s_red[0]) + double(s_red[1]) + double(s_red[2]) + double(s_red[3]);
+    atomicAdd(loss, part * double(inv_n));
+  }
 }
 
 int check_composite_args(const mt_composite_args* a, const char* name) {
@@ -972,7 +977,7 @@ extern "C" int mt_intersect_and_composite_backward(const mt_composite_args* args
 
 extern "C" int mt_image_l2_loss(const float* out, const float* background, const float* target,
                                 int32_t B, int32_t H, int32_t W, const int64_t* pixel_base,
-                                int64_t total_pixels, int64_t mean_pixels, float* loss,
+                                int64_t total_pixels, int64_t mean_pixels, double* loss,
                                 float* grad_out, mt_stream_t stream) {
   MT_REQUIRE(B >= 0 && H >= 0 && W >= 0 && total_pixels >= 0 && mean_pixels >= 0,
              "mt_image_l2_loss: negative size");

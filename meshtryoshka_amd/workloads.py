@@ -236,6 +236,7 @@ def sphere_field(
         seed=seed,
         field_fn=field_fn,
         extra={"generator": gen, "num_cameras": num_cameras, "image_shape": image_shape,
+               "centres": centres, "radii": radii,
                "resolutions": None if resolutions is None else [tuple(r) for r in resolutions]},
     )
     return spec
