@@ -1,7 +1,8 @@
 """Render serving with device-resident scenes and continuous batching.
 
 * ``ResidentScene``: a scene's extracted shells (vertices, faces, shell boundaries, SDF,
-  SH) kept on the device; tessellation happens once, at load.
+  and either the triangle vertices' SH or the grid SH as rows, ``GridColours``) kept on
+  the device; tessellation happens once, at load.
 * ``render_batch``: forward-only fused render of B cameras of one scene, each at its own
   resolution (a per-camera table of image and tile-grid sizes drives the key,
   delineation and compositing kernels), launched on the current stream with no host
@@ -40,18 +41,44 @@ _LAYOUT = to_layout(create_key_defines(("camera", 6), ("tile", 20), ("depth", 32
 
 
 @dataclass
+class GridColours:
+    """What a resident scene keeps to colour its triangle vertices from the grid for each
+    batch (``mt_create_triangle_vertices_colors_forward``, colours only) instead of
+    keeping their SH coefficients (192 bytes per vertex at degree 3). The colours are the
+    same bits as evaluating the per-vertex SH."""
+
+    grid_vertices: torch.Tensor  # f32[V_grid, 3]
+    grid_sdf: torch.Tensor  # f32[V_grid]
+    sh_rows: torch.Tensor  # f32[V_grid, mt_sh_row_floats] (mt_pack_sh_rows)
+    num_sh: int
+    lower_corners: torch.Tensor  # int32[4, columns]
+    level_sets: torch.Tensor  # f32[S]
+    vertex_offsets: torch.Tensor  # int32[S * columns + 1]
+
+
+@dataclass
 class ResidentScene:
     vertices: torch.Tensor  # f32[V, 3]
     faces: torch.Tensor  # int32[F, 3]
     face_boundaries: torch.Tensor  # int32[S]
     signed_distances: torch.Tensor  # f32[V]
-    spherical_harmonics: torch.Tensor  # f32[NSH, V, 3]
+    spherical_harmonics: torch.Tensor | None  # f32[NSH, V, 3], or None with ``grid``
     sh_degree: int
     sharpness: torch.Tensor  # f32[1]
     background: torch.Tensor  # f32[3]
     keys_per_pixel: float = 8.0  # measured density of (face, tile) keys per image pixel
     # (pixels, keys) of calibrated resolutions: the most keys one camera needed at that size
     key_table: list = field(default_factory=list)
+    grid: GridColours | None = None
+
+    def resident_bytes(self) -> int:
+        ts = [self.vertices, self.faces, self.face_boundaries, self.signed_distances,
+              self.spherical_harmonics]
+        if self.grid is not None:
+            g = self.grid
+            ts += [g.grid_vertices, g.grid_sdf, g.sh_rows, g.lower_corners, g.level_sets,
+                   g.vertex_offsets]
+        return sum(t.numel() * t.element_size() for t in ts if t is not None)
 
     @property
     def num_shells(self) -> int:
@@ -78,19 +105,40 @@ class ResidentScene:
 
 
 def tessellate_scene(voxels, grid_vertices, grid_sdf, grid_sh, level_sets, sh_degree, sharpness,
-                     background) -> ResidentScene:
+                     background, colours: str = "vertex") -> ResidentScene:
+    """Tessellate once and keep the shells resident. ``colours="vertex"`` keeps every
+    triangle vertex's SH (the plain render's inputs); ``colours="grid"`` keeps the grid SH
+    as rows and recomputes the batch cameras' colours from them per batch (far less
+    memory for large scenes; the same images)."""
     from . import triangle_extraction as te
 
+    if colours not in ("vertex", "grid"):
+        raise ValueError(f"colours must be 'vertex' or 'grid', not {colours!r}")
+    dev = grid_sdf.device
     with torch.no_grad():
-        tri = te.tessellate_voxels(grid_sdf, grid_vertices, grid_sh, voxels.neighbors,
+        sh_in = grid_sh if colours == "vertex" else grid_sh[:1]  # positions need no SH
+        tri = te.tessellate_voxels(grid_sdf, grid_vertices, sh_in, voxels.neighbors,
                                    voxels.lower_corners, voxels.upper_corners, voxels.indices,
                                    level_sets)
-    dev = grid_sdf.device
+        grid = None
+        if colours == "grid":
+            offsets = te.count_triangle_vertices(grid_sdf, voxels.lower_corners, level_sets)
+            te.compute_exclusive_cumsum(offsets)
+            n_grid = grid_vertices.shape[0]
+            rows = torch.empty((n_grid, _native.query("mt_sh_row_floats", sh_degree)),
+                               dtype=torch.float32, device=dev)
+            num_sh = grid_sh.shape[0]
+            _native.call("mt_pack_sh_rows", grid_sh.contiguous().data_ptr(), n_grid, num_sh,
+                         sh_degree, rows.data_ptr(), torch.cuda.current_stream(dev).cuda_stream)
+            grid = GridColours(grid_vertices.contiguous(), grid_sdf.detach().contiguous(), rows,
+                               num_sh, voxels.lower_corners.contiguous(),
+                               level_sets.contiguous(), offsets)
     return ResidentScene(
         tri.vertices.detach(), tri.faces, tri.face_boundaries, tri.signed_distances.detach(),
-        tri.spherical_harmonics.detach(), sh_degree,
+        tri.spherical_harmonics.detach() if colours == "vertex" else None, sh_degree,
         torch.tensor([float(sharpness)], device=dev, dtype=torch.float32),
         torch.as_tensor(background, dtype=torch.float32, device=dev).contiguous(),
+        grid=grid,
     )
 
 
@@ -161,7 +209,8 @@ def read_scene_checkpoint(path, grid_shape=None, level_sets=None, sh_degree=None
             "background": model.get("background", torch.zeros(3)).to(torch.float32)}
 
 
-def load_scene_checkpoint(path, device, vertex_transform=None, **overrides) -> ResidentScene:
+def load_scene_checkpoint(path, device, vertex_transform=None, colours: str = "vertex",
+                          **overrides) -> ResidentScene:
     """A resident scene from a checkpoint: voxels extracted on ``device``, shells tessellated
     at the level sets. ``vertex_transform`` maps the tessellated vertices (unit-cube grid
     coordinates) to world space, as the reference's contraction does (tessellator.py:82-83)."""
@@ -174,8 +223,12 @@ def load_scene_checkpoint(path, device, vertex_transform=None, **overrides) -> R
                          f"{vox.vertices.shape[0]} grid vertices")
     scene = tessellate_scene(vox, vox.vertices, c["signed_distances"].to(device),
                              c["spherical_harmonics"].to(device), c["level_sets"].to(device),
-                             c["sh_degree"], c["sharpness"], c["background"].to(device))
+                             c["sh_degree"], c["sharpness"], c["background"].to(device),
+                             colours=colours)
     if vertex_transform is not None:
+        if scene.grid is not None:
+            raise ValueError("a vertex transform needs colours='vertex' (grid colours are "
+                             "computed at the untransformed grid vertices)")
         scene.vertices = vertex_transform(scene.vertices).contiguous()
     return scene
 
@@ -307,9 +360,17 @@ def render_batch(scene: ResidentScene, extrinsics: torch.Tensor, intrinsics: tor
     call("mt_delineate_tiles", skeys.data_ptr(), k_cap, _LAYOUT, 0, b, 1, rows * cols,
          ranges.data_ptr(), n_dev.data_ptr(), tbase.data_ptr() if table else None, st)
     call("mt_camera_positions", E.data_ptr(), b, campos.data_ptr(), st)
-    call("mt_evaluate_sh_forward", scene.vertices.data_ptr(), v, scene.spherical_harmonics.data_ptr(),
-         scene.spherical_harmonics.shape[0], scene.sh_degree, campos.data_ptr(), b,
-         colors.data_ptr(), st)
+    if scene.grid is None:
+        call("mt_evaluate_sh_forward", scene.vertices.data_ptr(), v,
+             scene.spherical_harmonics.data_ptr(), scene.spherical_harmonics.shape[0],
+             scene.sh_degree, campos.data_ptr(), b, colors.data_ptr(), st)
+    else:  # colours only (parts 2), from the grid's SH rows (+4)
+        g = scene.grid
+        call("mt_create_triangle_vertices_colors_forward", g.grid_vertices.data_ptr(),
+             g.grid_sdf.data_ptr(), g.sh_rows.data_ptr(), g.grid_vertices.shape[0], g.num_sh,
+             scene.sh_degree, g.lower_corners.data_ptr(), g.lower_corners.shape[1],
+             g.level_sets.data_ptr(), s, g.vertex_offsets.data_ptr(), v, campos.data_ptr(), b,
+             None, None, None, colors.data_ptr(), overflow.data_ptr(), 2 | 4, st)
     h0, w0 = shapes[0]
     args = _native.CompositeArgs(
         xy.data_ptr(), colors.data_ptr(), scene.signed_distances.data_ptr(), v,

@@ -118,3 +118,61 @@ def test_scene_checkpoint_renders_the_same(tmp_path):
     torch.cuda.synchronize()
     assert torch.equal(imgs[0], imgs[1])
     assert float((imgs[0] < 0.999).float().mean()) > 0.01  # the scene is in view
+
+
+def test_grid_colours_render_the_same(scenes):
+    """A resident scene that keeps the grid SH as rows and colours each batch from them
+    renders the same bits as one that keeps every triangle vertex's SH, in less memory."""
+    dev = torch.device("cuda")
+    spec = workloads.sphere_field(seed=1, num_spheres=12, resolution=128, num_cameras=1,
+                                  image_shape=(64, 64), device=dev)
+    vox = te.extract_voxels(spec.occupancy.to(dev))
+    world, sdf, sh = workloads.finish_fields(spec, vox.vertices)
+    args = (vox, world.to(dev), sdf.to(dev), sh.to(dev), spec.level_sets.to(dev), spec.sh_degree,
+            spec.sharpness, spec.background)
+    per_vertex = serving.tessellate_scene(*args)
+    grid = serving.tessellate_scene(*args, colours="grid")
+    assert grid.spherical_harmonics is None and grid.grid is not None
+    assert torch.equal(grid.vertices, per_vertex.vertices) and torch.equal(grid.faces, per_vertex.faces)
+    assert grid.resident_bytes() < per_vertex.resident_bytes()
+    poses = _poses(3, 31, (64, 96)) + _poses(2, 32, (40, 72))
+    E = torch.stack([p[0] for p in poses]).cuda()
+    K = torch.stack([p[1] for p in poses]).cuda()
+    shapes = [p[2] for p in poses]
+    a = serving.render_batch(per_vertex, E, K, shapes, key_capacity=1 << 20)
+    b = serving.render_batch(grid, E, K, shapes, key_capacity=1 << 20)
+    torch.cuda.synchronize()
+    assert int(a.overflow) == 0 and int(b.overflow) == 0
+    for x, y in zip(a.images, b.images):
+        assert torch.equal(x, y)
+
+
+def test_arrivals_while_batches_are_in_flight(scenes):
+    """A second wave of requests submitted after the first wave's batches were launched
+    (and before they were retired) is admitted onto the free streams at once; polling
+    never blocks; every image equals the same request rendered alone."""
+    server = serving.RenderServer(scenes, hbm_budget_bytes=8 << 30, max_batch=4, num_streams=4,
+                                  keep_images=True)
+    wave1 = [_request(i, i % 2, p) for i, p in enumerate(_poses(6, 40, (64, 96)))]
+    for r in wave1:
+        server.submit(r)
+    launched1 = server.schedule()
+    assert launched1 == 2 and len(server.inflight) == 2  # one batch per scene
+    busy = sum(not job.event.query() for job in server.inflight)
+    wave2 = [_request(100 + i, i % 2, p) for i, p in enumerate(_poses(4, 41, (48, 80)))]
+    for r in wave2:
+        server.submit(r)
+    launched2 = server.schedule()
+    # Wave 2 took the two free streams while wave 1 was still counted in flight.
+    assert launched2 == 2 and len(server.inflight) == 4
+    print(f"wave 1 batches still running on the device when wave 2 was launched: {busy} of 2")
+    while server.queued or server.inflight:
+        server.poll()
+        server.schedule()
+    assert len(server.completed) == 10
+    for r in wave1 + wave2:
+        solo = serving.RenderServer(scenes, hbm_budget_bytes=8 << 30, max_batch=1, keep_images=True)
+        again = _request(r.rid, r.scene_id, (r.extrinsics, r.intrinsics, r.image_shape))
+        solo.submit(again)
+        solo.drain()
+        assert torch.equal(again.image, r.image), r.rid
