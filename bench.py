@@ -37,7 +37,7 @@ FP32_PEAK_TFLOPS = 157.3
 WORKLOADS = {
     "sphere-field-4cam": {},
     "sphere-field-32cam-mixed": {
-        "spheres": 256, "resolution": 512, "shells": 7, "cameras": 32, "camera_group": 16,
+        "spheres": 256, "resolution": 1024, "shells": 7, "cameras": 32, "camera_group": 4,
         "resolutions": ["320x240", "640x480", "1237x822", "1920x1080", "2473x1643"],
         "steps": 3, "warmup": 1,
     },
@@ -75,19 +75,24 @@ def parse_args(argv=None):
                    help="time only the batched step with --tile / --chunk (no plain path, no "
                         "autotuner, no CPU baseline): the run profiled per kernel")
     p.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
+    p.add_argument("--no-c3", action="store_true",
+                   help="skip the 32-camera mixed-resolution leg that follows the default workload")
+    p.add_argument("--c3-steps", type=int, default=3)
     p.add_argument("--only-serving", action="store_true",
                    help="run only the serving sweep (prints its JSON object)")
     p.add_argument("--no-mode-compare", action="store_true",
                    help="skip the fused-vs-explicit peak memory and gradient comparison")
     p.add_argument("--serve-spheres", type=int, nargs="+", default=[8, 16, 32, 64, 96, 128, 192, 256],
                    help="sphere count of each resident scene")
-    p.add_argument("--serve-resolution", type=int, default=512,
-                   help="virtual grid of the served scenes (512^3 gives ~1M..25M occupied cells "
-                        "for 8..256 spheres)")
+    p.add_argument("--serve-resolution", type=int, default=1024,
+                   help="virtual grid of the served scenes (the training generator's 1024^3)")
+    p.add_argument("--serve-colours", choices=("grid", "vertex"), default="grid",
+                   help="resident scenes keep the grid SH and colour each batch from it "
+                        "(grid), or keep every triangle vertex's SH (vertex)")
     p.add_argument("--serve-rates", type=float, nargs="+",
                    default=[50.0, 100.0, 200.0, 500.0, 1000.0, 2000.0])
     p.add_argument("--serve-seconds", type=float, default=3.0)
-    p.add_argument("--serve-budget-gib", type=int, default=128)
+    p.add_argument("--serve-budget-gib", type=int, default=96)
     p.add_argument("--serve-max-batch", type=int, default=16)
     p.add_argument("--serve-deadline", type=float, default=1.0,
                    help="requests not admitted within this many seconds of arrival are shed")
@@ -672,7 +677,7 @@ def serving_benchmark(args, device, log):
         del spec.occupancy
         sc = serving.tessellate_scene(vox, world.to(device), sdf.to(device), sh.to(device),
                                       spec.level_sets.to(device), spec.sh_degree, spec.sharpness,
-                                      spec.background)
+                                      spec.background, colours=args.serve_colours)
         cal = {}
         for shape in SERVE_SHAPES:
             poses = workloads.hemisphere_cameras(cal_gen, 4, shape)
@@ -682,6 +687,8 @@ def serving_benchmark(args, device, log):
         cells.append(int(vox.num_voxels))
         scenes[sid] = sc
         del vox, world, sdf, sh
+        log(f"[serve] scene {sid} ({ns} spheres): {cells[-1]} cells, {sc.faces.shape[0]} faces, "
+            f"{sc.resident_bytes() / 2**30:.1f} GiB resident")
     torch.cuda.synchronize()
     resident = torch.cuda.memory_allocated(device)
     log(f"[serve] {n} scenes ({min(cells)}..{max(cells)} cells) resident, "
@@ -715,6 +722,7 @@ def serving_benchmark(args, device, log):
         sweep.append(st)
     return {
         "scenes": n, "spheres": spheres, "cells": cells, "resolution": args.serve_resolution,
+        "colours": args.serve_colours,
         "shapes": [list(x) for x in SERVE_SHAPES], "hbm_budget_gib": args.serve_budget_gib,
         "max_batch": args.serve_max_batch, "deadline_s": args.serve_deadline,
         "resident_gib": round(resident / 2**30, 2),
@@ -723,6 +731,79 @@ def serving_benchmark(args, device, log):
 
 
 # ------------------------------------------------------------------------------------
+# The 32-camera mixed-resolution configuration (1024^3, 256 spheres, 7 shells), timed after
+# the headline workload with its autotuned tile and key-chunk.
+# ------------------------------------------------------------------------------------
+
+
+def c3_leg(args, device, log, tile_shape, chunk):
+    import copy
+
+    import torch
+
+    a = copy.copy(args)
+    for key, value in WORKLOADS["sphere-field-32cam-mixed"].items():
+        setattr(a, key, value)
+    log("[c3] building scene (1024^3, 256 spheres, 7 shells, 32 cameras of mixed resolution)")
+    t0 = time.time()
+    sc = build_scene(a, device)
+    torch.cuda.synchronize()
+    log(f"[c3] scene ready in {time.time() - t0:.1f} s: {sc.voxels.num_voxels} voxels")
+    torch.cuda.reset_peak_memory_stats()
+    base = torch.cuda.memory_allocated()
+    step = make_batched(sc, tile_shape, chunk, camera_group=a.camera_group)
+    inputs = (sc.world, sc.sdf.detach(), sc.sh.detach(), sc.target)
+    step.capture(*inputs)
+    for _ in range(a.warmup):
+        step.replay()
+    torch.cuda.synchronize()
+    log("[c3] captured; timing")
+    t0 = time.perf_counter()
+    for _ in range(args.c3_steps):
+        step.replay()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if step.overflowed():
+        raise RuntimeError("C3 batched step overflowed its capacities")
+    live = step.live_sizes()
+    peak = torch.cuda.max_memory_allocated() - base
+    ms = 1000 * elapsed / args.c3_steps
+    log(f"[c3] batched graph step: {ms:.1f} ms/step, loss {float(step.loss):.6f}, {live}, "
+        f"peak {peak / 2**30:.1f} GiB above the scene")
+    timer = LaunchTimer()
+    timer.start()
+    step.run(*inputs, hook=timer.hook)
+    torch.cuda.synchronize()
+    timer.collect()
+    table = {k: round(timer.step_ms(k, 1), 3) for k in sorted(timer.totals, key=lambda k: -timer.totals[k])
+             if k.startswith("mt_")}
+    out = {
+        "workload": workload_description(a, sc),
+        "steps": args.c3_steps, "warmup": a.warmup,
+        "ms_per_step": round(ms, 2), "steps_per_s": round(1000 / ms, 4),
+        "tile_shape": list(tile_shape), "chunk": chunk, "camera_group": a.camera_group,
+        "camera_groups": len(step.groups),
+        "sizes": {"N_vox": int(sc.voxels.num_voxels), "V_t": live["V_t"], "F": live["F"],
+                  "F_x3_over_2^31": round(3 * live["F"] / 2**31, 3), "K": live["K"],
+                  "K_max_group": live["K_max_group"], "pixels": sc.num_pixels},
+        "peak_gib_above_scene": round(peak / 2**30, 2),
+        "kernels_ms": table,
+        "plain_per_camera": {
+            "status": "not run: does not fit in HBM",
+            "reason": "the reference-structured step keeps every triangle vertex's SH "
+                      f"({16 * 12 * live['V_t'] / 2**30:.0f} GiB) and each render call's SH "
+                      "gradient of the same size plus its accumulation; the batched step "
+                      "never stores per-vertex SH. Images of one camera of every resolution "
+                      "are tested identical to the plain render (tests/test_gpu_config_c3_c4.py)",
+        },
+    }
+    del step, sc, inputs
+    import gc
+
+    gc.collect()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -902,8 +983,11 @@ def main():
         sub = subdivision_benchmark(device)
         log(f"[bench] subdivision: {sub}")
     serve = None
-    if not (args.no_serving or args.only_batched):
-        # Free the training step's arenas and the scene before loading the served scenes.
+    c3 = None
+    run_c3 = not (args.only_batched or args.no_c3) and args.workload == "sphere-field-4cam"
+    run_serve = not (args.no_serving or args.only_batched)
+    if run_c3 or run_serve:
+        # Free the training step's arenas and the scene before the next legs.
         import gc
 
         del step, step_args, with_stats, inputs
@@ -914,6 +998,9 @@ def main():
         torch.cuda.empty_cache()
         log(f"[bench] device memory after freeing the training step: "
             f"{torch.cuda.memory_allocated(device) / 2**30:.1f} GiB")
+    if run_c3:
+        c3 = c3_leg(args, device, log, best.tile_shape, best.chunk)
+    if run_serve:
         log("[bench] serving sweep")
         serve = serving_benchmark(args, device, log)
 
@@ -946,6 +1033,7 @@ def main():
         "regularizers": reg,
         "subdivision": sub,
         "render_modes": modes,
+        "c3": c3,
         "serving": serve,
         "cpu_baseline": cpu,
     }
