@@ -321,8 +321,10 @@ def algorithmic_bytes(z):
         "mt_evaluate_sh_forward": vt * (12 + nb) + b * vt * 12,
         "mt_intersect_and_composite_forward": 96 * k + 16 * px + 16 * tt,
         "mt_image_l2_loss": 16 * px + 12 * px + 16 * px,
+        # Gradients go only to the vertices of accepted hits (the flagged vertices, counted
+        # in the last group): per camera colour (12 B) and SDF (4 B), read and written.
         "mt_intersect_and_composite_backward": 96 * k + 32 * px + 16 * tt
-        + 2 * (12 * b * vt + 4 * g * vt),
+        + 2 * z.get("V_touched", vt) * (12 * b + 4 * g),
         "mt_evaluate_sh_backward": vt * (12 + nb) + b * vt * 12 + vt * nb,
         "mt_create_triangle_vertices_backward": n_lc * 16 + vt * (12 + 4 + nb) + 2 * n_lc * (4 + nb),
         # Fused vertex + colour kernels of the batched step (grid-vertex count ~ N_lc).

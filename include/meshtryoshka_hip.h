@@ -306,8 +306,8 @@ int mt_create_triangle_faces_i64(const int32_t* neighbors, const int32_t* indice
  * ------------------------------------------------------------------------------- */
 
 /* vertices f32[V, 3]; extrinsics f32[B, 4, 4]; intrinsics f32[B, 3, 3];
- * out_xy f32[B, V, 2]; out_depth f32[B, V]. Replaces project_vertices.slang:61-92
- * (.py:31-56). */
+ * out_xy f32[B, V, 2]; out_depth f32[B, V] (may be NULL: not written). Replaces
+ * project_vertices.slang:61-92 (.py:31-56). */
 int mt_project_vertices(const float* vertices, int64_t V, const float* extrinsics,
                         const float* intrinsics, int32_t B, float* out_xy, float* out_depth,
                         mt_stream_t stream);
@@ -395,6 +395,7 @@ int mt_generate_keys_single_pass(const float* xy, const float* depths, int64_t V
                                  mt_key_layout layout, int64_t* keys, int32_t* values,
                                  int64_t key_capacity, int64_t* n_keys, int32_t* overflow,
                                  void* workspace, size_t workspace_bytes, mt_stream_t stream);
+
 
 /* Workspace for mt_sort_keys on n pairs. */
 size_t mt_sort_workspace_bytes(int64_t n);

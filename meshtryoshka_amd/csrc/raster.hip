@@ -16,6 +16,24 @@ constexpr int kBlock = 256;
 // Projection (project_vertices.slang:51-92)
 // ------------------------------------------------------------------------------------
 
+// One world vertex through one camera (project_vertices.slang:51-69): p = E [v, 1] summed
+// left to right, depth = p.z, xy = (K p).xy / depth. e: the 4 x 4 world-to-camera matrix
+// (row-major), k: the 3 x 3 intrinsics.
+struct Projected {
+  float x, y, depth;
+};
+
+__device__ __forceinline__ Projected project_point(const float* e, const float* k, float x,
+                                                   float y, float z) {
+  float cs[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) cs[r] = ((e[4 * r + 0] * x + e[4 * r + 1] * y) + e[4 * r + 2] * z) + e[4 * r + 3];
+  const float depth = cs[2];
+  const float px = (k[0] * cs[0] + k[1] * cs[1]) + k[2] * cs[2];
+  const float py = (k[3] * cs[0] + k[4] * cs[1]) + k[5] * cs[2];
+  return Projected{px / depth, py / depth, depth};
+}
+
 // One thread per (vertex, camera); the B blocks of a vertex range are consecutive, so the
 // vertices are read from HBM once and from L2 by the other cameras' blocks.
 __global__ void __launch_bounds__(kBlock) project_kernel(const float* verts, int64_t V,
@@ -25,19 +43,10 @@ __global__ void __launch_bounds__(kBlock) project_kernel(const float* verts, int
   const int64_t v = int64_t(blockIdx.x / static_cast<unsigned>(B)) * blockDim.x + threadIdx.x;
   if (v >= V) return;
   const float x = verts[3 * v + 0], y = verts[3 * v + 1], z = verts[3 * v + 2];
-  {
-    const float* e = E + 16 * b;
-    const float* k = K + 9 * b;
-    float cs[3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) cs[r] = ((e[4 * r + 0] * x + e[4 * r + 1] * y) + e[4 * r + 2] * z) + e[4 * r + 3];
-    const float depth = cs[2];
-    const float px = (k[0] * cs[0] + k[1] * cs[1]) + k[2] * cs[2];
-    const float py = (k[3] * cs[0] + k[4] * cs[1]) + k[5] * cs[2];
-    const int64_t o = int64_t(b) * V + v;
-    reinterpret_cast<float2*>(out_xy)[o] = make_float2(px / depth, py / depth);
-    out_d[o] = depth;
-  }
+  const Projected p = project_point(E + 16 * b, K + 9 * b, x, y, z);
+  const int64_t o = int64_t(b) * V + v;
+  reinterpret_cast<float2*>(out_xy)[o] = make_float2(p.x, p.y);
+  if (out_d != nullptr) out_d[o] = p.depth;
 }
 
 // ------------------------------------------------------------------------------------
@@ -649,7 +658,7 @@ keys_single_pass_kernel(
     const uint32_t col = (s_org[lo] & 0xffffu) + uint32_t(local % w);
     const uint32_t tile = row * uint32_t(cols) + col;
     const uint64_t key = s_key_hi[lo] | ((uint64_t(tile) & tile_mask) << kf.tile_offset);
-    if (base + j >= kcap) {
+    if (base + j < 0 || base + j >= kcap) {
       if (overflow) atomicOr(overflow, 1);
       continue;
     }
@@ -912,6 +921,13 @@ extern "C" int mt_generate_keys_single_pass(
   MT_REQUIRE(cam_dims != nullptr ||
                  (row_min >= 0 && col_min >= 0 && row_max <= 65535 && col_max <= 65535),
              "mt_generate_keys_single_pass: the tile grid must lie in [0, 65535]^2");
+  // A block's key count (kPassFaces pairs, each at most the whole tile grid) is an int32;
+  // with per-camera grids (cam_dims) the caller checks the largest grid the same way.
+  MT_REQUIRE(cam_dims != nullptr ||
+                 int64_t(mt::kPassFaces) * (int64_t(row_max) - row_min) *
+                         (int64_t(col_max) - col_min) < (int64_t(1) << 31),
+             "mt_generate_keys_single_pass: %d x %d tiles per camera overflow a block's key count",
+             row_max - row_min, col_max - col_min);
   const size_t need = mt::single_pass_state_bytes(F, B);
   if (workspace == nullptr || workspace_bytes < need) {
     mt::set_error("mt_generate_keys_single_pass: workspace of %zu bytes is smaller than the %zu required",

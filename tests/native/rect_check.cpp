@@ -83,6 +83,30 @@ Tri random_triangle(int kind, double cx, double cy) {
       }
       break;
     }
+    case 5: {  // small, marching-cubes-like triangles (0.01 .. 3 px), any orientation
+      const double s = std::pow(10.0, uniform(-2.0, 0.5));
+      const double a = uniform(0.0, 6.283185307179586);
+      const double b = a + uniform(0.05, 3.09), r1 = s * uniform(0.2, 1.0), r2 = s * uniform(0.2, 1.0);
+      t.x[0] = float(cx);
+      t.y[0] = float(cy);
+      t.x[1] = float(cx + r1 * std::cos(a));
+      t.y[1] = float(cy + r1 * std::sin(a));
+      t.x[2] = float(cx + r2 * std::cos(b));
+      t.y[2] = float(cy + r2 * std::sin(b));
+      break;
+    }
+    case 6: {  // one small angle (2..6 degrees) at the first corner
+      const double s = std::pow(10.0, uniform(-2.0, 2.0));
+      const double a = uniform(0.0, 6.283185307179586), d = uniform(0.035, 0.105);
+      const double r1 = s * uniform(0.3, 1.0), r2 = s * uniform(0.3, 1.0);
+      t.x[0] = float(cx);
+      t.y[0] = float(cy);
+      t.x[1] = float(cx + r1 * std::cos(a));
+      t.y[1] = float(cy + r1 * std::sin(a));
+      t.x[2] = float(cx + r2 * std::cos(a + d));
+      t.y[2] = float(cy + r2 * std::sin(a + d));
+      break;
+    }
     default: {  // far away or huge coordinates
       const double s = std::pow(10.0, uniform(3.0, 9.0));
       for (int i = 0; i < 3; ++i) {
@@ -103,7 +127,7 @@ Tri random_triangle(int kind, double cx, double cy) {
 
 int main(int argc, char** argv) {
   const long n = argc > 1 ? std::atol(argv[1]) : 200000;
-  if (argc > 2) g_state = std::strtoull(argv[2], nullptr, 10) | 1ull;
+  if (argc > 2) g_state ^= (std::strtoull(argv[2], nullptr, 10) + 1) * 0xD1B54A32D192ED03ull;
   // (h, w): the autotuner's tile shapes, and multi-tile regions (the key kernels bound a
   // face's whole tile-aligned bounding box at once).
   static const int shapes[8][2] = {{8, 8},   {16, 16}, {32, 8}, {16, 32},
@@ -114,7 +138,7 @@ int main(int argc, char** argv) {
     const int th = sh[0], tw = sh[1];
     const int x0 = int(next_u64() % 160) * tw, y0 = int(next_u64() % 100) * th;
     const double cx = x0 + uniform(-4.0, tw + 4.0), cy = y0 + uniform(-4.0, th + 4.0);
-    const Tri t = random_triangle(int(next_u64() % 6), cx, cy);
+    const Tri t = random_triangle(int(next_u64() % 8), cx, cy);
     const mt::TriTerms terms = mt::tri_terms(t.x[0], t.y[0], t.x[1], t.y[1], t.x[2], t.y[2]);
     const mt::Rect r = mt::candidate_rect(terms, x0, y0, tw, th);
     area += (long long)r.w * r.h;
