@@ -268,6 +268,29 @@ int mt_upscale_parameters(const int32_t* old_vertex_occupancy, const int32_t* ol
                           int32_t NI, int32_t NJ, int32_t NKP, float* new_sdf, float* new_sh,
                           int64_t new_num_vertices, mt_stream_t stream);
 
+/* Vertex welding. A point p = points[i] f32[n, 3] is quantised per coordinate as
+ * q = rint(p * fl32(1 / tol)) (the reciprocal in double, rounded to float32, then a float32
+ * product rounded half to even: torch.round(points / tol) as ATen evaluates it on the device).
+ * mt_quantized_max_abs: max |q| over all coordinates into the device int64 *max_abs (non-finite
+ * values or magnitudes of 2^62 or more count as 2^62). Replaces index_vertices.py:17-18.
+ * mt_index_vertices: out int32[n], out[i] = the smallest input index j with q_j == q_i.
+ * max_abs must be at least the device value mt_quantized_max_abs wrote (the caller reads it
+ * back, as the reference does); bits = bit_length(max_abs) + 1 and 3 bits must not exceed 63
+ * (MT_ERR_INVALID_ARGUMENT, message "need <3 bits> bits, exceed 63 bits total"). The three
+ * coordinates are packed into one key as q + max_abs, so negative coordinates are packed
+ * without aliasing. Replaces index_vertices (index_vertices.py:6-30).
+ * mt_index_keys: out[i] = the smallest input index j with keys[j] == keys[i] (any int64 keys).
+ * Replaces index_keys (index_vertices.py:33-51).
+ * Both sort (key, input index) pairs with mt_sort_keys; the workspace is
+ * mt_index_vertices_workspace_bytes(n) for either; n < 2^31. */
+int mt_quantized_max_abs(const float* points, int64_t n, double tol, int64_t* max_abs,
+                         mt_stream_t stream);
+size_t mt_index_vertices_workspace_bytes(int64_t n);
+int mt_index_vertices(const float* points, int64_t n, double tol, int64_t max_abs, int32_t* out,
+                      void* workspace, size_t workspace_bytes, mt_stream_t stream);
+int mt_index_keys(const int64_t* keys, int64_t n, int32_t* out, void* workspace,
+                  size_t workspace_bytes, mt_stream_t stream);
+
 /* face_counts int32[S * num_voxels] (3 x triangles) and cell_codes uint8[S, num_voxels].
  * Replaces count_triangle_faces.slang:13-50 (.py:28-63). upper_corners
  * int32[4, num_voxels]. */

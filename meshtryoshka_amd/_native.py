@@ -20,6 +20,7 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libmeshtryoshka_hip.so"
 c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64
 c_f32 = ctypes.c_float
+c_f64 = ctypes.c_double
 c_p = ctypes.c_void_p
 c_sz = ctypes.c_size_t
 
@@ -113,6 +114,10 @@ SIGNATURES = {
     "mt_upscale_parameters": (
         c_i32, [c_p, c_p, c_i32, c_i32, c_i32, c_p, c_p, c_i64, c_i32, c_p, c_p, c_i32, c_i32,
                 c_i32, c_p, c_p, c_i64, c_p]),
+    "mt_quantized_max_abs": (c_i32, [c_p, c_i64, c_f64, c_p, c_p]),
+    "mt_index_vertices_workspace_bytes": (c_sz, [c_i64]),
+    "mt_index_vertices": (c_i32, [c_p, c_i64, c_f64, c_i64, c_p, c_p, c_sz, c_p]),
+    "mt_index_keys": (c_i32, [c_p, c_i64, c_p, c_p, c_sz, c_p]),
     "mt_compute_sdf_regularizers_forward": (
         c_i32, [c_p, c_p, c_p, c_i64, c_p, c_i64, c_i32, c_i32, c_i32, c_p, c_p, c_p, c_p]),
     "mt_compute_sdf_regularizers_backward": (

@@ -31,6 +31,7 @@ from .create_voxels import Voxels, create_voxels
 from .subdivision import UpscaledParameters as UpscaledParameters
 from .subdivision import dilate_occupancy as dilate_occupancy
 from .subdivision import index_vertices as index_vertices
+from .subdivision import index_keys as index_keys
 from .subdivision import upscale_occupancy as upscale_occupancy
 from .subdivision import upscale_parameters as upscale_parameters
 from .subdivision import write_occupancy as write_occupancy
@@ -120,7 +121,7 @@ __all__ = [
     "compute_exclusive_cumsum", "compute_sdf_regularizers", "compute_vertex_occupancy",
     "count_occupancy", "count_triangle_faces", "count_triangle_vertices",
     "create_triangle_faces", "create_triangle_vertices", "create_voxels", "dilate_occupancy",
-    "extract_voxels", "get_occupancy_mesh", "index_vertices", "pack_occupancy",
+    "extract_voxels", "get_occupancy_mesh", "index_keys", "index_vertices", "pack_occupancy",
     "tessellate_voxels", "unpack_occupancy", "upscale_occupancy", "upscale_parameters",
     "write_occupancy",
 ]

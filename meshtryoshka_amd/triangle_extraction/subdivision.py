@@ -101,27 +101,48 @@ def upscale_parameters(old_signed_distances: Tensor, old_spherical_harmonics: Te
     return UpscaledParameters(new_sdf, new_sh, new_occupancy)
 
 
+@record_function("index_vertices")
 def index_vertices(points: Tensor, tol: float = 1e-5) -> Tensor:
-    """int32[V]: the index of the first point (in input order) that lies on the same
-    ``tol``-quantised position (index_vertices.py:6-30)."""
-    q = torch.round(points / tol).to(torch.int64)
-    max_abs = int(q.abs().max().item()) if q.numel() else 0
+    """int32[V]: for every point, the input index of the first point (in input order) at the
+    same ``tol``-quantised position (index_vertices.py:6-30). A coordinate is quantised as
+    ``torch.round(points / tol)`` is on the device (a float32 multiply by ``1 / tol`` rounded
+    to float32, ties to even). The largest quantised magnitude is read back to size
+    the packed key, and a mesh that needs more than 63 key bits raises RuntimeError, as in the
+    reference. Coordinates are offset by that magnitude before packing, so negative
+    coordinates never alias (the reference packs them unbiased). An empty input gives an
+    empty output."""
+    _native.require_device(points)
+    assert points.dim() == 2 and points.shape[1] == 3 and points.dtype == torch.float32
+    points = points.contiguous()
+    (n, _) = points.shape
+    out = torch.empty((n,), dtype=torch.int32, device=points.device)
+    if n == 0:
+        return out
+    st = _native.stream(points)
+    max_abs = torch.empty((), dtype=torch.int64, device=points.device)
+    _native.call("mt_quantized_max_abs", points.data_ptr(), n, float(tol), max_abs.data_ptr(), st)
+    max_abs = int(max_abs.item())
     bits = max_abs.bit_length() + 1
     if bits * 3 > 63:
-        raise RuntimeError(f"need {bits*3} bits, exceed 63 bits total")
-    x, y, z = q.unbind(dim=1)
-    keys = (x << (2 * bits)) | (y << bits) | z
-    return index_keys(keys)
+        raise RuntimeError(f"need {bits * 3} bits, exceed 63 bits total")
+    ws = _native.workspace(_native.query("mt_index_vertices_workspace_bytes", n), points.device)
+    _native.call("mt_index_vertices", points.data_ptr(), n, float(tol), max_abs,
+                 out.data_ptr(), ws.data_ptr(), ws.numel(), st)
+    return out
 
 
+@record_function("index_keys")
 def index_keys(keys: Tensor) -> Tensor:
-    """int32[V]: for every key, the input index of its first occurrence (a stable sort, so
-    that "first" is the first in input order; index_vertices.py:33-51)."""
+    """int32[N]: for every int64 key, the input index of its first occurrence in input order
+    (index_vertices.py:33-51)."""
+    _native.require_device(keys)
+    assert keys.dim() == 1 and keys.dtype == torch.int64
+    keys = keys.contiguous()
     (n,) = keys.shape
-    sorted_keys, order = torch.sort(keys, stable=True)
-    _, inverse, counts = torch.unique_consecutive(sorted_keys, return_inverse=True,
-                                                  return_counts=True)
-    first_of_run = (counts.cumsum(dim=0) - counts)[inverse]
-    rank = torch.empty_like(order)
-    rank[order] = torch.arange(n, device=keys.device)
-    return order[first_of_run[rank]].to(torch.int32)
+    out = torch.empty((n,), dtype=torch.int32, device=keys.device)
+    if n == 0:
+        return out
+    ws = _native.workspace(_native.query("mt_index_vertices_workspace_bytes", n), keys.device)
+    _native.call("mt_index_keys", keys.data_ptr(), n, out.data_ptr(), ws.data_ptr(), ws.numel(),
+                 _native.stream(keys))
+    return out

@@ -125,12 +125,32 @@ def upscale_parameters(old_sdf, old_sh, old_packed, to_upscale_packed, new_resol
     return (sd * m).astype(F32), (sh * m[None, :, None]).astype(F32), new_packed
 
 
+def quantize(points, tol: float) -> np.ndarray:
+    """index_vertices.py:17, ``torch.round(points / tol).to(torch.int64)`` as torch evaluates
+    it on the GPU, where a float32 tensor divided by a host scalar is multiplied in float32 by
+    the scalar's reciprocal, taken in double and rounded to float32; np.rint rounds half to
+    even, as torch.round does. (On the CPU torch divides in float32; the two differ for about
+    one point in a thousand at tol = 1e-5.)"""
+    inv = F32(1.0 / tol)
+    return np.rint(np.asarray(points, dtype=F32) * inv).astype(np.int64)
+
+
+def index_keys(keys) -> np.ndarray:
+    """index_vertices.py:33-51 by brute force: the input index of each key's first occurrence."""
+    first = {}
+    keys = np.asarray(keys, dtype=np.int64)
+    out = np.empty(keys.shape[0], dtype=np.int32)
+    for n, key in enumerate(keys.tolist()):
+        out[n] = first.setdefault(key, n)
+    return out
+
+
 def index_vertices(points, tol: float = 1e-5) -> np.ndarray:
-    """index_vertices.py:6-51 by brute force: the index of the first point with the same
-    tol-quantised coordinates."""
-    q = np.round(np.asarray(points, dtype=F32) / F32(tol)).astype(np.int64)  # float32, as torch
+    """index_vertices.py:6-51 by brute force: the index of the first point (input order) with
+    the same quantised coordinates, compared as coordinate triples (no key packing)."""
+    q = quantize(points, tol)
     first = {}
     out = np.empty(q.shape[0], dtype=np.int32)
-    for n, key in enumerate(map(tuple, q)):
+    for n, key in enumerate(map(tuple, q.tolist())):
         out[n] = first.setdefault(key, n)
     return out

@@ -1,7 +1,7 @@
 """Subdivision ops on the GPU against the oracle, bit for bit: write / dilate / upscale
 occupancy (integers) and the SDF / SH resampling of upscale_parameters (FP32 in the
-reference's operation order), at integer and non-integer upscaling ratios; index_vertices on
-device tensors."""
+reference's operation order), at integer and non-integer upscaling ratios; vertex welding
+(index_vertices, index_keys) against first-seen brute force."""
 
 import numpy as np
 import pytest
@@ -63,12 +63,69 @@ def test_upscale_parameters(target, nsh):
     np.testing.assert_array_equal(got.spherical_harmonics.cpu().numpy(), want_sh)
 
 
-def test_index_vertices_on_device():
-    rng = np.random.default_rng(6)
-    base = rng.random((300, 3)).astype(np.float32)
-    pts = base[rng.integers(0, 300, 2000)]
-    got = te.index_vertices(torch.from_numpy(pts).cuda(), 1e-6)
-    assert np.array_equal(got.cpu().numpy(), osd.index_vertices(pts, 1e-6))
+def _welding_points(seed, n, spread):
+    """Points repeated in runs that cross the kernel's 2048-position tiles, negative
+    coordinates, ties at the quantisation step, and near-duplicates (within tol)."""
+    rng = np.random.default_rng(seed)
+    base = (rng.random((spread, 3)) * 2 - 1).astype(np.float32)
+    pts = base[rng.integers(0, spread, n)]
+    jitter = rng.random(n) < 0.2
+    pts[jitter] += rng.normal(0, 2e-7, (int(jitter.sum()), 3)).astype(np.float32)
+    ties = rng.random(n) < 0.05
+    pts[ties] = (rng.integers(-50, 50, (int(ties.sum()), 3)) + 0.5).astype(np.float32) * 1e-5
+    return pts
+
+
+def test_quantisation_matches_torch_on_device():
+    """The oracle's quantisation is torch.round(points / tol) as torch computes it on the
+    device, including the points where float32 division and multiplication by the float32
+    reciprocal round differently."""
+    rng = np.random.default_rng(11)
+    pts = rng.random((400_000, 3)).astype(np.float32) * 2 - 1
+    for tol in (1e-5, 1e-6, 1e-3):
+        want = osd.quantize(pts, tol)
+        got = torch.round(torch.from_numpy(pts).cuda() / tol).to(torch.int64).cpu().numpy()
+        differs = np.round(pts / np.float32(tol)).astype(np.int64) != want
+        assert differs.any()  # the points tell the two formulas apart
+        assert np.array_equal(got, want), tol
+
+
+@pytest.mark.parametrize("n,spread,tol", [(1, 1, 1e-5), (2048, 3, 1e-5), (2049, 700, 1e-6),
+                                          (300_000, 40_000, 1e-5), (300_000, 2, 1e-6)])
+def test_index_vertices_on_device(n, spread, tol):
+    pts = _welding_points(n + spread, n, spread)
+    got = te.index_vertices(torch.from_numpy(pts).cuda(), tol)
+    assert got.dtype == torch.int32
+    assert np.array_equal(got.cpu().numpy(), osd.index_vertices(pts, tol))
+
+
+def test_index_vertices_one_point_repeated():
+    """A single run over every tile: each tile finds the head in the first tile."""
+    pts = np.tile(np.float32([[0.25, -0.5, 0.125]]), (100_000, 1))
+    got = te.index_vertices(torch.from_numpy(pts).cuda(), 1e-5)
+    assert not bool(got.any())
+
+
+def test_index_vertices_limits():
+    dev = torch.device("cuda")
+    assert te.index_vertices(torch.empty((0, 3), device=dev), 1e-5).shape == (0,)
+    with pytest.raises(RuntimeError, match="exceed 63 bits total"):
+        te.index_vertices(torch.tensor([[0.0, 0.0, 1e3]], device=dev), 1e-5)  # |q| = 1e8: 3 x 28 bits
+    with pytest.raises(RuntimeError, match="exceed 63 bits total"):
+        te.index_vertices(torch.tensor([[0.0, float("nan"), 0.0]], device=dev), 1e-5)
+    # 1e3 / 1e-2 = 1e5 < 2^17: 18 bits a field, 54 in all, accepted.
+    got = te.index_vertices(torch.tensor([[1e3, 0, 0], [-1e3, 0, 0], [1e3, 0, 0]], device=dev), 1e-2)
+    assert got.tolist() == [0, 1, 0]
+
+
+@pytest.mark.parametrize("n,distinct", [(1, 1), (2047, 5), (2048, 2048), (4097, 1),
+                                        (500_000, 1000), (500_000, 400_000)])
+def test_index_keys_on_device(n, distinct):
+    rng = np.random.default_rng(n + distinct)
+    pool = rng.integers(-(2**63), 2**63 - 1, distinct, dtype=np.int64, endpoint=True)
+    keys = pool[rng.integers(0, distinct, n)]
+    got = te.index_keys(torch.from_numpy(keys).cuda())
+    assert np.array_equal(got.cpu().numpy(), osd.index_keys(keys))
 
 
 def test_empty_inputs():

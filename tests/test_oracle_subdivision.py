@@ -1,14 +1,13 @@
 """Pins the subdivision oracle (oracle/subdivision.py): dilation against scipy's binary
 dilation with a cube, upscaling against the identity and against 2x block repetition,
 parameter resampling against exact reproduction of old vertices and of linear and constant
-fields, index_vertices (the product's torch function) against brute force."""
+fields, the vertex quantisation against exact arithmetic and index_vertices / index_keys
+against hand-computed first-seen indices."""
 
 import numpy as np
 import pytest
-import torch
 from scipy import ndimage
 
-from meshtryoshka_amd.triangle_extraction.subdivision import index_vertices
 from oracle import subdivision as osd
 from oracle.extraction import pack_occupancy, unpack_occupancy
 
@@ -84,10 +83,33 @@ def test_upscale_parameters_keeps_constants_under_partial_occupancy():
     assert np.allclose(new_sdf, 0.375, rtol=1e-6) and np.allclose(new_sh, -1.5, rtol=1e-6)
 
 
-def test_index_vertices_first_seen():
-    rng = np.random.default_rng(5)
-    base = rng.random((200, 3)).astype(np.float32)
-    pts = base[rng.integers(0, 200, 1000)] + rng.normal(0, 1e-8, (1000, 3)).astype(np.float32)
-    got = index_vertices(torch.from_numpy(pts), 1e-5).numpy()
-    want = osd.index_vertices(pts, 1e-5)
-    assert np.array_equal(got, want)
+def test_quantize_is_the_float32_reciprocal_product():
+    """oracle.quantize follows torch's device arithmetic, rint(p * fl32(1 / tol)):
+    checked against exact arithmetic (a float32 times 100000 is exact in float64, so one
+    rounding to float32 gives the float32 product) on points where dividing by tol would
+    round the other way."""
+    tol = 1e-5
+    assert np.float32(1.0 / tol) == np.float32(100000.0)
+    pts = np.array([0.260195, 0.470165, 0.706965, 0.25, 0.75, 1e-5, 2.5e-5, -3.5e-5],
+                   dtype=np.float32)
+    want = [round(float(np.float32(float(p) * 100000.0))) for p in pts]  # ties to even
+    assert want[:3] == [26019, 47016, 70696] and want[5:] == [1, 2, -4]
+    assert np.array_equal(osd.quantize(pts[:, None], tol)[:, 0], want)
+    divided = np.round(pts / np.float32(tol)).astype(np.int64)
+    assert not np.array_equal(divided, want)  # the plausible mistake is caught
+
+
+def test_index_vertices_known_answer():
+    """Hand-computed first-seen indices at tol = 0.5 (q = rint(2 p)): ties go to even, -0
+    equals 0, and (0, -1, 0) / (3, -1, 2) stay distinct (an unbiased shift-and-or packing of
+    negative coordinates would merge them)."""
+    pts = np.array([[0, 0, 0], [1, 1, 1], [0.1, 0, 0], [1.2, 0.9, 1.1], [-1, -1, -1],
+                    [0.25, 0, 0], [0.75, 0, 0], [-0.0, 0, 0], [0, -0.5, 0], [1.5, -0.5, 1],
+                    [-1.1, -0.9, -1.2], [0.0, -0.6, 0.2]], dtype=np.float32)
+    want = [0, 1, 0, 1, 4, 0, 6, 0, 8, 9, 4, 8]
+    assert osd.index_vertices(pts, 0.5).tolist() == want
+
+
+def test_index_keys_known_answer():
+    keys = np.array([5, -3, 5, 7, -3, 5, -(2**63), 2**63 - 1, -(2**63)], dtype=np.int64)
+    assert osd.index_keys(keys).tolist() == [0, 1, 0, 3, 1, 0, 6, 7, 6]
