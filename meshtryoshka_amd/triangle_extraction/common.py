@@ -1,17 +1,11 @@
-"""Helpers shared by the extraction ops (triangle_extraction/common.py:13-51)."""
-
-from functools import wraps
-from typing import TypeVar
+"""Occupancy bit packing shared by the extraction ops (triangle_extraction/common.py:17-35).
+``ceildiv`` and the profiler-range decorator are the rasterizer's."""
 
 import torch
 from torch import Tensor
-from torch.profiler import record_function as record_function_torch
 
-T = TypeVar("T")
-
-
-def ceildiv(numerator: T, denominator: T) -> T:
-    return (numerator + denominator - 1) // denominator
+from ..triangle_rasterization.common import ceildiv as ceildiv
+from ..triangle_rasterization.common import record_function as record_function
 
 
 def pack_occupancy(occupancy: Tensor) -> Tensor:
@@ -33,17 +27,3 @@ def unpack_occupancy(packed_occupancy: Tensor) -> Tensor:
     bits = (occupancy[..., None] & powers) != 0
     i, j, kp = packed_occupancy.shape
     return bits.reshape(i, j, kp * 32)
-
-
-def record_function(tag: str):
-    """torch.profiler.record_function that keeps the wrapped signature."""
-
-    def decorator(f):
-        @wraps(f)
-        def wrapper(*args, **kwargs):
-            with record_function_torch(tag):
-                return f(*args, **kwargs)
-
-        return wrapper
-
-    return decorator

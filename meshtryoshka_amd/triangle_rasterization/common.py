@@ -1,53 +1,59 @@
-"""Tile grid and helpers (triangle_rasterization/common.py:11-60)."""
+"""Image tiling shared by the rasterizer ops. ``TileGrid`` keeps the reference's fields and
+properties (triangle_rasterization/common.py:11-60), since callers construct it by keyword
+and read ``grid_shape``, ``num_tiles_per_image`` and ``tile_numel``."""
 
 from dataclasses import dataclass
 from functools import wraps
-from math import prod
-from typing import TypeVar
 
-from torch.profiler import record_function as record_function_torch
-
-T = TypeVar("T")
+from torch.profiler import record_function as _profiler_range
 
 
-def ceildiv(numerator: T, denominator: T) -> T:
-    return (numerator + denominator - 1) // denominator
+def ceildiv(numerator, denominator):
+    """Integer division rounded towards positive infinity (for positive divisors)."""
+    return -(-numerator // denominator)
 
 
 @dataclass(frozen=True)
 class TileGrid:
-    # One tile's shape in pixels as (height, width).
+    """How a batch of images is cut into tiles.
+
+    ``tile_shape`` and ``image_shape`` are (height, width) in pixels; a partial tile at the
+    bottom or right edge counts as a whole tile. ``num_cameras`` images are rendered, each
+    ``num_shells`` times (one layer per level-set shell)."""
+
     tile_shape: tuple[int, int]
-    # The entire image's shape in pixels as (height, width).
     image_shape: tuple[int, int]
-    # The number of cameras being rendered.
     num_cameras: int
-    # The number of shells being rendered.
     num_shells: int
 
     @property
     def grid_shape(self) -> tuple[int, int]:
-        return tuple(ceildiv(i, t) for i, t in zip(self.image_shape, self.tile_shape))
+        """Tiles per image as (rows, columns)."""
+        (ih, iw), (th, tw) = self.image_shape, self.tile_shape
+        return (ceildiv(ih, th), ceildiv(iw, tw))
 
     @property
     def num_tiles_per_image(self) -> int:
-        return prod(self.grid_shape)
+        rows, cols = self.grid_shape
+        return rows * cols
 
     @property
     def tile_numel(self) -> int:
-        h, w = self.tile_shape
-        return h * w
+        """Pixels in one (whole) tile."""
+        return self.tile_shape[0] * self.tile_shape[1]
 
 
 def record_function(tag: str):
-    """torch.profiler.record_function that keeps the wrapped signature."""
+    """Decorator that runs the function inside a profiler range named ``tag`` (visible in
+    torch and rocprof traces); the wrapper carries the function's name, docstring and
+    signature."""
 
-    def decorator(f):
-        @wraps(f)
-        def wrapper(*args, **kwargs):
-            with record_function_torch(tag):
-                return f(*args, **kwargs)
+    def attach(fn):
+        @wraps(fn)
+        def ranged(*args, **kwargs):
+            with _profiler_range(tag):
+                return fn(*args, **kwargs)
 
-        return wrapper
+        return ranged
 
-    return decorator
+    return attach
