@@ -31,10 +31,13 @@ import itertools
 import time
 from collections import deque
 from dataclasses import dataclass, field
+from typing import Mapping
 
 import torch
 
 from . import _native
+from .tesseract import (Contraction, TesseractLayout, _initializer_offset, extract_tesseract_voxels,
+                        level_sets_from_shells, raw_from_sdf, sdf_from_raw, split_occupancy)
 from .triangle_rasterization.keys import create_key_defines, to_layout
 
 _LAYOUT = to_layout(create_key_defines(("camera", 6), ("tile", 20), ("depth", 32), ("shell", 6)))
@@ -143,93 +146,145 @@ def tessellate_scene(voxels, grid_vertices, grid_sdf, grid_sh, level_sets, sh_de
 
 
 # ------------------------------------------------------------------------------------
-# Scenes on disk: the reference trainer's checkpoint layout (trainer.py:111-137): a dict with
-# "model" = the model's state dict, whose tesseract entries hold the packed occupancy, the
-# raw per-vertex SDF and the SH split into the DC band and the rest (tesseract.py:161-214).
-# The reference keeps the grid shape, level sets and sharpness in its config; a checkpoint
-# written here carries them in an extra "scene" entry, and for a reference checkpoint they
-# are passed in. Single-component tesseracts (one cubic grid) only.
+# Scenes on disk, in the reference trainer's checkpoint layout (trainer.py:111-137): a dict
+# whose "model" entry is the model's state dict. Its tesseract entries hold the flat packed
+# occupancy of all components, the raw per-vertex SDF parameters (one row per component
+# vertex) and the SH split into the DC band and the rest (tesseract.py:161-214); "step" and
+# "tesseract.step" hold the training step. The reference keeps the grid layout, initializer,
+# shells, sharpness and contraction in its config: for its files they are passed in. A file
+# written here carries them in an extra "scene" entry as well.
 # ------------------------------------------------------------------------------------
 
 
+def _layout_dict(layout: TesseractLayout) -> dict:
+    def plain(x):
+        return {int(k): int(v) for k, v in x.items()} if isinstance(x, Mapping) else int(x)
+
+    return {"center_resolution": plain(layout.center_resolution),
+            "arm_resolution": plain(layout.arm_resolution),
+            "center_extent": float(layout.center_extent),
+            "disable_tesseract": bool(layout.disable_tesseract)}
+
+
 def save_scene_checkpoint(path, occupancy, signed_distances, spherical_harmonics, level_sets,
-                          sh_degree, sharpness, background, step: int = 0) -> None:
-    """Writes a scene (packed occupancy int32[i, j, k/32], grid SDF f32[V], grid SH
-    f32[NSH, V, 3]) in the checkpoint layout above."""
-    i, j, kp = occupancy.shape
-    sh = spherical_harmonics.detach().cpu()
+                          sh_degree, sharpness, background, *, initializer, step: int = 0,
+                          layout: TesseractLayout | None = None, contraction: Contraction | None = None
+                          ) -> None:
+    """Writes a scene in the layout above. ``occupancy`` is the flat packed occupancy of the
+    layout's components (or one cubic grid int32[n, n, n / 32], ``layout`` then defaulting
+    to that single grid); ``signed_distances`` f32[V] is the SDF to render at the
+    components' vertices and ``spherical_harmonics`` f32[NSH, V, 3]. ``initializer`` (as in
+    the reference's config: {"name": "sphere", "radius": r}, {"name": "point_cloud", ...} or
+    {"name": "none"}) says what a reader adds to the stored parameters, so they are stored
+    as the SDF minus that offset at ``step`` (to float32 rounding); the vertices it needs
+    are extracted on the occupancy's device."""
+    if layout is None:
+        n = occupancy.shape[0]
+        if occupancy.dim() != 3 or tuple(occupancy.shape) != (n, n, n // 32):
+            raise ValueError("without a layout the occupancy must be one cubic grid int32[n, n, n/32]")
+        layout = TesseractLayout.single_grid(n)
+    sdf = signed_distances.detach().to(torch.float32)
+    # Refuses unknown initializers and offsets that need data outside the file.
+    if _initializer_offset(initializer, torch.zeros((1, 3)), step) is not None:
+        vox, _ = extract_tesseract_voxels(occupancy, layout, step)
+        sdf = raw_from_sdf(sdf.to(vox.vertices.device), vox.vertices, initializer, step)
+    sh = spherical_harmonics.detach().cpu().to(torch.float32)
     model = {
+        "step": torch.tensor([step], dtype=torch.int32),
+        "background": torch.as_tensor(background, dtype=torch.float32).cpu(),
         "tesseract.step": torch.tensor(step, dtype=torch.int32),
         "tesseract.occupancy": occupancy.detach().cpu().reshape(-1).to(torch.int32),
-        "tesseract.signed_distances_raw": signed_distances.detach().cpu().to(torch.float32),
-        "tesseract.sh_rgb": sh[:1].to(torch.float32),
-        "tesseract.sh_rest": sh[1:].to(torch.float32) if sh.shape[0] > 1
-        else torch.empty([], dtype=torch.float32),
-        "background": torch.as_tensor(background, dtype=torch.float32).cpu(),
-        "step": torch.tensor([step], dtype=torch.int32),
+        "tesseract.signed_distances_raw": sdf.cpu(),
+        "tesseract.sh_rgb": sh[:1].clone(),
+        "tesseract.sh_rest": sh[1:].clone() if sh.shape[0] > 1 else torch.empty([], dtype=torch.float32),
+        "tessellator.step": torch.tensor([step], dtype=torch.int32),
     }
-    scene = {"grid_shape": [int(i), int(j), 32 * int(kp)],
+    scene = {"layout": _layout_dict(layout), "initializer": dict(initializer),
              "level_sets": [float(x) for x in torch.as_tensor(level_sets).cpu().tolist()],
              "sh_degree": int(sh_degree), "sharpness": float(sharpness)}
+    if contraction is not None:
+        scene["contraction"] = {"center_radius": contraction.center_radius,
+                                "arm_ratio": contraction.arm_ratio,
+                                "center_extent": contraction.center_extent,
+                                "disable_exponential": contraction.disable_exponential}
     torch.save({"model": model, "step": step, "scene": scene}, path)
 
 
-def read_scene_checkpoint(path, grid_shape=None, level_sets=None, sh_degree=None,
-                          sharpness=None) -> dict:
-    """The scene tensors of a checkpoint (loaded with ``weights_only=True``: nothing in the
-    file is executed), on the CPU. Arguments override or supply what the file lacks."""
+def read_scene_checkpoint(path, layout: TesseractLayout | None = None, initializer=None,
+                          level_sets=None, shells=None, sh_degree=None, sharpness=None,
+                          contraction: Contraction | None = None) -> dict:
+    """The scene of a checkpoint on the CPU (loaded with ``weights_only=True``: nothing in
+    the file is executed). Arguments override or supply what the file lacks; a reference
+    checkpoint needs the layout, the initializer, the sharpness and the level sets (or the
+    shells as CDF values, converted with the sharpness). The SDF stays the stored raw
+    parameters until the vertices are known (``load_scene_checkpoint``)."""
     content = torch.load(path, map_location="cpu", weights_only=True)
     model = content["model"]
     extra = content.get("scene", {})
-    grid_shape = tuple(grid_shape or extra.get("grid_shape") or ())
-    if len(grid_shape) != 3 or grid_shape[2] % 32:
-        raise ValueError("the grid shape (i, j, k with k % 32 == 0) is needed: reference "
-                         "checkpoints keep it in their config")
-    i, j, k = grid_shape
-    occupancy = model["tesseract.occupancy"].to(torch.int32)
-    if occupancy.numel() != i * j * (k // 32):
-        raise ValueError(f"the occupancy holds {occupancy.numel()} words, a {grid_shape} grid "
-                         f"{i * j * (k // 32)}: multi-component tesseracts are not supported")
-    sdf = model["tesseract.signed_distances_raw"].to(torch.float32)
+    if layout is None and "layout" in extra:
+        layout = TesseractLayout(**extra["layout"])
+    if layout is None:
+        raise ValueError("the tesseract layout (centre and arm resolutions, centre extent) is "
+                         "needed: reference checkpoints keep it in their config")
+    if initializer is None:
+        initializer = extra.get("initializer")
+    if initializer is None:
+        raise ValueError("the initializer is needed (the rendered SDF is the stored parameters "
+                         "plus its offset): reference checkpoints keep it in their config")
+    step_t = model.get("tesseract.step", model.get("step", content.get("step", 0)))
+    step = int(torch.as_tensor(step_t).reshape(-1)[0])
+    occupancy = model["tesseract.occupancy"].to(torch.int32).reshape(-1)
+    split_occupancy(occupancy, layout, step)  # checks the word count
+    raw = model["tesseract.signed_distances_raw"].to(torch.float32)
     sh = model["tesseract.sh_rgb"].to(torch.float32)
     rest = model.get("tesseract.sh_rest")
     if rest is not None and rest.dim() == 3 and rest.shape[0] > 0:
         sh = torch.cat([sh, rest.to(torch.float32)], dim=0)
     if sh_degree is None:
         sh_degree = extra.get("sh_degree", int(round(sh.shape[0] ** 0.5)) - 1)
-    if level_sets is None:
-        level_sets = extra.get("level_sets", [0.0])
     if sharpness is None:
         sharpness = extra.get("sharpness")
         if sharpness is None:
             raise ValueError("the sharpness is needed: reference checkpoints keep it in their config")
-    return {"occupancy": occupancy.view(i, j, k // 32), "signed_distances": sdf,
-            "spherical_harmonics": sh, "level_sets": torch.as_tensor(level_sets, dtype=torch.float32),
+    if level_sets is None and shells is not None:
+        level_sets = level_sets_from_shells(shells, float(sharpness))
+    if level_sets is None:
+        level_sets = extra.get("level_sets")
+        if level_sets is None:
+            raise ValueError("the level sets (or shells) are needed: reference checkpoints keep "
+                             "them in their config")
+    if contraction is None and "contraction" in extra:
+        contraction = Contraction(**extra["contraction"])
+    _initializer_offset(initializer, torch.zeros((1, 3)), step)  # refuses what cannot be applied
+    return {"occupancy": occupancy, "layout": layout, "initializer": dict(initializer),
+            "step": step, "signed_distances_raw": raw, "spherical_harmonics": sh,
+            "level_sets": torch.as_tensor(level_sets, dtype=torch.float32),
             "sh_degree": int(sh_degree), "sharpness": float(sharpness),
-            "background": model.get("background", torch.zeros(3)).to(torch.float32)}
+            "background": model.get("background", torch.zeros(3)).to(torch.float32),
+            "contraction": contraction}
 
 
-def load_scene_checkpoint(path, device, vertex_transform=None, colours: str = "vertex",
-                          **overrides) -> ResidentScene:
-    """A resident scene from a checkpoint: voxels extracted on ``device``, shells tessellated
-    at the level sets. ``vertex_transform`` maps the tessellated vertices (unit-cube grid
-    coordinates) to world space, as the reference's contraction does (tessellator.py:82-83)."""
-    from . import triangle_extraction as te
-
+def load_scene_checkpoint(path, device, colours: str = "vertex", **overrides) -> ResidentScene:
+    """A resident scene from a checkpoint: the tesseract's voxels extracted, mapped and
+    welded on ``device``, the SDF the reference renders (stored parameters plus the
+    initializer's offset at the checkpoint's step), shells tessellated at the level sets,
+    and, with a contraction, the triangle vertices mapped to world space as the reference's
+    tessellator does (tessellator.py:57)."""
     c = read_scene_checkpoint(path, **overrides)
-    vox = te.extract_voxels(c["occupancy"].to(device))
-    if vox.vertices.shape[0] != c["signed_distances"].shape[0]:
-        raise ValueError(f"{c['signed_distances'].shape[0]} SDF values for "
-                         f"{vox.vertices.shape[0]} grid vertices")
-    scene = tessellate_scene(vox, vox.vertices, c["signed_distances"].to(device),
-                             c["spherical_harmonics"].to(device), c["level_sets"].to(device),
-                             c["sh_degree"], c["sharpness"], c["background"].to(device),
-                             colours=colours)
-    if vertex_transform is not None:
-        if scene.grid is not None:
-            raise ValueError("a vertex transform needs colours='vertex' (grid colours are "
-                             "computed at the untransformed grid vertices)")
-        scene.vertices = vertex_transform(scene.vertices).contiguous()
+    vox, _ = extract_tesseract_voxels(c["occupancy"].to(device), c["layout"], c["step"])
+    if vox.vertices.shape[0] != c["signed_distances_raw"].shape[0]:
+        raise ValueError(f"{c['signed_distances_raw'].shape[0]} SDF values for "
+                         f"{vox.vertices.shape[0]} component vertices")
+    sdf = sdf_from_raw(c["signed_distances_raw"].to(device), vox.vertices, c["initializer"],
+                       c["step"]).contiguous()
+    if c["contraction"] is not None and colours == "grid":
+        raise ValueError("a contraction needs colours='vertex' (grid colours are computed at "
+                         "the uncontracted grid vertices)")
+    scene = tessellate_scene(vox, vox.vertices, sdf, c["spherical_harmonics"].to(device),
+                             c["level_sets"].to(device), c["sh_degree"], c["sharpness"],
+                             c["background"].to(device), colours=colours)
+    if c["contraction"] is not None:
+        scene.vertices = c["contraction"].uncontract(scene.vertices).contiguous()
     return scene
 
 

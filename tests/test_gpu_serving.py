@@ -8,6 +8,7 @@ from meshtryoshka_amd import serving
 from meshtryoshka_amd import triangle_extraction as te
 from meshtryoshka_amd import triangle_rasterization as tr
 from meshtryoshka_amd import workloads
+from meshtryoshka_amd.tesseract import Contraction, sdf_from_raw
 
 pytestmark = pytest.mark.gpu
 
@@ -94,8 +95,10 @@ def test_budget_limits_inflight_work(scenes):
 
 def test_scene_checkpoint_renders_the_same(tmp_path):
     """A scene written with save_scene_checkpoint and loaded back (voxels extracted and shells
-    tessellated on the GPU, unit-cube vertices mapped to world space by the same affine map)
-    renders the same image as the scene built in memory."""
+    tessellated on the GPU, unit-cube vertices mapped to world space by a contraction whose
+    inverse is the generator's 2 v - 1) renders the same image as the scene built in memory.
+    With the sphere initializer the file stores SDF - offset, and a reader gets the SDF back
+    to float32 rounding."""
     dev = torch.device("cuda")
     spec = workloads.sphere_field(seed=3, num_spheres=5, resolution=64, num_cameras=1,
                                   image_shape=(48, 64), device=dev)
@@ -109,8 +112,9 @@ def test_scene_checkpoint_renders_the_same(tmp_path):
     mem.vertices = to_world(mem.vertices).contiguous()
     path = tmp_path / "scene.ckpt"
     serving.save_scene_checkpoint(path, occ, sdf, sh, spec.level_sets, spec.sh_degree,
-                                  spec.sharpness, spec.background)
-    disk = serving.load_scene_checkpoint(path, dev, vertex_transform=to_world)
+                                  spec.sharpness, spec.background, initializer={"name": "none"},
+                                  contraction=Contraction(1.0, 0.0, 1.0))
+    disk = serving.load_scene_checkpoint(path, dev)
     assert torch.equal(disk.faces, mem.faces) and torch.equal(disk.vertices, mem.vertices)
     e, k, shape = _poses(1, 21, (48, 64))[0]
     imgs = [serving.render_batch(s, e[None].cuda(), k[None].cuda(), [shape]).images[0]
@@ -118,6 +122,15 @@ def test_scene_checkpoint_renders_the_same(tmp_path):
     torch.cuda.synchronize()
     assert torch.equal(imgs[0], imgs[1])
     assert float((imgs[0] < 0.999).float().mean()) > 0.01  # the scene is in view
+    sphere = tmp_path / "sphere.ckpt"
+    serving.save_scene_checkpoint(sphere, occ, sdf, sh, spec.level_sets, spec.sh_degree,
+                                  spec.sharpness, spec.background,
+                                  initializer={"name": "sphere", "radius": 0.25})
+    c = serving.read_scene_checkpoint(sphere)
+    stored = c["signed_distances_raw"].to(dev)
+    assert (stored - sdf.to(dev)).abs().max() > 0.1  # the offset was taken out
+    back = sdf_from_raw(stored, vox.vertices, c["initializer"], c["step"])
+    assert (back - sdf.to(dev)).abs().max() < 1e-6
 
 
 def test_grid_colours_render_the_same(scenes):

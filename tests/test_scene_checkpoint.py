@@ -1,20 +1,29 @@
-"""The scene checkpoint layout (the reference trainer's "model" entries plus a "scene" entry):
-a written file reads back with the same tensors through weights_only loading, the SH bands
-are split and joined as the reference's tesseract keeps them, and missing config is an
-error."""
+"""The scene checkpoint layout (the reference trainer's "model" entries plus a "scene"
+entry): a written file reads back with the same tensors through weights_only loading, the
+SH bands are split and joined as the reference's tesseract keeps them, a reference-layout
+file (no "scene" entry) needs its config passed in, and the SDF a reader renders is the
+stored parameters plus the named initializer's offset."""
 
+import numpy as np
 import pytest
 import torch
 
 from meshtryoshka_amd import serving
+from meshtryoshka_amd.tesseract import TesseractLayout, sdf_from_raw
+from oracle import tesseract as ot
+from oracle.extraction import extract_voxels, pack_occupancy
+
+NONE = {"name": "none"}
 
 
-def _scene(nsh):
+def _scene(nsh, n=32):
     gen = torch.Generator().manual_seed(0)
-    occ = torch.randint(-2**31, 2**31 - 1, (4, 5, 2), generator=gen, dtype=torch.int64).to(torch.int32)
-    sdf = torch.randn(100, generator=gen)
-    sh = torch.randn(nsh, 100, 3, generator=gen)
-    return occ, sdf, sh
+    occ = torch.rand((n, n, n), generator=gen) < 0.1
+    packed = torch.from_numpy(pack_occupancy(occ.numpy()))
+    nv = extract_voxels(packed.numpy()).vertices.shape[0]
+    sdf = torch.randn(nv, generator=gen)
+    sh = torch.randn(nsh, nv, 3, generator=gen)
+    return packed, sdf, sh
 
 
 @pytest.mark.parametrize("nsh,deg", [(16, 3), (1, 0)])
@@ -22,29 +31,70 @@ def test_round_trip(tmp_path, nsh, deg):
     occ, sdf, sh = _scene(nsh)
     path = tmp_path / "scene.ckpt"
     serving.save_scene_checkpoint(path, occ, sdf, sh, torch.tensor([0.1, 0.0, -0.1]), deg, 250.0,
-                                  torch.tensor([1.0, 0.5, 0.25]), step=7)
+                                  torch.tensor([1.0, 0.5, 0.25]), step=7, initializer=NONE)
     raw = torch.load(path, weights_only=True)
-    assert raw["step"] == 7 and raw["model"]["tesseract.sh_rgb"].shape == (1, 100, 3)
-    assert raw["model"]["tesseract.occupancy"].shape == (40,)
+    assert raw["step"] == 7 and raw["model"]["tesseract.sh_rgb"].shape == (1, sdf.shape[0], 3)
+    assert raw["model"]["tesseract.occupancy"].shape == (32 * 32,)
+    assert int(raw["model"]["tesseract.step"]) == 7
     c = serving.read_scene_checkpoint(path)
-    assert torch.equal(c["occupancy"], occ) and torch.equal(c["signed_distances"], sdf)
+    assert torch.equal(c["occupancy"], occ.reshape(-1)) and torch.equal(c["signed_distances_raw"], sdf)
     assert torch.equal(c["spherical_harmonics"], sh)
-    assert c["sh_degree"] == deg and c["sharpness"] == 250.0
+    assert c["sh_degree"] == deg and c["sharpness"] == 250.0 and c["step"] == 7
+    assert c["layout"] == TesseractLayout(32, 0, 1.0) and c["initializer"] == NONE
     assert torch.allclose(c["level_sets"], torch.tensor([0.1, 0.0, -0.1]))
     assert torch.equal(c["background"], torch.tensor([1.0, 0.5, 0.25]))
 
 
-def test_reference_layout_needs_config(tmp_path):
-    occ, sdf, sh = _scene(4)
-    model = {"tesseract.occupancy": occ.reshape(-1), "tesseract.signed_distances_raw": sdf,
-             "tesseract.sh_rgb": sh[:1], "tesseract.sh_rest": sh[1:]}
+def _reference_file(tmp_path, occ, raw, sh, step=500):
+    model = {"step": torch.tensor([step], dtype=torch.int32),
+             "background": torch.ones(3),
+             "tesseract.step": torch.tensor(step, dtype=torch.int32),
+             "tesseract.occupancy": occ.reshape(-1), "tesseract.signed_distances_raw": raw,
+             "tesseract.sh_rgb": sh[:1], "tesseract.sh_rest": sh[1:],
+             "tessellator.step": torch.tensor([step], dtype=torch.int32)}
     path = tmp_path / "ref.ckpt"
-    torch.save({"model": model, "step": 3}, path)  # no "scene" entry, as the reference writes
-    with pytest.raises(ValueError):
+    torch.save({"model": model, "optimizer": {}, "step": step, "benchmarks": {}}, path)
+    return path  # no "scene" entry, as the reference trainer writes
+
+
+def test_reference_layout_needs_config(tmp_path):
+    occ, raw, sh = _scene(4)
+    path = _reference_file(tmp_path, occ, raw, sh)
+    layout = TesseractLayout({0: 32}, {0: 0}, 1.0)
+    sphere = {"name": "sphere", "radius": 0.25}
+    with pytest.raises(ValueError, match="layout"):
         serving.read_scene_checkpoint(path)
-    with pytest.raises(ValueError):
-        serving.read_scene_checkpoint(path, grid_shape=(4, 5, 64))  # no sharpness
-    c = serving.read_scene_checkpoint(path, grid_shape=(4, 5, 64), sharpness=100.0)
-    assert c["sh_degree"] == 1 and torch.equal(c["spherical_harmonics"], sh)
-    with pytest.raises(ValueError):  # two components' worth of words
-        serving.read_scene_checkpoint(path, grid_shape=(4, 5, 32), sharpness=100.0)
+    with pytest.raises(ValueError, match="initializer"):
+        serving.read_scene_checkpoint(path, layout=layout)
+    with pytest.raises(ValueError, match="sharpness"):
+        serving.read_scene_checkpoint(path, layout=layout, initializer=sphere)
+    with pytest.raises(ValueError, match="level sets"):
+        serving.read_scene_checkpoint(path, layout=layout, initializer=sphere, sharpness=400.0)
+    with pytest.raises(ValueError, match="words"):  # seven components' worth expected
+        serving.read_scene_checkpoint(path, layout=TesseractLayout(32, 32, 0.5), initializer=sphere,
+                                      sharpness=400.0, shells=[0.5])
+    with pytest.raises(ValueError, match="point cloud"):  # step 500 is inside its blend
+        serving.read_scene_checkpoint(path, layout=layout, sharpness=400.0, shells=[0.5],
+                                      initializer={"name": "point_cloud", "interpolate_steps": 600,
+                                                   "num_frozen_steps": 100})
+    c = serving.read_scene_checkpoint(path, layout=layout, initializer=sphere, sharpness=400.0,
+                                      shells=[0.9, 0.5, 0.1])
+    assert c["sh_degree"] == 1 and torch.equal(c["spherical_harmonics"], sh) and c["step"] == 500
+    assert torch.allclose(c["level_sets"].double(),
+                          torch.from_numpy(ot.level_sets_from_shells([0.9, 0.5, 0.1], 400.0)))
+
+
+def test_reference_sphere_offset_is_applied(tmp_path):
+    """A reference-layout file with the default sphere initializer: the SDF a reader renders
+    is the stored raw parameters plus |xyz - 0.5| - radius at the grid's vertices."""
+    occ, raw, sh = _scene(1)
+    path = _reference_file(tmp_path, occ, raw, sh)
+    c = serving.read_scene_checkpoint(path, layout=TesseractLayout(32, 0, 1.0),
+                                      initializer={"name": "sphere", "radius": 0.25},
+                                      sharpness=400.0, shells=[0.5])
+    vox, _ = ot.extract_tesseract(c["occupancy"].numpy(), 32, 0, 1.0)
+    got = sdf_from_raw(c["signed_distances_raw"], torch.from_numpy(vox.vertices),
+                       c["initializer"], c["step"]).double().numpy()
+    want = raw.double().numpy() + ot.sphere_offset(vox.vertices, 0.25)
+    assert np.allclose(got, want, atol=1e-6)
+    assert np.abs(got - raw.double().numpy()).max() > 0.1  # the offset is not negligible
