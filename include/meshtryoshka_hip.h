@@ -195,10 +195,12 @@ size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(int64_t num_g
  * from grad_sdf f32[V] and
  * grad_colors f32[B, V, 3], using the forward's vertices f32[V, 3] and colors (relu is
  * active where the colour is > 0). Each column visits its crossings once; the origin's
- * and the axis ends' shares go through workspace rows to the grid vertices, each of which
- * is written once. This needs each grid vertex to be the origin / +axis end of at most one
- * column, as extract_voxels guarantees; otherwise nothing is written and bit 1 (value 2)
- * of *status is set. num_sh in {1, 4, 9, 16}. vertex_touched (uint8[V] or NULL): the flags
+ * and the axis ends' shares go through workspace rows to the grid vertices. When each grid
+ * vertex is the origin / +axis end of at most one column (as extract_voxels guarantees),
+ * each grid vertex gathers its rows and is written once; when columns share grid vertices
+ * (a multi-component grid whose coincident vertices were welded) the rows are added with
+ * atomics instead, decided on the device. status is not written (may be NULL).
+ * num_sh in {1, 4, 9, 16}. vertex_touched (uint8[V] or NULL): the flags
  * mt_intersect_and_composite_backward set; vertices without the flag have zero gradients
  * and are skipped. clear_gradients = 1 zeroes the grad_sdf and grad_colors entries it
  * reads (every nonzero entry when the flags are given), so that a caller that keeps them

@@ -243,6 +243,9 @@ class BatchedTrainingStep:
         th, tw = self.tile_shape
         target = target.contiguous().view(-1)
         table = not self.uniform
+        # Each run reports only its own capacity overflows (a fill kernel, so it is part of a
+        # captured replay too).
+        self.overflow.zero_()
         self.loss.zero_()
         self.sharp_grad.zero_()
         # --- tessellation counts (once for all camera groups)

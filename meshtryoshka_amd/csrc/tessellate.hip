@@ -543,7 +543,6 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_mask_kernel(
     const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
     int S, const int32_t* offsets, int64_t nt, const uint8_t* touched, const int32_t* maps,
     uint8_t* col_live, uint32_t* col_slots, int32_t* list, int32_t* col_pos, int32_t* count) {
-  if (maps[4 * ng] != 0) return;
   const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   uint32_t bits = 0, slots = 0;
   if (c < n) {
@@ -624,10 +623,9 @@ __global__ void zero_counts_kernel(int32_t* a, int32_t* b) {
   *b = 0;
 }
 
-// Phase 1, one thread per column: the column's crossings are visited once (axis by axis,
-// level sets in order); (1 - t) g goes to the column's origin, which no other column has
-// (written straight into the grid gradient), and t g to the axis end, summed per axis into
-// part[a][j][c] (plane-major: j = 0 SDF, 1 + 3k + ch SH) for phase 2. Each camera's term
+// Phase 1, one thread per live column: the column's crossings are visited once (axis by
+// axis, level sets in order); (1 - t) g is summed into the column's origin row and t g into
+// the row of the axis end (rows: [SDF, SH coefficient-major then channel]) for phase 2. Each camera's term
 // basis_k * gc is added to both sums directly (no per-vertex gradient array: 48 fewer
 // registers, 3 waves per SIMD instead of 2).
 template <int kNsh, int kDeg>
@@ -638,7 +636,6 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
     const int32_t* count, const uint8_t* col_live, const uint32_t* col_slots, const int32_t* maps,
     const uint8_t* touched, bool clear) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
-  if (maps[4 * ng] != 0) return;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= *count) return;
   const int64_t c = list[i];  // a column with some contributing vertex
@@ -753,10 +750,8 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kPad = adjoint_row_floats(kN);
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (maps[4 * ng] != 0) {  // the columns do not map one-to-one onto grid vertices
-    if (i == 0 && status) atomicOr(status, 2);
-    return;
-  }
+  (void)status;
+  if (maps[4 * ng] != 0) return;  // not one-to-one: colors_adjoint_scatter_kernel adds the rows
   // With a list (accumulating calls): only the listed grid vertices, which have live ends.
   if (list != nullptr && i >= *count) return;
   const int64_t g = list != nullptr ? int64_t(list[i]) : i;
@@ -785,6 +780,51 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
       const float x = k < kN ? v[k < kN ? k : 0][ch] : 0.f;
       *o = accumulate ? *o + x : x;
     }
+}
+
+// Phase 2 when some grid vertex is the origin or the +axis end of more than one column (a
+// multi-component grid whose coincident vertices were welded): one thread per live column
+// adds its rows to their grid vertices with atomics. A non-accumulating call clears the
+// grid gradients first (clear_grid_gradients_kernel, also decided on the device).
+__global__ void __launch_bounds__(kBlock) clear_grid_gradients_kernel(const int32_t* maps,
+                                                                      int64_t ng, int num_sh,
+                                                                      float* gg_d, float* gg_sh) {
+  if (maps[4 * ng] == 0) return;
+  const int64_t total = ng * (1 + 3 * int64_t(num_sh));
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < total; j += stride) {
+    if (j < ng) gg_d[j] = 0.f;
+    else gg_sh[j - ng] = 0.f;
+  }
+}
+
+template <int kNsh, int kDeg>
+__global__ void __launch_bounds__(kBlock) colors_adjoint_scatter_kernel(
+    int64_t ng, const int32_t* lower, int64_t n, const float* part, const int32_t* list,
+    const int32_t* count, const uint8_t* col_live, float* gg_d, float* gg_sh,
+    const int32_t* maps) {
+  constexpr int kN = (kDeg + 1) * (kDeg + 1);
+  constexpr int kPad = adjoint_row_floats(kN);
+  if (maps[4 * ng] == 0) return;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= *count) return;
+  const int64_t c = list[i];
+  const uint32_t axes = col_live[c];
+  for (int r = 0; r < 4; ++r) {  // rows 0..2: the +x / +y / +z ends; row 3: the origin
+    if (r < 3 && !((axes >> r) & 1u)) continue;  // unwritten row
+    const int32_t g = r < 3 ? lower[(r + 1) * n + c] : lower[c];
+    if (g < 0 || g >= ng) continue;
+    float sd = 0.f;
+    float v[kN][3];
+#pragma unroll
+    for (int k = 0; k < kN; ++k) v[k][0] = v[k][1] = v[k][2] = 0.f;
+    add_row<kN>(part + (i * 4 + r) * kPad, sd, v);
+    atomicAdd(gg_d + g, sd);
+#pragma unroll
+    for (int k = 0; k < kN; ++k)
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) atomicAdd(gg_sh + (int64_t(k) * ng + g) * 3 + ch, v[k][ch]);
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) count_faces_kernel(const float* sdf, const int32_t* lower,
@@ -1203,6 +1243,13 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
     mt::colors_adjoint_ends_kernel<N, D><<<ggrid, mt::kBlock, 0, st>>>(                          \
         num_grid_vertices, num_columns, part, col_live, col_pos, grid_sdf_grad, grid_sh_grad,     \
         maps, accumulate != 0, status, accumulate != 0 ? ends_list : nullptr, ends_count);        \
+    if (accumulate == 0)                                                                         \
+      mt::clear_grid_gradients_kernel<<<ggrid.x < 4096u ? ggrid.x : 4096u, mt::kBlock, 0, st>>>(  \
+          maps, num_grid_vertices, num_sh, grid_sdf_grad, grid_sh_grad);                          \
+    if (num_columns > 0)                                                                         \
+      mt::colors_adjoint_scatter_kernel<N, D><<<cgrid, mt::kBlock, 0, st>>>(                     \
+          num_grid_vertices, lower_corners, num_columns, part, list, live_count, col_live,        \
+          grid_sdf_grad, grid_sh_grad, maps);                                                     \
   } while (0)
   const int key = num_sh * 4 + sh_degree;
   switch (key) {

@@ -202,31 +202,73 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
     assert np.abs(got_d - gd_ref).max() <= 1e-3 * np.abs(gd_ref).max()
 
 
-def test_fused_adjoint_reports_shared_origins(case):
-    """Two columns sharing an origin cannot be gathered: status bit 2, outputs untouched."""
+def test_fused_adjoint_with_shared_origins(case):
+    """Columns that share grid vertices (as after welding a multi-component grid) cannot be
+    gathered: the rows are added with atomics instead. Overwriting, accumulating and flagged
+    calls all equal the separate SH and vertex adjoints on the same columns."""
     spec, vox, world, sdf, sh, ls, cams = case
     dev = sdf.device
     lower = vox.lower_corners.clone()
-    lower[0, 1] = lower[0, 0]
-    n, s, ng, nsh = lower.shape[1], ls.shape[0], world.shape[0], sh.shape[0]
-    off = _offsets(vox, sdf, ls)
+    lower[0, 1:400:2] = lower[0, 0:399:2]  # 200 columns take their neighbour's origin
+    lower[1, 5] = lower[1, 3]  # and one +x end is shared
+    n, s, ng, nsh, deg = lower.shape[1], ls.shape[0], world.shape[0], sh.shape[0], spec.sh_degree
+    off = _offsets(vox._replace(lower_corners=lower), sdf, ls)
     V = int(off[-1])
     b = cams.extrinsics.shape[0]
     f32 = dict(dtype=torch.float32, device=dev)
-    campos = torch.zeros((b, 3), **f32)
-    ws = _native.workspace(_native.query("mt_create_triangle_vertices_colors_backward_workspace_bytes",
-                                         ng, n, spec.sh_degree), dev)
-    gg_d = torch.full((ng,), 7.0, **f32)
-    gg_sh = torch.full((nsh, ng, 3), 7.0, **f32)
+    campos = torch.empty((b, 3), **f32)
+    _native.call("mt_camera_positions", cams.extrinsics.contiguous().data_ptr(), b, campos.data_ptr(), _stream())
+    tv, td = torch.empty((V, 3), **f32), torch.empty(V, **f32)
+    tsh, tt = torch.empty((nsh, V, 3), **f32), torch.empty(V, dtype=torch.uint8, device=dev)
+    ov = torch.zeros(1, dtype=torch.int32, device=dev)
+    _native.call("mt_create_triangle_vertices_forward", world.data_ptr(), sdf.data_ptr(), sh.data_ptr(),
+                 ng, nsh, lower.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V,
+                 tv.data_ptr(), td.data_ptr(), tsh.data_ptr(), tt.data_ptr(), ov.data_ptr(), _stream())
+    col = torch.empty((b, V, 3), **f32)
+    _native.call("mt_evaluate_sh_forward", tv.data_ptr(), V, tsh.data_ptr(), nsh, deg,
+                 campos.data_ptr(), b, col.data_ptr(), _stream())
+    gen = torch.Generator("cpu").manual_seed(12)
+    g_d = torch.randn(V, generator=gen).to(dev)
+    g_col = torch.randn((b, V, 3), generator=gen).to(dev)
+    # The separate adjoints (the vertex adjoint scatters with atomics on shared origins).
+    g_tsh = torch.zeros((nsh, V, 3), **f32)
+    _native.call("mt_evaluate_sh_backward", tv.data_ptr(), V, tsh.data_ptr(), nsh, deg,
+                 campos.data_ptr(), b, g_col.data_ptr(), g_tsh.data_ptr(), _stream())
+    ws = _native.workspace(_native.query("mt_create_triangle_vertices_backward_workspace_bytes", ng), dev)
+    want_d, want_sh = torch.zeros(ng, **f32), torch.zeros((nsh, ng, 3), **f32)
+    _native.call("mt_create_triangle_vertices_backward", sdf.data_ptr(), ng, nsh, lower.data_ptr(), n,
+                 ls.data_ptr(), s, off.data_ptr(), V, None, g_d.data_ptr(), g_tsh.data_ptr(), None,
+                 want_d.data_ptr(), want_sh.data_ptr(), ws.data_ptr(), ws.numel(), _stream())
+    ws2 = _native.workspace(_native.query("mt_create_triangle_vertices_colors_backward_workspace_bytes",
+                                          ng, n, deg), dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
-    zeros_v = torch.zeros(V, **f32)
-    zeros_c = torch.zeros((b, V, 3), **f32)
-    tv = torch.zeros((V, 3), **f32)
-    _native.call("mt_create_triangle_vertices_colors_backward", sdf.data_ptr(), ng, nsh,
-                 spec.sh_degree, lower.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V,
-                 tv.data_ptr(), zeros_c.data_ptr(), campos.data_ptr(), b, zeros_v.data_ptr(),
-                 zeros_c.data_ptr(), gg_d.data_ptr(), gg_sh.data_ptr(), ws.data_ptr(), ws.numel(), 0,
-                 None, 0, 0, status.data_ptr(), _stream())
+
+    def fused(gd_out, gs_out, c0, c1, gd_in, accumulate, touched=None, maps_ready=0):
+        _native.call("mt_create_triangle_vertices_colors_backward", sdf.data_ptr(), ng, nsh, deg,
+                     lower.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V, tv.data_ptr(),
+                     col[c0:].data_ptr(), campos[c0:].data_ptr(), c1 - c0, gd_in.data_ptr(),
+                     g_col[c0:].contiguous().data_ptr(), gd_out.data_ptr(), gs_out.data_ptr(),
+                     ws2.data_ptr(), ws2.numel(), accumulate,
+                     None if touched is None else touched.data_ptr(), 0, maps_ready,
+                     status.data_ptr(), _stream())
+
+    def close(got_d, got_sh):
+        torch.testing.assert_close(got_d, want_d, rtol=1e-5, atol=1e-6 * float(want_d.abs().max()))
+        torch.testing.assert_close(got_sh, want_sh, rtol=1e-5, atol=1e-6 * float(want_sh.abs().max()))
+
+    # Overwriting call: NaN-filled outputs are cleared, then the rows added.
+    gd1 = torch.full((ng,), float("nan"), **f32)
+    gs1 = torch.full((nsh, ng, 3), float("nan"), **f32)
+    fused(gd1, gs1, 0, b, g_d, 0)
+    close(gd1, gs1)
+    # Two accumulating calls over the cameras, reusing the maps.
+    gd2, gs2 = torch.zeros(ng, **f32), torch.zeros((nsh, ng, 3), **f32)
+    fused(gd2, gs2, 0, 1, g_d, 1, maps_ready=1)
+    fused(gd2, gs2, 1, b, torch.zeros_like(g_d), 1, maps_ready=1)
+    close(gd2, gs2)
+    # All vertices flagged: the same sums through the flagged path.
+    gd3, gs3 = torch.zeros(ng, **f32), torch.zeros((nsh, ng, 3), **f32)
+    fused(gd3, gs3, 0, b, g_d, 1, touched=torch.ones(V, dtype=torch.uint8, device=dev), maps_ready=1)
+    close(gd3, gs3)
     torch.cuda.synchronize()
-    assert int(status) == 2
-    assert bool((gg_d == 7.0).all()) and bool((gg_sh == 7.0).all())
+    assert int(status) == 0

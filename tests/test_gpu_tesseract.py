@@ -131,3 +131,49 @@ def test_reference_missing_corner_handling_renders_the_same():
     assert a.faces.shape[0] == b.faces.shape[0]
     assert torch.equal(va[a.faces.long()], vb[b.faces.long()])  # the same triangles
     assert torch.equal(ia, ib)
+
+
+def test_batched_step_on_the_welded_tesseract():
+    """Columns of different components share welded grid vertices, so the colour adjoint's
+    grid-vertex maps are not one-to-one and its rows are added with atomics: the batched
+    step's image, loss and grid gradients still match the plain path's (in the unit cube;
+    cameras look at its centre)."""
+    from meshtryoshka_amd import batched
+
+    c = serving.read_scene_checkpoint(FIXTURE, **CONFIG)
+    vox, _ = ts.extract_tesseract_voxels(c["occupancy"].cuda(), LAYOUT, c["step"])
+    sdf = ts.sdf_from_raw(c["signed_distances_raw"].cuda(), vox.vertices, SPHERE, c["step"])
+    sh = c["spherical_harmonics"].cuda().contiguous()
+    ls = c["level_sets"].cuda()
+    origins = vox.lower_corners[0]
+    assert torch.unique(origins).numel() < origins.numel()  # shared column origins
+    centre = (0.5, 0.5, 0.5)
+    E = torch.stack([workloads.look_at_w2c(torch.tensor(p, dtype=torch.float64), target=centre)
+                     for p in ([1.7, 1.4, 1.9], [-0.6, 1.5, 0.2], [0.4, -0.9, 1.6])]).float().cuda()
+    K = workloads.pinhole((H, W), 50.0)[None].repeat(3, 1, 1).cuda()
+    gen = torch.Generator("cpu").manual_seed(4)
+    target = torch.rand((3, 3, H, W), generator=gen).cuda()
+    sharp, bg = 400.0, (1.0, 1.0, 1.0)
+    # The plain path, with autograd.
+    d = sdf.clone().requires_grad_(True)
+    s = sh.clone().requires_grad_(True)
+    tri = te.tessellate_voxels(d, vox.vertices, s, vox.neighbors, vox.lower_corners,
+                               vox.upper_corners, vox.indices, ls)
+    img = tr.render(E, K, (H, W), tri.vertices.detach(), tri.faces, tri.face_boundaries,
+                    tri.signed_distances, tri.spherical_harmonics, 0,
+                    torch.tensor(sharp, device="cuda"), torch.tensor(bg, device="cuda"))
+    loss = ((img - target) ** 2).mean()
+    loss.backward()
+    assert float((img < 0.99).float().mean()) > 0.05
+    cap = batched.measure_capacities(vox, ls, E, K, (H, W), vox.vertices, sdf, sh).scaled(1.2)
+    step = batched.BatchedTrainingStep(vox, ls, E, K, (H, W), 1, 0, sharp, bg, cap)
+    step.capture(vox.vertices, sdf, sh, target)
+    for i in range(2):
+        step.replay()
+        torch.cuda.synchronize()
+        assert not step.overflowed(), i
+        rgb = (step.out[:, :3] + step.out[:, 3:4] * step.background[:, None, None]).clip(0, 1)
+        assert torch.equal(rgb, img.detach()), i
+        assert abs(float(step.loss) - float(loss)) <= 1e-6 * float(loss), i
+        for got, want in ((step.grid_sdf_grad, d.grad), (step.grid_sh_grad, s.grad)):
+            assert float((got - want).abs().max() / want.abs().max()) < 1e-5, i
