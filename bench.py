@@ -63,7 +63,8 @@ def parse_args(argv=None):
     p.add_argument("--workload", choices=sorted(WORKLOADS), default="sphere-field-4cam",
                    help="preset scene sizes (explicit options given on the command line win)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-sample-tiles", type=int, default=24)
+    p.add_argument("--cpu-pair-budget", type=int, default=1_500_000_000,
+                   help="(pixel, key) pairs the CPU baseline composites on its tile sample")
     p.add_argument("--cpu-slab", type=float, default=1 / 64,
                    help="fraction of the grid's z layers the CPU baseline runs on")
     p.add_argument("--breakdown", action="store_true", help="print the per-kernel table to stderr")
@@ -545,102 +546,181 @@ def subdivision_benchmark(device, resolution: int = 512, spheres: int = 64):
 # ------------------------------------------------------------------------------------
 
 
-def cpu_baseline(sc, args, live):
-    """The oracle (NumPy, single thread) on a bounded spatial sample of the same step.
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: every core the process may use (OMP_NUM_THREADS
+    caps it where set, as on a shared host)."""
+    n = os.cpu_count() or 1
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(cap))) if cap and cap.isdigit() else n
 
-    The sample is a slab of the occupancy grid (a fraction of its z layers, centred) with
-    fields drawn by the scene's own generator recipe. The oracle runs the full step on the
-    slab: tessellation, projection .. delineation, SH, fused composite forward + backward
-    (on a seeded set of non-empty tiles, scaled by key count), SH adjoint and vertex
-    adjoint. Each stage is then scaled to the full workload by its own unit: voxels for
-    tessellation and the vertex adjoint, faces for the front end, SH and the key count of the
-    composite (the oracle follows the reference: every key of every face's tile box)."""
-    import numpy as np
+
+def c0_forward(device, log, runs: int = 5):
+    """The dense 64^3 forward (S = 3, one 128 x 128 camera, 16 x 16 tiles, SH degree 0):
+    extraction, tessellation, keys, sort, tile runs, SH and the fused composite, timed in
+    full with the PyTorch-CPU transcription on the host threads and with the plain GPU path
+    (1 warm-up, median of ``runs``); the two images are compared."""
+    import statistics
+
     import torch
 
     from meshtryoshka_amd import triangle_extraction as te
-    from oracle import extraction as ox
-    from oracle import rasterization as orr
+    from meshtryoshka_amd import triangle_rasterization as tr
+    from meshtryoshka_amd import workloads
+    from oracle import cpu_torch as ct
 
-    packed = sc.spec.occupancy
-    res = packed.shape[0]
-    layers = max(2, int(round(res * args.cpu_slab)))
-    i0 = (res - layers) // 2
-    crop = torch.zeros_like(packed)
-    crop[i0 : i0 + layers] = packed[i0 : i0 + layers]
-    v = te.extract_voxels(crop.to(sc.world.device))
-    gen = torch.Generator("cpu").manual_seed(args.seed + 1)
-    world_t, sdf_t, sh_t = sc.spec.field_fn(v.vertices.cpu(), gen)
-    world, sdf, sh = world_t.numpy(), sdf_t.numpy(), sh_t.numpy()
-    lower, upper = v.lower_corners.cpu().numpy(), v.upper_corners.cpu().numpy()
-    nbr, idx = v.neighbors.cpu().numpy(), v.indices.cpu().numpy()
-    ls = sc.level_sets.cpu().numpy()
-    E = sc.cams.extrinsics.cpu().numpy()
-    K = sc.cams.intrinsics.cpu().numpy()
-    h, w = sc.cams.image_shape
-    b = E.shape[0]
-    th, tw = sc.spec.tile_shape
-    rows, cols = -(-h // th), -(-w // tw)
-    deg = sc.spec.sh_degree
+    spec = workloads.dense_sphere(seed=0, resolution=64, image_size=128)
+    packed = spec.occupancy.cpu()
+    vox0 = ct.extract_voxels(packed)
+    world, sdf, sh = workloads.finish_fields(spec, vox0.vertices)
+    world, sdf, sh = world.float(), sdf.float(), sh.float()
+    ls = spec.level_sets.float()
+    E, K = spec.cameras.extrinsics.float(), spec.cameras.intrinsics.float()
+    shape = spec.cameras.image_shape
 
-    t0 = time.perf_counter()
-    tri = ox.tessellate_voxels(sdf, world, sh, nbr, lower, upper, idx, ls)
-    t_tess = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    xy, depth = orr.project_vertices(tri.vertices, E, K)
-    tmin, tmax, cnt = orr.bound_triangles(xy, depth, tri.faces, (th, tw), (rows, cols), sc.spec.near_plane)
-    lay = orr.fused_layout()
-    keys, vals, _ = orr.generate_keys(depth, tri.faces, tri.face_boundaries, tmin, tmax, cnt, cols, lay)
-    sk, sv = orr.sort_keys(keys, vals)
-    ranges = orr.delineate_tiles(sk, lay, b, ls.shape[0], rows * cols, False)
-    col, _ = orr.evaluate_spherical_harmonics(tri.vertices, tri.spherical_harmonics, E, deg)
-    col = col.astype(np.float32)
-    t_front = time.perf_counter() - t0
-    rng = np.random.default_rng(0)
-    nonempty = np.argwhere(ranges[:, 0, :, 1] > ranges[:, 0, :, 0])
-    pick = nonempty[rng.choice(len(nonempty), size=min(args.cpu_sample_tiles, len(nonempty)), replace=False)]
-    sub = np.zeros_like(ranges[:, 0])
-    sub[pick[:, 0], pick[:, 1]] = ranges[pick[:, 0], 0, pick[:, 1]]
-    keys_sampled = int((sub[..., 1] - sub[..., 0]).sum())
-    t0 = time.perf_counter()
-    hits = orr.fused_hits(xy, tri.faces, sk, sv, sub, lay, ls.shape[0], (th, tw), (h, w))
-    img = orr.fused_forward_image(hits, col, tri.signed_distances, tri.faces, sc.spec.sharpness, b, (h, w))
-    g = np.ones_like(img)
-    orr.fused_backward(hits, col.astype(np.float64), tri.signed_distances.astype(np.float64), tri.faces,
-                       sc.spec.sharpness, img, g)
-    t_comp_sample = time.perf_counter() - t0
-    nv = tri.vertices.shape[0]
-    t0 = time.perf_counter()
-    gcol = rng.standard_normal((b, nv, 3)).astype(np.float32)
-    gsh = orr.evaluate_spherical_harmonics_backward(tri.vertices, tri.spherical_harmonics, E, deg, gcol)
-    t_shb = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ox.create_triangle_vertices_backward(tri.record, world.shape[0], sh.shape[0], None,
-                                         rng.standard_normal(nv).astype(np.float32), gsh)
-    t_vb = time.perf_counter() - t0
+    def cpu_run():
+        vox = ct.extract_voxels(packed)
+        tess = ct.tessellate(sdf, world, sh, vox, ls)
+        xy, depth = ct.project(tess.vertices, E, K)
+        sk, sv, ranges = ct.front_end(xy, depth, tess.faces, tess.face_boundaries, spec.tile_shape,
+                                      shape, spec.near_plane)
+        col, _ = ct.sh_forward(tess.vertices, tess.sh, E, spec.sh_degree)
+        res = ct.fused_composite(xy, tess.faces, tess.sdf, col, sk, sv, ranges, spec.tile_shape,
+                                 shape, spec.sharpness)
+        bg = torch.tensor(spec.background)[None, :, None, None]
+        return (res.image[:, :3] + res.image[:, 3:4] * bg).clamp(0, 1), tess, sk
 
-    vox_ratio = sc.voxels.num_voxels / max(v.num_voxels, 1)
-    face_ratio = live["F"] / max(tri.faces.shape[0], 1)
-    # The oracle composites every key of the reference's traversal (no culling): the full
-    # step's key count is the sample's, scaled by faces.
-    k_full = int(len(sk) * face_ratio)
-    t_comp = t_comp_sample * k_full / max(keys_sampled, 1)
-    step_s = (t_tess + t_vb) * vox_ratio + (t_front + t_shb) * face_ratio + t_comp
-    sample_s = t_tess + t_front + t_comp_sample + t_shb + t_vb
+    dev = [t.to(device) for t in (world, sdf, sh, ls, E, K)]
+    occ_d = spec.occupancy.to(device)
+    sharp = torch.tensor(spec.sharpness, device=device)
+    bg_d = torch.tensor(spec.background, device=device)
+
+    def gpu_run():
+        with torch.no_grad():
+            vox = te.extract_voxels(occ_d)
+            tri = te.tessellate_voxels(dev[1], dev[0], dev[2], vox.neighbors, vox.lower_corners,
+                                       vox.upper_corners, vox.indices, dev[3])
+            return tr.render(dev[4], dev[5], shape, tri.vertices, tri.faces, tri.face_boundaries,
+                             tri.signed_distances, tri.spherical_harmonics, spec.sh_degree, sharp,
+                             bg_d, tile_shape=spec.tile_shape, near_plane=spec.near_plane)
+
+    def median_s(fn, sync=None):
+        fn()
+        times = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            out = fn()
+            if sync:
+                sync()
+            times.append(time.perf_counter() - t0)
+        return statistics.median(times), out
+
+    cpu_s, (cpu_img, tess, sk) = median_s(cpu_run)
+    log(f"[bench] C0 forward on the CPU ({torch.get_num_threads()} threads): {1000 * cpu_s:.1f} ms")
+    gpu_s, gpu_img = median_s(gpu_run, torch.cuda.synchronize)
+    log(f"[bench] C0 forward on the GPU (plain path): {1000 * gpu_s:.2f} ms")
+    return {"cpu_ms": round(1000 * cpu_s, 2), "gpu_ms": round(1000 * gpu_s, 3),
+            "speedup": round(cpu_s / gpu_s, 1),
+            "image_max_abs_cpu_vs_gpu": float((cpu_img - gpu_img.cpu()).abs().max()),
+            "faces": int(tess.faces.shape[0]), "triangle_vertices": int(tess.vertices.shape[0]),
+            "keys": int(sk.numel()),
+            "workload": "dense 64^3 grid, S=3, one 128x128 camera, 16x16 tiles, SH degree 0"}
+
+
+def cpu_baseline(sc, args, live, log):
+    """The PyTorch-CPU transcription of the path (oracle/cpu_torch.py: FP32, vectorised,
+    on the host's intra-op thread pool) timed on the host cores.
+
+    * C0 forward in full (c0_forward), beside the plain GPU path.
+    * The measured workload's training step on a bounded sample, each stage scaled to the
+      full step by its own unit: a slab of the grid's z layers (extraction, tessellation
+      and the vertex adjoint, scaled by voxels); projection..tile runs, SH and the SH
+      adjoint on the slab's faces for every camera (scaled by faces); the fused composite
+      forward + backward of the L2 loss on a random sample of the slab's non-empty tiles
+      (scaled by the full step's unculled key count: the transcription, like the
+      reference, composites every key of every face's tile box)."""
+    import numpy as np
+    import torch
+
+    from oracle import cpu_torch as ct
+
+    threads = cpu_threads()
+    prev = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    try:
+        c0 = c0_forward(sc.world.device, log)
+        packed = sc.spec.occupancy.cpu()
+        res = packed.shape[0]
+        layers = max(2, int(round(res * args.cpu_slab)))
+        i0 = (res - layers) // 2
+        t0 = time.perf_counter()
+        vox = ct.extract_voxels(packed[i0 : i0 + layers].contiguous(), i0, res)
+        t_ext = time.perf_counter() - t0
+        gen = torch.Generator("cpu").manual_seed(args.seed + 1)
+        world, sdf, sh = (x.float().contiguous() for x in sc.spec.field_fn(vox.vertices, gen))
+        ls = sc.level_sets.cpu()
+        t0 = time.perf_counter()
+        tess = ct.tessellate(sdf, world, sh, vox, ls)
+        t_tess = time.perf_counter() - t0
+        log(f"[bench] CPU sample: slab of {layers}/{res} layers, {vox.upper.shape[1]} voxels, "
+            f"{tess.faces.shape[0]} faces; extraction {t_ext:.1f} s, tessellation {t_tess:.1f} s")
+        E, K = sc.cams.extrinsics.cpu(), sc.cams.intrinsics.cpu()
+        h, w = sc.cams.image_shape
+        deg = sc.spec.sh_degree
+        t0 = time.perf_counter()
+        xy, depth = ct.project(tess.vertices, E, K)
+        sk, sv, ranges = ct.front_end(xy, depth, tess.faces, tess.face_boundaries, sc.spec.tile_shape,
+                                      (h, w), sc.spec.near_plane)
+        col, pre = ct.sh_forward(tess.vertices, tess.sh, E, deg)
+        t_front = time.perf_counter() - t0
+        log(f"[bench] CPU sample: projection..tile runs and SH {t_front:.1f} s, {sk.numel()} keys")
+        cnt = ranges[..., 1] - ranges[..., 0]
+        nonempty = torch.nonzero(cnt > 0)
+        rng = np.random.default_rng(0)
+        pick = nonempty[torch.from_numpy(rng.permutation(nonempty.shape[0]))]
+        # Tiles up to a budget of (pixel, key) pairs.
+        th, tw = sc.spec.tile_shape
+        budget = args.cpu_pair_budget
+        take = int(torch.searchsorted(torch.cumsum(cnt[pick[:, 0], pick[:, 1]] * th * tw, 0),
+                                      budget)) + 1
+        pick = pick[: max(1, min(take, pick.shape[0]))]
+        keys_sampled = int(cnt[pick[:, 0], pick[:, 1]].sum())
+        t0 = time.perf_counter()
+        comp = ct.fused_composite(xy, tess.faces, tess.sdf, col, sk, sv, ranges, sc.spec.tile_shape,
+                                  (h, w), sc.spec.sharpness, sc.background.cpu(), sc.target.cpu(),
+                                  tiles=pick)
+        t_comp = time.perf_counter() - t0
+        log(f"[bench] CPU sample: composite fwd+bwd on {pick.shape[0]} tiles ({keys_sampled} keys, "
+            f"{comp.pairs} pixel-key pairs) {t_comp:.1f} s")
+        t0 = time.perf_counter()
+        g_sh = ct.sh_backward(tess.vertices, tess.sh, E, deg, pre, comp.colors_grad)
+        t_shb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ct.vertex_backward(tess, world.shape[0], comp.sdf_grad, g_sh)
+        t_vb = time.perf_counter() - t0
+    finally:
+        torch.set_num_threads(prev)
+    vox_ratio = sc.voxels.num_voxels / max(vox.upper.shape[1], 1)
+    face_ratio = live["F"] / max(tess.faces.shape[0], 1)
+    k_full = int(sk.numel() * face_ratio)
+    t_comp_full = t_comp * k_full / max(keys_sampled, 1)
+    step_s = (t_ext + t_tess + t_vb) * vox_ratio + (t_front + t_shb) * face_ratio + t_comp_full
+    sample_s = t_ext + t_tess + t_front + t_comp + t_shb + t_vb
     return {
         "value": round(1.0 / step_s, 6),
         "unit": "steps/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": (f"NumPy oracle (1 thread) on a slab of {layers}/{res} z layers of the same scene "
-                   f"({v.num_voxels} voxels, {tri.faces.shape[0]} faces, {len(sk)} keys; "
-                   f"{sample_s:.1f} s of CPU work): tessellation {t_tess:.1f} s and vertex adjoint "
-                   f"{t_vb:.1f} s scaled x{vox_ratio:.1f} by voxels; projection..delineation + SH "
-                   f"{t_front:.1f} s and SH adjoint {t_shb:.1f} s scaled x{face_ratio:.1f} by faces; "
-                   f"fused composite fwd+bwd on {len(pick)} random non-empty tiles "
-                   f"({keys_sampled} keys, {t_comp_sample:.1f} s) scaled to the step's {k_full} "
-                   f"unculled keys"),
+        "sample": (f"PyTorch-CPU transcription (FP32, vectorised, {threads} threads) on a slab of "
+                   f"{layers}/{res} z layers of the same scene ({vox.upper.shape[1]} voxels, "
+                   f"{tess.faces.shape[0]} faces, {sk.numel()} unculled keys for {E.shape[0]} "
+                   f"cameras; {sample_s:.1f} s of CPU work): extraction + tessellation "
+                   f"{t_ext + t_tess:.1f} s and vertex adjoint {t_vb:.1f} s scaled x{vox_ratio:.1f} "
+                   f"by voxels; projection..tile runs + SH {t_front:.1f} s and SH adjoint "
+                   f"{t_shb:.1f} s scaled x{face_ratio:.1f} by faces; fused composite fwd+bwd on "
+                   f"{pick.shape[0]} random non-empty tiles ({keys_sampled} keys, {t_comp:.1f} s) "
+                   f"scaled to the step's {k_full} unculled keys"),
         "seconds_per_step": round(step_s, 1),
+        "c0_forward": c0,
     }
 
 
@@ -972,11 +1052,9 @@ def main():
 
     cpu = None
     if not (args.no_cpu_baseline or args.only_batched or not sc.uniform):
-        log("[bench] CPU baseline (oracle on a bounded sample)")
-        from threadpoolctl import threadpool_limits
-
-        with threadpool_limits(limits=1):  # the baseline is a single-thread port
-            cpu = cpu_baseline(sc, args, live)
+        log(f"[bench] CPU baseline (PyTorch-CPU transcription, {cpu_threads()} threads)")
+        cpu = cpu_baseline(sc, args, live, log)
+        log(f"[bench] CPU baseline: {cpu['seconds_per_step']} s/step")
 
     workload = workload_description(args, sc)
     sub = None
