@@ -407,7 +407,10 @@ int mt_generate_keys_from_geometry(const float* xy, const float* depths, int64_t
  * stable sort cannot tell apart because keys of different cameras differ in the camera
  * field (B > 1 therefore needs 2^camera_bits >= B). Tile grids: at most 65535 rows and
  * columns (row_min, col_min >= 0 and row_max, col_max <= 65535; the same for the cam_dims
- * grids, which the caller checks). *n_keys (device) = the
+ * grids, which the caller checks). A block handles 1024 (face, camera) pairs and counts
+ * its keys in int32, so 1024 x (row_max - row_min) x (col_max - col_min) must stay below
+ * 2^31; with cam_dims, row_max and col_max (otherwise unused) bound every camera's grid
+ * for this check. *n_keys (device) = the
  * total key count (also when it exceeds key_capacity, which sets *overflow). Workspace:
  * mt_generate_keys_single_pass_workspace_bytes(F, B). */
 size_t mt_generate_keys_single_pass_workspace_bytes(int64_t F, int32_t B);

@@ -118,6 +118,8 @@ class BatchedTrainingStep:
         self.pixel_first = first
         self.num_pixels = first[-1]
         self.rows, self.cols = dims[0][2], dims[0][3]
+        # Bounds of every camera's tile grid (the key kernel sizes its count check by them).
+        self.max_rows, self.max_cols = max(d[2] for d in dims), max(d[3] for d in dims)
         self.T = self.rows * self.cols
         # Camera groups, each with its device-side table (camera indices local to the group).
         self.groups = []
@@ -316,7 +318,7 @@ class BatchedTrainingStep:
             n_keys = self.n_keys
             call("mt_generate_keys_single_pass", self.xy.data_ptr(), self.depth.data_ptr(), V,
                  self.faces.data_ptr(), F, self.num_faces_x3.data_ptr(), face_boundaries.data_ptr(),
-                 S, n, th, tw, 0, 0, self.rows, self.cols, self.near_plane, cam_dims, self.cull,
+                 S, n, th, tw, 0, 0, self.max_rows, self.max_cols, self.near_plane, cam_dims, self.cull,
                  self.cols, self.layout, self.keys.data_ptr(), self.values.data_ptr(), K,
                  n_keys.data_ptr(), self.overflow.data_ptr(), self.keys_ws.data_ptr(),
                  self.keys_ws.numel(), st)

@@ -471,11 +471,12 @@ __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
 // cameras of a slice are interleaved block by block. Keys of different cameras differ in their camera
 // field, so a stable sort orders both arrangements identically. The last block also stores
 // the total key count.
+constexpr int kPassThreads = 256;  // one wave per SIMD per block
 constexpr int kPassPer = 4;
-constexpr int kPassFaces = kPassPer * kBlock;
+constexpr int kPassFaces = kPassPer * kPassThreads;
 
 template <bool kPow2>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kPow2 ? 6 : 5)))
+__global__ void __launch_bounds__(kPassThreads) __attribute__((amdgpu_waves_per_eu(kPow2 ? 6 : 5)))
 keys_single_pass_kernel(
     const float* xy, const float* depths, int64_t V, const int32_t* faces, int64_t F,
     const int64_t* live_x3, const int32_t* boundaries, int S, int B, int lfpb, int nslices,
@@ -489,7 +490,7 @@ keys_single_pass_kernel(
   __shared__ uint32_t s_org[kPassFaces], s_ext[kPassFaces];
   __shared__ uint64_t s_key_hi[kPassFaces];
   __shared__ int16_t s_work[kPassFaces];
-  __shared__ int32_t s_wave[kBlock / kWave];
+  __shared__ int32_t s_wave[kPassThreads / kWave];
   __shared__ int32_t s_nwork;
   __shared__ int64_t s_q, s_block_start;
   if (threadIdx.x == 0) {
@@ -525,7 +526,7 @@ keys_single_pass_kernel(
     int32_t ca[kHoist], cb[kHoist], cc[kHoist];
 #pragma unroll
     for (int h = 0; h < kHoist; ++h) {
-      const int l = (i0 + h) * kBlock + threadIdx.x;
+      const int l = (i0 + h) * kPassThreads + threadIdx.x;
       const int b = bfirst + (l >> lfpb);
       const int64_t f = f0 + (l & (fpb - 1));
       ca[h] = cb[h] = cc[h] = -1;
@@ -540,7 +541,7 @@ keys_single_pass_kernel(
     bool valid[kHoist];
 #pragma unroll
     for (int h = 0; h < kHoist; ++h) {
-      const int l = (i0 + h) * kBlock + threadIdx.x;
+      const int l = (i0 + h) * kPassThreads + threadIdx.x;
       const int64_t o = int64_t(bfirst + (l >> lfpb)) * V;
       valid[h] = ca[h] >= 0 && cb[h] >= 0 && cc[h] >= 0 && ca[h] < V && cb[h] < V && cc[h] < V;
       if (valid[h]) {
@@ -554,7 +555,7 @@ keys_single_pass_kernel(
     }
 #pragma unroll
     for (int h = 0; h < kHoist; ++h) {
-      const int l = (i0 + h) * kBlock + threadIdx.x;
+      const int l = (i0 + h) * kPassThreads + threadIdx.x;
       const int b = bfirst + (l >> lfpb);
       const int64_t f = f0 + (l & (fpb - 1));
       int32_t count = 0;
@@ -591,7 +592,7 @@ keys_single_pass_kernel(
   __syncthreads();
   // 2. Pixel-level narrowing of the listed pairs.
   const int nwork = s_nwork;
-  for (int k = threadIdx.x; k < nwork; k += kBlock) {
+  for (int k = threadIdx.x; k < nwork; k += kPassThreads) {
     const int l = s_work[k];
     const int b = bfirst + (l >> lfpb);
     const int64_t f = f0 + (l & (fpb - 1));
@@ -611,14 +612,14 @@ keys_single_pass_kernel(
   // 3. Scan in pair order, look-back, keys.
   int32_t carry = 0;
   for (int i = 0; i < kPassPer; ++i) {
-    const int l = i * kBlock + threadIdx.x;
+    const int l = i * kPassThreads + threadIdx.x;
     const int32_t count = s_prefix[l];
     const int32_t inc = wave_inclusive_sum(count);
     if (lane == kWave - 1) s_wave[wave] = inc;
     __syncthreads();
     int32_t before = 0, total = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / kWave; ++w) {
+    for (int w = 0; w < kPassThreads / kWave; ++w) {
       if (w < wave) before += s_wave[w];
       total += s_wave[w];
     }
@@ -644,7 +645,7 @@ keys_single_pass_kernel(
   __syncthreads();
   const int64_t base = s_block_start;
   const uint64_t tile_mask = field_mask(kf.tile_bits);
-  for (int32_t j = threadIdx.x; j < total; j += kBlock) {
+  for (int32_t j = threadIdx.x; j < total; j += kPassThreads) {
     int lo = 0, hi = kPassFaces - 1;  // last pair whose prefix is <= j (it has keys)
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -668,12 +669,12 @@ keys_single_pass_kernel(
 }
 
 // log2 of the faces per block of the single-pass kernel: the largest power of two with
-// (faces x B) <= kPassFaces, at least kBlock faces (then the cameras come in slices of
-// kPassFaces / kBlock).
+// (faces x B) <= kPassFaces, at least kPassThreads faces (then the cameras come in slices of
+// kPassFaces / kPassThreads).
 __host__ inline int single_pass_lfpb(int B) {
   int l = 0;
   while ((int64_t(2) << l) * (B > 0 ? B : 1) <= kPassFaces) ++l;
-  constexpr int kMin = __builtin_ctz(kBlock);
+  constexpr int kMin = __builtin_ctz(kPassThreads);
   return l > kMin ? l : kMin;
 }
 
@@ -918,14 +919,13 @@ extern "C" int mt_generate_keys_single_pass(
   MT_REQUIRE(F >= 0 && B >= 0 && B <= 65535 && key_capacity >= 0 && tile_h > 0 && tile_w > 0,
              "mt_generate_keys_single_pass: bad sizes");
   MT_REQUIRE(n_keys != nullptr, "mt_generate_keys_single_pass: null key count");
-  MT_REQUIRE(cam_dims != nullptr ||
-                 (row_min >= 0 && col_min >= 0 && row_max <= 65535 && col_max <= 65535),
+  MT_REQUIRE(row_min >= 0 && col_min >= 0 && row_max <= 65535 && col_max <= 65535 &&
+                 row_min <= row_max && col_min <= col_max,
              "mt_generate_keys_single_pass: the tile grid must lie in [0, 65535]^2");
   // A block's key count (kPassFaces pairs, each at most the whole tile grid) is an int32;
-  // with per-camera grids (cam_dims) the caller checks the largest grid the same way.
-  MT_REQUIRE(cam_dims != nullptr ||
-                 int64_t(mt::kPassFaces) * (int64_t(row_max) - row_min) *
-                         (int64_t(col_max) - col_min) < (int64_t(1) << 31),
+  // with per-camera grids (cam_dims) row_max and col_max bound every camera's grid.
+  MT_REQUIRE(int64_t(mt::kPassFaces) * (int64_t(row_max) - row_min) *
+                     (int64_t(col_max) - col_min) < (int64_t(1) << 31),
              "mt_generate_keys_single_pass: %d x %d tiles per camera overflow a block's key count",
              row_max - row_min, col_max - col_min);
   const size_t need = mt::single_pass_state_bytes(F, B);
@@ -952,7 +952,7 @@ extern "C" int mt_generate_keys_single_pass(
   MT_REQUIRE(blocks < (int64_t(1) << 31), "mt_generate_keys_single_pass: too many face blocks");
   auto kernel = mt::pow2_tiles(tile_h, tile_w) ? mt::keys_single_pass_kernel<true>
                                                 : mt::keys_single_pass_kernel<false>;
-  kernel<<<static_cast<unsigned>(blocks), mt::kBlock, 0, st>>>(
+  kernel<<<static_cast<unsigned>(blocks), mt::kPassThreads, 0, st>>>(
       xy, depths, V, faces, F, num_faces_x3, shell_face_boundaries, S, B, lfpb,
       mt::single_pass_slices(B, lfpb), tile_h, tile_w, row_min, col_min, row_max, col_max, near_plane, cam_dims, cull, grid_cols,
       mt::to_fields(layout), keys, values, key_capacity, n_keys, overflow, state);
