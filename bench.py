@@ -97,6 +97,9 @@ def parse_args(argv=None):
     p.add_argument("--serve-max-batch", type=int, default=16)
     p.add_argument("--serve-deadline", type=float, default=1.0,
                    help="requests not admitted within this many seconds of arrival are shed")
+    p.add_argument("--calibrate-counters", action="store_true",
+                   help="run only three elementwise kernels that each read exactly 1 GiB and "
+                        "write exactly 1 GiB (the scale for rocprofv3's FETCH_SIZE / WRITE_SIZE)")
     p.add_argument("--probe", action="store_true",
                    help="build the scene, print its sizes and memory, and exit")
     args = p.parse_args(argv)
@@ -392,6 +395,37 @@ def profile_traffic(entry_point, key):
             if cells and cells[0] == entry_point and len(cells) >= 6:
                 try:
                     return int(float(cells[5]) * 2**30)  # "HBM GiB per launch" column
+                except ValueError:
+                    return None
+    return None
+
+
+# Device kernel behind each entry point that can be the step's longest, for the counter lookup.
+DOMINANT_KERNELS = {
+    "mt_generate_keys_single_pass": "mt::keys_single_pass_kernel<true>",
+    "mt_create_triangle_vertices_colors_forward": "mt::create_vertices_colors_kernel<3, true>",
+    "mt_intersect_and_composite_backward": "mt::fused_backward_kernel",
+    "mt_intersect_and_composite_forward": "mt::fused_forward_kernel",
+}
+# 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 VALU instruction.
+VALU_ISSUE_PEAK_G = 256 * 4 * 2.4 / 4
+
+
+def profile_counter(kernel, counter, key):
+    """A counter of ``kernel`` (largest dispatch) from the instruction-counter table of the
+    committed profile of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"batched_step_{key}.md")
+    if not kernel or not os.path.exists(path):
+        return None
+    header = None
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if cells and cells[0] == "kernel" and counter in cells:
+                header = cells
+            elif header and cells and cells[0] == kernel and len(cells) == len(header):
+                try:
+                    return float(cells[header.index(counter)])
                 except ValueError:
                     return None
     return None
@@ -902,6 +936,15 @@ def main():
     device = torch.device("cuda:0")
     torch.cuda.set_device(device)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.calibrate_counters:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=device)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.mul(src, 2.0, out=dst)
+            torch.cuda.synchronize()
+        log("[bench] calibration: 3 x (1 GiB read + 1 GiB written)")
+        return
     if args.only_serving:
         print(json.dumps({"serving": serving_benchmark(args, device, log)}), flush=True)
         return
@@ -1040,6 +1083,14 @@ def main():
             "traffic": profile_traffic(dom, workload_key(args, best)),
             "ms_per_launch": round(ms_launch, 4), "launches_per_step": launches,
             "algorithmic_bytes_per_launch": int(bytes_launch)}
+    instr = profile_counter(DOMINANT_KERNELS.get(dom, ""), "SQ_INSTS_VALU", workload_key(args, best))
+    if instr:
+        # VALU instruction issue: a wave64 VALU instruction takes 4 cycles of a 16-lane SIMD.
+        issue = instr / (ms_launch * 1e-3) / 1e9
+        roof["valu"] = {"achieved": round(issue, 1), "peak": VALU_ISSUE_PEAK_G, "unit": "G wave-instr/s",
+                        "frac": round(issue / VALU_ISSUE_PEAK_G, 4), "instructions_per_launch": int(instr),
+                        "source": "SQ_INSTS_VALU of the committed counter profile of this workload"}
+        roof["limiter"] = "VALU instruction issue and per-block latency (HBM fraction shown for the contract)"
     if dom.startswith("mt_intersect_and_composite"):
         fl = (58 * tests + (25 if dom.endswith("forward") else 60) * hits) / max(launches, 1)
         roof["valu"] = {"achieved": round(fl / (ms_launch * 1e-3) / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
