@@ -275,4 +275,81 @@ MT_HD Rect candidate_rect(const TriTerms& t, int tile_x0, int tile_y0, int tw, i
   return r;
 }
 
+// Pixel centres that a small, well-shaped triangle can hit, from its bounding box alone.
+//
+// Claim: if the certificate below holds, every pixel centre P farther than m / 2 outside the
+// triangle's bounding box (along x or y) fails the reference's barycentric test.
+// Why: let lambda be P's exact barycentric coordinates for the float corners p1, p2, p3,
+// W the box's larger side and d > 0 P's distance outside the box along one axis, say
+// P.x < xlo - d. Then sum_i lambda_i (p_i.x - xlo) = P.x - xlo < -d with every
+// p_i.x - xlo in [0, W], so some lambda_i <= -d / (2 W) (at most two are negative). The
+// float numerator of that coordinate is cr^2 lambda_i (cr the exact cross product of the
+// edges: each of the three Gram determinants equals cr^2) plus at most 24 u L^3 |P - p_k|
+// (u = 2^-24, L the longest edge, p_k the numerator's base corner; half of it from the
+// rounding of the edges and Gram entries, half from the five operations that evaluate the
+// numerator), and |P - p_k| <= 2 (d + W). So it is negative with room to spare, and its
+// quotient by D > 0 a negative number that does not round to -0, once
+// cr^2 > 192 u L^3 W (1 + W / d). The computed determinant then also has D > 0. With
+// L^3 <= sqrt(2) L^2 W and d >= m / 2 this is implied by the certificate
+//     cr^2 (1 - 2^-6) >= 2^-15 L^2 W^2 (1 + 2 W / m),
+// 2^-15 exceeding 192 sqrt(2) u by a factor 1.9 for the second-order terms; the float
+// evaluation of both sides is off by far less than the 2^-6 and 2^-10 factors. Corners
+// lie within (-2^12, 2^12), so the box ends widened by m in float are off by at most
+// 2^-10 < m / 2, and cr >= 2^-20 keeps every product normal.
+//
+// Result (tile-box-local pixel indices, like Rect): kCentresNone when no pixel centre of
+// the box [tile_x0, tile_x0 + tw) x [tile_y0, tile_y0 + th) is within the widened bounding
+// box; kCentresFew with their rectangle when there are at most max_pixels; otherwise, or
+// when the certificate fails, kCentresUnknown (the caller falls back to candidate_rect).
+constexpr float kCentreMargin = 0x1p-5f;
+enum CentreState { kCentresNone = 0, kCentresFew = 1, kCentresUnknown = 2 };
+struct CentreRange {
+  int x, y, w, h;
+  int state;
+};
+
+MT_HD CentreRange centre_range(float ax, float ay, float bx, float by, float cx, float cy,
+                               int tile_x0, int tile_y0, int tw, int th, int max_pixels) {
+  CentreRange out{0, 0, 0, 0, kCentresUnknown};
+  constexpr float kLim = 0x1p12f;
+  // NaN fails every comparison.
+  const bool inside = __builtin_fabsf(ax) < kLim && __builtin_fabsf(ay) < kLim &&
+                      __builtin_fabsf(bx) < kLim && __builtin_fabsf(by) < kLim &&
+                      __builtin_fabsf(cx) < kLim && __builtin_fabsf(cy) < kLim;
+  if (!inside) return out;
+  const float e12x = bx - ax, e12y = by - ay, e13x = cx - ax, e13y = cy - ay;
+  const float e23x = cx - bx, e23y = cy - by;
+  const float cr = __builtin_fabsf(e12x * e13y - e12y * e13x);
+  const float l12 = e12x * e12x + e12y * e12y, l13 = e13x * e13x + e13y * e13y;
+  const float l23 = e23x * e23x + e23y * e23y;
+  const float lsq = __builtin_fmaxf(__builtin_fmaxf(l12, l13), l23) * (1.0f + 0x1p-10f);
+  const float xlo = __builtin_fminf(__builtin_fminf(ax, bx), cx);
+  const float xhi = __builtin_fmaxf(__builtin_fmaxf(ax, bx), cx);
+  const float ylo = __builtin_fminf(__builtin_fminf(ay, by), cy);
+  const float yhi = __builtin_fmaxf(__builtin_fmaxf(ay, by), cy);
+  const float wb = __builtin_fmaxf(xhi - xlo, yhi - ylo) * (1.0f + 0x1p-10f);
+  const float rhs = 0x1p-15f * lsq * wb * wb * (1.0f + (2.0f / kCentreMargin) * wb);
+  if (!(cr >= 0x1p-20f) || !(cr * cr * (1.0f - 0x1p-6f) >= rhs * (1.0f + 0x1p-10f))) return out;
+  // Pixel X has its centre at X + 0.5.
+  int x0 = int(__builtin_ceilf(xlo - kCentreMargin - 0.5f));
+  int x1 = int(__builtin_floorf(xhi + kCentreMargin - 0.5f));
+  int y0 = int(__builtin_ceilf(ylo - kCentreMargin - 0.5f));
+  int y1 = int(__builtin_floorf(yhi + kCentreMargin - 0.5f));
+  x0 = x0 > tile_x0 ? x0 : tile_x0;
+  y0 = y0 > tile_y0 ? y0 : tile_y0;
+  x1 = x1 < tile_x0 + tw - 1 ? x1 : tile_x0 + tw - 1;
+  y1 = y1 < tile_y0 + th - 1 ? y1 : tile_y0 + th - 1;
+  if (x1 < x0 || y1 < y0) {
+    out.state = kCentresNone;
+    return out;
+  }
+  if ((x1 - x0 + 1) * (y1 - y0 + 1) > max_pixels) return out;
+  out.x = x0 - tile_x0;
+  out.y = y0 - tile_y0;
+  out.w = x1 - x0 + 1;
+  out.h = y1 - y0 + 1;
+  out.state = kCentresFew;
+  return out;
+}
+
 }  // namespace mt

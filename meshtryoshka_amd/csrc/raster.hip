@@ -132,12 +132,13 @@ __device__ __forceinline__ TileRect face_tile_rect(const float* xy, const float*
 }
 
 // Narrows a face's tile rectangle to the tiles where some pixel can pass the barycentric
-// test. The candidate pixel rectangle over the whole tile-aligned box (geometry.h, the
-// bound the composites use per tile) excludes the other pixels; when it holds at most
-// kExactPixels pixels, each of them is put through the test itself (the reference's float
+// test. For a small, well-shaped face the pixel centres near its bounding box are the only
+// candidates (geometry.h, centre_range; most sub-pixel faces have none and get no key);
+// otherwise the candidate pixel rectangle over the whole tile-aligned box (geometry.h, the
+// bound the composites use per tile) excludes the other pixels. When at most kExactPixels
+// candidates remain, each of them is put through the test itself (the reference's float
 // operations, as the composites evaluate them), and only tiles with an accepted pixel
-// keep a key. Keys dropped this way cannot change any pixel or gradient; most sub-pixel
-// faces cover no pixel centre at all and get no key.
+// keep a key. Keys dropped this way cannot change any pixel or gradient.
 constexpr int kExactPixels = 16;
 
 // The pixel tests of cull_tiles (below) for a candidate rectangle q of at most kExactPixels
@@ -183,7 +184,17 @@ __device__ __forceinline__ TileRect exact_tiles(TileRect r, Rect q, const float*
   return n;
 }
 
-// Face f (whose three corners the caller has checked), camera offset o into xy.
+// The pixel centres of face (a, b, c) in its tile-aligned box r that can hit
+// (geometry.h, centre_range): none, a few, or unknown.
+__device__ __forceinline__ CentreRange box_centres(float2 a, float2 bb, float2 c, TileRect r,
+                                                   int th, int tw) {
+  return centre_range(a.x, a.y, bb.x, bb.y, c.x, c.y, r.x0 * tw, r.y0 * th, (r.x1 - r.x0) * tw,
+                      (r.y1 - r.y0) * th, kExactPixels);
+}
+
+// Face f (whose three corners the caller has checked), camera offset o into xy. Small,
+// well-shaped faces take their candidate pixels from the bounding box (box_centres); the
+// others from the candidate rectangle.
 template <bool kPow2>
 __device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, const int32_t* faces,
                                                int64_t f, int64_t o, const TileSize<kPow2>& ts) {
@@ -192,19 +203,23 @@ __device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, cons
   const float2* pxy = reinterpret_cast<const float2*>(xy);
   const float2 a = pxy[o + faces[3 * f + 0]], bb = pxy[o + faces[3 * f + 1]];
   const float2 c = pxy[o + faces[3 * f + 2]];
-  const TriTerms t = tri_terms(a.x, a.y, bb.x, bb.y, c.x, c.y);
-  const int px0 = r.x0 * tw, py0 = r.y0 * th;
-  const Rect q = candidate_rect(t, px0, py0, (r.x1 - r.x0) * tw, (r.y1 - r.y0) * th);
   TileRect n;
   n.y0 = n.y1 = r.y0;
   n.x0 = n.x1 = r.x0;
-  if (q.w <= 0 || q.h <= 0) return n;
-  if (q.w * q.h > kExactPixels) {
-    n.y0 = r.y0 + ts.row(q.y);
-    n.y1 = r.y0 + ts.row(q.y + q.h - 1) + 1;
-    n.x0 = r.x0 + ts.col(q.x);
-    n.x1 = r.x0 + ts.col(q.x + q.w - 1) + 1;
-    return n;
+  const CentreRange cen = box_centres(a, bb, c, r, th, tw);
+  if (cen.state == kCentresNone) return n;
+  Rect q{cen.x, cen.y, cen.w, cen.h};
+  if (cen.state != kCentresFew) {
+    const TriTerms t = tri_terms(a.x, a.y, bb.x, bb.y, c.x, c.y);
+    q = candidate_rect(t, r.x0 * tw, r.y0 * th, (r.x1 - r.x0) * tw, (r.y1 - r.y0) * th);
+    if (q.w <= 0 || q.h <= 0) return n;
+    if (q.w * q.h > kExactPixels) {
+      n.y0 = r.y0 + ts.row(q.y);
+      n.y1 = r.y0 + ts.row(q.y + q.h - 1) + 1;
+      n.x0 = r.x0 + ts.col(q.x);
+      n.x1 = r.x0 + ts.col(q.x + q.w - 1) + 1;
+      return n;
+    }
   }
   return exact_tiles(r, q, xy, faces, f, o, ts);
 }
@@ -460,8 +475,9 @@ __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
 // Single pass: the counts, their scan and the keys in one kernel. A block takes its camera
 // slice (kPassFaces / 2^lfpb cameras) and face block (2^lfpb faces; pairs are camera-major
 // inside the block) from a ticket, slice-major, and
-//   1. bounds each pair (culling, tile-aligned box) and lists the pairs whose box needs
-//      the pixel-level narrowing (cull_tiles) in LDS,
+//   1. bounds each pair (culling, tile-aligned box), drops the pairs whose bounding box
+//      reaches no pixel centre (box_centres), and lists the others for the pixel-level
+//      narrowing (cull_tiles) in LDS, grouped by the route they will take,
 //   2. narrows the listed pairs with every thread busy (back-facing, clipped and keyless
 //      pairs take no thread),
 //   3. scans the key counts in pair order, finds the block's first output position by
@@ -491,19 +507,21 @@ keys_single_pass_kernel(
   __shared__ uint64_t s_key_hi[kPassFaces];
   __shared__ int16_t s_work[kPassFaces];
   __shared__ int32_t s_wave[kPassThreads / kWave];
-  __shared__ int32_t s_nwork;
+  __shared__ int32_t s_nwork[2];  // pairs listed for each route of step 2
   __shared__ int64_t s_q, s_block_start;
   if (threadIdx.x == 0) {
     s_q = atomicAdd(st.ticket, 1u);
-    s_nwork = 0;
+    s_nwork[0] = s_nwork[1] = 0;
   }
   __syncthreads();
-  const int64_t q = s_q;
+  // The host keeps the block count below 2^31, so block indices are 32-bit.
+  const uint32_t q = uint32_t(s_q);
   const int fpb = 1 << lfpb;
-  const int64_t nfb = (F + fpb - 1) >> lfpb;  // face blocks
-  const int64_t nblk = nfb * nslices;
-  const int64_t f0 = (q % nfb) << lfpb;  // first face of the block
-  const int bfirst = int(q / nfb) * (kPassFaces >> lfpb);  // first camera of the slice
+  const uint32_t nfb = uint32_t((F + fpb - 1) >> lfpb);  // face blocks
+  const uint32_t nblk = nfb * uint32_t(nslices);
+  const uint32_t slice = q / nfb;
+  const int64_t f0 = int64_t(q - slice * nfb) << lfpb;  // first face of the block
+  const int bfirst = int(slice) * (kPassFaces >> lfpb);  // first camera of the slice
   const TileSize<kPow2> ts(th, tw);
   const int lane = __lane_id();
   const int wave = threadIdx.x / kWave;
@@ -517,83 +535,120 @@ keys_single_pass_kernel(
     const float mean_depth = ((depths[o + ia] + depths[o + ib]) + depths[o + ic]) / 3.f;
     return pack_key(kf, uint32_t(b), shell, 0u, mean_depth);
   };
-  // 1. Bounds, kHoist pairs at a time: their corner indices, then their corners, are all
-  // loaded before any is used, so that the loads overlap.
+  // 1. Bounds. A pair slot i (pairs i * kPassThreads + threadIdx.x) holds one camera for the
+  // whole block (lfpb >= log2(kPassThreads)), so its camera's arrays are uniform; corner
+  // indices are checked without branches and used as 32-bit offsets.
   const float2* pxy = reinterpret_cast<const float2*>(xy);
+  const int64_t nlive64 = fend - f0;
+  const int nlive = nlive64 <= 0 ? 0 : (nlive64 >= fpb ? fpb : int(nlive64));  // live faces here
+  const int32_t* fblock = faces + 3 * f0;
+  const uint32_t vlim = V < 0x80000000ll ? uint32_t(V) : 0x80000000u;
+  // Corner indices of slot i's face (-1 when the face is not live), loaded before use.
+  auto face_corners = [&](int i, int32_t& ia, int32_t& ib, int32_t& ic) {
+    const int local = (i * kPassThreads + int(threadIdx.x)) & (fpb - 1);
+    ia = ib = ic = -1;
+    if (local < nlive) {
+      ia = fblock[3 * local + 0];
+      ib = fblock[3 * local + 1];
+      ic = fblock[3 * local + 2];
+    }
+  };
+  // Bounds slot i's pair (face corners ia, ib, ic) and lists it for step 2.
+  auto bound_slot = [&](int i, int32_t ia, int32_t ib, int32_t ic, float2 pa, float2 pb, float2 pc,
+                        float za, float zb, float zc, bool valid) {
+    const int l = i * kPassThreads + threadIdx.x;
+    const int b = bfirst + (l >> lfpb);
+    int32_t count = 0;
+    int route = -1;  // step 2: 0 = the pixel centres near the box, 1 = candidate rectangle
+    if (valid) {
+      int r0 = rmin, c0 = cmin, r1 = rmax, c1 = cmax;
+      if (cam_dims) {  // this camera's own tile grid
+        r0 = 0;
+        c0 = 0;
+        r1 = cam_dims[4 * b + 2];
+        c1 = cam_dims[4 * b + 3];
+      }
+      const TileRect r = corner_tile_rect(pa, pb, pc, za, zb, zc, ts, r0, c0, r1, c1, near_plane);
+      s_org[l] = (uint32_t(r.y0) << 16) | uint32_t(r.x0);
+      s_ext[l] = (uint32_t(r.y1 - r.y0) << 16) | uint32_t(r.x1 - r.x0);
+      count = (r.y1 - r.y0) * (r.x1 - r.x0);
+      if (count > 0 && cull) {
+        // Faces whose bounding box reaches no pixel centre have no keys; the others are
+        // narrowed in step 2.
+        const int state = box_centres(pa, pb, pc, r, ts.th, ts.tw).state;
+        route = state == kCentresFew ? 0 : (state == kCentresUnknown ? 1 : -1);
+        count = 0;  // set by step 2
+      } else if (count > 0) {
+        s_key_hi[l] = key_hi(b, f0 + (l & (fpb - 1)), ia, ib, ic);
+      }
+    }
+    s_prefix[l] = count;
+    // Append the listed pairs, one LDS atomic per wave and route: route 0 from the front
+    // of s_work, route 1 from the back, so that the waves of step 2 take one route each.
+    const uint64_t m0 = __ballot(route == 0), m1 = __ballot(route == 1);
+    int at0 = 0, at1 = 0;
+    if (lane == 0) {
+      if (m0 != 0) at0 = atomicAdd(&s_nwork[0], __popcll(m0));
+      if (m1 != 0) at1 = atomicAdd(&s_nwork[1], __popcll(m1));
+    }
+    at0 = __shfl(at0, 0, kWave);
+    at1 = __shfl(at1, 0, kWave);
+    if (route == 0) s_work[at0 + __popcll(m0 & lanemask_lt())] = static_cast<int16_t>(l);
+    if (route == 1)
+      s_work[kPassFaces - 1 - (at1 + __popcll(m1 & lanemask_lt()))] = static_cast<int16_t>(l);
+  };
+  // kHoist slots at a time: their corners are all loaded before any is used, so that the
+  // loads overlap. With lfpb == log2(kPassThreads) every slot holds the same face (the
+  // slice's cameras), whose corner indices are then read once.
   constexpr int kHoist = 2;
+  const bool shared_face = (fpb == kPassThreads);
+  int32_t sa = -1, sb = -1, sc = -1;
+  if (shared_face) face_corners(0, sa, sb, sc);
 #pragma unroll
   for (int i0 = 0; i0 < kPassPer; i0 += kHoist) {
     int32_t ca[kHoist], cb[kHoist], cc[kHoist];
-#pragma unroll
-    for (int h = 0; h < kHoist; ++h) {
-      const int l = (i0 + h) * kPassThreads + threadIdx.x;
-      const int b = bfirst + (l >> lfpb);
-      const int64_t f = f0 + (l & (fpb - 1));
-      ca[h] = cb[h] = cc[h] = -1;
-      if (b < B && f < fend) {
-        ca[h] = faces[3 * f + 0];
-        cb[h] = faces[3 * f + 1];
-        cc[h] = faces[3 * f + 2];
-      }
-    }
+    bool valid[kHoist];
     float2 pa[kHoist], pb[kHoist], pc[kHoist];
     float za[kHoist], zb[kHoist], zc[kHoist];
-    bool valid[kHoist];
 #pragma unroll
     for (int h = 0; h < kHoist; ++h) {
-      const int l = (i0 + h) * kPassThreads + threadIdx.x;
-      const int64_t o = int64_t(bfirst + (l >> lfpb)) * V;
-      valid[h] = ca[h] >= 0 && cb[h] >= 0 && cc[h] >= 0 && ca[h] < V && cb[h] < V && cc[h] < V;
-      if (valid[h]) {
-        pa[h] = pxy[o + ca[h]];
-        pb[h] = pxy[o + cb[h]];
-        pc[h] = pxy[o + cc[h]];
-        za[h] = depths[o + ca[h]];
-        zb[h] = depths[o + cb[h]];
-        zc[h] = depths[o + cc[h]];
+      const int i = i0 + h;
+      if (shared_face) {
+        ca[h] = sa;
+        cb[h] = sb;
+        cc[h] = sc;
+      } else {
+        face_corners(i, ca[h], cb[h], cc[h]);
       }
     }
 #pragma unroll
     for (int h = 0; h < kHoist; ++h) {
-      const int l = (i0 + h) * kPassThreads + threadIdx.x;
-      const int b = bfirst + (l >> lfpb);
-      const int64_t f = f0 + (l & (fpb - 1));
-      int32_t count = 0;
-      bool listed = false;
+      const int i = i0 + h;
+      const int b = bfirst + ((i * kPassThreads) >> lfpb);  // uniform
+      // Negative indices are >= 2^31 as unsigned numbers, so one comparison each.
+      valid[h] = (b < B) & (uint32_t(ca[h]) < vlim) & (uint32_t(cb[h]) < vlim) &
+                 (uint32_t(cc[h]) < vlim);
+      const float2* cxy = pxy + int64_t(b) * V;
+      const float* cd = depths + int64_t(b) * V;
       if (valid[h]) {
-        int r0 = rmin, c0 = cmin, r1 = rmax, c1 = cmax;
-        if (cam_dims) {  // this camera's own tile grid
-          r0 = 0;
-          c0 = 0;
-          r1 = cam_dims[4 * b + 2];
-          c1 = cam_dims[4 * b + 3];
-        }
-        const TileRect r = corner_tile_rect(pa[h], pb[h], pc[h], za[h], zb[h], zc[h], ts, r0, c0,
-                                            r1, c1, near_plane);
-        s_org[l] = (uint32_t(r.y0) << 16) | uint32_t(r.x0);
-        s_ext[l] = (uint32_t(r.y1 - r.y0) << 16) | uint32_t(r.x1 - r.x0);
-        count = (r.y1 - r.y0) * (r.x1 - r.x0);
-        if (count > 0 && cull) {
-          listed = true;
-          count = 0;  // set by step 2
-        } else if (count > 0) {
-          s_key_hi[l] = key_hi(b, f, ca[h], cb[h], cc[h]);
-        }
+        pa[h] = cxy[uint32_t(ca[h])];
+        pb[h] = cxy[uint32_t(cb[h])];
+        pc[h] = cxy[uint32_t(cc[h])];
+        za[h] = cd[uint32_t(ca[h])];
+        zb[h] = cd[uint32_t(cb[h])];
+        zc[h] = cd[uint32_t(cc[h])];
       }
-      s_prefix[l] = count;
-      // Append the listed pairs: one LDS atomic per wave.
-      const uint64_t m = __ballot(listed);
-      int at = 0;
-      if (lane == 0 && m != 0) at = atomicAdd(&s_nwork, __popcll(m));
-      at = __shfl(at, 0, kWave);
-      if (listed) s_work[at + __popcll(m & lanemask_lt())] = static_cast<int16_t>(l);
     }
+#pragma unroll
+    for (int h = 0; h < kHoist; ++h)
+      bound_slot(i0 + h, ca[h], cb[h], cc[h], pa[h], pb[h], pc[h], za[h], zb[h], zc[h], valid[h]);
   }
   __syncthreads();
-  // 2. Pixel-level narrowing of the listed pairs.
-  const int nwork = s_nwork;
+  // 2. Pixel-level narrowing of the listed pairs: route 0 (positions [0, n0) of s_work)
+  // then route 1 (the last n1 positions); a wave mixes the routes only where they meet.
+  const int n0 = s_nwork[0], nwork = n0 + s_nwork[1];
   for (int k = threadIdx.x; k < nwork; k += kPassThreads) {
-    const int l = s_work[k];
+    const int l = s_work[k < n0 ? k : kPassFaces - 1 - (k - n0)];
     const int b = bfirst + (l >> lfpb);
     const int64_t f = f0 + (l & (fpb - 1));
     TileRect r;

@@ -4,9 +4,14 @@
 // test accepts the triangle must lie inside the rectangle. Host IEEE float arithmetic
 // with -ffp-contract=off evaluates the same expressions as the GPU kernels.
 //
+// The same sweep checks the bounding-box route (centre_range): when it reports no
+// candidate pixel centre, no pixel of the region may accept the triangle; when it reports a
+// few, every accepted pixel must lie among them.
+//
 // Prints "pairs=<tested> hits=<accepted> outside=<violations> area=<sum of rect areas>
-// full=<whole-tile fallbacks> pixels=<sum of tile / region pixel counts>" and exits 1 on
-// any violation.
+// full=<whole-tile fallbacks> pixels=<sum of tile / region pixel counts> centre_none=<...>
+// centre_few=<...> centre_unknown=<...> centre_outside=<violations>" and exits 1 on any
+// violation.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -133,6 +138,7 @@ int main(int argc, char** argv) {
   static const int shapes[8][2] = {{8, 8},   {16, 16}, {32, 8}, {16, 32},
                                    {24, 40}, {40, 72}, {8, 96}, {64, 16}};
   long long pairs = 0, hits = 0, outside = 0, area = 0, full = 0, pixels = 0;
+  long long cnone = 0, cfew = 0, cunknown = 0, coutside = 0;
   for (long it = 0; it < n; ++it) {
     const int* sh = shapes[next_u64() % 8];
     const int th = sh[0], tw = sh[1];
@@ -148,6 +154,17 @@ int main(int argc, char** argv) {
       std::printf("bad rectangle %d %d %d %d\n", r.x, r.y, r.w, r.h);
       return 1;
     }
+    const mt::CentreRange cen =
+        mt::centre_range(t.x[0], t.y[0], t.x[1], t.y[1], t.x[2], t.y[2], x0, y0, tw, th, 16);
+    cnone += cen.state == mt::kCentresNone;
+    cfew += cen.state == mt::kCentresFew;
+    cunknown += cen.state == mt::kCentresUnknown;
+    if (cen.state == mt::kCentresFew &&
+        (cen.x < 0 || cen.y < 0 || cen.w <= 0 || cen.h <= 0 || cen.x + cen.w > tw ||
+         cen.y + cen.h > th || cen.w * cen.h > 16)) {
+      std::printf("bad centre range %d %d %d %d\n", cen.x, cen.y, cen.w, cen.h);
+      return 1;
+    }
     for (int j = 0; j < th; ++j) {
       for (int i = 0; i < tw; ++i) {
         const float px = float(x0 + i) + 0.5f, py = float(y0 + j) + 0.5f;
@@ -156,6 +173,17 @@ int main(int argc, char** argv) {
         ++pairs;
         if (!mt::bary_accept(u, v, w)) continue;
         ++hits;
+        const bool in_centres = cen.state == mt::kCentresUnknown ||
+                                (cen.state == mt::kCentresFew && i >= cen.x && i < cen.x + cen.w &&
+                                 j >= cen.y && j < cen.y + cen.h);
+        if (!in_centres) {
+          ++coutside;
+          if (coutside <= 10)
+            std::printf("centre violation: tri (%a,%a) (%a,%a) (%a,%a) tile (%d,%d,%dx%d) pixel "
+                        "(%d,%d) state %d range (%d,%d,%d,%d)\n",
+                        t.x[0], t.y[0], t.x[1], t.y[1], t.x[2], t.y[2], x0, y0, th, tw, i, j,
+                        cen.state, cen.x, cen.y, cen.w, cen.h);
+        }
         const bool in = i >= r.x && i < r.x + r.w && j >= r.y && j < r.y + r.h;
         if (!in) {
           ++outside;
@@ -168,7 +196,8 @@ int main(int argc, char** argv) {
       }
     }
   }
-  std::printf("pairs=%lld hits=%lld outside=%lld area=%lld full=%lld pixels=%lld\n", pairs, hits, outside,
-              area, full, pixels);
-  return outside == 0 ? 0 : 1;
+  std::printf("pairs=%lld hits=%lld outside=%lld area=%lld full=%lld pixels=%lld centre_none=%lld "
+              "centre_few=%lld centre_unknown=%lld centre_outside=%lld\n",
+              pairs, hits, outside, area, full, pixels, cnone, cfew, cunknown, coutside);
+  return outside == 0 && coutside == 0 ? 0 : 1;
 }
