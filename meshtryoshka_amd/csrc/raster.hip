@@ -490,6 +490,7 @@ __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
 constexpr int kPassThreads = 256;  // one wave per SIMD per block
 constexpr int kPassPer = 4;
 constexpr int kPassFaces = kPassPer * kPassThreads;
+constexpr int kMaxPassShells = 64;
 
 template <bool kPow2>
 __global__ void __launch_bounds__(kPassThreads) __attribute__((amdgpu_waves_per_eu(kPow2 ? 6 : 5)))
@@ -509,6 +510,8 @@ keys_single_pass_kernel(
   __shared__ int32_t s_wave[kPassThreads / kWave];
   __shared__ int32_t s_nwork[2];  // pairs listed for each route of step 2
   __shared__ int64_t s_q, s_block_start;
+  __shared__ int64_t s_bounds[kMaxPassShells];  // the shells' face boundaries
+  if (int(threadIdx.x) < S) s_bounds[threadIdx.x] = boundaries[threadIdx.x];
   if (threadIdx.x == 0) {
     s_q = atomicAdd(st.ticket, 1u);
     s_nwork[0] = s_nwork[1] = 0;
@@ -528,13 +531,18 @@ keys_single_pass_kernel(
   const int64_t live = live_x3 != nullptr ? int64_t(*live_x3) / 3 : F;
   const int64_t fend = live < F ? live : F;  // faces below it are keyed if their corners are valid
   // Pair l: camera bfirst + (l >> lfpb), face f0 + (l & (fpb - 1)).
+  // The first shell s with f < boundaries[s] (S if none), from the block's LDS copy.
+  auto shell_of = [&](int64_t f) {
+    uint32_t shell = 0;
+    while (shell < uint32_t(S) && !(f < s_bounds[shell])) ++shell;
+    return shell;
+  };
   auto key_hi = [&](int b, int64_t f, int64_t ia, int64_t ib, int64_t ic) {
     const int64_t o = int64_t(b) * V;
-    uint32_t shell = 0;
-    while (shell < uint32_t(S) && !(f < boundaries[shell])) ++shell;
     const float mean_depth = ((depths[o + ia] + depths[o + ib]) + depths[o + ic]) / 3.f;
-    return pack_key(kf, uint32_t(b), shell, 0u, mean_depth);
+    return pack_key(kf, uint32_t(b), shell_of(f), 0u, mean_depth);
   };
+
   // 1. Bounds. A pair slot i (pairs i * kPassThreads + threadIdx.x) holds one camera for the
   // whole block (lfpb >= log2(kPassThreads)), so its camera's arrays are uniform; corner
   // indices are checked without branches and used as 32-bit offsets.
@@ -577,6 +585,9 @@ keys_single_pass_kernel(
         // narrowed in step 2.
         const int state = box_centres(pa, pb, pc, r, ts.th, ts.tw).state;
         route = state == kCentresFew ? 0 : (state == kCentresUnknown ? 1 : -1);
+        // Listed pairs keep their corner depths' sum (the reference's order) in s_key_hi until
+        // step 2 makes it the key's high part, so that step 2 reads no depths from memory.
+        if (route >= 0) s_key_hi[l] = __float_as_uint((za + zb) + zc);
         count = 0;  // set by step 2
       } else if (count > 0) {
         s_key_hi[l] = key_hi(b, f0 + (l & (fpb - 1)), ia, ib, ic);
@@ -661,7 +672,10 @@ keys_single_pass_kernel(
     s_org[l] = (uint32_t(r.y0) << 16) | uint32_t(r.x0);
     s_ext[l] = (uint32_t(r.y1 - r.y0) << 16) | uint32_t(r.x1 - r.x0);
     s_prefix[l] = count;
-    if (count > 0) s_key_hi[l] = key_hi(b, f, faces[3 * f + 0], faces[3 * f + 1], faces[3 * f + 2]);
+    if (count > 0) {
+      const float mean_depth = __uint_as_float(uint32_t(s_key_hi[l])) / 3.f;
+      s_key_hi[l] = pack_key(kf, uint32_t(b), shell_of(f), 0u, mean_depth);
+    }
   }
   __syncthreads();
   // 3. Scan in pair order, look-back, keys.
@@ -974,6 +988,8 @@ extern "C" int mt_generate_keys_single_pass(
   MT_REQUIRE(F >= 0 && B >= 0 && B <= 65535 && key_capacity >= 0 && tile_h > 0 && tile_w > 0,
              "mt_generate_keys_single_pass: bad sizes");
   MT_REQUIRE(n_keys != nullptr, "mt_generate_keys_single_pass: null key count");
+  MT_REQUIRE(S >= 0 && S <= mt::kMaxPassShells, "mt_generate_keys_single_pass: %d shells (at most %d)",
+             S, mt::kMaxPassShells);
   MT_REQUIRE(row_min >= 0 && col_min >= 0 && row_max <= 65535 && col_max <= 65535 &&
                  row_min <= row_max && col_min <= col_max,
              "mt_generate_keys_single_pass: the tile grid must lie in [0, 65535]^2");
