@@ -142,22 +142,15 @@ __device__ __forceinline__ TileRect face_tile_rect(const float* xy, const float*
 constexpr int kExactPixels = 16;
 
 // The pixel tests of cull_tiles (below) for a candidate rectangle q of at most kExactPixels
-// pixels. The face's corners and the triangle's terms are loaded and computed again (behind
-// a compiler memory barrier), so that they do not stay in registers across the candidate
-// rectangle.
+// pixels of the face (a, bb, c): the tiles of its tile rectangle r that hold an accepted pixel.
 template <bool kPow2>
-__device__ __forceinline__ TileRect exact_tiles(TileRect r, Rect q, const float* xy,
-                                                const int32_t* faces, int64_t f, int64_t o,
-                                                const TileSize<kPow2>& ts) {
+__device__ __forceinline__ TileRect exact_tiles_at(TileRect r, Rect q, float2 a, float2 bb,
+                                                   float2 c, const TileSize<kPow2>& ts) {
   const int th = ts.th, tw = ts.tw;
   TileRect n;
   n.y0 = n.y1 = r.y0;
   n.x0 = n.x1 = r.x0;
   const int px0 = r.x0 * tw, py0 = r.y0 * th;
-  asm volatile("" ::: "memory");
-  const float2* pxy = reinterpret_cast<const float2*>(xy);
-  const float2 a = pxy[o + faces[3 * f + 0]], bb = pxy[o + faces[3 * f + 1]];
-  const float2 c = pxy[o + faces[3 * f + 2]];
   const TriTerms t = tri_terms(a.x, a.y, bb.x, bb.y, c.x, c.y);
   const float ad = fabsf(t.D);
   const float rd = (ad >= 0x1p-60f && ad <= 0x1p60f) ? __builtin_amdgcn_rcpf(t.D) : 0.0f;
@@ -182,6 +175,19 @@ __device__ __forceinline__ TileRect exact_tiles(TileRect r, Rect q, const float*
   n.x0 = ts.col(ax0);
   n.x1 = ts.col(ax1) + 1;
   return n;
+}
+
+// exact_tiles_at with the corners loaded again (behind a compiler memory barrier), so that
+// they do not stay in registers across the candidate rectangle.
+template <bool kPow2>
+__device__ __forceinline__ TileRect exact_tiles(TileRect r, Rect q, const float* xy,
+                                                const int32_t* faces, int64_t f, int64_t o,
+                                                const TileSize<kPow2>& ts) {
+  asm volatile("" ::: "memory");
+  const float2* pxy = reinterpret_cast<const float2*>(xy);
+  const float2 a = pxy[o + faces[3 * f + 0]], bb = pxy[o + faces[3 * f + 1]];
+  const float2 c = pxy[o + faces[3 * f + 2]];
+  return exact_tiles_at(r, q, a, bb, c, ts);
 }
 
 // The pixel centres of face (a, b, c) in its tile-aligned box r that can hit
@@ -491,6 +497,7 @@ constexpr int kPassThreads = 256;  // one wave per SIMD per block
 constexpr int kPassPer = 4;
 constexpr int kPassFaces = kPassPer * kPassThreads;
 constexpr int kMaxPassShells = 64;
+constexpr int kStaged = 256;
 
 template <bool kPow2>
 __global__ void __launch_bounds__(kPassThreads) __attribute__((amdgpu_waves_per_eu(kPow2 ? 6 : 5)))
@@ -505,8 +512,15 @@ keys_single_pass_kernel(
   // Tile rectangles packed 16 + 16 bits: origin (row << 16 | column), extent (height << 16 |
   // width); grids have at most 65535 tile rows and columns.
   __shared__ uint32_t s_org[kPassFaces], s_ext[kPassFaces];
-  __shared__ uint64_t s_key_hi[kPassFaces];
+  // Corner depths' sum (the reference's order, bits of a float) of the pairs with keys: the
+  // key's depth field is its third, made when the keys are written.
+  __shared__ uint32_t s_zsum[kPassFaces];
   __shared__ int16_t s_work[kPassFaces];
+  // The first kStaged pairs of route 0: their corners and candidate pixel centres (x | y << 10
+  // | (w - 1) << 20 | (h - 1) << 24, or ~0 when they do not fit), so that step 2 reads
+  // nothing from memory for them.
+  __shared__ float2 s_corner[kStaged * 3];
+  __shared__ uint32_t s_centres[kStaged];
   __shared__ int32_t s_wave[kPassThreads / kWave];
   __shared__ int32_t s_nwork[2];  // pairs listed for each route of step 2
   __shared__ int64_t s_q, s_block_start;
@@ -537,11 +551,6 @@ keys_single_pass_kernel(
     while (shell < uint32_t(S) && !(f < s_bounds[shell])) ++shell;
     return shell;
   };
-  auto key_hi = [&](int b, int64_t f, int64_t ia, int64_t ib, int64_t ic) {
-    const int64_t o = int64_t(b) * V;
-    const float mean_depth = ((depths[o + ia] + depths[o + ib]) + depths[o + ic]) / 3.f;
-    return pack_key(kf, uint32_t(b), shell_of(f), 0u, mean_depth);
-  };
 
   // 1. Bounds. A pair slot i (pairs i * kPassThreads + threadIdx.x) holds one camera for the
   // whole block (lfpb >= log2(kPassThreads)), so its camera's arrays are uniform; corner
@@ -568,6 +577,7 @@ keys_single_pass_kernel(
     const int b = bfirst + (l >> lfpb);
     int32_t count = 0;
     int route = -1;  // step 2: 0 = the pixel centres near the box, 1 = candidate rectangle
+    uint32_t centres = ~0u;  // route 0: the packed candidate centres
     if (valid) {
       int r0 = rmin, c0 = cmin, r1 = rmax, c1 = cmax;
       if (cam_dims) {  // this camera's own tile grid
@@ -583,14 +593,16 @@ keys_single_pass_kernel(
       if (count > 0 && cull) {
         // Faces whose bounding box reaches no pixel centre have no keys; the others are
         // narrowed in step 2.
-        const int state = box_centres(pa, pb, pc, r, ts.th, ts.tw).state;
-        route = state == kCentresFew ? 0 : (state == kCentresUnknown ? 1 : -1);
-        // Listed pairs keep their corner depths' sum (the reference's order) in s_key_hi until
-        // step 2 makes it the key's high part, so that step 2 reads no depths from memory.
-        if (route >= 0) s_key_hi[l] = __float_as_uint((za + zb) + zc);
+        const CentreRange cen = box_centres(pa, pb, pc, r, ts.th, ts.tw);
+        route = cen.state == kCentresFew ? 0 : (cen.state == kCentresUnknown ? 1 : -1);
+        centres = (cen.x < 1024 && cen.y < 1024)
+                      ? uint32_t(cen.x) | (uint32_t(cen.y) << 10) | (uint32_t(cen.w - 1) << 20) |
+                            (uint32_t(cen.h - 1) << 24)
+                      : ~0u;
+        if (route >= 0) s_zsum[l] = __float_as_uint((za + zb) + zc);
         count = 0;  // set by step 2
       } else if (count > 0) {
-        s_key_hi[l] = key_hi(b, f0 + (l & (fpb - 1)), ia, ib, ic);
+        s_zsum[l] = __float_as_uint((za + zb) + zc);
       }
     }
     s_prefix[l] = count;
@@ -604,7 +616,16 @@ keys_single_pass_kernel(
     }
     at0 = __shfl(at0, 0, kWave);
     at1 = __shfl(at1, 0, kWave);
-    if (route == 0) s_work[at0 + __popcll(m0 & lanemask_lt())] = static_cast<int16_t>(l);
+    if (route == 0) {
+      const int pos = at0 + __popcll(m0 & lanemask_lt());
+      s_work[pos] = static_cast<int16_t>(l);
+      if (pos < kStaged) {
+        s_corner[3 * pos + 0] = pa;
+        s_corner[3 * pos + 1] = pb;
+        s_corner[3 * pos + 2] = pc;
+        s_centres[pos] = centres;
+      }
+    }
     if (route == 1)
       s_work[kPassFaces - 1 - (at1 + __popcll(m1 & lanemask_lt()))] = static_cast<int16_t>(l);
   };
@@ -667,15 +688,18 @@ keys_single_pass_kernel(
     r.x0 = int(s_org[l] & 0xffffu);
     r.y1 = r.y0 + int(s_ext[l] >> 16);
     r.x1 = r.x0 + int(s_ext[l] & 0xffffu);
-    r = cull_tiles(r, xy, faces, f, int64_t(b) * V, ts);
+    const uint32_t cpk = (k < n0 && k < kStaged) ? s_centres[k] : ~0u;
+    if (cpk != ~0u) {  // staged by step 1: the same candidates cull_tiles would find
+      const Rect q{int(cpk & 1023u), int((cpk >> 10) & 1023u), int((cpk >> 20) & 15u) + 1,
+                   int((cpk >> 24) & 15u) + 1};
+      r = exact_tiles_at(r, q, s_corner[3 * k + 0], s_corner[3 * k + 1], s_corner[3 * k + 2], ts);
+    } else {
+      r = cull_tiles(r, xy, faces, f, int64_t(b) * V, ts);
+    }
     const int32_t count = (r.y1 - r.y0) * (r.x1 - r.x0);
     s_org[l] = (uint32_t(r.y0) << 16) | uint32_t(r.x0);
     s_ext[l] = (uint32_t(r.y1 - r.y0) << 16) | uint32_t(r.x1 - r.x0);
     s_prefix[l] = count;
-    if (count > 0) {
-      const float mean_depth = __uint_as_float(uint32_t(s_key_hi[l])) / 3.f;
-      s_key_hi[l] = pack_key(kf, uint32_t(b), shell_of(f), 0u, mean_depth);
-    }
   }
   __syncthreads();
   // 3. Scan in pair order, look-back, keys.
@@ -727,13 +751,16 @@ keys_single_pass_kernel(
     const uint32_t row = (s_org[lo] >> 16) + uint32_t(local / w);
     const uint32_t col = (s_org[lo] & 0xffffu) + uint32_t(local % w);
     const uint32_t tile = row * uint32_t(cols) + col;
-    const uint64_t key = s_key_hi[lo] | ((uint64_t(tile) & tile_mask) << kf.tile_offset);
+    const int64_t f = f0 + (lo & (fpb - 1));
+    const float mean_depth = __uint_as_float(s_zsum[lo]) / 3.f;
+    const uint64_t key = pack_key(kf, uint32_t(b), shell_of(f), 0u, mean_depth) |
+                         ((uint64_t(tile) & tile_mask) << kf.tile_offset);
     if (base + j < 0 || base + j >= kcap) {
       if (overflow) atomicOr(overflow, 1);
       continue;
     }
     keys[base + j] = static_cast<int64_t>(key);
-    values[base + j] = static_cast<int32_t>(f0 + (lo & (fpb - 1)));
+    values[base + j] = static_cast<int32_t>(f);
   }
 }
 
