@@ -308,15 +308,14 @@ struct CentreRange {
   int state;
 };
 
-MT_HD CentreRange centre_range(float ax, float ay, float bx, float by, float cx, float cy,
-                               int tile_x0, int tile_y0, int tw, int th, int max_pixels) {
-  CentreRange out{0, 0, 0, 0, kCentresUnknown};
+// The certificate of centre_range (corners inside (-2^12, 2^12), cr >= 2^-20 and the shape
+// inequality above).
+MT_HD bool centre_certificate(float ax, float ay, float bx, float by, float cx, float cy) {
   constexpr float kLim = 0x1p12f;
   // NaN fails every comparison.
   const bool inside = __builtin_fabsf(ax) < kLim && __builtin_fabsf(ay) < kLim &&
                       __builtin_fabsf(bx) < kLim && __builtin_fabsf(by) < kLim &&
                       __builtin_fabsf(cx) < kLim && __builtin_fabsf(cy) < kLim;
-  if (!inside) return out;
   const float e12x = bx - ax, e12y = by - ay, e13x = cx - ax, e13y = cy - ay;
   const float e23x = cx - bx, e23y = cy - by;
   const float cr = __builtin_fabsf(e12x * e13y - e12y * e13x);
@@ -329,7 +328,29 @@ MT_HD CentreRange centre_range(float ax, float ay, float bx, float by, float cx,
   const float yhi = __builtin_fmaxf(__builtin_fmaxf(ay, by), cy);
   const float wb = __builtin_fmaxf(xhi - xlo, yhi - ylo) * (1.0f + 0x1p-10f);
   const float rhs = 0x1p-15f * lsq * wb * wb * (1.0f + (2.0f / kCentreMargin) * wb);
-  if (!(cr >= 0x1p-20f) || !(cr * cr * (1.0f - 0x1p-6f) >= rhs * (1.0f + 0x1p-10f))) return out;
+  return inside && cr >= 0x1p-20f && cr * cr * (1.0f - 0x1p-6f) >= rhs * (1.0f + 0x1p-10f);
+}
+
+// True when no pixel centre lies within the triangle's bounding box widened by
+// kCentreMargin (no clamp to a tile box). With centre_certificate it proves that no pixel
+// accepts the triangle; centre_range then reports kCentresNone for every tile box.
+MT_HD bool centre_box_empty(float ax, float ay, float bx, float by, float cx, float cy) {
+  const float xlo = __builtin_fminf(__builtin_fminf(ax, bx), cx);
+  const float xhi = __builtin_fmaxf(__builtin_fmaxf(ax, bx), cx);
+  const float ylo = __builtin_fminf(__builtin_fminf(ay, by), cy);
+  const float yhi = __builtin_fmaxf(__builtin_fmaxf(ay, by), cy);
+  return __builtin_ceilf(xlo - kCentreMargin - 0.5f) > __builtin_floorf(xhi + kCentreMargin - 0.5f) ||
+         __builtin_ceilf(ylo - kCentreMargin - 0.5f) > __builtin_floorf(yhi + kCentreMargin - 0.5f);
+}
+
+MT_HD CentreRange centre_range(float ax, float ay, float bx, float by, float cx, float cy,
+                               int tile_x0, int tile_y0, int tw, int th, int max_pixels) {
+  CentreRange out{0, 0, 0, 0, kCentresUnknown};
+  if (!centre_certificate(ax, ay, bx, by, cx, cy)) return out;
+  const float xlo = __builtin_fminf(__builtin_fminf(ax, bx), cx);
+  const float xhi = __builtin_fmaxf(__builtin_fmaxf(ax, bx), cx);
+  const float ylo = __builtin_fminf(__builtin_fminf(ay, by), cy);
+  const float yhi = __builtin_fmaxf(__builtin_fmaxf(ay, by), cy);
   // Pixel X has its centre at X + 0.5.
   int x0 = int(__builtin_ceilf(xlo - kCentreMargin - 0.5f));
   int x1 = int(__builtin_floorf(xhi + kCentreMargin - 0.5f));

@@ -177,18 +177,6 @@ __device__ __forceinline__ TileRect exact_tiles_at(TileRect r, Rect q, float2 a,
   return n;
 }
 
-// exact_tiles_at with the corners loaded again (behind a compiler memory barrier), so that
-// they do not stay in registers across the candidate rectangle.
-template <bool kPow2>
-__device__ __forceinline__ TileRect exact_tiles(TileRect r, Rect q, const float* xy,
-                                                const int32_t* faces, int64_t f, int64_t o,
-                                                const TileSize<kPow2>& ts) {
-  asm volatile("" ::: "memory");
-  const float2* pxy = reinterpret_cast<const float2*>(xy);
-  const float2 a = pxy[o + faces[3 * f + 0]], bb = pxy[o + faces[3 * f + 1]];
-  const float2 c = pxy[o + faces[3 * f + 2]];
-  return exact_tiles_at(r, q, a, bb, c, ts);
-}
 
 // The pixel centres of face (a, b, c) in its tile-aligned box r that can hit
 // (geometry.h, centre_range): none, a few, or unknown.
@@ -198,17 +186,15 @@ __device__ __forceinline__ CentreRange box_centres(float2 a, float2 bb, float2 c
                       (r.y1 - r.y0) * th, kExactPixels);
 }
 
-// Face f (whose three corners the caller has checked), camera offset o into xy. Small,
-// well-shaped faces take their candidate pixels from the bounding box (box_centres); the
-// others from the candidate rectangle.
-template <bool kPow2>
-__device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, const int32_t* faces,
-                                               int64_t f, int64_t o, const TileSize<kPow2>& ts) {
+// Face (a, bb, c) with tile rectangle r. Small, well-shaped faces take their candidate pixels
+// from the bounding box (box_centres); the others from the candidate rectangle. reload(a,
+// bb, c) may load the corners again before the pixel tests, so that they need not stay in
+// registers across the candidate rectangle.
+template <bool kPow2, typename Reload>
+__device__ __forceinline__ TileRect cull_tiles_at(TileRect r, float2 a, float2 bb, float2 c,
+                                                  const TileSize<kPow2>& ts, Reload reload) {
   if (r.y1 <= r.y0 || r.x1 <= r.x0) return r;
   const int th = ts.th, tw = ts.tw;
-  const float2* pxy = reinterpret_cast<const float2*>(xy);
-  const float2 a = pxy[o + faces[3 * f + 0]], bb = pxy[o + faces[3 * f + 1]];
-  const float2 c = pxy[o + faces[3 * f + 2]];
   TileRect n;
   n.y0 = n.y1 = r.y0;
   n.x0 = n.x1 = r.x0;
@@ -226,8 +212,29 @@ __device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, cons
       n.x1 = r.x0 + ts.col(q.x + q.w - 1) + 1;
       return n;
     }
+    reload(a, bb, c);
   }
-  return exact_tiles(r, q, xy, faces, f, o, ts);
+  return exact_tiles_at(r, q, a, bb, c, ts);
+}
+
+// cull_tiles_at for face f (whose three corners the caller has checked), camera offset o
+// into xy.
+template <bool kPow2>
+__device__ __forceinline__ TileRect cull_tiles(TileRect r, const float* xy, const int32_t* faces,
+                                               int64_t f, int64_t o, const TileSize<kPow2>& ts) {
+  if (r.y1 <= r.y0 || r.x1 <= r.x0) return r;
+  const float2* pxy = reinterpret_cast<const float2*>(xy);
+  auto load = [&](float2& a, float2& bb, float2& c) {
+    a = pxy[o + faces[3 * f + 0]];
+    bb = pxy[o + faces[3 * f + 1]];
+    c = pxy[o + faces[3 * f + 2]];
+  };
+  float2 a, bb, c;
+  load(a, bb, c);
+  return cull_tiles_at(r, a, bb, c, ts, [&](float2& a2, float2& b2, float2& c2) {
+    asm volatile("" ::: "memory");
+    load(a2, b2, c2);
+  });
 }
 
 // Face whose keys are generated: live (below the device-side face count) and with all
@@ -516,12 +523,10 @@ keys_single_pass_kernel(
   // key's depth field is its third, made when the keys are written.
   __shared__ uint32_t s_zsum[kPassFaces];
   __shared__ int16_t s_work[kPassFaces];
-  // The first kStaged pairs of route 0: their corners and candidate pixel centres (x | y << 10
-  // | (w - 1) << 20 | (h - 1) << 24, or ~0 when they do not fit), so that step 2 reads
-  // nothing from memory for them.
+  // The corners of the first kStaged pairs of route 0, so that step 2 reads nothing from
+  // memory for them.
   __shared__ float2 s_corner[kStaged * 3];
-  __shared__ uint32_t s_centres[kStaged];
-  __shared__ int32_t s_wave[kPassThreads / kWave];
+  __shared__ int32_t s_wave[kPassPer][kPassThreads / kWave];
   __shared__ int32_t s_nwork[2];  // pairs listed for each route of step 2
   __shared__ int64_t s_q, s_block_start;
   __shared__ int64_t s_bounds[kMaxPassShells];  // the shells' face boundaries
@@ -577,7 +582,6 @@ keys_single_pass_kernel(
     const int b = bfirst + (l >> lfpb);
     int32_t count = 0;
     int route = -1;  // step 2: 0 = the pixel centres near the box, 1 = candidate rectangle
-    uint32_t centres = ~0u;  // route 0: the packed candidate centres
     if (valid) {
       int r0 = rmin, c0 = cmin, r1 = rmax, c1 = cmax;
       if (cam_dims) {  // this camera's own tile grid
@@ -593,12 +597,11 @@ keys_single_pass_kernel(
       if (count > 0 && cull) {
         // Faces whose bounding box reaches no pixel centre have no keys; the others are
         // narrowed in step 2.
-        const CentreRange cen = box_centres(pa, pb, pc, r, ts.th, ts.tw);
-        route = cen.state == kCentresFew ? 0 : (cen.state == kCentresUnknown ? 1 : -1);
-        centres = (cen.x < 1024 && cen.y < 1024)
-                      ? uint32_t(cen.x) | (uint32_t(cen.y) << 10) | (uint32_t(cen.w - 1) << 20) |
-                            (uint32_t(cen.h - 1) << 24)
-                      : ~0u;
+        // Only the certificate and the box's emptiness here: the candidate centres of the
+        // others are found in step 2 (cull_tiles_at), which keeps this part short.
+        const bool cert = centre_certificate(pa.x, pa.y, pb.x, pb.y, pc.x, pc.y);
+        const bool none = cert && centre_box_empty(pa.x, pa.y, pb.x, pb.y, pc.x, pc.y);
+        route = none ? -1 : (cert ? 0 : 1);
         if (route >= 0) s_zsum[l] = __float_as_uint((za + zb) + zc);
         count = 0;  // set by step 2
       } else if (count > 0) {
@@ -623,7 +626,6 @@ keys_single_pass_kernel(
         s_corner[3 * pos + 0] = pa;
         s_corner[3 * pos + 1] = pb;
         s_corner[3 * pos + 2] = pc;
-        s_centres[pos] = centres;
       }
     }
     if (route == 1)
@@ -688,11 +690,9 @@ keys_single_pass_kernel(
     r.x0 = int(s_org[l] & 0xffffu);
     r.y1 = r.y0 + int(s_ext[l] >> 16);
     r.x1 = r.x0 + int(s_ext[l] & 0xffffu);
-    const uint32_t cpk = (k < n0 && k < kStaged) ? s_centres[k] : ~0u;
-    if (cpk != ~0u) {  // staged by step 1: the same candidates cull_tiles would find
-      const Rect q{int(cpk & 1023u), int((cpk >> 10) & 1023u), int((cpk >> 20) & 15u) + 1,
-                   int((cpk >> 24) & 15u) + 1};
-      r = exact_tiles_at(r, q, s_corner[3 * k + 0], s_corner[3 * k + 1], s_corner[3 * k + 2], ts);
+    if (k < n0 && k < kStaged) {  // corners staged by step 1
+      r = cull_tiles_at(r, s_corner[3 * k + 0], s_corner[3 * k + 1], s_corner[3 * k + 2], ts,
+                        [](float2&, float2&, float2&) {});
     } else {
       r = cull_tiles(r, xy, faces, f, int64_t(b) * V, ts);
     }
@@ -702,23 +702,29 @@ keys_single_pass_kernel(
     s_prefix[l] = count;
   }
   __syncthreads();
-  // 3. Scan in pair order, look-back, keys.
-  int32_t carry = 0;
+  // 3. Scan in pair order (slot-major), look-back, keys. The slots are scanned together:
+  // wave sums of all of them, one barrier, then each slot's offset from the earlier slots'
+  // totals.
+  int32_t cnt[kPassPer], inc[kPassPer];
+#pragma unroll
   for (int i = 0; i < kPassPer; ++i) {
-    const int l = i * kPassThreads + threadIdx.x;
-    const int32_t count = s_prefix[l];
-    const int32_t inc = wave_inclusive_sum(count);
-    if (lane == kWave - 1) s_wave[wave] = inc;
-    __syncthreads();
-    int32_t before = 0, total = 0;
+    cnt[i] = s_prefix[i * kPassThreads + threadIdx.x];
+    inc[i] = wave_inclusive_sum(cnt[i]);
+    if (lane == kWave - 1) s_wave[i][wave] = inc[i];
+  }
+  __syncthreads();
+  int32_t carry = 0;
+#pragma unroll
+  for (int i = 0; i < kPassPer; ++i) {
+    int32_t before = 0, slot_total = 0;
 #pragma unroll
     for (int w = 0; w < kPassThreads / kWave; ++w) {
-      if (w < wave) before += s_wave[w];
-      total += s_wave[w];
+      const int32_t x = s_wave[i][w];
+      before += w < wave ? x : 0;
+      slot_total += x;
     }
-    s_prefix[l] = carry + before + inc - count;
-    carry += total;
-    __syncthreads();  // s_wave is reused by the next round
+    s_prefix[i * kPassThreads + threadIdx.x] = carry + before + inc[i] - cnt[i];
+    carry += slot_total;
   }
   const int32_t total = carry;
   if (threadIdx.x < kWave) {

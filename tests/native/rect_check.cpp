@@ -6,12 +6,13 @@
 //
 // The same sweep checks the bounding-box route (centre_range): when it reports no
 // candidate pixel centre, no pixel of the region may accept the triangle; when it reports a
-// few, every accepted pixel must lie among them.
+// few, every accepted pixel must lie among them. Likewise, when centre_certificate holds and
+// centre_box_empty reports an empty widened box, no pixel may accept the triangle.
 //
 // Prints "pairs=<tested> hits=<accepted> outside=<violations> area=<sum of rect areas>
 // full=<whole-tile fallbacks> pixels=<sum of tile / region pixel counts> centre_none=<...>
-// centre_few=<...> centre_unknown=<...> centre_outside=<violations>" and exits 1 on any
-// violation.
+// centre_few=<...> centre_unknown=<...> centre_outside=<violations> box_none=<...>
+// box_outside=<violations>" and exits 1 on any violation.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -138,7 +139,7 @@ int main(int argc, char** argv) {
   static const int shapes[8][2] = {{8, 8},   {16, 16}, {32, 8}, {16, 32},
                                    {24, 40}, {40, 72}, {8, 96}, {64, 16}};
   long long pairs = 0, hits = 0, outside = 0, area = 0, full = 0, pixels = 0;
-  long long cnone = 0, cfew = 0, cunknown = 0, coutside = 0;
+  long long cnone = 0, cfew = 0, cunknown = 0, coutside = 0, bnone = 0, boutside = 0;
   for (long it = 0; it < n; ++it) {
     const int* sh = shapes[next_u64() % 8];
     const int th = sh[0], tw = sh[1];
@@ -156,6 +157,9 @@ int main(int argc, char** argv) {
     }
     const mt::CentreRange cen =
         mt::centre_range(t.x[0], t.y[0], t.x[1], t.y[1], t.x[2], t.y[2], x0, y0, tw, th, 16);
+    const bool box_none = mt::centre_certificate(t.x[0], t.y[0], t.x[1], t.y[1], t.x[2], t.y[2]) &&
+                          mt::centre_box_empty(t.x[0], t.y[0], t.x[1], t.y[1], t.x[2], t.y[2]);
+    bnone += box_none;
     cnone += cen.state == mt::kCentresNone;
     cfew += cen.state == mt::kCentresFew;
     cunknown += cen.state == mt::kCentresUnknown;
@@ -184,6 +188,9 @@ int main(int argc, char** argv) {
                         t.x[0], t.y[0], t.x[1], t.y[1], t.x[2], t.y[2], x0, y0, th, tw, i, j,
                         cen.state, cen.x, cen.y, cen.w, cen.h);
         }
+        if (box_none && ++boutside <= 10)
+          std::printf("box violation: tri (%a,%a) (%a,%a) (%a,%a) pixel (%d,%d)\n", t.x[0], t.y[0],
+                      t.x[1], t.y[1], t.x[2], t.y[2], x0 + i, y0 + j);
         const bool in = i >= r.x && i < r.x + r.w && j >= r.y && j < r.y + r.h;
         if (!in) {
           ++outside;
@@ -197,7 +204,7 @@ int main(int argc, char** argv) {
     }
   }
   std::printf("pairs=%lld hits=%lld outside=%lld area=%lld full=%lld pixels=%lld centre_none=%lld "
-              "centre_few=%lld centre_unknown=%lld centre_outside=%lld\n",
-              pairs, hits, outside, area, full, pixels, cnone, cfew, cunknown, coutside);
-  return outside == 0 && coutside == 0 ? 0 : 1;
+              "centre_few=%lld centre_unknown=%lld centre_outside=%lld box_none=%lld box_outside=%lld\n",
+              pairs, hits, outside, area, full, pixels, cnone, cfew, cunknown, coutside, bnone, boutside);
+  return outside == 0 && coutside == 0 && boutside == 0 ? 0 : 1;
 }

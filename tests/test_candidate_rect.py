@@ -36,6 +36,7 @@ def test_every_hit_lies_in_its_candidate_rectangle(rect_check, seed):
     fields = dict(kv.split("=") for kv in res.stdout.split("\n")[-2].split())
     assert int(fields["outside"]) == 0
     assert int(fields["centre_outside"]) == 0
+    assert int(fields["box_outside"]) == 0 and int(fields["box_none"]) > 5000
     assert int(fields["hits"]) > 100000  # the sample exercises plenty of accepted pixels
     # Both answers of the bounding-box route occur often, and so does its fallback.
     assert min(int(fields["centre_none"]), int(fields["centre_few"]),
