@@ -396,22 +396,25 @@ int mt_generate_keys_from_geometry(const float* xy, const float* depths, int64_t
                                    int32_t* values, int64_t key_capacity, int32_t* overflow,
                                    mt_stream_t stream);
 
-/* The three calls above in one kernel (batched training step and serving; no single
+/* The three calls above in one pass (batched training step and serving; no single
  * counterpart in the reference: fuses bound_triangles.slang:144-192, the tile-count cumsum
- * of generate_keys.py:40-47 and generate_keys.slang:4-68). Blocks of faces (for all B
- * cameras) take their index from a ticket, count their keys, find their first output
- * position by decoupled look-back over the earlier blocks, and write their keys, so each
- * face's tile rectangle is computed once. For every camera, the keys and values are those
- * of mt_key_block_counts + scan + mt_generate_keys_from_geometry with the same arguments,
- * in the same order; the cameras' sequences are interleaved block by block, which a
- * stable sort cannot tell apart because keys of different cameras differ in the camera
- * field (B > 1 therefore needs 2^camera_bits >= B). Tile grids: at most 65535 rows and
- * columns (row_min, col_min >= 0 and row_max, col_max <= 65535; the same for the cam_dims
- * grids, which the caller checks). A block handles 1024 (face, camera) pairs and counts
- * its keys in int32, so 1024 x (row_max - row_min) x (col_max - col_min) must stay below
- * 2^31; with cam_dims, row_max and col_max (otherwise unused) bound every camera's grid
- * for this check. *n_keys (device) = the
- * total key count (also when it exceeds key_capacity, which sets *overflow). Workspace:
+ * of generate_keys.py:40-47 and generate_keys.slang:4-68). A workgroup takes one unit of
+ * (face, camera) pairs (a block of faces for a slice of the cameras), counts the unit's
+ * keys, reserves an output range with one atomic add and writes the keys into the scratch
+ * arrays; a scan of the unit counts and a copy kernel then place every unit's keys in unit
+ * order in keys / values. So each face's tile rectangle is computed once and no workgroup
+ * waits for another. For every camera, the keys and values are those of
+ * mt_key_block_counts + scan + mt_generate_keys_from_geometry with the same arguments, in
+ * the same order; the cameras' sequences are interleaved unit by unit, which a stable sort
+ * cannot tell apart because keys of different cameras differ in the camera field (B > 1
+ * therefore needs 2^camera_bits >= B). Tile grids: at most 65535 rows and columns
+ * (row_min, col_min >= 0 and row_max, col_max <= 65535; the same for the cam_dims grids,
+ * which the caller checks). A unit holds 1024 (face, camera) pairs and counts its keys in
+ * int32, so 1024 x (row_max - row_min) x (col_max - col_min) must stay below 2^31; with
+ * cam_dims, row_max and col_max (otherwise unused) bound every camera's grid for this
+ * check. *n_keys (device) = the total key count (also when it exceeds key_capacity, which
+ * sets *overflow). scratch_keys / scratch_values: key_capacity entries each, separate from
+ * keys and values (their contents are overwritten). Workspace:
  * mt_generate_keys_single_pass_workspace_bytes(F, B). */
 size_t mt_generate_keys_single_pass_workspace_bytes(int64_t F, int32_t B);
 int mt_generate_keys_single_pass(const float* xy, const float* depths, int64_t V,
@@ -422,6 +425,7 @@ int mt_generate_keys_single_pass(const float* xy, const float* depths, int64_t V
                                  const int32_t* cam_dims, int32_t cull, int32_t grid_cols,
                                  mt_key_layout layout, int64_t* keys, int32_t* values,
                                  int64_t key_capacity, int64_t* n_keys, int32_t* overflow,
+                                 int64_t* scratch_keys, int32_t* scratch_values,
                                  void* workspace, size_t workspace_bytes, mt_stream_t stream);
 
 

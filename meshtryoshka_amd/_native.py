@@ -155,7 +155,8 @@ SIGNATURES = {
     "mt_generate_keys_single_pass": (
         c_i32,
         [c_p, c_p, c_i64, c_p, c_i64, c_p, c_p, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32,
-         c_i32, c_f32, c_p, c_i32, c_i32, KeyLayout, c_p, c_p, c_i64, c_p, c_p, c_p, c_sz, c_p],
+         c_i32, c_f32, c_p, c_i32, c_i32, KeyLayout, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_sz,
+         c_p],
     ),
     "mt_sort_workspace_bytes": (c_sz, [c_i64]),
     "mt_sort_keys": (c_i32, [c_p, c_p, c_i64, c_p, c_p, c_p, c_sz, c_p, c_p]),

@@ -320,8 +320,8 @@ class BatchedTrainingStep:
                  self.faces.data_ptr(), F, self.num_faces_x3.data_ptr(), face_boundaries.data_ptr(),
                  S, n, th, tw, 0, 0, self.max_rows, self.max_cols, self.near_plane, cam_dims, self.cull,
                  self.cols, self.layout, self.keys.data_ptr(), self.values.data_ptr(), K,
-                 n_keys.data_ptr(), self.overflow.data_ptr(), self.keys_ws.data_ptr(),
-                 self.keys_ws.numel(), st)
+                 n_keys.data_ptr(), self.overflow.data_ptr(), self.sorted_keys.data_ptr(),
+                 self.sorted_values.data_ptr(), self.keys_ws.data_ptr(), self.keys_ws.numel(), st)
             self.group_keys[gi : gi + 1].copy_(n_keys)
             call("mt_sort_keys", self.keys.data_ptr(), self.values.data_ptr(), K,
                  self.sorted_keys.data_ptr(), self.sorted_values.data_ptr(), self.sort_ws.data_ptr(),

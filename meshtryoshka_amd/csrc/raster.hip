@@ -4,7 +4,6 @@
 // tile bounds are reproducible bit for bit.
 #include "common.h"
 #include "geometry.h"
-#include "lookback.h"
 #include "sh.h"
 
 namespace mt {
@@ -485,21 +484,25 @@ __global__ void __launch_bounds__(kBlock) keys_from_geometry_kernel(
   }
 }
 
-// Single pass: the counts, their scan and the keys in one kernel. A block takes its camera
-// slice (kPassFaces / 2^lfpb cameras) and face block (2^lfpb faces; pairs are camera-major
-// inside the block) from a ticket, slice-major, and
+// Single pass: the keys of every (camera, face) pair without per-pair arrays. A block owns
+// one unit: a camera slice (kPassFaces / 2^lfpb cameras) of a face block (2^lfpb faces;
+// pairs are camera-major inside the unit). Units are numbered slice-major; the blocks of
+// one XCD (equal blockIdx mod 8, the hardware's round-robin) take consecutive units, so
+// that the corners a unit shares with its neighbours stay in that XCD's L2. A block
 //   1. bounds each pair (culling, tile-aligned box), drops the pairs whose bounding box
 //      reaches no pixel centre (box_centres), and lists the others for the pixel-level
 //      narrowing (cull_tiles) in LDS, grouped by the route they will take,
 //   2. narrows the listed pairs with every thread busy (back-facing, clipped and keyless
 //      pairs take no thread),
-//   3. scans the key counts in pair order, finds the block's first output position by
-//      decoupled look-back over the earlier blocks' published counts, and writes the keys.
-// A face's corner indices are read from memory once per camera slice. The keys of one
-// camera come out in (face, tile) order, the order of mt_generate_keys_from_geometry; the
-// cameras of a slice are interleaved block by block. Keys of different cameras differ in their camera
-// field, so a stable sort orders both arrangements identically. The last block also stores
-// the total key count.
+//   3. scans the key counts in pair order, reserves its output range with one atomic add on
+//      a key counter (no block waits for another), records the unit's count and range, and
+//      writes its keys there.
+// reorder_keys_kernel then moves every unit's keys to their place in unit order (the scan
+// of the unit counts). A face's corner indices are read from memory once per camera slice.
+// The keys of one camera come out in (face, tile) order, the order of
+// mt_generate_keys_from_geometry; the cameras of a slice are interleaved unit by unit. Keys
+// of different cameras differ in their camera field, so a stable sort orders both
+// arrangements identically.
 constexpr int kPassThreads = 256;  // one wave per SIMD per block
 constexpr int kPassPer = 4;
 constexpr int kPassFaces = kPassPer * kPassThreads;
@@ -514,7 +517,7 @@ keys_single_pass_kernel(
     int th, int tw,
     int rmin, int cmin, int rmax, int cmax, float near_plane, const int32_t* cam_dims, int cull,
     int grid_cols, KeyFields kf, int64_t* keys, int32_t* values, int64_t kcap, int64_t* n_keys,
-    int32_t* overflow, ScanState st) {
+    int32_t* overflow, unsigned long long* key_counter, int32_t* unit_count, int64_t* unit_at) {
   __shared__ int32_t s_prefix[kPassFaces];  // key count, then its exclusive scan
   // Tile rectangles packed 16 + 16 bits: origin (row << 16 | column), extent (height << 16 |
   // width); grids have at most 65535 tile rows and columns.
@@ -528,19 +531,18 @@ keys_single_pass_kernel(
   __shared__ float2 s_corner[kStaged * 3];
   __shared__ int32_t s_wave[kPassPer][kPassThreads / kWave];
   __shared__ int32_t s_nwork[2];  // pairs listed for each route of step 2
-  __shared__ int64_t s_q, s_block_start;
+  __shared__ int64_t s_block_start;
   __shared__ int64_t s_bounds[kMaxPassShells];  // the shells' face boundaries
   if (int(threadIdx.x) < S) s_bounds[threadIdx.x] = boundaries[threadIdx.x];
-  if (threadIdx.x == 0) {
-    s_q = atomicAdd(st.ticket, 1u);
-    s_nwork[0] = s_nwork[1] = 0;
-  }
-  __syncthreads();
-  // The host keeps the block count below 2^31, so block indices are 32-bit.
-  const uint32_t q = uint32_t(s_q);
+  if (threadIdx.x == 0) s_nwork[0] = s_nwork[1] = 0;
+  // The host keeps the unit count below 2^31, so unit indices are 32-bit.
   const int fpb = 1 << lfpb;
   const uint32_t nfb = uint32_t((F + fpb - 1) >> lfpb);  // face blocks
-  const uint32_t nblk = nfb * uint32_t(nslices);
+  const uint32_t nblk = nfb * uint32_t(nslices);         // units
+  const uint32_t per_xcd = (nblk + 7u) / 8u;
+  const uint32_t q = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);  // this block's unit
+  if (q >= nblk) return;  // the grid is rounded up to a multiple of 8
+  __syncthreads();
   const uint32_t slice = q / nfb;
   const int64_t f0 = int64_t(q - slice * nfb) << lfpb;  // first face of the block
   const int bfirst = int(slice) * (kPassFaces >> lfpb);  // first camera of the slice
@@ -702,7 +704,7 @@ keys_single_pass_kernel(
     s_prefix[l] = count;
   }
   __syncthreads();
-  // 3. Scan in pair order (slot-major), look-back, keys. The slots are scanned together:
+  // 3. Scan in pair order (slot-major), output range, keys. The slots are scanned together:
   // wave sums of all of them, one barrier, then each slot's offset from the earlier slots'
   // totals.
   int32_t cnt[kPassPer], inc[kPassPer];
@@ -727,19 +729,13 @@ keys_single_pass_kernel(
     carry += slot_total;
   }
   const int32_t total = carry;
-  if (threadIdx.x < kWave) {
-    int64_t prefix = 0;
-    if (q == 0) {
-      if (threadIdx.x == 0) store_word(&st.words[0], kPrefix, total);
-    } else {
-      if (threadIdx.x == 0) store_word(&st.words[q], kAggregate, total);
-      prefix = look_back(st, q);
-      if (threadIdx.x == 0) store_word(&st.words[q], kPrefix, prefix + int64_t(total));
-    }
-    if (threadIdx.x == 0) {
-      s_block_start = prefix;
-      if (q == nblk - 1 && n_keys) *n_keys = prefix + int64_t(total);
-    }
+  if (threadIdx.x == 0) {
+    const int64_t at =
+        total > 0 ? static_cast<int64_t>(atomicAdd(key_counter, static_cast<unsigned long long>(total)))
+                  : 0;
+    s_block_start = at;
+    unit_count[q] = total;
+    unit_at[q] = at;
   }
   __syncthreads();
   const int64_t base = s_block_start;
@@ -790,16 +786,69 @@ __global__ void fill_i64_kernel(int64_t* p, int64_t n, int64_t value) {
   if (i < n) p[i] = value;
 }
 
+// One wave per unit: moves the unit's keys from the range it reserved (src) to its place in
+// unit order (dst, after the keys of every earlier unit: incl is the inclusive scan of the
+// unit counts). Entries past the capacity were never written (the key kernel raised the
+// overflow flag) and are skipped. The last unit's wave stores the total key count.
+constexpr int kReorderWaves = 4;
+__global__ void __launch_bounds__(kReorderWaves * kWave) reorder_keys_kernel(
+    const int64_t* src_keys, const int32_t* src_values, const int32_t* unit_count,
+    const int64_t* unit_at, const int64_t* incl, int64_t units, int64_t kcap, int64_t* keys,
+    int32_t* values, int64_t* n_keys) {
+  const int64_t u = int64_t(blockIdx.x) * kReorderWaves + threadIdx.x / kWave;
+  if (u >= units) return;
+  const int lane = __lane_id();
+  const int32_t n = unit_count[u];
+  const int64_t from = unit_at[u], to = incl[u] - n;
+  for (int32_t j = lane; j < n; j += kWave) {
+    if (from + j < kcap && to + j < kcap) {
+      keys[to + j] = src_keys[from + j];
+      values[to + j] = src_values[from + j];
+    }
+  }
+  if (u == units - 1 && lane == 0 && n_keys) *n_keys = incl[u];
+}
+
 int64_t single_pass_blocks(int64_t F, int B) {
   if (B <= 0) return 0;
   const int lfpb = single_pass_lfpb(B);
   return ((F + (int64_t(1) << lfpb) - 1) >> lfpb) * single_pass_slices(B, lfpb);
 }
 
-size_t single_pass_state_bytes(int64_t F, int B) {
-  const int64_t blocks = single_pass_blocks(F, B);
-  return 256 + (static_cast<size_t>(blocks) * 8 + 255) / 256 * 256;
+// Workspace of the single pass: the key counter (256 bytes), each unit's key count (int32)
+// and reserved range start (int64), the inclusive scan of the counts (int64), and that
+// scan's own workspace.
+struct SinglePassState {
+  unsigned long long* counter;
+  int32_t* unit_count;
+  int64_t* unit_at;
+  int64_t* incl;
+  void* scan;
+  size_t scan_bytes;
+  size_t total;
+};
+
+inline size_t pad256(size_t x) { return (x + 255) / 256 * 256; }
+
+SinglePassState single_pass_state(void* ws, int64_t F, int B) {
+  const size_t units = static_cast<size_t>(single_pass_blocks(F, B));
+  const size_t o_count = 256, o_at = o_count + pad256(units * 4), o_incl = o_at + pad256(units * 8);
+  const size_t o_scan = o_incl + pad256(units * 8);
+  SinglePassState s{};
+  s.scan_bytes = mt_scan_workspace_bytes(static_cast<int64_t>(units));
+  s.total = o_scan + s.scan_bytes;
+  if (ws != nullptr) {
+    char* p = static_cast<char*>(ws);
+    s.counter = reinterpret_cast<unsigned long long*>(p);
+    s.unit_count = reinterpret_cast<int32_t*>(p + o_count);
+    s.unit_at = reinterpret_cast<int64_t*>(p + o_at);
+    s.incl = reinterpret_cast<int64_t*>(p + o_incl);
+    s.scan = p + o_scan;
+  }
+  return s;
 }
+
+size_t single_pass_state_bytes(int64_t F, int B) { return single_pass_state(nullptr, F, B).total; }
 
 // ------------------------------------------------------------------------------------
 // Delineation (delineate_tiles.slang:12-58)
@@ -1017,10 +1066,25 @@ extern "C" int mt_generate_keys_single_pass(
     int32_t tile_h, int32_t tile_w, int32_t row_min, int32_t col_min, int32_t row_max,
     int32_t col_max, float near_plane, const int32_t* cam_dims, int32_t cull, int32_t grid_cols,
     mt_key_layout layout, int64_t* keys, int32_t* values, int64_t key_capacity, int64_t* n_keys,
-    int32_t* overflow, void* workspace, size_t workspace_bytes, mt_stream_t stream) {
+    int32_t* overflow, int64_t* scratch_keys, int32_t* scratch_values, void* workspace,
+    size_t workspace_bytes, mt_stream_t stream) {
   MT_REQUIRE(F >= 0 && B >= 0 && B <= 65535 && key_capacity >= 0 && tile_h > 0 && tile_w > 0,
              "mt_generate_keys_single_pass: bad sizes");
   MT_REQUIRE(n_keys != nullptr, "mt_generate_keys_single_pass: null key count");
+  {
+    // The keys are written to the scratch arrays first and then moved to keys / values.
+    const auto lo = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
+    const uintptr_t n8 = static_cast<uintptr_t>(key_capacity) * 8, n4 = static_cast<uintptr_t>(key_capacity) * 4;
+    const auto apart = [&](const void* a, uintptr_t na, const void* b, uintptr_t nb) {
+      return lo(a) + na <= lo(b) || lo(b) + nb <= lo(a);
+    };
+    MT_REQUIRE(key_capacity == 0 ||
+                   (keys && values && scratch_keys && scratch_values &&
+                    apart(keys, n8, scratch_keys, n8) && apart(keys, n8, scratch_values, n4) &&
+                    apart(values, n4, scratch_keys, n8) && apart(values, n4, scratch_values, n4)),
+               "mt_generate_keys_single_pass: keys, values and the two scratch arrays must be "
+               "separate device arrays of key_capacity entries");
+  }
   MT_REQUIRE(S >= 0 && S <= mt::kMaxPassShells, "mt_generate_keys_single_pass: %d shells (at most %d)",
              S, mt::kMaxPassShells);
   MT_REQUIRE(row_min >= 0 && col_min >= 0 && row_max <= 65535 && col_max <= 65535 &&
@@ -1042,24 +1106,33 @@ extern "C" int mt_generate_keys_single_pass(
   MT_REQUIRE(B <= mt::kPassFaces, "mt_generate_keys_single_pass: more than %d cameras", mt::kPassFaces);
   MT_REQUIRE(B <= 1 || (layout.camera_bits < 31 && (int64_t(1) << layout.camera_bits) >= B),
              "mt_generate_keys_single_pass: the key layout's camera field cannot tell %d cameras apart", B);
-  const int64_t blocks = mt::single_pass_blocks(F, B);
+  const int64_t units = mt::single_pass_blocks(F, B);
   const int lfpb = mt::single_pass_lfpb(B);
-  mt::ScanState state;
-  state.ticket = static_cast<uint32_t*>(workspace);
-  state.words = reinterpret_cast<uint64_t*>(static_cast<char*>(workspace) + 256);
-  mt::reset_scan_state<<<mt::blocks_for(blocks > 0 ? blocks : 1, 256), 256, 0, st>>>(
-      state.ticket, state.words, blocks);
-  if (blocks == 0) {
+  const mt::SinglePassState state = mt::single_pass_state(workspace, F, B);
+  if (units == 0) {
     mt::fill_i64_kernel<<<1, 64, 0, st>>>(n_keys, 1, 0);
     return mt::check_launch("mt_generate_keys_single_pass");
   }
-  MT_REQUIRE(blocks < (int64_t(1) << 31), "mt_generate_keys_single_pass: too many face blocks");
+  MT_REQUIRE(units < (int64_t(1) << 31) - 8, "mt_generate_keys_single_pass: too many face blocks");
+  // The key counter starts at zero on every call (a fill kernel, so the call can be captured).
+  mt::fill_i64_kernel<<<1, 64, 0, st>>>(reinterpret_cast<int64_t*>(state.counter), 1, 0);
   auto kernel = mt::pow2_tiles(tile_h, tile_w) ? mt::keys_single_pass_kernel<true>
                                                 : mt::keys_single_pass_kernel<false>;
-  kernel<<<static_cast<unsigned>(blocks), mt::kPassThreads, 0, st>>>(
+  const int64_t grid = (units + 7) / 8 * 8;  // every XCD the same number of blocks
+  kernel<<<static_cast<unsigned>(grid), mt::kPassThreads, 0, st>>>(
       xy, depths, V, faces, F, num_faces_x3, shell_face_boundaries, S, B, lfpb,
       mt::single_pass_slices(B, lfpb), tile_h, tile_w, row_min, col_min, row_max, col_max, near_plane, cam_dims, cull, grid_cols,
-      mt::to_fields(layout), keys, values, key_capacity, n_keys, overflow, state);
+      mt::to_fields(layout), scratch_keys, scratch_values, key_capacity, n_keys, overflow,
+      state.counter, state.unit_count, state.unit_at);
+  int rc = mt::check_launch("mt_generate_keys_single_pass");
+  if (rc != MT_OK) return rc;
+  rc = mt_inclusive_scan_i32_to_i64(state.unit_count, state.incl, units, state.scan,
+                                    state.scan_bytes, stream);
+  if (rc != MT_OK) return rc;
+  mt::reorder_keys_kernel<<<static_cast<unsigned>((units + mt::kReorderWaves - 1) / mt::kReorderWaves),
+                            mt::kReorderWaves * mt::kWave, 0, st>>>(
+      scratch_keys, scratch_values, state.unit_count, state.unit_at, state.incl, units,
+      key_capacity, keys, values, n_keys);
   return mt::check_launch("mt_generate_keys_single_pass");
 }
 

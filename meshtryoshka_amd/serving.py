@@ -410,7 +410,8 @@ def render_batch(scene: ResidentScene, extrinsics: torch.Tensor, intrinsics: tor
          f, live_x3.data_ptr(), scene.face_boundaries.data_ptr(), s, b, th, tw, 0, 0,
          max(d[2] for d in dims), max(d[3] for d in dims),
          float(near_plane), dims_p, 1, cols, _LAYOUT, keys.data_ptr(), vals.data_ptr(), k_cap,
-         n_dev.data_ptr(), overflow.data_ptr(), keys_ws.data_ptr(), keys_ws.numel(), st)
+         n_dev.data_ptr(), overflow.data_ptr(), skeys.data_ptr(), svals.data_ptr(),
+         keys_ws.data_ptr(), keys_ws.numel(), st)
     call("mt_sort_keys", keys.data_ptr(), vals.data_ptr(), k_cap, skeys.data_ptr(), svals.data_ptr(),
          sort_ws.data_ptr(), sort_ws.numel(), n_dev.data_ptr(), st)
     call("mt_delineate_tiles", skeys.data_ptr(), k_cap, _LAYOUT, 0, b, 1, rows * cols,

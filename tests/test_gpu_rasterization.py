@@ -315,11 +315,11 @@ def test_tile_shape_invariance(scene):
 
 @pytest.mark.parametrize("cull", [0, 1])
 def test_single_pass_keys_match_three_pass(cull):
-    """mt_generate_keys_single_pass (counts, look-back scan and keys in one kernel) writes
-    the same total as mt_key_block_counts + scan + mt_generate_keys_from_geometry and, per
-    camera, the same keys and values in the same order (cameras are interleaved by face
-    block), so that the stable sort gives identical results; with and without culling, for
-    several cameras (thousands of face blocks chained by look-back)."""
+    """mt_generate_keys_single_pass (each unit's keys counted and written into a reserved
+    range, then moved to unit order) writes the same total as mt_key_block_counts + scan +
+    mt_generate_keys_from_geometry and, per camera, the same keys and values in the same
+    order (cameras are interleaved by face block), so that the stable sort gives identical
+    results; with and without culling, for several cameras (thousands of units)."""
     dev = torch.device("cuda")
     spec = workloads.sphere_field(seed=4, num_spheres=10, resolution=192, num_cameras=5,
                                   image_shape=(150, 230), device=dev)
@@ -361,10 +361,13 @@ def test_single_pass_keys_match_three_pass(cull):
     vals2 = torch.full((k + 7,), -1, dtype=torch.int32, device=dev)
     n_keys = torch.zeros(1, dtype=torch.int64, device=dev)
     ws2 = _native.workspace(_native.query("mt_generate_keys_single_pass_workspace_bytes", f, b), dev)
+    scratch_k = torch.empty(k + 7, dtype=torch.int64, device=dev)
+    scratch_v = torch.empty(k + 7, dtype=torch.int32, device=dev)
     _native.call("mt_generate_keys_single_pass", xy.data_ptr(), depth.data_ptr(), v, faces.data_ptr(),
                  f, None, bounds.data_ptr(), bounds.numel(), b, th, tw, 0, 0, rows, cols, 0.2, None,
                  cull, cols, layout, keys2.data_ptr(), vals2.data_ptr(), k + 7, n_keys.data_ptr(),
-                 over.data_ptr(), ws2.data_ptr(), ws2.numel(), st)
+                 over.data_ptr(), scratch_k.data_ptr(), scratch_v.data_ptr(), ws2.data_ptr(),
+                 ws2.numel(), st)
     torch.cuda.synchronize()
     assert int(over) == 0
     assert int(n_keys) == k
@@ -377,15 +380,28 @@ def test_single_pass_keys_match_three_pass(cull):
     order, order2 = torch.sort(keys, stable=True).indices, torch.sort(keys2[:k], stable=True).indices
     assert torch.equal(keys[order], keys2[:k][order2]) and torch.equal(vals[order], vals2[:k][order2])
     assert bool((keys2[k:] == -1).all())
-    # Too small a capacity: the count is still exact, the overflow flag is raised and
-    # nothing is written past the capacity.
+    # Too small a capacity: the count is still exact, the overflow flag is raised, nothing
+    # is written past the capacity, and whatever is written below it is the right key.
     small = torch.full((k // 2 + 1,), -1, dtype=torch.int64, device=dev)
     small_v = torch.full((k // 2 + 1,), -1, dtype=torch.int32, device=dev)
+    scratch_small_k = torch.full((k // 2 + 1,), -1, dtype=torch.int64, device=dev)
+    scratch_small_v = torch.full((k // 2 + 1,), -1, dtype=torch.int32, device=dev)
     _native.call("mt_generate_keys_single_pass", xy.data_ptr(), depth.data_ptr(), v, faces.data_ptr(),
                  f, None, bounds.data_ptr(), bounds.numel(), b, th, tw, 0, 0, rows, cols, 0.2, None,
                  cull, cols, layout, small.data_ptr(), small_v.data_ptr(), k // 2, n_keys.data_ptr(),
-                 over.data_ptr(), ws2.data_ptr(), ws2.numel(), st)
+                 over.data_ptr(), scratch_small_k.data_ptr(), scratch_small_v.data_ptr(),
+                 ws2.data_ptr(), ws2.numel(), st)
     torch.cuda.synchronize()
     assert int(over) == 1 and int(n_keys) == k
-    assert torch.equal(small[: k // 2], keys2[: k // 2])
-    assert int(small[k // 2]) == -1
+    written = small[: k // 2] != -1
+    assert bool(written.any())
+    assert torch.equal(small[: k // 2][written], keys2[: k // 2][written])
+    assert torch.equal(small_v[: k // 2][written], vals2[: k // 2][written])
+    assert int(small[k // 2]) == -1 and int(scratch_small_k[k // 2]) == -1
+    # Scratch arrays that overlap the outputs are refused.
+    with pytest.raises(RuntimeError, match="separate device arrays"):
+        _native.call("mt_generate_keys_single_pass", xy.data_ptr(), depth.data_ptr(), v,
+                     faces.data_ptr(), f, None, bounds.data_ptr(), bounds.numel(), b, th, tw, 0, 0,
+                     rows, cols, 0.2, None, cull, cols, layout, keys2.data_ptr(), vals2.data_ptr(),
+                     k + 7, n_keys.data_ptr(), over.data_ptr(), keys2.data_ptr(), scratch_v.data_ptr(),
+                     ws2.data_ptr(), ws2.numel(), st)
