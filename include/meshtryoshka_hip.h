@@ -161,14 +161,17 @@ int mt_create_triangle_vertices_backward(
  * the vertices only, 2 = the colours only (the vertices are recomputed, not written), 3 =
  * both; split, the colours can run on another stream beside the work that needs only the
  * vertices. Adding 4 to parts reads grid_sh as SH rows (mt_pack_sh_rows' layout, 16-byte
- * aligned) instead of [num_sh, num_grid_vertices, 3]: the colours are the same bits. */
+ * aligned) instead of [num_sh, num_grid_vertices, 3]: the colours are the same bits.
+ * colors_stride 0 writes out_colors as f32[B, num_triangle_vertices, 3]; s >= B writes
+ * f32[num_triangle_vertices, s, 3] (a vertex's cameras together: one contiguous record per
+ * vertex, which the batched step's colour adjoint reads once). */
 int mt_create_triangle_vertices_colors_forward(
     const float* grid_vertices, const float* grid_sdf, const float* grid_sh,
     int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree, const int32_t* lower_corners,
     int64_t num_columns, const float* level_sets, int32_t S, const int32_t* vertex_offsets,
     int64_t num_triangle_vertices, const float* camera_positions, int32_t B, float* out_vertices,
     float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, int32_t parts,
-    mt_stream_t stream);
+    int32_t colors_stride, mt_stream_t stream);
 
 /* Floats per SH row for an SH degree (0..3; -1 otherwise): 3 (deg+1)^2 rounded up to a
  * multiple of 4. No counterpart in the reference (a layout of this library). */
@@ -207,7 +210,8 @@ size_t mt_create_triangle_vertices_colors_backward_workspace_bytes(int64_t num_g
  * zero between steps needs no zero-fill (no counterpart in the reference). maps_ready = 1
  * reuses the grid-vertex maps (the first part of the workspace) that an earlier call built
  * over the same lower_corners, num_columns and num_grid_vertices, with the workspace not
- * written since except by such calls; 0 rebuilds them. */
+ * written since except by such calls; 0 rebuilds them. colors_stride: the layout of colors
+ * and grad_colors, as in the forward (0: f32[B, V, 3]; s >= B: f32[V, s, 3]). */
 int mt_create_triangle_vertices_colors_backward(
     const float* grid_sdf, int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree,
     const int32_t* lower_corners, int64_t num_columns, const float* level_sets, int32_t S,
@@ -215,7 +219,8 @@ int mt_create_triangle_vertices_colors_backward(
     const float* colors, const float* camera_positions, int32_t B, float* grad_sdf,
     float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
     size_t workspace_bytes, int32_t accumulate, const uint8_t* vertex_touched,
-    int32_t clear_gradients, int32_t maps_ready, int32_t* status, mt_stream_t stream);
+    int32_t clear_gradients, int32_t maps_ready, int32_t* status, int32_t colors_stride,
+    mt_stream_t stream);
 
 /* SDF regularizers (replace compute_sdf_regularizers.slang:18-128, .py:26-139). Per voxel:
  * the Eikonal loss at the voxel centre, (|(voxel_x dx, voxel_y dy, voxel_z dz)| - 1)^2 with
@@ -413,7 +418,10 @@ int mt_generate_keys_from_geometry(const float* xy, const float* depths, int64_t
  * int32, so 1024 x (row_max - row_min) x (col_max - col_min) must stay below 2^31; with
  * cam_dims, row_max and col_max (otherwise unused) bound every camera's grid for this
  * check. *n_keys (device) = the total key count (also when it exceeds key_capacity, which
- * sets *overflow). scratch_keys / scratch_values: key_capacity entries each, separate from
+ * sets *overflow; the keys below the capacity are then each a generated key or key 0 with
+ * value 0, so that the sort, the delineation and the composites stay within their arrays
+ * while the caller discards the result). scratch_keys / scratch_values: key_capacity
+ * entries each, separate from
  * keys and values (their contents are overwritten). Workspace:
  * mt_generate_keys_single_pass_workspace_bytes(F, B). */
 size_t mt_generate_keys_single_pass_workspace_bytes(int64_t F, int32_t B);
@@ -472,7 +480,7 @@ int mt_evaluate_sh_backward(const float* vertices, int64_t V, const float* sh, i
 /* Arguments shared by the fused intersect-and-composite kernels. */
 typedef struct mt_composite_args {
   const float* xy;            /* f32[B, V, 2] projected vertices */
-  const float* colors;        /* f32[B, V, 3] */
+  const float* colors;        /* f32[B, V, 3] (or f32[V, colors_stride, 3]) */
   const float* sdf;           /* f32[V] */
   int64_t num_vertices;
   const int32_t* faces;       /* int32[F, 3] */
@@ -500,6 +508,9 @@ typedef struct mt_composite_args {
   /* Optional (may be NULL), backward only: uint8[V], set to 1 for every vertex that
    * receives a gradient (the caller zeroes it); lets the colour adjoint skip the others. */
   uint8_t* vertex_touched;
+  /* Layout of colors and colors_grad: 0 = f32[B, V, 3] (camera-major, the reference's);
+   * s >= B = f32[V, s, 3] (each vertex's cameras together, the batched step's layout). */
+  int32_t colors_stride;
 } mt_composite_args;
 
 /* out f32[B, 4, H, W] (rgb, transmittance). Replaces intersect_and_composite_forward

@@ -65,6 +65,8 @@ class CompositeArgs(ctypes.Structure):
         ("pixel_base", c_p),
         ("total_tiles", c_i64),
         ("vertex_touched", c_p),
+        # 0: colours f32[B, V, 3]; s > 0: f32[V, s, 3] (the batched step's layout).
+        ("colors_stride", c_i32),
     ]
 
 
@@ -97,7 +99,7 @@ SIGNATURES = {
     "mt_create_triangle_vertices_colors_forward": (
         c_i32,
         [c_p, c_p, c_p, c_i64, c_i32, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_i32, c_p,
-         c_p, c_p, c_p, c_p, c_i32, c_p],
+         c_p, c_p, c_p, c_p, c_i32, c_i32, c_p],
     ),
     "mt_sh_row_floats": (c_i64, [c_i32]),
     "mt_pack_sh_rows": (c_i32, [c_p, c_i64, c_i32, c_i32, c_p, c_p]),
@@ -105,7 +107,7 @@ SIGNATURES = {
     "mt_create_triangle_vertices_colors_backward": (
         c_i32,
         [c_p, c_i64, c_i32, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_i32, c_p,
-         c_p, c_p, c_p, c_p, c_sz, c_i32, c_p, c_i32, c_i32, c_p, c_p],
+         c_p, c_p, c_p, c_p, c_sz, c_i32, c_p, c_i32, c_i32, c_p, c_i32, c_p],
     ),
     "mt_count_triangle_faces": (c_i32, [c_p, c_p, c_i64, c_p, c_i64, c_p, c_i32, c_p, c_p, c_p]),
     "mt_write_occupancy": (c_i32, [c_p, c_i64, c_p, c_i64, c_i32, c_i32, c_p]),

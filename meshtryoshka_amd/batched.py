@@ -158,7 +158,9 @@ class BatchedTrainingStep:
         self.sorted_values = torch.zeros(K, **i32)
         self.ranges = torch.zeros((max(g["tiles"] for g in self.groups), 2), **i64)
         self.campos = torch.zeros((self.B, 3), **f32)
-        self.colors = torch.zeros((G, V, 3), **f32)
+        # Vertex-major colours [V, G, 3]: a vertex's cameras are one contiguous record, which
+        # the colour adjoint reads (and whose gradients it reads and clears) at once.
+        self.colors = torch.zeros((V, G, 3), **f32)
         # Each grid vertex's SH coefficients as one contiguous row, repacked every step.
         row = _native.query("mt_sh_row_floats", self.sh_degree)
         self.sh_rows = torch.empty((self.n_grid, row), **f32)
@@ -223,6 +225,7 @@ class BatchedTrainingStep:
             g["cam_dims"].data_ptr() if table else None,
             g["tile_base"].data_ptr() if table else None,
             g["pixel_base"].data_ptr() if table else None, g["tiles"] if table else 0,
+            None, self.G,
         )
 
     def run(self, grid_vertices: torch.Tensor, grid_sdf: torch.Tensor, grid_sh: torch.Tensor,
@@ -286,7 +289,7 @@ class BatchedTrainingStep:
                  self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V,
                  self.campos[c0:].data_ptr(), n, self.tv.data_ptr(), self.td.data_ptr(),
                  self.ttype.data_ptr(), self.colors.data_ptr(), self.overflow.data_ptr(),
-                 (3 if gi == 0 else 2) | 4, st)
+                 (3 if gi == 0 else 2) | 4, self.G, st)
             # Faces (first group only) and the group's projection both need only the vertices:
             # without per-launch timing they run side by side on two streams.
             overlap = gi == 0 and hook is None
@@ -353,7 +356,7 @@ class BatchedTrainingStep:
                  self.grid_sdf_grad.data_ptr(), self.grid_sh_grad.data_ptr(),
                  self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(), 1, self.touched.data_ptr(), 1,
                  int(self.maps_built),
-                 self.overflow.data_ptr(), st)
+                 self.overflow.data_ptr(), self.G, st)
             # The adjoint's grid-vertex maps stay in its workspace (they depend only on the
             # columns): every later call reuses them, captured replays included.
             self.maps_built = True

@@ -324,15 +324,26 @@ struct Sample {
   int64_t v[3];
 };
 
+// Camera cam's part of a colour array (colours or their gradients): its first float and the
+// floats between consecutive vertices. colors_stride 0: f32[B, V, 3] (camera-major);
+// s > 0: f32[V, s, 3] (each vertex's cameras together).
+__device__ __forceinline__ int64_t camera_colors_offset(int stride, int cam, int64_t V) {
+  return stride > 0 ? 3 * int64_t(cam) : 3 * int64_t(cam) * V;
+}
+__device__ __forceinline__ int64_t vertex_colors_step(int stride) {
+  return stride > 0 ? 3 * int64_t(stride) : 3;
+}
+
 __device__ __forceinline__ Sample interpolate_hit(const mt_composite_args& a, const float* col,
                                                   int32_t face, const Bary& bc) {
   Sample s;
   s.v[0] = a.faces[3 * int64_t(face) + 0];
   s.v[1] = a.faces[3 * int64_t(face) + 1];
   s.v[2] = a.faces[3 * int64_t(face) + 2];
-  const float* ca = col + 3 * s.v[0];
-  const float* cb = col + 3 * s.v[1];
-  const float* cc = col + 3 * s.v[2];
+  const int64_t vs = vertex_colors_step(a.colors_stride);
+  const float* ca = col + vs * s.v[0];
+  const float* cb = col + vs * s.v[1];
+  const float* cc = col + vs * s.v[2];
   s.sdf = (bc.u * a.sdf[s.v[0]] + bc.v * a.sdf[s.v[1]]) + bc.w * a.sdf[s.v[2]];
   s.r = (bc.u * ca[0] + bc.v * cb[0]) + bc.w * cc[0];
   s.g = (bc.u * ca[1] + bc.v * cb[1]) + bc.w * cc[1];
@@ -354,7 +365,7 @@ __global__ void __launch_bounds__(512) fused_forward_kernel(mt_composite_args a,
   const float k = a.sharpness[0];
   const float px = float(tc.x) + 0.5f, py = float(tc.y) + 0.5f;
   const float2* xy = reinterpret_cast<const float2*>(a.xy) + int64_t(tc.cam) * a.num_vertices;
-  const float* col = a.colors + int64_t(tc.cam) * a.num_vertices * 3;
+  const float* col = a.colors + camera_colors_offset(a.colors_stride, tc.cam, a.num_vertices);
 
   float r = 0.f, g = 0.f, bl = 0.f, trans = 1.f;
   float outer_cdf = sdf_to_cdf(kDummySdf, k);
@@ -423,13 +434,14 @@ __device__ __forceinline__ void scatter_sample(const mt_composite_args& a, int c
                                                float dr, float dg, float db, float dsdf,
                                                float* colors_grad, float* sdf_grad) {
   const float wgt[3] = {bu, bv, bw};
-  float* cgrad = colors_grad + int64_t(cam) * a.num_vertices * 3;
+  float* cgrad = colors_grad + camera_colors_offset(a.colors_stride, cam, a.num_vertices);
+  const int64_t vs = vertex_colors_step(a.colors_stride);
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     if (a.vertex_touched) a.vertex_touched[v[q]] = 1;
-    atomicAdd(cgrad + 3 * v[q] + 0, wgt[q] * dr);
-    atomicAdd(cgrad + 3 * v[q] + 1, wgt[q] * dg);
-    atomicAdd(cgrad + 3 * v[q] + 2, wgt[q] * db);
+    atomicAdd(cgrad + vs * v[q] + 0, wgt[q] * dr);
+    atomicAdd(cgrad + vs * v[q] + 1, wgt[q] * dg);
+    atomicAdd(cgrad + vs * v[q] + 2, wgt[q] * db);
     atomicAdd(sdf_grad + v[q], wgt[q] * dsdf);
   }
 }
@@ -459,9 +471,10 @@ __device__ __forceinline__ void scatter_sample_lds(float* acc, float bu, float b
 // the next chunk (the sums start at zero and only hit triangles' sums become nonzero).
 __device__ __forceinline__ void flush_chunk_grads(const ChunkLds& L, float* acc, int count,
                                                   const int32_t* faces, int64_t num_vertices,
-                                                  int cam, float* colors_grad, float* sdf_grad,
-                                                  uint8_t* touched) {
-  float* cgrad = colors_grad + int64_t(cam) * num_vertices * 3;
+                                                  int cam, int colors_stride, float* colors_grad,
+                                                  float* sdf_grad, uint8_t* touched) {
+  float* cgrad = colors_grad + camera_colors_offset(colors_stride, cam, num_vertices);
+  const int64_t vs = vertex_colors_step(colors_stride);
   for (int j = threadIdx.x; j < count; j += blockDim.x) {
     float* g = acc + kGradSlots * j;
     float e[kGradSlots];
@@ -479,7 +492,7 @@ __device__ __forceinline__ void flush_chunk_grads(const ChunkLds& L, float* acc,
       if (touched) touched[v] = 1;
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        if (e[4 * q + c] != 0.0f) atomicAdd(cgrad + 3 * v + c, e[4 * q + c]);
+        if (e[4 * q + c] != 0.0f) atomicAdd(cgrad + vs * v + c, e[4 * q + c]);
       if (e[4 * q + 3] != 0.0f) atomicAdd(sdf_grad + v, e[4 * q + 3]);
     }
 #pragma unroll
@@ -510,7 +523,7 @@ __global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a
   const float k = a.sharpness[0];
   const float px = float(tc.x) + 0.5f, py = float(tc.y) + 0.5f;
   const float2* xy = reinterpret_cast<const float2*>(a.xy) + int64_t(tc.cam) * a.num_vertices;
-  const float* col = a.colors + int64_t(tc.cam) * a.num_vertices * 3;
+  const float* col = a.colors + camera_colors_offset(a.colors_stride, tc.cam, a.num_vertices);
 
   float fr = 0.f, fg = 0.f, fb = 0.f, ft = 0.f;    // final accumulator
   float gr = 0.f, gg = 0.f, gb = 0.f, gt = 0.f;    // its gradient
@@ -625,8 +638,8 @@ __global__ void __launch_bounds__(512) fused_backward_kernel(mt_composite_args a
       L.live[tc.pix] = active ? 1 : 0;
     }
     const bool more = __syncthreads_or(active);  // also: every LDS sum of the chunk is in
-    flush_chunk_grads(L, gacc, count, a.faces, a.num_vertices, tc.cam, colors_grad, sdf_grad,
-                      a.vertex_touched);
+    flush_chunk_grads(L, gacc, count, a.faces, a.num_vertices, tc.cam, a.colors_stride,
+                      colors_grad, sdf_grad, a.vertex_touched);
     if (!more) break;
   }
   if (pending) {  // its triangle's chunk has been flushed
@@ -915,6 +928,8 @@ int check_composite_args(const mt_composite_args* a, const char* name) {
              kMaxTilePixels);
   MT_REQUIRE(a->chunk >= 0 && a->chunk <= kMaxChunk, "%s: chunk must be in [0, %d]", name, kMaxChunk);
   MT_REQUIRE(a->num_cameras >= 0 && a->grid_rows >= 0 && a->grid_cols >= 0, "%s: bad grid", name);
+  MT_REQUIRE(a->colors_stride == 0 || a->colors_stride >= a->num_cameras,
+             "%s: colors_stride must be 0 or at least num_cameras", name);
   MT_REQUIRE(int64_t(a->num_cameras) * a->grid_rows * a->grid_cols < (int64_t(1) << 31),
              "%s: too many tiles", name);
   MT_REQUIRE((a->cam_dims == nullptr) == (a->tile_base == nullptr) &&

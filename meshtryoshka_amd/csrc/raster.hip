@@ -788,8 +788,11 @@ __global__ void fill_i64_kernel(int64_t* p, int64_t n, int64_t value) {
 
 // One wave per unit: moves the unit's keys from the range it reserved (src) to its place in
 // unit order (dst, after the keys of every earlier unit: incl is the inclusive scan of the
-// unit counts). Entries past the capacity were never written (the key kernel raised the
-// overflow flag) and are skipped. The last unit's wave stores the total key count.
+// unit counts). The last unit's wave stores the total key count. When the total exceeds the
+// capacity (the key kernel raised the overflow flag), a unit's keys whose reserved range
+// fell past the capacity were never written: their places below the capacity get key 0
+// with value 0 (camera 0, tile 0, shell 0, face 0), so that every key below the capacity
+// indexes valid tile ranges and faces in the stages after the sort.
 constexpr int kReorderWaves = 4;
 __global__ void __launch_bounds__(kReorderWaves * kWave) reorder_keys_kernel(
     const int64_t* src_keys, const int32_t* src_values, const int32_t* unit_count,
@@ -801,9 +804,10 @@ __global__ void __launch_bounds__(kReorderWaves * kWave) reorder_keys_kernel(
   const int32_t n = unit_count[u];
   const int64_t from = unit_at[u], to = incl[u] - n;
   for (int32_t j = lane; j < n; j += kWave) {
-    if (from + j < kcap && to + j < kcap) {
-      keys[to + j] = src_keys[from + j];
-      values[to + j] = src_values[from + j];
+    if (to + j < kcap) {
+      const bool kept = from + j < kcap;
+      keys[to + j] = kept ? src_keys[from + j] : 0;
+      values[to + j] = kept ? src_values[from + j] : 0;
     }
   }
   if (u == units - 1 && lane == 0 && n_keys) *n_keys = incl[u];

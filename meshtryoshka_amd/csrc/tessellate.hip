@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
     const float* gv, const float* gsdf, const float* gsh, int64_t ng, const int32_t* lower,
     int64_t n, const float* level_sets, int S, const int32_t* offsets, int64_t nt,
     const float* cams, int B, float* ov, float* od, uint8_t* otype, float* colors,
-    int32_t* overflow, int parts) {
+    int32_t* overflow, int parts, int cstride) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 3 * kN + 1;  // padded: consecutive rows start in different banks
   constexpr int kR = sh_row_floats(kDeg);
@@ -479,13 +479,15 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
           }
         }
       }
-      for (int b = 0; b < B; ++b) {
+      // Camera b's colour at (b nt + idx) 3 (cstride 0) or (idx cstride + b) 3.
+      float* cout = colors + 3 * (cstride > 0 ? idx * cstride : idx);
+      const int64_t cstep = cstride > 0 ? 3 : 3 * nt;
+      for (int b = 0; b < B; ++b, cout += cstep) {
         const float3 d = view_dir(p, cams + 3 * b);
         float basis[16];
         sh_basis(d.x, d.y, d.z, kDeg, basis);
 #pragma unroll
-        for (int ch = 0; ch < 3; ++ch)
-          colors[(int64_t(b) * nt + idx) * 3 + ch] = fmaxf(sh_channel_pre<kN>(basis, coef, ch), 0.0f);
+        for (int ch = 0; ch < 3; ++ch) cout[ch] = fmaxf(sh_channel_pre<kN>(basis, coef, ch), 0.0f);
       }
     }
     __syncthreads();  // the next level set reuses the lists
@@ -634,7 +636,7 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
     int S, const int32_t* offsets, int64_t nt, const float* tv, const float* colors,
     const float* cams, int B, float* g_d, float* g_col, float* part, const int32_t* list,
     const int32_t* count, const uint8_t* col_live, const uint32_t* col_slots, const int32_t* maps,
-    const uint8_t* touched, bool clear) {
+    const uint8_t* touched, bool clear, int cstride) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= *count) return;
@@ -669,9 +671,11 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_columns_kernel(
       acc0_d += w0 * gd;
       acce_d += t * gd;
       const float3 p = vertex_at(tv, here);
-      const float* cb = colors + 3 * here;  // camera b's colour of this vertex: cb + b * 3 nt
-      float* gb = g_col + 3 * here;
-      const int64_t stride = 3 * nt;
+      // Camera b's colour of this vertex: at (b nt + here) 3 (cstride 0) or (here cstride + b) 3.
+      const int64_t first = 3 * (cstride > 0 ? here * cstride : here);
+      const float* cb = colors + first;
+      float* gb = g_col + first;
+      const int64_t stride = cstride > 0 ? 3 : 3 * nt;
       for (int b = 0; b < B; ++b, cb += stride, gb += stride) {
         // The view direction with the hardware reciprocal square root (~1 ulp): the
         // gradient tolerates it, and it saves the correctly rounded sqrt and division that
@@ -1092,9 +1096,11 @@ extern "C" int mt_create_triangle_vertices_colors_forward(
     int64_t num_columns, const float* level_sets, int32_t S, const int32_t* vertex_offsets,
     int64_t num_triangle_vertices, const float* camera_positions, int32_t B, float* out_vertices,
     float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, int32_t parts,
-    mt_stream_t stream) {
+    int32_t colors_stride, mt_stream_t stream) {
   MT_REQUIRE((parts & 3) != 0 && (parts & ~7) == 0,
              "mt_create_triangle_vertices_colors_forward: parts %d", parts);
+  MT_REQUIRE(colors_stride == 0 || colors_stride >= B,
+             "mt_create_triangle_vertices_colors_forward: colors_stride must be 0 or at least B");
   MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0 && B >= 0,
              "mt_create_triangle_vertices_colors_forward: negative size");
   MT_REQUIRE(sh_degree >= 0 && sh_degree <= 3 && (sh_degree + 1) * (sh_degree + 1) <= num_sh,
@@ -1112,12 +1118,12 @@ extern "C" int mt_create_triangle_vertices_colors_forward(
       mt::create_vertices_colors_kernel<D, true><<<grid, mt::kVcBlock, 0, st>>>(                \
           grid_vertices, grid_sdf, grid_sh, num_grid_vertices, lower_corners, num_columns,     \
           level_sets, S, vertex_offsets, num_triangle_vertices, camera_positions, B,           \
-          out_vertices, out_sdf, out_types, out_colors, overflow, parts);                      \
+          out_vertices, out_sdf, out_types, out_colors, overflow, parts, colors_stride);       \
     else                                                                                        \
       mt::create_vertices_colors_kernel<D, false><<<grid, mt::kVcBlock, 0, st>>>(               \
           grid_vertices, grid_sdf, grid_sh, num_grid_vertices, lower_corners, num_columns,     \
           level_sets, S, vertex_offsets, num_triangle_vertices, camera_positions, B,           \
-          out_vertices, out_sdf, out_types, out_colors, overflow, parts);                      \
+          out_vertices, out_sdf, out_types, out_colors, overflow, parts, colors_stride);       \
   } while (0)
   // Without colours the degree-0 instance stages no SH rows (less LDS, more blocks per CU).
   switch ((parts & 2) != 0 && B > 0 ? sh_degree : 0) {
@@ -1173,9 +1179,12 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
     const float* colors, const float* camera_positions, int32_t B, float* grad_sdf,
     float* grad_colors, float* grid_sdf_grad, float* grid_sh_grad, void* workspace,
     size_t workspace_bytes, int32_t accumulate, const uint8_t* vertex_touched,
-    int32_t clear_gradients, int32_t maps_ready, int32_t* status, mt_stream_t stream) {
+    int32_t clear_gradients, int32_t maps_ready, int32_t* status, int32_t colors_stride,
+    mt_stream_t stream) {
   MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0 && B >= 0,
              "mt_create_triangle_vertices_colors_backward: negative size");
+  MT_REQUIRE(colors_stride == 0 || colors_stride >= B,
+             "mt_create_triangle_vertices_colors_backward: colors_stride must be 0 or at least B");
   MT_REQUIRE(num_sh == 1 || num_sh == 4 || num_sh == 9 || num_sh == 16,
              "mt_create_triangle_vertices_colors_backward: %d SH coefficients (need 1, 4, 9 or 16)",
              num_sh);
@@ -1239,7 +1248,7 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
           grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S,                \
           vertex_offsets, num_triangle_vertices, vertices, colors, camera_positions, B, grad_sdf, \
           grad_colors, part, list, live_count, col_live, col_slots, maps, vertex_touched,       \
-          clear_gradients != 0);                                                                \
+          clear_gradients != 0, colors_stride);                                                 \
     mt::colors_adjoint_ends_kernel<N, D><<<ggrid, mt::kBlock, 0, st>>>(                          \
         num_grid_vertices, num_columns, part, col_live, col_pos, grid_sdf_grad, grid_sh_grad,     \
         maps, accumulate != 0, status, accumulate != 0 ? ends_list : nullptr, ends_count);        \

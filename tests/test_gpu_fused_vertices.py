@@ -71,7 +71,7 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
     _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(), sdf.data_ptr(),
                  sh.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(), n, ls.data_ptr(), s,
                  off.data_ptr(), V, campos.data_ptr(), b, tv2.data_ptr(), td2.data_ptr(),
-                 tt2.data_ptr(), col2.data_ptr(), ov.data_ptr(), 3, _stream())
+                 tt2.data_ptr(), col2.data_ptr(), ov.data_ptr(), 3, 0, _stream())
     assert int(ov) == 0
     assert torch.equal(tv, tv2) and torch.equal(td, td2) and torch.equal(tt, tt2)
     assert torch.equal(col, col2)
@@ -83,14 +83,14 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
     _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(), sdf.data_ptr(),
                  sh.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(), n, ls.data_ptr(), s,
                  off.data_ptr(), V, campos.data_ptr(), b, tv3.data_ptr(), td3.data_ptr(),
-                 tt3.data_ptr(), col3.data_ptr(), ov.data_ptr(), 1, _stream())
+                 tt3.data_ptr(), col3.data_ptr(), ov.data_ptr(), 1, 0, _stream())
     assert torch.equal(tv3, tv) and torch.equal(td3, td) and torch.equal(tt3, tt)
     assert bool((col3 == 7.0).all())
     tv4 = torch.full((V, 3), 7.0, **f32)
     _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(), sdf.data_ptr(),
                  sh.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(), n, ls.data_ptr(), s,
                  off.data_ptr(), V, campos.data_ptr(), b, tv4.data_ptr(), td3.data_ptr(),
-                 tt3.data_ptr(), col3.data_ptr(), ov.data_ptr(), 2, _stream())
+                 tt3.data_ptr(), col3.data_ptr(), ov.data_ptr(), 2, 0, _stream())
     assert torch.equal(col3, col) and bool((tv4 == 7.0).all())
     # SH rows: a permutation of grid_sh's first (deg+1)^2 planes, zero-padded; the colour
     # pass reading them gives the same bits.
@@ -109,12 +109,28 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
                      sdf.data_ptr(), rows.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(),
                      n, ls.data_ptr(), s, off.data_ptr(), V, campos.data_ptr(), b, tv5.data_ptr(),
                      td5.data_ptr(), tt5.data_ptr(), col5.data_ptr(), ov.data_ptr(), parts,
-                     _stream())
+                     0, _stream())
         assert torch.equal(col5, col)
         if parts & 1:
             assert torch.equal(tv5, tv) and torch.equal(td5, td) and torch.equal(tt5, tt)
         else:
             assert bool((tv5 == 7.0).all())
+
+    # Vertex-major colours (colors_stride >= b): the same values, each vertex's cameras
+    # together; a larger stride leaves the unused camera slots untouched.
+    for stride in (b, b + 2):
+        col6 = torch.full((V, stride, 3), 7.0, **f32)
+        _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(),
+                     sdf.data_ptr(), rows.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(),
+                     n, ls.data_ptr(), s, off.data_ptr(), V, campos.data_ptr(), b, None, None,
+                     None, col6.data_ptr(), ov.data_ptr(), 2 | 4, stride, _stream())
+        assert torch.equal(col6[:, :b].permute(1, 0, 2), col)
+        assert bool((col6[:, b:] == 7.0).all())
+    with pytest.raises(RuntimeError, match="colors_stride"):
+        _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(),
+                     sdf.data_ptr(), rows.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(),
+                     n, ls.data_ptr(), s, off.data_ptr(), V, campos.data_ptr(), b, None, None,
+                     None, col3.data_ptr(), ov.data_ptr(), 2 | 4, b - 1, _stream())
 
     # Adjoint: random upstream gradients for SDF and colours.
     gen = torch.Generator("cpu").manual_seed(11)
@@ -138,7 +154,7 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
                  vox.lower_corners.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V, tv2.data_ptr(),
                  col2.data_ptr(), campos.data_ptr(), b, g_d.data_ptr(), g_col.data_ptr(),
                  gg_d2.data_ptr(), gg_sh2.data_ptr(), ws2.data_ptr(), ws2.numel(), 0, None, 0, 0,
-                 status.data_ptr(), _stream())
+                 status.data_ptr(), 0, _stream())
     assert int(status) == 0
     # Same terms, summed per column and axis first: equal up to float summation order.
     torch.testing.assert_close(gg_d2, gg_d, rtol=1e-5, atol=1e-6 * float(gg_d.abs().max()))
@@ -161,7 +177,7 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
                      tv2.data_ptr(), col2.data_ptr(), campos.data_ptr(), b, gd_in.data_ptr(),
                      gc_in.data_ptr(), gd_o.data_ptr(), gs_o.data_ptr(), ws2.data_ptr(),
                      ws2.numel(), 0, None if touched is None else touched.data_ptr(), clear,
-                     int(touched is not None), status.data_ptr(), _stream())
+                     int(touched is not None), status.data_ptr(), 0, _stream())
         outs.append((gd_o, gs_o))
         if clear:
             assert not bool(gd_in.any()) and not bool(gc_in.any())
@@ -170,6 +186,24 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
     assert int(status) == 0
     for o in outs[1:]:
         assert torch.equal(outs[0][0], o[0]) and torch.equal(outs[0][1], o[1])
+    # The same flagged, clearing call on vertex-major colours and colour gradients (stride
+    # b + 1): the same sums, bit for bit, and the gradients left zero.
+    stride = b + 1
+    col_vm = torch.zeros((V, stride, 3), **f32)
+    col_vm[:, :b] = col2.permute(1, 0, 2)
+    gc_vm = torch.full((V, stride, 3), 0.0, **f32)
+    gc_vm[:, :b] = g_col_s.permute(1, 0, 2)
+    gd_in = g_d_s.clone()
+    gd_o = torch.full((ng,), float("nan"), **f32)
+    gs_o = torch.full((nsh, ng, 3), float("nan"), **f32)
+    _native.call("mt_create_triangle_vertices_colors_backward", sdf.data_ptr(), ng, nsh, deg,
+                 vox.lower_corners.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V,
+                 tv2.data_ptr(), col_vm.data_ptr(), campos.data_ptr(), b, gd_in.data_ptr(),
+                 gc_vm.data_ptr(), gd_o.data_ptr(), gs_o.data_ptr(), ws2.data_ptr(), ws2.numel(), 0,
+                 flags.to(torch.uint8).data_ptr(), 1, 1, status.data_ptr(), stride, _stream())
+    assert int(status) == 0
+    assert torch.equal(gd_o, outs[0][0]) and torch.equal(gs_o, outs[0][1])
+    assert not bool(gd_in.any()) and not bool(gc_vm.any())
 
     # Accumulating mode adds to what the grid gradients hold: splitting the cameras into
     # two calls (the second accumulating) gives the one-call result.
@@ -181,7 +215,7 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
                      vox.lower_corners.data_ptr(), n, ls.data_ptr(), s, off.data_ptr(), V,
                      tv2.data_ptr(), col2[c0:].data_ptr(), campos[c0:].data_ptr(), c1 - c0,
                      gd_part.data_ptr(), g_col[c0:].data_ptr(), gg_d3.data_ptr(), gg_sh3.data_ptr(),
-                     ws2.data_ptr(), ws2.numel(), i, None, 0, i, status.data_ptr(), _stream())
+                     ws2.data_ptr(), ws2.numel(), i, None, 0, i, status.data_ptr(), 0, _stream())
     assert int(status) == 0
     torch.testing.assert_close(gg_d3, gg_d2, rtol=1e-5, atol=1e-6 * float(gg_d.abs().max()))
     torch.testing.assert_close(gg_sh3, gg_sh2, rtol=1e-5, atol=1e-6 * float(gg_sh.abs().max()))
@@ -250,7 +284,7 @@ def test_fused_adjoint_with_shared_origins(case):
                      g_col[c0:].contiguous().data_ptr(), gd_out.data_ptr(), gs_out.data_ptr(),
                      ws2.data_ptr(), ws2.numel(), accumulate,
                      None if touched is None else touched.data_ptr(), 0, maps_ready,
-                     status.data_ptr(), _stream())
+                     status.data_ptr(), 0, _stream())
 
     def close(got_d, got_sh):
         torch.testing.assert_close(got_d, want_d, rtol=1e-5, atol=1e-6 * float(want_d.abs().max()))
