@@ -392,7 +392,9 @@ def profile_traffic(entry_point, key):
     with open(path) as f:
         for line in f:
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if cells and cells[0] == entry_point and len(cells) >= 6:
+            # Rows are named by the entry point, possibly followed by a note in parentheses.
+            if cells and (cells[0] == entry_point or cells[0].startswith(entry_point + " (")) \
+                    and len(cells) >= 6:
                 try:
                     return int(float(cells[5]) * 2**30)  # "HBM GiB per launch" column
                 except ValueError:
