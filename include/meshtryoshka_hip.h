@@ -121,6 +121,15 @@ int mt_count_triangle_vertices(const float* sdf, const int32_t* lower_corners,
                                int64_t num_columns, const float* level_sets, int32_t S,
                                int32_t* counts, mt_stream_t stream);
 
+/* mt_count_triangle_vertices that also writes axes uint8[S * num_columns]: bit a set when
+ * the column's axis-a edge crosses the level set (the column then has a vertex of type a,
+ * at its first vertex plus the set bits below a). Batched training step; no counterpart in
+ * the reference (its create_triangle_faces.slang:3-78 searches the vertex types instead):
+ * mt_create_triangle_faces_axes_i64 reads these bits. */
+int mt_count_triangle_vertices_axes(const float* sdf, const int32_t* lower_corners,
+                                    int64_t num_columns, const float* level_sets, int32_t S,
+                                    int32_t* counts, uint8_t* axes, mt_stream_t stream);
+
 /* Forward of CreateTriangleVertices (create_triangle_vertices.slang:145-199,
  * .py:31-102). vertex_offsets int32[S * num_columns + 1] (exclusive). grid_sh
  * f32[num_sh, num_grid_vertices, 3]. Outputs f32[V, 3], f32[V], f32[num_sh, V, 3],
@@ -330,6 +339,18 @@ int mt_create_triangle_faces_i64(const int32_t* neighbors, const int32_t* indice
                                  const uint8_t* cell_codes, int32_t S, const int64_t* face_offsets,
                                  int32_t* faces, int32_t* voxel_indices, int64_t face_capacity,
                                  int64_t vertex_capacity, int32_t* overflow, mt_stream_t stream);
+
+/* mt_create_triangle_faces_i64 for the batched training step: each corner's vertex comes
+ * from its owner column's crossing-axis bits (column_axes uint8[S * num_columns], from
+ * mt_count_triangle_vertices_axes) instead of a search of the column's vertex types, one
+ * dependent load fewer per corner; no voxel indices. faces are the same as
+ * mt_create_triangle_faces_i64's. Replaces create_triangle_faces.slang:3-78. */
+int mt_create_triangle_faces_axes_i64(const int32_t* neighbors, int64_t num_voxels,
+                                      int64_t num_columns, const int32_t* vertex_offsets,
+                                      const uint8_t* column_axes, const uint8_t* cell_codes,
+                                      int32_t S, const int64_t* face_offsets, int32_t* faces,
+                                      int64_t face_capacity, int64_t vertex_capacity,
+                                      int32_t* overflow, mt_stream_t stream);
 
 /* ---------------------------------------------------------------------------------
  * Rasterization (triangle_rasterization/__init__.py:22-329)

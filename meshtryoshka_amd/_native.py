@@ -85,6 +85,7 @@ SIGNATURES = {
         [c_p, c_p, c_p, c_p, c_i32, c_i32, c_i32, c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_p],
     ),
     "mt_count_triangle_vertices": (c_i32, [c_p, c_p, c_i64, c_p, c_i32, c_p, c_p]),
+    "mt_count_triangle_vertices_axes": (c_i32, [c_p, c_p, c_i64, c_p, c_i32, c_p, c_p, c_p]),
     "mt_create_triangle_vertices_forward": (
         c_i32,
         [c_p, c_p, c_p, c_i64, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p,
@@ -131,6 +132,9 @@ SIGNATURES = {
     "mt_create_triangle_faces_i64": (
         c_i32,
         [c_p, c_p, c_i64, c_i64, c_p, c_p, c_p, c_i32, c_p, c_p, c_p, c_i64, c_i64, c_p, c_p],
+    ),
+    "mt_create_triangle_faces_axes_i64": (
+        c_i32, [c_p, c_i64, c_i64, c_p, c_p, c_p, c_i32, c_p, c_p, c_i64, c_i64, c_p, c_p]
     ),
     "mt_project_vertices": (c_i32, [c_p, c_i64, c_p, c_p, c_i32, c_p, c_p, c_p]),
     "mt_bound_triangles": (

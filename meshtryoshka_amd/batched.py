@@ -148,7 +148,9 @@ class BatchedTrainingStep:
         self.td = torch.zeros(V, **f32)
         self.ttype = torch.zeros(V, dtype=torch.uint8, device=dev)
         self.faces = torch.zeros((F, 3), **i32)
-        self.voxel_indices = torch.zeros(F, **i32)
+        # Per (level set, column): which of its axis edges cross (its vertices' types), for
+        # the faces pass. The step needs no voxel index per face.
+        self.column_axes = torch.zeros(S * self.n_col, dtype=torch.uint8, device=dev)
         self.xy = torch.zeros((G, V, 2), **f32)
         self.depth = torch.zeros((G, V), **f32)
         self.n_keys = torch.zeros(1, **i64)  # key count of the current camera group
@@ -254,8 +256,9 @@ class BatchedTrainingStep:
         self.loss.zero_()
         self.sharp_grad.zero_()
         # --- tessellation counts (once for all camera groups)
-        call("mt_count_triangle_vertices", grid_sdf.data_ptr(), v.lower_corners.data_ptr(),
-             self.n_col, self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), st)
+        call("mt_count_triangle_vertices_axes", grid_sdf.data_ptr(), v.lower_corners.data_ptr(),
+             self.n_col, self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(),
+             self.column_axes.data_ptr(), st)
         call("mt_exclusive_scan_i32", self.vertex_offsets.data_ptr(), self.vertex_offsets.numel(),
              self.scan_ws.data_ptr(), self.scan_ws.numel(), st)
         call("mt_count_triangle_faces", grid_sdf.data_ptr(), v.lower_corners.data_ptr(), self.n_col,
@@ -299,12 +302,10 @@ class BatchedTrainingStep:
                     self.side.wait_stream(main)
                 with torch.cuda.stream(self.side if overlap else main):
                     fst = torch.cuda.current_stream(self.device).cuda_stream
-                    call("mt_create_triangle_faces_i64", v.neighbors.data_ptr(),
-                         v.indices.data_ptr(), self.n_vox, self.n_col,
-                         self.vertex_offsets.data_ptr(), self.ttype.data_ptr(),
+                    call("mt_create_triangle_faces_axes_i64", v.neighbors.data_ptr(), self.n_vox,
+                         self.n_col, self.vertex_offsets.data_ptr(), self.column_axes.data_ptr(),
                          self.codes.data_ptr(), S, self.face_offsets.data_ptr(),
-                         self.faces.data_ptr(), self.voxel_indices.data_ptr(), F, V,
-                         self.overflow.data_ptr(), fst)
+                         self.faces.data_ptr(), F, V, self.overflow.data_ptr(), fst)
                     torch.floor_divide(self.face_offsets[self.n_vox :: self.n_vox][:S], 3,
                                        out=self.face_bound64)
                     self.face_boundaries.copy_(self.face_bound64)

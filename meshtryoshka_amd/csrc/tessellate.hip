@@ -46,16 +46,22 @@ __device__ __forceinline__ float edge_t(float d0, float d1, float iso) {
   return fminf(fmaxf(t, 0.0f), 1.0f);
 }
 
+// axes (may be NULL): bit a of axes[s n + c] set when the column's axis-a edge crosses
+// level set s, i.e. when the column has a vertex of type a there (vertices are written in
+// axis order, so its index is the column's first vertex plus the set bits below a).
 __global__ void __launch_bounds__(kBlock) count_vertices_kernel(const float* sdf, const int32_t* lower,
                                                                 int64_t n, const float* level_sets,
-                                                                int S, int32_t* counts) {
+                                                                int S, int32_t* counts,
+                                                                uint8_t* axes) {
   const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const ColumnSdf col = load_column(sdf, lower, n, c);
   for (int s = 0; s < S; ++s) {
     const float iso = level_sets[s];
-    counts[s * n + c] = int(crosses(col.d0, col.d[0], iso)) + int(crosses(col.d0, col.d[1], iso)) +
-                        int(crosses(col.d0, col.d[2], iso));
+    const int x0 = crosses(col.d0, col.d[0], iso), x1 = crosses(col.d0, col.d[1], iso);
+    const int x2 = crosses(col.d0, col.d[2], iso);
+    counts[s * n + c] = x0 + x1 + x2;
+    if (axes) axes[s * n + c] = static_cast<uint8_t>(x0 | (x1 << 1) | (x2 << 2));
   }
 }
 
@@ -865,7 +871,11 @@ __global__ void __launch_bounds__(kBlock) count_faces_kernel(const float* sdf, c
 // triangle vertex, and consecutive threads write consecutive corners.
 // Face offsets are int32 (the reference's layout) or int64 (more than 2^31 - 1 face
 // corners); within one block they span at most 256 voxels x 15 corners.
-template <typename FOff>
+// kAxes: a corner's vertex is found from the owner column's crossing-axis bits (axes, as
+// count_vertices_kernel writes them; one byte beside the offsets) instead of by reading the
+// types of its vertices, which saves one dependent load per corner. voxel_indices may be
+// NULL (not written).
+template <typename FOff, bool kAxes>
 __global__ void __launch_bounds__(kBlock) create_faces_kernel(
     const int32_t* neighbors, const int32_t* indices, int64_t n, int64_t ncol,
     const int32_t* voffsets, const uint8_t* vtypes, const uint8_t* codes, int S,
@@ -921,8 +931,17 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
         const int64_t owner = (slot < 0) ? vv : neighbors[slot * n + vv];
         const int64_t vo = s * ncol + owner;
         const int32_t start = voffsets[vo];
-        int32_t cnt = voffsets[vo + 1] - start;
         target[c] = -1;
+        if constexpr (kAxes) {
+          const uint32_t m = vtypes[vo];  // the owner's crossing axes at this level set
+          if (int64_t(start) + __popc(m) > vcap) {  // never written
+            if (overflow) atomicOr(overflow, 1);
+          } else if ((m >> axis) & 1u) {
+            target[c] = start + __popc(m & ((1u << axis) - 1u));
+          }
+          continue;
+        }
+        int32_t cnt = voffsets[vo + 1] - start;
         if (int64_t(start) + cnt > vcap) {  // vertices past the capacity were never written
           if (overflow) atomicOr(overflow, 1);
           cnt = 0;
@@ -946,7 +965,7 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
       faces[3 * tri + 0] = target[2];
       faces[3 * tri + 1] = target[1];
       faces[3 * tri + 2] = target[0];
-      voxel_indices[tri] = indices[vv];
+      if (voxel_indices) voxel_indices[tri] = indices[vv];
     }
     __syncthreads();  // the next level set reuses the LDS lists
   }
@@ -964,8 +983,24 @@ extern "C" int mt_count_triangle_vertices(const float* sdf, const int32_t* lower
   if (num_columns == 0 || S == 0) return MT_OK;
   mt::count_vertices_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0,
                               static_cast<hipStream_t>(stream)>>>(sdf, lower_corners, num_columns,
-                                                                  level_sets, S, counts);
+                                                                  level_sets, S, counts, nullptr);
   return mt::check_launch("mt_count_triangle_vertices");
+}
+
+extern "C" int mt_count_triangle_vertices_axes(const float* sdf, const int32_t* lower_corners,
+                                               int64_t num_columns, const float* level_sets,
+                                               int32_t S, int32_t* counts, uint8_t* axes,
+                                               mt_stream_t stream) {
+  MT_REQUIRE(num_columns >= 0 && S >= 0, "mt_count_triangle_vertices_axes: negative size");
+  MT_REQUIRE(int64_t(S) * num_columns < (int64_t(1) << 31),
+             "mt_count_triangle_vertices_axes: S * columns exceeds int32 offsets");
+  MT_REQUIRE(axes != nullptr || num_columns == 0 || S == 0,
+             "mt_count_triangle_vertices_axes: null axes");
+  if (num_columns == 0 || S == 0) return MT_OK;
+  mt::count_vertices_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0,
+                              static_cast<hipStream_t>(stream)>>>(sdf, lower_corners, num_columns,
+                                                                  level_sets, S, counts, axes);
+  return mt::check_launch("mt_count_triangle_vertices_axes");
 }
 
 extern "C" int mt_create_triangle_vertices_forward(
@@ -1064,7 +1099,7 @@ extern "C" int mt_create_triangle_faces(const int32_t* neighbors, const int32_t*
                                         mt_stream_t stream) {
   MT_REQUIRE(num_voxels >= 0 && S >= 0 && face_capacity >= 0, "mt_create_triangle_faces: bad sizes");
   if (num_voxels == 0 || S == 0) return MT_OK;
-  mt::create_faces_kernel<int32_t><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
+  mt::create_faces_kernel<int32_t, false><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
                                      static_cast<hipStream_t>(stream)>>>(
       neighbors, indices, num_voxels, num_columns, vertex_offsets, vertex_types, cell_codes, S,
       face_offsets, faces, voxel_indices, face_capacity, vertex_capacity, overflow);
@@ -1083,11 +1118,29 @@ extern "C" int mt_create_triangle_faces_i64(const int32_t* neighbors, const int3
              "mt_create_triangle_faces_i64: bad sizes");
   MT_REQUIRE(face_capacity < (int64_t(1) << 31), "mt_create_triangle_faces_i64: face ids are int32");
   if (num_voxels == 0 || S == 0) return MT_OK;
-  mt::create_faces_kernel<int64_t><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
+  mt::create_faces_kernel<int64_t, false><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
                                      static_cast<hipStream_t>(stream)>>>(
       neighbors, indices, num_voxels, num_columns, vertex_offsets, vertex_types, cell_codes, S,
       face_offsets, faces, voxel_indices, face_capacity, vertex_capacity, overflow);
   return mt::check_launch("mt_create_triangle_faces_i64");
+}
+
+extern "C" int mt_create_triangle_faces_axes_i64(const int32_t* neighbors, int64_t num_voxels,
+                                                 int64_t num_columns, const int32_t* vertex_offsets,
+                                                 const uint8_t* column_axes,
+                                                 const uint8_t* cell_codes, int32_t S,
+                                                 const int64_t* face_offsets, int32_t* faces,
+                                                 int64_t face_capacity, int64_t vertex_capacity,
+                                                 int32_t* overflow, mt_stream_t stream) {
+  MT_REQUIRE(num_voxels >= 0 && S >= 0 && face_capacity >= 0,
+             "mt_create_triangle_faces_axes_i64: bad sizes");
+  MT_REQUIRE(face_capacity < (int64_t(1) << 31), "mt_create_triangle_faces_axes_i64: face ids are int32");
+  if (num_voxels == 0 || S == 0) return MT_OK;
+  mt::create_faces_kernel<int64_t, true><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
+                                           static_cast<hipStream_t>(stream)>>>(
+      neighbors, nullptr, num_voxels, num_columns, vertex_offsets, column_axes, cell_codes, S,
+      face_offsets, faces, nullptr, face_capacity, vertex_capacity, overflow);
+  return mt::check_launch("mt_create_triangle_faces_axes_i64");
 }
 
 extern "C" int mt_create_triangle_vertices_colors_forward(

@@ -113,6 +113,55 @@ def test_stage_outputs_match_oracle(scene):
     np.testing.assert_array_equal(codes.cpu().numpy(), scene.tri.cell_codes)
 
 
+def test_faces_from_crossing_axes_match_oracle(scene):
+    """The batched step's faces pass (mt_count_triangle_vertices_axes +
+    mt_create_triangle_faces_axes_i64: each corner's vertex from the owner column's
+    crossing-axis bits) gives the oracle's faces; the bits agree with the counts and with
+    the vertex types, and a vertex capacity below the vertex count raises the flag."""
+    from meshtryoshka_amd import _native
+
+    v = scene.vox
+    sdf, lower, upper = to_dev(scene.sdf), to_dev(v.lower_corners), to_dev(v.upper_corners)
+    ls = to_dev(scene.level_sets)
+    n, s, nvox = lower.shape[1], ls.shape[0], upper.shape[1]
+    st = torch.cuda.current_stream().cuda_stream
+    counts = torch.zeros(s * n + 1, dtype=torch.int32, device="cuda")
+    axes = torch.full((s * n,), 255, dtype=torch.uint8, device="cuda")
+    _native.call("mt_count_triangle_vertices_axes", sdf.data_ptr(), lower.data_ptr(), n, ls.data_ptr(),
+                 s, counts.data_ptr(), axes.data_ptr(), st)
+    np.testing.assert_array_equal(counts.cpu().numpy(), scene.tri.vertex_counts)
+    bits = axes.to(torch.int32)
+    pop = (bits & 1) + ((bits >> 1) & 1) + ((bits >> 2) & 1)
+    assert bool((bits < 8).all()) and torch.equal(pop, counts[:-1])
+    # The k-th vertex of a column has the type of its k-th set bit (the oracle's types).
+    offsets = torch.as_tensor(scene.tri.vertex_offsets.astype(np.int64))
+    types = torch.as_tensor(scene.tri.vertex_types.astype(np.int64))
+    b = bits.cpu().to(torch.int64)
+    for a in range(3):
+        has = ((b >> a) & 1).bool()
+        rank = ((b & ((1 << a) - 1)) & 1) + (((b & ((1 << a) - 1)) >> 1) & 1)
+        idx = offsets[:-1][has] + rank[has]
+        assert bool((types[idx] == a).all())
+    fc, codes = te.count_triangle_faces(sdf, lower, upper, ls)
+    foff = torch.zeros(s * nvox + 1, dtype=torch.int64, device="cuda")
+    foff[1:] = torch.cumsum(fc[:-1].to(torch.int64), 0)  # counts carry one trailing entry
+    nf = int(foff[-1]) // 3
+    voff = counts.clone()
+    te.compute_exclusive_cumsum(voff)
+    nv = int(voff[-1])
+    faces = torch.full((nf, 3), -7, dtype=torch.int32, device="cuda")
+    flag = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _native.call("mt_create_triangle_faces_axes_i64", to_dev(v.neighbors).data_ptr(), nvox, n,
+                 voff.data_ptr(), axes.data_ptr(), codes.data_ptr(), s, foff.data_ptr(),
+                 faces.data_ptr(), nf, nv, flag.data_ptr(), st)
+    assert int(flag) == 0
+    np.testing.assert_array_equal(faces.cpu().numpy(), scene.tri.faces)
+    _native.call("mt_create_triangle_faces_axes_i64", to_dev(v.neighbors).data_ptr(), nvox, n,
+                 voff.data_ptr(), axes.data_ptr(), codes.data_ptr(), s, foff.data_ptr(),
+                 faces.data_ptr(), nf, nv // 2, flag.data_ptr(), st)
+    assert int(flag) == 1
+
+
 def test_create_vertices_backward_matches_oracle(scene):
     v = scene.vox
     grid_v = to_dev(scene.world).requires_grad_(True)
