@@ -885,11 +885,18 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
   __shared__ int32_t s_start[kBlock];
   __shared__ uint8_t s_code[kBlock];
   __shared__ uint8_t s_owner[kMaxTri];  // the block-local voxel of each of the block's triangles
+  // The voxels' neighbour columns (slots 0..5: +x, +y, +z, +xy, +yz, +xz), read once for all
+  // level sets, so that a corner's owner column costs no dependent global load.
+  __shared__ int32_t s_nbr[6][kBlock];
   __shared__ int s_total;
   const int64_t v_first = int64_t(blockIdx.x) * kBlock;
   const int last = static_cast<int>((v_first + kBlock < n ? v_first + kBlock : n) - 1 - v_first);
   const int64_t v = v_first + threadIdx.x;
   const bool valid = int(threadIdx.x) <= last;
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s_nbr[q][threadIdx.x] = neighbors[q * n + v];
+  }
   for (int s = 0; s < S; ++s) {
     const int64_t base = foffsets[int64_t(s) * n + v_first];
     if (valid) {
@@ -928,7 +935,7 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
         const int e = MT_MC_CASE_EDGES[MT_MC_CASE_START[code] + q0 + c];
         const int slot = MT_MC_EDGE_OWNER[e][0];
         const int axis = MT_MC_EDGE_OWNER[e][1];
-        const int64_t owner = (slot < 0) ? vv : neighbors[slot * n + vv];
+        const int64_t owner = (slot < 0) ? vv : s_nbr[slot][lo];
         const int64_t vo = s * ncol + owner;
         const int32_t start = voffsets[vo];
         target[c] = -1;
