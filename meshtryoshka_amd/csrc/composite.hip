@@ -215,6 +215,10 @@ __device__ __forceinline__ TileCoord locate_tile(const mt_composite_args& a, int
 
 // Stage keys [first, first + count) of one tile: terms, candidate rectangle, face; clear
 // the hit masks. Returns nothing; callers barrier afterwards.
+// Largest candidate box taken from the bounding box of a certified triangle (larger boxes
+// use the candidate rectangle, which is tighter for thin diagonal triangles).
+constexpr int kStageCentrePixels = 16;
+
 __device__ __forceinline__ void stage_chunk(const ChunkLds& L, const float2* xy,
                                             const int32_t* faces, const int32_t* values,
                                             const int64_t* keys, int shell_offset,
@@ -231,7 +235,14 @@ __device__ __forceinline__ void stage_chunk(const ChunkLds& L, const float2* xy,
     const int64_t vc = faces[3 * int64_t(face) + 2];
     const float2 p1 = xy[va], p2 = xy[vb], p3 = xy[vc];
     const TriTerms t = tri_terms(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
-    const Rect r = candidate_rect(t, x0, y0, tw, th);
+    // A small, well-shaped triangle's candidates are the pixel centres near its bounding
+    // box (geometry.h, centre_range: float arithmetic under a proven shape certificate);
+    // the others get the candidate rectangle (double precision).
+    const CentreRange cen = centre_range(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y, x0, y0, tw, th,
+                                         kStageCentrePixels);
+    const Rect r = cen.state == kCentresNone  ? Rect{0, 0, 0, 0}
+                   : cen.state == kCentresFew ? Rect{cen.x, cen.y, cen.w, cen.h}
+                                              : candidate_rect(t, x0, y0, tw, th);
     L.geo[j] = to_staged(t, shell);
     L.rect[j] = make_uint2(uint32_t(r.x) | (uint32_t(r.w) << 16), uint32_t(r.y) | (uint32_t(r.h) << 16));
     L.face[j] = face;
