@@ -471,6 +471,30 @@ int mt_sort_keys(const int64_t* keys_in, const int32_t* values_in, int64_t n, in
                  int32_t* values_out, void* workspace, size_t workspace_bytes,
                  const int64_t* n_device, mt_stream_t stream);
 
+/* Tile-bucketed sort, part 1 (batched step and serving; no single counterpart in the
+ * reference, whose sort_keys.cu:215-270 sorts all bits at once): mt_sort_keys over the key
+ * bits [low_bits, 64) only (signed order on bit 63), stable, so keys with equal high bits
+ * keep their input order. Same workspace as mt_sort_keys. With the fused layout and
+ * low_bits = the tile field's offset, the keys come out grouped by (camera, tile); a
+ * delineation then gives each tile's range, and mt_sort_key_ranges orders each range by
+ * the low bits (depth, shell): together the stable sort by the whole key, i.e.
+ * mt_sort_keys' output. */
+int mt_sort_keys_high(const int64_t* keys_in, const int32_t* values_in, int64_t n,
+                      int64_t* keys_out, int32_t* values_out, void* workspace,
+                      size_t workspace_bytes, const int64_t* n_device, int32_t low_bits,
+                      mt_stream_t stream);
+
+/* Tile-bucketed sort, part 2: stably sorts, in place, each range [ranges[2r], ranges[2r+1])
+ * (r < num_ranges; disjoint; empty ranges allowed) of keys / values by the key bits
+ * [0, low_bits) (unsigned), 1 <= low_bits <= 63. A range of up to 4096 keys is sorted by one
+ * 256-thread workgroup and up to 16384 keys by a 1024-thread workgroup with the keys in
+ * registers and LDS (radix passes of 8 bits over the bits that vary inside the range);
+ * longer ranges pass through tmp_keys / tmp_values (n entries each, separate arrays, their
+ * contents overwritten). */
+int mt_sort_key_ranges(int64_t* keys, int32_t* values, int64_t n, const int64_t* ranges,
+                       int64_t num_ranges, int32_t low_bits, int64_t* tmp_keys,
+                       int32_t* tmp_values, mt_stream_t stream);
+
 /* [start, end) of every (camera[, shell], tile) run of the sorted keys into the
  * zero-initialised int64[B, S', T, 2] (S' = S if separate_shells else 1). If n_device
  * is not NULL, min(*n_device, n) keys are live. With tile_base int64[B + 1] (cameras of

@@ -327,12 +327,20 @@ class BatchedTrainingStep:
                  n_keys.data_ptr(), self.overflow.data_ptr(), self.sorted_keys.data_ptr(),
                  self.sorted_values.data_ptr(), self.keys_ws.data_ptr(), self.keys_ws.numel(), st)
             self.group_keys[gi : gi + 1].copy_(n_keys)
-            call("mt_sort_keys", self.keys.data_ptr(), self.values.data_ptr(), K,
+            # Tile-bucketed sort: radix passes over the (camera, tile) bits only, the tile
+            # ranges, then each tile's keys ordered by (depth, shell) inside one workgroup
+            # (the unsorted key arrays serve as its temporary space): the same order as the
+            # full sort of the keys.
+            low = self.layout.tile_offset
+            call("mt_sort_keys_high", self.keys.data_ptr(), self.values.data_ptr(), K,
                  self.sorted_keys.data_ptr(), self.sorted_values.data_ptr(), self.sort_ws.data_ptr(),
-                 self.sort_ws.numel(), n_keys.data_ptr(), st)
+                 self.sort_ws.numel(), n_keys.data_ptr(), low, st)
             call("mt_delineate_tiles", self.sorted_keys.data_ptr(), K, self.layout, 0, n, 1, self.T,
                  self.ranges.data_ptr(), n_keys.data_ptr(),
                  g["tile_base"].data_ptr() if table else None, st)
+            call("mt_sort_key_ranges", self.sorted_keys.data_ptr(), self.sorted_values.data_ptr(), K,
+                 self.ranges.data_ptr(), g["tiles"], low, self.keys.data_ptr(),
+                 self.values.data_ptr(), st)
             args = self._composite_args(g)
             p0 = self.pixel_first[c0]
             out = self.out_flat[4 * p0 :]

@@ -33,7 +33,7 @@ constexpr int kPasses = 8;
 
 struct SortWorkspace {
   int32_t* info;             // [0..7] active, [8..15] src, [16..23] dst, [24] passes
-  unsigned long long* bits;  // [0] OR of all keys, [1] AND of all keys
+  unsigned long long* bits;  // [0] OR of all keys, [1] AND of all keys, [2] bits sorted by
   int32_t* counts;           // [256][tiles] keys per (digit, tile) of the current pass
   int64_t* offsets;          // [256][tiles] inclusive scan of counts
   void* scan_ws;
@@ -124,7 +124,7 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t key, const Runs& r, int pa
 }
 
 __device__ __forceinline__ uint64_t varying_bits(const SortWorkspace& w) {
-  return w.bits[0] & ~w.bits[1];
+  return w.bits[0] & ~w.bits[1] & w.bits[2];
 }
 
 __device__ __forceinline__ int64_t live_count(int64_t n, const int64_t* n_device) {
@@ -133,10 +133,11 @@ __device__ __forceinline__ int64_t live_count(int64_t n, const int64_t* n_device
   return d < n ? (d < 0 ? 0 : d) : n;
 }
 
-__global__ void init_kernel(SortWorkspace w) {
+__global__ void init_kernel(SortWorkspace w, uint64_t sorted_bits) {
   if (threadIdx.x == 0) {
     w.bits[0] = 0ull;
     w.bits[1] = ~0ull;
+    w.bits[2] = sorted_bits;
   }
 }
 
@@ -385,27 +386,386 @@ unsigned capped_blocks(int64_t n, int64_t per_block, int64_t cap) {
 }  // namespace
 }  // namespace mt
 
+namespace mt {
+namespace {
+
+// ------------------------------------------------------------------------------------
+// Sorting ranges in place by their low key bits (the second half of a tile-bucketed
+// sort: mt_sort_keys_high orders the keys by their high bits, e.g. camera | tile, a
+// delineation gives each tile's range, and every range is then sorted by the remaining
+// bits, e.g. depth | shell, inside one workgroup). Stable, so the result is the stable
+// sort by the whole key.
+// ------------------------------------------------------------------------------------
+
+// Per pass of a block sort: the items of the block are in position order (item i of lane l
+// of wave w is position w kIt 64 + i 64 + l). Each item's digit is ranked among its wave's
+// lanes by ballots (rank_tile's multi-split, generalised to kIt items) and the wave's
+// running count of the digit in LDS; the waves' counts are scanned per digit and over the
+// digits, and pos receives each item's place in the stable order by this digit.
+template <int kThreads, int kIt>
+struct BlockRank {
+  static constexpr int kW = kThreads / kWave;
+  uint32_t wave_count[kW][kRadix];
+  uint32_t digit_start[kRadix];
+  uint32_t digit_total[kRadix];
+  uint32_t scan_wave[kRadix / kWave];
+};
+
+// live: the wave's items at or past this index hold no key (a bound the wave agrees on), so
+// their ranking is skipped. Ends with a barrier.
+template <int kThreads, int kIt>
+__device__ __forceinline__ void block_rank(const uint32_t (&digits)[kIt], uint32_t (&pos)[kIt],
+                                           BlockRank<kThreads, kIt>& R, int live = kIt) {
+  constexpr int kW = kThreads / kWave;
+  for (int i = threadIdx.x; i < kW * kRadix; i += kThreads) (&R.wave_count[0][0])[i] = 0u;
+  __syncthreads();
+  const int wave = threadIdx.x / kWave;
+  const int lane = __lane_id();
+  const uint64_t lt = lanemask_lt();
+  uint32_t ranks[kIt];
+#pragma unroll
+  for (int i = 0; i < kIt; ++i) {
+    ranks[i] = 0u;
+    if (i >= live) continue;
+    const bool valid = digits[i] < kRadix;
+    const uint32_t dd = valid ? digits[i] : 0u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const bool set = (dd >> bit) & 1u;
+      const uint64_t m = __ballot(set);
+      peers &= set ? m : ~m;
+    }
+    const uint32_t before = static_cast<uint32_t>(__popcll(peers & lt));
+    const uint32_t old = valid ? R.wave_count[wave][dd] : 0u;
+    ranks[i] = old + before;
+    if (valid && before == 0) R.wave_count[wave][dd] = old + static_cast<uint32_t>(__popcll(peers));
+  }
+  __syncthreads();
+  // Thread t < kOwners owns digits [t kDpt, (t + 1) kDpt): the waves' offsets inside each
+  // digit's run, then the digits' starts by a scan over the owners (in digit order).
+  constexpr int kDpt = kRadix / kThreads > 0 ? kRadix / kThreads : 1;
+  constexpr int kOwners = kRadix / kDpt;  // 256 or 64: whole waves
+  const int t = threadIdx.x;
+  uint32_t tot[kDpt];
+  uint32_t local = 0, inc = 0;
+  if (t < kOwners) {
+#pragma unroll
+    for (int j = 0; j < kDpt; ++j) {
+      const int d = t * kDpt + j;
+      uint32_t total = 0;
+#pragma unroll
+      for (int q = 0; q < kW; ++q) {
+        const uint32_t c = R.wave_count[q][d];
+        R.wave_count[q][d] = total;
+        total += c;
+      }
+      tot[j] = total;
+      local += total;
+    }
+    inc = wave_inclusive_sum(local);
+    if (lane == kWave - 1) R.scan_wave[t / kWave] = inc;
+  }
+  __syncthreads();
+  if (t < kOwners) {
+    uint32_t run = inc - local;
+    for (int q = 0; q < t / kWave; ++q) run += R.scan_wave[q];
+#pragma unroll
+    for (int j = 0; j < kDpt; ++j) {
+      const int d = t * kDpt + j;
+      R.digit_start[d] = run;
+      R.digit_total[d] = tot[j];
+      run += tot[j];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kIt; ++i)
+    pos[i] = (i < live && digits[i] < kRadix)
+                 ? R.digit_start[digits[i]] + R.wave_count[wave][digits[i]] + ranks[i]
+                 : 0xffffffffu;
+}
+
+// Varying bits of the range's keys below low_mask, and the runs that compact them.
+template <int kThreads>
+__device__ __forceinline__ void range_runs(unsigned long long o, unsigned long long a,
+                                           unsigned long long* s_oa, Runs* runs, int* passes,
+                                           uint64_t low_mask) {
+#pragma unroll
+  for (int dd = kWave / 2; dd >= 1; dd >>= 1) {
+    o |= __shfl_xor(o, dd, kWave);
+    a &= __shfl_xor(a, dd, kWave);
+  }
+  if (threadIdx.x == 0) {
+    s_oa[0] = 0ull;
+    s_oa[1] = ~0ull;
+  }
+  __syncthreads();
+  if (__lane_id() == 0) {
+    atomicOr(&s_oa[0], o);
+    atomicAnd(&s_oa[1], a);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t v = s_oa[0] & ~s_oa[1] & low_mask;
+    make_runs(v, runs);
+    *passes = (__popcll(v) + 7) / 8;
+  }
+  __syncthreads();
+}
+
+template <int kThreads, int kIt>
+struct RangeSortLds {
+  int64_t buf[kThreads * kIt];  // keys, then (as int32) values, in the pass's new order
+  BlockRank<kThreads, kIt> rank;
+  unsigned long long oa[2];
+  Runs runs;
+  int passes;
+};
+
+// One workgroup per range of at most kThreads * kIt keys (ranges of other sizes are left
+// to the other kernels): the keys and values stay in registers and LDS for every pass.
+template <int kThreads, int kIt>
+__global__ void __launch_bounds__(kThreads) range_sort_kernel(int64_t* keys, int32_t* values,
+                                                              const int64_t* ranges,
+                                                              int64_t num_ranges, int64_t n,
+                                                              int64_t min_size, uint64_t low_mask) {
+  constexpr int kCap = kThreads * kIt;
+  constexpr int kPerWave = kIt * kWave;
+  extern __shared__ __align__(16) char s_dyn[];
+  auto& L = *reinterpret_cast<RangeSortLds<kThreads, kIt>*>(s_dyn);
+  const int wave = threadIdx.x / kWave;
+  const int lane = __lane_id();
+  for (int64_t r = blockIdx.x; r < num_ranges; r += gridDim.x) {
+    int64_t start = ranges[2 * r], end = ranges[2 * r + 1];
+    start = start < 0 ? 0 : (start > n ? n : start);
+    end = end < start ? start : (end > n ? n : end);
+    const int64_t m = end - start;
+    if (m <= min_size || m > kCap) continue;  // uniform over the block
+    int64_t k[kIt];
+    int32_t v[kIt];
+    unsigned long long o = 0ull, a = ~0ull;
+#pragma unroll
+    for (int i = 0; i < kIt; ++i) {
+      const int p = wave * kPerWave + i * kWave + lane;
+      const bool valid = p < m;
+      k[i] = valid ? keys[start + p] : 0;
+      v[i] = valid ? values[start + p] : 0;
+      if (valid) {
+        o |= static_cast<uint64_t>(k[i]);
+        a &= static_cast<uint64_t>(k[i]);
+      }
+    }
+    range_runs<kThreads>(o, a, L.oa, &L.runs, &L.passes, low_mask);
+    const int passes = L.passes;
+    // Items of this wave that hold some key (the same for every lane of the wave).
+    const int64_t in_wave = m - int64_t(wave) * kPerWave;
+    const int live = in_wave <= 0 ? 0 : int(in_wave >= kPerWave ? kIt : (in_wave + kWave - 1) / kWave);
+    for (int pass = 0; pass < passes; ++pass) {
+      uint32_t dg[kIt], pos[kIt];
+#pragma unroll
+      for (int i = 0; i < kIt; ++i) {
+        const int p = wave * kPerWave + i * kWave + lane;
+        dg[i] = p < m ? digit_of(static_cast<uint64_t>(k[i]), L.runs, pass) : kRadix;
+      }
+      block_rank<kThreads, kIt>(dg, pos, L.rank, live);
+#pragma unroll
+      for (int i = 0; i < kIt; ++i)
+        if (pos[i] != 0xffffffffu) L.buf[pos[i]] = k[i];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < kIt; ++i) {
+        const int p = wave * kPerWave + i * kWave + lane;
+        if (p < m) k[i] = L.buf[p];
+      }
+      __syncthreads();
+      int32_t* vb = reinterpret_cast<int32_t*>(L.buf);
+#pragma unroll
+      for (int i = 0; i < kIt; ++i)
+        if (pos[i] != 0xffffffffu) vb[pos[i]] = v[i];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < kIt; ++i) {
+        const int p = wave * kPerWave + i * kWave + lane;
+        if (p < m) v[i] = vb[p];
+      }
+      __syncthreads();
+    }
+    if (passes > 0) {
+#pragma unroll
+      for (int i = 0; i < kIt; ++i) {
+        const int p = wave * kPerWave + i * kWave + lane;
+        if (p < m) {
+          keys[start + p] = k[i];
+          values[start + p] = v[i];
+        }
+      }
+    }
+    __syncthreads();  // the next range reuses the LDS
+  }
+}
+
+// Ranges longer than kThreads * kIt: one workgroup per range runs the passes through
+// global memory (the range in keys / values and the same places of tmp, alternately),
+// chunk by chunk in order: a histogram of the whole range, then each chunk's items go to
+// their digit's next places. Stable like the in-LDS passes.
+template <int kThreads, int kIt>
+__global__ void __launch_bounds__(kThreads) long_range_sort_kernel(
+    int64_t* keys, int32_t* values, int64_t* tmp_keys, int32_t* tmp_values,
+    const int64_t* ranges, int64_t num_ranges, int64_t n, int64_t min_size, uint64_t low_mask) {
+  constexpr int kCap = kThreads * kIt;
+  constexpr int kPerWave = kIt * kWave;
+  __shared__ BlockRank<kThreads, kIt> R;
+  __shared__ uint32_t s_hist[kRadix];
+  __shared__ int64_t s_next[kRadix];
+  __shared__ unsigned long long s_oa[2];
+  __shared__ Runs s_runs;
+  __shared__ int s_passes;
+  const int wave = threadIdx.x / kWave;
+  const int lane = __lane_id();
+  for (int64_t r = blockIdx.x; r < num_ranges; r += gridDim.x) {
+    int64_t start = ranges[2 * r], end = ranges[2 * r + 1];
+    start = start < 0 ? 0 : (start > n ? n : start);
+    end = end < start ? start : (end > n ? n : end);
+    const int64_t m = end - start;
+    if (m <= min_size) continue;  // uniform over the block
+    unsigned long long o = 0ull, a = ~0ull;
+    for (int64_t j = threadIdx.x; j < m; j += kThreads) {
+      const uint64_t kk = static_cast<uint64_t>(keys[start + j]);
+      o |= kk;
+      a &= kk;
+    }
+    range_runs<kThreads>(o, a, s_oa, &s_runs, &s_passes, low_mask);
+    const int passes = s_passes;
+    for (int pass = 0; pass < passes; ++pass) {
+      const bool from_tmp = (pass & 1) != 0;
+      const int64_t* src_k = (from_tmp ? tmp_keys : keys) + start;
+      const int32_t* src_v = (from_tmp ? tmp_values : values) + start;
+      int64_t* dst_k = (from_tmp ? keys : tmp_keys) + start;
+      int32_t* dst_v = (from_tmp ? values : tmp_values) + start;
+      for (int d = threadIdx.x; d < kRadix; d += kThreads) s_hist[d] = 0u;
+      __syncthreads();
+      for (int64_t j = threadIdx.x; j < m; j += kThreads)
+        atomicAdd(&s_hist[digit_of(static_cast<uint64_t>(src_k[j]), s_runs, pass)], 1u);
+      __syncthreads();
+      if (threadIdx.x < kRadix) {  // exclusive scan of the histogram (one wave per 64 digits)
+        const uint32_t c = s_hist[threadIdx.x];
+        const uint32_t inc = wave_inclusive_sum(c);
+        if (lane == kWave - 1) R.scan_wave[threadIdx.x / kWave] = inc;
+        s_next[threadIdx.x] = inc - c;
+      }
+      __syncthreads();
+      if (threadIdx.x < kRadix) {
+        int64_t before = 0;
+        for (int q = 0; q < int(threadIdx.x) / kWave; ++q) before += R.scan_wave[q];
+        s_next[threadIdx.x] += before;
+      }
+      __syncthreads();
+      for (int64_t c0 = 0; c0 < m; c0 += kCap) {
+        const int64_t cm = (m - c0) < kCap ? (m - c0) : kCap;
+        int64_t k[kIt];
+        int32_t v[kIt];
+        uint32_t dg[kIt], pos[kIt];
+#pragma unroll
+        for (int i = 0; i < kIt; ++i) {
+          const int p = wave * kPerWave + i * kWave + lane;
+          const bool valid = p < cm;
+          k[i] = valid ? src_k[c0 + p] : 0;
+          v[i] = valid ? src_v[c0 + p] : 0;
+          dg[i] = valid ? digit_of(static_cast<uint64_t>(k[i]), s_runs, pass) : kRadix;
+        }
+        block_rank<kThreads, kIt>(dg, pos, R);
+#pragma unroll
+        for (int i = 0; i < kIt; ++i) {
+          if (pos[i] == 0xffffffffu) continue;
+          // pos[i] - digit_start = the item's place among the chunk's items of its digit.
+          const int64_t at = s_next[dg[i]] + (pos[i] - R.digit_start[dg[i]]);
+          dst_k[at] = k[i];
+          dst_v[at] = v[i];
+        }
+        __syncthreads();
+        if (threadIdx.x < kRadix) s_next[threadIdx.x] += R.digit_total[threadIdx.x];
+        __syncthreads();
+      }
+    }
+    if (passes & 1) {  // the last pass wrote tmp: copy the range back
+      for (int64_t j = threadIdx.x; j < m; j += kThreads) {
+        keys[start + j] = tmp_keys[start + j];
+        values[start + j] = tmp_values[start + j];
+      }
+    }
+    __syncthreads();  // the next range reuses the LDS
+  }
+}
+
+
+
+}  // namespace
+}  // namespace mt
+
+extern "C" int mt_sort_key_ranges(int64_t* keys, int32_t* values, int64_t n,
+                                  const int64_t* ranges, int64_t num_ranges, int32_t low_bits,
+                                  int64_t* tmp_keys, int32_t* tmp_values, mt_stream_t stream) {
+  using namespace mt;
+  MT_REQUIRE(n >= 0 && num_ranges >= 0, "mt_sort_key_ranges: negative size");
+  MT_REQUIRE(low_bits >= 0 && low_bits <= 63, "mt_sort_key_ranges: low_bits %d (0..63)", low_bits);
+  if (n == 0 || num_ranges == 0 || low_bits == 0) return MT_OK;
+  MT_REQUIRE(keys && values && ranges && tmp_keys && tmp_values, "mt_sort_key_ranges: null pointer");
+  MT_REQUIRE(tmp_keys != keys && tmp_values != values,
+             "mt_sort_key_ranges: the temporary arrays must be separate from the keys and values");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint64_t low_mask = (1ull << low_bits) - 1ull;
+  // (threads, items): (64, 16) up to 1024 keys, (256, 16) up to 4096, (512, 16) up to
+  // 8192 (75 KB of LDS, two workgroups per CU), (1024, 16) up to 16384 (146 KB); longer
+  // ranges through the temporary arrays.
+  const size_t lds_a = sizeof(RangeSortLds<64, 16>), lds_b = sizeof(RangeSortLds<256, 16>);
+  const size_t lds_c = sizeof(RangeSortLds<512, 16>), lds_d = sizeof(RangeSortLds<1024, 16>);
+  static bool attributes_set = false;
+  if (!attributes_set) {
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&range_sort_kernel<512, 16>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_c));
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&range_sort_kernel<1024, 16>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_d));
+    attributes_set = true;
+  }
+  const auto grid = [&](int64_t cap) {
+    return static_cast<unsigned>(num_ranges < cap ? num_ranges : cap);
+  };
+  range_sort_kernel<64, 16><<<grid(1 << 20), 64, lds_a, s>>>(keys, values, ranges, num_ranges, n, 1,
+                                                              low_mask);
+  range_sort_kernel<256, 16><<<grid(1 << 18), 256, lds_b, s>>>(keys, values, ranges, num_ranges, n,
+                                                               1024, low_mask);
+  range_sort_kernel<512, 16><<<grid(8192), 512, lds_c, s>>>(keys, values, ranges, num_ranges, n,
+                                                             4096, low_mask);
+  range_sort_kernel<1024, 16><<<grid(4096), 1024, lds_d, s>>>(keys, values, ranges, num_ranges, n,
+                                                              8192, low_mask);
+  long_range_sort_kernel<1024, 8><<<grid(4096), 1024, 0, s>>>(
+      keys, values, tmp_keys, tmp_values, ranges, num_ranges, n, 16384, low_mask);
+  return check_launch("mt_sort_key_ranges");
+}
+
 extern "C" size_t mt_sort_workspace_bytes(int64_t n) { return mt::workspace_bytes(n < 1 ? 1 : n); }
 
-extern "C" int mt_sort_keys(const int64_t* keys_in, const int32_t* values_in, int64_t n,
-                            int64_t* keys_out, int32_t* values_out, void* workspace,
-                            size_t workspace_bytes, const int64_t* n_device,
-                            mt_stream_t stream) {
-  using namespace mt;
-  MT_REQUIRE(n >= 0, "mt_sort_keys: negative length");
+namespace mt {
+namespace {
+
+// The LSD sort over the key bits in sorted_bits (all 64 for mt_sort_keys).
+int sort_impl(const int64_t* keys_in, const int32_t* values_in, int64_t n, int64_t* keys_out,
+              int32_t* values_out, void* workspace, size_t workspace_bytes,
+              const int64_t* n_device, uint64_t sorted_bits, hipStream_t s, const char* name) {
+  MT_REQUIRE(n >= 0, "%s: negative length", name);
   if (n == 0) return MT_OK;
-  MT_REQUIRE(keys_in && values_in && keys_out && values_out, "mt_sort_keys: null pointer");
-  MT_REQUIRE(keys_in != keys_out && values_in != values_out, "mt_sort_keys: in-place sort is not supported");
+  MT_REQUIRE(keys_in && values_in && keys_out && values_out, "%s: null pointer", name);
+  MT_REQUIRE(keys_in != keys_out && values_in != values_out, "%s: in-place sort is not supported", name);
   const int64_t tiles = tiles_for(n);
-  MT_REQUIRE(tiles < (int64_t(1) << 31), "mt_sort_keys: too many keys");
+  MT_REQUIRE(tiles < (int64_t(1) << 31), "%s: too many keys", name);
   const size_t need = mt::workspace_bytes(n);
   if (workspace == nullptr || workspace_bytes < need) {
-    set_error("mt_sort_keys: workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
+    set_error("%s: workspace of %zu bytes is smaller than the %zu required", name, workspace_bytes, need);
     return MT_ERR_WORKSPACE_TOO_SMALL;
   }
-  hipStream_t s = static_cast<hipStream_t>(stream);
   SortWorkspace w = carve(workspace, n);
-  init_kernel<<<1, 64, 0, s>>>(w);
+  init_kernel<<<1, 64, 0, s>>>(w, sorted_bits);
   mask_kernel<<<capped_blocks(n, 256 * 16, 4096), 256, 0, s>>>(keys_in, n, w, n_device);
   prepare_kernel<<<1, 64, 0, s>>>(w);
   const unsigned grid = static_cast<unsigned>(tiles);
@@ -419,5 +779,26 @@ extern "C" int mt_sort_keys(const int64_t* keys_in, const int32_t* values_in, in
   }
   copy_if_idle_kernel<<<capped_blocks(n, 256 * 4, 65536), 256, 0, s>>>(
       keys_in, values_in, keys_out, values_out, n, w.info, n_device);
-  return check_launch("mt_sort_keys");
+  return check_launch(name);
+}
+
+}  // namespace
+}  // namespace mt
+
+extern "C" int mt_sort_keys(const int64_t* keys_in, const int32_t* values_in, int64_t n,
+                            int64_t* keys_out, int32_t* values_out, void* workspace,
+                            size_t workspace_bytes, const int64_t* n_device,
+                            mt_stream_t stream) {
+  return mt::sort_impl(keys_in, values_in, n, keys_out, values_out, workspace, workspace_bytes,
+                       n_device, ~0ull, static_cast<hipStream_t>(stream), "mt_sort_keys");
+}
+
+extern "C" int mt_sort_keys_high(const int64_t* keys_in, const int32_t* values_in, int64_t n,
+                                 int64_t* keys_out, int32_t* values_out, void* workspace,
+                                 size_t workspace_bytes, const int64_t* n_device,
+                                 int32_t low_bits, mt_stream_t stream) {
+  MT_REQUIRE(low_bits >= 0 && low_bits <= 64, "mt_sort_keys_high: low_bits %d", low_bits);
+  const uint64_t high = low_bits >= 64 ? 0ull : (~0ull << low_bits);
+  return mt::sort_impl(keys_in, values_in, n, keys_out, values_out, workspace, workspace_bytes,
+                       n_device, high, static_cast<hipStream_t>(stream), "mt_sort_keys_high");
 }

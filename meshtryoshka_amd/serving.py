@@ -412,10 +412,15 @@ def render_batch(scene: ResidentScene, extrinsics: torch.Tensor, intrinsics: tor
          float(near_plane), dims_p, 1, cols, _LAYOUT, keys.data_ptr(), vals.data_ptr(), k_cap,
          n_dev.data_ptr(), overflow.data_ptr(), skeys.data_ptr(), svals.data_ptr(),
          keys_ws.data_ptr(), keys_ws.numel(), st)
-    call("mt_sort_keys", keys.data_ptr(), vals.data_ptr(), k_cap, skeys.data_ptr(), svals.data_ptr(),
-         sort_ws.data_ptr(), sort_ws.numel(), n_dev.data_ptr(), st)
+    # Tile-bucketed sort (as in the training step): (camera, tile) bits, tile ranges, then
+    # each tile's keys by (depth, shell), with the unsorted key arrays as temporary space.
+    low = _LAYOUT.tile_offset
+    call("mt_sort_keys_high", keys.data_ptr(), vals.data_ptr(), k_cap, skeys.data_ptr(),
+         svals.data_ptr(), sort_ws.data_ptr(), sort_ws.numel(), n_dev.data_ptr(), low, st)
     call("mt_delineate_tiles", skeys.data_ptr(), k_cap, _LAYOUT, 0, b, 1, rows * cols,
          ranges.data_ptr(), n_dev.data_ptr(), tbase.data_ptr() if table else None, st)
+    call("mt_sort_key_ranges", skeys.data_ptr(), svals.data_ptr(), k_cap, ranges.data_ptr(),
+         tile_base[-1], low, keys.data_ptr(), vals.data_ptr(), st)
     call("mt_camera_positions", E.data_ptr(), b, campos.data_ptr(), st)
     if scene.grid is None:
         call("mt_evaluate_sh_forward", scene.vertices.data_ptr(), v,

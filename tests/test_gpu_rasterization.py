@@ -164,6 +164,68 @@ def test_sort_scattered_varying_bits(mask):
     np.testing.assert_array_equal(out.triangle_indices.cpu().numpy(), values[order])
 
 
+@pytest.mark.parametrize("depths", ["few", "one", "uniform"])
+def test_tile_bucketed_sort_equals_full_sort(depths):
+    """mt_sort_keys_high over the (camera, tile) bits, then mt_sort_key_ranges over each
+    tile's range by the (depth, shell) bits, gives exactly the stable sort of the whole
+    keys (mt_sort_keys' order). Ranges of every size class: up to 4096 keys (one small
+    workgroup), up to 16384 (one large workgroup), longer (through the temporary arrays),
+    with both signs of the key, ties, one, few or many varying low bits (odd and even pass
+    counts), empty ranges and slots that hold no keys."""
+    from meshtryoshka_amd import _native
+
+    rng = np.random.default_rng({"few": 1, "one": 2, "uniform": 3}[depths])
+    lay = orr.fused_layout()
+    sizes = [1, 5, 300, 4095, 4096, 4097, 9000, 16383, 16384, 16385, 40000, 70001]
+    sizes += [int(x) for x in rng.integers(1, 3000, 120)]
+    cam, tile = [], []
+    for i, m in enumerate(sizes):
+        cam.append(np.full(m, (i * 13) % 64))  # cameras >= 32 set the key's sign bit
+        tile.append(np.full(m, 1 + i * 7))
+    cam, tile = np.concatenate(cam), np.concatenate(tile)
+    n = cam.size
+    if depths == "few":
+        depth = rng.choice(np.float32([0.5, 1.25, 2.0, 2.0000002, 3.75]), n)
+    elif depths == "one":
+        depth = np.full(n, 1.5, np.float32)
+    else:
+        depth = rng.uniform(0.5, 4.0, n).astype(np.float32)
+    shell = rng.integers(0, 5, n) if depths != "one" else np.zeros(n, np.int64)
+    keys = orr.pack_keys(lay, cam, shell, tile, depth)
+    perm = rng.permutation(n)
+    keys = keys[perm]
+    values = rng.permutation(n).astype(np.int32)
+    order = np.argsort(keys, kind="stable")
+    low = lay["tile"][0]  # camera | tile above, depth | shell below
+    st = torch.cuda.current_stream().cuda_stream
+    k_in, v_in = to_dev(keys), to_dev(values)
+    k_out = torch.empty_like(k_in)
+    v_out = torch.empty_like(v_in)
+    ws = _native.workspace(_native.query("mt_sort_workspace_bytes", n), k_in.device)
+    _native.call("mt_sort_keys_high", k_in.data_ptr(), v_in.data_ptr(), n, k_out.data_ptr(),
+                 v_out.data_ptr(), ws.data_ptr(), ws.numel(), None, low, st)
+    # Grouped by the high bits, in input order inside each group.
+    hi = k_out.cpu().numpy() >> np.int64(low)
+    want_hi = keys[np.argsort(keys >> np.int64(low), kind="stable")]
+    np.testing.assert_array_equal(k_out.cpu().numpy(), want_hi)
+    # Ranges of equal high bits, listed in a shuffled slot table with empty slots.
+    starts = np.flatnonzero(np.r_[True, hi[1:] != hi[:-1]])
+    ends = np.r_[starts[1:], n]
+    table = np.zeros((starts.size + 40, 2), np.int64)
+    slots = rng.permutation(table.shape[0])[: starts.size]
+    table[slots, 0], table[slots, 1] = starts, ends
+    tmp_k = torch.full_like(k_in, -1)
+    tmp_v = torch.full_like(v_in, -1)
+    _native.call("mt_sort_key_ranges", k_out.data_ptr(), v_out.data_ptr(), n,
+                 to_dev(table).data_ptr(), table.shape[0], low, tmp_k.data_ptr(), tmp_v.data_ptr(), st)
+    np.testing.assert_array_equal(k_out.cpu().numpy(), keys[order])
+    np.testing.assert_array_equal(v_out.cpu().numpy(), values[order])
+    # Overlapping scratch is refused on the host.
+    with pytest.raises(RuntimeError, match="separate"):
+        _native.call("mt_sort_key_ranges", k_out.data_ptr(), v_out.data_ptr(), n,
+                     to_dev(table).data_ptr(), table.shape[0], low, k_out.data_ptr(), tmp_v.data_ptr(), st)
+
+
 def test_spherical_harmonics(scene):
     t = scene.tri
     verts = to_dev(t.vertices)
