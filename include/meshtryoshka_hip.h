@@ -485,15 +485,20 @@ int mt_sort_keys_high(const int64_t* keys_in, const int32_t* values_in, int64_t 
                       mt_stream_t stream);
 
 /* Tile-bucketed sort, part 2: stably sorts, in place, each range [ranges[2r], ranges[2r+1])
- * (r < num_ranges; disjoint; empty ranges allowed) of keys / values by the key bits
- * [0, low_bits) (unsigned), 1 <= low_bits <= 63. A range of up to 4096 keys is sorted by one
- * 256-thread workgroup and up to 16384 keys by a 1024-thread workgroup with the keys in
- * registers and LDS (radix passes of 8 bits over the bits that vary inside the range);
- * longer ranges pass through tmp_keys / tmp_values (n entries each, separate arrays, their
- * contents overwritten). */
+ * (r < num_ranges < 2^31; disjoint; empty ranges allowed) of keys / values by the key bits
+ * [0, low_bits) (unsigned), 1 <= low_bits <= 63. A first kernel lists the ranges by size;
+ * ranges of up to 1024, 4096, 8192 and 16384 keys are then sorted by one workgroup of 64,
+ * 256, 512 and 1024 threads with the keys in registers and LDS (radix passes of 8 bits over
+ * the bits that vary inside the range); longer ranges pass through tmp_keys / tmp_values
+ * (n entries each, separate arrays, their contents overwritten). workspace: at least
+ * mt_sort_key_ranges_workspace_bytes(num_ranges) bytes (the size lists; no initial
+ * contents needed). No counterpart in the reference (its sort_keys.cu:215-270 sorts the
+ * whole key at once); part 1 and part 2 together replace it. */
+size_t mt_sort_key_ranges_workspace_bytes(int64_t num_ranges);
 int mt_sort_key_ranges(int64_t* keys, int32_t* values, int64_t n, const int64_t* ranges,
                        int64_t num_ranges, int32_t low_bits, int64_t* tmp_keys,
-                       int32_t* tmp_values, mt_stream_t stream);
+                       int32_t* tmp_values, void* workspace, size_t workspace_bytes,
+                       mt_stream_t stream);
 
 /* [start, end) of every (camera[, shell], tile) run of the sorted keys into the
  * zero-initialised int64[B, S', T, 2] (S' = S if separate_shells else 1). If n_device

@@ -167,7 +167,8 @@ SIGNATURES = {
     "mt_sort_workspace_bytes": (c_sz, [c_i64]),
     "mt_sort_keys": (c_i32, [c_p, c_p, c_i64, c_p, c_p, c_p, c_sz, c_p, c_p]),
     "mt_sort_keys_high": (c_i32, [c_p, c_p, c_i64, c_p, c_p, c_p, c_sz, c_p, c_i32, c_p]),
-    "mt_sort_key_ranges": (c_i32, [c_p, c_p, c_i64, c_p, c_i64, c_i32, c_p, c_p, c_p]),
+    "mt_sort_key_ranges_workspace_bytes": (c_sz, [c_i64]),
+    "mt_sort_key_ranges": (c_i32, [c_p, c_p, c_i64, c_p, c_i64, c_i32, c_p, c_p, c_p, c_sz, c_p]),
     "mt_delineate_tiles": (c_i32, [c_p, c_i64, KeyLayout, c_i32, c_i32, c_i32, c_i32, c_p, c_p, c_p,
                                    c_p]),
     "mt_camera_positions": (c_i32, [c_p, c_i32, c_p, c_p]),

@@ -190,6 +190,8 @@ class BatchedTrainingStep:
         self.keys_ws = _native.workspace(
             _native.query("mt_generate_keys_single_pass_workspace_bytes", F, G), dev)
         self.sort_ws = _native.workspace(_native.query("mt_sort_workspace_bytes", K), dev)
+        self.range_ws = _native.workspace(
+            _native.query("mt_sort_key_ranges_workspace_bytes", self.ranges.shape[0]), dev)
         self.vbwd_ws = _native.workspace(
             _native.query("mt_create_triangle_vertices_colors_backward_workspace_bytes",
                           self.n_grid, self.n_col, sh_degree), dev)
@@ -340,7 +342,7 @@ class BatchedTrainingStep:
                  g["tile_base"].data_ptr() if table else None, st)
             call("mt_sort_key_ranges", self.sorted_keys.data_ptr(), self.sorted_values.data_ptr(), K,
                  self.ranges.data_ptr(), g["tiles"], low, self.keys.data_ptr(),
-                 self.values.data_ptr(), st)
+                 self.values.data_ptr(), self.range_ws.data_ptr(), self.range_ws.numel(), st)
             args = self._composite_args(g)
             p0 = self.pixel_first[c0]
             out = self.out_flat[4 * p0 :]

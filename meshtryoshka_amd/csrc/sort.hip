@@ -528,7 +528,8 @@ struct RangeSortLds {
 template <int kThreads, int kIt>
 __global__ void __launch_bounds__(kThreads) range_sort_kernel(int64_t* keys, int32_t* values,
                                                               const int64_t* ranges,
-                                                              int64_t num_ranges, int64_t n,
+                                                              const int32_t* list,
+                                                              const int32_t* list_count, int64_t n,
                                                               int64_t min_size, uint64_t low_mask) {
   constexpr int kCap = kThreads * kIt;
   constexpr int kPerWave = kIt * kWave;
@@ -536,7 +537,9 @@ __global__ void __launch_bounds__(kThreads) range_sort_kernel(int64_t* keys, int
   auto& L = *reinterpret_cast<RangeSortLds<kThreads, kIt>*>(s_dyn);
   const int wave = threadIdx.x / kWave;
   const int lane = __lane_id();
-  for (int64_t r = blockIdx.x; r < num_ranges; r += gridDim.x) {
+  const int32_t listed = *list_count;
+  for (int32_t li = blockIdx.x; li < listed; li += gridDim.x) {
+    const int64_t r = list[li];
     int64_t start = ranges[2 * r], end = ranges[2 * r + 1];
     start = start < 0 ? 0 : (start > n ? n : start);
     end = end < start ? start : (end > n ? n : end);
@@ -612,7 +615,8 @@ __global__ void __launch_bounds__(kThreads) range_sort_kernel(int64_t* keys, int
 template <int kThreads, int kIt>
 __global__ void __launch_bounds__(kThreads) long_range_sort_kernel(
     int64_t* keys, int32_t* values, int64_t* tmp_keys, int32_t* tmp_values,
-    const int64_t* ranges, int64_t num_ranges, int64_t n, int64_t min_size, uint64_t low_mask) {
+    const int64_t* ranges, const int32_t* list, const int32_t* list_count, int64_t n,
+    int64_t min_size, uint64_t low_mask) {
   constexpr int kCap = kThreads * kIt;
   constexpr int kPerWave = kIt * kWave;
   __shared__ BlockRank<kThreads, kIt> R;
@@ -623,7 +627,9 @@ __global__ void __launch_bounds__(kThreads) long_range_sort_kernel(
   __shared__ int s_passes;
   const int wave = threadIdx.x / kWave;
   const int lane = __lane_id();
-  for (int64_t r = blockIdx.x; r < num_ranges; r += gridDim.x) {
+  const int32_t listed = *list_count;
+  for (int32_t li = blockIdx.x; li < listed; li += gridDim.x) {
+    const int64_t r = list[li];
     int64_t start = ranges[2 * r], end = ranges[2 * r + 1];
     start = start < 0 ? 0 : (start > n ? n : start);
     end = end < start ? start : (end > n ? n : end);
@@ -700,47 +706,114 @@ __global__ void __launch_bounds__(kThreads) long_range_sort_kernel(
 
 
 
+// Size classes of the range sort (keys per range): up to 1024, 4096, 8192, 16384, longer.
+constexpr int kRangeClasses = 5;
+constexpr int64_t kClassMax[kRangeClasses - 1] = {1024, 4096, 8192, 16384};
+
+// Lists every range of two or more keys in its size class (one atomic per wave and class),
+// so that each class's kernel visits only its own ranges.
+__global__ void __launch_bounds__(256) classify_ranges_kernel(const int64_t* ranges,
+                                                              int64_t num_ranges, int64_t n,
+                                                              int32_t* lists, int32_t* counts) {
+  const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int cls = -1;
+  if (r < num_ranges) {
+    int64_t start = ranges[2 * r], end = ranges[2 * r + 1];
+    start = start < 0 ? 0 : (start > n ? n : start);
+    end = end < start ? start : (end > n ? n : end);
+    const int64_t m = end - start;
+    if (m >= 2) {
+      cls = kRangeClasses - 1;
+      for (int c = kRangeClasses - 2; c >= 0; --c)
+        if (m <= kClassMax[c]) cls = c;
+    }
+  }
+  const int lane = __lane_id();
+#pragma unroll
+  for (int c = 0; c < kRangeClasses; ++c) {
+    const uint64_t m = __ballot(cls == c);
+    if (m == 0) continue;
+    int at = 0;
+    if (lane == 0) at = atomicAdd(&counts[c], __popcll(m));
+    at = __shfl(at, 0, kWave);
+    if (cls == c) lists[c * num_ranges + at + __popcll(m & lanemask_lt())] = static_cast<int32_t>(r);
+  }
+}
+
+__global__ void zero_class_counts_kernel(int32_t* counts) {
+  if (threadIdx.x < kRangeClasses) counts[threadIdx.x] = 0;
+}
+
 }  // namespace
 }  // namespace mt
 
+extern "C" size_t mt_sort_key_ranges_workspace_bytes(int64_t num_ranges) {
+  const size_t r = static_cast<size_t>(num_ranges < 1 ? 1 : num_ranges);
+  return 256 + (r * 4 * mt::kRangeClasses + 255) / 256 * 256;
+}
+
 extern "C" int mt_sort_key_ranges(int64_t* keys, int32_t* values, int64_t n,
                                   const int64_t* ranges, int64_t num_ranges, int32_t low_bits,
-                                  int64_t* tmp_keys, int32_t* tmp_values, mt_stream_t stream) {
+                                  int64_t* tmp_keys, int32_t* tmp_values, void* workspace,
+                                  size_t workspace_bytes, mt_stream_t stream) {
   using namespace mt;
   MT_REQUIRE(n >= 0 && num_ranges >= 0, "mt_sort_key_ranges: negative size");
+  MT_REQUIRE(num_ranges < (int64_t(1) << 31), "mt_sort_key_ranges: too many ranges");
   MT_REQUIRE(low_bits >= 0 && low_bits <= 63, "mt_sort_key_ranges: low_bits %d (0..63)", low_bits);
   if (n == 0 || num_ranges == 0 || low_bits == 0) return MT_OK;
   MT_REQUIRE(keys && values && ranges && tmp_keys && tmp_values, "mt_sort_key_ranges: null pointer");
   MT_REQUIRE(tmp_keys != keys && tmp_values != values,
              "mt_sort_key_ranges: the temporary arrays must be separate from the keys and values");
+  const size_t need = mt_sort_key_ranges_workspace_bytes(num_ranges);
+  if (workspace == nullptr || workspace_bytes < need) {
+    set_error("mt_sort_key_ranges: workspace of %zu bytes is smaller than the %zu required",
+              workspace_bytes, need);
+    return MT_ERR_WORKSPACE_TOO_SMALL;
+  }
   hipStream_t s = static_cast<hipStream_t>(stream);
   const uint64_t low_mask = (1ull << low_bits) - 1ull;
-  // (threads, items): (64, 16) up to 1024 keys, (256, 16) up to 4096, (512, 16) up to
-  // 8192 (75 KB of LDS, two workgroups per CU), (1024, 16) up to 16384 (146 KB); longer
-  // ranges through the temporary arrays.
+  int32_t* counts = static_cast<int32_t*>(workspace);
+  int32_t* lists = reinterpret_cast<int32_t*>(static_cast<char*>(workspace) + 256);
+  zero_class_counts_kernel<<<1, 64, 0, s>>>(counts);
+  classify_ranges_kernel<<<blocks_for(num_ranges, 256), 256, 0, s>>>(ranges, num_ranges, n, lists,
+                                                                      counts);
+  // (threads, items) per class: (64, 16), (256, 16), (512, 16) (75 KB of LDS, two
+  // workgroups per CU), (1024, 16) (146 KB); longer ranges through the temporary arrays.
   const size_t lds_a = sizeof(RangeSortLds<64, 16>), lds_b = sizeof(RangeSortLds<256, 16>);
   const size_t lds_c = sizeof(RangeSortLds<512, 16>), lds_d = sizeof(RangeSortLds<1024, 16>);
   static bool attributes_set = false;
   if (!attributes_set) {
-    hipFuncSetAttribute(reinterpret_cast<const void*>(&range_sort_kernel<512, 16>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_c));
-    hipFuncSetAttribute(reinterpret_cast<const void*>(&range_sort_kernel<1024, 16>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_d));
+    const hipError_t e1 =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&range_sort_kernel<512, 16>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_c));
+    const hipError_t e2 =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&range_sort_kernel<1024, 16>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_d));
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      set_error("mt_sort_key_ranges: cannot allow %zu bytes of LDS: %s", lds_d,
+                hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+      return MT_ERR_HIP;
+    }
     attributes_set = true;
   }
+  // Each class's workgroups loop over its list (its length is read on the device); the
+  // grids cover the GPU a few times over.
   const auto grid = [&](int64_t cap) {
     return static_cast<unsigned>(num_ranges < cap ? num_ranges : cap);
   };
-  range_sort_kernel<64, 16><<<grid(1 << 20), 64, lds_a, s>>>(keys, values, ranges, num_ranges, n, 1,
-                                                              low_mask);
-  range_sort_kernel<256, 16><<<grid(1 << 18), 256, lds_b, s>>>(keys, values, ranges, num_ranges, n,
-                                                               1024, low_mask);
-  range_sort_kernel<512, 16><<<grid(8192), 512, lds_c, s>>>(keys, values, ranges, num_ranges, n,
-                                                             4096, low_mask);
-  range_sort_kernel<1024, 16><<<grid(4096), 1024, lds_d, s>>>(keys, values, ranges, num_ranges, n,
-                                                              8192, low_mask);
-  long_range_sort_kernel<1024, 8><<<grid(4096), 1024, 0, s>>>(
-      keys, values, tmp_keys, tmp_values, ranges, num_ranges, n, 16384, low_mask);
+  const int32_t* list = lists;
+  const int64_t stride = num_ranges;
+  range_sort_kernel<64, 16><<<grid(8192), 64, lds_a, s>>>(keys, values, ranges, list, counts + 0, n,
+                                                           1, low_mask);
+  range_sort_kernel<256, 16><<<grid(2048), 256, lds_b, s>>>(keys, values, ranges, list + stride,
+                                                            counts + 1, n, kClassMax[0], low_mask);
+  range_sort_kernel<512, 16><<<grid(1024), 512, lds_c, s>>>(keys, values, ranges, list + 2 * stride,
+                                                            counts + 2, n, kClassMax[1], low_mask);
+  range_sort_kernel<1024, 16><<<grid(512), 1024, lds_d, s>>>(keys, values, ranges, list + 3 * stride,
+                                                             counts + 3, n, kClassMax[2], low_mask);
+  long_range_sort_kernel<1024, 8><<<grid(512), 1024, 0, s>>>(
+      keys, values, tmp_keys, tmp_values, ranges, list + 4 * stride, counts + 4, n, kClassMax[3],
+      low_mask);
   return check_launch("mt_sort_key_ranges");
 }
 

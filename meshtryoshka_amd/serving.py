@@ -347,6 +347,7 @@ def _batch_plan(scene: ResidentScene, shapes, tile_shape, k_cap):
         ("keys_ws", _native.query("mt_generate_keys_single_pass_workspace_bytes",
                                   scene.faces.shape[0], b), u8),
         ("sort_ws", _native.query("mt_sort_workspace_bytes", k_cap), u8),
+        ("range_ws", _native.query("mt_sort_key_ranges_workspace_bytes", tile_base[-1]), u8),
     ]
     return dims, tile_base, pixel_base, plan
 
@@ -401,7 +402,7 @@ def render_batch(scene: ResidentScene, extrinsics: torch.Tensor, intrinsics: tor
     t["ranges"].zero_()
     xy, depth, colors, out = t["xy"], t["depth"], t["colors"], t["out"]
     keys, vals, skeys, svals = t["keys"], t["vals"], t["skeys"], t["svals"]
-    keys_ws, sort_ws, n_dev = t["keys_ws"], t["sort_ws"], t["n_keys"]
+    keys_ws, sort_ws, range_ws, n_dev = t["keys_ws"], t["sort_ws"], t["range_ws"], t["n_keys"]
     live_x3, overflow, ranges, campos = t["live_x3"], t["overflow"], t["ranges"], t["campos"]
     dims_p = cam_dims.data_ptr() if table else None
     call("mt_project_vertices", scene.vertices.data_ptr(), v, E.data_ptr(), K.data_ptr(), b,
@@ -420,7 +421,8 @@ def render_batch(scene: ResidentScene, extrinsics: torch.Tensor, intrinsics: tor
     call("mt_delineate_tiles", skeys.data_ptr(), k_cap, _LAYOUT, 0, b, 1, rows * cols,
          ranges.data_ptr(), n_dev.data_ptr(), tbase.data_ptr() if table else None, st)
     call("mt_sort_key_ranges", skeys.data_ptr(), svals.data_ptr(), k_cap, ranges.data_ptr(),
-         tile_base[-1], low, keys.data_ptr(), vals.data_ptr(), st)
+         tile_base[-1], low, keys.data_ptr(), vals.data_ptr(), range_ws.data_ptr(),
+         range_ws.numel(), st)
     call("mt_camera_positions", E.data_ptr(), b, campos.data_ptr(), st)
     if scene.grid is None:
         call("mt_evaluate_sh_forward", scene.vertices.data_ptr(), v,

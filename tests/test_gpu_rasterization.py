@@ -168,15 +168,17 @@ def test_sort_scattered_varying_bits(mask):
 def test_tile_bucketed_sort_equals_full_sort(depths):
     """mt_sort_keys_high over the (camera, tile) bits, then mt_sort_key_ranges over each
     tile's range by the (depth, shell) bits, gives exactly the stable sort of the whole
-    keys (mt_sort_keys' order). Ranges of every size class: up to 4096 keys (one small
-    workgroup), up to 16384 (one large workgroup), longer (through the temporary arrays),
+    keys (mt_sort_keys' order). Ranges of every size class: up to 1024, 4096, 8192 and
+    16384 keys (one workgroup of 64, 256, 512 or 1024 threads), longer (through the
+    temporary arrays), each class's bounds on both sides,
     with both signs of the key, ties, one, few or many varying low bits (odd and even pass
     counts), empty ranges and slots that hold no keys."""
     from meshtryoshka_amd import _native
 
     rng = np.random.default_rng({"few": 1, "one": 2, "uniform": 3}[depths])
     lay = orr.fused_layout()
-    sizes = [1, 5, 300, 4095, 4096, 4097, 9000, 16383, 16384, 16385, 40000, 70001]
+    sizes = [1, 2, 5, 300, 1024, 1025, 4095, 4096, 4097, 8192, 8193, 9000, 16383, 16384, 16385,
+             40000, 70001]
     sizes += [int(x) for x in rng.integers(1, 3000, 120)]
     cam, tile = [], []
     for i, m in enumerate(sizes):
@@ -216,14 +218,22 @@ def test_tile_bucketed_sort_equals_full_sort(depths):
     table[slots, 0], table[slots, 1] = starts, ends
     tmp_k = torch.full_like(k_in, -1)
     tmp_v = torch.full_like(v_in, -1)
-    _native.call("mt_sort_key_ranges", k_out.data_ptr(), v_out.data_ptr(), n,
-                 to_dev(table).data_ptr(), table.shape[0], low, tmp_k.data_ptr(), tmp_v.data_ptr(), st)
+    rbytes = _native.query("mt_sort_key_ranges_workspace_bytes", table.shape[0])
+    rws = torch.full((rbytes,), 0xA5, dtype=torch.uint8, device=k_in.device)  # no initial contents
+    t_dev = to_dev(table)
+    _native.call("mt_sort_key_ranges", k_out.data_ptr(), v_out.data_ptr(), n, t_dev.data_ptr(),
+                 table.shape[0], low, tmp_k.data_ptr(), tmp_v.data_ptr(), rws.data_ptr(), rbytes, st)
     np.testing.assert_array_equal(k_out.cpu().numpy(), keys[order])
     np.testing.assert_array_equal(v_out.cpu().numpy(), values[order])
     # Overlapping scratch is refused on the host.
     with pytest.raises(RuntimeError, match="separate"):
-        _native.call("mt_sort_key_ranges", k_out.data_ptr(), v_out.data_ptr(), n,
-                     to_dev(table).data_ptr(), table.shape[0], low, k_out.data_ptr(), tmp_v.data_ptr(), st)
+        _native.call("mt_sort_key_ranges", k_out.data_ptr(), v_out.data_ptr(), n, t_dev.data_ptr(),
+                     table.shape[0], low, k_out.data_ptr(), tmp_v.data_ptr(), rws.data_ptr(), rbytes,
+                     st)
+    with pytest.raises(RuntimeError, match="workspace"):
+        _native.call("mt_sort_key_ranges", k_out.data_ptr(), v_out.data_ptr(), n, t_dev.data_ptr(),
+                     table.shape[0], low, tmp_k.data_ptr(), tmp_v.data_ptr(), rws.data_ptr(),
+                     rbytes - 1, st)
 
 
 def test_spherical_harmonics(scene):
