@@ -317,6 +317,7 @@ def algorithmic_bytes(z):
         "mt_count_triangle_vertices_axes": n_lc * 16 + n_lc * 4 + 5 * s * n_lc,
         "mt_create_triangle_faces_axes_i64": n_vox * 28 + s * n_vox * (1 + 8) + 5 * s * n_lc + f * 12,
         "mt_project_vertices": g * vt * 12 + b * vt * 12,
+        "mt_project_vertices_live": g * vt * 12 + b * vt * 12,  # live vertices only
         "mt_bound_triangles": f * 12 + b * vt * 12 + b * f * 20,
         "mt_inclusive_scan_i32_to_i64": 12 * blocks,  # over (camera, 256-face block)
         "mt_key_block_counts": g * f * 12 + b * vt * 12 + 4 * blocks,

@@ -363,6 +363,15 @@ int mt_project_vertices(const float* vertices, int64_t V, const float* extrinsic
                         const float* intrinsics, int32_t B, float* out_xy, float* out_depth,
                         mt_stream_t stream);
 
+/* mt_project_vertices over the first min(*num_live, V) vertices only (num_live: int32 on
+ * the device, e.g. the last entry of a vertex-offset scan); the outputs keep the row stride
+ * V and their entries past the live count are not written. For callers whose vertex
+ * arrays have a capacity above the live count (the batched training step); the reference
+ * projects exactly its V vertices (project_vertices.py:31-56). */
+int mt_project_vertices_live(const float* vertices, int64_t V, const int32_t* num_live,
+                             const float* extrinsics, const float* intrinsics, int32_t B,
+                             float* out_xy, float* out_depth, mt_stream_t stream);
+
 /* Back-face / near-plane culling and tile bounds. faces int32[F, 3]; outputs
  * tile_min int32[B, F, 2] (row, col) inclusive, tile_max int32[B, F, 2] exclusive,
  * counts int32[B, F]. If num_faces_x3 is not NULL, only faces below (*num_faces_x3) / 3

@@ -137,6 +137,7 @@ SIGNATURES = {
         c_i32, [c_p, c_i64, c_i64, c_p, c_p, c_p, c_i32, c_p, c_p, c_i64, c_i64, c_p, c_p]
     ),
     "mt_project_vertices": (c_i32, [c_p, c_i64, c_p, c_p, c_i32, c_p, c_p, c_p]),
+    "mt_project_vertices_live": (c_i32, [c_p, c_i64, c_p, c_p, c_p, c_i32, c_p, c_p, c_p]),
     "mt_bound_triangles": (
         c_i32,
         [c_p, c_p, c_i64, c_p, c_i64, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_f32,

@@ -315,7 +315,9 @@ class BatchedTrainingStep:
                     hook("face_boundaries")
             face_boundaries = self.face_boundaries
             # --- rasterization of the group's cameras
-            call("mt_project_vertices", self.tv.data_ptr(), V, self.extrinsics[c0:].data_ptr(),
+            # Live vertices only (the arrays' capacity has a margin above the vertex count).
+            call("mt_project_vertices_live", self.tv.data_ptr(), V,
+                 self.vertex_offsets[S * self.n_col :].data_ptr(), self.extrinsics[c0:].data_ptr(),
                  self.intrinsics[c0:].data_ptr(), n, self.xy.data_ptr(), self.depth.data_ptr(), st)
             if overlap:
                 main.wait_stream(self.side)

@@ -33,19 +33,61 @@ __device__ __forceinline__ Projected project_point(const float* e, const float* 
   return Projected{px / depth, py / depth, depth};
 }
 
-// One thread per (vertex, camera); the B blocks of a vertex range are consecutive, so the
-// vertices are read from HBM once and from L2 by the other cameras' blocks.
+// Four consecutive vertices per thread, every camera: the vertices are read once (three
+// 16-byte loads), and each camera's xy and depth of the four are written with 16-byte
+// stores where the output offset allows (8-byte or 4-byte stores otherwise; vec_in /
+// vec_out: the vertex / output arrays are 16-byte aligned). Vertices at or past *num_live
+// (when given) are neither read nor written.
 __global__ void __launch_bounds__(kBlock) project_kernel(const float* verts, int64_t V,
-                                                         const float* E, const float* K, int B,
-                                                         float* out_xy, float* out_d) {
-  const int b = static_cast<int>(blockIdx.x % static_cast<unsigned>(B));
-  const int64_t v = int64_t(blockIdx.x / static_cast<unsigned>(B)) * blockDim.x + threadIdx.x;
-  if (v >= V) return;
-  const float x = verts[3 * v + 0], y = verts[3 * v + 1], z = verts[3 * v + 2];
-  const Projected p = project_point(E + 16 * b, K + 9 * b, x, y, z);
-  const int64_t o = int64_t(b) * V + v;
-  reinterpret_cast<float2*>(out_xy)[o] = make_float2(p.x, p.y);
-  if (out_d != nullptr) out_d[o] = p.depth;
+                                                         const int32_t* num_live, const float* E,
+                                                         const float* K, int B, float* out_xy,
+                                                         float* out_d, bool vec_in, bool vec_out) {
+  const int64_t v0 = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+  int64_t n = V;
+  if (num_live != nullptr) {
+    const int64_t live = *num_live;
+    n = live < n ? (live < 0 ? 0 : live) : n;
+  }
+  if (v0 >= n) return;
+  const bool full = v0 + 4 <= n;
+  float c[12];
+  if (full && vec_in) {
+    const float4* src = reinterpret_cast<const float4*>(verts + 3 * v0);  // 48 * (v0 / 4) bytes
+    const float4 a = src[0], bb = src[1], cc = src[2];
+    const float e[12] = {a.x, a.y, a.z, a.w, bb.x, bb.y, bb.z, bb.w, cc.x, cc.y, cc.z, cc.w};
+#pragma unroll
+    for (int i = 0; i < 12; ++i) c[i] = e[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) c[i] = v0 + i / 3 < n ? verts[3 * v0 + i] : 0.0f;
+  }
+  for (int b = 0; b < B; ++b) {
+    Projected p[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      p[i] = project_point(E + 16 * b, K + 9 * b, c[3 * i], c[3 * i + 1], c[3 * i + 2]);
+    const int64_t o = int64_t(b) * V + v0;
+    if (full && vec_out && (o & 1) == 0) {  // 16-byte aligned xy
+      float4* xy4 = reinterpret_cast<float4*>(out_xy + 2 * o);
+      xy4[0] = make_float4(p[0].x, p[0].y, p[1].x, p[1].y);
+      xy4[1] = make_float4(p[2].x, p[2].y, p[3].x, p[3].y);
+      if (out_d == nullptr) continue;
+      if ((o & 3) == 0) {
+        *reinterpret_cast<float4*>(out_d + o) =
+            make_float4(p[0].depth, p[1].depth, p[2].depth, p[3].depth);
+      } else {
+        *reinterpret_cast<float2*>(out_d + o) = make_float2(p[0].depth, p[1].depth);
+        *reinterpret_cast<float2*>(out_d + o + 2) = make_float2(p[2].depth, p[3].depth);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (v0 + i >= n) break;
+        reinterpret_cast<float2*>(out_xy)[o + i] = make_float2(p[i].x, p[i].y);
+        if (out_d != nullptr) out_d[o + i] = p[i].depth;
+      }
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------
@@ -985,17 +1027,41 @@ __global__ void __launch_bounds__(kBlock) sh_backward_kernel(const float* verts,
 }  // namespace
 }  // namespace mt
 
+namespace mt {
+namespace {
+int project_vertices(const char* name, const float* vertices, int64_t V, const int32_t* num_live,
+                     const float* extrinsics, const float* intrinsics, int32_t B, float* out_xy,
+                     float* out_depth, mt_stream_t stream) {
+  MT_REQUIRE(V >= 0 && B >= 0, "%s: negative size", name);
+  if (V == 0 || B == 0) return MT_OK;
+  MT_REQUIRE(vertices && extrinsics && intrinsics && out_xy, "%s: null pointer", name);
+  const bool vec_in = (reinterpret_cast<uintptr_t>(vertices) & 15) == 0;
+  const bool vec_out = (reinterpret_cast<uintptr_t>(out_xy) & 15) == 0 &&
+                       (reinterpret_cast<uintptr_t>(out_depth) & 15) == 0;
+  const int64_t blocks = ((V + 3) / 4 + kBlock - 1) / kBlock;
+  MT_REQUIRE(blocks <= INT32_MAX, "%s: %lld vertices are too many for one launch", name,
+             static_cast<long long>(V));
+  project_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+      vertices, V, num_live, extrinsics, intrinsics, B, out_xy, out_depth, vec_in, vec_out);
+  return check_launch(name);
+}
+}  // namespace
+}  // namespace mt
+
 extern "C" int mt_project_vertices(const float* vertices, int64_t V, const float* extrinsics,
                                    const float* intrinsics, int32_t B, float* out_xy,
                                    float* out_depth, mt_stream_t stream) {
-  MT_REQUIRE(V >= 0 && B >= 0, "mt_project_vertices: negative size");
-  if (V == 0 || B == 0) return MT_OK;
-  const int64_t blocks = int64_t(mt::blocks_for(V, mt::kBlock)) * B;
-  MT_REQUIRE(blocks <= INT32_MAX, "mt_project_vertices: %lld vertices x %d cameras is too many for one launch",
-             static_cast<long long>(V), B);
-  mt::project_kernel<<<static_cast<unsigned>(blocks), mt::kBlock, 0, static_cast<hipStream_t>(stream)>>>(
-      vertices, V, extrinsics, intrinsics, B, out_xy, out_depth);
-  return mt::check_launch("mt_project_vertices");
+  return mt::project_vertices("mt_project_vertices", vertices, V, nullptr, extrinsics, intrinsics,
+                              B, out_xy, out_depth, stream);
+}
+
+extern "C" int mt_project_vertices_live(const float* vertices, int64_t V, const int32_t* num_live,
+                                        const float* extrinsics, const float* intrinsics,
+                                        int32_t B, float* out_xy, float* out_depth,
+                                        mt_stream_t stream) {
+  MT_REQUIRE(num_live != nullptr, "mt_project_vertices_live: num_live is NULL");
+  return mt::project_vertices("mt_project_vertices_live", vertices, V, num_live, extrinsics,
+                              intrinsics, B, out_xy, out_depth, stream);
 }
 
 extern "C" int mt_bound_triangles(const float* xy, const float* depths, int64_t V,

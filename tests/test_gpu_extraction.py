@@ -151,12 +151,13 @@ def test_faces_from_crossing_axes_match_oracle(scene):
     nv = int(voff[-1])
     faces = torch.full((nf, 3), -7, dtype=torch.int32, device="cuda")
     flag = torch.zeros(1, dtype=torch.int32, device="cuda")
-    _native.call("mt_create_triangle_faces_axes_i64", to_dev(v.neighbors).data_ptr(), nvox, n,
+    nbr = to_dev(v.neighbors)  # kept alive across the calls
+    _native.call("mt_create_triangle_faces_axes_i64", nbr.data_ptr(), nvox, n,
                  voff.data_ptr(), axes.data_ptr(), codes.data_ptr(), s, foff.data_ptr(),
                  faces.data_ptr(), nf, nv, flag.data_ptr(), st)
     assert int(flag) == 0
     np.testing.assert_array_equal(faces.cpu().numpy(), scene.tri.faces)
-    _native.call("mt_create_triangle_faces_axes_i64", to_dev(v.neighbors).data_ptr(), nvox, n,
+    _native.call("mt_create_triangle_faces_axes_i64", nbr.data_ptr(), nvox, n,
                  voff.data_ptr(), axes.data_ptr(), codes.data_ptr(), s, foff.data_ptr(),
                  faces.data_ptr(), nf, nv // 2, flag.data_ptr(), st)
     assert int(flag) == 1

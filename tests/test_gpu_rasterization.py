@@ -57,6 +57,46 @@ def _front_end(sc, explicit=False):
     return xy, depth, b, pk, spk, ranges
 
 
+@pytest.mark.parametrize("v_count,live,shift_in,shift_out", [
+    (1001, None, 0, 0), (1001, 777, 1, 0), (1000, 998, 0, 1), (3, 2, 1, 1), (4096, 4096, 0, 0)])
+def test_projection_live_and_unaligned(scene, v_count, live, shift_in, shift_out):
+    """The projection kernel's 16-byte paths and their fallbacks: vertex counts that are not
+    a multiple of four, vertex / output arrays that start off a 16-byte boundary (views one
+    row / one entry into a buffer), odd camera offsets (3 cameras, odd V), and the live-count
+    entry (entries past the live count stay untouched). Bit-exact against the oracle."""
+    rng = np.random.default_rng(v_count + 7 * (live or 0))
+    E = np.tile(scene.E[:1], (3, 1, 1)).astype(np.float32)
+    K = np.tile(scene.K[:1], (3, 1, 1)).astype(np.float32)
+    E[1, :3, 3] += np.float32([0.25, -0.5, 0.125])
+    K[2, 0, 0] *= np.float32(1.5)
+    verts = rng.uniform(-1.0, 1.0, (v_count, 3)).astype(np.float32)
+    want_xy, want_d = orr.project_vertices(verts, E, K)
+    B = E.shape[0]
+    vbuf = torch.zeros((v_count + 1, 3), device="cuda")
+    vbuf[shift_in:shift_in + v_count] = to_dev(verts)
+    vv = vbuf[shift_in:shift_in + v_count]
+    xbuf = torch.full((B * v_count * 2 + 2,), -7.0, device="cuda")
+    dbuf = torch.full((B * v_count + 1,), -7.0, device="cuda")
+    xy = xbuf[2 * shift_out:2 * shift_out + B * v_count * 2]
+    d = dbuf[shift_out:shift_out + B * v_count]
+    st = torch.cuda.current_stream().cuda_stream
+    E_d, K_d = to_dev(E), to_dev(K)  # kept alive until the kernel has read them
+    if live is None:
+        _native.call("mt_project_vertices", vv.data_ptr(), v_count, E_d.data_ptr(), K_d.data_ptr(),
+                     B, xy.data_ptr(), d.data_ptr(), st)
+        n = v_count
+    else:
+        nl = torch.tensor([live], dtype=torch.int32, device="cuda")
+        _native.call("mt_project_vertices_live", vv.data_ptr(), v_count, nl.data_ptr(),
+                     E_d.data_ptr(), K_d.data_ptr(), B, xy.data_ptr(), d.data_ptr(), st)
+        n = live
+    got_xy = xy.view(B, v_count, 2).cpu().numpy()
+    got_d = d.view(B, v_count).cpu().numpy()
+    assert np.array_equal(got_xy[:, :n].view(np.uint32), want_xy[:, :n].view(np.uint32))
+    assert np.array_equal(got_d[:, :n].view(np.uint32), want_d[:, :n].view(np.uint32))
+    assert (got_xy[:, n:] == -7.0).all() and (got_d[:, n:] == -7.0).all()
+
+
 def test_front_end_bit_exact(scene):
     xy, depth, b, pk, spk, ranges = _front_end(scene)
     assert np.array_equal(xy.cpu().numpy().view(np.uint32), scene.xy.view(np.uint32))
@@ -104,8 +144,9 @@ def test_keys_from_geometry_match_oracle(scene):
         keys = torch.full((want_n + 7,), -1, dtype=torch.int64, device="cuda")
         vals = torch.full((want_n + 7,), -1, dtype=torch.int32, device="cuda")
         over = torch.zeros(1, dtype=torch.int32, device="cuda")
+        fb = to_dev(t.face_boundaries)  # kept alive until the kernel has read it
         _native.call("mt_generate_keys_from_geometry", xy.data_ptr(), depth.data_ptr(), v,
-                     faces.data_ptr(), f, live_x3.data_ptr(), to_dev(t.face_boundaries).data_ptr(),
+                     faces.data_ptr(), f, live_x3.data_ptr(), fb.data_ptr(),
                      scene.S, b, th, tw, 0, 0, scene.rows, scene.cols, float(scene.spec.near_plane),
                      None, 0, offs.data_ptr(), scene.cols, layout, keys.data_ptr(), vals.data_ptr(),
                      keys.numel(), over.data_ptr(), st)
