@@ -308,7 +308,7 @@ def algorithmic_bytes(z):
     blocks = b * -(-f // 256)
     return {
         "mt_count_triangle_vertices": n_lc * 16 + n_lc * 4 + 4 * s * n_lc,
-        "mt_exclusive_scan_i32": 8 * (s * n_lc + 1) + 8 * (s * n_vox + 1),  # both scans
+        "mt_exclusive_scan_i32": 8 * (s * n_lc + 1),  # vertex counts, in place
         "mt_create_triangle_vertices_forward": n_lc * 16 + n_lc * (12 + 4 + nb) + 4 * s * n_lc
         + vt * (12 + 4 + nb + 1),
         "mt_count_triangle_faces": n_vox * 32 + n_lc * 4 + s * n_vox * (4 + 1),
@@ -319,7 +319,7 @@ def algorithmic_bytes(z):
         "mt_project_vertices": g * vt * 12 + b * vt * 12,
         "mt_project_vertices_live": g * vt * 12 + b * vt * 12,  # live vertices only
         "mt_bound_triangles": f * 12 + b * vt * 12 + b * f * 20,
-        "mt_inclusive_scan_i32_to_i64": 12 * blocks,  # over (camera, 256-face block)
+        "mt_inclusive_scan_i32_to_i64": 12 * s * n_vox,  # face counts -> int64 offsets
         "mt_key_block_counts": g * f * 12 + b * vt * 12 + 4 * blocks,
         "mt_generate_keys_from_geometry": g * f * 12 + b * vt * 12 + k * 12 + 8 * blocks,
         # One pass: faces once per group, projected vertices once, keys written once.

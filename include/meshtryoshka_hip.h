@@ -65,11 +65,11 @@ size_t mt_scan_workspace_bytes(int64_t n);
 int mt_exclusive_scan_i32(int32_t* data, int64_t n, void* workspace, size_t workspace_bytes,
                           mt_stream_t stream);
 
-/* Largest n accepted by mt_exclusive_scan_i32_noalloc (2^28 elements = 65536 tiles). */
+/* Largest n accepted by mt_exclusive_scan_i32_noalloc (2^28 elements = 16384 tiles). */
 #define MT_SCAN_NOALLOC_MAX_ELEMENTS (1LL << 28)
 
 /* mt_exclusive_scan_i32 without a workspace and without any device allocation: the
- * look-back state (512 KiB + 256 B) lives in a device array inside the library, one copy
+ * look-back state (128 KiB + 256 B) lives in a device array inside the library, one copy
  * per GPU, held until the process ends. No counterpart in the reference: it is for
  * callers that cannot allocate (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc). n <= MT_SCAN_NOALLOC_MAX_ELEMENTS, else
@@ -496,8 +496,8 @@ int mt_sort_keys_high(const int64_t* keys_in, const int32_t* values_in, int64_t 
 /* Tile-bucketed sort, part 2: stably sorts, in place, each range [ranges[2r], ranges[2r+1])
  * (r < num_ranges < 2^31; disjoint; empty ranges allowed) of keys / values by the key bits
  * [0, low_bits) (unsigned), 1 <= low_bits <= 63. A first kernel lists the ranges by size;
- * ranges of up to 1024, 4096, 8192 and 16384 keys are then sorted by one workgroup of 64,
- * 256, 512 and 1024 threads with the keys in registers and LDS (radix passes of 8 bits over
+ * ranges of up to 1024, 2048, 4096, 8192 and 16384 keys are then sorted by one workgroup of
+ * 64, 128, 256, 512 and 1024 threads with the keys in registers and LDS (radix passes of 8 bits over
  * the bits that vary inside the range); longer ranges pass through tmp_keys / tmp_values
  * (n entries each, separate arrays, their contents overwritten). workspace: at least
  * mt_sort_key_ranges_workspace_bytes(num_ranges) bytes (the size lists; no initial

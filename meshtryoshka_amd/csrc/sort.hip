@@ -706,9 +706,10 @@ __global__ void __launch_bounds__(kThreads) long_range_sort_kernel(
 
 
 
-// Size classes of the range sort (keys per range): up to 1024, 4096, 8192, 16384, longer.
-constexpr int kRangeClasses = 5;
-constexpr int64_t kClassMax[kRangeClasses - 1] = {1024, 4096, 8192, 16384};
+// Size classes of the range sort (keys per range): up to 1024, 2048, 4096, 8192, 16384,
+// longer.
+constexpr int kRangeClasses = 6;
+constexpr int64_t kClassMax[kRangeClasses - 1] = {1024, 2048, 4096, 8192, 16384};
 
 // Lists every range of two or more keys in its size class (one atomic per wave and class),
 // so that each class's kernel visits only its own ranges.
@@ -777,9 +778,10 @@ extern "C" int mt_sort_key_ranges(int64_t* keys, int32_t* values, int64_t n,
   zero_class_counts_kernel<<<1, 64, 0, s>>>(counts);
   classify_ranges_kernel<<<blocks_for(num_ranges, 256), 256, 0, s>>>(ranges, num_ranges, n, lists,
                                                                       counts);
-  // (threads, items) per class: (64, 16), (256, 16), (512, 16) (75 KB of LDS, two
+  // (threads, items) per class: (64, 16), (128, 16), (256, 16), (512, 16) (75 KB of LDS, two
   // workgroups per CU), (1024, 16) (146 KB); longer ranges through the temporary arrays.
-  const size_t lds_a = sizeof(RangeSortLds<64, 16>), lds_b = sizeof(RangeSortLds<256, 16>);
+  const size_t lds_a = sizeof(RangeSortLds<64, 16>), lds_a2 = sizeof(RangeSortLds<128, 16>);
+  const size_t lds_b = sizeof(RangeSortLds<256, 16>);
   const size_t lds_c = sizeof(RangeSortLds<512, 16>), lds_d = sizeof(RangeSortLds<1024, 16>);
   static bool attributes_set = false;
   if (!attributes_set) {
@@ -801,18 +803,20 @@ extern "C" int mt_sort_key_ranges(int64_t* keys, int32_t* values, int64_t n,
   const auto grid = [&](int64_t cap) {
     return static_cast<unsigned>(num_ranges < cap ? num_ranges : cap);
   };
-  const int32_t* list = lists;
   const int64_t stride = num_ranges;
+  const int32_t* list = lists;
   range_sort_kernel<64, 16><<<grid(8192), 64, lds_a, s>>>(keys, values, ranges, list, counts + 0, n,
                                                            1, low_mask);
-  range_sort_kernel<256, 16><<<grid(2048), 256, lds_b, s>>>(keys, values, ranges, list + stride,
-                                                            counts + 1, n, kClassMax[0], low_mask);
-  range_sort_kernel<512, 16><<<grid(1024), 512, lds_c, s>>>(keys, values, ranges, list + 2 * stride,
+  range_sort_kernel<128, 16><<<grid(4096), 128, lds_a2, s>>>(keys, values, ranges, list + stride,
+                                                             counts + 1, n, kClassMax[0], low_mask);
+  range_sort_kernel<256, 16><<<grid(2048), 256, lds_b, s>>>(keys, values, ranges, list + 2 * stride,
                                                             counts + 2, n, kClassMax[1], low_mask);
-  range_sort_kernel<1024, 16><<<grid(512), 1024, lds_d, s>>>(keys, values, ranges, list + 3 * stride,
-                                                             counts + 3, n, kClassMax[2], low_mask);
+  range_sort_kernel<512, 16><<<grid(1024), 512, lds_c, s>>>(keys, values, ranges, list + 3 * stride,
+                                                            counts + 3, n, kClassMax[2], low_mask);
+  range_sort_kernel<1024, 16><<<grid(512), 1024, lds_d, s>>>(keys, values, ranges, list + 4 * stride,
+                                                             counts + 4, n, kClassMax[3], low_mask);
   long_range_sort_kernel<1024, 8><<<grid(512), 1024, 0, s>>>(
-      keys, values, tmp_keys, tmp_values, ranges, list + 4 * stride, counts + 4, n, kClassMax[3],
+      keys, values, tmp_keys, tmp_values, ranges, list + 5 * stride, counts + 5, n, kClassMax[4],
       low_mask);
   return check_launch("mt_sort_key_ranges");
 }

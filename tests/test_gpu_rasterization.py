@@ -209,8 +209,8 @@ def test_sort_scattered_varying_bits(mask):
 def test_tile_bucketed_sort_equals_full_sort(depths):
     """mt_sort_keys_high over the (camera, tile) bits, then mt_sort_key_ranges over each
     tile's range by the (depth, shell) bits, gives exactly the stable sort of the whole
-    keys (mt_sort_keys' order). Ranges of every size class: up to 1024, 4096, 8192 and
-    16384 keys (one workgroup of 64, 256, 512 or 1024 threads), longer (through the
+    keys (mt_sort_keys' order). Ranges of every size class: up to 1024, 2048, 4096, 8192
+    and 16384 keys (one workgroup of 64, 128, 256, 512 or 1024 threads), longer (through the
     temporary arrays), each class's bounds on both sides,
     with both signs of the key, ties, one, few or many varying low bits (odd and even pass
     counts), empty ranges and slots that hold no keys."""
@@ -218,7 +218,7 @@ def test_tile_bucketed_sort_equals_full_sort(depths):
 
     rng = np.random.default_rng({"few": 1, "one": 2, "uniform": 3}[depths])
     lay = orr.fused_layout()
-    sizes = [1, 2, 5, 300, 1024, 1025, 4095, 4096, 4097, 8192, 8193, 9000, 16383, 16384, 16385,
+    sizes = [1, 2, 5, 300, 1024, 1025, 2048, 2049, 4095, 4096, 4097, 8192, 8193, 9000, 16383, 16384, 16385,
              40000, 70001]
     sizes += [int(x) for x in rng.integers(1, 3000, 120)]
     cam, tile = [], []
