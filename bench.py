@@ -316,6 +316,8 @@ def algorithmic_bytes(z):
         # Batched faces: corners from the columns' crossing-axis bits, no voxel indices.
         "mt_count_triangle_vertices_axes": n_lc * 16 + n_lc * 4 + 5 * s * n_lc,
         "mt_create_triangle_faces_axes_i64": n_vox * 28 + s * n_vox * (1 + 8) + 5 * s * n_lc + f * 12,
+        # Faces from one (first vertex << 3 | axes) word per (level set, column).
+        "mt_create_triangle_faces_words_i64": n_vox * 28 + s * n_vox * (1 + 8) + 4 * s * n_lc + f * 12,
         "mt_project_vertices": g * vt * 12 + b * vt * 12,
         "mt_project_vertices_live": g * vt * 12 + b * vt * 12,  # live vertices only
         "mt_bound_triangles": f * 12 + b * vt * 12 + b * f * 20,
@@ -340,7 +342,7 @@ def algorithmic_bytes(z):
         "mt_pack_sh_rows": 2 * n_lc * nb,
         # The first group also writes the vertices; later groups write colours only.
         "mt_create_triangle_vertices_colors_forward": g * (n_lc * (16 + 4 + 12 + nb) + 4 * s * n_lc)
-        + vt * 17 + b * vt * 12,
+        + vt * 17 + b * vt * 12 + 4 * s * n_lc,  # + the column words (first group)
         # Columns, offsets and flags once per group; the flagged vertices' position, SDF and
         # per-camera colour / colour-gradient reads plus the clearing writes; the grid
         # gradients written once.

@@ -173,14 +173,19 @@ int mt_create_triangle_vertices_backward(
  * aligned) instead of [num_sh, num_grid_vertices, 3]: the colours are the same bits.
  * colors_stride 0 writes out_colors as f32[B, num_triangle_vertices, 3]; s >= B writes
  * f32[num_triangle_vertices, s, 3] (a vertex's cameras together: one contiguous record per
- * vertex, which the batched step's colour adjoint reads once). */
+ * vertex, which the batched step's colour adjoint reads once). column_words (may be NULL):
+ * also writes uint32[S * num_columns], word s * num_columns + c = (vertex_offsets[s *
+ * num_columns + c] << 3) | the column's crossing axes at level set s (bit a: axis a), for
+ * mt_create_triangle_faces_words_i64; needs num_triangle_vertices <=
+ * MT_COLUMN_WORD_MAX_VERTICES. */
+#define MT_COLUMN_WORD_MAX_VERTICES (1LL << 29)
 int mt_create_triangle_vertices_colors_forward(
     const float* grid_vertices, const float* grid_sdf, const float* grid_sh,
     int64_t num_grid_vertices, int32_t num_sh, int32_t sh_degree, const int32_t* lower_corners,
     int64_t num_columns, const float* level_sets, int32_t S, const int32_t* vertex_offsets,
     int64_t num_triangle_vertices, const float* camera_positions, int32_t B, float* out_vertices,
     float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, int32_t parts,
-    int32_t colors_stride, mt_stream_t stream);
+    int32_t colors_stride, uint32_t* column_words, mt_stream_t stream);
 
 /* Floats per SH row for an SH degree (0..3; -1 otherwise): 3 (deg+1)^2 rounded up to a
  * multiple of 4. No counterpart in the reference (a layout of this library). */
@@ -351,6 +356,18 @@ int mt_create_triangle_faces_axes_i64(const int32_t* neighbors, int64_t num_voxe
                                       int32_t S, const int64_t* face_offsets, int32_t* faces,
                                       int64_t face_capacity, int64_t vertex_capacity,
                                       int32_t* overflow, mt_stream_t stream);
+
+/* mt_create_triangle_faces_axes_i64 with each (level set, column)'s first vertex and
+ * crossing axes in one word (column_words, as mt_create_triangle_vertices_colors_forward
+ * writes them): one 4-byte load per corner instead of an offset and an axes byte.
+ * vertex_capacity <= MT_COLUMN_WORD_MAX_VERTICES. Same faces. Replaces
+ * create_triangle_faces.slang:3-78 in the batched training step. */
+int mt_create_triangle_faces_words_i64(const int32_t* neighbors, int64_t num_voxels,
+                                       int64_t num_columns, const uint32_t* column_words,
+                                       const uint8_t* cell_codes, int32_t S,
+                                       const int64_t* face_offsets, int32_t* faces,
+                                       int64_t face_capacity, int64_t vertex_capacity,
+                                       int32_t* overflow, mt_stream_t stream);
 
 /* ---------------------------------------------------------------------------------
  * Rasterization (triangle_rasterization/__init__.py:22-329)

@@ -100,7 +100,7 @@ SIGNATURES = {
     "mt_create_triangle_vertices_colors_forward": (
         c_i32,
         [c_p, c_p, c_p, c_i64, c_i32, c_i32, c_p, c_i64, c_p, c_i32, c_p, c_i64, c_p, c_i32, c_p,
-         c_p, c_p, c_p, c_p, c_i32, c_i32, c_p],
+         c_p, c_p, c_p, c_p, c_i32, c_i32, c_p, c_p],
     ),
     "mt_sh_row_floats": (c_i64, [c_i32]),
     "mt_pack_sh_rows": (c_i32, [c_p, c_i64, c_i32, c_i32, c_p, c_p]),
@@ -135,6 +135,9 @@ SIGNATURES = {
     ),
     "mt_create_triangle_faces_axes_i64": (
         c_i32, [c_p, c_i64, c_i64, c_p, c_p, c_p, c_i32, c_p, c_p, c_i64, c_i64, c_p, c_p]
+    ),
+    "mt_create_triangle_faces_words_i64": (
+        c_i32, [c_p, c_i64, c_i64, c_p, c_p, c_i32, c_p, c_p, c_i64, c_i64, c_p, c_p]
     ),
     "mt_project_vertices": (c_i32, [c_p, c_i64, c_p, c_p, c_i32, c_p, c_p, c_p]),
     "mt_project_vertices_live": (c_i32, [c_p, c_i64, c_p, c_p, c_p, c_i32, c_p, c_p, c_p]),

@@ -148,9 +148,14 @@ class BatchedTrainingStep:
         self.td = torch.zeros(V, **f32)
         self.ttype = torch.zeros(V, dtype=torch.uint8, device=dev)
         self.faces = torch.zeros((F, 3), **i32)
-        # Per (level set, column): which of its axis edges cross (its vertices' types), for
-        # the faces pass. The step needs no voxel index per face.
-        self.column_axes = torch.zeros(S * self.n_col, dtype=torch.uint8, device=dev)
+        # The step needs no voxel index per face. Faces find their corners' vertices from one word per (level set, column), (first
+        # vertex << 3) | crossing axes, written by the vertex pass; past 2^29 vertices from the
+        # offsets and the count pass's crossing-axis bytes.
+        self.use_words = V <= (1 << 29)
+        if self.use_words:
+            self.column_words = torch.zeros(S * self.n_col, dtype=torch.int32, device=dev)
+        else:
+            self.column_axes = torch.zeros(S * self.n_col, dtype=torch.uint8, device=dev)
         self.xy = torch.zeros((G, V, 2), **f32)
         self.depth = torch.zeros((G, V), **f32)
         self.n_keys = torch.zeros(1, **i64)  # key count of the current camera group
@@ -258,9 +263,13 @@ class BatchedTrainingStep:
         self.loss.zero_()
         self.sharp_grad.zero_()
         # --- tessellation counts (once for all camera groups)
-        call("mt_count_triangle_vertices_axes", grid_sdf.data_ptr(), v.lower_corners.data_ptr(),
-             self.n_col, self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(),
-             self.column_axes.data_ptr(), st)
+        if self.use_words:
+            call("mt_count_triangle_vertices", grid_sdf.data_ptr(), v.lower_corners.data_ptr(),
+                 self.n_col, self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), st)
+        else:
+            call("mt_count_triangle_vertices_axes", grid_sdf.data_ptr(), v.lower_corners.data_ptr(),
+                 self.n_col, self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(),
+                 self.column_axes.data_ptr(), st)
         call("mt_exclusive_scan_i32", self.vertex_offsets.data_ptr(), self.vertex_offsets.numel(),
              self.scan_ws.data_ptr(), self.scan_ws.numel(), st)
         call("mt_count_triangle_faces", grid_sdf.data_ptr(), v.lower_corners.data_ptr(), self.n_col,
@@ -294,7 +303,8 @@ class BatchedTrainingStep:
                  self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V,
                  self.campos[c0:].data_ptr(), n, self.tv.data_ptr(), self.td.data_ptr(),
                  self.ttype.data_ptr(), self.colors.data_ptr(), self.overflow.data_ptr(),
-                 (3 if gi == 0 else 2) | 4, self.G, st)
+                 (3 if gi == 0 else 2) | 4, self.G,
+                 self.column_words.data_ptr() if gi == 0 and self.use_words else None, st)
             # Faces (first group only) and the group's projection both need only the vertices:
             # without per-launch timing they run side by side on two streams.
             overlap = gi == 0 and hook is None
@@ -304,10 +314,17 @@ class BatchedTrainingStep:
                     self.side.wait_stream(main)
                 with torch.cuda.stream(self.side if overlap else main):
                     fst = torch.cuda.current_stream(self.device).cuda_stream
-                    call("mt_create_triangle_faces_axes_i64", v.neighbors.data_ptr(), self.n_vox,
-                         self.n_col, self.vertex_offsets.data_ptr(), self.column_axes.data_ptr(),
-                         self.codes.data_ptr(), S, self.face_offsets.data_ptr(),
-                         self.faces.data_ptr(), F, V, self.overflow.data_ptr(), fst)
+                    if self.use_words:
+                        call("mt_create_triangle_faces_words_i64", v.neighbors.data_ptr(),
+                             self.n_vox, self.n_col, self.column_words.data_ptr(),
+                             self.codes.data_ptr(), S, self.face_offsets.data_ptr(),
+                             self.faces.data_ptr(), F, V, self.overflow.data_ptr(), fst)
+                    else:
+                        call("mt_create_triangle_faces_axes_i64", v.neighbors.data_ptr(),
+                             self.n_vox, self.n_col, self.vertex_offsets.data_ptr(),
+                             self.column_axes.data_ptr(), self.codes.data_ptr(), S,
+                             self.face_offsets.data_ptr(), self.faces.data_ptr(), F, V,
+                             self.overflow.data_ptr(), fst)
                     torch.floor_divide(self.face_offsets[self.n_vox :: self.n_vox][:S], 3,
                                        out=self.face_bound64)
                     self.face_boundaries.copy_(self.face_bound64)

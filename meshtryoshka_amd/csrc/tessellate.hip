@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
     const float* gv, const float* gsdf, const float* gsh, int64_t ng, const int32_t* lower,
     int64_t n, const float* level_sets, int S, const int32_t* offsets, int64_t nt,
     const float* cams, int B, float* ov, float* od, uint8_t* otype, float* colors,
-    int32_t* overflow, int parts, int cstride) {
+    int32_t* overflow, int parts, int cstride, uint32_t* words) {
   constexpr int kN = (kDeg + 1) * (kDeg + 1);
   constexpr int kRow = 3 * kN + 1;  // padded: consecutive rows start in different banks
   constexpr int kR = sh_row_floats(kDeg);
@@ -413,10 +413,13 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
     const float iso = level_sets[s];
     const int64_t base = offsets[int64_t(s) * n + c_first];
     if (valid) {
-      int pos = static_cast<int>(offsets[int64_t(s) * n + c] - base);
+      const int32_t first = offsets[int64_t(s) * n + c];
+      int pos = static_cast<int>(first - base);
+      uint32_t axes = 0;
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         if (!crosses(col.d0, col.d[a], iso)) continue;
+        axes |= 1u << a;
         s_v1[pos] = col.v[a];
         s_t[pos] = edge_t(col.d0, col.d[a], iso);
         s_d1[pos] = col.d[a];
@@ -425,6 +428,11 @@ __global__ void __launch_bounds__(kVcBlock) create_vertices_colors_kernel(
         ++pos;
       }
       if (c == c_last) s_total = pos;
+      if (words != nullptr) {  // offsets past the word's 29 bits only occur on overflow
+        constexpr int32_t kMaxFirst = int32_t(MT_COLUMN_WORD_MAX_VERTICES - 1);
+        const int32_t f = first < 0 || first > kMaxFirst ? kMaxFirst : first;
+        words[int64_t(s) * n + c] = (static_cast<uint32_t>(f) << 3) | axes;
+      }
     }
     __syncthreads();
     const int total = s_total;
@@ -871,11 +879,14 @@ __global__ void __launch_bounds__(kBlock) count_faces_kernel(const float* sdf, c
 // triangle vertex, and consecutive threads write consecutive corners.
 // Face offsets are int32 (the reference's layout) or int64 (more than 2^31 - 1 face
 // corners); within one block they span at most 256 voxels x 15 corners.
-// kAxes: a corner's vertex is found from the owner column's crossing-axis bits (axes, as
-// count_vertices_kernel writes them; one byte beside the offsets) instead of by reading the
-// types of its vertices, which saves one dependent load per corner. voxel_indices may be
-// NULL (not written).
-template <typename FOff, bool kAxes>
+// kMode 1 (kAxesMode): a corner's vertex is found from the owner column's crossing-axis
+// bits (vtypes holds them, as count_vertices_kernel writes them; one byte beside the
+// offsets) instead of by reading the types of its vertices, which saves one dependent load
+// per corner. kMode 2 (kWordsMode): voffsets holds one word per (level set, column),
+// (first vertex << 3) | crossing axes, so a corner costs one 4-byte load. voxel_indices may
+// be NULL (not written).
+constexpr int kTypesMode = 0, kAxesMode = 1, kWordsMode = 2;
+template <typename FOff, int kMode>
 __global__ void __launch_bounds__(kBlock) create_faces_kernel(
     const int32_t* neighbors, const int32_t* indices, int64_t n, int64_t ncol,
     const int32_t* voffsets, const uint8_t* vtypes, const uint8_t* codes, int S,
@@ -937,10 +948,18 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
         const int axis = MT_MC_EDGE_OWNER[e][1];
         const int64_t owner = (slot < 0) ? vv : s_nbr[slot][lo];
         const int64_t vo = s * ncol + owner;
-        const int32_t start = voffsets[vo];
         target[c] = -1;
-        if constexpr (kAxes) {
-          const uint32_t m = vtypes[vo];  // the owner's crossing axes at this level set
+        if constexpr (kMode != kTypesMode) {
+          int32_t start;
+          uint32_t m;  // the owner's crossing axes at this level set
+          if constexpr (kMode == kWordsMode) {
+            const uint32_t w = reinterpret_cast<const uint32_t*>(voffsets)[vo];
+            start = static_cast<int32_t>(w >> 3);
+            m = w & 7u;
+          } else {
+            start = voffsets[vo];
+            m = vtypes[vo];
+          }
           if (int64_t(start) + __popc(m) > vcap) {  // never written
             if (overflow) atomicOr(overflow, 1);
           } else if ((m >> axis) & 1u) {
@@ -948,6 +967,7 @@ __global__ void __launch_bounds__(kBlock) create_faces_kernel(
           }
           continue;
         }
+        const int32_t start = voffsets[vo];
         int32_t cnt = voffsets[vo + 1] - start;
         if (int64_t(start) + cnt > vcap) {  // vertices past the capacity were never written
           if (overflow) atomicOr(overflow, 1);
@@ -1106,7 +1126,7 @@ extern "C" int mt_create_triangle_faces(const int32_t* neighbors, const int32_t*
                                         mt_stream_t stream) {
   MT_REQUIRE(num_voxels >= 0 && S >= 0 && face_capacity >= 0, "mt_create_triangle_faces: bad sizes");
   if (num_voxels == 0 || S == 0) return MT_OK;
-  mt::create_faces_kernel<int32_t, false><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
+  mt::create_faces_kernel<int32_t, mt::kTypesMode><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
                                      static_cast<hipStream_t>(stream)>>>(
       neighbors, indices, num_voxels, num_columns, vertex_offsets, vertex_types, cell_codes, S,
       face_offsets, faces, voxel_indices, face_capacity, vertex_capacity, overflow);
@@ -1125,7 +1145,7 @@ extern "C" int mt_create_triangle_faces_i64(const int32_t* neighbors, const int3
              "mt_create_triangle_faces_i64: bad sizes");
   MT_REQUIRE(face_capacity < (int64_t(1) << 31), "mt_create_triangle_faces_i64: face ids are int32");
   if (num_voxels == 0 || S == 0) return MT_OK;
-  mt::create_faces_kernel<int64_t, false><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
+  mt::create_faces_kernel<int64_t, mt::kTypesMode><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
                                      static_cast<hipStream_t>(stream)>>>(
       neighbors, indices, num_voxels, num_columns, vertex_offsets, vertex_types, cell_codes, S,
       face_offsets, faces, voxel_indices, face_capacity, vertex_capacity, overflow);
@@ -1143,11 +1163,31 @@ extern "C" int mt_create_triangle_faces_axes_i64(const int32_t* neighbors, int64
              "mt_create_triangle_faces_axes_i64: bad sizes");
   MT_REQUIRE(face_capacity < (int64_t(1) << 31), "mt_create_triangle_faces_axes_i64: face ids are int32");
   if (num_voxels == 0 || S == 0) return MT_OK;
-  mt::create_faces_kernel<int64_t, true><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
+  mt::create_faces_kernel<int64_t, mt::kAxesMode><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
                                            static_cast<hipStream_t>(stream)>>>(
       neighbors, nullptr, num_voxels, num_columns, vertex_offsets, column_axes, cell_codes, S,
       face_offsets, faces, nullptr, face_capacity, vertex_capacity, overflow);
   return mt::check_launch("mt_create_triangle_faces_axes_i64");
+}
+
+extern "C" int mt_create_triangle_faces_words_i64(const int32_t* neighbors, int64_t num_voxels,
+                                                  int64_t num_columns, const uint32_t* column_words,
+                                                  const uint8_t* cell_codes, int32_t S,
+                                                  const int64_t* face_offsets, int32_t* faces,
+                                                  int64_t face_capacity, int64_t vertex_capacity,
+                                                  int32_t* overflow, mt_stream_t stream) {
+  MT_REQUIRE(num_voxels >= 0 && S >= 0 && face_capacity >= 0,
+             "mt_create_triangle_faces_words_i64: bad sizes");
+  MT_REQUIRE(face_capacity < (int64_t(1) << 31), "mt_create_triangle_faces_words_i64: face ids are int32");
+  MT_REQUIRE(vertex_capacity <= MT_COLUMN_WORD_MAX_VERTICES,
+             "mt_create_triangle_faces_words_i64: %lld vertices exceed the 2^29 a column word holds",
+             static_cast<long long>(vertex_capacity));
+  if (num_voxels == 0 || S == 0) return MT_OK;
+  mt::create_faces_kernel<int64_t, mt::kWordsMode><<<mt::blocks_for(num_voxels, mt::kBlock), mt::kBlock, 0,
+                                                     static_cast<hipStream_t>(stream)>>>(
+      neighbors, nullptr, num_voxels, num_columns, reinterpret_cast<const int32_t*>(column_words),
+      nullptr, cell_codes, S, face_offsets, faces, nullptr, face_capacity, vertex_capacity, overflow);
+  return mt::check_launch("mt_create_triangle_faces_words_i64");
 }
 
 extern "C" int mt_create_triangle_vertices_colors_forward(
@@ -1156,9 +1196,12 @@ extern "C" int mt_create_triangle_vertices_colors_forward(
     int64_t num_columns, const float* level_sets, int32_t S, const int32_t* vertex_offsets,
     int64_t num_triangle_vertices, const float* camera_positions, int32_t B, float* out_vertices,
     float* out_sdf, uint8_t* out_types, float* out_colors, int32_t* overflow, int32_t parts,
-    int32_t colors_stride, mt_stream_t stream) {
+    int32_t colors_stride, uint32_t* column_words, mt_stream_t stream) {
   MT_REQUIRE((parts & 3) != 0 && (parts & ~7) == 0,
              "mt_create_triangle_vertices_colors_forward: parts %d", parts);
+  MT_REQUIRE(column_words == nullptr || num_triangle_vertices <= MT_COLUMN_WORD_MAX_VERTICES,
+             "mt_create_triangle_vertices_colors_forward: %lld vertices exceed the 2^29 a column word holds",
+             static_cast<long long>(num_triangle_vertices));
   MT_REQUIRE(colors_stride == 0 || colors_stride >= B,
              "mt_create_triangle_vertices_colors_forward: colors_stride must be 0 or at least B");
   MT_REQUIRE(num_sh >= 0 && S >= 0 && num_columns >= 0 && B >= 0,
@@ -1178,12 +1221,14 @@ extern "C" int mt_create_triangle_vertices_colors_forward(
       mt::create_vertices_colors_kernel<D, true><<<grid, mt::kVcBlock, 0, st>>>(                \
           grid_vertices, grid_sdf, grid_sh, num_grid_vertices, lower_corners, num_columns,     \
           level_sets, S, vertex_offsets, num_triangle_vertices, camera_positions, B,           \
-          out_vertices, out_sdf, out_types, out_colors, overflow, parts, colors_stride);       \
+          out_vertices, out_sdf, out_types, out_colors, overflow, parts, colors_stride,        \
+          column_words);                                                                        \
     else                                                                                        \
       mt::create_vertices_colors_kernel<D, false><<<grid, mt::kVcBlock, 0, st>>>(               \
           grid_vertices, grid_sdf, grid_sh, num_grid_vertices, lower_corners, num_columns,     \
           level_sets, S, vertex_offsets, num_triangle_vertices, camera_positions, B,           \
-          out_vertices, out_sdf, out_types, out_colors, overflow, parts, colors_stride);       \
+          out_vertices, out_sdf, out_types, out_colors, overflow, parts, colors_stride,        \
+          column_words);                                                                        \
   } while (0)
   // Without colours the degree-0 instance stages no SH rows (less LDS, more blocks per CU).
   switch ((parts & 2) != 0 && B > 0 ? sh_degree : 0) {

@@ -434,7 +434,7 @@ def render_batch(scene: ResidentScene, extrinsics: torch.Tensor, intrinsics: tor
              g.grid_sdf.data_ptr(), g.sh_rows.data_ptr(), g.grid_vertices.shape[0], g.num_sh,
              scene.sh_degree, g.lower_corners.data_ptr(), g.lower_corners.shape[1],
              g.level_sets.data_ptr(), s, g.vertex_offsets.data_ptr(), v, campos.data_ptr(), b,
-             None, None, None, colors.data_ptr(), overflow.data_ptr(), 2 | 4, 0, st)
+             None, None, None, colors.data_ptr(), overflow.data_ptr(), 2 | 4, 0, None, st)
     h0, w0 = shapes[0]
     args = _native.CompositeArgs(
         xy.data_ptr(), colors.data_ptr(), scene.signed_distances.data_ptr(), v,

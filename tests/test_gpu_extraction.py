@@ -161,6 +161,33 @@ def test_faces_from_crossing_axes_match_oracle(scene):
                  voff.data_ptr(), axes.data_ptr(), codes.data_ptr(), s, foff.data_ptr(),
                  faces.data_ptr(), nf, nv // 2, flag.data_ptr(), st)
     assert int(flag) == 1
+    # Column words ((first vertex << 3) | crossing axes), written by the vertex pass, and the
+    # faces read from them: the same faces; a capacity below the vertex count raises the flag.
+    world, sh = to_dev(scene.world), to_dev(scene.sh)
+    words = torch.full((s * n,), 0xFFFFFFF, dtype=torch.int32, device="cuda")
+    tv = torch.empty((nv, 3), dtype=torch.float32, device="cuda")
+    td = torch.empty(nv, dtype=torch.float32, device="cuda")
+    tt = torch.empty(nv, dtype=torch.uint8, device="cuda")
+    _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(), sdf.data_ptr(),
+                 sh.data_ptr(), world.shape[0], sh.shape[0], 0, lower.data_ptr(), n, ls.data_ptr(),
+                 s, voff.data_ptr(), nv, None, 0, tv.data_ptr(), td.data_ptr(), tt.data_ptr(), None,
+                 flag.data_ptr(), 1, 0, words.data_ptr(), st)
+    want = (voff[:-1].to(torch.int64) << 3) | axes.to(torch.int64)
+    assert torch.equal(words.to(torch.int64) & 0xFFFFFFFF, want)
+    flag.zero_()
+    faces.fill_(-7)
+    _native.call("mt_create_triangle_faces_words_i64", nbr.data_ptr(), nvox, n, words.data_ptr(),
+                 codes.data_ptr(), s, foff.data_ptr(), faces.data_ptr(), nf, nv, flag.data_ptr(), st)
+    assert int(flag) == 0
+    np.testing.assert_array_equal(faces.cpu().numpy(), scene.tri.faces)
+    _native.call("mt_create_triangle_faces_words_i64", nbr.data_ptr(), nvox, n, words.data_ptr(),
+                 codes.data_ptr(), s, foff.data_ptr(), faces.data_ptr(), nf, nv // 2, flag.data_ptr(),
+                 st)
+    assert int(flag) == 1
+    with pytest.raises(RuntimeError, match="2\\^29"):
+        _native.call("mt_create_triangle_faces_words_i64", nbr.data_ptr(), nvox, n,
+                     words.data_ptr(), codes.data_ptr(), s, foff.data_ptr(), faces.data_ptr(), nf,
+                     (1 << 29) + 1, flag.data_ptr(), st)
 
 
 def test_create_vertices_backward_matches_oracle(scene):

@@ -71,7 +71,7 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
     _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(), sdf.data_ptr(),
                  sh.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(), n, ls.data_ptr(), s,
                  off.data_ptr(), V, campos.data_ptr(), b, tv2.data_ptr(), td2.data_ptr(),
-                 tt2.data_ptr(), col2.data_ptr(), ov.data_ptr(), 3, 0, _stream())
+                 tt2.data_ptr(), col2.data_ptr(), ov.data_ptr(), 3, 0, None, _stream())
     assert int(ov) == 0
     assert torch.equal(tv, tv2) and torch.equal(td, td2) and torch.equal(tt, tt2)
     assert torch.equal(col, col2)
@@ -83,14 +83,14 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
     _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(), sdf.data_ptr(),
                  sh.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(), n, ls.data_ptr(), s,
                  off.data_ptr(), V, campos.data_ptr(), b, tv3.data_ptr(), td3.data_ptr(),
-                 tt3.data_ptr(), col3.data_ptr(), ov.data_ptr(), 1, 0, _stream())
+                 tt3.data_ptr(), col3.data_ptr(), ov.data_ptr(), 1, 0, None, _stream())
     assert torch.equal(tv3, tv) and torch.equal(td3, td) and torch.equal(tt3, tt)
     assert bool((col3 == 7.0).all())
     tv4 = torch.full((V, 3), 7.0, **f32)
     _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(), sdf.data_ptr(),
                  sh.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(), n, ls.data_ptr(), s,
                  off.data_ptr(), V, campos.data_ptr(), b, tv4.data_ptr(), td3.data_ptr(),
-                 tt3.data_ptr(), col3.data_ptr(), ov.data_ptr(), 2, 0, _stream())
+                 tt3.data_ptr(), col3.data_ptr(), ov.data_ptr(), 2, 0, None, _stream())
     assert torch.equal(col3, col) and bool((tv4 == 7.0).all())
     # SH rows: a permutation of grid_sh's first (deg+1)^2 planes, zero-padded; the colour
     # pass reading them gives the same bits.
@@ -109,7 +109,7 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
                      sdf.data_ptr(), rows.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(),
                      n, ls.data_ptr(), s, off.data_ptr(), V, campos.data_ptr(), b, tv5.data_ptr(),
                      td5.data_ptr(), tt5.data_ptr(), col5.data_ptr(), ov.data_ptr(), parts,
-                     0, _stream())
+                     0, None, _stream())
         assert torch.equal(col5, col)
         if parts & 1:
             assert torch.equal(tv5, tv) and torch.equal(td5, td) and torch.equal(tt5, tt)
@@ -123,14 +123,14 @@ def test_fused_forward_and_adjoint_match_separate_kernels(case):
         _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(),
                      sdf.data_ptr(), rows.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(),
                      n, ls.data_ptr(), s, off.data_ptr(), V, campos.data_ptr(), b, None, None,
-                     None, col6.data_ptr(), ov.data_ptr(), 2 | 4, stride, _stream())
+                     None, col6.data_ptr(), ov.data_ptr(), 2 | 4, stride, None, _stream())
         assert torch.equal(col6[:, :b].permute(1, 0, 2), col)
         assert bool((col6[:, b:] == 7.0).all())
     with pytest.raises(RuntimeError, match="colors_stride"):
         _native.call("mt_create_triangle_vertices_colors_forward", world.data_ptr(),
                      sdf.data_ptr(), rows.data_ptr(), ng, nsh, deg, vox.lower_corners.data_ptr(),
                      n, ls.data_ptr(), s, off.data_ptr(), V, campos.data_ptr(), b, None, None,
-                     None, col3.data_ptr(), ov.data_ptr(), 2 | 4, b - 1, _stream())
+                     None, col3.data_ptr(), ov.data_ptr(), 2 | 4, b - 1, None, _stream())
 
     # Adjoint: random upstream gradients for SDF and colours.
     gen = torch.Generator("cpu").manual_seed(11)
