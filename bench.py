@@ -348,8 +348,9 @@ def algorithmic_bytes(z):
         # gradients written once.
         "mt_create_triangle_vertices_colors_backward": g * (n_lc * (16 + 4) + 4 * s * n_lc + vt)
         + z.get("V_touched", vt) * (12 + 8 + 36 * b) + n_lc * (4 + nb),
-        # Per group the vertex flags and tile ranges; once per step the grid gradients.
-        "zero_fill": g * vt + 16 * tt + n_lc * (4 + nb),
+        # Per group the vertex flags and tile ranges (the first group's adjoint writes the
+        # grid gradients whole, so they are not cleared).
+        "zero_fill": g * vt + 16 * tt,
     }
 
 

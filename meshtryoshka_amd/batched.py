@@ -282,10 +282,9 @@ class BatchedTrainingStep:
         call("mt_camera_positions", self.extrinsics.data_ptr(), self.B, self.campos.data_ptr(), st)
         call("mt_pack_sh_rows", grid_sh.data_ptr(), self.n_grid, self.num_sh, self.sh_degree,
              self.sh_rows.data_ptr(), st)
-        # The grid gradients start from zero and every camera group adds to them, so the
-        # adjoint touches only the grid vertices that receive a gradient.
-        self.grid_sdf_grad.zero_()
-        self.grid_sh_grad.zero_()
+        # The first camera group's adjoint writes every grid gradient (zero where no vertex
+        # contributes), so they need no clearing; later groups add to them over the grid
+        # vertices that receive a gradient.
         for gi, g in enumerate(self.groups):
             c0, n = g["c0"], g["n"]
             cam_dims = g["cam_dims"].data_ptr() if table else None
@@ -377,14 +376,16 @@ class BatchedTrainingStep:
             call("mt_intersect_and_composite_backward", args, out.data_ptr(), grad.data_ptr(),
                  self.colors_grad.data_ptr(), self.td_grad.data_ptr(), self.sharp_grad.data_ptr(),
                  self.quirk, st)
-            # SH adjoint fused into the vertex adjoint, adding to the grid gradients.
+            # SH adjoint fused into the vertex adjoint: the first group writes the grid
+            # gradients, later groups add to them.
             call("mt_create_triangle_vertices_colors_backward", grid_sdf.data_ptr(), self.n_grid,
                  self.num_sh, self.sh_degree, v.lower_corners.data_ptr(), self.n_col,
                  self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V,
                  self.tv.data_ptr(), self.colors.data_ptr(), self.campos[c0:].data_ptr(), n,
                  self.td_grad.data_ptr(), self.colors_grad.data_ptr(),
                  self.grid_sdf_grad.data_ptr(), self.grid_sh_grad.data_ptr(),
-                 self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(), 1, self.touched.data_ptr(), 1,
+                 self.vbwd_ws.data_ptr(), self.vbwd_ws.numel(), int(gi > 0),
+                 self.touched.data_ptr(), 1,
                  int(self.maps_built),
                  self.overflow.data_ptr(), self.G, st)
             # The adjoint's grid-vertex maps stay in its workspace (they depend only on the
