@@ -77,7 +77,8 @@ class BatchedTrainingStep:
                  intrinsics: torch.Tensor, image_shape, num_sh: int, sh_degree: int,
                  sharpness: float, background, capacities: Capacities, tile_shape=(16, 16),
                  chunk: int = 0, near_plane: float = 0.2, reference_quirk: bool = False,
-                 camera_group: int | None = None, cull_keys: bool = True):
+                 camera_group: int | None = None, cull_keys: bool = True,
+                 faces_beside_colours: bool = True):
         dev = voxels.lower_corners.device
         self.device = dev
         self.voxels = voxels
@@ -151,7 +152,10 @@ class BatchedTrainingStep:
         # The step needs no voxel index per face. Faces find their corners' vertices from one word per (level set, column), (first
         # vertex << 3) | crossing axes, written by the vertex pass; past 2^29 vertices from the
         # offsets and the count pass's crossing-axis bytes.
-        self.use_words = V <= (1 << 29)
+        # faces_beside_colours: the faces pass (crossing-axis bytes) runs on a second stream
+        # beside the first group's vertex and colour pass instead of after it.
+        self.faces_early = bool(faces_beside_colours)
+        self.use_words = V <= (1 << 29) and not self.faces_early
         if self.use_words:
             self.column_words = torch.zeros(S * self.n_col, dtype=torch.int32, device=dev)
         else:
@@ -294,23 +298,11 @@ class BatchedTrainingStep:
             self.ranges[: g["tiles"]].zero_()
             if hook is not None:
                 hook("zero_fill")
-            # Vertices and the first group's colours in one pass, later groups' colours alone
-            # (the positions are the same); the per-vertex SH array is never stored.
-            call("mt_create_triangle_vertices_colors_forward", grid_vertices.data_ptr(),
-                 grid_sdf.data_ptr(), self.sh_rows.data_ptr(), self.n_grid, self.num_sh,
-                 self.sh_degree, v.lower_corners.data_ptr(), self.n_col,
-                 self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V,
-                 self.campos[c0:].data_ptr(), n, self.tv.data_ptr(), self.td.data_ptr(),
-                 self.ttype.data_ptr(), self.colors.data_ptr(), self.overflow.data_ptr(),
-                 (3 if gi == 0 else 2) | 4, self.G,
-                 self.column_words.data_ptr() if gi == 0 and self.use_words else None, st)
-            # Faces (first group only) and the group's projection both need only the vertices:
-            # without per-launch timing they run side by side on two streams.
+            main = torch.cuda.current_stream(self.device)
             overlap = gi == 0 and hook is None
-            if gi == 0:
-                main = torch.cuda.current_stream(self.device)
-                if overlap:
-                    self.side.wait_stream(main)
+            early = overlap and self.faces_early
+
+            def faces_pass():
                 with torch.cuda.stream(self.side if overlap else main):
                     fst = torch.cuda.current_stream(self.device).cuda_stream
                     if self.use_words:
@@ -327,8 +319,29 @@ class BatchedTrainingStep:
                     torch.floor_divide(self.face_offsets[self.n_vox :: self.n_vox][:S], 3,
                                        out=self.face_bound64)
                     self.face_boundaries.copy_(self.face_bound64)
-                if hook is not None:
-                    hook("face_boundaries")
+
+            if early:  # the faces need only the counts: beside the vertex and colour pass
+                self.side.wait_stream(main)
+                faces_pass()
+            # Vertices and the first group's colours in one pass, later groups' colours alone
+            # (the positions are the same); the per-vertex SH array is never stored.
+            call("mt_create_triangle_vertices_colors_forward", grid_vertices.data_ptr(),
+                 grid_sdf.data_ptr(), self.sh_rows.data_ptr(), self.n_grid, self.num_sh,
+                 self.sh_degree, v.lower_corners.data_ptr(), self.n_col,
+                 self.level_sets.data_ptr(), S, self.vertex_offsets.data_ptr(), V,
+                 self.campos[c0:].data_ptr(), n, self.tv.data_ptr(), self.td.data_ptr(),
+                 self.ttype.data_ptr(), self.colors.data_ptr(), self.overflow.data_ptr(),
+                 (3 if gi == 0 else 2) | 4, self.G,
+                 self.column_words.data_ptr() if gi == 0 and self.use_words else None, st)
+            # Otherwise the faces (first group only) and the group's projection, which both
+            # need only the vertices, run side by side on two streams (without per-launch
+            # timing).
+            if gi == 0 and not early:
+                if overlap:
+                    self.side.wait_stream(main)
+                faces_pass()
+            if gi == 0 and hook is not None:
+                hook("face_boundaries")
             face_boundaries = self.face_boundaries
             # --- rasterization of the group's cameras
             # Live vertices only (the arrays' capacity has a margin above the vertex count).

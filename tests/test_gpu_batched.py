@@ -69,9 +69,12 @@ def test_batched_eager_matches_plain(scene):
     assert _rel(step.grid_sh_grad, gsh) < 1e-5
 
 
-def test_graph_replays_match_eager(scene):
+@pytest.mark.parametrize("faces_beside_colours", [True, False])
+def test_graph_replays_match_eager(scene, faces_beside_colours):
+    """Both placements of the faces pass: beside the vertex and colour pass on a second
+    stream (crossing-axis bytes), or after it (column words) beside the projection."""
     spec, vox, world, sdf, sh, cams, target = scene
-    step = make_step(scene)
+    step = make_step(scene, faces_beside_colours=faces_beside_colours)
     step.run(world, sdf, sh, target)
     ref_out = step.out.clone()
     ref_sdf = step.grid_sdf_grad.clone()
@@ -92,7 +95,7 @@ def test_graph_replays_match_eager(scene):
     # Replays follow the static inputs.
     static[1].add_(0.001)
     step.replay()
-    other = make_step(scene)
+    other = make_step(scene, faces_beside_colours=not faces_beside_colours)
     other.run(static[0], static[1], static[2], static[3])
     assert torch.equal(step.out, other.out)
 
