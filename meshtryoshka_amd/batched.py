@@ -266,6 +266,14 @@ class BatchedTrainingStep:
         self.overflow.zero_()
         self.loss.zero_()
         self.sharp_grad.zero_()
+        # The SH rows need only grid_sh: without per-launch timing they are packed on a second
+        # stream beside the tessellation counts and scans.
+        main = torch.cuda.current_stream(self.device)
+        if hook is None:
+            self.side.wait_stream(main)
+        with torch.cuda.stream(self.side if hook is None else main):
+            call("mt_pack_sh_rows", grid_sh.data_ptr(), self.n_grid, self.num_sh, self.sh_degree,
+                 self.sh_rows.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream)
         # --- tessellation counts (once for all camera groups)
         if self.use_words:
             call("mt_count_triangle_vertices", grid_sdf.data_ptr(), v.lower_corners.data_ptr(),
@@ -284,8 +292,8 @@ class BatchedTrainingStep:
              self.face_offsets[1:].data_ptr(), S * self.n_vox, self.scan_ws.data_ptr(),
              self.scan_ws.numel(), st)
         call("mt_camera_positions", self.extrinsics.data_ptr(), self.B, self.campos.data_ptr(), st)
-        call("mt_pack_sh_rows", grid_sh.data_ptr(), self.n_grid, self.num_sh, self.sh_degree,
-             self.sh_rows.data_ptr(), st)
+        if hook is None:
+            main.wait_stream(self.side)
         # The first camera group's adjoint writes every grid gradient (zero where no vertex
         # contributes), so they need no clearing; later groups add to them over the grid
         # vertices that receive a gradient.
