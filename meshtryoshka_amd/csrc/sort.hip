@@ -846,7 +846,10 @@ int sort_impl(const int64_t* keys_in, const int32_t* values_in, int64_t n, int64
   mask_kernel<<<capped_blocks(n, 256 * 16, 4096), 256, 0, s>>>(keys_in, n, w, n_device);
   prepare_kernel<<<1, 64, 0, s>>>(w);
   const unsigned grid = static_cast<unsigned>(tiles);
-  for (int p = 0; p < kPasses; ++p) {
+  // The varying bits are a subset of sorted_bits: passes past those bits are never active,
+  // so they are not launched (the device decides among the rest).
+  const int max_passes = (__builtin_popcountll(sorted_bits) + 7) / 8;
+  for (int p = 0; p < max_passes && p < kPasses; ++p) {
     count_kernel<<<grid, kBlock, 0, s>>>(keys_in, values_in, keys_out, values_out, n, p, w, n_device);
     // Inactive passes leave the counts as they are; scanning them is harmless.
     if (int rc = inclusive_scan_i32_to_i64(w.counts, w.offsets, tiles * kRadix, w.scan_ws,
