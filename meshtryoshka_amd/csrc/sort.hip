@@ -146,7 +146,20 @@ __global__ void __launch_bounds__(256) mask_kernel(const int64_t* keys, int64_t 
   n = live_count(n, n_device);
   uint64_t o = 0ull, a = ~0ull;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+  // Eight independent loads in flight per thread and round.
+  constexpr int kUnroll = 8;
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (; i + (kUnroll - 1) * stride < n; i += kUnroll * stride) {
+    uint64_t k[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) k[u] = static_cast<uint64_t>(keys[i + u * stride]);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      o |= k[u];
+      a &= k[u];
+    }
+  }
+  for (; i < n; i += stride) {
     const uint64_t k = static_cast<uint64_t>(keys[i]);
     o |= k;
     a &= k;
@@ -156,7 +169,19 @@ __global__ void __launch_bounds__(256) mask_kernel(const int64_t* keys, int64_t 
     o |= __shfl_xor(o, d, kWave);
     a &= __shfl_xor(a, d, kWave);
   }
+  // One atomic pair per workgroup: the pairs on the two shared words serialise.
+  __shared__ uint64_t s_o[256 / kWave], s_a[256 / kWave];
   if (__lane_id() == 0) {
+    s_o[threadIdx.x / kWave] = o;
+    s_a[threadIdx.x / kWave] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 1; q < 256 / kWave; ++q) {
+      o |= s_o[q];
+      a &= s_a[q];
+    }
     atomicOr(&w.bits[0], static_cast<unsigned long long>(o));
     atomicAnd(&w.bits[1], static_cast<unsigned long long>(a));
   }
