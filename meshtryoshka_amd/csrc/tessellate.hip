@@ -555,11 +555,11 @@ __device__ __forceinline__ void add_row(const float* row, float& d, float (&acc)
 // columns with any set bit are appended to list (one atomic per wave).
 constexpr int kSlotLevels = 10;
 
-__global__ void __launch_bounds__(kBlock) colors_adjoint_mask_kernel(
-    const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
-    int S, const int32_t* offsets, int64_t nt, const uint8_t* touched, const int32_t* maps,
-    uint8_t* col_live, uint32_t* col_slots, int32_t* list, int32_t* col_pos, int32_t* count) {
-  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// Phase 0 for column c: bits of col_live[c] and col_slots[c] (written here); returns the live bits.
+__device__ __forceinline__ uint32_t adjoint_column_mask(
+    const float* gsdf, const int32_t* lower, int64_t n, const float* level_sets, int S,
+    const int32_t* offsets, int64_t nt, const uint8_t* touched, uint8_t* col_live,
+    uint32_t* col_slots, int64_t c) {
   uint32_t bits = 0, slots = 0;
   if (c < n) {
     const ColumnSdf col = load_column(gsdf, lower, n, c);
@@ -608,29 +608,65 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_mask_kernel(
     col_live[c] = static_cast<uint8_t>(bits);
     col_slots[c] = slots;
   }
-  // One atomic per block: waves count their live columns, thread 0 reserves the block's
-  // range, each wave writes its part in lane order.
-  __shared__ int s_count[kBlock / kWave];
+  return bits;
+}
+
+// kMaskRounds columns per thread (consecutive runs of kBlock), so that a workgroup appends
+// kMaskRounds x kBlock columns' live ones with one atomic on the shared list counter (the
+// counter's atomics serialise).
+constexpr int kMaskRounds = 4;
+
+__global__ void __launch_bounds__(kBlock) colors_adjoint_mask_kernel(
+    const float* gsdf, int64_t ng, const int32_t* lower, int64_t n, const float* level_sets,
+    int S, const int32_t* offsets, int64_t nt, const uint8_t* touched, const int32_t* maps,
+    uint8_t* col_live, uint32_t* col_slots, int32_t* list, int32_t* col_pos, int32_t* count) {
+  (void)ng;
+  (void)maps;
+  const int64_t c0 = int64_t(blockIdx.x) * (kBlock * kMaskRounds) + threadIdx.x;
+  uint32_t live[kMaskRounds];
+#pragma unroll
+  for (int r = 0; r < kMaskRounds; ++r) {
+    const int64_t c = c0 + int64_t(r) * kBlock;
+    live[r] = c < n ? adjoint_column_mask(gsdf, lower, n, level_sets, S, offsets, nt, touched,
+                                          col_live, col_slots, c)
+                    : 0u;
+  }
+  __shared__ int s_count[kMaskRounds][kBlock / kWave];
   __shared__ int s_base;
-  const uint64_t m = __ballot(bits != 0);
   const int lane = __lane_id();
   const int wave = threadIdx.x / kWave;
-  if (lane == 0) s_count[wave] = __popcll(m);
+  uint64_t m[kMaskRounds];
+#pragma unroll
+  for (int r = 0; r < kMaskRounds; ++r) {
+    m[r] = __ballot(live[r] != 0);
+    if (lane == 0) s_count[r][wave] = __popcll(m[r]);
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     int total = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / kWave; ++w) total += s_count[w];
+    for (int r = 0; r < kMaskRounds; ++r)
+#pragma unroll
+      for (int w = 0; w < kBlock / kWave; ++w) total += s_count[r][w];
     s_base = total > 0 ? atomicAdd(count, total) : 0;
   }
   __syncthreads();
-  int at = s_base;
+  int at = s_base;  // rounds in order, waves in order inside a round
 #pragma unroll
-  for (int w = 0; w < kBlock / kWave; ++w) at += w < wave ? s_count[w] : 0;
-  if (bits != 0) {
-    const int32_t i = at + __popcll(m & lanemask_lt());
-    list[i] = static_cast<int32_t>(c);
-    col_pos[c] = i;
+  for (int r = 0; r < kMaskRounds; ++r) {
+    int before = 0, round_total = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) {
+      before += w < wave ? s_count[r][w] : 0;
+      round_total += s_count[r][w];
+    }
+    if (live[r] != 0) {
+      const int64_t c = c0 + int64_t(r) * kBlock;
+      const int32_t i = at + before + __popcll(m[r] & lanemask_lt());
+      list[i] = static_cast<int32_t>(c);
+      col_pos[c] = i;
+    }
+    at += round_total;
   }
 }
 
@@ -1335,7 +1371,8 @@ extern "C" int mt_create_triangle_vertices_colors_backward(
   }
   mt::zero_counts_kernel<<<1, 1, 0, st>>>(live_count, ends_count);
   if (num_columns > 0)
-    mt::colors_adjoint_mask_kernel<<<mt::blocks_for(num_columns, mt::kBlock), mt::kBlock, 0, st>>>(
+    mt::colors_adjoint_mask_kernel<<<mt::blocks_for(num_columns, mt::kBlock * mt::kMaskRounds),
+                                     mt::kBlock, 0, st>>>(
         grid_sdf, num_grid_vertices, lower_corners, num_columns, level_sets, S, vertex_offsets,
         num_triangle_vertices, vertex_touched, maps, col_live, col_slots, list, col_pos,
         live_count);
