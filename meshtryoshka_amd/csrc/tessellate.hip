@@ -826,14 +826,20 @@ __global__ void __launch_bounds__(kBlock) colors_adjoint_ends_kernel(
       add_row<kN>(part + (int64_t(col_pos[e]) * 4 + a) * kPad, sd, v);
   }
   gg_d[g] = accumulate ? gg_d[g] + sd : sd;
+  struct F3 {
+    float x, y, z;
+  };
 #pragma unroll
-  for (int k = 0; k < kNsh; ++k)
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-      float* o = gg_sh + (int64_t(k) * ng + g) * 3 + ch;
-      const float x = k < kN ? v[k < kN ? k : 0][ch] : 0.f;
-      *o = accumulate ? *o + x : x;
+  for (int k = 0; k < kNsh; ++k) {
+    F3* o = reinterpret_cast<F3*>(gg_sh + (int64_t(k) * ng + g) * 3);
+    F3 x{0.f, 0.f, 0.f};
+    if (k < kN) x = F3{v[k < kN ? k : 0][0], v[k < kN ? k : 0][1], v[k < kN ? k : 0][2]};
+    if (accumulate) {  // read-modify-write of the three channels
+      const F3 y = *o;
+      x = F3{y.x + x.x, y.y + x.y, y.z + x.z};
     }
+    *o = x;  // one 12-byte store per plane
+  }
 }
 
 // Phase 2 when some grid vertex is the origin or the +axis end of more than one column (a
